@@ -1,0 +1,1018 @@
+// dpm_api.hip -- the C ABI (include/dpmring.h) over the gfx950 kernels.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/dpmring.h"
+#include "dpm_internal.h"
+
+using namespace dpm;
+
+namespace {
+
+constexpr double kWCA = 1.122462048;  // defs.h:17
+
+int fail(dpm_set_s* S, int code, const std::string& msg) {
+  if (S) S->err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                       \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return fail(S, DPM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+#define CHECK_HANDLE(h)                 \
+  if (!(h)) return DPM_ERR_ARG;         \
+  dpm_set_s* S = (h);                   \
+  if (hipSetDevice(S->device) != hipSuccess) return fail(S, DPM_ERR_HIP, "hipSetDevice");
+
+template <class T>
+hipError_t dalloc(T** p, long n) {
+  if (n <= 0) n = 1;
+  return hipMalloc((void**)p, sizeof(T) * n);
+}
+
+template <class T>
+hipError_t up(T* d, const std::vector<T>& h, hipStream_t st) {
+  if (h.empty()) return hipSuccess;
+  return hipMemcpyAsync(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, st);
+}
+
+long pow2ceil(long x) {
+  long p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+void bump(dpm_set_s* S) { S->version++; }
+
+void set_pos_view(dpm_set_s* S) {
+  S->V.x = S->pos[S->cur][0];
+  S->V.y = S->pos[S->cur][1];
+  S->V.xn = S->pos[1 - S->cur][0];
+  S->V.yn = S->pos[1 - S->cur][1];
+}
+
+void swap_pos(dpm_set_s* S) {
+  S->cur = 1 - S->cur;
+  set_pos_view(S);
+}
+
+// scalar/vector field -> device component pointers
+bool field_ptrs(dpm_set_s* S, int f, double** c0, double** c1, long* n, int* ncomp) {
+  const View& V = S->V;
+  *c1 = nullptr;
+  *ncomp = 1;
+  switch (f) {
+    case DPM_FIELD_POS: *c0 = S->pos[S->cur][0]; *c1 = S->pos[S->cur][1]; *n = S->nv; *ncomp = 2; return true;
+    case DPM_FIELD_VEL: *c0 = V.vx; *c1 = V.vy; *n = S->nv; *ncomp = 2; return true;
+    case DPM_FIELD_FORCE: *c0 = V.fx; *c1 = V.fy; *n = S->nv; *ncomp = 2; return true;
+    case DPM_FIELD_ENERGY: *c0 = V.ev; *n = S->nv; return true;
+    case DPM_FIELD_RAD: *c0 = V.rad; *n = S->nv; return true;
+    case DPM_FIELD_L0: *c0 = V.l0; *n = S->nv; return true;
+    case DPM_FIELD_THETA0: *c0 = V.th0; *n = S->nv; return true;
+    case DPM_FIELD_THETA: *c0 = V.th; *n = S->nv; return true;
+    case DPM_FIELD_LENGTH: *c0 = V.len; *n = S->nv; return true;
+    case DPM_FIELD_A0: *c0 = V.a0; *n = S->np; return true;
+    case DPM_FIELD_AREA: *c0 = V.area; *n = S->np; return true;
+    case DPM_FIELD_PERIMETER: *c0 = V.perim; *n = S->np; return true;
+    case DPM_FIELD_PPOS: *c0 = V.cx; *c1 = V.cy; *n = S->np; *ncomp = 2; return true;
+    case DPM_FIELD_PENERGY: *c0 = V.pe; *n = S->np; return true;
+    case DPM_FIELD_PANGLE: *c0 = V.ang; *n = S->np; return true;
+    case DPM_FIELD_LASTPOS: *c0 = V.lx; *c1 = V.ly; *n = S->nv; *ncomp = 2; return true;
+    case DPM_FIELD_BOX: *c0 = V.bx; *c1 = V.by; *n = S->nsys; *ncomp = 2; return true;
+  }
+  return false;
+}
+
+// (Re)configure the per-system cell grids for the current cutDistance, boxes and radii.
+int configure_cells(dpm_set_s* S) {
+  const long B = S->nsys;
+  std::vector<int> ncx(B), ncy(B), coff(B), brute(B);
+  long C = 0;
+  double maxMean = 1;
+  for (long s = 0; s < B; s++) {
+    const double rc = (1 + S->P.cutDistance) * 2 * S->h_maxrad[s];
+    const double Lx = S->h_box[2 * s], Ly = S->h_box[2 * s + 1];
+    int nx = rc > 0 ? (int)floor(Lx / rc) : 0;
+    int ny = rc > 0 ? (int)floor(Ly / rc) : 0;
+    if (nx > 4096) nx = 4096;
+    if (ny > 4096) ny = 4096;
+    if (nx < 3 || ny < 3) {
+      brute[s] = 1;
+      nx = ny = 0;
+    }
+    ncx[s] = nx;
+    ncy[s] = ny;
+    coff[s] = (int)C;
+    C += (long)nx * ny;
+    if (nx > 0) {
+      const double mean = (double)(S->h_svoff[s + 1] - S->h_svoff[s]) / ((double)nx * ny);
+      maxMean = std::max(maxMean, mean);
+    }
+  }
+  int cap = (int)pow2ceil((long)(4 * maxMean + 16));
+  if (S->G.cap > cap && S->G.ncells == C) cap = S->G.cap;  // keep a grown capacity
+  std::vector<int> cellSys(C);
+  for (long s = 0; s < B; s++)
+    for (long c = 0; c < (long)ncx[s] * ncy[s]; c++) cellSys[coff[s] + c] = (int)s;
+  // free and reallocate
+  hipFree(S->G.count); hipFree(S->G.slots); hipFree(S->G.cellSys);
+  S->G.count = S->G.slots = S->G.cellSys = nullptr;
+  HIPCHK(dalloc(&S->G.count, C));
+  HIPCHK(dalloc(&S->G.slots, C * cap));
+  HIPCHK(dalloc(&S->G.cellSys, C));
+  HIPCHK(up(S->G.cellSys, cellSys, S->stream));
+  HIPCHK(up(S->G.ncx, ncx, S->stream));
+  HIPCHK(up(S->G.ncy, ncy, S->stream));
+  HIPCHK(up(S->G.coff, coff, S->stream));
+  HIPCHK(up(S->G.brute, brute, S->stream));
+  HIPCHK(launch_fill_int(S->G.count, 0, C, S->stream));
+  S->G.ncells = (int)C;
+  S->G.cap = cap;
+  bump(S);
+  return DPM_OK;
+}
+
+int alloc_list(dpm_set_s* S, int W) {
+  hipFree(S->V.nbr);
+  S->V.nbr = nullptr;
+  HIPCHK(dalloc(&S->V.nbr, (long)W * S->nv));
+  HIPCHK(launch_fill_int(S->V.nbr, -1, (long)W * S->nv, S->stream));
+  S->V.W = W;
+  bump(S);
+  return DPM_OK;
+}
+
+// Host-driven full rebuild of every system's list (calcNeighborList semantics).
+int rebuild_all(dpm_set_s* S) {
+  for (int attempt = 0; attempt < 8; attempt++) {
+    HIPCHK(launch_fill_int(S->d_flagNext, 1, S->nsys, S->stream));
+    HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
+    HIPCHK(launch_fill_int(S->d_maxcnt, 0, 1, S->stream));
+    HIPCHK(launch_prologue(S, 0, 0));
+    HIPCHK(launch_bin(S));
+    HIPCHK(launch_build(S));
+    int err = 0, maxcnt = 0;
+    HIPCHK(hipMemcpyAsync(&err, S->d_err, sizeof(int), hipMemcpyDeviceToHost, S->stream));
+    HIPCHK(hipMemcpyAsync(&maxcnt, S->d_maxcnt, sizeof(int), hipMemcpyDeviceToHost, S->stream));
+    HIPCHK(hipStreamSynchronize(S->stream));
+    if (err & ERR_CELL_OVERFLOW) {
+      S->G.cap *= 2;
+      long C = S->G.ncells;
+      hipFree(S->G.slots);
+      S->G.slots = nullptr;
+      HIPCHK(dalloc(&S->G.slots, C * S->G.cap));
+      bump(S);
+      continue;
+    }
+    // keep head room so device-side rebuilds during a loop do not overflow
+    const int want = (int)pow2ceil(std::max(8, 2 * maxcnt));
+    if ((err & ERR_LIST_OVERFLOW) || want > S->V.W) {
+      int rc = alloc_list(S, want);
+      if (rc) return rc;
+      if (err & ERR_LIST_OVERFLOW) continue;
+    }
+    // the rebuild counted as an update; the reference's calcNeighborList does not
+    std::vector<int> zero(S->nsys, 0);
+    (void)zero;
+    S->listBuilt = true;
+    return DPM_OK;
+  }
+  return fail(S, DPM_ERR_OVERFLOW, "neighbour list rebuild did not converge");
+}
+
+int check_err_after_loop(dpm_set_s* S) {
+  int err = 0;
+  HIPCHK(hipMemcpyAsync(&err, S->d_err, sizeof(int), hipMemcpyDeviceToHost, S->stream));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  if (err) return fail(S, DPM_ERR_OVERFLOW, err & ERR_LIST_OVERFLOW ? "neighbour list overflow during loop"
+                                                                    : "cell overflow during loop");
+  return DPM_OK;
+}
+
+int reduce_host(dpm_set_s* S) {
+  HIPCHK(launch_reduce(S));
+  S->h_red.resize(S->nsys * 8);
+  HIPCHK(hipMemcpyAsync(S->h_red.data(), S->d_red, sizeof(double) * S->nsys * 8, hipMemcpyDeviceToHost,
+                        S->stream));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
+int require_list(dpm_set_s* S) {
+  if (S->P.interaction == INT_ALL) return DPM_OK;
+  if (!S->listBuilt) return fail(S, DPM_ERR_STATE, "neighbour list not built: call dpm_calc_neighbor_list first");
+  return DPM_OK;
+}
+
+// Launch sequence of a velocity-Verlet family loop (NVE / Langevin): see dpm_langevin_loop.
+int verlet_sequence(dpm_set_s* S, long steps, bool langevin) {
+  const double h = 0.5 * S->P.dt;
+  const int bumpA = S->P.particle == PART_ACTIVE ? 1 : 0;
+  HIPCHK(launch_start(S, h, S->P.dt));
+  swap_pos(S);
+  for (long i = 0; i < steps; i++) {
+    HIPCHK(launch_prologue(S, langevin ? 1 : 0, bumpA));
+    HIPCHK(launch_bin(S));
+    HIPCHK(launch_build(S));
+    StepArgs A{};
+    A.h = h;
+    A.dt = S->P.dt;
+    A.xout = S->pos[1 - S->cur][0];
+    A.yout = S->pos[1 - S->cur][1];
+    A.flagNext = S->d_flagNext;
+    A.ctr = S->d_ctr;
+    A.err = S->d_err;
+    const bool more = i + 1 < steps;
+    int mode = MODE_KICK | (langevin ? MODE_LANGEVIN : 0) | (more ? MODE_START : 0);
+    HIPCHK(launch_step(S, mode, A));
+    if (more) swap_pos(S);
+  }
+  return DPM_OK;
+}
+
+// Run `steps` steps, replaying a captured HIP graph of the whole launch sequence when the
+// same (kind, steps, buffer parity) has been seen with unchanged launch arguments.
+int run_verlet(dpm_set_s* S, long steps, bool langevin) {
+  if (steps <= 0) return DPM_OK;
+  int rc = require_list(S);
+  if (rc) return rc;
+  HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
+  if (S->graphVersion != S->version) {
+    for (auto& kv : S->graphs) hipGraphExecDestroy(kv.second);
+    S->graphs.clear();
+    S->graphVersion = S->version;
+  }
+  const bool useGraph = steps >= 4 && getenv("DPM_NO_GRAPH") == nullptr;
+  if (!useGraph) return verlet_sequence(S, steps, langevin);
+  auto key = std::make_tuple(langevin ? 1 : 2, steps, S->cur);
+  auto it = S->graphs.find(key);
+  if (it == S->graphs.end()) {
+    const int cur0 = S->cur;
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
+    int r2 = verlet_sequence(S, steps, langevin);
+    hipError_t e = hipStreamEndCapture(S->stream, &g);
+    if (r2) return r2;
+    HIPCHK(e);
+    hipGraphExec_t ge;
+    HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    hipGraphDestroy(g);
+    S->graphs[key] = ge;
+    // the capture advanced the host-side buffer parity; restore it and replay
+    S->cur = cur0;
+    set_pos_view(S);
+    it = S->graphs.find(key);
+  }
+  HIPCHK(hipGraphLaunch(it->second, S->stream));
+  // parity after the sequence: start swaps once, each of steps-1 middle steps swaps once
+  if ((steps % 2) == 1) swap_pos(S);
+  return DPM_OK;
+}
+
+int fire_iteration(dpm_set_s* S) {
+  HIPCHK(launch_fire_begin(S));
+  S->fireCur = 1 - S->fireCur;
+  // fire_mid reads the state written by fire_begin (now fireCur)
+  HIPCHK(launch_fire_mid(S));
+  swap_pos(S);
+  HIPCHK(launch_prologue(S, 0, S->P.particle == PART_ACTIVE ? 1 : 0));
+  HIPCHK(launch_bin(S));
+  HIPCHK(launch_build(S));
+  HIPCHK(launch_fire_step(S));
+  return DPM_OK;
+}
+
+}  // namespace
+
+// ======================================================================= ABI
+extern "C" {
+
+int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPerSystem,
+               const long* numVertexInParticleList) {
+  if (!out || numSystems <= 0 || !ringsPerSystem || !numVertexInParticleList) return DPM_ERR_ARG;
+  *out = nullptr;
+  dpm_set_s* S = new dpm_set_s();
+  S->device = device;
+  if (hipSetDevice(device) != hipSuccess) {
+    delete S;
+    return DPM_ERR_HIP;
+  }
+  S->nsys = numSystems;
+  long np = 0, nv = 0;
+  S->h_svoff.push_back(0);
+  S->h_sroff.push_back(0);
+  for (long s = 0; s < numSystems; s++) {
+    if (ringsPerSystem[s] <= 0) { delete S; return DPM_ERR_ARG; }
+    for (long k = 0; k < ringsPerSystem[s]; k++) {
+      const long n = numVertexInParticleList[np];
+      if (n < 3 || n > 64) { delete S; return DPM_ERR_UNSUPPORTED; }
+      S->h_rnv.push_back((int)n);
+      S->h_rfirst.push_back((int)nv);
+      S->h_rsys.push_back((int)s);
+      for (long q = 0; q < n; q++) {
+        S->h_vring.push_back((int)np);
+        S->h_vsys.push_back((int)s);
+        S->h_vprev.push_back((int)(q == 0 ? nv + n - 1 : nv + q - 1));
+      }
+      nv += n;
+      np++;
+    }
+    S->h_svoff.push_back((int)nv);
+    S->h_sroff.push_back((int)np);
+  }
+  if (nv > (1L << 30)) { delete S; return DPM_ERR_UNSUPPORTED; }
+  S->np = np;
+  S->nv = nv;
+  // ring-aligned wave tiles, greedy, never crossing a system
+  S->h_stile.push_back(0);
+  for (long s = 0; s < numSystems; s++) {
+    long r = S->h_sroff[s];
+    while (r < S->h_sroff[s + 1]) {
+      const int v0 = S->h_rfirst[r];
+      int v1 = v0;
+      while (r < S->h_sroff[s + 1] && (v1 + S->h_rnv[r] - v0) <= 64) {
+        v1 += S->h_rnv[r];
+        r++;
+      }
+      S->h_tile_v0.push_back(v0);
+      S->h_tile_v1.push_back(v1);
+      S->h_tile_sys.push_back((int)s);
+    }
+    S->h_stile.push_back((int)S->h_tile_v0.size());
+  }
+  S->V.nv = (int)nv;
+  S->V.np = (int)np;
+  S->V.nsys = (int)numSystems;
+  S->V.ntiles = (int)S->h_tile_v0.size();
+  S->h_box.assign(2 * numSystems, 1.0);
+  S->h_maxrad.assign(numSystems, 0.0);
+#define A(p, n) if (dalloc(&(p), (n)) != hipSuccess) { dpm_destroy(S); return DPM_ERR_OOM; }
+  for (int b = 0; b < 2; b++) for (int c = 0; c < 2; c++) A(S->pos[b][c], nv);
+  View& V = S->V;
+  A(V.vx, nv); A(V.vy, nv); A(V.fx, nv); A(V.fy, nv); A(V.rad, nv); A(V.l0, nv); A(V.th0, nv);
+  A(V.th, nv); A(V.len, nv); A(V.ev, nv); A(V.es, nv); A(V.lx, nv); A(V.ly, nv);
+  A(V.vring, nv); A(V.vsys, nv); A(S->d_vprev, nv);
+  A(V.rfirst, np); A(V.rnv, np); A(V.rsys, np);
+  A(V.a0, np); A(V.area, np); A(V.perim, np); A(V.cx, np); A(V.cy, np); A(V.pe, np); A(V.ang, np);
+  A(V.bx, numSystems); A(V.by, numSystems);
+  A(V.svoff, numSystems + 1); A(V.sroff, numSystems + 1);
+  A(V.ncnt, nv);
+  A(V.tile_v0, V.ntiles); A(V.tile_v1, V.ntiles); A(V.tile_sys, V.ntiles);
+  A(S->d_stile, numSystems + 1);
+  A(S->G.ncx, numSystems); A(S->G.ncy, numSystems); A(S->G.coff, numSystems); A(S->G.brute, numSystems);
+  A(S->d_flagNext, numSystems); A(S->d_flagCur, numSystems); A(S->d_updateCount, numSystems);
+  A(S->d_err, 1); A(S->d_maxcnt, 1); A(S->d_ctr, 4);
+  A(S->d_fire[0], numSystems); A(S->d_fire[1], numSystems);
+  A(S->d_part, (long)V.ntiles * 4);
+  A(S->d_red, numSystems * 8); A(S->d_scale, numSystems);
+#undef A
+  set_pos_view(S);
+  hipStream_t st = nullptr;
+  bool ok = up(V.vring, S->h_vring, st) == hipSuccess && up(V.vsys, S->h_vsys, st) == hipSuccess &&
+            up(S->d_vprev, S->h_vprev, st) == hipSuccess && up(V.rfirst, S->h_rfirst, st) == hipSuccess &&
+            up(V.rnv, S->h_rnv, st) == hipSuccess && up(V.rsys, S->h_rsys, st) == hipSuccess &&
+            up(V.svoff, S->h_svoff, st) == hipSuccess && up(V.sroff, S->h_sroff, st) == hipSuccess &&
+            up(V.tile_v0, S->h_tile_v0, st) == hipSuccess && up(V.tile_v1, S->h_tile_v1, st) == hipSuccess &&
+            up(V.tile_sys, S->h_tile_sys, st) == hipSuccess && up(S->d_stile, S->h_stile, st) == hipSuccess;
+  std::vector<double> ones(numSystems, 1.0);
+  ok = ok && up(V.bx, ones, st) == hipSuccess && up(V.by, ones, st) == hipSuccess;
+  // zero every state array
+  double* zd[] = {S->pos[0][0], S->pos[0][1], S->pos[1][0], S->pos[1][1], V.vx, V.vy, V.fx, V.fy, V.rad, V.l0,
+                  V.th0, V.th, V.len, V.ev, V.es, V.lx, V.ly};
+  for (double* p : zd) ok = ok && hipMemsetAsync(p, 0, sizeof(double) * nv, st) == hipSuccess;
+  double* zp[] = {V.a0, V.area, V.perim, V.cx, V.cy, V.pe, V.ang};
+  for (double* p : zp) ok = ok && hipMemsetAsync(p, 0, sizeof(double) * np, st) == hipSuccess;
+  ok = ok && hipMemsetAsync(V.ncnt, 0, sizeof(int) * nv, st) == hipSuccess;
+  ok = ok && hipMemsetAsync(S->d_flagNext, 0, sizeof(int) * numSystems, st) == hipSuccess;
+  ok = ok && hipMemsetAsync(S->d_flagCur, 0, sizeof(int) * numSystems, st) == hipSuccess;
+  ok = ok && hipMemsetAsync(S->d_updateCount, 0, sizeof(int) * numSystems, st) == hipSuccess;
+  ok = ok && hipMemsetAsync(S->d_err, 0, sizeof(int), st) == hipSuccess;
+  ok = ok && hipMemsetAsync(S->d_ctr, 0, sizeof(unsigned long long) * 4, st) == hipSuccess;
+  ok = ok && hipMemsetAsync(S->d_part, 0, sizeof(double) * V.ntiles * 4, st) == hipSuccess;
+  ok = ok && hipDeviceSynchronize() == hipSuccess;
+  if (!ok) {
+    dpm_destroy(S);
+    return DPM_ERR_HIP;
+  }
+  // defaults of the DPM2D constructor, DPM2D.cu:41-94
+  S->P.potential = POT_HARMONIC;
+  S->P.interaction = INT_VV;
+  S->P.monomer = MONO_HARMONIC;
+  S->P.particle = PART_PASSIVE;
+  S->P.dt = 1e-3;
+  S->P.ea = 1e5; S->P.el = 20; S->P.eb = 10; S->P.ec = 1;
+  S->P.gamma = 1; S->P.noise = 0;
+  S->P.cutDistance = 1;
+  S->P.cutoff = 0;
+  S->P.seed = 0;
+  S->V.W = 0;
+  if (hipStreamCreateWithFlags(&S->ownStream, hipStreamNonBlocking) != hipSuccess) {
+    dpm_destroy(S);
+    return DPM_ERR_HIP;
+  }
+  S->stream = S->ownStream;
+  if (alloc_list(S, 8) != DPM_OK) { dpm_destroy(S); return DPM_ERR_OOM; }
+  *out = S;
+  return DPM_OK;
+}
+
+int dpm_destroy(dpm_set_t h) {
+  if (!h) return DPM_ERR_ARG;
+  dpm_set_s* S = h;
+  hipSetDevice(S->device);
+  if (S->stream) hipStreamSynchronize(S->stream);
+  hipDeviceSynchronize();
+  for (auto& kv : S->graphs) hipGraphExecDestroy(kv.second);
+  View& V = S->V;
+  void* ptrs[] = {S->pos[0][0], S->pos[0][1], S->pos[1][0], S->pos[1][1], V.vx, V.vy, V.fx, V.fy, V.rad,
+                  V.l0, V.th0, V.th, V.len, V.ev, V.es, V.lx, V.ly, V.vring, V.vsys, S->d_vprev, V.rfirst,
+                  V.rnv, V.rsys, V.a0, V.area, V.perim, V.cx, V.cy, V.pe, V.ang, V.bx, V.by, V.svoff, V.sroff,
+                  V.ncnt, V.nbr, V.tile_v0, V.tile_v1, V.tile_sys, S->d_stile, S->G.ncx, S->G.ncy, S->G.coff,
+                  S->G.brute, S->G.count, S->G.slots, S->G.cellSys, S->d_flagNext, S->d_flagCur,
+                  S->d_updateCount, S->d_err, S->d_maxcnt, S->d_ctr, S->d_fire[0], S->d_fire[1], S->d_part,
+                  S->d_red, S->d_scale};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  if (S->ownStream) hipStreamDestroy(S->ownStream);
+  delete S;
+  return DPM_OK;
+}
+
+int dpm_set_stream(dpm_set_t h, void* stream) {
+  CHECK_HANDLE(h);
+  HIPCHK(hipStreamSynchronize(S->stream));
+  S->stream = stream ? (hipStream_t)stream : S->ownStream;
+  bump(S);
+  return DPM_OK;
+}
+
+const char* dpm_last_error(dpm_set_t h) { return h ? h->err.c_str() : "null handle"; }
+
+int dpm_get_sizes(dpm_set_t h, long* numSystems, long* numParticles, long* numVertices) {
+  if (!h) return DPM_ERR_ARG;
+  if (numSystems) *numSystems = h->nsys;
+  if (numParticles) *numParticles = h->np;
+  if (numVertices) *numVertices = h->nv;
+  return DPM_OK;
+}
+
+int dpm_synchronize(dpm_set_t h) {
+  CHECK_HANDLE(h);
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
+int dpm_set_field(dpm_set_t h, int field, const double* host, long n) {
+  CHECK_HANDLE(h);
+  double *c0, *c1;
+  long cnt;
+  int nc;
+  if (!host || !field_ptrs(S, field, &c0, &c1, &cnt, &nc)) return fail(S, DPM_ERR_ARG, "bad field");
+  if (n != cnt * nc) return fail(S, DPM_ERR_ARG, "field size mismatch");
+  if (nc == 1) {
+    HIPCHK(hipMemcpyAsync(c0, host, sizeof(double) * cnt, hipMemcpyHostToDevice, S->stream));
+  } else {
+    std::vector<double> a(cnt), b(cnt);
+    for (long i = 0; i < cnt; i++) { a[i] = host[2 * i]; b[i] = host[2 * i + 1]; }
+    HIPCHK(hipMemcpyAsync(c0, a.data(), sizeof(double) * cnt, hipMemcpyHostToDevice, S->stream));
+    HIPCHK(hipMemcpyAsync(c1, b.data(), sizeof(double) * cnt, hipMemcpyHostToDevice, S->stream));
+    HIPCHK(hipStreamSynchronize(S->stream));
+  }
+  if (field == DPM_FIELD_BOX) {
+    for (long i = 0; i < 2 * cnt; i++) S->h_box[i] = host[i];
+    if (S->cutSet) { int rc = configure_cells(S); if (rc) return rc; }
+  }
+  if (field == DPM_FIELD_RAD) {
+    for (long s = 0; s < S->nsys; s++) {
+      double m = 0;
+      for (int v = S->h_svoff[s]; v < S->h_svoff[s + 1]; v++) m = std::max(m, host[v]);
+      S->h_maxrad[s] = m;
+    }
+    if (S->cutSet) { int rc = configure_cells(S); if (rc) return rc; }
+  }
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
+int dpm_get_field(dpm_set_t h, int field, double* host, long n) {
+  CHECK_HANDLE(h);
+  double *c0, *c1;
+  long cnt;
+  int nc;
+  if (!host || !field_ptrs(S, field, &c0, &c1, &cnt, &nc)) return fail(S, DPM_ERR_ARG, "bad field");
+  if (n != cnt * nc) return fail(S, DPM_ERR_ARG, "field size mismatch");
+  if (nc == 1) {
+    HIPCHK(hipMemcpyAsync(host, c0, sizeof(double) * cnt, hipMemcpyDeviceToHost, S->stream));
+    HIPCHK(hipStreamSynchronize(S->stream));
+  } else {
+    std::vector<double> a(cnt), b(cnt);
+    HIPCHK(hipMemcpyAsync(a.data(), c0, sizeof(double) * cnt, hipMemcpyDeviceToHost, S->stream));
+    HIPCHK(hipMemcpyAsync(b.data(), c1, sizeof(double) * cnt, hipMemcpyDeviceToHost, S->stream));
+    HIPCHK(hipStreamSynchronize(S->stream));
+    for (long i = 0; i < cnt; i++) { host[2 * i] = a[i]; host[2 * i + 1] = b[i]; }
+  }
+  return DPM_OK;
+}
+
+int dpm_device_ptr(dpm_set_t h, int field, int component, void** ptr) {
+  CHECK_HANDLE(h);
+  double *c0, *c1;
+  long cnt;
+  int nc;
+  if (!ptr || !field_ptrs(S, field, &c0, &c1, &cnt, &nc) || component < 0 || component >= nc)
+    return fail(S, DPM_ERR_ARG, "bad field/component");
+  *ptr = component == 0 ? (void*)c0 : (void*)c1;
+  return DPM_OK;
+}
+
+int dpm_set_types(dpm_set_t h, int potential, int interaction, int monomer, int particle) {
+  CHECK_HANDLE(h);
+  if (potential < 0 || potential > 3 || (interaction != INT_VV && interaction != INT_SMOOTH && interaction != INT_ALL) ||
+      monomer < 0 || monomer > 1 || particle < 0 || particle > 1)
+    return fail(S, DPM_ERR_ARG, "bad type");
+  S->P.potential = potential;
+  S->P.interaction = interaction;
+  S->P.monomer = monomer;
+  S->P.particle = particle;
+  bump(S);
+  return DPM_OK;
+}
+
+int dpm_set_energy_costs(dpm_set_t h, double ea, double el, double eb, double ec) {
+  CHECK_HANDLE(h);
+  S->P.ea = ea; S->P.el = el; S->P.eb = eb; S->P.ec = ec;
+  bump(S);
+  return DPM_OK;
+}
+
+int dpm_set_attraction_constants(dpm_set_t h, double l1, double l2) {
+  CHECK_HANDLE(h);
+  S->P.l1 = l1; S->P.l2 = l2;
+  bump(S);
+  return DPM_OK;
+}
+
+// setLJcutoff, DPM2D.cu:1922-1932
+int dpm_set_lj_cutoff(dpm_set_t h, double LJcutoff) {
+  CHECK_HANDLE(h);
+  S->P.LJcutoff = LJcutoff;
+  const double ratio6 = 1 / pow(LJcutoff, 6);
+  S->P.LJecut = 4 * S->P.ec * (ratio6 * ratio6 - ratio6);
+  S->P.LJfshift = 24 * S->P.ec * (2 * ratio6 - 1) * ratio6 / LJcutoff;
+  bump(S);
+  return DPM_OK;
+}
+
+int dpm_set_fene_constants(dpm_set_t h, double stiff, double ext) {
+  CHECK_HANDLE(h);
+  S->P.stiff = stiff;
+  S->P.extSq = ext * ext;
+  bump(S);
+  return DPM_OK;
+}
+
+int dpm_set_time_step(dpm_set_t h, double dt) {
+  CHECK_HANDLE(h);
+  S->P.dt = dt;
+  if (S->simKind == 1) S->P.noise = sqrt(2. * S->Tinject * S->P.gamma / dt);
+  bump(S);
+  return DPM_OK;
+}
+
+int dpm_set_self_propulsion(dpm_set_t h, double driving, double taup) {
+  CHECK_HANDLE(h);
+  S->P.driving = driving;
+  S->P.taup = taup;
+  bump(S);
+  return DPM_OK;
+}
+
+int dpm_set_seed(dpm_set_t h, uint64_t seed) {
+  CHECK_HANDLE(h);
+  S->P.seed = seed;
+  S->injectCalls = 0;
+  HIPCHK(hipMemsetAsync(S->d_ctr, 0, sizeof(unsigned long long) * 4, S->stream));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  bump(S);
+  return DPM_OK;
+}
+
+// setDisplacementCutoff, DPM2D.cu:1300-1329
+int dpm_set_displacement_cutoff(dpm_set_t h, double cutoff, double size, double* cutDistance) {
+  CHECK_HANDLE(h);
+  double cd = 1;
+  switch (S->P.potential) {
+    case POT_HARMONIC: cd = 1; break;
+    case POT_LJ: cd = S->P.LJcutoff; break;
+    case POT_WCA: cd = kWCA; break;
+    case POT_ADHESIVE: cd = 1 + S->P.l2; break;
+  }
+  cd += cutoff;
+  S->P.cutDistance = cd;
+  S->P.cutoff = cutoff * size;
+  S->cutSet = true;
+  if (cutDistance) *cutDistance = cd;
+  bump(S);
+  return configure_cells(S);
+}
+
+int dpm_calc_neighbor_list(dpm_set_t h, double cutDistance) {
+  CHECK_HANDLE(h);
+  if (cutDistance != S->P.cutDistance || !S->cutSet) {
+    S->P.cutDistance = cutDistance;
+    S->cutSet = true;
+    int rc = configure_cells(S);
+    if (rc) return rc;
+  }
+  return rebuild_all(S);
+}
+
+int dpm_check_neighbors(dpm_set_t h) {
+  CHECK_HANDLE(h);
+  int rc = require_list(S);
+  if (rc) return rc;
+  // displacement test against lastPos; the drift kernels do the same test in loops
+  std::vector<double> x(S->nv), y(S->nv), lx(S->nv), ly(S->nv);
+  HIPCHK(hipMemcpyAsync(x.data(), S->V.x, sizeof(double) * S->nv, hipMemcpyDeviceToHost, S->stream));
+  HIPCHK(hipMemcpyAsync(y.data(), S->V.y, sizeof(double) * S->nv, hipMemcpyDeviceToHost, S->stream));
+  HIPCHK(hipMemcpyAsync(lx.data(), S->V.lx, sizeof(double) * S->nv, hipMemcpyDeviceToHost, S->stream));
+  HIPCHK(hipMemcpyAsync(ly.data(), S->V.ly, sizeof(double) * S->nv, hipMemcpyDeviceToHost, S->stream));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  std::vector<int> flags(S->nsys, 0);
+  for (long v = 0; v < S->nv; v++) {
+    const int s = S->h_vsys[v];
+    const double Lx = S->h_box[2 * s], Ly = S->h_box[2 * s + 1];
+    double dx = x[v] - lx[v], dy = y[v] - ly[v];
+    dx -= Lx * round(dx / Lx);
+    dy -= Ly * round(dy / Ly);
+    if (3 * sqrt(dx * dx + dy * dy) > S->P.cutoff) flags[s] = 1;
+  }
+  HIPCHK(hipMemcpyAsync(S->d_flagNext, flags.data(), sizeof(int) * S->nsys, hipMemcpyHostToDevice, S->stream));
+  HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
+  HIPCHK(launch_prologue(S, 0, 0));
+  HIPCHK(launch_bin(S));
+  HIPCHK(launch_build(S));
+  return check_err_after_loop(S);
+}
+
+int dpm_get_neighbor_list(dpm_set_t h, long* list, long* counts, long cap, long* width) {
+  CHECK_HANDLE(h);
+  const long nv = S->nv, W = S->V.W;
+  std::vector<int> nbr((size_t)W * nv), cnt(nv);
+  HIPCHK(hipMemcpyAsync(nbr.data(), S->V.nbr, sizeof(int) * W * nv, hipMemcpyDeviceToHost, S->stream));
+  HIPCHK(hipMemcpyAsync(cnt.data(), S->V.ncnt, sizeof(int) * nv, hipMemcpyDeviceToHost, S->stream));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  for (long v = 0; v < nv; v++) {
+    if (counts) counts[v] = cnt[v];
+    if (list)
+      for (long k = 0; k < cap; k++) list[v * cap + k] = (k < cnt[v] && k < W) ? nbr[k * nv + v] : -1;
+  }
+  if (width) *width = W;
+  return DPM_OK;
+}
+
+int dpm_reset_last_positions(dpm_set_t h) {
+  CHECK_HANDLE(h);
+  HIPCHK(launch_copy_lastpos(S));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
+int dpm_get_update_count(dpm_set_t h, long* perSystem) {
+  CHECK_HANDLE(h);
+  std::vector<int> c(S->nsys);
+  HIPCHK(hipMemcpyAsync(c.data(), S->d_updateCount, sizeof(int) * S->nsys, hipMemcpyDeviceToHost, S->stream));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  for (long s = 0; s < S->nsys; s++) perSystem[s] = c[s];
+  return DPM_OK;
+}
+
+int dpm_reset_update_count(dpm_set_t h) {
+  CHECK_HANDLE(h);
+  HIPCHK(launch_fill_int(S->d_updateCount, 0, S->nsys, S->stream));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
+int dpm_calc_particle_shape(dpm_set_t h) {
+  CHECK_HANDLE(h);
+  HIPCHK(launch_shape_only(S));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
+int dpm_calc_force_energy(dpm_set_t h) {
+  CHECK_HANDLE(h);
+  int rc = require_list(S);
+  if (rc) return rc;
+  // the prologue only bumps the active-noise counter here (no rebuild is pending-latched
+  // into a build: bin/build are not launched)
+  HIPCHK(launch_prologue(S, 0, S->P.particle == PART_ACTIVE ? 1 : 0));
+  StepArgs A{};
+  A.ctr = S->d_ctr;
+  A.err = S->d_err;
+  HIPCHK(launch_step(S, MODE_FORCE, A));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
+int dpm_init_langevin(dpm_set_t h, double T, double gamma, int readState, int zeroMomentum) {
+  CHECK_HANDLE(h);
+  S->simKind = 1;
+  S->Tinject = T;
+  S->P.gamma = gamma;
+  S->P.noise = sqrt(2. * T * gamma / S->P.dt);
+  bump(S);
+  HIPCHK(launch_copy_lastpos(S));
+  if (!readState) {
+    HIPCHK(launch_inject(S, sqrt(T), S->injectCalls++));
+    if (zeroMomentum) HIPCHK(launch_sub_mean(S));
+  }
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
+int dpm_init_nve(dpm_set_t h, double T, int readState, int zeroMomentum) {
+  CHECK_HANDLE(h);
+  S->simKind = 2;
+  S->Tinject = T;
+  bump(S);
+  HIPCHK(launch_copy_lastpos(S));
+  if (!readState) {
+    HIPCHK(launch_inject(S, sqrt(T), S->injectCalls++));
+    if (zeroMomentum) HIPCHK(launch_sub_mean(S));
+  }
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
+int dpm_langevin_loop(dpm_set_t h, long steps) {
+  CHECK_HANDLE(h);
+  if (S->simKind != 1) return fail(S, DPM_ERR_STATE, "dpm_init_langevin not called");
+  int rc = run_verlet(S, steps, true);
+  if (rc) return rc;
+  return check_err_after_loop(S);
+}
+
+int dpm_nve_loop(dpm_set_t h, long steps) {
+  CHECK_HANDLE(h);
+  if (S->simKind != 2 && S->simKind != 3) return fail(S, DPM_ERR_STATE, "dpm_init_nve not called");
+  int rc = run_verlet(S, steps, false);
+  if (rc) return rc;
+  return check_err_after_loop(S);
+}
+
+static int temperature_scale(dpm_set_s* S, double T) {
+  int rc = reduce_host(S);
+  if (rc) return rc;
+  std::vector<double> sc(S->nsys);
+  for (long s = 0; s < S->nsys; s++) {
+    const double n = S->h_svoff[s + 1] - S->h_svoff[s];
+    const double temp = 2. * (0.5 * S->h_red[s * 8 + 2]) / (n * 2);
+    sc[s] = sqrt(T / temp);
+  }
+  HIPCHK(hipMemcpyAsync(S->d_scale, sc.data(), sizeof(double) * S->nsys, hipMemcpyHostToDevice, S->stream));
+  HIPCHK(launch_scale_vel(S));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
+int dpm_init_nve_rescale(dpm_set_t h, double T) {
+  CHECK_HANDLE(h);
+  S->simKind = 3;
+  S->Tinject = T;
+  bump(S);
+  HIPCHK(launch_copy_lastpos(S));
+  HIPCHK(launch_inject(S, sqrt(T), S->injectCalls++));
+  return temperature_scale(S, T);
+}
+
+int dpm_nve_rescale_loop(dpm_set_t h, long steps) {
+  CHECK_HANDLE(h);
+  if (S->simKind != 3) return fail(S, DPM_ERR_STATE, "dpm_init_nve_rescale not called");
+  for (long i = 0; i < steps; i++) {
+    int rc = temperature_scale(S, S->Tinject);  // NVERescale::injectKineticEnergy, Simulator.cu:222-238
+    if (rc) return rc;
+    rc = run_verlet(S, 1, false);
+    if (rc) return rc;
+  }
+  return check_err_after_loop(S);
+}
+
+int dpm_adjust_temperature(dpm_set_t h, double T) {
+  CHECK_HANDLE(h);
+  return temperature_scale(S, T);
+}
+
+int dpm_init_fire(dpm_set_t h, const double* p, long minStep, long numStep) {
+  CHECK_HANDLE(h);
+  if (!p) return fail(S, DPM_ERR_ARG, "null FIRE params");
+  S->fp.a_start = p[0]; S->fp.f_dec = p[1]; S->fp.f_inc = p[2]; S->fp.f_a = p[3];
+  S->fp.dt_max = p[5];
+  S->fp.minStep = minStep;
+  S->fp.tol = -1;
+  std::vector<FireState> fs(S->nsys);
+  for (auto& f : fs) {
+    f.dt = p[4]; f.a = p[6]; f.P = 0; f.numStep = numStep; f.iters = 0; f.converged = 0; f.pad = 0;
+  }
+  for (int b = 0; b < 2; b++)
+    HIPCHK(hipMemcpyAsync(S->d_fire[b], fs.data(), sizeof(FireState) * S->nsys, hipMemcpyHostToDevice, S->stream));
+  S->fireCur = 0;
+  S->fireInit = true;
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
+}  // extern "C"
+
+namespace dpm {
+__global__ void k_fire_partials(View V, double* part);
+hipError_t launch_fire_partials(dpm_set_s* S);
+}  // namespace dpm
+
+extern "C" {
+
+int dpm_fire_loop(dpm_set_t h, long iterations) {
+  CHECK_HANDLE(h);
+  if (!S->fireInit) return fail(S, DPM_ERR_STATE, "dpm_init_fire not called");
+  int rc = require_list(S);
+  if (rc) return rc;
+  HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
+  HIPCHK(launch_fire_partials(S));
+  for (long i = 0; i < iterations; i++) {
+    rc = fire_iteration(S);
+    if (rc) return rc;
+  }
+  return check_err_after_loop(S);
+}
+
+int dpm_get_fire_state(dpm_set_t h, double* out) {
+  CHECK_HANDLE(h);
+  std::vector<FireState> fs(S->nsys);
+  HIPCHK(hipMemcpyAsync(fs.data(), S->d_fire[S->fireCur], sizeof(FireState) * S->nsys, hipMemcpyDeviceToHost,
+                        S->stream));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  for (long s = 0; s < S->nsys; s++) {
+    out[5 * s + 0] = fs[s].dt;
+    out[5 * s + 1] = fs[s].a;
+    out[5 * s + 2] = (double)fs[s].numStep;
+    out[5 * s + 3] = fs[s].P;
+    out[5 * s + 4] = (double)fs[s].iters;
+  }
+  return DPM_OK;
+}
+
+int dpm_get_potential_energy(dpm_set_t h, double* out) {
+  CHECK_HANDLE(h);
+  int rc = reduce_host(S);
+  if (rc) return rc;
+  for (long s = 0; s < S->nsys; s++)
+    out[s] = S->h_red[s * 8 + 0] + (S->P.interaction == INT_SMOOTH ? S->h_red[s * 8 + 1] : 0.0);
+  return DPM_OK;
+}
+
+int dpm_get_kinetic_energy(dpm_set_t h, double* out) {
+  CHECK_HANDLE(h);
+  int rc = reduce_host(S);
+  if (rc) return rc;
+  for (long s = 0; s < S->nsys; s++) out[s] = 0.5 * S->h_red[s * 8 + 2];
+  return DPM_OK;
+}
+
+int dpm_get_temperature(dpm_set_t h, double* out) {
+  CHECK_HANDLE(h);
+  int rc = reduce_host(S);
+  if (rc) return rc;
+  for (long s = 0; s < S->nsys; s++) {
+    const double n = S->h_svoff[s + 1] - S->h_svoff[s];
+    out[s] = 2. * (0.5 * S->h_red[s * 8 + 2]) / (n * 2);
+  }
+  return DPM_OK;
+}
+
+int dpm_get_max_unbalanced_force(dpm_set_t h, double* out) {
+  CHECK_HANDLE(h);
+  int rc = reduce_host(S);
+  if (rc) return rc;
+  for (long s = 0; s < S->nsys; s++) out[s] = sqrt(S->h_red[s * 8 + 3]);
+  return DPM_OK;
+}
+
+int dpm_get_total_force_magnitude(dpm_set_t h, double* out) {
+  CHECK_HANDLE(h);
+  int rc = reduce_host(S);
+  if (rc) return rc;
+  for (long s = 0; s < S->nsys; s++) {
+    const double n = S->h_svoff[s + 1] - S->h_svoff[s];
+    out[s] = sqrt(S->h_red[s * 8 + 4] / (n * 2));
+  }
+  return DPM_OK;
+}
+
+int dpm_get_phi(dpm_set_t h, double* out) {
+  CHECK_HANDLE(h);
+  int rc = reduce_host(S);
+  if (rc) return rc;
+  for (long s = 0; s < S->nsys; s++)
+    out[s] = (S->h_red[s * 8 + 5] + kPI * S->h_red[s * 8 + 7]) / (S->h_box[2 * s] * S->h_box[2 * s + 1]);
+  return DPM_OK;
+}
+
+int dpm_get_preferred_phi(dpm_set_t h, double* out) {
+  CHECK_HANDLE(h);
+  int rc = reduce_host(S);
+  if (rc) return rc;
+  for (long s = 0; s < S->nsys; s++)
+    out[s] = (S->h_red[s * 8 + 6] + kPI * S->h_red[s * 8 + 7]) / (S->h_box[2 * s] * S->h_box[2 * s + 1]);
+  return DPM_OK;
+}
+
+int dpm_get_particle_shapes(dpm_set_t h, double* shape3) {
+  CHECK_HANDLE(h);
+  std::vector<double> a(S->np), p(S->np);
+  HIPCHK(hipMemcpyAsync(a.data(), S->V.area, sizeof(double) * S->np, hipMemcpyDeviceToHost, S->stream));
+  HIPCHK(hipMemcpyAsync(p.data(), S->V.perim, sizeof(double) * S->np, hipMemcpyDeviceToHost, S->stream));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  for (long r = 0; r < S->np; r++) {
+    shape3[3 * r] = a[r];
+    shape3[3 * r + 1] = p[r];
+    shape3[3 * r + 2] = p[r] * p[r] / (4 * kPI * a[r]);
+  }
+  return DPM_OK;
+}
+
+int dpm_scale_vertices(dpm_set_t h, const double* scale) {
+  CHECK_HANDLE(h);
+  HIPCHK(hipMemcpyAsync(S->d_scale, scale, sizeof(double) * S->nsys, hipMemcpyHostToDevice, S->stream));
+  HIPCHK(launch_scale_vertices(S));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  for (long s = 0; s < S->nsys; s++) S->h_maxrad[s] *= scale[s];
+  if (S->cutSet) return configure_cells(S);
+  return DPM_OK;
+}
+
+int dpm_time_step_kernel(dpm_set_t h, long reps, double* avgUs, double* meanNeighbors) {
+  CHECK_HANDLE(h);
+  int rc = require_list(S);
+  if (rc) return rc;
+  if (reps <= 0) return fail(S, DPM_ERR_ARG, "reps must be positive");
+  double *svx = nullptr, *svy = nullptr;
+  HIPCHK(dalloc(&svx, S->nv));
+  HIPCHK(dalloc(&svy, S->nv));
+  HIPCHK(hipMemcpyAsync(svx, S->V.vx, sizeof(double) * S->nv, hipMemcpyDeviceToDevice, S->stream));
+  HIPCHK(hipMemcpyAsync(svy, S->V.vy, sizeof(double) * S->nv, hipMemcpyDeviceToDevice, S->stream));
+  StepArgs A{};
+  A.h = 0.5 * S->P.dt;
+  A.dt = S->P.dt;
+  A.xout = S->pos[1 - S->cur][0];
+  A.yout = S->pos[1 - S->cur][1];
+  A.flagNext = S->d_flagNext;
+  A.ctr = S->d_ctr;
+  A.err = S->d_err;
+  const int mode = MODE_KICK | MODE_LANGEVIN | MODE_START;
+  HIPCHK(launch_step(S, mode, A));  // warm caches and code
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, S->stream));
+  for (long i = 0; i < reps; i++) HIPCHK(launch_step(S, mode, A));
+  HIPCHK(hipEventRecord(e1, S->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  *avgUs = 1000.0 * ms / reps;
+  HIPCHK(hipMemcpyAsync(S->V.vx, svx, sizeof(double) * S->nv, hipMemcpyDeviceToDevice, S->stream));
+  HIPCHK(hipMemcpyAsync(S->V.vy, svy, sizeof(double) * S->nv, hipMemcpyDeviceToDevice, S->stream));
+  HIPCHK(launch_fill_int(S->d_flagNext, 0, S->nsys, S->stream));
+  std::vector<int> cnt(S->nv);
+  HIPCHK(hipMemcpyAsync(cnt.data(), S->V.ncnt, sizeof(int) * S->nv, hipMemcpyDeviceToHost, S->stream));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  hipFree(svx);
+  hipFree(svy);
+  double t = 0;
+  for (int c : cnt) t += c;
+  if (meanNeighbors) *meanNeighbors = t / S->nv;
+  return DPM_OK;
+}
+
+int dpm_force_hvp(dpm_set_t h, const double* ux, const double* uy, double* outx, double* outy) {
+  CHECK_HANDLE(h);
+  return fail(S, DPM_ERR_UNSUPPORTED, "dpm_force_hvp: not available in this build");
+}
+
+int dpm_fire_minimize(dpm_set_t h, double tol, long maxIter, long* iterations) {
+  CHECK_HANDLE(h);
+  return fail(S, DPM_ERR_UNSUPPORTED, "dpm_fire_minimize: not available in this build");
+}
+
+}  // extern "C"

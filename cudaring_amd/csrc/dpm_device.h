@@ -1,0 +1,278 @@
+// dpm_device.h -- device-side arithmetic of the DPM2D vertex engine (gfx950).
+//
+// Layout: structure-of-arrays FP64 per vertex (x, y, vx, vy, fx, fy, rad, l0,
+// theta0 ...), int32 topology, per-ring and per-system arrays.  A "set" holds
+// B independent systems (packings) stored back to back; every ring lies inside
+// one system and every vertex inside one ring.
+//
+// The pair and shape formulas restate include/cudaKernel.cuh of
+// farceri/cudaRing (line numbers cited per function).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dpm {
+
+constexpr double kPI = 3.141592653589793238462643383279502884197;  // defs.h:14
+constexpr double kWCAcut = 1.122462048;                            // defs.h:17
+
+enum Potential : int { POT_HARMONIC = 0, POT_LJ = 1, POT_ADHESIVE = 2, POT_WCA = 3 };
+enum Interaction : int { INT_VV = 0, INT_SMOOTH = 1, INT_CELL_SMOOTH = 2, INT_ALL = 3 };
+enum Monomer : int { MONO_HARMONIC = 0, MONO_FENE = 1 };
+enum ParticleKind : int { PART_PASSIVE = 0, PART_ACTIVE = 1, PART_ACTIVE_SHAPE = 2 };
+
+// Philox stream tags (counter word 3, high byte)
+enum RngTag : uint32_t { TAG_THERMAL = 1, TAG_INJECT = 2, TAG_ACTIVE = 3 };
+
+struct Params {
+  int potential, interaction, monomer, particle;
+  double ea, el, eb, ec, l1, l2, LJcutoff, LJecut, LJfshift, stiff, extSq;
+  double dt;
+  double driving, taup;
+  double gamma, noise;
+  double cutDistance, cutoff;  // neighbour list radius factor and rebuild threshold
+  uint64_t seed;
+};
+
+// Device view of a set.  Positions are double buffered (x/y current, xn/yn next).
+struct View {
+  int nv, np, nsys, W;
+  double *x, *y, *xn, *yn;
+  double *vx, *vy, *fx, *fy;
+  double *rad, *l0, *th0, *th, *len, *ev, *es;
+  double *lx, *ly;
+  int *vring, *vsys;
+  int *rfirst, *rnv, *rsys;
+  double *a0, *area, *perim, *cx, *cy, *pe, *ang;
+  double *bx, *by;
+  int *svoff, *sroff;  // per-system vertex / ring offsets (nsys + 1)
+  int *nbr, *ncnt;     // ELL neighbour list nbr[k * nv + v], count[v]
+  int *tile_v0, *tile_v1, *tile_sys;  // ring-aligned wave tiles (<= 64 vertices, one system)
+  int ntiles;
+};
+
+// ------------------------------------------------------------ geometry ---
+// pbcDistance, cudaKernel.cuh:70-74 (round = half away from zero)
+__device__ __forceinline__ double pbc(double delta, double size) {
+  return delta - size * round(delta / size);
+}
+
+// ----------------------------------------------------------- Philox RNG ---
+// Philox4x32-10 (Salmon et al., SC'11): 10 rounds, 9 key bumps.
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+    uint32_t hi0 = __umulhi(0xD2511F53u, c[0]), lo0 = 0xD2511F53u * c[0];
+    uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]), lo1 = 0xCD9E8D57u * c[2];
+    uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+  }
+}
+
+__device__ __forceinline__ double u01(uint32_t hi, uint32_t lo) {
+  uint64_t v = ((uint64_t)hi << 32) | lo;
+  return ((double)(v >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// Gaussian pair for (system, tag, call counter, local ring index): Box-Muller on Philox.
+__device__ __forceinline__ void gauss_pair(uint64_t seed, uint32_t sys, uint32_t tag, uint64_t counter,
+                                           uint32_t index, double& z0, double& z1) {
+  uint32_t c[4] = {index, (uint32_t)counter, (uint32_t)(counter >> 32), (tag << 24) | (sys & 0xFFFFFFu)};
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  double uA = u01(c[0], c[1]);
+  double uB = u01(c[2], c[3]);
+  double r = sqrt(-2.0 * log(uA));
+  double a = 2.0 * kPI * uB;
+  z0 = r * cos(a);
+  z1 = r * sin(a);
+}
+
+// ---------------------------------------------------------- pair terms ---
+// Directed pair on `this` from `other` at separation (dx, dy) = this - other:
+// calcContactInteraction 531-544, calcLJInteraction 562-583,
+// calcAdhesiveInteraction 604-626, calcWCAInteraction 653-668.
+// Adds to (fx, fy) and returns the directed-visit energy.
+template <int POT>
+__device__ __forceinline__ double pair_force(const Params& P, double dx, double dy, double radSum,
+                                             double& fx, double& fy) {
+  double distance = sqrt(dx * dx + dy * dy);
+  if constexpr (POT == POT_HARMONIC) {
+    double overlap = 1 - distance / radSum;
+    if (overlap > 0) {
+      double gm = P.ec * overlap / radSum;
+      fx += gm * dx / distance;
+      fy += gm * dy / distance;
+      return (0.5 * P.ec * overlap * overlap) * 0.5;
+    }
+    return 0.0;
+  } else if constexpr (POT == POT_LJ) {
+    double ratio = radSum / distance;
+    double r2 = ratio * ratio;
+    double ratio6 = r2 * r2 * r2;
+    double ratio12 = ratio6 * ratio6;
+    if (distance <= (P.LJcutoff * radSum)) {
+      double forceShift = P.LJfshift / radSum;
+      double gm = 24 * P.ec * (2 * ratio12 - ratio6) / distance - forceShift;
+      fx += gm * dx / distance;
+      fy += gm * dy / distance;
+      return 0.5 * (4 * P.ec * (ratio12 - ratio6) - P.LJecut - fabs(forceShift) * (distance - P.LJcutoff * radSum));
+    }
+    return 0.0;
+  } else if constexpr (POT == POT_ADHESIVE) {
+    double overlap = 1 - distance / radSum;
+    if (distance < (1 + P.l2) * radSum) {
+      double gm, epot;
+      if (distance < (1 + P.l1) * radSum) {
+        gm = P.ec * overlap / radSum;
+        epot = 0.5 * P.ec * (overlap * overlap - P.l1 * P.l2) * 0.5;
+      } else {
+        gm = -(P.ec * P.l1 / (P.l2 - P.l1)) * (overlap + P.l2) / radSum;
+        epot = -(0.5 * (P.ec * P.l1 / (P.l2 - P.l1)) * (overlap + P.l2) * (overlap + P.l2)) * 0.5;
+      }
+      fx += gm * dx / distance;
+      fy += gm * dy / distance;
+      return epot;
+    }
+    return 0.0;
+  } else {  // WCA
+    double ratio = radSum / distance;
+    double r2 = ratio * ratio;
+    double ratio6 = r2 * r2 * r2;
+    double ratio12 = ratio6 * ratio6;
+    if (distance <= (kWCAcut * radSum)) {
+      double gm = 24 * P.ec * (2 * ratio12 - ratio6) / distance;
+      fx += gm * dx / distance;
+      fy += gm * dy / distance;
+      return 0.5 * P.ec * (4 * (ratio12 - ratio6) + 1);
+    }
+    return 0.0;
+  }
+}
+
+// calcGradMultipleAndEnergy, cudaKernel.cuh:806-858: scalar gradient multiple and energy.
+template <int POT>
+__device__ __forceinline__ double grad_multiple(const Params& P, double distance, double radSum, double& epot) {
+  if constexpr (POT == POT_HARMONIC) {
+    double overlap = 1 - distance / radSum;
+    if (overlap > 0) {
+      epot = 0.5 * P.ec * overlap * overlap * 0.5;
+      return P.ec * overlap / radSum;
+    }
+    return 0;
+  } else if constexpr (POT == POT_LJ) {
+    double ratio = radSum / distance;
+    double r2 = ratio * ratio;
+    double ratio6 = r2 * r2 * r2;
+    double ratio12 = ratio6 * ratio6;
+    if (distance <= (P.LJcutoff * radSum)) {
+      double forceShift = P.LJfshift / radSum;
+      epot = 0.5 * (4 * P.ec * (ratio12 - ratio6) - P.LJecut - fabs(forceShift) * (distance - P.LJcutoff * radSum));
+      return 24 * P.ec * (2 * ratio12 - ratio6) / distance - forceShift;
+    }
+    return 0;
+  } else if constexpr (POT == POT_ADHESIVE) {
+    double overlap = 1 - distance / radSum;
+    if (distance < (1 + P.l1) * radSum) {
+      epot = 0.5 * P.ec * (overlap * overlap - P.l1 * P.l2) * 0.5;
+      return P.ec * overlap / radSum;
+    } else if (distance < (1 + P.l2) * radSum) {
+      epot = -(0.5 * (P.ec * P.l1 / (P.l2 - P.l1)) * (overlap + P.l2) * (overlap + P.l2)) * 0.5;
+      return -(P.ec * P.l1 / (P.l2 - P.l1)) * (overlap + P.l2) / radSum;
+    }
+    return 0;
+  } else {
+    double ratio = radSum / distance;
+    double r2 = ratio * ratio;
+    double ratio6 = r2 * r2 * r2;
+    double ratio12 = ratio6 * ratio6;
+    if (distance <= (kWCAcut * radSum)) {
+      epot = 0.5 * P.ec * (4 * (ratio12 - ratio6) + 1);
+      return 4 * P.ec * (12 * ratio12 - 6 * ratio6) / distance;
+    }
+    return 0;
+  }
+}
+
+// ------------------------------------------------- smooth vertex-segment ---
+// One evaluation of the smooth interaction of vertex i with the segment
+// (prev(j), j) of its listed neighbour j, kernelCalcSmoothInteraction
+// cudaKernel.cuh:922-975 with getProjection 861-864, getProjectionPos 866-871,
+// calcCross 873-875, calcVertexSegmentInteraction 877-897 and
+// calcVertexVertexInteraction 899-912.  Instead of scattering to three
+// vertices with atomics, the caller asks for the share of one role:
+//   ROLE_THIS  -> force on i,  ROLE_OTHER -> force on j,  ROLE_PREV -> force on prev(j).
+// The energy e (added by the reference to both rings) is returned for ROLE_THIS
+// and ROLE_OTHER.  Inputs: ti = r_i, tj = r_j, tp = r_prev(j), tq = r_prev(prev(j)).
+enum { ROLE_THIS = 0, ROLE_OTHER = 1, ROLE_PREV = 2 };
+
+template <int POT, int ROLE>
+__device__ __forceinline__ double smooth_pair(const Params& P, double Lx, double Ly, double tix, double tiy,
+                                              double tjx, double tjy, double tpx, double tpy, double tqx,
+                                              double tqy, double radSum, double& fx, double& fy) {
+  // segment = other - previous, length, projection of i on (prev, other)
+  double sgx = pbc(tjx - tpx, Lx), sgy = pbc(tjy - tpy, Ly);
+  double length = sqrt(sgx * sgx + sgy * sgy);
+  double ipx = pbc(tix - tpx, Lx), ipy = pbc(tiy - tpy, Ly);  // this - previous
+  double proj = (ipx * pbc(tjx - tpx, Lx) + ipy * pbc(tjy - tpy, Ly)) / (length * length);
+  // previous segment = secondPrevious - previous
+  double psx = pbc(tqx - tpx, Lx), psy = pbc(tqy - tpy, Ly);
+  double previousLength = sqrt(psx * psx + psy * psy);
+  double previousProj = (pbc(tix - tqx, Lx) * pbc(tpx - tqx, Lx) + pbc(tiy - tqy, Ly) * pbc(tpy - tqy, Ly)) /
+                        (previousLength * previousLength);
+  double energy = 0.0;
+  if (proj >= 0 && proj < 1) {
+    double reduced = fmax(0.0, fmin(1.0, proj));
+    double qx = tpx + reduced * sgx, qy = tpy + reduced * sgy;
+    double dqx = pbc(tix - qx, Lx), dqy = pbc(tiy - qy, Ly);
+    double epot = 0.0;
+    double gm = grad_multiple<POT>(P, sqrt(dqx * dqx + dqy * dqy), radSum, epot);
+    if (gm != 0) {
+      // calcCross: (prev - other)_x (other - this)_y - (other - this)_x (prev - other)_y
+      double pox = pbc(tpx - tjx, Lx), poy = pbc(tpy - tjy, Ly);
+      double cross = pox * pbc(tjy - tiy, Ly) - pbc(tjx - tix, Lx) * poy;
+      double absCross = fabs(cross);
+      double sign = absCross > 0 ? cross / absCross : 0.0;
+      if constexpr (ROLE == ROLE_THIS) {
+        fx += gm * sign * pbc(tpy - tjy, Ly) / length;
+        fy += gm * sign * pbc(tjx - tpx, Lx) / length;
+      } else if constexpr (ROLE == ROLE_OTHER) {
+        fx += gm * (sign * pbc(tiy - tpy, Ly) + absCross * pox / (length * length)) / length;
+        fy += gm * (sign * pbc(tpx - tix, Lx) + absCross * poy / (length * length)) / length;
+      } else {
+        fx += gm * (sign * pbc(tjy - tiy, Ly) - absCross * pox / (length * length)) / length;
+        fy += gm * (sign * pbc(tix - tjx, Lx) - absCross * poy / (length * length)) / length;
+      }
+      energy += epot;
+    }
+    if (previousProj >= 0 && previousProj < 1) {
+      // concave: subtract the pair (prev(j), i); calcVertexVertexInteraction(previousPos, thisPos)
+      double dx = pbc(tpx - tix, Lx), dy = pbc(tpy - tiy, Ly);
+      double d = sqrt(dx * dx + dy * dy);
+      double ep = 0.0;
+      double g = grad_multiple<POT>(P, d, radSum, ep);
+      if (g != 0) {
+        if constexpr (ROLE == ROLE_THIS) { fx += g * dx / d; fy += g * dy / d; }
+        if constexpr (ROLE == ROLE_PREV) { fx += -g * dx / d; fy += -g * dy / d; }
+        energy -= ep;
+      }
+    }
+  } else if (proj < 0) {
+    if (previousProj >= 1) {
+      // convex corner: add the pair (i, prev(j)); calcVertexVertexInteraction(thisPos, previousPos)
+      double dx = pbc(tix - tpx, Lx), dy = pbc(tiy - tpy, Ly);
+      double d = sqrt(dx * dx + dy * dy);
+      double ep = 0.0;
+      double g = grad_multiple<POT>(P, d, radSum, ep);
+      if (g != 0) {
+        if constexpr (ROLE == ROLE_THIS) { fx += g * dx / d; fy += g * dy / d; }
+        if constexpr (ROLE == ROLE_PREV) { fx += -g * dx / d; fy += -g * dy / d; }
+        energy += ep;
+      }
+    }
+  }
+  return energy;
+}
+
+}  // namespace dpm

@@ -1,0 +1,135 @@
+// dpm_internal.h -- host-side handle of a system set and the launcher interface
+// between the C ABI (dpm_api.hip) and the kernels (dpm_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "dpm_device.h"
+
+namespace dpm {
+
+// step-kernel epilogue bits
+enum : int { MODE_FORCE = 0, MODE_KICK = 1, MODE_LANGEVIN = 2, MODE_START = 4, MODE_FIRE = 8 };
+
+// error bits written by kernels into Set::d_err
+enum : int { ERR_LIST_OVERFLOW = 1, ERR_CELL_OVERFLOW = 2 };
+
+// per-system FIRE scalars (double buffered by iteration parity)
+struct FireState {
+  double dt, a, P;
+  long long numStep, iters;
+  int converged, pad;
+};
+
+struct FireParams {
+  double a_start, f_dec, f_inc, f_a, dt_max;
+  long long minStep;
+  double tol;
+};
+
+struct CellGrid {
+  int* count;      // [ncells]
+  int* slots;      // [ncells * cap]
+  int* cellSys;    // [ncells]
+  int* ncx;        // per system
+  int* ncy;
+  int* coff;       // per-system cell offset
+  int* brute;      // per system: 1 = all-pairs within the system (box < 3 cells)
+  int ncells, cap;
+};
+
+struct StepArgs {
+  double h;                       // kick time step (0.5 dt)
+  double dt;                      // drift time step
+  double* xout;                   // drift destination (MODE_START)
+  double* yout;
+  int* flagNext;                  // per-system rebuild request (set by the drift)
+  const unsigned long long* ctr;  // [0] thermal calls, [1] active calls (already bumped)
+  FireState* fire;                // MODE_FIRE: per-system state of this iteration
+  double* part;                   // MODE_FIRE: per-tile partial sums [ntiles * 4]
+  int* err;
+};
+
+}  // namespace dpm
+
+struct dpm_set_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipStream_t ownStream = nullptr;  // created with the handle; graph capture needs a non-NULL stream
+  long nsys = 0, np = 0, nv = 0;
+  std::string err;
+
+  // host copies of topology
+  std::vector<int> h_rnv, h_rfirst, h_rsys, h_vring, h_vsys, h_vprev, h_svoff, h_sroff;
+  std::vector<int> h_tile_v0, h_tile_v1, h_tile_sys, h_stile;  // h_stile: per-system tile offsets
+  std::vector<double> h_box;    // 2 per system
+  std::vector<double> h_maxrad; // per system
+
+  dpm::View V{};
+  dpm::Params P{};
+  double* pos[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [buffer][component]
+  int cur = 0;
+  int* d_vprev = nullptr;
+  int* d_stile = nullptr;
+
+  // neighbour machinery
+  dpm::CellGrid G{};
+  int* d_flagNext = nullptr;
+  int* d_flagCur = nullptr;
+  int* d_updateCount = nullptr;
+  int* d_err = nullptr;
+  int* d_maxcnt = nullptr;
+  unsigned long long* d_ctr = nullptr;  // thermal, active
+  bool listBuilt = false;
+  bool cutSet = false;
+
+  // integrators
+  int simKind = 0;  // 1 langevin, 2 nve, 3 nve rescale
+  double Tinject = 0;
+  unsigned long long injectCalls = 0;
+
+  // FIRE
+  dpm::FireState* d_fire[2] = {nullptr, nullptr};
+  int fireCur = 0;
+  dpm::FireParams fp{};
+  double* d_part = nullptr;  // per-tile partials [ntiles * 4]
+  bool fireInit = false;
+
+  // reductions
+  double* d_red = nullptr;   // per-system outputs [nsys * 8]
+  double* d_scale = nullptr; // per-system scalars
+  std::vector<double> h_red;
+
+  // graph cache: key = (kind, steps, cur)
+  std::map<std::tuple<int, long, int>, hipGraphExec_t> graphs;
+  long version = 0;  // bumped when launch arguments change
+  long graphVersion = -1;
+};
+
+namespace dpm {
+
+// launchers (dpm_kernels.hip); all return hipError_t of the last launch
+hipError_t launch_shape_only(dpm_set_s* S);
+hipError_t launch_step(dpm_set_s* S, int mode, const StepArgs& A);
+hipError_t launch_start(dpm_set_s* S, double h, double dt);
+hipError_t launch_prologue(dpm_set_s* S, int bumpThermal, int bumpActive);
+hipError_t launch_bin(dpm_set_s* S);
+hipError_t launch_build(dpm_set_s* S);
+hipError_t launch_fill_int(int* p, int value, long n, hipStream_t st);
+hipError_t launch_reduce(dpm_set_s* S);
+hipError_t launch_inject(dpm_set_s* S, double amplitude, unsigned long long call);
+hipError_t launch_sub_mean(dpm_set_s* S);
+hipError_t launch_scale_vel(dpm_set_s* S);
+hipError_t launch_scale_vertices(dpm_set_s* S);
+hipError_t launch_copy_lastpos(dpm_set_s* S);
+hipError_t launch_fire_begin(dpm_set_s* S);
+hipError_t launch_fire_mid(dpm_set_s* S);
+hipError_t launch_fire_step(dpm_set_s* S);
+hipError_t launch_hvp(dpm_set_s* S, const double* ux, const double* uy, double* ox, double* oy);
+
+}  // namespace dpm

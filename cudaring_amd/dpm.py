@@ -1,0 +1,337 @@
+"""Python binding of libdpmring.so (the C ABI in include/dpmring.h).
+
+`DPMSet` holds B >= 1 independent DPM2D systems on one GPU.  Method names follow
+the reference's DPM2D class (include/DPM2D.h of farceri/cudaRing); per-system
+observables return numpy arrays of length B.  The library is required: if it
+is missing or fails to load this module raises, there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdpmring.so")
+
+FIELDS = ["pos", "vel", "force", "energy", "rad", "l0", "theta0", "theta", "length", "a0", "area",
+          "perimeter", "ppos", "penergy", "pangle", "lastpos", "box"]
+FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
+POT = {"harmonic": 0, "lj": 1, "adhesive": 2, "wca": 3}
+INTER = {"vertexVertex": 0, "vertexSmooth": 1, "all": 3}
+MONO = {"harmonic": 0, "FENE": 1}
+PART = {"passive": 0, "active": 1}
+ERRORS = {-1: "bad argument", -2: "HIP error", -3: "out of memory", -4: "unsupported",
+          -5: "overflow", -6: "bad state"}
+
+# every exported symbol with (restype, argtypes); tests check this list against the header
+_vp, _d, _l, _i = C.c_void_p, C.c_double, C.c_long, C.c_int
+SIGNATURES = {
+    "dpm_create": (_i, [C.POINTER(_vp), _i, _l, _vp, _vp]),
+    "dpm_destroy": (_i, [_vp]),
+    "dpm_set_stream": (_i, [_vp, _vp]),
+    "dpm_last_error": (C.c_char_p, [_vp]),
+    "dpm_get_sizes": (_i, [_vp, _vp, _vp, _vp]),
+    "dpm_synchronize": (_i, [_vp]),
+    "dpm_set_field": (_i, [_vp, _i, _vp, _l]),
+    "dpm_get_field": (_i, [_vp, _i, _vp, _l]),
+    "dpm_device_ptr": (_i, [_vp, _i, _i, C.POINTER(_vp)]),
+    "dpm_set_types": (_i, [_vp, _i, _i, _i, _i]),
+    "dpm_set_energy_costs": (_i, [_vp, _d, _d, _d, _d]),
+    "dpm_set_attraction_constants": (_i, [_vp, _d, _d]),
+    "dpm_set_lj_cutoff": (_i, [_vp, _d]),
+    "dpm_set_fene_constants": (_i, [_vp, _d, _d]),
+    "dpm_set_time_step": (_i, [_vp, _d]),
+    "dpm_set_self_propulsion": (_i, [_vp, _d, _d]),
+    "dpm_set_seed": (_i, [_vp, C.c_uint64]),
+    "dpm_set_displacement_cutoff": (_i, [_vp, _d, _d, C.POINTER(_d)]),
+    "dpm_calc_neighbor_list": (_i, [_vp, _d]),
+    "dpm_check_neighbors": (_i, [_vp]),
+    "dpm_get_neighbor_list": (_i, [_vp, _vp, _vp, _l, C.POINTER(_l)]),
+    "dpm_reset_last_positions": (_i, [_vp]),
+    "dpm_get_update_count": (_i, [_vp, _vp]),
+    "dpm_reset_update_count": (_i, [_vp]),
+    "dpm_calc_particle_shape": (_i, [_vp]),
+    "dpm_calc_force_energy": (_i, [_vp]),
+    "dpm_init_langevin": (_i, [_vp, _d, _d, _i, _i]),
+    "dpm_langevin_loop": (_i, [_vp, _l]),
+    "dpm_init_nve": (_i, [_vp, _d, _i, _i]),
+    "dpm_nve_loop": (_i, [_vp, _l]),
+    "dpm_init_nve_rescale": (_i, [_vp, _d]),
+    "dpm_nve_rescale_loop": (_i, [_vp, _l]),
+    "dpm_adjust_temperature": (_i, [_vp, _d]),
+    "dpm_init_fire": (_i, [_vp, _vp, _l, _l]),
+    "dpm_fire_loop": (_i, [_vp, _l]),
+    "dpm_get_fire_state": (_i, [_vp, _vp]),
+    "dpm_fire_minimize": (_i, [_vp, _d, _l, _vp]),
+    "dpm_get_potential_energy": (_i, [_vp, _vp]),
+    "dpm_get_kinetic_energy": (_i, [_vp, _vp]),
+    "dpm_get_temperature": (_i, [_vp, _vp]),
+    "dpm_get_max_unbalanced_force": (_i, [_vp, _vp]),
+    "dpm_get_total_force_magnitude": (_i, [_vp, _vp]),
+    "dpm_get_phi": (_i, [_vp, _vp]),
+    "dpm_get_preferred_phi": (_i, [_vp, _vp]),
+    "dpm_get_particle_shapes": (_i, [_vp, _vp]),
+    "dpm_scale_vertices": (_i, [_vp, _vp]),
+    "dpm_force_hvp": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "dpm_time_step_kernel": (_i, [_vp, _l, C.POINTER(_d), C.POINTER(_d)]),
+}
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libdpmring.so and declare every entry point.  Raises if it is absent."""
+    global _lib
+    if _lib is not None and path == LIB_PATH:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built; run __graft_entry__.build() (hipcc, gfx950)")
+    L = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path == LIB_PATH:
+        _lib = L
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class DPMError(RuntimeError):
+    pass
+
+
+class DPMSet:
+    """B independent DPM2D systems on one device (B = 1 is the reference's single DPM2D)."""
+
+    def __init__(self, nvp, rings_per_system=None, device=0):
+        self.L = load_library()
+        nvp = np.ascontiguousarray(nvp, dtype=np.int64)
+        if rings_per_system is None:
+            rings_per_system = [len(nvp)]
+        rps = np.ascontiguousarray(rings_per_system, dtype=np.int64)
+        h = C.c_void_p()
+        rc = self.L.dpm_create(C.byref(h), device, len(rps), _p(rps), _p(nvp))
+        if rc != 0:
+            raise DPMError(f"dpm_create failed: {ERRORS.get(rc, rc)}")
+        self.h = h
+        self.nvp = nvp
+        self.rps = rps
+        self.nsys = len(rps)
+        self.np = len(nvp)
+        self.nv = int(nvp.sum())
+        ring_off = np.concatenate([[0], np.cumsum(rps)])
+        vfirst = np.concatenate([[0], np.cumsum(nvp)])
+        self.vert_off = vfirst[ring_off]
+        self.ring_off = ring_off
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.L.dpm_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self.L.dpm_last_error(self.h)
+            raise DPMError(f"{what}: {ERRORS.get(rc, rc)}: {msg.decode() if msg else ''}")
+
+    # ---------------------------------------------------------------- fields
+    def field_size(self, name):
+        n = {"v": self.nv, "r": self.np, "s": self.nsys}
+        kind = {"pos": ("v", 2), "vel": ("v", 2), "force": ("v", 2), "lastpos": ("v", 2),
+                "ppos": ("r", 2), "box": ("s", 2)}
+        if name in kind:
+            k, c = kind[name]
+            return n[k] * c
+        if name in ("a0", "area", "perimeter", "penergy", "pangle"):
+            return self.np
+        return self.nv
+
+    def set(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        self._chk(self.L.dpm_set_field(self.h, FIELD_ID[name], _p(a), a.size), f"set {name}")
+
+    def get(self, name):
+        a = np.empty(self.field_size(name), dtype=np.float64)
+        self._chk(self.L.dpm_get_field(self.h, FIELD_ID[name], _p(a), a.size), f"get {name}")
+        return a
+
+    def device_ptr(self, name, component=0):
+        p = C.c_void_p()
+        self._chk(self.L.dpm_device_ptr(self.h, FIELD_ID[name], component, C.byref(p)), "device_ptr")
+        return p.value
+
+    def load_state(self, st):
+        for k in ("box", "rad", "pos", "l0", "theta0", "a0", "vel", "pangle"):
+            if k in st and st[k] is not None:
+                self.set(k, st[k])
+
+    # ------------------------------------------------------------ parameters
+    def set_types(self, potential="harmonic", interaction="vertexVertex", monomer="harmonic",
+                  particle="passive"):
+        self._chk(self.L.dpm_set_types(self.h, POT[potential], INTER[interaction], MONO[monomer],
+                                       PART[particle]), "set_types")
+
+    def set_energy_costs(self, ea, el, eb, ec):
+        self._chk(self.L.dpm_set_energy_costs(self.h, ea, el, eb, ec), "set_energy_costs")
+
+    def set_attraction(self, l1, l2):
+        self._chk(self.L.dpm_set_attraction_constants(self.h, l1, l2), "set_attraction")
+
+    def set_lj_cutoff(self, c):
+        self._chk(self.L.dpm_set_lj_cutoff(self.h, c), "set_lj_cutoff")
+
+    def set_fene(self, stiff, ext):
+        self._chk(self.L.dpm_set_fene_constants(self.h, stiff, ext), "set_fene")
+
+    def set_dt(self, dt):
+        self._chk(self.L.dpm_set_time_step(self.h, dt), "set_dt")
+
+    def set_self_propulsion(self, driving, taup):
+        self._chk(self.L.dpm_set_self_propulsion(self.h, driving, taup), "set_self_propulsion")
+
+    def set_seed(self, seed):
+        self._chk(self.L.dpm_set_seed(self.h, seed), "set_seed")
+
+    # ------------------------------------------------------------ neighbours
+    def set_displacement_cutoff(self, cutoff, size):
+        cd = C.c_double()
+        self._chk(self.L.dpm_set_displacement_cutoff(self.h, cutoff, size, C.byref(cd)), "set_cutoff")
+        return cd.value
+
+    def calc_neighbor_list(self, cut):
+        self._chk(self.L.dpm_calc_neighbor_list(self.h, cut), "calc_neighbor_list")
+
+    def check_neighbors(self):
+        self._chk(self.L.dpm_check_neighbors(self.h), "check_neighbors")
+
+    def neighbors(self, cap=None):
+        w = C.c_long()
+        self._chk(self.L.dpm_get_neighbor_list(self.h, None, None, 0, C.byref(w)), "neighbors")
+        cap = cap or max(w.value, 1)
+        lst = np.empty(self.nv * cap, dtype=np.int64)
+        cnt = np.empty(self.nv, dtype=np.int64)
+        self._chk(self.L.dpm_get_neighbor_list(self.h, _p(lst), _p(cnt), cap, C.byref(w)), "neighbors")
+        lst = lst.reshape(self.nv, cap)
+        return [list(lst[v, :cnt[v]]) for v in range(self.nv)], w.value, cnt
+
+    def update_count(self):
+        a = np.zeros(self.nsys, dtype=np.int64)
+        self._chk(self.L.dpm_get_update_count(self.h, _p(a)), "update_count")
+        return a
+
+    def reset_update_count(self):
+        self._chk(self.L.dpm_reset_update_count(self.h), "reset_update_count")
+
+    def reset_last_positions(self):
+        self._chk(self.L.dpm_reset_last_positions(self.h), "reset_last_positions")
+
+    # ---------------------------------------------------------------- forces
+    def calc_particle_shape(self):
+        self._chk(self.L.dpm_calc_particle_shape(self.h), "calc_particle_shape")
+
+    def calc_force_energy(self):
+        self._chk(self.L.dpm_calc_force_energy(self.h), "calc_force_energy")
+
+    # ----------------------------------------------------------- integrators
+    def init_langevin(self, T, gamma, read_state=False, zero_momentum=False):
+        self._chk(self.L.dpm_init_langevin(self.h, T, gamma, int(read_state), int(zero_momentum)),
+                  "init_langevin")
+
+    def langevin(self, n):
+        self._chk(self.L.dpm_langevin_loop(self.h, n), "langevin_loop")
+
+    def init_nve(self, T, read_state=False, zero_momentum=False):
+        self._chk(self.L.dpm_init_nve(self.h, T, int(read_state), int(zero_momentum)), "init_nve")
+
+    def nve(self, n):
+        self._chk(self.L.dpm_nve_loop(self.h, n), "nve_loop")
+
+    def init_nve_rescale(self, T):
+        self._chk(self.L.dpm_init_nve_rescale(self.h, T), "init_nve_rescale")
+
+    def nve_rescale(self, n):
+        self._chk(self.L.dpm_nve_rescale_loop(self.h, n), "nve_rescale_loop")
+
+    def adjust_temperature(self, T):
+        self._chk(self.L.dpm_adjust_temperature(self.h, T), "adjust_temperature")
+
+    # ------------------------------------------------------------------ FIRE
+    def init_fire(self, params7, min_step=20, num_step=0):
+        p = np.ascontiguousarray(params7, dtype=np.float64)
+        self._chk(self.L.dpm_init_fire(self.h, _p(p), min_step, num_step), "init_fire")
+
+    def fire(self, n):
+        self._chk(self.L.dpm_fire_loop(self.h, n), "fire_loop")
+
+    def fire_state(self):
+        o = np.zeros(5 * self.nsys)
+        self._chk(self.L.dpm_get_fire_state(self.h, _p(o)), "fire_state")
+        o = o.reshape(self.nsys, 5)
+        return {"dt": o[:, 0], "alpha": o[:, 1], "numStep": o[:, 2].astype(np.int64), "P": o[:, 3],
+                "iters": o[:, 4].astype(np.int64)}
+
+    def fire_minimize(self, tol, max_iter):
+        it = np.zeros(self.nsys, dtype=np.int64)
+        self._chk(self.L.dpm_fire_minimize(self.h, tol, max_iter, _p(it)), "fire_minimize")
+        return it
+
+    # ----------------------------------------------------------- observables
+    def _per_sys(self, fn, what):
+        o = np.zeros(self.nsys)
+        self._chk(fn(self.h, _p(o)), what)
+        return o
+
+    def potential_energy(self):
+        return self._per_sys(self.L.dpm_get_potential_energy, "potential_energy")
+
+    def kinetic_energy(self):
+        return self._per_sys(self.L.dpm_get_kinetic_energy, "kinetic_energy")
+
+    def temperature(self):
+        return self._per_sys(self.L.dpm_get_temperature, "temperature")
+
+    def max_unbalanced_force(self):
+        return self._per_sys(self.L.dpm_get_max_unbalanced_force, "max_unbalanced_force")
+
+    def total_force_magnitude(self):
+        return self._per_sys(self.L.dpm_get_total_force_magnitude, "total_force_magnitude")
+
+    def phi(self):
+        return self._per_sys(self.L.dpm_get_phi, "phi")
+
+    def preferred_phi(self):
+        return self._per_sys(self.L.dpm_get_preferred_phi, "preferred_phi")
+
+    def shapes(self):
+        o = np.zeros(3 * self.np)
+        self._chk(self.L.dpm_get_particle_shapes(self.h, _p(o)), "shapes")
+        return o.reshape(self.np, 3)
+
+    def scale_vertices(self, scale):
+        s = np.ascontiguousarray(np.broadcast_to(np.asarray(scale, dtype=np.float64), (self.nsys,)))
+        self._chk(self.L.dpm_scale_vertices(self.h, _p(s)), "scale_vertices")
+
+    def synchronize(self):
+        self._chk(self.L.dpm_synchronize(self.h), "synchronize")
+
+    def set_stream(self, stream_ptr):
+        self._chk(self.L.dpm_set_stream(self.h, C.c_void_p(stream_ptr)), "set_stream")
+
+    def time_step_kernel(self, reps=200):
+        us, kbar = C.c_double(), C.c_double()
+        self._chk(self.L.dpm_time_step_kernel(self.h, reps, C.byref(us), C.byref(kbar)), "time_step_kernel")
+        return us.value, kbar.value
+
+    def force_hvp(self, ux_ptr, uy_ptr, ox_ptr, oy_ptr):
+        self._chk(self.L.dpm_force_hvp(self.h, C.c_void_p(ux_ptr), C.c_void_p(uy_ptr),
+                                       C.c_void_p(ox_ptr), C.c_void_p(oy_ptr)), "force_hvp")

@@ -1,0 +1,189 @@
+/*
+ * dpmring.h -- C ABI of the MI355X (gfx950) DPM2D vertex engine, libdpmring.so.
+ *
+ * The reference (farceri/cudaRing) has no FFI: its seam is the DPM2D / FIRE /
+ * SimInterface C++ class API over raw thrust::device_vector pointers and
+ * __constant__ device globals.  This ABI is what those classes (re-created in
+ * cudaring_amd/host/) and the Python/torch wrapper call.  Each entry point
+ * names the reference method it stands in for.
+ *
+ * Conventions
+ *   - every call returns an int status (DPM_OK = 0, negative on error) and never
+ *     throws; dpm_last_error() returns the message of the last failure;
+ *   - a handle is a "set" of B >= 1 independent systems (packings) stored back to
+ *     back on one device; single-system use is B = 1.  The reference's
+ *     one-DPM2D-per-process limit (device __constant__ state) does not apply;
+ *   - host-side 2-vectors (positions, velocities, forces, ring centres) use the
+ *     reference's interleaved layout (x0, y0, x1, y1, ...); on the device they are
+ *     stored as structure-of-arrays FP64;
+ *   - vertex and ring indices are global over the set, in system order;
+ *   - `stream` arguments are hipStream_t passed as void*; NULL = default stream;
+ *   - a handle is not thread-safe; distinct handles are independent.
+ */
+#ifndef DPMRING_H
+#define DPMRING_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DPM_OK 0
+#define DPM_ERR_ARG (-1)
+#define DPM_ERR_HIP (-2)
+#define DPM_ERR_OOM (-3)
+#define DPM_ERR_UNSUPPORTED (-4)
+#define DPM_ERR_OVERFLOW (-5)
+#define DPM_ERR_STATE (-6)
+
+typedef struct dpm_set_s* dpm_set_t;
+
+/* field ids for dpm_set_field / dpm_get_field (sizes: V = vertices, R = rings, B = systems) */
+enum {
+  DPM_FIELD_POS = 0,       /* 2V  setVertexPositions / getVertexPositions, DPM2D.h:357-358     */
+  DPM_FIELD_VEL = 1,       /* 2V  setVertexVelocities / getVertexVelocities, DPM2D.h:363-364    */
+  DPM_FIELD_FORCE = 2,     /* 2V  setVertexForces / getVertexForces, DPM2D.h:365-366            */
+  DPM_FIELD_ENERGY = 3,    /* V   d_energy (per-vertex shape + vertex-vertex energy)            */
+  DPM_FIELD_RAD = 4,       /* V   setVertexRadii / getVertexRadii, DPM2D.h:283-284              */
+  DPM_FIELD_L0 = 5,        /* V   setRestLengths / getRestLengths, DPM2D.h:287-288              */
+  DPM_FIELD_THETA0 = 6,    /* V   setRestAngles / getRestAngles, DPM2D.h:289-290                */
+  DPM_FIELD_THETA = 7,     /* V   getSegmentAngles, DPM2D.h:292                                  */
+  DPM_FIELD_LENGTH = 8,    /* V   getSegmentLengths, DPM2D.h:291                                 */
+  DPM_FIELD_A0 = 9,        /* R   setRestAreas / getRestAreas, DPM2D.h:285-286                  */
+  DPM_FIELD_AREA = 10,     /* R   getAreas, DPM2D.h:294                                          */
+  DPM_FIELD_PERIMETER = 11,/* R   getPerimeters, DPM2D.h:295                                     */
+  DPM_FIELD_PPOS = 12,     /* 2R  getParticlePositions (ring centres), DPM2D.h:318               */
+  DPM_FIELD_PENERGY = 13,  /* R   getParticleEnergies (smooth interaction energy), DPM2D.h:325   */
+  DPM_FIELD_PANGLE = 14,   /* R   setParticleAngles / getParticleAngles, DPM2D.h:326-327         */
+  DPM_FIELD_LASTPOS = 15,  /* 2V  d_lastPos (neighbour-list reference positions)               */
+  DPM_FIELD_BOX = 16,      /* 2B  setBoxSize / getBoxSize, DPM2D.h:275-276 (per system)        */
+  DPM_FIELD_COUNT = 17
+};
+
+/* potential / interaction / monomer / particle enums keep simControlStruct's order, DPM2D.h:22-32 */
+enum { DPM_POT_HARMONIC = 0, DPM_POT_LJ = 1, DPM_POT_ADHESIVE = 2, DPM_POT_WCA = 3 };
+enum { DPM_INT_VERTEX_VERTEX = 0, DPM_INT_VERTEX_SMOOTH = 1, DPM_INT_ALL = 3 };
+enum { DPM_MONO_HARMONIC = 0, DPM_MONO_FENE = 1 };
+enum { DPM_PART_PASSIVE = 0, DPM_PART_ACTIVE = 1 };
+
+/* ---- lifecycle ------------------------------------------------------------ */
+/* Replaces DPM2D(long nParticles, long dim, long nVertexPerParticle) (DPM2D.h:47) followed by
+ * setNumVertexInParticleList (DPM2D.h:267) and initParticleIdList (DPM2D.h:219), for B systems:
+ * ringsPerSystem[B], numVertexInParticleList[sum(ringsPerSystem)] (3 <= n <= 64). */
+int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPerSystem,
+               const long* numVertexInParticleList);
+int dpm_destroy(dpm_set_t h);
+int dpm_set_stream(dpm_set_t h, void* stream);
+const char* dpm_last_error(dpm_set_t h);
+/* getNumParticles / getNumVertices (DPM2D.h:259-263), summed over the set */
+int dpm_get_sizes(dpm_set_t h, long* numSystems, long* numParticles, long* numVertices);
+int dpm_synchronize(dpm_set_t h);
+
+/* ---- state ---------------------------------------------------------------- */
+/* the host_vector setters and getters of DPM2D.h:267-366 (see the field ids above) */
+int dpm_set_field(dpm_set_t h, int field, const double* host, long n);
+int dpm_get_field(dpm_set_t h, int field, double* host, long n);
+/* device pointers of the SoA storage, for zero-copy use by a caller on the same stream:
+ * component 0 = x (or the scalar field), 1 = y.  Positions point at the current buffer. */
+int dpm_device_ptr(dpm_set_t h, int field, int component, void** ptr);
+
+/* ---- parameters ----------------------------------------------------------- */
+/* setPotentialType / setInteractionType / setMonomerType / setParticleType, DPM2D.h:225-250 */
+int dpm_set_types(dpm_set_t h, int potential, int interaction, int monomer, int particle);
+/* setEnergyCosts(ea, el, eb, ec), DPM2D.h:525 */
+int dpm_set_energy_costs(dpm_set_t h, double ea, double el, double eb, double ec);
+/* setAttractionConstants(l1, l2), DPM2D.h:535 */
+int dpm_set_attraction_constants(dpm_set_t h, double l1, double l2);
+/* setLJcutoff, DPM2D.h:539 (energy and force shifts use the current ec) */
+int dpm_set_lj_cutoff(dpm_set_t h, double LJcutoff);
+/* setFENEconstants, DPM2D.h:541 */
+int dpm_set_fene_constants(dpm_set_t h, double stiff, double ext);
+/* setTimeStep, DPM2D.h:545 */
+int dpm_set_time_step(dpm_set_t h, double dt);
+/* setSelfPropulsionParams(driving, taup), DPM2D.h:527 */
+int dpm_set_self_propulsion(dpm_set_t h, double driving, double taup);
+/* seed of the Philox streams that replace thrust::default_random_engine + lrand48
+ * (defs.h:30-79, DPM2D.cu:43); resets the per-stream call counters */
+int dpm_set_seed(dpm_set_t h, uint64_t seed);
+
+/* ---- neighbours ----------------------------------------------------------- */
+/* setDisplacementCutoff(cutoff, size) -> cutDistance, DPM2D.h:447 / DPM2D.cu:1300-1329 */
+int dpm_set_displacement_cutoff(dpm_set_t h, double cutoff, double size, double* cutDistance);
+/* calcNeighborList(cutDistance), DPM2D.h:640 / DPM2D.cu:3479-3507 (built with a cell list;
+ * the list holds the same sets as the reference's all-pairs build) */
+int dpm_calc_neighbor_list(dpm_set_t h, double cutDistance);
+/* checkNeighbors(), DPM2D.h:459 / DPM2D.cu:1408-1423: device-side displacement test and rebuild */
+int dpm_check_neighbors(dpm_set_t h);
+/* getNeighbors + getMaxNeighborList (DPM2D.h:411-412): rows of `cap` ascending global ids,
+ * -1 padded, plus counts[V]; returns the device row capacity in *width */
+int dpm_get_neighbor_list(dpm_set_t h, long* list, long* counts, long cap, long* width);
+/* resetLastPositions (DPM2D.h:359), getUpdateCount / resetUpdateCount (DPM2D.h:449-450) */
+int dpm_reset_last_positions(dpm_set_t h);
+int dpm_get_update_count(dpm_set_t h, long* perSystem);
+int dpm_reset_update_count(dpm_set_t h);
+
+/* ---- forces --------------------------------------------------------------- */
+/* calcParticleShape + calcParticlePositions, DPM2D.h:303-304 */
+int dpm_calc_particle_shape(dpm_set_t h);
+/* calcForceEnergy(), DPM2D.h:547 / DPM2D.cu:1962-2104 */
+int dpm_calc_force_energy(dpm_set_t h);
+
+/* ---- integrators ---------------------------------------------------------- */
+/* initLangevin(T, gamma, readState) / langevinLoop(), DPM2D.h:680-682.  zeroMomentum
+ * removes the mean velocity after injection (an addition; the reference does not). */
+int dpm_init_langevin(dpm_set_t h, double T, double gamma, int readState, int zeroMomentum);
+int dpm_langevin_loop(dpm_set_t h, long steps);
+/* initNVE(T, readState) / NVELoop(), DPM2D.h:684-686 */
+int dpm_init_nve(dpm_set_t h, double T, int readState, int zeroMomentum);
+int dpm_nve_loop(dpm_set_t h, long steps);
+/* initNVERescale(T) / NVERescaleLoop(), DPM2D.h:688-690 */
+int dpm_init_nve_rescale(dpm_set_t h, double T);
+int dpm_nve_rescale_loop(dpm_set_t h, long steps);
+/* adjustTemperature(T), DPM2D.h:437 (per system) */
+int dpm_adjust_temperature(dpm_set_t h, double T);
+
+/* ---- FIRE ----------------------------------------------------------------- */
+/* initFIRE(params{a_start, f_dec, f_inc, f_a, dt, dt_max, a}, minStep, numStep, numDOF),
+ * DPM2D.h:665; the state is per system */
+int dpm_init_fire(dpm_set_t h, const double* params7, long minStep, long numStep);
+/* vertexFIRELoop() x iterations, DPM2D.h:669 / FIRE.cu:397-411 (all systems advance) */
+int dpm_fire_loop(dpm_set_t h, long iterations);
+/* FIRE state per system: out[5*s + {0..4}] = fire_dt, a, numStep, last P = v.F, iterations */
+int dpm_get_fire_state(dpm_set_t h, double* out);
+/* The compressFIRE.cpp:121-131 loop for every system at once: iterate each system until
+ * getMaxUnbalancedForce() <= tol or maxIter; converged systems stop moving.  Runs without
+ * host synchronisation; iterations[s] receives the per-system count. */
+int dpm_fire_minimize(dpm_set_t h, double tol, long maxIter, long* iterations);
+
+/* ---- observables (per system) --------------------------------------------- */
+/* getPotentialEnergy / getKineticEnergy / getTemperature, DPM2D.h:417-420 */
+int dpm_get_potential_energy(dpm_set_t h, double* perSystem);
+int dpm_get_kinetic_energy(dpm_set_t h, double* perSystem);
+int dpm_get_temperature(dpm_set_t h, double* perSystem);
+/* getMaxUnbalancedForce / getTotalForceMagnitude, DPM2D.h:408-409 */
+int dpm_get_max_unbalanced_force(dpm_set_t h, double* perSystem);
+int dpm_get_total_force_magnitude(dpm_set_t h, double* perSystem);
+/* getPhi / getPreferredPhi, DPM2D.h:439-440 */
+int dpm_get_phi(dpm_set_t h, double* perSystem);
+int dpm_get_preferred_phi(dpm_set_t h, double* perSystem);
+/* getParticleShapes (A, P, P^2/4piA per ring), DPM2D.h:296 */
+int dpm_get_particle_shapes(dpm_set_t h, double* shape3);
+/* scaleVertices(scale) per system, DPM2D.h:492 / DPM2D.cu:1802-1811 */
+int dpm_scale_vertices(dpm_set_t h, const double* scalePerSystem);
+
+/* ---- measurement (no reference counterpart) ------------------------------- */
+/* Average duration of the fused Langevin step kernel (ring shape + interaction + thermostat +
+ * kicks + drift) over `reps` back-to-back launches on the handle's stream, timed with hipEvents
+ * on that stream.  Velocities are restored afterwards; *meanNeighbors = mean list length. */
+int dpm_time_step_kernel(dpm_set_t h, long reps, double* avgMicroseconds, double* meanNeighbors);
+
+/* ---- adjoint (no reference counterpart: differentiation is an addition) ---- */
+/* Hessian-vector product of the potential energy: out = H u (device SoA arrays of length V,
+ * caller-owned, on the handle's stream), evaluated at the current positions. */
+int dpm_force_hvp(dpm_set_t h, const double* ux, const double* uy, double* outx, double* outy);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DPMRING_H */

@@ -1,0 +1,1040 @@
+/*
+ * dpm_oracle.c -- CPU oracle for the DPM2D vertex engine (TEST INFRASTRUCTURE).
+ *
+ * A host restatement of the reference's arithmetic, written for checking the
+ * HIP library.  It follows the reference's expression order so that results
+ * agree to rounding; where the reference sums with atomics (smooth interaction)
+ * the oracle fixes the sequential order of the reference's thread loop.
+ * Citations are to files of farceri/cudaRing.
+ */
+#include "dpm_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+static const double PI = 3.141592653589793238462643383279502884197; /* defs.h:14 */
+static const double WCAcut = 1.122462048;                            /* defs.h:17 */
+
+enum { TAG_THERMAL = 1, TAG_INJECT = 2, TAG_ACTIVE = 3 };
+
+struct orc_state {
+  long numParticles, numVertices;
+  long *numVertexInParticleList, *firstVertexInParticleId, *particleIdList;
+  double boxSize[2];
+  int potentialType, interactionType, monomerType, particleType;
+  double dt, ea, el, eb, ec, l1, l2, LJcutoff, LJecut, LJfshift, stiff, extSq;
+  double driving, taup;
+  double *pos, *vel, *force, *energy, *lastPos;
+  double *rad, *l0, *theta0, *theta, *length;
+  double *a0, *area, *perimeter, *particlePos, *particleEnergy, *particleAngle;
+  long *neighborList, neighborListSize, *maxNeighborList, maxNeighbors;
+  double cutDistance, cutoff;
+  long updateCount;
+  int shift;
+  int simKind;
+  double Tinject, gamma, noise;
+  double a_start, a, f_dec, f_inc, f_a, fire_dt_max, fire_dt, lastP;
+  long minStep, numStep;
+  uint64_t seed;
+  uint32_t systemId;
+  uint64_t thermalCalls, injectCalls, activeCalls;
+};
+
+/* ------------------------------------------------------------------ RNG --- */
+/* Philox4x32-10: 10 rounds of the 4x32 Philox S-box with 9 key bumps
+ * (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3"). */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; r++) {
+    if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static double u01_from(uint32_t hi, uint32_t lo) {
+  uint64_t v = ((uint64_t)hi << 32) | lo;
+  return ((double)(v >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+void orc_gauss_pair(uint64_t seed, uint32_t systemId, uint32_t tag, uint64_t counter,
+                    uint32_t index, double* z0, double* z1) {
+  uint32_t ctr[4] = {index, (uint32_t)counter, (uint32_t)(counter >> 32),
+                     (tag << 24) | (systemId & 0xFFFFFFu)};
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  uint32_t out[4];
+  orc_philox4x32_10(ctr, key, out);
+  double u1 = u01_from(out[0], out[1]);
+  double u2 = u01_from(out[2], out[3]);
+  double r = sqrt(-2.0 * log(u1));
+  double ang = 2.0 * PI * u2;
+  *z0 = r * cos(ang);
+  *z1 = r * sin(ang);
+}
+
+/* ------------------------------------------------------------ lifecycle --- */
+orc_state* orc_create(long numParticles, const long* nvp) {
+  orc_state* s = (orc_state*)calloc(1, sizeof(orc_state));
+  s->numParticles = numParticles;
+  s->numVertexInParticleList = (long*)malloc(sizeof(long) * numParticles);
+  s->firstVertexInParticleId = (long*)malloc(sizeof(long) * numParticles);
+  long nv = 0;
+  for (long p = 0; p < numParticles; p++) {
+    s->numVertexInParticleList[p] = nvp[p];
+    s->firstVertexInParticleId[p] = nv; /* initParticleIdList, DPM2D.cu:282-300 */
+    nv += nvp[p];
+  }
+  s->numVertices = nv;
+  s->particleIdList = (long*)malloc(sizeof(long) * nv);
+  for (long p = 0; p < numParticles; p++)
+    for (long k = 0; k < nvp[p]; k++) s->particleIdList[s->firstVertexInParticleId[p] + k] = p;
+  long n2 = 2 * nv, np = numParticles;
+  s->pos = calloc(n2, 8); s->vel = calloc(n2, 8); s->force = calloc(n2, 8);
+  s->lastPos = calloc(n2, 8); s->energy = calloc(nv, 8);
+  s->rad = calloc(nv, 8); s->l0 = calloc(nv, 8); s->theta0 = calloc(nv, 8);
+  s->theta = calloc(nv, 8); s->length = calloc(nv, 8);
+  s->a0 = calloc(np, 8); s->area = calloc(np, 8); s->perimeter = calloc(np, 8);
+  s->particlePos = calloc(2 * np, 8); s->particleEnergy = calloc(np, 8);
+  s->particleAngle = calloc(np, 8);
+  s->maxNeighborList = (long*)calloc(nv, sizeof(long));
+  s->neighborList = NULL;
+  s->neighborListSize = 0;
+  /* defaults of the DPM2D constructor, DPM2D.cu:41-94 */
+  s->boxSize[0] = s->boxSize[1] = 1.0;
+  s->dt = 1e-3; s->ea = 1e5; s->el = 20; s->eb = 10; s->ec = 1;
+  s->cutDistance = 1; s->updateCount = 0; s->shift = 0;
+  s->gamma = 1; s->noise = 0;
+  return s;
+}
+
+void orc_destroy(orc_state* s) {
+  if (!s) return;
+  free(s->numVertexInParticleList); free(s->firstVertexInParticleId); free(s->particleIdList);
+  free(s->pos); free(s->vel); free(s->force); free(s->lastPos); free(s->energy);
+  free(s->rad); free(s->l0); free(s->theta0); free(s->theta); free(s->length);
+  free(s->a0); free(s->area); free(s->perimeter); free(s->particlePos);
+  free(s->particleEnergy); free(s->particleAngle); free(s->maxNeighborList);
+  free(s->neighborList);
+  free(s);
+}
+
+long orc_num_vertices(const orc_state* s) { return s->numVertices; }
+
+static double* field_ptr(const orc_state* s, int f) {
+  switch (f) {
+    case ORC_F_POS: return s->pos; case ORC_F_VEL: return s->vel;
+    case ORC_F_FORCE: return s->force; case ORC_F_ENERGY: return s->energy;
+    case ORC_F_RAD: return s->rad; case ORC_F_L0: return s->l0;
+    case ORC_F_THETA0: return s->theta0; case ORC_F_THETA: return s->theta;
+    case ORC_F_LENGTH: return s->length; case ORC_F_A0: return s->a0;
+    case ORC_F_AREA: return s->area; case ORC_F_PERIMETER: return s->perimeter;
+    case ORC_F_PPOS: return s->particlePos; case ORC_F_PENERGY: return s->particleEnergy;
+    case ORC_F_PANGLE: return s->particleAngle; case ORC_F_LASTPOS: return s->lastPos;
+    case ORC_F_BOX: return (double*)s->boxSize;
+  }
+  return NULL;
+}
+
+long orc_field_size(const orc_state* s, int f) {
+  switch (f) {
+    case ORC_F_POS: case ORC_F_VEL: case ORC_F_FORCE: case ORC_F_LASTPOS: return 2 * s->numVertices;
+    case ORC_F_ENERGY: case ORC_F_RAD: case ORC_F_L0: case ORC_F_THETA0: case ORC_F_THETA:
+    case ORC_F_LENGTH: return s->numVertices;
+    case ORC_F_A0: case ORC_F_AREA: case ORC_F_PERIMETER: case ORC_F_PENERGY:
+    case ORC_F_PANGLE: return s->numParticles;
+    case ORC_F_PPOS: return 2 * s->numParticles;
+    case ORC_F_BOX: return 2;
+  }
+  return -1;
+}
+
+int orc_set_field(orc_state* s, int f, const double* data, long n) {
+  double* p = field_ptr(s, f);
+  if (!p || n != orc_field_size(s, f)) return -1;
+  memcpy(p, data, sizeof(double) * n);
+  return 0;
+}
+
+int orc_get_field(const orc_state* s, int f, double* data, long n) {
+  double* p = field_ptr(s, f);
+  if (!p || n != orc_field_size(s, f)) return -1;
+  memcpy(data, p, sizeof(double) * n);
+  return 0;
+}
+
+/* ----------------------------------------------------------- parameters --- */
+void orc_set_types(orc_state* s, int pot, int inter, int mono, int part) {
+  s->potentialType = pot; s->interactionType = inter; s->monomerType = mono; s->particleType = part;
+}
+/* setEnergyCosts, DPM2D.cu:1875-1884 */
+void orc_set_energy_costs(orc_state* s, double ea, double el, double eb, double ec) {
+  s->ea = ea; s->el = el; s->eb = eb; s->ec = ec;
+}
+/* setAttractionConstants, DPM2D.cu:1915-1920 */
+void orc_set_attraction_constants(orc_state* s, double l1, double l2) { s->l1 = l1; s->l2 = l2; }
+/* setLJcutoff, DPM2D.cu:1922-1932 (uses the ec in force at call time) */
+void orc_set_lj_cutoff(orc_state* s, double LJcutoff) {
+  s->LJcutoff = LJcutoff;
+  double ratio6 = 1 / pow(LJcutoff, 6);
+  s->LJecut = 4 * s->ec * (ratio6 * ratio6 - ratio6);
+  s->LJfshift = 24 * s->ec * (2 * ratio6 - 1) * ratio6 / LJcutoff;
+}
+/* setFENEconstants, DPM2D.cu:1934-1939 */
+void orc_set_fene_constants(orc_state* s, double stiff, double ext) { s->stiff = stiff; s->extSq = ext * ext; }
+void orc_set_time_step(orc_state* s, double dt) { s->dt = dt; }
+void orc_set_self_propulsion(orc_state* s, double driving, double taup) { s->driving = driving; s->taup = taup; }
+void orc_set_rng(orc_state* s, uint64_t seed, uint32_t systemId) {
+  s->seed = seed; s->systemId = systemId;
+  s->thermalCalls = s->injectCalls = s->activeCalls = 0;
+}
+
+/* ------------------------------------------------------------- geometry --- */
+/* pbcDistance, cudaKernel.cuh:70-74: round() is half-away-from-zero */
+double orc_pbc_distance(double x1, double x2, double size) {
+  double delta = x1 - x2;
+  return delta - size * round(delta / size);
+}
+
+/* getNextId, cudaKernel.cuh:220-230 */
+long orc_next_id(const orc_state* s, long vertexId, long particleId) {
+  if (vertexId == s->numVertices - 1) return vertexId - s->numVertexInParticleList[particleId] + 1;
+  long nextId = vertexId + 1;
+  if (s->particleIdList[nextId] == particleId) return nextId;
+  return nextId - s->numVertexInParticleList[particleId];
+}
+
+/* getPreviousId, cudaKernel.cuh:233-243 */
+long orc_previous_id(const orc_state* s, long vertexId, long particleId) {
+  if (vertexId == 0) return s->numVertexInParticleList[particleId] - 1;
+  long previousId = vertexId - 1;
+  if (s->particleIdList[previousId] == particleId) return previousId;
+  return previousId + s->numVertexInParticleList[particleId];
+}
+
+static inline double pbcd(const orc_state* s, double x1, double x2, int dim) {
+  return orc_pbc_distance(x1, x2, s->boxSize[dim]);
+}
+
+/* calcAngle, cudaKernel.cuh:423-427 */
+static double calc_angle(const double* n, const double* p) {
+  double midSine = n[0] * p[1] - n[1] * p[0];
+  double midCosine = n[0] * p[0] + n[1] * p[1];
+  return atan2(midSine, midCosine);
+}
+
+/* kernelCalcParticleShape, cudaKernel.cuh:1571-1606 (one ring walked serially) */
+void orc_calc_particle_shape(orc_state* s) {
+  for (long pId = 0; pId < s->numParticles; pId++) {
+    double tempArea = 0.0;
+    s->perimeter[pId] = 0.0;
+    long firstId = s->firstVertexInParticleId[pId];
+    double delta[2], nextPos[2], currentPos[2], nextSegment[2], previousPos[2], previousSegment[2];
+    currentPos[0] = s->pos[firstId * 2]; currentPos[1] = s->pos[firstId * 2 + 1];
+    for (long cur = firstId; cur < firstId + s->numVertexInParticleList[pId]; cur++) {
+      long nextId = orc_next_id(s, cur, pId);
+      long previousId = orc_previous_id(s, cur, pId);
+      double segmentLength = 0.0;
+      for (int dim = 0; dim < 2; dim++) {
+        delta[dim] = pbcd(s, s->pos[nextId * 2 + dim], currentPos[dim], dim);
+        nextPos[dim] = currentPos[dim] + delta[dim];
+        segmentLength += delta[dim] * delta[dim];
+        delta[dim] = pbcd(s, s->pos[previousId * 2 + dim], currentPos[dim], dim);
+        previousPos[dim] = currentPos[dim] + delta[dim];
+      }
+      for (int dim = 0; dim < 2; dim++) {
+        nextSegment[dim] = nextPos[dim] - currentPos[dim];
+        previousSegment[dim] = currentPos[dim] - previousPos[dim];
+      }
+      s->theta[cur] = calc_angle(nextSegment, previousSegment);
+      s->length[cur] = sqrt(segmentLength);
+      s->perimeter[pId] += s->length[cur];
+      tempArea += currentPos[0] * nextPos[1] - nextPos[0] * currentPos[1];
+      currentPos[0] = nextPos[0]; currentPos[1] = nextPos[1];
+    }
+    s->area[pId] = fabs(tempArea) * 0.5;
+  }
+}
+
+/* calcParticlePos, cudaKernel.cuh:365-388, launched by kernelCalcParticlePositions 1608-1613 */
+void orc_calc_particle_positions(orc_state* s) {
+  for (long pId = 0; pId < s->numParticles; pId++) {
+    double currentPos[2], delta[2], partPos[2];
+    long firstId = s->firstVertexInParticleId[pId];
+    currentPos[0] = s->pos[firstId * 2]; currentPos[1] = s->pos[firstId * 2 + 1];
+    partPos[0] = currentPos[0]; partPos[1] = currentPos[1];
+    for (long cur = firstId; cur < firstId + s->numVertexInParticleList[pId] - 1; cur++) {
+      long nextId = orc_next_id(s, cur, pId);
+      for (int dim = 0; dim < 2; dim++) {
+        delta[dim] = pbcd(s, s->pos[nextId * 2 + dim], currentPos[dim], dim);
+        currentPos[dim] += delta[dim];
+        partPos[dim] += currentPos[dim];
+      }
+    }
+    for (int dim = 0; dim < 2; dim++) s->particlePos[pId * 2 + dim] = partPos[dim] / s->numVertexInParticleList[pId];
+  }
+}
+
+/* ---------------------------------------------------------- shape force --- */
+/* calcAreaForceEnergy, cudaKernel.cuh:429-435 */
+static double area_force_energy(const orc_state* s, double pA0, double pA, const double* nPos,
+                                const double* pPos, double* f) {
+  double deltaA = (pA / pA0) - 1.;
+  double gradMultiple = s->ea * deltaA / pA0;
+  f[0] += 0.5 * gradMultiple * (pPos[1] - nPos[1]);
+  f[1] += 0.5 * gradMultiple * (nPos[0] - pPos[0]);
+  return (0.5 * s->ea * deltaA * deltaA);
+}
+
+/* calcPerimeterForceEnergy, cudaKernel.cuh:437-447 */
+static double perimeter_force_energy(const orc_state* s, double tL0, double pL0, double nLength,
+                                     double pLength, const double* vPos, const double* nPos,
+                                     const double* pPos, double* f) {
+  double pDelta = (pLength / pL0) - 1.;
+  double nDelta = (nLength / tL0) - 1.;
+  for (int dim = 0; dim < 2; dim++)
+    f[dim] += s->el * ((nDelta * (nPos[dim] - vPos[dim]) / (tL0 * nLength)) -
+                       (pDelta * (vPos[dim] - pPos[dim]) / (pL0 * pLength)));
+  return (0.5 * s->el * pDelta * pDelta);
+}
+
+/* calcFENEPerimeterForceEnergy, cudaKernel.cuh:449-455 (energy sign kept as in the reference) */
+static double fene_force_energy(const orc_state* s, double tL0, double pL0, double nLength,
+                                double pLength, const double* vPos, const double* nPos,
+                                const double* pPos, double* f) {
+  for (int dim = 0; dim < 2; dim++)
+    f[dim] += 2 * s->el * s->stiff * s->extSq *
+              ((nLength / (s->extSq * tL0 * tL0 - nLength * nLength)) * (nPos[dim] - vPos[dim]) / nLength -
+               (pLength / (s->extSq * pL0 * pL0 - pLength * pLength)) * (vPos[dim] - pPos[dim]) / pLength);
+  return s->el * s->stiff * s->extSq * log(1 - pLength * pLength / (s->extSq * pL0 * pL0));
+}
+
+/* calcBendingForceEnergy, cudaKernel.cuh:457-471 */
+static double bending_force_energy(const orc_state* s, const double* preSeg, const double* nextSeg,
+                                   double thisD, double nextD, double preD, double* f) {
+  double preN[2] = {preSeg[1], -preSeg[0]};
+  double nextN[2] = {nextSeg[1], -nextSeg[0]};
+  double preVar = (thisD - preD) / (preSeg[0] * preSeg[0] + preSeg[1] * preSeg[1]);
+  double nextVar = (thisD - nextD) / (nextSeg[0] * nextSeg[0] + nextSeg[1] * nextSeg[1]);
+  for (int dim = 0; dim < 2; dim++) f[dim] += s->eb * (preVar * preN[dim] + nextVar * nextN[dim]);
+  return (0.5 * s->eb * thisD * thisD);
+}
+
+/* kernelCalcShapeForceEnergy, cudaKernel.cuh:474-529: zeroes F and overwrites energy */
+void orc_calc_shape_force_energy(orc_state* s) {
+  for (long v = 0; v < s->numVertices; v++) {
+    long p = s->particleIdList[v];
+    double vertexPos[2], nextPos[2], previousPos[2], partPos[2], nextSegment[2], previousSegment[2];
+    double particleArea = s->area[p];
+    double shapeEnergy = 0.0;
+    long nextId = orc_next_id(s, v, p);
+    long previousId = orc_previous_id(s, v, p);
+    double* f = &s->force[v * 2];
+    for (int dim = 0; dim < 2; dim++) {
+      f[dim] = 0.0;
+      partPos[dim] = s->particlePos[p * 2 + dim];
+    }
+    for (int dim = 0; dim < 2; dim++) { /* getRelativeVertexPos, cudaKernel.cuh:273-279 */
+      vertexPos[dim] = pbcd(s, s->pos[v * 2 + dim], partPos[dim], dim);
+      nextPos[dim] = pbcd(s, s->pos[nextId * 2 + dim], partPos[dim], dim);
+      previousPos[dim] = pbcd(s, s->pos[previousId * 2 + dim], partPos[dim], dim);
+    }
+    shapeEnergy += area_force_energy(s, s->a0[p], particleArea, nextPos, previousPos, f) /
+                   s->numVertexInParticleList[p];
+    for (int dim = 0; dim < 2; dim++) {
+      nextSegment[dim] = nextPos[dim] - vertexPos[dim];
+      previousSegment[dim] = vertexPos[dim] - previousPos[dim];
+    }
+    double previousLength = sqrt(previousSegment[0] * previousSegment[0] + previousSegment[1] * previousSegment[1]);
+    double nextLength = sqrt(nextSegment[0] * nextSegment[0] + nextSegment[1] * nextSegment[1]);
+    if (s->monomerType == ORC_MONO_HARMONIC)
+      shapeEnergy += perimeter_force_energy(s, s->l0[v], s->l0[previousId], nextLength, previousLength,
+                                            vertexPos, nextPos, previousPos, f);
+    else
+      shapeEnergy += fene_force_energy(s, s->l0[v], s->l0[previousId], nextLength, previousLength,
+                                       vertexPos, nextPos, previousPos, f);
+    double previousAngleDelta = s->theta[previousId] - s->theta0[previousId];
+    double thisAngleDelta = s->theta[v] - s->theta0[v];
+    double nextAngleDelta = s->theta[nextId] - s->theta0[nextId];
+    shapeEnergy += bending_force_energy(s, previousSegment, nextSegment, thisAngleDelta, nextAngleDelta,
+                                        previousAngleDelta, f);
+    s->energy[v] = shapeEnergy;
+  }
+}
+
+/* --------------------------------------------------------- pair forces --- */
+/* calcDeltaAndDistance (normal geometry), cudaKernel.cuh:160-171 */
+static double delta_and_distance(const orc_state* s, const double* a, const double* b, double* d) {
+  double distanceSq = 0.0;
+  for (int dim = 0; dim < 2; dim++) {
+    d[dim] = pbcd(s, a[dim], b[dim], dim);
+    distanceSq += d[dim] * d[dim];
+  }
+  return sqrt(distanceSq);
+}
+
+static double calc_distance(const orc_state* s, const double* a, const double* b) {
+  double d[2];
+  return delta_and_distance(s, a, b, d);
+}
+
+/* Directed pair force on `this` from `other` and the directed-visit energy:
+ * calcContactInteraction 531-544, calcLJInteraction 562-583,
+ * calcAdhesiveInteraction 604-626, calcWCAInteraction 653-668. */
+static double pair_interaction(const orc_state* s, const double* thisPos, const double* otherPos,
+                               double radSum, double* f) {
+  double delta[2];
+  double distance = delta_and_distance(s, thisPos, otherPos, delta);
+  switch (s->potentialType) {
+    case ORC_HARMONIC: {
+      double overlap = 1 - distance / radSum;
+      if (overlap > 0) {
+        double gradMultiple = s->ec * overlap / radSum;
+        for (int dim = 0; dim < 2; dim++) f[dim] += gradMultiple * delta[dim] / distance;
+        return (0.5 * s->ec * overlap * overlap) * 0.5;
+      }
+      return 0.0;
+    }
+    case ORC_LJ: {
+      double ratio = radSum / distance;
+      double ratio6 = pow(ratio, 6);
+      double ratio12 = ratio6 * ratio6;
+      if (distance <= (s->LJcutoff * radSum)) {
+        double forceShift = s->LJfshift / radSum;
+        double gradMultiple = 24 * s->ec * (2 * ratio12 - ratio6) / distance - forceShift;
+        for (int dim = 0; dim < 2; dim++) f[dim] += gradMultiple * delta[dim] / distance;
+        return 0.5 * (4 * s->ec * (ratio12 - ratio6) - s->LJecut -
+                      fabs(forceShift) * (distance - s->LJcutoff * radSum));
+      }
+      return 0.0;
+    }
+    case ORC_ADHESIVE: {
+      double gradMultiple = 0.0, epot = 0.0;
+      double overlap = 1 - distance / radSum;
+      if (distance < (1 + s->l2) * radSum) {
+        if (distance < (1 + s->l1) * radSum) {
+          gradMultiple = s->ec * overlap / radSum;
+          epot = 0.5 * s->ec * (overlap * overlap - s->l1 * s->l2) * 0.5;
+        } else {
+          gradMultiple = -(s->ec * s->l1 / (s->l2 - s->l1)) * (overlap + s->l2) / radSum;
+          epot = -(0.5 * (s->ec * s->l1 / (s->l2 - s->l1)) * (overlap + s->l2) * (overlap + s->l2)) * 0.5;
+        }
+        for (int dim = 0; dim < 2; dim++) f[dim] += gradMultiple * delta[dim] / distance;
+        return epot;
+      }
+      return 0.0;
+    }
+    case ORC_WCA: {
+      double ratio = radSum / distance;
+      double ratio6 = pow(ratio, 6);
+      double ratio12 = ratio6 * ratio6;
+      if (distance <= (WCAcut * radSum)) {
+        double gradMultiple = 24 * s->ec * (2 * ratio12 - ratio6) / distance;
+        for (int dim = 0; dim < 2; dim++) f[dim] += gradMultiple * delta[dim] / distance;
+        return 0.5 * s->ec * (4 * (ratio12 - ratio6) + 1);
+      }
+      return 0.0;
+    }
+  }
+  return 0.0;
+}
+
+/* calcGradMultipleAndEnergy, cudaKernel.cuh:806-858 (scalar form used by the smooth kernel) */
+double orc_pair_grad_multiple_and_energy(const orc_state* s, double distance, double radSum, double* epot) {
+  double overlap, ratio, ratio6, ratio12;
+  switch (s->potentialType) {
+    case ORC_HARMONIC:
+      overlap = 1 - distance / radSum;
+      if (overlap > 0) {
+        *epot = 0.5 * s->ec * overlap * overlap * 0.5;
+        return s->ec * overlap / radSum;
+      }
+      return 0;
+    case ORC_LJ:
+      ratio = radSum / distance;
+      ratio12 = pow(ratio, 12);
+      ratio6 = pow(ratio, 6);
+      if (distance <= (s->LJcutoff * radSum)) {
+        double forceShift = s->LJfshift / radSum;
+        *epot = 0.5 * (4 * s->ec * (ratio12 - ratio6) - s->LJecut - fabs(forceShift) * (distance - s->LJcutoff * radSum));
+        return 24 * s->ec * (2 * ratio12 - ratio6) / distance - forceShift;
+      }
+      return 0;
+    case ORC_ADHESIVE:
+      overlap = 1 - distance / radSum;
+      if (distance < (1 + s->l1) * radSum) {
+        *epot = 0.5 * s->ec * (overlap * overlap - s->l1 * s->l2) * 0.5;
+        return s->ec * overlap / radSum;
+      } else if ((distance >= (1 + s->l1) * radSum) && (distance < (1 + s->l2) * radSum)) {
+        *epot = -(0.5 * (s->ec * s->l1 / (s->l2 - s->l1)) * (overlap + s->l2) * (overlap + s->l2)) * 0.5;
+        return -(s->ec * s->l1 / (s->l2 - s->l1)) * (overlap + s->l2) / radSum;
+      }
+      return 0;
+    case ORC_WCA:
+      ratio = radSum / distance;
+      ratio6 = pow(ratio, 6);
+      ratio12 = ratio6 * ratio6;
+      if (distance <= (WCAcut * radSum)) {
+        *epot = 0.5 * s->ec * (4 * (ratio12 - ratio6) + 1);
+        return 4 * s->ec * (12 * ratio12 - 6 * ratio6) / distance;
+      }
+      return 0;
+  }
+  return 0;
+}
+
+/* kernelCalcVertexInteraction, cudaKernel.cuh:690-722 (gather over the Verlet row) */
+void orc_calc_vertex_interaction(orc_state* s) {
+  long W = s->neighborListSize;
+  for (long v = 0; v < s->numVertices; v++) {
+    double thisPos[2] = {s->pos[v * 2], s->pos[v * 2 + 1]};
+    double thisRad = s->rad[v];
+    for (long k = 0; k < s->maxNeighborList[v]; k++) {
+      long otherId = s->neighborList[v * W + k];
+      if ((v != otherId) && (otherId != -1)) {
+        double otherPos[2] = {s->pos[otherId * 2], s->pos[otherId * 2 + 1]};
+        double radSum = thisRad + s->rad[otherId];
+        s->energy[v] += pair_interaction(s, thisPos, otherPos, radSum, &s->force[v * 2]);
+      }
+    }
+  }
+}
+
+/* kernelCalcAllToAllVertexInteraction, cudaKernel.cuh:725-757 */
+void orc_calc_all_to_all_interaction(orc_state* s) {
+  for (long v = 0; v < s->numVertices; v++) {
+    double thisPos[2] = {s->pos[v * 2], s->pos[v * 2 + 1]};
+    double thisRad = s->rad[v];
+    for (long o = 0; o < s->numVertices; o++) {
+      if ((s->particleIdList[v] != s->particleIdList[o]) && (o != -1)) {
+        double otherPos[2] = {s->pos[o * 2], s->pos[o * 2 + 1]};
+        double radSum = thisRad + s->rad[o];
+        s->energy[v] += pair_interaction(s, thisPos, otherPos, radSum, &s->force[v * 2]);
+      }
+    }
+  }
+}
+
+/* getProjection, cudaKernel.cuh:861-864 */
+static double projection(const orc_state* s, const double* thisPos, const double* otherPos,
+                         const double* previousPos, double length) {
+  return (pbcd(s, thisPos[0], previousPos[0], 0) * pbcd(s, otherPos[0], previousPos[0], 0) +
+          pbcd(s, thisPos[1], previousPos[1], 1) * pbcd(s, otherPos[1], previousPos[1], 1)) /
+         (length * length);
+}
+
+/* calcCross, cudaKernel.cuh:873-875 */
+static double cross_of(const orc_state* s, const double* thisPos, const double* otherPos,
+                       const double* previousPos) {
+  return pbcd(s, previousPos[0], otherPos[0], 0) * pbcd(s, otherPos[1], thisPos[1], 1) -
+         pbcd(s, otherPos[0], thisPos[0], 0) * pbcd(s, previousPos[1], otherPos[1], 1);
+}
+
+/* calcVertexSegmentInteraction, cudaKernel.cuh:877-897.  The reference divides by |cross|;
+ * a vertex exactly on the segment line gives 0/0 there, the oracle takes sign 0. */
+static double vertex_segment(orc_state* s, const double* thisPos, const double* projPos,
+                             const double* otherPos, const double* previousPos, double length,
+                             double radSum, double* tF, double* oF, double* pF) {
+  double epot = 0.0;
+  double gm = orc_pair_grad_multiple_and_energy(s, calc_distance(s, thisPos, projPos), radSum, &epot);
+  if (gm != 0) {
+    double cross = cross_of(s, thisPos, otherPos, previousPos);
+    double absCross = fabs(cross);
+    double sign = absCross > 0 ? cross / absCross : 0.0;
+    double L2 = length * length;
+    tF[0] += gm * sign * pbcd(s, previousPos[1], otherPos[1], 1) / length;
+    tF[1] += gm * sign * pbcd(s, otherPos[0], previousPos[0], 0) / length;
+    oF[0] += gm * (sign * pbcd(s, thisPos[1], previousPos[1], 1) + absCross * pbcd(s, previousPos[0], otherPos[0], 0) / L2) / length;
+    oF[1] += gm * (sign * pbcd(s, previousPos[0], thisPos[0], 0) + absCross * pbcd(s, previousPos[1], otherPos[1], 1) / L2) / length;
+    pF[0] += gm * (sign * pbcd(s, otherPos[1], thisPos[1], 1) - absCross * pbcd(s, previousPos[0], otherPos[0], 0) / L2) / length;
+    pF[1] += gm * (sign * pbcd(s, thisPos[0], otherPos[0], 0) - absCross * pbcd(s, previousPos[1], otherPos[1], 1) / L2) / length;
+    return epot;
+  }
+  return 0.;
+}
+
+/* calcVertexVertexInteraction, cudaKernel.cuh:899-912 */
+static double vertex_vertex(orc_state* s, const double* thisPos, const double* previousPos, double radSum,
+                            double* thisF, double* otherF) {
+  double delta[2], epot = 0.0;
+  double gm = orc_pair_grad_multiple_and_energy(s, calc_distance(s, thisPos, previousPos), radSum, &epot);
+  if (gm != 0) {
+    double distance = delta_and_distance(s, thisPos, previousPos, delta);
+    for (int dim = 0; dim < 2; dim++) {
+      thisF[dim] += gm * delta[dim] / distance;
+      otherF[dim] += -gm * delta[dim] / distance;
+    }
+    return epot;
+  }
+  return 0.;
+}
+
+/* kernelCalcSmoothInteraction, cudaKernel.cuh:922-975.  The reference scatters with
+ * atomicAdd; the oracle applies the same updates in vertex order, then list order. */
+void orc_calc_smooth_interaction(orc_state* s) {
+  long W = s->neighborListSize;
+  for (long v = 0; v < s->numVertices; v++) {
+    double thisPos[2] = {s->pos[v * 2], s->pos[v * 2 + 1]};
+    double thisRad = s->rad[v];
+    long particleId = s->particleIdList[v];
+    for (long k = 0; k < s->maxNeighborList[v]; k++) {
+      long otherId = s->neighborList[v * W + k];
+      if (!((v != otherId) && (otherId != -1))) continue;
+      double otherPos[2] = {s->pos[otherId * 2], s->pos[otherId * 2 + 1]};
+      double radSum = thisRad + s->rad[otherId];
+      long otherParticleId = s->particleIdList[otherId];
+      long previousId = orc_previous_id(s, otherId, otherParticleId);
+      double previousPos[2] = {s->pos[previousId * 2], s->pos[previousId * 2 + 1]};
+      double segment[2] = {pbcd(s, otherPos[0], previousPos[0], 0), pbcd(s, otherPos[1], previousPos[1], 1)};
+      double length = sqrt(segment[0] * segment[0] + segment[1] * segment[1]);
+      double proj = projection(s, thisPos, otherPos, previousPos, length);
+      long secondPreviousId = orc_previous_id(s, previousId, otherParticleId);
+      double secondPreviousPos[2] = {s->pos[secondPreviousId * 2], s->pos[secondPreviousId * 2 + 1]};
+      double previousSegment[2] = {pbcd(s, secondPreviousPos[0], previousPos[0], 0),
+                                   pbcd(s, secondPreviousPos[1], previousPos[1], 1)};
+      double previousLength = sqrt(previousSegment[0] * previousSegment[0] + previousSegment[1] * previousSegment[1]);
+      double previousProj = projection(s, thisPos, previousPos, secondPreviousPos, previousLength);
+      int isPreviousVertexSegment = 0;
+      if (proj >= 0 && proj < 1) {
+        double reduced = fmax(0.0, fmin(1.0, proj)); /* getProjectionPos, 866-871 */
+        double projPos[2] = {previousPos[0] + reduced * segment[0], previousPos[1] + reduced * segment[1]};
+        double inter = vertex_segment(s, thisPos, projPos, otherPos, previousPos, length, radSum,
+                                      &s->force[v * 2], &s->force[otherId * 2], &s->force[previousId * 2]);
+        s->particleEnergy[particleId] += inter;
+        s->particleEnergy[otherParticleId] += inter;
+        if (previousProj >= 0 && previousProj < 1) {
+          isPreviousVertexSegment = 1;
+          inter = vertex_vertex(s, previousPos, thisPos, radSum, &s->force[v * 2], &s->force[previousId * 2]);
+          s->particleEnergy[particleId] += -inter;
+          s->particleEnergy[otherParticleId] += -inter;
+        }
+      } else if (proj < 0) {
+        if (previousProj >= 1 && isPreviousVertexSegment == 0) {
+          double inter = vertex_vertex(s, thisPos, previousPos, radSum, &s->force[v * 2], &s->force[previousId * 2]);
+          s->particleEnergy[particleId] += inter;
+          s->particleEnergy[otherParticleId] += inter;
+        }
+      }
+    }
+  }
+}
+
+/* addSelfPropulsion, DPM2D.cu:1995-2035 (one Gaussian per ring per call) */
+void orc_add_self_propulsion(orc_state* s) {
+  double amplitude = 0.;
+  if (s->taup != 0) amplitude = sqrt(2.0 * s->dt / s->taup);
+  uint64_t call = s->activeCalls++;
+  for (long p = 0; p < s->numParticles; p++) {
+    double z0, z1;
+    orc_gauss_pair(s->seed, s->systemId, TAG_ACTIVE, call, (uint32_t)p, &z0, &z1);
+    double a = s->particleAngle[p];
+    a += amplitude * z0;
+    a = a + PI;
+    a = a - 2.0 * PI * floor(a / (2.0 * PI));
+    a = a - PI;
+    s->particleAngle[p] = a;
+  }
+  for (long v = 0; v < s->numVertices; v++) {
+    long p = s->particleIdList[v];
+    for (int dim = 0; dim < 2; dim++)
+      s->force[v * 2 + dim] += s->driving * ((1 - dim) * cos(s->particleAngle[p]) + dim * sin(s->particleAngle[p]));
+  }
+}
+
+/* calcForceEnergyGPU, DPM2D.cu:2058-2104 with calcShapeForceEnergy 1976-1993 */
+void orc_calc_force_energy(orc_state* s) {
+  orc_calc_particle_shape(s);
+  orc_calc_particle_positions(s);
+  orc_calc_shape_force_energy(s);
+  memset(s->particleEnergy, 0, sizeof(double) * s->numParticles);
+  switch (s->interactionType) {
+    case ORC_VERTEX_VERTEX: orc_calc_vertex_interaction(s); break;
+    case ORC_VERTEX_SMOOTH: orc_calc_smooth_interaction(s); break;
+    case ORC_ALL: orc_calc_all_to_all_interaction(s); break;
+    default: orc_calc_smooth_interaction(s); break;
+  }
+  if (s->particleType == ORC_ACTIVE) orc_add_self_propulsion(s);
+}
+
+/* ----------------------------------------------------------- neighbours --- */
+/* setDisplacementCutoff, DPM2D.cu:1300-1329 */
+double orc_set_displacement_cutoff(orc_state* s, double cutoff_, double size_) {
+  switch (s->potentialType) {
+    case ORC_HARMONIC: s->cutDistance = 1; break;
+    case ORC_LJ: s->cutDistance = s->LJcutoff; break;
+    case ORC_WCA: s->cutDistance = WCAcut; break;
+    case ORC_ADHESIVE: s->cutDistance = 1 + s->l2; break;
+  }
+  s->cutDistance += cutoff_;
+  s->cutoff = cutoff_ * size_;
+  return s->cutDistance;
+}
+
+double orc_get_cut_distance(const orc_state* s) { return s->cutDistance; }
+
+/* kernelCalcNeighborList, cudaKernel.cuh:1735-1763: ascending otherId; a non-neighbour
+ * writes -1 into the next free slot (later overwritten), so the row holds the
+ * neighbours followed by -1. */
+static void build_list(orc_state* s, double cutDistance) {
+  long W = s->neighborListSize;
+  for (long v = 0; v < s->numVertices; v++) {
+    long added = 0;
+    double thisPos[2] = {s->pos[v * 2], s->pos[v * 2 + 1]};
+    double thisRad = s->rad[v];
+    long particleId = s->particleIdList[v];
+    for (long o = 0; o < s->numVertices; o++) {
+      if (s->particleIdList[o] == particleId) continue;
+      double otherPos[2] = {s->pos[o * 2], s->pos[o * 2 + 1]};
+      double radSum = thisRad + s->rad[o];
+      int isNeighbor = (-(1 - calc_distance(s, thisPos, otherPos) / radSum) < cutDistance);
+      if (added < W) s->neighborList[v * W + added] = isNeighbor ? o : -1;
+      added += isNeighbor;
+    }
+    s->maxNeighborList[v] = added;
+  }
+}
+
+/* calcNeighborList, DPM2D.cu:3479-3507: grow-only power-of-two row width */
+void orc_calc_neighbor_list(orc_state* s, double cutDistance) {
+  long nv = s->numVertices;
+  memset(s->maxNeighborList, 0, sizeof(long) * nv);
+  long W = s->neighborListSize;
+  for (long i = 0; i < nv * W; i++) s->neighborList[i] = -1;
+  build_list(s, cutDistance);
+  long maxN = -1;
+  for (long v = 0; v < nv; v++) if (s->maxNeighborList[v] > maxN) maxN = s->maxNeighborList[v];
+  s->maxNeighbors = maxN;
+  if (maxN > s->neighborListSize) {
+    s->neighborListSize = (long)pow(2, ceil(log2((double)maxN)));
+    W = s->neighborListSize;
+    s->neighborList = (long*)realloc(s->neighborList, sizeof(long) * nv * W);
+    for (long i = 0; i < nv * W; i++) s->neighborList[i] = -1;
+    build_list(s, cutDistance);
+  }
+}
+
+long orc_neighbor_list_size(const orc_state* s) { return s->neighborListSize; }
+
+/* copies the first min(count, cap) entries of each row into list[v*cap + k] */
+long orc_get_neighbors(const orc_state* s, long* list, long* counts, long cap) {
+  long W = s->neighborListSize;
+  for (long v = 0; v < s->numVertices; v++) {
+    counts[v] = s->maxNeighborList[v];
+    for (long k = 0; k < cap; k++) list[v * cap + k] = (k < counts[v] && k < W) ? s->neighborList[v * W + k] : -1;
+  }
+  return W;
+}
+
+void orc_reset_last_positions(orc_state* s) { memcpy(s->lastPos, s->pos, sizeof(double) * 2 * s->numVertices); }
+long orc_get_update_count(const orc_state* s) { return s->updateCount; }
+void orc_reset_update_count(orc_state* s) { s->updateCount = 0; }
+
+/* checkDisplacement + kernelCheckVertexDisplacement, DPM2D.cu:1390-1406, cudaKernel.cuh:1906-1914.
+ * removeCOMDrift (DPM2D.cu:1348-1374, run when `shift`) reads out of bounds in the
+ * reference; it is treated as a no-op here. */
+int orc_check_neighbors(orc_state* s) {
+  int any = 0;
+  for (long v = 0; v < s->numVertices; v++) {
+    double disp = calc_distance(s, &s->pos[v * 2], &s->lastPos[v * 2]);
+    if (3 * disp > s->cutoff) { any = 1; break; }
+  }
+  if (any) {
+    orc_calc_neighbor_list(s, s->cutDistance);
+    orc_reset_last_positions(s);
+    s->updateCount += 1;
+  }
+  return any;
+}
+
+/* ---------------------------------------------------------- integrators --- */
+static void update_position(orc_state* s, double ts) {
+  for (long i = 0; i < 2 * s->numVertices; i++) s->pos[i] += ts * s->vel[i];
+}
+static void update_velocity(orc_state* s, double ts) {
+  for (long i = 0; i < 2 * s->numVertices; i++) s->vel[i] += ts * s->force[i];
+}
+
+/* Langevin::injectKineticEnergy, Simulator.cu:32-52: one Gaussian pair per ring,
+ * amplitude sqrt(T).  zeroMomentum (an addition) subtracts the mean vertex velocity. */
+void orc_inject_kinetic_energy(orc_state* s, int zeroMomentum) {
+  double amplitude = sqrt(s->Tinject);
+  uint64_t call = s->injectCalls++;
+  for (long v = 0; v < s->numVertices; v++) {
+    long p = s->particleIdList[v];
+    double z0, z1;
+    orc_gauss_pair(s->seed, s->systemId, TAG_INJECT, call, (uint32_t)p, &z0, &z1);
+    s->vel[v * 2] = amplitude * z0;
+    s->vel[v * 2 + 1] = amplitude * z1;
+  }
+  if (zeroMomentum) {
+    double m[2] = {0, 0};
+    for (long v = 0; v < s->numVertices; v++) { m[0] += s->vel[v * 2]; m[1] += s->vel[v * 2 + 1]; }
+    m[0] /= s->numVertices; m[1] /= s->numVertices;
+    for (long v = 0; v < s->numVertices; v++) { s->vel[v * 2] -= m[0]; s->vel[v * 2 + 1] -= m[1]; }
+  }
+}
+
+/* initLangevin, DPM2D.cu:3713-3753 (langevin noise type) */
+void orc_init_langevin(orc_state* s, double T, double gamma, int readState, int zeroMomentum) {
+  s->simKind = ORC_SIM_LANGEVIN;
+  s->Tinject = T;
+  s->gamma = gamma;
+  s->noise = sqrt(2. * T * gamma / s->dt);
+  orc_reset_last_positions(s);
+  if (!readState) orc_inject_kinetic_energy(s, zeroMomentum);
+}
+
+/* initNVE, DPM2D.cu:3760-3772 */
+void orc_init_nve(orc_state* s, double T, int readState, int zeroMomentum) {
+  s->simKind = ORC_SIM_NVE;
+  s->Tinject = T;
+  orc_reset_last_positions(s);
+  s->shift = 1;
+  if (!readState) orc_inject_kinetic_energy(s, zeroMomentum);
+}
+
+double orc_get_kinetic_energy(const orc_state* s) {
+  double k = 0;
+  for (long i = 0; i < 2 * s->numVertices; i++) k += s->vel[i] * s->vel[i];
+  return 0.5 * k;
+}
+
+double orc_get_temperature(const orc_state* s) {
+  return 2. * orc_get_kinetic_energy(s) / (s->numVertices * 2);
+}
+
+/* initNVERescale, DPM2D.cu:3779-3817: per-ring Gaussian, then rescale to T */
+void orc_init_nve_rescale(orc_state* s, double T) {
+  s->simKind = ORC_SIM_NVE_RESCALE;
+  s->Tinject = T;
+  orc_reset_last_positions(s);
+  s->shift = 1;
+  orc_inject_kinetic_energy(s, 0);
+  double scale = sqrt(T / orc_get_temperature(s));
+  for (long i = 0; i < 2 * s->numVertices; i++) s->vel[i] *= scale;
+}
+
+/* Langevin::updateThermalVel, Simulator.cu:54-77: F += noise*xi_ring - gamma*v */
+static void thermal_forces(orc_state* s) {
+  uint64_t call = s->thermalCalls++;
+  for (long v = 0; v < s->numVertices; v++) {
+    long p = s->particleIdList[v];
+    double z[2];
+    orc_gauss_pair(s->seed, s->systemId, TAG_THERMAL, call, (uint32_t)p, &z[0], &z[1]);
+    for (int dim = 0; dim < 2; dim++)
+      s->force[v * 2 + dim] += s->noise * z[dim] - s->gamma * s->vel[v * 2 + dim];
+  }
+}
+
+/* Langevin::integrate, Simulator.cu:23-30 */
+void orc_langevin_loop(orc_state* s, long steps) {
+  for (long n = 0; n < steps; n++) {
+    update_velocity(s, 0.5 * s->dt);
+    update_position(s, s->dt);
+    orc_check_neighbors(s);
+    orc_calc_force_energy(s);
+    thermal_forces(s);
+    update_velocity(s, 0.5 * s->dt);
+  }
+}
+
+/* NVE::integrate, Simulator.cu:206-212 */
+void orc_nve_loop(orc_state* s, long steps) {
+  for (long n = 0; n < steps; n++) {
+    update_velocity(s, 0.5 * s->dt);
+    update_position(s, s->dt);
+    orc_check_neighbors(s);
+    orc_calc_force_energy(s);
+    update_velocity(s, 0.5 * s->dt);
+  }
+}
+
+/* NVERescale::integrate + injectKineticEnergy, Simulator.cu:215-238 */
+void orc_nve_rescale_loop(orc_state* s, long steps) {
+  for (long n = 0; n < steps; n++) {
+    double scale = sqrt(s->Tinject / orc_get_temperature(s));
+    for (long i = 0; i < 2 * s->numVertices; i++) s->vel[i] *= scale;
+    update_velocity(s, 0.5 * s->dt);
+    update_position(s, s->dt);
+    orc_check_neighbors(s);
+    orc_calc_force_energy(s);
+    update_velocity(s, 0.5 * s->dt);
+  }
+}
+
+/* ----------------------------------------------------------------- FIRE --- */
+/* initFIRE + FIRE::initMinimizer, DPM2D.cu:3660-3675, FIRE.cu:29-41 */
+void orc_init_fire(orc_state* s, const double* p, long minStep, long numStep) {
+  s->a_start = p[0]; s->f_dec = p[1]; s->f_inc = p[2]; s->f_a = p[3];
+  s->fire_dt = p[4]; s->fire_dt_max = p[5]; s->a = p[6];
+  s->minStep = minStep; s->numStep = numStep; s->lastP = 0;
+}
+
+/* checkFIREParams, FIRE.cu:338-371: the back-step uses the already-decreased dt */
+static void check_fire_params(orc_state* s) {
+  double vDotF = 0;
+  for (long i = 0; i < 2 * s->numVertices; i++) vDotF += s->vel[i] * s->force[i];
+  s->lastP = vDotF;
+  if (vDotF > 0) {
+    s->numStep++;
+    if (s->numStep > s->minStep) {
+      s->fire_dt = fmin(s->fire_dt * s->f_inc, s->fire_dt_max);
+      s->a *= s->f_a;
+    }
+  } else {
+    s->numStep = 0;
+    s->fire_dt = fmax(s->fire_dt * s->f_dec, s->fire_dt_max / 10);
+    s->a = s->a_start;
+    double h = 0.5 * s->fire_dt;
+    for (long i = 0; i < 2 * s->numVertices; i++) s->pos[i] -= h * s->vel[i];
+    memset(s->vel, 0, sizeof(double) * 2 * s->numVertices);
+  }
+}
+
+/* bendVertexVelocityTowardsForce, FIRE.cu:374-394 */
+static void bend_velocity(orc_state* s) {
+  double vv = 0, ff = 0;
+  for (long i = 0; i < 2 * s->numVertices; i++) vv += s->vel[i] * s->vel[i];
+  for (long i = 0; i < 2 * s->numVertices; i++) ff += s->force[i] * s->force[i];
+  if (ff > 0) {
+    double ratio = sqrt(vv / ff);
+    double fa = s->a;
+    for (long i = 0; i < 2 * s->numVertices; i++)
+      s->vel[i] = (1 - fa) * s->vel[i] + fa * s->force[i] * ratio;
+  }
+}
+
+/* FIRE::minimizerVertexLoop, FIRE.cu:397-411 */
+void orc_fire_loop(orc_state* s, long iterations) {
+  for (long it = 0; it < iterations; it++) {
+    check_fire_params(s);
+    update_velocity(s, 0.5 * s->fire_dt);
+    bend_velocity(s);
+    update_position(s, s->fire_dt);
+    orc_check_neighbors(s);
+    orc_calc_force_energy(s);
+    update_velocity(s, 0.5 * s->fire_dt);
+  }
+}
+
+void orc_get_fire_state(const orc_state* s, double* out) {
+  out[0] = s->fire_dt; out[1] = s->a; out[2] = (double)s->numStep; out[3] = s->lastP;
+  out[4] = (double)s->updateCount;
+}
+
+/* driver loop of compressFIRE.cpp:121-131: iterate while max|F_c| > tol */
+long orc_fire_minimize(orc_state* s, double tol, long maxIter) {
+  long it = 0;
+  double fc = orc_get_max_unbalanced_force(s);
+  while (fc > tol && it != maxIter) {
+    orc_fire_loop(s, 1);
+    fc = orc_get_max_unbalanced_force(s);
+    it++;
+  }
+  return it;
+}
+
+/* ---------------------------------------------------------- observables --- */
+/* getPotentialEnergy, DPM2D.cu:1062-1080 ('all' returns the real sum here, see A9) */
+double orc_get_potential_energy(orc_state* s) {
+  double e = 0;
+  for (long v = 0; v < s->numVertices; v++) e += s->energy[v];
+  if (s->interactionType == ORC_VERTEX_SMOOTH || s->interactionType == ORC_CELL_SMOOTH)
+    for (long p = 0; p < s->numParticles; p++) e += s->particleEnergy[p];
+  return e;
+}
+
+/* getMaxUnbalancedForce, DPM2D.cu:997-1005: sqrt of the largest squared component */
+double orc_get_max_unbalanced_force(const orc_state* s) {
+  double m = -1;
+  for (long i = 0; i < 2 * s->numVertices; i++) {
+    double f2 = s->force[i] * s->force[i];
+    if (f2 > m) m = f2;
+  }
+  return sqrt(m);
+}
+
+/* getTotalForceMagnitude, DPM2D.cu:986-995 */
+double orc_get_total_force_magnitude(const orc_state* s) {
+  double t = 0;
+  for (long i = 0; i < 2 * s->numVertices; i++) t += s->force[i] * s->force[i];
+  return sqrt(t / (s->numVertices * 2));
+}
+
+/* vertex-disk correction of kernelCalcVertexArea, cudaKernel.cuh:1636-1642 */
+static double vertex_area_sum(const orc_state* s) {
+  double t = 0;
+  for (long p = 0; p < s->numParticles; p++) {
+    double r = s->rad[s->firstVertexInParticleId[p]];
+    t += r * r * (0.5 * s->numVertexInParticleList[p] - 1);
+  }
+  return t;
+}
+
+/* getPhi, DPM2D.cu:1202-1220 */
+double orc_get_phi(orc_state* s) {
+  double phi = 0;
+  for (long p = 0; p < s->numParticles; p++) phi += s->area[p];
+  phi += PI * vertex_area_sum(s);
+  return phi / (s->boxSize[0] * s->boxSize[1]);
+}
+
+/* getPreferredPhi, DPM2D.cu:1222-1238 */
+double orc_get_preferred_phi(const orc_state* s) {
+  double phi = 0;
+  for (long p = 0; p < s->numParticles; p++) phi += s->a0[p];
+  phi += PI * vertex_area_sum(s);
+  return phi / (s->boxSize[0] * s->boxSize[1]);
+}
+
+/* getMeanParticleSize, DPM2D.cu:758-760 */
+double orc_get_mean_particle_size(const orc_state* s) {
+  double t = 0;
+  for (long p = 0; p < s->numParticles; p++) t += s->a0[p];
+  return 2 * sqrt(t / (PI * s->numParticles));
+}
+
+/* getMeanVertexRadius, DPM2D.cu:770-772 */
+double orc_get_mean_vertex_radius(const orc_state* s) {
+  double t = 0;
+  for (long v = 0; v < s->numVertices; v++) t += s->rad[v];
+  return t / s->numVertices;
+}
+
+/* adjustTemperature, DPM2D.cu:1185-1200 */
+void orc_adjust_temperature(orc_state* s, double targetTemp) {
+  double scale = sqrt(targetTemp / orc_get_temperature(s));
+  for (long i = 0; i < 2 * s->numVertices; i++) s->vel[i] *= scale;
+}
+
+/* scaleVertices, DPM2D.cu:1802-1811 with kernelScaleVertexPositions, cudaKernel.cuh:1644-1658:
+ * positions are scaled about the stored particlePos (not recomputed, as in the reference). */
+void orc_scale_vertices(orc_state* s, double scale) {
+  for (long p = 0; p < s->numParticles; p++) {
+    long first = s->firstVertexInParticleId[p];
+    long last = first + s->numVertexInParticleList[p];
+    for (long v = first; v < last; v++)
+      for (int dim = 0; dim < 2; dim++) {
+        double distance = s->pos[v * 2 + dim] - s->particlePos[p * 2 + dim];
+        s->pos[v * 2 + dim] += (scale - 1) * distance;
+      }
+  }
+  for (long p = 0; p < s->numParticles; p++) { s->a0[p] *= scale * scale; s->area[p] *= scale * scale; }
+  for (long v = 0; v < s->numVertices; v++) { s->l0[v] *= scale; s->rad[v] *= scale; }
+}
+
+/* getParticleShapes, DPM2D.cu:739-756 */
+void orc_get_particle_shapes(orc_state* s, double* shape) {
+  for (long p = 0; p < s->numParticles; p++) {
+    shape[p * 3] = s->area[p];
+    shape[p * 3 + 1] = s->perimeter[p];
+    shape[p * 3 + 2] = s->perimeter[p] * s->perimeter[p] / (4 * PI * s->area[p]);
+  }
+}

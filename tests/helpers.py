@@ -1,0 +1,37 @@
+"""Shared builders for the oracle and the GPU set from one synthetic state."""
+import numpy as np
+
+from cudaring_amd import packing
+
+CUTOFF = 0.5  # runNVT.cpp:37 displacement cutoff
+
+
+def oracle_from_state(oracle_mod, st, potential="harmonic", interaction="vertexVertex", costs=(1, 1, 0.01, 1),
+                      l1=0.1, l2=0.2, lj=1.5, dt=1e-3, seed=7, system_id=0, particle="passive"):
+    o = oracle_mod.Oracle(st["nvp"])
+    o.load_state(st)
+    o.set_types(potential, interaction, "harmonic", particle)
+    o.set_energy_costs(*costs)
+    o.set_attraction(l1, l2)
+    o.set_lj_cutoff(lj)
+    o.set_dt(dt)
+    o.set_rng(seed, system_id)
+    return o
+
+
+def gpu_from_state(st, potential="harmonic", interaction="vertexVertex", costs=(1, 1, 0.01, 1), l1=0.1, l2=0.2,
+                   lj=1.5, dt=1e-3, seed=7, particle="passive", rings_per_system=None):
+    from cudaring_amd import DPMSet
+    g = DPMSet(st["nvp"], rings_per_system=rings_per_system)
+    g.load_state(st)
+    g.set_types(potential, interaction, "harmonic", particle)
+    g.set_energy_costs(*costs)
+    g.set_attraction(l1, l2)
+    g.set_lj_cutoff(lj)
+    g.set_dt(dt)
+    g.set_seed(seed)
+    return g
+
+
+def small_state(nrings=12, phi=0.7, seed=3):
+    return packing.random_system(nrings, phi, seed)

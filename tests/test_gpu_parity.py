@@ -1,0 +1,179 @@
+"""GPU parity: libdpmring.so (through the C ABI) against the CPU oracle on identical
+seeded states.  Bars: forces and energies within 1e-12 relative (max |diff| over
+the largest term), neighbour lists identical as sets, short NVE / Langevin / FIRE
+windows within 1e-10 before chaotic divergence."""
+import numpy as np
+import pytest
+
+from cudaring_amd import packing
+from tests.conftest import rel_err
+from tests.helpers import gpu_from_state, oracle_from_state
+
+pytestmark = pytest.mark.gpu
+
+TOL_F = 1e-12
+POTS = ["harmonic", "wca", "lj", "adhesive"]
+INTERS = ["vertexVertex", "vertexSmooth"]
+
+
+def prepare(oracle_mod, st, pot, inter, skin=0.5, **kw):
+    o = oracle_from_state(oracle_mod, st, potential=pot, interaction=inter, **kw)
+    g = gpu_from_state(st, potential=pot, interaction=inter, **kw)
+    size = 2 * st["rad"].mean()
+    c1 = o.set_displacement_cutoff(skin, size)
+    c2 = g.set_displacement_cutoff(skin, size)
+    assert c1 == c2
+    o.calc_neighbor_list(c1)
+    g.calc_neighbor_list(c2)
+    return o, g
+
+
+@pytest.mark.parametrize("inter", INTERS)
+@pytest.mark.parametrize("pot", POTS)
+def test_force_energy_parity(oracle_mod, pot, inter):
+    print(f"case force_energy pot={pot} inter={inter}", flush=True)
+    st = packing.random_system(12, 0.8, 101)
+    o, g = prepare(oracle_mod, st, pot, inter)
+    o.calc_force_energy()
+    g.calc_force_energy()
+    assert rel_err(g.get("force"), o.get("force")) < TOL_F
+    assert rel_err(g.get("energy"), o.get("energy")) < TOL_F
+    if inter == "vertexSmooth":
+        assert rel_err(g.get("penergy"), o.get("penergy")) < TOL_F
+    assert abs(g.potential_energy()[0] - o.potential_energy()) <= TOL_F * abs(o.potential_energy())
+    for f in ("area", "perimeter", "theta", "length", "ppos"):
+        assert rel_err(g.get(f), o.get(f)) < 1e-13, f
+
+
+@pytest.mark.parametrize("nrings,phi", [(3, 0.5), (12, 0.8), (80, 0.85)])
+def test_neighbor_sets(oracle_mod, nrings, phi):
+    print(f"case neighbor_sets nrings={nrings}", flush=True)
+    st = packing.random_system(nrings, phi, 7 + nrings)
+    o, g = prepare(oracle_mod, st, "wca", "vertexSmooth")
+    ro, _, _ = o.neighbors()
+    rg, _, _ = g.neighbors()
+    assert [sorted(r) for r in rg] == [sorted(r) for r in ro]
+    assert rg == ro  # rows are ascending on both sides
+
+
+def test_ring_size_edges(oracle_mod):
+    print("case ring_size_edges", flush=True)
+    rng = np.random.default_rng(5)
+    nvp = np.array([3, 64, 5, 33, 64, 3, 17, 40])
+    a0, l0, th0, rad = packing.ring_shape_params(nvp, 1.1, n_min=16)
+    L = np.sqrt(packing.preferred_phi_area(nvp, a0, rad) / 0.6)
+    pos = packing.place_rings(nvp, a0, rng.random((len(nvp), 2)) * L, rng=rng, jitter=1e-3, l0=l0)
+    st = {"nvp": nvp, "box": np.array([L, L]), "pos": pos.ravel(), "rad": rad, "l0": l0, "theta0": th0, "a0": a0}
+    for inter in INTERS:
+        o, g = prepare(oracle_mod, st, "harmonic", inter)
+        o.calc_force_energy()
+        g.calc_force_energy()
+        assert rel_err(g.get("force"), o.get("force")) < TOL_F
+        assert rel_err(g.get("energy"), o.get("energy")) < TOL_F
+
+
+@pytest.mark.parametrize("pot,inter", [("harmonic", "vertexVertex"), ("wca", "vertexSmooth")])
+def test_nve_window(oracle_mod, pot, inter):
+    print(f"case nve_window {pot} {inter}", flush=True)
+    st = packing.lattice_system(16, 0.8, 17)
+    o, g = prepare(oracle_mod, st, pot, inter, dt=0.005)
+    o.calc_force_energy(); g.calc_force_energy()
+    o.init_nve(1e-2, zero_momentum=True); g.init_nve(1e-2, zero_momentum=True)
+    assert rel_err(g.get("vel"), o.get("vel")) < 1e-15
+    for chunk in (1, 7, 42):
+        o.nve(chunk); g.nve(chunk)
+        assert rel_err(g.get("pos"), o.get("pos"), scale=1.0) < 1e-11
+        assert rel_err(g.get("vel"), o.get("vel")) < 1e-9
+        assert rel_err(g.get("force"), o.get("force")) < 1e-9
+    assert list(g.update_count()) == [o.update_count()]
+
+
+@pytest.mark.parametrize("inter", INTERS)
+def test_langevin_window(oracle_mod, inter):
+    print(f"case langevin_window {inter}", flush=True)
+    st = packing.lattice_system(16, 0.85, 23)
+    o, g = prepare(oracle_mod, st, "wca", inter, dt=0.002, seed=99)
+    o.calc_force_energy(); g.calc_force_energy()
+    o.init_langevin(1e-2, 1.0); g.init_langevin(1e-2, 1.0)
+    for chunk in (1, 5, 30):
+        o.langevin(chunk); g.langevin(chunk)
+        assert rel_err(g.get("pos"), o.get("pos"), scale=1.0) < 1e-11
+        assert rel_err(g.get("vel"), o.get("vel")) < 1e-9
+    assert abs(g.temperature()[0] - o.temperature()) < 1e-9 * o.temperature()
+
+
+def test_active_force(oracle_mod):
+    print("case active_force", flush=True)
+    st = packing.lattice_system(16, 0.8, 31)
+    st["pangle"] = np.random.default_rng(1).uniform(-np.pi, np.pi, 16)
+    o, g = prepare(oracle_mod, st, "wca", "vertexSmooth", particle="active", dt=0.01)
+    o.set_self_propulsion(0.1, 100.0); g.set_self_propulsion(0.1, 100.0)
+    for _ in range(3):
+        o.calc_force_energy(); g.calc_force_energy()
+        assert rel_err(g.get("pangle"), o.get("pangle"), scale=np.pi) < 1e-14
+        assert rel_err(g.get("force"), o.get("force")) < TOL_F
+    o.init_langevin(1e-3, 1.0); g.init_langevin(1e-3, 1.0)
+    o.langevin(20); g.langevin(20)
+    assert rel_err(g.get("pos"), o.get("pos"), scale=1.0) < 1e-11
+
+
+def test_fire_sequence(oracle_mod):
+    print("case fire_sequence", flush=True)
+    st = packing.lattice_system(16, 0.86, 41, perturb=0.3)
+    o, g = prepare(oracle_mod, st, "harmonic", "vertexVertex", costs=(1, 1, 0.1, 1))
+    o.calc_force_energy(); g.calc_force_energy()
+    dt = 0.01 * o.mean_particle_size()
+    prm = [0.2, 0.5, 1.1, 0.99, dt, 10 * dt, 0.2]
+    o.init_fire(prm, 20, 0); g.init_fire(prm, 20, 0)
+    for chunk in (1, 10, 60):
+        o.fire(chunk); g.fire(chunk)
+        fo, fg = o.fire_state(), g.fire_state()
+        assert fg["numStep"][0] == fo["numStep"]
+        assert fg["dt"][0] == pytest.approx(fo["dt"], rel=1e-14)
+        assert fg["alpha"][0] == pytest.approx(fo["alpha"], rel=1e-14)
+        assert rel_err(g.get("pos"), o.get("pos"), scale=1.0) < 1e-11
+    assert g.max_unbalanced_force()[0] == pytest.approx(o.max_unbalanced_force(), rel=1e-8)
+
+
+def test_batched_set_matches_each_system(oracle_mod):
+    print("case batched_set", flush=True)
+    systems = [packing.random_system(n, 0.75, 200 + n) for n in (5, 12, 9)]
+    cat = packing.concat_systems(systems)
+    g = gpu_from_state(cat, potential="harmonic", interaction="vertexSmooth", seed=5,
+                       rings_per_system=cat["rings_per_system"])
+    size = 2 * np.mean(cat["rad"])
+    cut = g.set_displacement_cutoff(0.4, size)
+    g.calc_neighbor_list(cut)
+    g.calc_force_energy()
+    g.init_langevin(1e-2, 1.0)
+    g.langevin(10)
+    F, X = g.get("force").reshape(-1, 2), g.get("pos").reshape(-1, 2)
+    U = g.potential_energy()
+    off = g.vert_off
+    for s, st in enumerate(systems):
+        o = oracle_from_state(oracle_mod, st, potential="harmonic", interaction="vertexSmooth", seed=5, system_id=s)
+        o.set_displacement_cutoff(0.4, size)
+        o.calc_neighbor_list(cut)
+        o.calc_force_energy()
+        o.init_langevin(1e-2, 1.0)
+        o.langevin(10)
+        assert rel_err(X[off[s]:off[s + 1]].ravel(), o.get("pos"), scale=1.0) < 1e-11
+        assert rel_err(F[off[s]:off[s + 1]].ravel(), o.get("force")) < 1e-9
+        assert U[s] == pytest.approx(o.potential_energy(), rel=1e-10)
+
+
+def test_observables(oracle_mod):
+    print("case observables", flush=True)
+    st = packing.random_system(20, 0.8, 77)
+    o, g = prepare(oracle_mod, st, "harmonic", "vertexVertex")
+    o.calc_force_energy(); g.calc_force_energy()
+    o.init_nve(1e-2); g.init_nve(1e-2)
+    assert g.kinetic_energy()[0] == pytest.approx(o.kinetic_energy(), rel=1e-13)
+    assert g.temperature()[0] == pytest.approx(o.temperature(), rel=1e-13)
+    assert g.max_unbalanced_force()[0] == pytest.approx(o.max_unbalanced_force(), rel=1e-12)
+    assert g.phi()[0] == pytest.approx(o.phi(), rel=1e-13)
+    assert g.preferred_phi()[0] == pytest.approx(o.preferred_phi(), rel=1e-13)
+    np.testing.assert_allclose(g.shapes(), o.shapes(), rtol=1e-12)
+    g.scale_vertices(1.01); o.scale_vertices(1.01)
+    assert rel_err(g.get("pos"), o.get("pos"), scale=1.0) < 1e-13
+    assert rel_err(g.get("a0"), o.get("a0")) < 1e-15
