@@ -1,5 +1,6 @@
 // dpm_api.hip -- the C ABI (include/dpmring.h) over the gfx950 kernels.
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -91,6 +92,18 @@ bool field_ptrs(dpm_set_s* S, int f, double** c0, double** c1, long* n, int* nco
   return false;
 }
 
+int upload_box(dpm_set_s* S) {
+  std::vector<double> ix(S->nsys), iy(S->nsys);
+  for (long s = 0; s < S->nsys; s++) {
+    ix[s] = 1.0 / S->h_box[2 * s];
+    iy[s] = 1.0 / S->h_box[2 * s + 1];
+  }
+  HIPCHK(up(S->V.ibx, ix, S->stream));
+  HIPCHK(up(S->V.iby, iy, S->stream));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
 // (Re)configure the per-system cell grids for the current cutDistance, boxes and radii.
 int configure_cells(dpm_set_s* S) {
   const long B = S->nsys;
@@ -122,10 +135,11 @@ int configure_cells(dpm_set_s* S) {
   std::vector<int> cellSys(C);
   for (long s = 0; s < B; s++)
     for (long c = 0; c < (long)ncx[s] * ncy[s]; c++) cellSys[coff[s] + c] = (int)s;
-  // free and reallocate
-  hipFree(S->G.count); hipFree(S->G.slots); hipFree(S->G.cellSys);
-  S->G.count = S->G.slots = S->G.cellSys = nullptr;
-  HIPCHK(dalloc(&S->G.count, C));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  hipFree(S->G.count[0]); hipFree(S->G.count[1]); hipFree(S->G.slots); hipFree(S->G.cellSys);
+  S->G.count[0] = S->G.count[1] = S->G.slots = S->G.cellSys = nullptr;
+  HIPCHK(dalloc(&S->G.count[0], C));
+  HIPCHK(dalloc(&S->G.count[1], C));
   HIPCHK(dalloc(&S->G.slots, C * cap));
   HIPCHK(dalloc(&S->G.cellSys, C));
   HIPCHK(up(S->G.cellSys, cellSys, S->stream));
@@ -133,7 +147,9 @@ int configure_cells(dpm_set_s* S) {
   HIPCHK(up(S->G.ncy, ncy, S->stream));
   HIPCHK(up(S->G.coff, coff, S->stream));
   HIPCHK(up(S->G.brute, brute, S->stream));
-  HIPCHK(launch_fill_int(S->G.count, 0, C, S->stream));
+  HIPCHK(launch_fill_int(S->G.count[0], 0, C, S->stream));
+  HIPCHK(launch_fill_int(S->G.count[1], 0, C, S->stream));
+  HIPCHK(hipStreamSynchronize(S->stream));
   S->G.ncells = (int)C;
   S->G.cap = cap;
   bump(S);
@@ -141,47 +157,63 @@ int configure_cells(dpm_set_s* S) {
 }
 
 int alloc_list(dpm_set_s* S, int W) {
+  HIPCHK(hipStreamSynchronize(S->stream));
   hipFree(S->V.nbr);
+  hipFree(S->V.nbrp);
   S->V.nbr = nullptr;
+  S->V.nbrp = nullptr;
   HIPCHK(dalloc(&S->V.nbr, (long)W * S->nv));
+  HIPCHK(dalloc(&S->V.nbrp, (long)W * S->nv));
   HIPCHK(launch_fill_int(S->V.nbr, -1, (long)W * S->nv, S->stream));
+  HIPCHK(launch_fill_int((int*)S->V.nbrp, 0, 2L * W * S->nv, S->stream));
+  HIPCHK(launch_fill_int(S->V.ncnt, 0, S->nv, S->stream));
   S->V.W = W;
+  S->listBuilt = false;
   bump(S);
   return DPM_OK;
 }
 
-// Host-driven full rebuild of every system's list (calcNeighborList semantics).
-int rebuild_all(dpm_set_s* S) {
+// Rebuild the rows of the systems flagged in `flags` (host array) from the current positions.
+// calcNeighborList (countUpdates = 0) and checkNeighbors (countUpdates = 1) semantics.  A
+// wider row needs new storage: then every system is rebuilt into it before returning.
+int rebuild(dpm_set_s* S, const std::vector<int>& flags, int countUpdates) {
+  const int b = S->cur;
   for (int attempt = 0; attempt < 8; attempt++) {
-    HIPCHK(launch_fill_int(S->d_flagNext, 1, S->nsys, S->stream));
+    int any = 0;
+    for (int f : flags) any |= f;
+    HIPCHK(hipMemcpyAsync(S->d_flag[b], flags.data(), sizeof(int) * S->nsys, hipMemcpyHostToDevice, S->stream));
+    HIPCHK(hipMemcpyAsync(S->d_any + b, &any, sizeof(int), hipMemcpyHostToDevice, S->stream));
     HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
     HIPCHK(launch_fill_int(S->d_maxcnt, 0, 1, S->stream));
-    HIPCHK(launch_prologue(S, 0, 0));
-    HIPCHK(launch_bin(S));
-    HIPCHK(launch_build(S));
+    HIPCHK(launch_fill_int(S->G.count[b], 0, S->G.ncells, S->stream));
+    HIPCHK(launch_bin(S, 0, 0));
+    HIPCHK(launch_build_rows(S, countUpdates));
     int err = 0, maxcnt = 0;
     HIPCHK(hipMemcpyAsync(&err, S->d_err, sizeof(int), hipMemcpyDeviceToHost, S->stream));
     HIPCHK(hipMemcpyAsync(&maxcnt, S->d_maxcnt, sizeof(int), hipMemcpyDeviceToHost, S->stream));
     HIPCHK(hipStreamSynchronize(S->stream));
+    // leave both parities' flags and the cells clean
+    HIPCHK(launch_fill_int(S->d_flag[0], 0, S->nsys, S->stream));
+    HIPCHK(launch_fill_int(S->d_flag[1], 0, S->nsys, S->stream));
+    HIPCHK(launch_fill_int(S->d_any, 0, 2, S->stream));
+    HIPCHK(launch_fill_int(S->G.count[b], 0, S->G.ncells, S->stream));
+    HIPCHK(hipStreamSynchronize(S->stream));
     if (err & ERR_CELL_OVERFLOW) {
       S->G.cap *= 2;
-      long C = S->G.ncells;
       hipFree(S->G.slots);
       S->G.slots = nullptr;
-      HIPCHK(dalloc(&S->G.slots, C * S->G.cap));
+      HIPCHK(dalloc(&S->G.slots, (long)S->G.ncells * S->G.cap));
       bump(S);
       continue;
     }
     // keep head room so device-side rebuilds during a loop do not overflow
     const int want = (int)pow2ceil(std::max(8, 2 * maxcnt));
-    if ((err & ERR_LIST_OVERFLOW) || want > S->V.W) {
-      int rc = alloc_list(S, want);
+    if (want > S->V.W || (err & ERR_LIST_OVERFLOW)) {
+      int rc = alloc_list(S, std::max(want, S->V.W));
       if (rc) return rc;
-      if (err & ERR_LIST_OVERFLOW) continue;
+      std::vector<int> all(S->nsys, 1);
+      return rebuild(S, all, 0);
     }
-    // the rebuild counted as an update; the reference's calcNeighborList does not
-    std::vector<int> zero(S->nsys, 0);
-    (void)zero;
     S->listBuilt = true;
     return DPM_OK;
   }
@@ -212,81 +244,134 @@ int require_list(dpm_set_s* S) {
   return DPM_OK;
 }
 
-// Launch sequence of a velocity-Verlet family loop (NVE / Langevin): see dpm_langevin_loop.
+// One loop step's launches at position parity cur: bin (normally idle) + step.  The FIRE step
+// also needs the per-system FIRE state and the tile partial buffer.
+int loop_step(dpm_set_s* S, int mode, int bumpT) {
+  HIPCHK(launch_bin(S, bumpT, S->P.particle == PART_ACTIVE ? 1 : 0));
+  if (mode == MODE_FIRE) {
+    HIPCHK(launch_fire_step(S));
+    return DPM_OK;
+  }
+  StepArgs A = loop_args(S);
+  HIPCHK(launch_step(S, mode, A));
+  return DPM_OK;
+}
+
+// Launch sequence of a velocity-Verlet loop (NVE / Langevin):
+//   start: v += dt/2 F, x' = x + dt v, displacement test          (buffer b -> 1-b)
+//   per step: bin, step [rebuild rows if flagged; F(x'); thermostat; v += dt/2 F;
+//             unless last: v += dt/2 F, x'' = x' + dt v, displacement test]
+// The fused form performs the reference's operations in the reference's order.
 int verlet_sequence(dpm_set_s* S, long steps, bool langevin) {
   const double h = 0.5 * S->P.dt;
-  const int bumpA = S->P.particle == PART_ACTIVE ? 1 : 0;
+  HIPCHK(launch_fill_int(S->d_flag[1 - S->cur], 0, S->nsys, S->stream));
+  HIPCHK(launch_fill_int(S->d_any + 1 - S->cur, 0, 1, S->stream));
   HIPCHK(launch_start(S, h, S->P.dt));
   swap_pos(S);
   for (long i = 0; i < steps; i++) {
-    HIPCHK(launch_prologue(S, langevin ? 1 : 0, bumpA));
-    HIPCHK(launch_bin(S));
-    HIPCHK(launch_build(S));
-    StepArgs A{};
-    A.h = h;
-    A.dt = S->P.dt;
-    A.xout = S->pos[1 - S->cur][0];
-    A.yout = S->pos[1 - S->cur][1];
-    A.flagNext = S->d_flagNext;
-    A.ctr = S->d_ctr;
-    A.err = S->d_err;
     const bool more = i + 1 < steps;
-    int mode = MODE_KICK | (langevin ? MODE_LANGEVIN : 0) | (more ? MODE_START : 0);
-    HIPCHK(launch_step(S, mode, A));
+    const int mode = MODE_KICK | (langevin ? MODE_LANGEVIN : 0) | (more ? MODE_START : 0);
+    int rc = loop_step(S, mode, langevin ? 1 : 0);
+    if (rc) return rc;
     if (more) swap_pos(S);
   }
   return DPM_OK;
 }
 
-// Run `steps` steps, replaying a captured HIP graph of the whole launch sequence when the
-// same (kind, steps, buffer parity) has been seen with unchanged launch arguments.
-int run_verlet(dpm_set_s* S, long steps, bool langevin) {
-  if (steps <= 0) return DPM_OK;
-  int rc = require_list(S);
-  if (rc) return rc;
-  HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
-  if (S->graphVersion != S->version) {
-    for (auto& kv : S->graphs) hipGraphExecDestroy(kv.second);
-    S->graphs.clear();
-    S->graphVersion = S->version;
-  }
-  const bool useGraph = steps >= 4 && getenv("DPM_NO_GRAPH") == nullptr;
-  if (!useGraph) return verlet_sequence(S, steps, langevin);
-  auto key = std::make_tuple(langevin ? 1 : 2, steps, S->cur);
+void drop_graphs(dpm_set_s* S) {
+  for (auto& kv : S->graphs) hipGraphExecDestroy(kv.second);
+  S->graphs.clear();
+  S->graphVersion = S->version;
+}
+
+// Capture `seq` into a graph cached under `key`, or replay the cached one.  `seq` must leave
+// the host-side parity state exactly as a direct run would; the caller advances it.
+template <class F>
+int run_graph(dpm_set_s* S, std::tuple<int, long, int> key, F seq) {
+  if (S->graphVersion != S->version) drop_graphs(S);
   auto it = S->graphs.find(key);
   if (it == S->graphs.end()) {
-    const int cur0 = S->cur;
+    const int cur0 = S->cur, f0 = S->fireCur;
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
-    int r2 = verlet_sequence(S, steps, langevin);
+    int r2 = seq();
     hipError_t e = hipStreamEndCapture(S->stream, &g);
+    S->cur = cur0;
+    S->fireCur = f0;
+    set_pos_view(S);
     if (r2) return r2;
     HIPCHK(e);
     hipGraphExec_t ge;
     HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
     hipGraphDestroy(g);
     S->graphs[key] = ge;
-    // the capture advanced the host-side buffer parity; restore it and replay
-    S->cur = cur0;
-    set_pos_view(S);
     it = S->graphs.find(key);
   }
   HIPCHK(hipGraphLaunch(it->second, S->stream));
-  // parity after the sequence: start swaps once, each of steps-1 middle steps swaps once
-  if ((steps % 2) == 1) swap_pos(S);
+  return DPM_OK;
+}
+
+bool graphs_enabled() { return getenv("DPM_NO_GRAPH") == nullptr; }
+
+// Run `steps` Verlet steps (a captured graph per (kind, steps, parity) when steps >= 4).
+int run_verlet(dpm_set_s* S, long steps, bool langevin) {
+  if (steps <= 0) return DPM_OK;
+  int rc = require_list(S);
+  if (rc) return rc;
+  HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
+  if (steps < 4 || !graphs_enabled()) return verlet_sequence(S, steps, langevin);
+  rc = run_graph(S, std::make_tuple(langevin ? 1 : 2, steps, S->cur),
+                 [&]() { return verlet_sequence(S, steps, langevin); });
+  if (rc) return rc;
+  if ((steps % 2) == 1) swap_pos(S);  // the sequence swaps the buffers once per step
   return DPM_OK;
 }
 
 int fire_iteration(dpm_set_s* S) {
   HIPCHK(launch_fire_begin(S));
-  S->fireCur = 1 - S->fireCur;
-  // fire_mid reads the state written by fire_begin (now fireCur)
+  S->fireCur = 1 - S->fireCur;  // fire_mid and the step read the state fire_begin wrote
   HIPCHK(launch_fire_mid(S));
   swap_pos(S);
-  HIPCHK(launch_prologue(S, 0, S->P.particle == PART_ACTIVE ? 1 : 0));
-  HIPCHK(launch_bin(S));
-  HIPCHK(launch_build(S));
-  HIPCHK(launch_fire_step(S));
+  return loop_step(S, MODE_FIRE, 0);
+}
+
+int fire_sequence(dpm_set_s* S, long iterations) {
+  HIPCHK(launch_fill_int(S->d_flag[1 - S->cur], 0, S->nsys, S->stream));
+  HIPCHK(launch_fill_int(S->d_any + 1 - S->cur, 0, 1, S->stream));
+  HIPCHK(launch_fire_partials(S));
+  for (long i = 0; i < iterations; i++) {
+    int rc = fire_iteration(S);
+    if (rc) return rc;
+  }
+  return DPM_OK;
+}
+
+// FIRE iterations; the sequence flips the FIRE state and position buffers once per iteration.
+int run_fire(dpm_set_s* S, long iterations) {
+  if (iterations <= 0) return DPM_OK;
+  if (iterations < 4 || !graphs_enabled()) return fire_sequence(S, iterations);
+  int rc = run_graph(S, std::make_tuple(3 + 2 * S->fireCur, iterations, S->cur),
+                     [&]() { return fire_sequence(S, iterations); });
+  if (rc) return rc;
+  if ((iterations % 2) == 1) {
+    swap_pos(S);
+    S->fireCur = 1 - S->fireCur;
+  }
+  return DPM_OK;
+}
+
+int temperature_scale(dpm_set_s* S, double T) {
+  int rc = reduce_host(S);
+  if (rc) return rc;
+  std::vector<double> sc(S->nsys);
+  for (long s = 0; s < S->nsys; s++) {
+    const double n = S->h_svoff[s + 1] - S->h_svoff[s];
+    const double temp = 2. * (0.5 * S->h_red[s * 8 + 2]) / (n * 2);
+    sc[s] = sqrt(T / temp);
+  }
+  HIPCHK(hipMemcpyAsync(S->d_scale, sc.data(), sizeof(double) * S->nsys, hipMemcpyHostToDevice, S->stream));
+  HIPCHK(launch_scale_vel(S));
+  HIPCHK(hipStreamSynchronize(S->stream));
   return DPM_OK;
 }
 
@@ -299,12 +384,9 @@ int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPer
                const long* numVertexInParticleList) {
   if (!out || numSystems <= 0 || !ringsPerSystem || !numVertexInParticleList) return DPM_ERR_ARG;
   *out = nullptr;
+  if (hipSetDevice(device) != hipSuccess) return DPM_ERR_HIP;
   dpm_set_s* S = new dpm_set_s();
   S->device = device;
-  if (hipSetDevice(device) != hipSuccess) {
-    delete S;
-    return DPM_ERR_HIP;
-  }
   S->nsys = numSystems;
   long np = 0, nv = 0;
   S->h_svoff.push_back(0);
@@ -320,7 +402,6 @@ int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPer
       for (long q = 0; q < n; q++) {
         S->h_vring.push_back((int)np);
         S->h_vsys.push_back((int)s);
-        S->h_vprev.push_back((int)(q == 0 ? nv + n - 1 : nv + q - 1));
       }
       nv += n;
       np++;
@@ -331,7 +412,7 @@ int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPer
   if (nv > (1L << 30)) { delete S; return DPM_ERR_UNSUPPORTED; }
   S->np = np;
   S->nv = nv;
-  // ring-aligned wave tiles, greedy, never crossing a system
+  // ring-aligned tiles of at most 64 vertices, greedy, never crossing a system
   S->h_stile.push_back(0);
   for (long s = 0; s < numSystems; s++) {
     long r = S->h_sroff[s];
@@ -348,56 +429,62 @@ int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPer
     }
     S->h_stile.push_back((int)S->h_tile_v0.size());
   }
-  S->V.nv = (int)nv;
-  S->V.np = (int)np;
-  S->V.nsys = (int)numSystems;
-  S->V.ntiles = (int)S->h_tile_v0.size();
+  View& V = S->V;
+  V.nv = (int)nv;
+  V.np = (int)np;
+  V.nsys = (int)numSystems;
+  V.ntiles = (int)S->h_tile_v0.size();
   S->h_box.assign(2 * numSystems, 1.0);
   S->h_maxrad.assign(numSystems, 0.0);
+  if (hipStreamCreateWithFlags(&S->ownStream, hipStreamNonBlocking) != hipSuccess) {
+    delete S;
+    return DPM_ERR_HIP;
+  }
+  S->stream = S->ownStream;
 #define A(p, n) if (dalloc(&(p), (n)) != hipSuccess) { dpm_destroy(S); return DPM_ERR_OOM; }
   for (int b = 0; b < 2; b++) for (int c = 0; c < 2; c++) A(S->pos[b][c], nv);
-  View& V = S->V;
   A(V.vx, nv); A(V.vy, nv); A(V.fx, nv); A(V.fy, nv); A(V.rad, nv); A(V.l0, nv); A(V.th0, nv);
   A(V.th, nv); A(V.len, nv); A(V.ev, nv); A(V.es, nv); A(V.lx, nv); A(V.ly, nv);
-  A(V.vring, nv); A(V.vsys, nv); A(S->d_vprev, nv);
+  A(V.vring, nv); A(V.vsys, nv);
   A(V.rfirst, np); A(V.rnv, np); A(V.rsys, np);
   A(V.a0, np); A(V.area, np); A(V.perim, np); A(V.cx, np); A(V.cy, np); A(V.pe, np); A(V.ang, np);
-  A(V.bx, numSystems); A(V.by, numSystems);
+  A(V.bx, numSystems); A(V.by, numSystems); A(V.ibx, numSystems); A(V.iby, numSystems);
   A(V.svoff, numSystems + 1); A(V.sroff, numSystems + 1);
   A(V.ncnt, nv);
   A(V.tile_v0, V.ntiles); A(V.tile_v1, V.ntiles); A(V.tile_sys, V.ntiles);
-  A(S->d_stile, numSystems + 1);
+  A(V.stile, numSystems + 1);
   A(S->G.ncx, numSystems); A(S->G.ncy, numSystems); A(S->G.coff, numSystems); A(S->G.brute, numSystems);
-  A(S->d_flagNext, numSystems); A(S->d_flagCur, numSystems); A(S->d_updateCount, numSystems);
-  A(S->d_err, 1); A(S->d_maxcnt, 1); A(S->d_ctr, 4);
+  A(S->d_flag[0], numSystems); A(S->d_flag[1], numSystems); A(S->d_any, 2); A(S->d_updateCount, numSystems);
+  A(S->d_err, 1); A(S->d_maxcnt, 1); A(S->d_ctr, 4); A(S->d_notDone, 1);
   A(S->d_fire[0], numSystems); A(S->d_fire[1], numSystems);
   A(S->d_part, (long)V.ntiles * 4);
   A(S->d_red, numSystems * 8); A(S->d_scale, numSystems);
 #undef A
   set_pos_view(S);
-  hipStream_t st = nullptr;
+  hipStream_t st = S->stream;
   bool ok = up(V.vring, S->h_vring, st) == hipSuccess && up(V.vsys, S->h_vsys, st) == hipSuccess &&
-            up(S->d_vprev, S->h_vprev, st) == hipSuccess && up(V.rfirst, S->h_rfirst, st) == hipSuccess &&
-            up(V.rnv, S->h_rnv, st) == hipSuccess && up(V.rsys, S->h_rsys, st) == hipSuccess &&
-            up(V.svoff, S->h_svoff, st) == hipSuccess && up(V.sroff, S->h_sroff, st) == hipSuccess &&
-            up(V.tile_v0, S->h_tile_v0, st) == hipSuccess && up(V.tile_v1, S->h_tile_v1, st) == hipSuccess &&
-            up(V.tile_sys, S->h_tile_sys, st) == hipSuccess && up(S->d_stile, S->h_stile, st) == hipSuccess;
+            up(V.rfirst, S->h_rfirst, st) == hipSuccess && up(V.rnv, S->h_rnv, st) == hipSuccess &&
+            up(V.rsys, S->h_rsys, st) == hipSuccess && up(V.svoff, S->h_svoff, st) == hipSuccess &&
+            up(V.sroff, S->h_sroff, st) == hipSuccess && up(V.tile_v0, S->h_tile_v0, st) == hipSuccess &&
+            up(V.tile_v1, S->h_tile_v1, st) == hipSuccess && up(V.tile_sys, S->h_tile_sys, st) == hipSuccess &&
+            up(V.stile, S->h_stile, st) == hipSuccess;
   std::vector<double> ones(numSystems, 1.0);
-  ok = ok && up(V.bx, ones, st) == hipSuccess && up(V.by, ones, st) == hipSuccess;
-  // zero every state array
+  ok = ok && up(V.bx, ones, st) == hipSuccess && up(V.by, ones, st) == hipSuccess &&
+       up(V.ibx, ones, st) == hipSuccess && up(V.iby, ones, st) == hipSuccess;
   double* zd[] = {S->pos[0][0], S->pos[0][1], S->pos[1][0], S->pos[1][1], V.vx, V.vy, V.fx, V.fy, V.rad, V.l0,
                   V.th0, V.th, V.len, V.ev, V.es, V.lx, V.ly};
   for (double* p : zd) ok = ok && hipMemsetAsync(p, 0, sizeof(double) * nv, st) == hipSuccess;
   double* zp[] = {V.a0, V.area, V.perim, V.cx, V.cy, V.pe, V.ang};
   for (double* p : zp) ok = ok && hipMemsetAsync(p, 0, sizeof(double) * np, st) == hipSuccess;
   ok = ok && hipMemsetAsync(V.ncnt, 0, sizeof(int) * nv, st) == hipSuccess;
-  ok = ok && hipMemsetAsync(S->d_flagNext, 0, sizeof(int) * numSystems, st) == hipSuccess;
-  ok = ok && hipMemsetAsync(S->d_flagCur, 0, sizeof(int) * numSystems, st) == hipSuccess;
+  ok = ok && hipMemsetAsync(S->d_flag[0], 0, sizeof(int) * numSystems, st) == hipSuccess;
+  ok = ok && hipMemsetAsync(S->d_flag[1], 0, sizeof(int) * numSystems, st) == hipSuccess;
+  ok = ok && hipMemsetAsync(S->d_any, 0, sizeof(int) * 2, st) == hipSuccess;
   ok = ok && hipMemsetAsync(S->d_updateCount, 0, sizeof(int) * numSystems, st) == hipSuccess;
   ok = ok && hipMemsetAsync(S->d_err, 0, sizeof(int), st) == hipSuccess;
   ok = ok && hipMemsetAsync(S->d_ctr, 0, sizeof(unsigned long long) * 4, st) == hipSuccess;
   ok = ok && hipMemsetAsync(S->d_part, 0, sizeof(double) * V.ntiles * 4, st) == hipSuccess;
-  ok = ok && hipDeviceSynchronize() == hipSuccess;
+  ok = ok && hipStreamSynchronize(st) == hipSuccess;
   if (!ok) {
     dpm_destroy(S);
     return DPM_ERR_HIP;
@@ -413,12 +500,8 @@ int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPer
   S->P.cutDistance = 1;
   S->P.cutoff = 0;
   S->P.seed = 0;
-  S->V.W = 0;
-  if (hipStreamCreateWithFlags(&S->ownStream, hipStreamNonBlocking) != hipSuccess) {
-    dpm_destroy(S);
-    return DPM_ERR_HIP;
-  }
-  S->stream = S->ownStream;
+  S->fp.tol = -1;
+  if (const char* e = getenv("DPM_SPLIT")) S->split = std::max(1, std::min(8, atoi(e)));
   if (alloc_list(S, 8) != DPM_OK) { dpm_destroy(S); return DPM_ERR_OOM; }
   *out = S;
   return DPM_OK;
@@ -429,16 +512,15 @@ int dpm_destroy(dpm_set_t h) {
   dpm_set_s* S = h;
   hipSetDevice(S->device);
   if (S->stream) hipStreamSynchronize(S->stream);
-  hipDeviceSynchronize();
   for (auto& kv : S->graphs) hipGraphExecDestroy(kv.second);
   View& V = S->V;
   void* ptrs[] = {S->pos[0][0], S->pos[0][1], S->pos[1][0], S->pos[1][1], V.vx, V.vy, V.fx, V.fy, V.rad,
-                  V.l0, V.th0, V.th, V.len, V.ev, V.es, V.lx, V.ly, V.vring, V.vsys, S->d_vprev, V.rfirst,
-                  V.rnv, V.rsys, V.a0, V.area, V.perim, V.cx, V.cy, V.pe, V.ang, V.bx, V.by, V.svoff, V.sroff,
-                  V.ncnt, V.nbr, V.tile_v0, V.tile_v1, V.tile_sys, S->d_stile, S->G.ncx, S->G.ncy, S->G.coff,
-                  S->G.brute, S->G.count, S->G.slots, S->G.cellSys, S->d_flagNext, S->d_flagCur,
-                  S->d_updateCount, S->d_err, S->d_maxcnt, S->d_ctr, S->d_fire[0], S->d_fire[1], S->d_part,
-                  S->d_red, S->d_scale};
+                  V.l0, V.th0, V.th, V.len, V.ev, V.es, V.lx, V.ly, V.vring, V.vsys, V.rfirst, V.rnv, V.rsys,
+                  V.a0, V.area, V.perim, V.cx, V.cy, V.pe, V.ang, V.bx, V.by, V.ibx, V.iby, V.svoff, V.sroff,
+                  V.ncnt, V.nbr, V.nbrp, V.tile_v0, V.tile_v1, V.tile_sys, V.stile, S->G.ncx, S->G.ncy,
+                  S->G.coff, S->G.brute, S->G.count[0], S->G.count[1], S->G.slots, S->G.cellSys, S->d_flag[0],
+                  S->d_flag[1], S->d_any, S->d_updateCount, S->d_err, S->d_maxcnt, S->d_ctr, S->d_notDone,
+                  S->d_fire[0], S->d_fire[1], S->d_part, S->d_red, S->d_scale};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (S->ownStream) hipStreamDestroy(S->ownStream);
@@ -470,6 +552,14 @@ int dpm_synchronize(dpm_set_t h) {
   return DPM_OK;
 }
 
+int dpm_set_launch_config(dpm_set_t h, int wavesPerTile) {
+  CHECK_HANDLE(h);
+  if (wavesPerTile < 1 || wavesPerTile > 8) return fail(S, DPM_ERR_ARG, "wavesPerTile must be in 1..8");
+  S->split = wavesPerTile;
+  bump(S);
+  return DPM_OK;
+}
+
 int dpm_set_field(dpm_set_t h, int field, const double* host, long n) {
   CHECK_HANDLE(h);
   double *c0, *c1;
@@ -486,9 +576,12 @@ int dpm_set_field(dpm_set_t h, int field, const double* host, long n) {
     HIPCHK(hipMemcpyAsync(c1, b.data(), sizeof(double) * cnt, hipMemcpyHostToDevice, S->stream));
     HIPCHK(hipStreamSynchronize(S->stream));
   }
+  HIPCHK(hipStreamSynchronize(S->stream));
   if (field == DPM_FIELD_BOX) {
     for (long i = 0; i < 2 * cnt; i++) S->h_box[i] = host[i];
-    if (S->cutSet) { int rc = configure_cells(S); if (rc) return rc; }
+    int rc = upload_box(S);
+    if (rc) return rc;
+    if (S->cutSet) { rc = configure_cells(S); if (rc) return rc; }
   }
   if (field == DPM_FIELD_RAD) {
     for (long s = 0; s < S->nsys; s++) {
@@ -498,7 +591,6 @@ int dpm_set_field(dpm_set_t h, int field, const double* host, long n) {
     }
     if (S->cutSet) { int rc = configure_cells(S); if (rc) return rc; }
   }
-  HIPCHK(hipStreamSynchronize(S->stream));
   return DPM_OK;
 }
 
@@ -535,8 +627,9 @@ int dpm_device_ptr(dpm_set_t h, int field, int component, void** ptr) {
 
 int dpm_set_types(dpm_set_t h, int potential, int interaction, int monomer, int particle) {
   CHECK_HANDLE(h);
-  if (potential < 0 || potential > 3 || (interaction != INT_VV && interaction != INT_SMOOTH && interaction != INT_ALL) ||
-      monomer < 0 || monomer > 1 || particle < 0 || particle > 1)
+  if (potential < 0 || potential > 3 ||
+      (interaction != INT_VV && interaction != INT_SMOOTH && interaction != INT_ALL) || monomer < 0 ||
+      monomer > 1 || particle < 0 || particle > 1)
     return fail(S, DPM_ERR_ARG, "bad type");
   S->P.potential = potential;
   S->P.interaction = interaction;
@@ -632,14 +725,15 @@ int dpm_calc_neighbor_list(dpm_set_t h, double cutDistance) {
     int rc = configure_cells(S);
     if (rc) return rc;
   }
-  return rebuild_all(S);
+  std::vector<int> all(S->nsys, 1);
+  return rebuild(S, all, 0);
 }
 
+// checkNeighbors: displacement test against lastPos (kernelCheckVertexDisplacement) per system
 int dpm_check_neighbors(dpm_set_t h) {
   CHECK_HANDLE(h);
   int rc = require_list(S);
   if (rc) return rc;
-  // displacement test against lastPos; the drift kernels do the same test in loops
   std::vector<double> x(S->nv), y(S->nv), lx(S->nv), ly(S->nv);
   HIPCHK(hipMemcpyAsync(x.data(), S->V.x, sizeof(double) * S->nv, hipMemcpyDeviceToHost, S->stream));
   HIPCHK(hipMemcpyAsync(y.data(), S->V.y, sizeof(double) * S->nv, hipMemcpyDeviceToHost, S->stream));
@@ -655,12 +749,7 @@ int dpm_check_neighbors(dpm_set_t h) {
     dy -= Ly * round(dy / Ly);
     if (3 * sqrt(dx * dx + dy * dy) > S->P.cutoff) flags[s] = 1;
   }
-  HIPCHK(hipMemcpyAsync(S->d_flagNext, flags.data(), sizeof(int) * S->nsys, hipMemcpyHostToDevice, S->stream));
-  HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
-  HIPCHK(launch_prologue(S, 0, 0));
-  HIPCHK(launch_bin(S));
-  HIPCHK(launch_build(S));
-  return check_err_after_loop(S);
+  return rebuild(S, flags, 1);
 }
 
 int dpm_get_neighbor_list(dpm_set_t h, long* list, long* counts, long cap, long* width) {
@@ -713,12 +802,14 @@ int dpm_calc_force_energy(dpm_set_t h) {
   CHECK_HANDLE(h);
   int rc = require_list(S);
   if (rc) return rc;
-  // the prologue only bumps the active-noise counter here (no rebuild is pending-latched
-  // into a build: bin/build are not launched)
-  HIPCHK(launch_prologue(S, 0, S->P.particle == PART_ACTIVE ? 1 : 0));
+  // a standalone force call counts as one active-noise draw (addSelfPropulsion runs inside
+  // calcForceEnergy in the reference); with the any-flag cleared the bin launch only bumps it
+  HIPCHK(launch_fill_int(S->d_any + S->cur, 0, 1, S->stream));
+  HIPCHK(launch_bin(S, 0, S->P.particle == PART_ACTIVE ? 1 : 0));
   StepArgs A{};
   A.ctr = S->d_ctr;
   A.err = S->d_err;
+  A.manage = 0;
   HIPCHK(launch_step(S, MODE_FORCE, A));
   HIPCHK(hipStreamSynchronize(S->stream));
   return DPM_OK;
@@ -770,21 +861,6 @@ int dpm_nve_loop(dpm_set_t h, long steps) {
   return check_err_after_loop(S);
 }
 
-static int temperature_scale(dpm_set_s* S, double T) {
-  int rc = reduce_host(S);
-  if (rc) return rc;
-  std::vector<double> sc(S->nsys);
-  for (long s = 0; s < S->nsys; s++) {
-    const double n = S->h_svoff[s + 1] - S->h_svoff[s];
-    const double temp = 2. * (0.5 * S->h_red[s * 8 + 2]) / (n * 2);
-    sc[s] = sqrt(T / temp);
-  }
-  HIPCHK(hipMemcpyAsync(S->d_scale, sc.data(), sizeof(double) * S->nsys, hipMemcpyHostToDevice, S->stream));
-  HIPCHK(launch_scale_vel(S));
-  HIPCHK(hipStreamSynchronize(S->stream));
-  return DPM_OK;
-}
-
 int dpm_init_nve_rescale(dpm_set_t h, double T) {
   CHECK_HANDLE(h);
   S->simKind = 3;
@@ -798,10 +874,13 @@ int dpm_init_nve_rescale(dpm_set_t h, double T) {
 int dpm_nve_rescale_loop(dpm_set_t h, long steps) {
   CHECK_HANDLE(h);
   if (S->simKind != 3) return fail(S, DPM_ERR_STATE, "dpm_init_nve_rescale not called");
+  int rc = require_list(S);
+  if (rc) return rc;
+  HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
   for (long i = 0; i < steps; i++) {
-    int rc = temperature_scale(S, S->Tinject);  // NVERescale::injectKineticEnergy, Simulator.cu:222-238
+    rc = temperature_scale(S, S->Tinject);  // NVERescale::injectKineticEnergy, Simulator.cu:222-238
     if (rc) return rc;
-    rc = run_verlet(S, 1, false);
+    rc = verlet_sequence(S, 1, false);
     if (rc) return rc;
   }
   return check_err_after_loop(S);
@@ -819,6 +898,7 @@ int dpm_init_fire(dpm_set_t h, const double* p, long minStep, long numStep) {
   S->fp.dt_max = p[5];
   S->fp.minStep = minStep;
   S->fp.tol = -1;
+  S->fp.maxIter = 0;
   std::vector<FireState> fs(S->nsys);
   for (auto& f : fs) {
     f.dt = p[4]; f.a = p[6]; f.P = 0; f.numStep = numStep; f.iters = 0; f.converged = 0; f.pad = 0;
@@ -827,29 +907,60 @@ int dpm_init_fire(dpm_set_t h, const double* p, long minStep, long numStep) {
     HIPCHK(hipMemcpyAsync(S->d_fire[b], fs.data(), sizeof(FireState) * S->nsys, hipMemcpyHostToDevice, S->stream));
   S->fireCur = 0;
   S->fireInit = true;
+  bump(S);
   HIPCHK(hipStreamSynchronize(S->stream));
   return DPM_OK;
 }
-
-}  // extern "C"
-
-namespace dpm {
-__global__ void k_fire_partials(View V, double* part);
-hipError_t launch_fire_partials(dpm_set_s* S);
-}  // namespace dpm
-
-extern "C" {
 
 int dpm_fire_loop(dpm_set_t h, long iterations) {
   CHECK_HANDLE(h);
   if (!S->fireInit) return fail(S, DPM_ERR_STATE, "dpm_init_fire not called");
   int rc = require_list(S);
   if (rc) return rc;
+  if (S->fp.tol >= 0) {
+    S->fp.tol = -1;
+    bump(S);
+  }
   HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
-  HIPCHK(launch_fire_partials(S));
-  for (long i = 0; i < iterations; i++) {
-    rc = fire_iteration(S);
+  rc = run_fire(S, iterations);
+  if (rc) return rc;
+  return check_err_after_loop(S);
+}
+
+// The compressFIRE.cpp:121-131 loop for every system at once.  A system whose max |F_c| has
+// dropped to tol (or that has done maxIter iterations) freezes while the others continue; the
+// host only polls an all-converged flag between chunks of iterations.
+int dpm_fire_minimize(dpm_set_t h, double tol, long maxIter, long* iterations) {
+  CHECK_HANDLE(h);
+  if (!S->fireInit) return fail(S, DPM_ERR_STATE, "dpm_init_fire not called");
+  int rc = require_list(S);
+  if (rc) return rc;
+  if (tol < 0 || maxIter < 0) return fail(S, DPM_ERR_ARG, "tol and maxIter must be >= 0");
+  if (S->fp.tol != tol || S->fp.maxIter != maxIter) {
+    S->fp.tol = tol;
+    S->fp.maxIter = maxIter;
+    bump(S);
+  }
+  HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
+  long done = 0;
+  const long chunk = 256;
+  while (done <= maxIter) {
+    rc = run_fire(S, chunk);
     if (rc) return rc;
+    done += chunk;
+    int notDone = 0;
+    HIPCHK(launch_fill_int(S->d_notDone, 0, 1, S->stream));
+    HIPCHK(launch_fire_done(S, S->d_notDone));
+    HIPCHK(hipMemcpyAsync(&notDone, S->d_notDone, sizeof(int), hipMemcpyDeviceToHost, S->stream));
+    HIPCHK(hipStreamSynchronize(S->stream));
+    if (!notDone) break;
+  }
+  if (iterations) {
+    std::vector<FireState> fs(S->nsys);
+    HIPCHK(hipMemcpyAsync(fs.data(), S->d_fire[S->fireCur], sizeof(FireState) * S->nsys, hipMemcpyDeviceToHost,
+                          S->stream));
+    HIPCHK(hipStreamSynchronize(S->stream));
+    for (long s = 0; s < S->nsys; s++) iterations[s] = fs[s].iters;
   }
   return check_err_after_loop(S);
 }
@@ -969,14 +1080,10 @@ int dpm_time_step_kernel(dpm_set_t h, long reps, double* avgUs, double* meanNeig
   HIPCHK(dalloc(&svy, S->nv));
   HIPCHK(hipMemcpyAsync(svx, S->V.vx, sizeof(double) * S->nv, hipMemcpyDeviceToDevice, S->stream));
   HIPCHK(hipMemcpyAsync(svy, S->V.vy, sizeof(double) * S->nv, hipMemcpyDeviceToDevice, S->stream));
-  StepArgs A{};
-  A.h = 0.5 * S->P.dt;
-  A.dt = S->P.dt;
-  A.xout = S->pos[1 - S->cur][0];
-  A.yout = S->pos[1 - S->cur][1];
-  A.flagNext = S->d_flagNext;
-  A.ctr = S->d_ctr;
-  A.err = S->d_err;
+  // the Langevin step kernel as launched inside loops, with the current parity's rebuild flags
+  // clear (no rebuild); the drift writes into the other position buffer only
+  HIPCHK(launch_fill_int(S->d_flag[S->cur], 0, S->nsys, S->stream));
+  StepArgs A = loop_args(S);
   const int mode = MODE_KICK | MODE_LANGEVIN | MODE_START;
   HIPCHK(launch_step(S, mode, A));  // warm caches and code
   hipEvent_t e0, e1;
@@ -993,7 +1100,8 @@ int dpm_time_step_kernel(dpm_set_t h, long reps, double* avgUs, double* meanNeig
   *avgUs = 1000.0 * ms / reps;
   HIPCHK(hipMemcpyAsync(S->V.vx, svx, sizeof(double) * S->nv, hipMemcpyDeviceToDevice, S->stream));
   HIPCHK(hipMemcpyAsync(S->V.vy, svy, sizeof(double) * S->nv, hipMemcpyDeviceToDevice, S->stream));
-  HIPCHK(launch_fill_int(S->d_flagNext, 0, S->nsys, S->stream));
+  HIPCHK(launch_fill_int(S->d_flag[1 - S->cur], 0, S->nsys, S->stream));
+  HIPCHK(launch_fill_int(S->d_any, 0, 2, S->stream));
   std::vector<int> cnt(S->nv);
   HIPCHK(hipMemcpyAsync(cnt.data(), S->V.ncnt, sizeof(int) * S->nv, hipMemcpyDeviceToHost, S->stream));
   HIPCHK(hipStreamSynchronize(S->stream));
@@ -1008,11 +1116,6 @@ int dpm_time_step_kernel(dpm_set_t h, long reps, double* avgUs, double* meanNeig
 int dpm_force_hvp(dpm_set_t h, const double* ux, const double* uy, double* outx, double* outy) {
   CHECK_HANDLE(h);
   return fail(S, DPM_ERR_UNSUPPORTED, "dpm_force_hvp: not available in this build");
-}
-
-int dpm_fire_minimize(dpm_set_t h, double tol, long maxIter, long* iterations) {
-  CHECK_HANDLE(h);
-  return fail(S, DPM_ERR_UNSUPPORTED, "dpm_fire_minimize: not available in this build");
 }
 
 }  // extern "C"

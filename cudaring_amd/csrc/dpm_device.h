@@ -45,11 +45,19 @@ struct View {
   int *vring, *vsys;
   int *rfirst, *rnv, *rsys;
   double *a0, *area, *perim, *cx, *cy, *pe, *ang;
-  double *bx, *by;
+  double *bx, *by, *ibx, *iby;  // per-system box and its reciprocal
   int *svoff, *sroff;  // per-system vertex / ring offsets (nsys + 1)
   int *nbr, *ncnt;     // ELL neighbour list nbr[k * nv + v], count[v]
-  int *tile_v0, *tile_v1, *tile_sys;  // ring-aligned wave tiles (<= 64 vertices, one system)
+  int2* nbrp;          // beside each entry u: (prev(u), prev(prev(u))) in u's ring
+  int *tile_v0, *tile_v1, *tile_sys;  // ring-aligned tiles (<= 64 vertices, one system)
+  int* stile;          // per-system first tile (nsys + 1)
   int ntiles;
+};
+
+// Box of one system with its reciprocal: the minimum image multiplies by 1/L instead of
+// dividing; round(d * (1/L)) differs from round(d / L) only at exact half-box ties.
+struct Box {
+  double Lx, Ly, iLx, iLy;
 };
 
 // ------------------------------------------------------------ geometry ---
@@ -57,6 +65,8 @@ struct View {
 __device__ __forceinline__ double pbc(double delta, double size) {
   return delta - size * round(delta / size);
 }
+__device__ __forceinline__ double pbx(double delta, const Box& b) { return delta - b.Lx * round(delta * b.iLx); }
+__device__ __forceinline__ double pby(double delta, const Box& b) { return delta - b.Ly * round(delta * b.iLy); }
 
 // ----------------------------------------------------------- Philox RNG ---
 // Philox4x32-10 (Salmon et al., SC'11): 10 rounds, 9 key bumps.
@@ -208,50 +218,50 @@ __device__ __forceinline__ double grad_multiple(const Params& P, double distance
 enum { ROLE_THIS = 0, ROLE_OTHER = 1, ROLE_PREV = 2 };
 
 template <int POT, int ROLE>
-__device__ __forceinline__ double smooth_pair(const Params& P, double Lx, double Ly, double tix, double tiy,
-                                              double tjx, double tjy, double tpx, double tpy, double tqx,
-                                              double tqy, double radSum, double& fx, double& fy) {
+__device__ __forceinline__ double smooth_pair(const Params& P, const Box& B, double tix, double tiy, double tjx,
+                                              double tjy, double tpx, double tpy, double tqx, double tqy,
+                                              double radSum, double& fx, double& fy) {
   // segment = other - previous, length, projection of i on (prev, other)
-  double sgx = pbc(tjx - tpx, Lx), sgy = pbc(tjy - tpy, Ly);
-  double length = sqrt(sgx * sgx + sgy * sgy);
-  double ipx = pbc(tix - tpx, Lx), ipy = pbc(tiy - tpy, Ly);  // this - previous
-  double proj = (ipx * pbc(tjx - tpx, Lx) + ipy * pbc(tjy - tpy, Ly)) / (length * length);
+  const double sgx = pbx(tjx - tpx, B), sgy = pby(tjy - tpy, B);
+  const double length = sqrt(sgx * sgx + sgy * sgy);
+  const double ipx = pbx(tix - tpx, B), ipy = pby(tiy - tpy, B);  // this - previous
+  const double proj = (ipx * sgx + ipy * sgy) / (length * length);
   // previous segment = secondPrevious - previous
-  double psx = pbc(tqx - tpx, Lx), psy = pbc(tqy - tpy, Ly);
-  double previousLength = sqrt(psx * psx + psy * psy);
-  double previousProj = (pbc(tix - tqx, Lx) * pbc(tpx - tqx, Lx) + pbc(tiy - tqy, Ly) * pbc(tpy - tqy, Ly)) /
-                        (previousLength * previousLength);
+  const double psx = pbx(tqx - tpx, B), psy = pby(tqy - tpy, B);
+  const double previousLength = sqrt(psx * psx + psy * psy);
+  const double previousProj = (pbx(tix - tqx, B) * pbx(tpx - tqx, B) + pby(tiy - tqy, B) * pby(tpy - tqy, B)) /
+                              (previousLength * previousLength);
   double energy = 0.0;
   if (proj >= 0 && proj < 1) {
-    double reduced = fmax(0.0, fmin(1.0, proj));
-    double qx = tpx + reduced * sgx, qy = tpy + reduced * sgy;
-    double dqx = pbc(tix - qx, Lx), dqy = pbc(tiy - qy, Ly);
+    const double reduced = fmax(0.0, fmin(1.0, proj));
+    const double qx = tpx + reduced * sgx, qy = tpy + reduced * sgy;
+    const double dqx = pbx(tix - qx, B), dqy = pby(tiy - qy, B);
     double epot = 0.0;
-    double gm = grad_multiple<POT>(P, sqrt(dqx * dqx + dqy * dqy), radSum, epot);
+    const double gm = grad_multiple<POT>(P, sqrt(dqx * dqx + dqy * dqy), radSum, epot);
     if (gm != 0) {
       // calcCross: (prev - other)_x (other - this)_y - (other - this)_x (prev - other)_y
-      double pox = pbc(tpx - tjx, Lx), poy = pbc(tpy - tjy, Ly);
-      double cross = pox * pbc(tjy - tiy, Ly) - pbc(tjx - tix, Lx) * poy;
-      double absCross = fabs(cross);
-      double sign = absCross > 0 ? cross / absCross : 0.0;
+      const double pox = pbx(tpx - tjx, B), poy = pby(tpy - tjy, B);
+      const double cross = pox * pby(tjy - tiy, B) - pbx(tjx - tix, B) * poy;
+      const double absCross = fabs(cross);
+      const double sign = absCross > 0 ? cross / absCross : 0.0;
       if constexpr (ROLE == ROLE_THIS) {
-        fx += gm * sign * pbc(tpy - tjy, Ly) / length;
-        fy += gm * sign * pbc(tjx - tpx, Lx) / length;
+        fx += gm * sign * pby(tpy - tjy, B) / length;
+        fy += gm * sign * pbx(tjx - tpx, B) / length;
       } else if constexpr (ROLE == ROLE_OTHER) {
-        fx += gm * (sign * pbc(tiy - tpy, Ly) + absCross * pox / (length * length)) / length;
-        fy += gm * (sign * pbc(tpx - tix, Lx) + absCross * poy / (length * length)) / length;
+        fx += gm * (sign * pby(tiy - tpy, B) + absCross * pox / (length * length)) / length;
+        fy += gm * (sign * pbx(tpx - tix, B) + absCross * poy / (length * length)) / length;
       } else {
-        fx += gm * (sign * pbc(tjy - tiy, Ly) - absCross * pox / (length * length)) / length;
-        fy += gm * (sign * pbc(tix - tjx, Lx) - absCross * poy / (length * length)) / length;
+        fx += gm * (sign * pby(tjy - tiy, B) - absCross * pox / (length * length)) / length;
+        fy += gm * (sign * pbx(tix - tjx, B) - absCross * poy / (length * length)) / length;
       }
       energy += epot;
     }
     if (previousProj >= 0 && previousProj < 1) {
       // concave: subtract the pair (prev(j), i); calcVertexVertexInteraction(previousPos, thisPos)
-      double dx = pbc(tpx - tix, Lx), dy = pbc(tpy - tiy, Ly);
-      double d = sqrt(dx * dx + dy * dy);
+      const double dx = pbx(tpx - tix, B), dy = pby(tpy - tiy, B);
+      const double d = sqrt(dx * dx + dy * dy);
       double ep = 0.0;
-      double g = grad_multiple<POT>(P, d, radSum, ep);
+      const double g = grad_multiple<POT>(P, d, radSum, ep);
       if (g != 0) {
         if constexpr (ROLE == ROLE_THIS) { fx += g * dx / d; fy += g * dy / d; }
         if constexpr (ROLE == ROLE_PREV) { fx += -g * dx / d; fy += -g * dy / d; }
@@ -261,10 +271,10 @@ __device__ __forceinline__ double smooth_pair(const Params& P, double Lx, double
   } else if (proj < 0) {
     if (previousProj >= 1) {
       // convex corner: add the pair (i, prev(j)); calcVertexVertexInteraction(thisPos, previousPos)
-      double dx = pbc(tix - tpx, Lx), dy = pbc(tiy - tpy, Ly);
-      double d = sqrt(dx * dx + dy * dy);
+      const double dx = pbx(tix - tpx, B), dy = pby(tiy - tpy, B);
+      const double d = sqrt(dx * dx + dy * dy);
       double ep = 0.0;
-      double g = grad_multiple<POT>(P, d, radSum, ep);
+      const double g = grad_multiple<POT>(P, d, radSum, ep);
       if (g != 0) {
         if constexpr (ROLE == ROLE_THIS) { fx += g * dx / d; fy += g * dy / d; }
         if constexpr (ROLE == ROLE_PREV) { fx += -g * dx / d; fy += -g * dy / d; }

@@ -29,11 +29,14 @@ struct FireState {
 struct FireParams {
   double a_start, f_dec, f_inc, f_a, dt_max;
   long long minStep;
-  double tol;
+  double tol;          // < 0: no convergence test (every system advances)
+  long long maxIter;   // per-system iteration cap when tol >= 0
 };
 
+// Per-system cell lists.  Counts are double buffered by position-buffer parity: the
+// step that consumes count[b] clears count[1 - b] for the next step's binning.
 struct CellGrid {
-  int* count;      // [ncells]
+  int* count[2];   // [ncells] each
   int* slots;      // [ncells * cap]
   int* cellSys;    // [ncells]
   int* ncx;        // per system
@@ -48,11 +51,18 @@ struct StepArgs {
   double dt;                      // drift time step
   double* xout;                   // drift destination (MODE_START)
   double* yout;
-  int* flagNext;                  // per-system rebuild request (set by the drift)
+  int* flagCur;                   // rebuild flags of this step's positions
+  int* flagOut;                   // rebuild flags of the drift destination
+  int* anyOut;                    // any-flag of the drift destination
+  int* cellCur;                   // cell counts binned from this step's positions
+  int* cellOther;                 // cell counts of the other parity, cleared here
   const unsigned long long* ctr;  // [0] thermal calls, [1] active calls (already bumped)
   FireState* fire;                // MODE_FIRE: per-system state of this iteration
   double* part;                   // MODE_FIRE: per-tile partial sums [ntiles * 4]
   int* err;
+  int* maxcnt;
+  int* updateCount;
+  int manage;                     // inside loops: rebuild flagged rows, clear cells, count updates
 };
 
 }  // namespace dpm
@@ -63,9 +73,10 @@ struct dpm_set_s {
   hipStream_t ownStream = nullptr;  // created with the handle; graph capture needs a non-NULL stream
   long nsys = 0, np = 0, nv = 0;
   std::string err;
+  int split = 4;  // wavefronts per tile in the step kernel (1..8)
 
   // host copies of topology
-  std::vector<int> h_rnv, h_rfirst, h_rsys, h_vring, h_vsys, h_vprev, h_svoff, h_sroff;
+  std::vector<int> h_rnv, h_rfirst, h_rsys, h_vring, h_vsys, h_svoff, h_sroff;
   std::vector<int> h_tile_v0, h_tile_v1, h_tile_sys, h_stile;  // h_stile: per-system tile offsets
   std::vector<double> h_box;    // 2 per system
   std::vector<double> h_maxrad; // per system
@@ -74,13 +85,11 @@ struct dpm_set_s {
   dpm::Params P{};
   double* pos[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [buffer][component]
   int cur = 0;
-  int* d_vprev = nullptr;
-  int* d_stile = nullptr;
 
   // neighbour machinery
   dpm::CellGrid G{};
-  int* d_flagNext = nullptr;
-  int* d_flagCur = nullptr;
+  int* d_flag[2] = {nullptr, nullptr};  // per-system rebuild flags, by position parity
+  int* d_any = nullptr;                 // [2] any-flag by parity
   int* d_updateCount = nullptr;
   int* d_err = nullptr;
   int* d_maxcnt = nullptr;
@@ -99,6 +108,7 @@ struct dpm_set_s {
   dpm::FireParams fp{};
   double* d_part = nullptr;  // per-tile partials [ntiles * 4]
   bool fireInit = false;
+  int* d_notDone = nullptr;
 
   // reductions
   double* d_red = nullptr;   // per-system outputs [nsys * 8]
@@ -117,9 +127,8 @@ namespace dpm {
 hipError_t launch_shape_only(dpm_set_s* S);
 hipError_t launch_step(dpm_set_s* S, int mode, const StepArgs& A);
 hipError_t launch_start(dpm_set_s* S, double h, double dt);
-hipError_t launch_prologue(dpm_set_s* S, int bumpThermal, int bumpActive);
-hipError_t launch_bin(dpm_set_s* S);
-hipError_t launch_build(dpm_set_s* S);
+hipError_t launch_bin(dpm_set_s* S, int bumpThermal, int bumpActive);
+hipError_t launch_build_rows(dpm_set_s* S, int countUpdates);
 hipError_t launch_fill_int(int* p, int value, long n, hipStream_t st);
 hipError_t launch_reduce(dpm_set_s* S);
 hipError_t launch_inject(dpm_set_s* S, double amplitude, unsigned long long call);
@@ -129,7 +138,9 @@ hipError_t launch_scale_vertices(dpm_set_s* S);
 hipError_t launch_copy_lastpos(dpm_set_s* S);
 hipError_t launch_fire_begin(dpm_set_s* S);
 hipError_t launch_fire_mid(dpm_set_s* S);
+hipError_t launch_fire_partials(dpm_set_s* S);
 hipError_t launch_fire_step(dpm_set_s* S);
-hipError_t launch_hvp(dpm_set_s* S, const double* ux, const double* uy, double* ox, double* oy);
+hipError_t launch_fire_done(dpm_set_s* S, int* notDone);
+StepArgs loop_args(dpm_set_s* S);
 
 }  // namespace dpm

@@ -1,30 +1,46 @@
 // dpm_kernels.hip -- gfx950 kernels of the DPM2D vertex engine.
 //
-// Work decomposition: vertices are cut into ring-aligned wave tiles of at most
-// 64 vertices (whole rings, one system per tile).  One wavefront owns a tile:
-// ring perimeters, areas and centres are segmented wave reductions, ring
-// neighbours (prev/next vertex, their angles and rest lengths) come from lane
-// shuffles, so the reference's three per-ring/per-vertex shape kernels collapse
-// into one pass with no second read of the positions.  The interparticle force
-// is a deterministic gather over an ELL (column-major) int32 neighbour list.
+// Work decomposition.  Vertices are cut into ring-aligned tiles of at most 64
+// vertices (whole rings, one system per tile).  A step launches one workgroup per
+// tile with SPLIT wavefronts:
+//   * wave 0, one lane per vertex, does the ring geometry and the shape force:
+//     ring perimeters, areas and centres are segmented wave scans, ring
+//     neighbours (prev/next vertex, their angles and rest lengths) come from lane
+//     shuffles, so the reference's three shape kernels collapse into one pass;
+//   * all SPLIT waves share the interparticle gather over the ELL neighbour list
+//     (entry k goes to wave k mod SPLIT) and reduce their partial forces through
+//     LDS in a fixed order;
+//   * wave 0 applies the integrator epilogue (thermostat, kicks, drift,
+//     displacement test).
+// Neighbour rows are rebuilt inside the step kernel when the system's rebuild
+// flag is set: the cell lists were filled by the preceding k_bin launch, and a
+// tile's rows (including next(v) of each of its vertices) are all built by the
+// same workgroup before its gather starts.  Rebuild flags and cell counts are
+// double buffered by position-buffer parity, so a step costs two launches
+// (k_bin, normally idle, and k_step) with no host synchronisation.
 #include "dpm_internal.h"
 
 namespace dpm {
 
 #define WAVE 64
-#define BLOCK 256
+#define MAXSPLIT 8
 
 __device__ __forceinline__ double shfl_d(double v, int src) { return __shfl(v, src, WAVE); }
 __device__ __forceinline__ int shfl_i(int v, int src) { return __shfl(v, src, WAVE); }
 
-// segmented inclusive scan over lanes [segS, lane], broadcast of the segment total
-__device__ __forceinline__ double seg_sum(double v, int lane, int segS, int segE) {
+// segmented inclusive scan over lanes [segS, lane]
+__device__ __forceinline__ double seg_scan(double v, int lane, int segS) {
 #pragma unroll
   for (int d = 1; d < WAVE; d <<= 1) {
     double t = __shfl_up(v, d, WAVE);
     if (lane - d >= segS) v += t;
   }
-  return __shfl(v, segE, WAVE);
+  return v;
+}
+
+// segment total, broadcast to every lane of the segment
+__device__ __forceinline__ double seg_sum(double v, int lane, int segS, int segE) {
+  return __shfl(seg_scan(v, lane, segS), segE, WAVE);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -39,55 +55,66 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
+__device__ __forceinline__ Box box_of(const View& V, int s) { return Box{V.bx[s], V.by[s], V.ibx[s], V.iby[s]}; }
+
 // Lane context of a ring-aligned tile.
 struct Lane {
   int lane, tile, v0, v, r, s, segS, segE, nl, pl, n;
   bool act;
 };
 
-__device__ __forceinline__ bool lane_ctx(const View& V, Lane& L) {
-  L.lane = threadIdx.x & (WAVE - 1);
-  L.tile = (blockIdx.x * BLOCK + threadIdx.x) / WAVE;
-  if (L.tile >= V.ntiles) return false;
-  L.v0 = V.tile_v0[L.tile];
-  const int v1 = V.tile_v1[L.tile];
-  L.s = V.tile_sys[L.tile];
-  L.act = (L.v0 + L.lane) < v1;
-  L.v = L.act ? L.v0 + L.lane : L.v0;
+__device__ __forceinline__ void lane_ctx(const View& V, int tile, int lane, Lane& L) {
+  L.lane = lane;
+  L.tile = tile;
+  L.v0 = V.tile_v0[tile];
+  const int v1 = V.tile_v1[tile];
+  L.s = V.tile_sys[tile];
+  L.act = (L.v0 + lane) < v1;
+  L.v = L.act ? L.v0 + lane : L.v0;
   L.r = V.vring[L.v];
   const int first = V.rfirst[L.r];
   L.n = V.rnv[L.r];
-  L.segS = L.act ? first - L.v0 : L.lane;
+  L.segS = L.act ? first - L.v0 : lane;
   const int segN = L.act ? L.n : 1;
   L.segE = L.segS + segN - 1;
-  L.nl = (L.lane < L.segE) ? L.lane + 1 : L.segS;
-  L.pl = (L.lane > L.segS) ? L.lane - 1 : L.segE;
+  L.nl = (lane < L.segE) ? lane + 1 : L.segS;
+  L.pl = (lane > L.segS) ? lane - 1 : L.segE;
+}
+
+// wave-per-tile kernels (4 tiles per 256-thread block)
+__device__ __forceinline__ bool wave_tile_ctx(const View& V, Lane& L) {
+  const int tile = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  if (tile >= V.ntiles) return false;
+  lane_ctx(V, tile, threadIdx.x & (WAVE - 1), L);
   return true;
 }
 
-// Ring geometry + shape force for one lane.  Restates kernelCalcParticleShape
-// (cudaKernel.cuh:1571-1606), calcParticlePos (365-388) and
-// kernelCalcShapeForceEnergy (474-529) with the area/perimeter/bending terms of
-// 429-471.  Writes theta, length and the ring area/perimeter/centre.
+// Ring geometry + shape force for one lane (wave-uniform call).  Restates
+// kernelCalcParticleShape (cudaKernel.cuh:1571-1606), calcParticlePos (365-388)
+// and kernelCalcShapeForceEnergy (474-529) with the area/perimeter/bending
+// terms of 429-471.  Writes theta, length and the ring area/perimeter/centre.
 struct ShapeOut {
   double fx, fy, e;
-  double x, y, xn, yn, xp, yp;  // raw positions of this, next and previous vertex
 };
 
-__device__ __forceinline__ ShapeOut ring_shape(const View& V, const Params& P, const Lane& L, double Lx,
-                                               double Ly, bool write) {
+__device__ __forceinline__ ShapeOut ring_shape(const View& V, const Params& P, const Lane& L, const Box& B,
+                                               double x, double y, bool write) {
   ShapeOut o;
-  const double x = V.x[L.v], y = V.y[L.v];
   const double xn = shfl_d(x, L.nl), yn = shfl_d(y, L.nl);
   const double xp = shfl_d(x, L.pl), yp = shfl_d(y, L.pl);
   const double xf = shfl_d(x, L.segS), yf = shfl_d(y, L.segS);
-  o.x = x; o.y = y; o.xn = xn; o.yn = yn; o.xp = xp; o.yp = yp;
-  // segments of the unwrapped walk
-  const double dnx = pbc(xn - x, Lx), dny = pbc(yn - y, Ly);
-  const double psx = -pbc(xp - x, Lx), psy = -pbc(yp - y, Ly);
+  // segments of the reference's unwrapped walk
+  const double dnx = pbx(xn - x, B), dny = pby(yn - y, B);
+  const double psx = -pbx(xp - x, B), psy = -pby(yp - y, B);
   const double theta = atan2(dnx * psy - dny * psx, dnx * psx + dny * psy);  // calcAngle 423-427
   const double seglen = sqrt(dnx * dnx + dny * dny);
-  const double ux = pbc(x - xf, Lx), uy = pbc(y - yf, Ly);
+  // Unwrapped position relative to the first vertex: the reference walks the ring one
+  // minimum-image segment at a time; here that walk is an exclusive segmented scan of the
+  // segment vectors, valid for rings wider than half the box.
+  const double Ix = seg_scan(L.act ? dnx : 0.0, L.lane, L.segS);
+  const double Iy = seg_scan(L.act ? dny : 0.0, L.lane, L.segS);
+  const double ux = L.act ? Ix - dnx : 0.0;
+  const double uy = L.act ? Iy - dny : 0.0;
   const double shoe = L.act ? (ux * dny - dnx * uy) : 0.0;  // shoelace: cur x next = cur x delta
   const double perim = seg_sum(L.act ? seglen : 0.0, L.lane, L.segS, L.segE);
   const double shoeSum = seg_sum(shoe, L.lane, L.segS, L.segE);
@@ -107,9 +134,9 @@ __device__ __forceinline__ ShapeOut ring_shape(const View& V, const Params& P, c
     }
   }
   // positions relative to the ring centre (getRelativeVertexPos 273-279)
-  const double rvx = pbc(x - comx, Lx), rvy = pbc(y - comy, Ly);
-  const double rnx = pbc(xn - comx, Lx), rny = pbc(yn - comy, Ly);
-  const double rpx = pbc(xp - comx, Lx), rpy = pbc(yp - comy, Ly);
+  const double rvx = pbx(x - comx, B), rvy = pby(y - comy, B);
+  const double rnx = pbx(xn - comx, B), rny = pby(yn - comy, B);
+  const double rpx = pbx(xp - comx, B), rpy = pby(yp - comy, B);
   const double a0 = V.a0[L.r];
   // area term (calcAreaForceEnergy 429-435)
   const double deltaA = (area / a0) - 1.;
@@ -150,62 +177,229 @@ __device__ __forceinline__ ShapeOut ring_shape(const View& V, const Params& P, c
   return o;
 }
 
+// --------------------------------------------------------- neighbours ---
+__device__ __forceinline__ void cell_of(double x, double y, const Box& B, int ncx, int ncy, int& cx, int& cy) {
+  const double wx = x - B.Lx * floor(x * B.iLx), wy = y - B.Ly * floor(y * B.iLy);
+  cx = (int)(wx * B.iLx * ncx);
+  cy = (int)(wy * B.iLy * ncy);
+  cx = cx < 0 ? 0 : (cx >= ncx ? ncx - 1 : cx);
+  cy = cy < 0 ? 0 : (cy >= ncy ? ncy - 1 : cy);
+}
+
+// previous and second previous vertex of u in its ring (getPreviousId 233-243)
+__device__ __forceinline__ int2 prev_pair(const View& V, int u) {
+  const int ring = V.vring[u];
+  const int f = V.rfirst[ring], n = V.rnv[ring];
+  const int p1 = (u == f) ? f + n - 1 : u - 1;
+  const int p2 = (p1 == f) ? f + n - 1 : p1 - 1;
+  return make_int2(p1, p2);
+}
+
+// insert j (with its prev pair) into the ascending ELL row of v that holds c entries
+__device__ __forceinline__ void row_insert(const View& V, int v, int c, int j, int2 pj) {
+  const long nv = V.nv;
+  int k = c;
+  while (k > 0) {
+    const int prev = V.nbr[(long)(k - 1) * nv + v];
+    if (prev <= j) break;
+    V.nbr[(long)k * nv + v] = prev;
+    V.nbrp[(long)k * nv + v] = V.nbrp[(long)(k - 1) * nv + v];
+    k--;
+  }
+  V.nbr[(long)k * nv + v] = j;
+  V.nbrp[(long)k * nv + v] = pj;
+}
+
+// Rebuild the row of vertex v.  kernelCalcNeighborList predicate (cudaKernel.cuh:1735-1763):
+// j in list(v) iff rings differ and -(1 - d/radSum) < cutDistance; rows stay ascending as in
+// the reference.  Candidates come from the 3x3 cell block (or the whole system when the box
+// is narrower than three cells).  Also resets lastPos (resetLastPositions).
+__device__ void build_row(const View& V, const Params& P, const CellGrid& G, const int* count, int v, int s,
+                          const Box& B, int* err, int* maxcnt) {
+  const double x = V.x[v], y = V.y[v], rv = V.rad[v];
+  const int r = V.vring[v];
+  const int W = V.W;
+  int c = 0;
+  if (G.brute[s]) {
+    const int b0 = V.svoff[s], b1 = V.svoff[s + 1];
+    for (int j = b0; j < b1; j++) {
+      if (V.vring[j] == r) continue;
+      const double dx = pbx(x - V.x[j], B), dy = pby(y - V.y[j], B);
+      const double d = sqrt(dx * dx + dy * dy);
+      if (-(1 - d / (rv + V.rad[j])) < P.cutDistance) {
+        if (c < W) {
+          V.nbr[(long)c * V.nv + v] = j;
+          V.nbrp[(long)c * V.nv + v] = prev_pair(V, j);
+        }
+        c++;
+      }
+    }
+  } else {
+    const int ncx = G.ncx[s], ncy = G.ncy[s];
+    int cx, cy;
+    cell_of(x, y, B, ncx, ncy, cx, cy);
+    for (int oy = -1; oy <= 1; oy++) {
+      const int yy = (cy + oy + ncy) % ncy;
+      for (int ox = -1; ox <= 1; ox++) {
+        const int xx = (cx + ox + ncx) % ncx;
+        const int cell = G.coff[s] + yy * ncx + xx;
+        const int m = min(count[cell], G.cap);
+        for (int q = 0; q < m; q++) {
+          const int j = G.slots[(long)cell * G.cap + q];
+          if (V.vring[j] == r) continue;
+          const double dx = pbx(x - V.x[j], B), dy = pby(y - V.y[j], B);
+          const double d = sqrt(dx * dx + dy * dy);
+          if (-(1 - d / (rv + V.rad[j])) < P.cutDistance) {
+            if (c < W) row_insert(V, v, c, j, prev_pair(V, j));
+            c++;
+          }
+        }
+      }
+    }
+  }
+  if (c > W) atomicOr(err, ERR_LIST_OVERFLOW);
+  atomicMax(maxcnt, c);
+  V.ncnt[v] = c < W ? c : W;
+  V.lx[v] = x;
+  V.ly[v] = y;
+}
+
+// Bin the positions of flagged systems into the cells of this parity.  Thread 0 first bumps
+// the RNG call counters of the step and clears the other parity's flags (they were consumed
+// by the previous step).
+__global__ void k_bin(View V, CellGrid G, int* count, const int* flagCur, const int* anyCur, int* flagOther,
+                      int* anyOther, unsigned long long* ctr, int bumpT, int bumpA, int* err) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) {
+    ctr[0] += bumpT;
+    ctr[1] += bumpA;
+    *anyOther = 0;
+  }
+  if (t < V.nsys) flagOther[t] = 0;
+  if (*anyCur == 0) return;
+  if (t >= V.nv) return;
+  const int s = V.vsys[t];
+  if (!flagCur[s] || G.brute[s]) return;
+  const Box B = box_of(V, s);
+  int cx, cy;
+  cell_of(V.x[t], V.y[t], B, G.ncx[s], G.ncy[s], cx, cy);
+  const int c = G.coff[s] + cy * G.ncx[s] + cx;
+  const int slot = atomicAdd(&count[c], 1);
+  if (slot < G.cap) G.slots[(long)c * G.cap + slot] = t;
+  else atomicOr(err, ERR_CELL_OVERFLOW);
+}
+
+// Host-driven rebuild of the flagged systems' rows (calcNeighborList / checkNeighbors).
+__global__ void k_build_rows(View V, Params P, CellGrid G, const int* count, const int* flagCur, int* updateCount,
+                             int countUpdates, int* err, int* maxcnt) {
+  Lane L;
+  if (!wave_tile_ctx(V, L)) return;
+  if (!flagCur[L.s]) return;
+  if (countUpdates && L.tile == V.stile[L.s] && L.lane == 0) updateCount[L.s] += 1;
+  if (!L.act) return;
+  build_row(V, P, G, count, L.v, L.s, box_of(V, L.s), err, maxcnt);
+}
+
 // -------------------------------------------------------------- step ---
 template <int INTER, int POT, int MODE>
-__global__ __launch_bounds__(BLOCK) void k_step(View V, Params P, StepArgs A) {
+__global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepArgs A, CellGrid G) {
+  __shared__ double red[MAXSPLIT][WAVE][4];
+  const int w = threadIdx.x / WAVE;
+  const int nw = blockDim.x / WAVE;
   Lane L;
-  if (!lane_ctx(V, L)) return;
-  const double Lx = V.bx[L.s], Ly = V.by[L.s];
-  ShapeOut so = ring_shape(V, P, L, Lx, Ly, true);
-  double fx = so.fx, fy = so.fy, e = so.e;
-  const double x = so.x, y = so.y;
+  lane_ctx(V, blockIdx.x, threadIdx.x & (WAVE - 1), L);
+  const Box B = box_of(V, L.s);
+
+  // loop bookkeeping: clear the other parity's cells, count this step's rebuild, rebuild rows
+  bool rebuild = false;
+  if (A.manage) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < G.ncells; c += gridDim.x * blockDim.x)
+      A.cellOther[c] = 0;
+    rebuild = A.flagCur[L.s] != 0;
+    if (rebuild && L.tile == V.stile[L.s] && threadIdx.x == 0) A.updateCount[L.s] += 1;
+    if (rebuild && w == 0 && L.act) build_row(V, P, G, A.cellCur, L.v, L.s, B, A.err, A.maxcnt);
+    if (rebuild) __syncthreads();
+  }
+
+  const double x = V.x[L.v], y = V.y[L.v];
   const double radv = V.rad[L.v];
-  double es = 0.0;
+  double fx = 0, fy = 0, e = 0, es = 0;
+
+  // ---- interparticle gather, entries k = w, w + nw, ...
   if constexpr (INTER == INT_VV) {
     // kernelCalcVertexInteraction, cudaKernel.cuh:690-722
     const int c = L.act ? V.ncnt[L.v] : 0;
-    for (int k = 0; k < c; k++) {
+    for (int k = w; k < c; k += nw) {
       const int j = V.nbr[(long)k * V.nv + L.v];
-      e += pair_force<POT>(P, pbc(x - V.x[j], Lx), pbc(y - V.y[j], Ly), radv + V.rad[j], fx, fy);
+      e += pair_force<POT>(P, pbx(x - V.x[j], B), pby(y - V.y[j], B), radv + V.rad[j], fx, fy);
     }
   } else if constexpr (INTER == INT_SMOOTH) {
-    // kernelCalcSmoothInteraction, cudaKernel.cuh:922-975, as a gather over three roles
-    const int ppl = shfl_i(L.pl, L.pl);
-    const double xpp = shfl_d(x, ppl), ypp = shfl_d(y, ppl);
-    const double radn = shfl_d(radv, L.nl);
+    // kernelCalcSmoothInteraction, cudaKernel.cuh:922-975, as a gather over three roles:
+    // v as `this` and as `other` against every u in list(v), and v as `previous` of
+    // next(v) against every u in list(next(v)).
+    // ring neighbours of v (inactive lanes past the tile end use v itself: always in bounds)
+    const int vn = L.act ? L.v0 + L.nl : L.v;
+    const int vp = L.act ? L.v0 + L.pl : L.v;
+    const int vpp = L.act ? L.v0 + ((L.pl > L.segS) ? L.pl - 1 : L.segE) : L.v;
+    const double xn = V.x[vn], yn = V.y[vn];
+    const double xp = V.x[vp], yp = V.y[vp];
+    const double xpp = V.x[vpp], ypp = V.y[vpp];
+    const double radn = V.rad[vn];
     const int c = L.act ? V.ncnt[L.v] : 0;
-    for (int k = 0; k < c; k++) {
-      const int u = V.nbr[(long)k * V.nv + L.v];
+    for (int k = w; k < c; k += nw) {
+      const long idx = (long)k * V.nv + L.v;
+      const int u = V.nbr[idx];
+      const int2 pp = V.nbrp[idx];
       const double xu = V.x[u], yu = V.y[u], ru = V.rad[u];
-      // previous and second previous vertex of u in its ring (getPreviousId 233-243)
-      const int ru_ring = V.vring[u];
-      const int uf = V.rfirst[ru_ring], un = V.rnv[ru_ring];
-      const int p1 = (u == uf) ? uf + un - 1 : u - 1;
-      const int p2 = (p1 == uf) ? uf + un - 1 : p1 - 1;
       const double radSum = radv + ru;
-      es += smooth_pair<POT, ROLE_THIS>(P, Lx, Ly, x, y, xu, yu, V.x[p1], V.y[p1], V.x[p2], V.y[p2], radSum, fx, fy);
-      es += smooth_pair<POT, ROLE_OTHER>(P, Lx, Ly, xu, yu, x, y, so.xp, so.yp, xpp, ypp, radSum, fx, fy);
+      es += smooth_pair<POT, ROLE_THIS>(P, B, x, y, xu, yu, V.x[pp.x], V.y[pp.x], V.x[pp.y], V.y[pp.y], radSum,
+                                        fx, fy);
+      es += smooth_pair<POT, ROLE_OTHER>(P, B, xu, yu, x, y, xp, yp, xpp, ypp, radSum, fx, fy);
     }
-    const int vn = L.v0 + L.nl;
     const int cn = L.act ? V.ncnt[vn] : 0;
-    for (int k = 0; k < cn; k++) {
+    for (int k = w; k < cn; k += nw) {
       const int u = V.nbr[(long)k * V.nv + vn];
-      const double radSum = V.rad[u] + radn;
-      smooth_pair<POT, ROLE_PREV>(P, Lx, Ly, V.x[u], V.y[u], so.xn, so.yn, x, y, so.xp, so.yp, radSum, fx, fy);
-    }
-    const double pe = seg_sum(L.act ? es : 0.0, L.lane, L.segS, L.segE);
-    if (L.act) {
-      V.es[L.v] = es;
-      if (L.lane == L.segS) V.pe[L.r] = pe;
+      smooth_pair<POT, ROLE_PREV>(P, B, V.x[u], V.y[u], xn, yn, x, y, xp, yp, V.rad[u] + radn, fx, fy);
     }
   } else {
     // kernelCalcAllToAllVertexInteraction, cudaKernel.cuh:725-757 (test mode)
     if (L.act) {
       const int b0 = V.svoff[L.s], b1 = V.svoff[L.s + 1];
-      for (int j = b0; j < b1; j++) {
+      for (int j = b0 + w; j < b1; j += nw) {
         if (V.vring[j] == L.r) continue;
-        e += pair_force<POT>(P, pbc(x - V.x[j], Lx), pbc(y - V.y[j], Ly), radv + V.rad[j], fx, fy);
+        e += pair_force<POT>(P, pbx(x - V.x[j], B), pby(y - V.y[j], B), radv + V.rad[j], fx, fy);
       }
+    }
+  }
+
+  // ---- reduce the waves' partials in wave order (deterministic)
+  if (nw > 1) {
+    if (w > 0) {
+      red[w][L.lane][0] = fx;
+      red[w][L.lane][1] = fy;
+      red[w][L.lane][2] = e;
+      red[w][L.lane][3] = es;
+    }
+    __syncthreads();
+    if (w > 0) return;
+    for (int q = 1; q < nw; q++) {
+      fx += red[q][L.lane][0];
+      fy += red[q][L.lane][1];
+      e += red[q][L.lane][2];
+      es += red[q][L.lane][3];
+    }
+  }
+
+  // ---- wave 0: shape force first, then the interparticle sum (the reference's order)
+  const ShapeOut so = ring_shape(V, P, L, B, x, y, true);
+  fx = so.fx + fx;
+  fy = so.fy + fy;
+  e = so.e + e;
+  if constexpr (INTER == INT_SMOOTH) {
+    const double pe = seg_sum(L.act ? es : 0.0, L.lane, L.segS, L.segE);
+    if (L.act) {
+      V.es[L.v] = es;
+      if (L.lane == L.segS) V.pe[L.r] = pe;
     }
   }
   // active self propulsion, addSelfPropulsion DPM2D.cu:1995-2035
@@ -220,17 +414,10 @@ __global__ __launch_bounds__(BLOCK) void k_step(View V, Params P, StepArgs A) {
     a = a - kPI;
     fx += P.driving * ((1 - 0) * cos(a) + 0 * sin(a));
     fy += P.driving * ((1 - 1) * cos(a) + 1 * sin(a));
-    // every lane of the ring has read the old angle before the head lane writes
-    __builtin_amdgcn_wave_barrier();
+    // the store depends on every lane's load of the old angle, which has returned
     if (L.act && L.lane == L.segS) V.ang[L.r] = a;
   }
-  if (!L.act) {
-    if constexpr ((MODE & MODE_FIRE) != 0) {
-      // inactive lanes still take part in the tile reductions below
-    } else {
-      return;
-    }
-  }
+
   double vx = 0, vy = 0;
   if constexpr (MODE != MODE_FORCE) {
     vx = V.vx[L.v];
@@ -263,6 +450,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(View V, Params P, StepArgs A) {
     V.vx[L.v] = vx; V.vy[L.v] = vy;
     return;
   }
+  if (!L.act) return;
   V.fx[L.v] = fx;
   V.fy[L.v] = fy;
   V.ev[L.v] = e;
@@ -277,138 +465,39 @@ __global__ __launch_bounds__(BLOCK) void k_step(View V, Params P, StepArgs A) {
     const double xn = x + A.dt * vx, yn = y + A.dt * vy;
     A.xout[L.v] = xn;
     A.yout[L.v] = yn;
-    const double ddx = pbc(xn - V.lx[L.v], Lx), ddy = pbc(yn - V.ly[L.v], Ly);
-    if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) atomicOr(&A.flagNext[L.s], 1);
+    const double ddx = pbx(xn - V.lx[L.v], B), ddy = pby(yn - V.ly[L.v], B);
+    if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) {
+      atomicOr(&A.flagOut[L.s], 1);
+      atomicOr(A.anyOut, 1);
+    }
   }
   V.vx[L.v] = vx;
   V.vy[L.v] = vy;
 }
 
 // geometry only (calcParticleShape + calcParticlePositions)
-__global__ __launch_bounds__(BLOCK) void k_shape(View V, Params P) {
+__global__ void k_shape(View V, Params P) {
   Lane L;
-  if (!lane_ctx(V, L)) return;
-  ring_shape(V, P, L, V.bx[L.s], V.by[L.s], true);
+  if (!wave_tile_ctx(V, L)) return;
+  ring_shape(V, P, L, box_of(V, L.s), V.x[L.v], V.y[L.v], true);
 }
 
 // first half step of a velocity-Verlet call: v += h F; x' = x + dt v; displacement test
-__global__ void k_start(View V, Params P, double h, double dt, double* xo, double* yo, int* flagNext) {
+__global__ void k_start(View V, Params P, double h, double dt, double* xo, double* yo, int* flagOut, int* anyOut) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= V.nv) return;
   const int s = V.vsys[v];
-  double vx = V.vx[v] + h * V.fx[v];
-  double vy = V.vy[v] + h * V.fy[v];
+  const Box B = box_of(V, s);
+  const double vx = V.vx[v] + h * V.fx[v];
+  const double vy = V.vy[v] + h * V.fy[v];
   const double xn = V.x[v] + dt * vx, yn = V.y[v] + dt * vy;
   V.vx[v] = vx; V.vy[v] = vy;
   xo[v] = xn; yo[v] = yn;
-  const double ddx = pbc(xn - V.lx[v], V.bx[s]), ddy = pbc(yn - V.ly[v], V.by[s]);
-  if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) atomicOr(&flagNext[s], 1);
-}
-
-// --------------------------------------------------------- neighbours ---
-// Prologue of a step: bump RNG call counters, latch rebuild requests, clear the
-// cells of systems that rebuild.
-__global__ void k_prologue(int nsys, int* flagCur, const int* flagNext, unsigned long long* ctr, int bumpT,
-                           int bumpA, CellGrid G) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t == 0) {
-    ctr[0] += bumpT;
-    ctr[1] += bumpA;
+  const double ddx = pbx(xn - V.lx[v], B), ddy = pby(yn - V.ly[v], B);
+  if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) {
+    atomicOr(&flagOut[s], 1);
+    atomicOr(anyOut, 1);
   }
-  if (t < nsys) flagCur[t] = flagNext[t];
-  if (t < G.ncells && flagNext[G.cellSys[t]]) G.count[t] = 0;
-}
-
-__device__ __forceinline__ void cell_of(double x, double y, double Lx, double Ly, int ncx, int ncy, int& cx,
-                                        int& cy) {
-  double wx = x - Lx * floor(x / Lx), wy = y - Ly * floor(y / Ly);
-  cx = (int)(wx * ncx / Lx);
-  cy = (int)(wy * ncy / Ly);
-  cx = cx < 0 ? 0 : (cx >= ncx ? ncx - 1 : cx);
-  cy = cy < 0 ? 0 : (cy >= ncy ? ncy - 1 : cy);
-}
-
-__global__ void k_bin(View V, const int* flagCur, CellGrid G, int* err) {
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= V.nv) return;
-  const int s = V.vsys[v];
-  if (!flagCur[s] || G.brute[s]) return;
-  int cx, cy;
-  cell_of(V.x[v], V.y[v], V.bx[s], V.by[s], G.ncx[s], G.ncy[s], cx, cy);
-  const int c = G.coff[s] + cy * G.ncx[s] + cx;
-  const int slot = atomicAdd(&G.count[c], 1);
-  if (slot < G.cap) G.slots[(long)c * G.cap + slot] = v;
-  else atomicOr(err, ERR_CELL_OVERFLOW);
-}
-
-// insert j into the ascending ELL row of v (count c entries present)
-__device__ __forceinline__ void row_insert(int* nbr, long nv, int v, int c, int j) {
-  int k = c;
-  while (k > 0) {
-    const int prev = nbr[(long)(k - 1) * nv + v];
-    if (prev <= j) break;
-    nbr[(long)k * nv + v] = prev;
-    k--;
-  }
-  nbr[(long)k * nv + v] = j;
-}
-
-// kernelCalcNeighborList predicate (cudaKernel.cuh:1735-1763): j in list(i) iff rings differ
-// and -(1 - d/radSum) < cutDistance.  Rows are kept ascending as in the reference.
-__global__ void k_build(View V, Params P, const int* flagCur, int* flagNext, int* updateCount, CellGrid G,
-                        int* err, int* maxcnt) {
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v < V.nsys) {
-    flagNext[v] = 0;
-    if (flagCur[v]) updateCount[v] += 1;
-  }
-  if (v >= V.nv) return;
-  const int s = V.vsys[v];
-  if (!flagCur[s]) return;
-  const double Lx = V.bx[s], Ly = V.by[s];
-  const double x = V.x[v], y = V.y[v], rv = V.rad[v];
-  const int r = V.vring[v];
-  const int W = V.W;
-  int c = 0;
-  if (G.brute[s]) {
-    const int b0 = V.svoff[s], b1 = V.svoff[s + 1];
-    for (int j = b0; j < b1; j++) {
-      if (V.vring[j] == r) continue;
-      const double dx = pbc(x - V.x[j], Lx), dy = pbc(y - V.y[j], Ly);
-      const double d = sqrt(dx * dx + dy * dy);
-      if (-(1 - d / (rv + V.rad[j])) < P.cutDistance) {
-        if (c < W) V.nbr[(long)c * V.nv + v] = j;
-        c++;
-      }
-    }
-  } else {
-    const int ncx = G.ncx[s], ncy = G.ncy[s];
-    int cx, cy;
-    cell_of(x, y, Lx, Ly, ncx, ncy, cx, cy);
-    for (int oy = -1; oy <= 1; oy++) {
-      const int yy = (cy + oy + ncy) % ncy;
-      for (int ox = -1; ox <= 1; ox++) {
-        const int xx = (cx + ox + ncx) % ncx;
-        const int cell = G.coff[s] + yy * ncx + xx;
-        const int m = min(G.count[cell], G.cap);
-        for (int q = 0; q < m; q++) {
-          const int j = G.slots[(long)cell * G.cap + q];
-          if (V.vring[j] == r) continue;
-          const double dx = pbc(x - V.x[j], Lx), dy = pbc(y - V.y[j], Ly);
-          const double d = sqrt(dx * dx + dy * dy);
-          if (-(1 - d / (rv + V.rad[j])) < P.cutDistance) {
-            if (c < W) row_insert(V.nbr, V.nv, v, c, j);
-            c++;
-          }
-        }
-      }
-    }
-  }
-  if (c > W) atomicOr(err, ERR_LIST_OVERFLOW);
-  atomicMax(maxcnt, c);
-  V.ncnt[v] = c < W ? c : W;
-  V.lx[v] = x;
-  V.ly[v] = y;
 }
 
 __global__ void k_fill_int(int* p, int value, long n) {
@@ -427,19 +516,19 @@ __global__ void k_copy_lastpos(View V) {
 // One block per system, fixed striding and tree order (deterministic).
 // out[s*8 + q]: 0 sum e_v, 1 sum pe, 2 sum v^2, 3 max F_c^2, 4 sum F^2,
 //               5 sum area, 6 sum a0, 7 sum vertex-disk area r^2 (n/2 - 1)
-__global__ __launch_bounds__(BLOCK) void k_reduce(View V, double* out) {
+__global__ __launch_bounds__(256) void k_reduce(View V, double* out) {
   const int s = blockIdx.x;
-  __shared__ double sh[8][BLOCK / WAVE];
+  __shared__ double sh[8][4];
   double acc[8] = {0, 0, 0, -1, 0, 0, 0, 0};
   const int b0 = V.svoff[s], b1 = V.svoff[s + 1];
-  for (int v = b0 + threadIdx.x; v < b1; v += BLOCK) {
+  for (int v = b0 + threadIdx.x; v < b1; v += 256) {
     acc[0] += V.ev[v];
     acc[2] += V.vx[v] * V.vx[v] + V.vy[v] * V.vy[v];
     acc[3] = fmax(acc[3], fmax(V.fx[v] * V.fx[v], V.fy[v] * V.fy[v]));
     acc[4] += V.fx[v] * V.fx[v] + V.fy[v] * V.fy[v];
   }
   const int r0 = V.sroff[s], r1 = V.sroff[s + 1];
-  for (int r = r0 + threadIdx.x; r < r1; r += BLOCK) {
+  for (int r = r0 + threadIdx.x; r < r1; r += 256) {
     acc[1] += V.pe[r];
     acc[5] += V.area[r];
     acc[6] += V.a0[r];
@@ -449,14 +538,14 @@ __global__ __launch_bounds__(BLOCK) void k_reduce(View V, double* out) {
   const int w = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
 #pragma unroll
   for (int q = 0; q < 8; q++) {
-    double t = (q == 3) ? wave_max(acc[q]) : wave_sum(acc[q]);
+    const double t = (q == 3) ? wave_max(acc[q]) : wave_sum(acc[q]);
     if (lane == 0) sh[q][w] = t;
   }
   __syncthreads();
   if (threadIdx.x < 8) {
     const int q = threadIdx.x;
     double t = sh[q][0];
-    for (int i = 1; i < BLOCK / WAVE; i++) t = (q == 3) ? fmax(t, sh[q][i]) : t + sh[q][i];
+    for (int i = 1; i < 4; i++) t = (q == 3) ? fmax(t, sh[q][i]) : t + sh[q][i];
     out[s * 8 + q] = t;
   }
 }
@@ -472,25 +561,25 @@ __global__ void k_inject(View V, Params P, double amplitude, unsigned long long 
   V.vy[v] = amplitude * z1;
 }
 
-// subtract the per-system mean velocity; sums come from a dedicated reduction
-__global__ __launch_bounds__(BLOCK) void k_sub_mean(View V) {
+// subtract the per-system mean velocity
+__global__ __launch_bounds__(256) void k_sub_mean(View V) {
   const int s = blockIdx.x;
-  __shared__ double sh[2][BLOCK / WAVE];
+  __shared__ double sh[2][4];
   __shared__ double mean[2];
   const int b0 = V.svoff[s], b1 = V.svoff[s + 1];
   double ax = 0, ay = 0;
-  for (int v = b0 + threadIdx.x; v < b1; v += BLOCK) { ax += V.vx[v]; ay += V.vy[v]; }
+  for (int v = b0 + threadIdx.x; v < b1; v += 256) { ax += V.vx[v]; ay += V.vy[v]; }
   const int w = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
   ax = wave_sum(ax); ay = wave_sum(ay);
   if (lane == 0) { sh[0][w] = ax; sh[1][w] = ay; }
   __syncthreads();
   if (threadIdx.x < 2) {
     double t = 0;
-    for (int i = 0; i < BLOCK / WAVE; i++) t += sh[threadIdx.x][i];
+    for (int i = 0; i < 4; i++) t += sh[threadIdx.x][i];
     mean[threadIdx.x] = t / (b1 - b0);
   }
   __syncthreads();
-  for (int v = b0 + threadIdx.x; v < b1; v += BLOCK) { V.vx[v] -= mean[0]; V.vy[v] -= mean[1]; }
+  for (int v = b0 + threadIdx.x; v < b1; v += 256) { V.vx[v] -= mean[0]; V.vy[v] -= mean[1]; }
 }
 
 // v *= scale[s]
@@ -521,16 +610,33 @@ __global__ void k_scale_vertices(View V, const double* scale) {
 }
 
 // --------------------------------------------------------------- FIRE ---
+// sum of the per-tile partials q of system s, all 64 lanes striding over the tiles; the
+// xor-butterfly leaves the same value in every lane (addition is commutative)
+__device__ __forceinline__ double sys_sum(const double* part, int q, int t0, int t1, int lane) {
+  double acc = 0;
+  for (int t = t0 + lane; t < t1; t += WAVE) acc += part[t * 4 + q];
+  return wave_sum(acc);
+}
+
+__device__ __forceinline__ double sys_max(const double* part, int q, int t0, int t1, int lane) {
+  double acc = -1;
+  for (int t = t0 + lane; t < t1; t += WAVE) acc = fmax(acc, part[t * 4 + q]);
+  return wave_max(acc);
+}
+
 // FIRE::minimizerVertexLoop, FIRE.cu:397-411, split at its two global reductions.
 // fire_begin: checkFIREParams (338-371) + half kick + partial |v|^2, |F|^2.
-__global__ __launch_bounds__(BLOCK) void k_fire_begin(View V, FireParams fp, const FireState* fin, FireState* fout,
-                                                      const double* part, double* partOut, const int* stile) {
+// With tol >= 0 a system whose max |F_c| <= tol (or that reached maxIter) is frozen.
+__global__ void k_fire_begin(View V, FireParams fp, const FireState* fin, FireState* fout, double* part) {
   Lane L;
-  if (!lane_ctx(V, L)) return;
+  if (!wave_tile_ctx(V, L)) return;
   FireState fs = fin[L.s];
-  // P = v.F over the system, summed tile by tile in fixed order
-  double Psum = 0;
-  for (int t = stile[L.s]; t < stile[L.s + 1]; t++) Psum += part[t * 4 + 0];
+  const int t0 = V.stile[L.s], t1 = V.stile[L.s + 1];
+  const double Psum = sys_sum(part, 0, t0, t1, L.lane);
+  if (!fs.converged && fp.tol >= 0) {
+    const double fmax2 = sys_max(part, 3, t0, t1, L.lane);
+    if (sqrt(fmax2) <= fp.tol || fs.iters >= fp.maxIter) fs.converged = 1;
+  }
   bool reset = false;
   if (!fs.converged) {
     fs.P = Psum;
@@ -548,7 +654,7 @@ __global__ __launch_bounds__(BLOCK) void k_fire_begin(View V, FireParams fp, con
     }
     fs.iters++;
   }
-  if (L.tile == stile[L.s] && L.lane == 0) fout[L.s] = fs;
+  if (L.tile == t0 && L.lane == 0) fout[L.s] = fs;
   double vx = 0, vy = 0, fx = 0, fy = 0;
   if (L.act) {
     vx = V.vx[L.v]; vy = V.vy[L.v]; fx = V.fx[L.v]; fy = V.fy[L.v];
@@ -569,18 +675,21 @@ __global__ __launch_bounds__(BLOCK) void k_fire_begin(View V, FireParams fp, con
   const double tv = wave_sum(vx * vx + vy * vy);
   const double tf = wave_sum(fx * fx + fy * fy);
   if (L.lane == 0) {
-    partOut[L.tile * 4 + 1] = tv;
-    partOut[L.tile * 4 + 2] = tf;
+    part[L.tile * 4 + 1] = tv;
+    part[L.tile * 4 + 2] = tf;
   }
 }
 
 // fire_mid: bendVertexVelocityTowardsForce (374-394) + drift + displacement test
-__global__ __launch_bounds__(BLOCK) void k_fire_mid(View V, Params P, const FireState* fcur, const double* part,
-                                                    const int* stile, double* xo, double* yo, int* flagNext) {
+__global__ void k_fire_mid(View V, Params P, const FireState* fcur, const double* part, double* xo, double* yo,
+                           int* flagOut, int* anyOut) {
   Lane L;
-  if (!lane_ctx(V, L)) return;
-  if (!L.act) return;
+  if (!wave_tile_ctx(V, L)) return;
   const FireState fs = fcur[L.s];
+  const int t0 = V.stile[L.s], t1 = V.stile[L.s + 1];
+  const double vv = sys_sum(part, 1, t0, t1, L.lane);
+  const double ff = sys_sum(part, 2, t0, t1, L.lane);
+  if (!L.act) return;
   double vx = V.vx[L.v], vy = V.vy[L.v];
   const double x = V.x[L.v], y = V.y[L.v];
   if (fs.converged) {
@@ -588,8 +697,6 @@ __global__ __launch_bounds__(BLOCK) void k_fire_mid(View V, Params P, const Fire
     yo[L.v] = y;
     return;
   }
-  double vv = 0, ff = 0;
-  for (int t = stile[L.s]; t < stile[L.s + 1]; t++) { vv += part[t * 4 + 1]; ff += part[t * 4 + 2]; }
   if (ff > 0) {
     const double ratio = sqrt(vv / ff);
     const double fa = fs.a;
@@ -601,15 +708,18 @@ __global__ __launch_bounds__(BLOCK) void k_fire_mid(View V, Params P, const Fire
   V.vy[L.v] = vy;
   xo[L.v] = xn;
   yo[L.v] = yn;
-  const double Lx = V.bx[L.s], Ly = V.by[L.s];
-  const double ddx = pbc(xn - V.lx[L.v], Lx), ddy = pbc(yn - V.ly[L.v], Ly);
-  if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) atomicOr(&flagNext[L.s], 1);
+  const Box B = box_of(V, L.s);
+  const double ddx = pbx(xn - V.lx[L.v], B), ddy = pby(yn - V.ly[L.v], B);
+  if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) {
+    atomicOr(&flagOut[L.s], 1);
+    atomicOr(anyOut, 1);
+  }
 }
 
 // per-tile partials of P = v.F and max F_c^2 for the current state (start of a FIRE call)
-__global__ __launch_bounds__(BLOCK) void k_fire_partials(View V, double* part) {
+__global__ void k_fire_partials(View V, double* part) {
   Lane L;
-  if (!lane_ctx(V, L)) return;
+  if (!wave_tile_ctx(V, L)) return;
   double pvf = 0, f2 = -1;
   if (L.act) {
     const double vx = V.vx[L.v], vy = V.vy[L.v], fx = V.fx[L.v], fy = V.fy[L.v];
@@ -624,26 +734,34 @@ __global__ __launch_bounds__(BLOCK) void k_fire_partials(View V, double* part) {
   }
 }
 
+// per-system FIRE summary after a minimisation: all-converged flag
+__global__ void k_fire_done(const FireState* fs, int nsys, int* notDone) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nsys && !fs[s].converged) atomicOr(notDone, 1);
+}
+
 // --------------------------------------------------------- launchers ---
 static inline int grid_v(long n) { return (int)((n + 255) / 256); }
-static inline int grid_t(int ntiles) { return (ntiles + 3) / 4; }
+static inline int grid_w(int ntiles) { return (ntiles + 3) / 4; }
 
 template <int INTER, int POT>
 static hipError_t step_mode(dpm_set_s* S, int mode, const StepArgs& A) {
-  const int g = grid_t(S->V.ntiles);
+  const int g = S->V.ntiles;
+  const int b = WAVE * S->split;
+  hipStream_t st = S->stream;
   switch (mode) {
-    case MODE_FORCE: k_step<INTER, POT, MODE_FORCE><<<g, BLOCK, 0, S->stream>>>(S->V, S->P, A); break;
-    case MODE_KICK: k_step<INTER, POT, MODE_KICK><<<g, BLOCK, 0, S->stream>>>(S->V, S->P, A); break;
+    case MODE_FORCE: k_step<INTER, POT, MODE_FORCE><<<g, b, 0, st>>>(S->V, S->P, A, S->G); break;
+    case MODE_KICK: k_step<INTER, POT, MODE_KICK><<<g, b, 0, st>>>(S->V, S->P, A, S->G); break;
     case MODE_KICK | MODE_LANGEVIN:
-      k_step<INTER, POT, MODE_KICK | MODE_LANGEVIN><<<g, BLOCK, 0, S->stream>>>(S->V, S->P, A);
+      k_step<INTER, POT, MODE_KICK | MODE_LANGEVIN><<<g, b, 0, st>>>(S->V, S->P, A, S->G);
       break;
     case MODE_KICK | MODE_START:
-      k_step<INTER, POT, MODE_KICK | MODE_START><<<g, BLOCK, 0, S->stream>>>(S->V, S->P, A);
+      k_step<INTER, POT, MODE_KICK | MODE_START><<<g, b, 0, st>>>(S->V, S->P, A, S->G);
       break;
     case MODE_KICK | MODE_LANGEVIN | MODE_START:
-      k_step<INTER, POT, MODE_KICK | MODE_LANGEVIN | MODE_START><<<g, BLOCK, 0, S->stream>>>(S->V, S->P, A);
+      k_step<INTER, POT, MODE_KICK | MODE_LANGEVIN | MODE_START><<<g, b, 0, st>>>(S->V, S->P, A, S->G);
       break;
-    case MODE_FIRE: k_step<INTER, POT, MODE_FIRE><<<g, BLOCK, 0, S->stream>>>(S->V, S->P, A); break;
+    case MODE_FIRE: k_step<INTER, POT, MODE_FIRE><<<g, b, 0, st>>>(S->V, S->P, A, S->G); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -661,17 +779,24 @@ static hipError_t step_pot(dpm_set_s* S, int mode, const StepArgs& A) {
 }
 
 hipError_t launch_step(dpm_set_s* S, int mode, const StepArgs& A) {
+  // refuse argument sets the chosen epilogue would dereference as null
+  if (!A.ctr) return hipErrorInvalidValue;
+  if ((mode & MODE_FIRE) && (!A.fire || !A.part)) return hipErrorInvalidValue;
+  if ((mode & MODE_START) && (!A.xout || !A.yout || !A.flagOut || !A.anyOut)) return hipErrorInvalidValue;
+  if (A.manage && (!A.flagCur || !A.cellCur || !A.cellOther || !A.err || !A.maxcnt || !A.updateCount))
+    return hipErrorInvalidValue;
   switch (S->P.interaction) {
     case INT_VV: return step_pot<INT_VV>(S, mode, A);
     case INT_SMOOTH: return step_pot<INT_SMOOTH>(S, mode, A);
     case INT_ALL: {
       if (mode != MODE_FORCE) return hipErrorInvalidValue;
-      const int g = grid_t(S->V.ntiles);
+      const int g = S->V.ntiles, b = WAVE * S->split;
+      hipStream_t st = S->stream;
       switch (S->P.potential) {
-        case POT_HARMONIC: k_step<INT_ALL, POT_HARMONIC, MODE_FORCE><<<g, BLOCK, 0, S->stream>>>(S->V, S->P, A); break;
-        case POT_LJ: k_step<INT_ALL, POT_LJ, MODE_FORCE><<<g, BLOCK, 0, S->stream>>>(S->V, S->P, A); break;
-        case POT_ADHESIVE: k_step<INT_ALL, POT_ADHESIVE, MODE_FORCE><<<g, BLOCK, 0, S->stream>>>(S->V, S->P, A); break;
-        case POT_WCA: k_step<INT_ALL, POT_WCA, MODE_FORCE><<<g, BLOCK, 0, S->stream>>>(S->V, S->P, A); break;
+        case POT_HARMONIC: k_step<INT_ALL, POT_HARMONIC, MODE_FORCE><<<g, b, 0, st>>>(S->V, S->P, A, S->G); break;
+        case POT_LJ: k_step<INT_ALL, POT_LJ, MODE_FORCE><<<g, b, 0, st>>>(S->V, S->P, A, S->G); break;
+        case POT_ADHESIVE: k_step<INT_ALL, POT_ADHESIVE, MODE_FORCE><<<g, b, 0, st>>>(S->V, S->P, A, S->G); break;
+        case POT_WCA: k_step<INT_ALL, POT_WCA, MODE_FORCE><<<g, b, 0, st>>>(S->V, S->P, A, S->G); break;
       }
       return hipGetLastError();
     }
@@ -680,33 +805,31 @@ hipError_t launch_step(dpm_set_s* S, int mode, const StepArgs& A) {
 }
 
 hipError_t launch_shape_only(dpm_set_s* S) {
-  k_shape<<<grid_t(S->V.ntiles), BLOCK, 0, S->stream>>>(S->V, S->P);
+  k_shape<<<grid_w(S->V.ntiles), 256, 0, S->stream>>>(S->V, S->P);
   return hipGetLastError();
 }
 
 hipError_t launch_start(dpm_set_s* S, double h, double dt) {
   const int nb = 1 - S->cur;
-  k_start<<<grid_v(S->nv), 256, 0, S->stream>>>(S->V, S->P, h, dt, S->pos[nb][0], S->pos[nb][1], S->d_flagNext);
+  k_start<<<grid_v(S->nv), 256, 0, S->stream>>>(S->V, S->P, h, dt, S->pos[nb][0], S->pos[nb][1],
+                                                S->d_flag[nb], S->d_any + nb);
   return hipGetLastError();
 }
 
-hipError_t launch_prologue(dpm_set_s* S, int bumpThermal, int bumpActive) {
-  long n = S->nsys > S->G.ncells ? S->nsys : S->G.ncells;
-  if (n < 1) n = 1;
-  k_prologue<<<grid_v(n), 256, 0, S->stream>>>((int)S->nsys, S->d_flagCur, S->d_flagNext, S->d_ctr, bumpThermal,
-                                                bumpActive, S->G);
-  return hipGetLastError();
-}
-
-hipError_t launch_bin(dpm_set_s* S) {
-  k_bin<<<grid_v(S->nv), 256, 0, S->stream>>>(S->V, S->d_flagCur, S->G, S->d_err);
-  return hipGetLastError();
-}
-
-hipError_t launch_build(dpm_set_s* S) {
+hipError_t launch_bin(dpm_set_s* S, int bumpThermal, int bumpActive) {
+  const int b = S->cur;
   long n = S->nv > S->nsys ? S->nv : S->nsys;
-  k_build<<<grid_v(n), 256, 0, S->stream>>>(S->V, S->P, S->d_flagCur, S->d_flagNext, S->d_updateCount, S->G,
-                                             S->d_err, S->d_maxcnt);
+  k_bin<<<grid_v(n), 256, 0, S->stream>>>(S->V, S->G, S->G.count[b], S->d_flag[b], S->d_any + b,
+                                          S->d_flag[1 - b], S->d_any + 1 - b, S->d_ctr, bumpThermal, bumpActive,
+                                          S->d_err);
+  return hipGetLastError();
+}
+
+hipError_t launch_build_rows(dpm_set_s* S, int countUpdates) {
+  const int b = S->cur;
+  k_build_rows<<<grid_w(S->V.ntiles), 256, 0, S->stream>>>(S->V, S->P, S->G, S->G.count[b], S->d_flag[b],
+                                                             S->d_updateCount, countUpdates, S->d_err,
+                                                             S->d_maxcnt);
   return hipGetLastError();
 }
 
@@ -717,7 +840,7 @@ hipError_t launch_fill_int(int* p, int value, long n, hipStream_t st) {
 }
 
 hipError_t launch_reduce(dpm_set_s* S) {
-  k_reduce<<<(int)S->nsys, BLOCK, 0, S->stream>>>(S->V, S->d_red);
+  k_reduce<<<(int)S->nsys, 256, 0, S->stream>>>(S->V, S->d_red);
   return hipGetLastError();
 }
 
@@ -727,7 +850,7 @@ hipError_t launch_inject(dpm_set_s* S, double amplitude, unsigned long long call
 }
 
 hipError_t launch_sub_mean(dpm_set_s* S) {
-  k_sub_mean<<<(int)S->nsys, BLOCK, 0, S->stream>>>(S->V);
+  k_sub_mean<<<(int)S->nsys, 256, 0, S->stream>>>(S->V);
   return hipGetLastError();
 }
 
@@ -748,31 +871,53 @@ hipError_t launch_copy_lastpos(dpm_set_s* S) {
 }
 
 hipError_t launch_fire_begin(dpm_set_s* S) {
-  k_fire_begin<<<grid_t(S->V.ntiles), BLOCK, 0, S->stream>>>(S->V, S->fp, S->d_fire[S->fireCur],
-                                                              S->d_fire[1 - S->fireCur], S->d_part, S->d_part,
-                                                              S->d_stile);
+  k_fire_begin<<<grid_w(S->V.ntiles), 256, 0, S->stream>>>(S->V, S->fp, S->d_fire[S->fireCur],
+                                                            S->d_fire[1 - S->fireCur], S->d_part);
   return hipGetLastError();
 }
 
 hipError_t launch_fire_mid(dpm_set_s* S) {
   const int nb = 1 - S->cur;
-  k_fire_mid<<<grid_t(S->V.ntiles), BLOCK, 0, S->stream>>>(S->V, S->P, S->d_fire[1 - S->fireCur], S->d_part,
-                                                            S->d_stile, S->pos[nb][0], S->pos[nb][1],
-                                                            S->d_flagNext);
+  k_fire_mid<<<grid_w(S->V.ntiles), 256, 0, S->stream>>>(S->V, S->P, S->d_fire[S->fireCur], S->d_part,
+                                                          S->pos[nb][0], S->pos[nb][1], S->d_flag[nb],
+                                                          S->d_any + nb);
   return hipGetLastError();
 }
 
 hipError_t launch_fire_partials(dpm_set_s* S) {
-  k_fire_partials<<<grid_t(S->V.ntiles), BLOCK, 0, S->stream>>>(S->V, S->d_part);
+  k_fire_partials<<<grid_w(S->V.ntiles), 256, 0, S->stream>>>(S->V, S->d_part);
   return hipGetLastError();
 }
 
-hipError_t launch_fire_step(dpm_set_s* S) {
+hipError_t launch_fire_done(dpm_set_s* S, int* notDone) {
+  k_fire_done<<<grid_v(S->nsys), 256, 0, S->stream>>>(S->d_fire[S->fireCur], (int)S->nsys, notDone);
+  return hipGetLastError();
+}
+
+StepArgs loop_args(dpm_set_s* S) {
+  const int b = S->cur;
   StepArgs A{};
+  A.h = 0.5 * S->P.dt;
+  A.dt = S->P.dt;
+  A.xout = S->pos[1 - b][0];
+  A.yout = S->pos[1 - b][1];
+  A.flagCur = S->d_flag[b];
+  A.flagOut = S->d_flag[1 - b];
+  A.anyOut = S->d_any + 1 - b;
+  A.cellCur = S->G.count[b];
+  A.cellOther = S->G.count[1 - b];
   A.ctr = S->d_ctr;
-  A.fire = S->d_fire[1 - S->fireCur];
-  A.part = S->d_part;
   A.err = S->d_err;
+  A.maxcnt = S->d_maxcnt;
+  A.updateCount = S->d_updateCount;
+  A.manage = 1;
+  return A;
+}
+
+hipError_t launch_fire_step(dpm_set_s* S) {
+  StepArgs A = loop_args(S);
+  A.fire = S->d_fire[S->fireCur];
+  A.part = S->d_part;
   return launch_step(S, MODE_FIRE, A);
 }
 

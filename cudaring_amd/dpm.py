@@ -74,6 +74,7 @@ SIGNATURES = {
     "dpm_scale_vertices": (_i, [_vp, _vp]),
     "dpm_force_hvp": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "dpm_time_step_kernel": (_i, [_vp, _l, C.POINTER(_d), C.POINTER(_d)]),
+    "dpm_set_launch_config": (_i, [_vp, _i]),
 }
 
 _lib = None
@@ -326,6 +327,9 @@ class DPMSet:
 
     def set_stream(self, stream_ptr):
         self._chk(self.L.dpm_set_stream(self.h, C.c_void_p(stream_ptr)), "set_stream")
+
+    def set_launch_config(self, waves_per_tile):
+        self._chk(self.L.dpm_set_launch_config(self.h, waves_per_tile), "set_launch_config")
 
     def time_step_kernel(self, reps=200):
         us, kbar = C.c_double(), C.c_double()

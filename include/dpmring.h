@@ -172,7 +172,10 @@ int dpm_get_particle_shapes(dpm_set_t h, double* shape3);
 /* scaleVertices(scale) per system, DPM2D.h:492 / DPM2D.cu:1802-1811 */
 int dpm_scale_vertices(dpm_set_t h, const double* scalePerSystem);
 
-/* ---- measurement (no reference counterpart) ------------------------------- */
+/* ---- launch configuration and measurement (no reference counterpart) ------- */
+/* Wavefronts per ring tile in the step kernel (1..8): the gather over a tile's neighbour rows
+ * is split across them.  Chosen by the autotuner (cudaring_amd/autotune.py). */
+int dpm_set_launch_config(dpm_set_t h, int wavesPerTile);
 /* Average duration of the fused Langevin step kernel (ring shape + interaction + thermostat +
  * kicks + drift) over `reps` back-to-back launches on the handle's stream, timed with hipEvents
  * on that stream.  Velocities are restored afterwards; *meanNeighbors = mean list length. */

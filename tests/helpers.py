@@ -35,3 +35,13 @@ def gpu_from_state(st, potential="harmonic", interaction="vertexVertex", costs=(
 
 def small_state(nrings=12, phi=0.7, seed=3):
     return packing.random_system(nrings, phi, seed)
+
+
+def relaxed_state(nrings, phi, seed, final=("wca", "vertexSmooth")):
+    """A compressed, FIRE-relaxed packing prepared on the GPU (deterministic), so that a stiff
+    potential starts without deep ring overlaps.  GPU tests only."""
+    from cudaring_amd import DPMSet, protocols
+    s, st = protocols.prepared_state(lambda nvp: DPMSet(nvp), nrings, phi, seed, dphi=0.05, iters=400,
+                                     final=final, final_iters=1500)
+    s.close()
+    return st

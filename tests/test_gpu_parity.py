@@ -7,7 +7,7 @@ import pytest
 
 from cudaring_amd import packing
 from tests.conftest import rel_err
-from tests.helpers import gpu_from_state, oracle_from_state
+from tests.helpers import gpu_from_state, oracle_from_state, relaxed_state
 
 pytestmark = pytest.mark.gpu
 
@@ -75,7 +75,7 @@ def test_ring_size_edges(oracle_mod):
 @pytest.mark.parametrize("pot,inter", [("harmonic", "vertexVertex"), ("wca", "vertexSmooth")])
 def test_nve_window(oracle_mod, pot, inter):
     print(f"case nve_window {pot} {inter}", flush=True)
-    st = packing.lattice_system(16, 0.8, 17)
+    st = packing.lattice_system(16, 0.8, 17) if pot == "harmonic" else relaxed_state(16, 0.8, 17)
     o, g = prepare(oracle_mod, st, pot, inter, dt=0.005)
     o.calc_force_energy(); g.calc_force_energy()
     o.init_nve(1e-2, zero_momentum=True); g.init_nve(1e-2, zero_momentum=True)
@@ -91,7 +91,7 @@ def test_nve_window(oracle_mod, pot, inter):
 @pytest.mark.parametrize("inter", INTERS)
 def test_langevin_window(oracle_mod, inter):
     print(f"case langevin_window {inter}", flush=True)
-    st = packing.lattice_system(16, 0.85, 23)
+    st = relaxed_state(16, 0.85, 23, final=("wca", inter))
     o, g = prepare(oracle_mod, st, "wca", inter, dt=0.002, seed=99)
     o.calc_force_energy(); g.calc_force_energy()
     o.init_langevin(1e-2, 1.0); g.init_langevin(1e-2, 1.0)
@@ -104,7 +104,7 @@ def test_langevin_window(oracle_mod, inter):
 
 def test_active_force(oracle_mod):
     print("case active_force", flush=True)
-    st = packing.lattice_system(16, 0.8, 31)
+    st = relaxed_state(16, 0.8, 31)
     st["pangle"] = np.random.default_rng(1).uniform(-np.pi, np.pi, 16)
     o, g = prepare(oracle_mod, st, "wca", "vertexSmooth", particle="active", dt=0.01)
     o.set_self_propulsion(0.1, 100.0); g.set_self_propulsion(0.1, 100.0)
@@ -177,3 +177,44 @@ def test_observables(oracle_mod):
     g.scale_vertices(1.01); o.scale_vertices(1.01)
     assert rel_err(g.get("pos"), o.get("pos"), scale=1.0) < 1e-13
     assert rel_err(g.get("a0"), o.get("a0")) < 1e-15
+
+
+@pytest.mark.parametrize("inter", INTERS)
+def test_split_invariance(oracle_mod, inter):
+    print(f"case split_invariance {inter}", flush=True)
+    st = packing.random_system(40, 0.8, 55)
+    o, g = prepare(oracle_mod, st, "wca", inter)
+    o.calc_force_energy()
+    ref = o.get("force")
+    for split in (1, 2, 3, 4, 8):
+        g.set_launch_config(split)
+        g.calc_force_energy()
+        assert rel_err(g.get("force"), ref) < TOL_F, split
+
+
+def test_batched_fire_minimize(oracle_mod):
+    print("case batched_fire_minimize", flush=True)
+    systems = [packing.lattice_system(n, 0.8, 300 + n, perturb=0.3) for n in (9, 16, 12, 20)]
+    cat = packing.concat_systems(systems)
+    g = gpu_from_state(cat, potential="harmonic", interaction="vertexVertex", costs=(1, 1, 0.1, 1),
+                       rings_per_system=cat["rings_per_system"])
+    size = 2 * np.mean(cat["rad"])
+    cut = g.set_displacement_cutoff(0.3, size)
+    g.calc_neighbor_list(cut)
+    g.calc_force_energy()
+    dt = 0.01 * 2.0
+    prm = [0.2, 0.5, 1.1, 0.99, dt, 10 * dt, 0.2]
+    g.init_fire(prm, 20, 0)
+    it = g.fire_minimize(1e-10, 100000)
+    fmax = g.max_unbalanced_force()
+    assert np.all(fmax <= 1e-10), fmax
+    U = g.potential_energy()
+    for s, st in enumerate(systems):
+        o = oracle_from_state(oracle_mod, st, potential="harmonic", interaction="vertexVertex", costs=(1, 1, 0.1, 1))
+        o.set_displacement_cutoff(0.3, size)
+        o.calc_neighbor_list(cut)
+        o.calc_force_energy()
+        o.init_fire(prm, 20, 0)
+        ito = o.fire_minimize(1e-10, 100000)
+        assert abs(int(it[s]) - ito) <= max(5, 0.02 * ito), (it[s], ito)
+        assert U[s] == pytest.approx(o.potential_energy(), rel=1e-9)

@@ -69,3 +69,17 @@ def test_ring_across_boundary_and_image_shift(oracle_mod):
     o.set("pos", (base + np.array([L, -2 * L])).ravel())
     o.L.orc_calc_particle_shape(o.h)
     assert o.get("area")[0] == pytest.approx(A0, rel=1e-12)
+
+
+def test_ring_wider_than_half_box(oracle_mod):
+    # the reference walks the ring one minimum-image segment at a time, so a ring wider than
+    # L/2 keeps its true area and perimeter
+    n, R, L = 64, 2.26, 8.9
+    o = oracle_mod.Oracle([n])
+    o.set("box", [L, L])
+    o.set("pos", polygon(n, R, 1.0, 7.5).ravel())
+    o.L.orc_calc_particle_shape(o.h)
+    o.L.orc_calc_particle_positions(o.h)
+    assert o.get("area")[0] == pytest.approx(0.5 * n * R * R * np.sin(2 * np.pi / n), rel=1e-12)
+    assert o.get("perimeter")[0] == pytest.approx(2 * n * R * np.sin(np.pi / n), rel=1e-12)
+    np.testing.assert_allclose(o.get("ppos"), [1.0, 7.5], atol=1e-12)
