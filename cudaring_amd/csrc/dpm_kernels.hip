@@ -180,10 +180,11 @@ __device__ __forceinline__ ShapeOut ring_shape(const View& V, const Params& P, c
 // --------------------------------------------------------- neighbours ---
 __device__ __forceinline__ void cell_of(double x, double y, const Box& B, int ncx, int ncy, int& cx, int& cy) {
   const double wx = x - B.Lx * floor(x * B.iLx), wy = y - B.Ly * floor(y * B.iLy);
-  cx = (int)(wx * B.iLx * ncx);
-  cy = (int)(wy * B.iLy * ncy);
-  cx = cx < 0 ? 0 : (cx >= ncx ? ncx - 1 : cx);
-  cy = cy < 0 ? 0 : (cy >= ncy ? ncy - 1 : cy);
+  // clamp in floating point first (fmax maps NaN to 0), so the conversion is always defined
+  const double fx = fmin(fmax(wx * B.iLx * ncx, 0.0), ncx - 1.0);
+  const double fy = fmin(fmax(wy * B.iLy * ncy, 0.0), ncy - 1.0);
+  cx = (int)fx;
+  cy = (int)fy;
 }
 
 // previous and second previous vertex of u in its ring (getPreviousId 233-243)
