@@ -40,7 +40,7 @@ def alg_bytes_per_step(nv, np_, kbar):
 
 
 def prepare(args):
-    from cudaring_amd import DPMSet, protocols
+    from cudaring_amd import DPMSet, autotune, protocols
     t0 = time.perf_counter()
     s, st = protocols.prepared_state(lambda nvp: DPMSet(nvp), NRINGS, PHI, SEED, costs=COSTS,
                                      final=("wca", "vertexSmooth"), iters=args.prep_iters,
@@ -54,7 +54,8 @@ def prepare(args):
     s.calc_neighbor_list(cut)
     s.calc_force_energy()
     s.init_langevin(T, GAMMA)
-    return s, st, dt, time.perf_counter() - t0
+    waves, wtimes = autotune.tune_waves_per_tile(s)
+    return s, st, dt, time.perf_counter() - t0, {"waves_per_tile": waves, "kernel_us": autotune.summary(wtimes)}
 
 
 def cpu_baseline(st, dt, steps):
@@ -96,7 +97,7 @@ def main():
         sys.exit(2)
 
     import torch  # device plumbing: synchronisation around the timed region
-    s, st, dt, prep_s = prepare(args)
+    s, st, dt, prep_s, tuned = prepare(args)
     nv, np_ = s.nv, s.np
 
     done = 0
@@ -140,7 +141,7 @@ def main():
         "config": {"workload": "NVT Langevin, 1024 bidisperse rings 16/22 (19456 vertices), WCA "
                                "vertex-segment, phi=0.85, T=1e-2, gamma=1, dt=1e-3 sigma, skin 0.5",
                    "vertices": nv, "rings": np_, "dt": dt, "rebuilds_in_timed_region": rebuilds,
-                   "T_end": temp, "prep_s": round(prep_s, 1)},
+                   "T_end": temp, "prep_s": round(prep_s, 1), "autotune": tuned},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -1,0 +1,54 @@
+"""Autotuner for the launch parameters of a system set.
+
+The step kernel runs one workgroup per ring tile; how many wavefronts share a
+tile's neighbour gather (1..8) trades latency hiding against idle lanes and LDS
+traffic, and the best value depends on the neighbour count and on how many
+tiles the set has.  `tune_waves_per_tile` times the fused Langevin step kernel
+for each candidate on the caller's own state (velocities are restored by the
+library after timing) and keeps the fastest.  `tune_skin` picks the neighbour
+skin (setDisplacementCutoff's `cutoff`) that minimises the measured time of a
+short stretch of the real loop, rebuilds included.
+"""
+import time
+
+import numpy as np
+
+
+def tune_waves_per_tile(s, candidates=(1, 2, 4, 8), reps=100):
+    """Time the step kernel for each waves-per-tile value; keep and return the fastest."""
+    times = {}
+    for w in candidates:
+        s.set_launch_config(w)
+        times[w] = min(s.time_step_kernel(reps)[0] for _ in range(2))
+    best = min(times, key=times.get)
+    s.set_launch_config(best)
+    return best, times
+
+
+def tune_skin(s, size, candidates=(0.2, 0.3, 0.5, 0.8), steps=400, kind="langevin"):
+    """Pick the displacement cutoff (skin) minimising the wall time of `steps` loop steps.
+
+    The state is advanced while tuning, so call it on a warm-up stretch of the run."""
+    times = {}
+    run = s.langevin if kind == "langevin" else s.nve
+    for c in candidates:
+        cut = s.set_displacement_cutoff(c, size)
+        s.calc_neighbor_list(cut)
+        run(steps)  # capture the graph for this configuration
+        s.synchronize()
+        t0 = time.perf_counter()
+        run(steps)
+        s.synchronize()
+        times[c] = time.perf_counter() - t0
+    best = min(times, key=times.get)
+    cut = s.set_displacement_cutoff(best, size)
+    s.calc_neighbor_list(cut)
+    return best, times
+
+
+def summary(times):
+    return {str(k): round(float(v), 4) for k, v in sorted(times.items())}
+
+
+def choose(values, times):
+    return values[int(np.argmin([times[v] for v in values]))]

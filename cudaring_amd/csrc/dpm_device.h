@@ -285,4 +285,64 @@ __device__ __forceinline__ double smooth_pair(const Params& P, const Box& B, dou
   return energy;
 }
 
+// The same evaluation for the pair (i, segment (prev(j), j)) returning both the share of j
+// (ROLE_OTHER) and the share of prev(j) (ROLE_PREV) in one pass; the caller owns j, so both
+// receivers lie in its own ring.  Returns the energy e added to both rings.
+template <int POT>
+__device__ __forceinline__ double smooth_pair_op(const Params& P, const Box& B, double tix, double tiy, double tjx,
+                                                 double tjy, double tpx, double tpy, double tqx, double tqy,
+                                                 double radSum, double& ox, double& oy, double& px, double& py) {
+  const double sgx = pbx(tjx - tpx, B), sgy = pby(tjy - tpy, B);
+  const double length = sqrt(sgx * sgx + sgy * sgy);
+  const double ipx = pbx(tix - tpx, B), ipy = pby(tiy - tpy, B);
+  const double proj = (ipx * sgx + ipy * sgy) / (length * length);
+  const double psx = pbx(tqx - tpx, B), psy = pby(tqy - tpy, B);
+  const double previousLength = sqrt(psx * psx + psy * psy);
+  const double previousProj = (pbx(tix - tqx, B) * pbx(tpx - tqx, B) + pby(tiy - tqy, B) * pby(tpy - tqy, B)) /
+                              (previousLength * previousLength);
+  double energy = 0.0;
+  if (proj >= 0 && proj < 1) {
+    const double reduced = fmax(0.0, fmin(1.0, proj));
+    const double qx = tpx + reduced * sgx, qy = tpy + reduced * sgy;
+    const double dqx = pbx(tix - qx, B), dqy = pby(tiy - qy, B);
+    double epot = 0.0;
+    const double gm = grad_multiple<POT>(P, sqrt(dqx * dqx + dqy * dqy), radSum, epot);
+    if (gm != 0) {
+      const double pox = pbx(tpx - tjx, B), poy = pby(tpy - tjy, B);
+      const double cross = pox * pby(tjy - tiy, B) - pbx(tjx - tix, B) * poy;
+      const double absCross = fabs(cross);
+      const double sign = absCross > 0 ? cross / absCross : 0.0;
+      ox += gm * (sign * pby(tiy - tpy, B) + absCross * pox / (length * length)) / length;
+      oy += gm * (sign * pbx(tpx - tix, B) + absCross * poy / (length * length)) / length;
+      px += gm * (sign * pby(tjy - tiy, B) - absCross * pox / (length * length)) / length;
+      py += gm * (sign * pbx(tix - tjx, B) - absCross * poy / (length * length)) / length;
+      energy += epot;
+    }
+    if (previousProj >= 0 && previousProj < 1) {
+      const double dx = pbx(tpx - tix, B), dy = pby(tpy - tiy, B);
+      const double d = sqrt(dx * dx + dy * dy);
+      double ep = 0.0;
+      const double g = grad_multiple<POT>(P, d, radSum, ep);
+      if (g != 0) {
+        px += -g * dx / d;
+        py += -g * dy / d;
+        energy -= ep;
+      }
+    }
+  } else if (proj < 0) {
+    if (previousProj >= 1) {
+      const double dx = pbx(tix - tpx, B), dy = pby(tiy - tpy, B);
+      const double d = sqrt(dx * dx + dy * dy);
+      double ep = 0.0;
+      const double g = grad_multiple<POT>(P, d, radSum, ep);
+      if (g != 0) {
+        px += -g * dx / d;
+        py += -g * dy / d;
+        energy += ep;
+      }
+    }
+  }
+  return energy;
+}
+
 }  // namespace dpm

@@ -2,14 +2,15 @@
 //
 // Work decomposition.  Vertices are cut into ring-aligned tiles of at most 64
 // vertices (whole rings, one system per tile).  A step launches one workgroup per
-// tile with SPLIT wavefronts:
+// tile with nw wavefronts (the autotuned "waves per tile"):
 //   * wave 0, one lane per vertex, does the ring geometry and the shape force:
 //     ring perimeters, areas and centres are segmented wave scans, ring
 //     neighbours (prev/next vertex, their angles and rest lengths) come from lane
 //     shuffles, so the reference's three shape kernels collapse into one pass;
-//   * all SPLIT waves share the interparticle gather over the ELL neighbour list
-//     (entry k goes to wave k mod SPLIT) and reduce their partial forces through
-//     LDS in a fixed order;
+//     it also draws the ring's Philox numbers;
+//   * meanwhile waves 1..nw-1 share the interparticle gather over the ELL
+//     neighbour list and reduce their partial forces through LDS in a fixed
+//     order (with nw = 1 wave 0 does the gather itself);
 //   * wave 0 applies the integrator epilogue (thermostat, kicks, drift,
 //     displacement test).
 // Neighbour rows are rebuilt inside the step kernel when the system's rebuild
@@ -302,9 +303,11 @@ __global__ void k_build_rows(View V, Params P, CellGrid G, const int* count, con
 }
 
 // -------------------------------------------------------------- step ---
+// Waves of a tile: with nw > 1, wave 0 does the ring geometry, shape force and random draws
+// while waves 1..nw-1 share the neighbour gather; with nw == 1 wave 0 does both in turn.
 template <int INTER, int POT, int MODE>
 __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepArgs A, CellGrid G) {
-  __shared__ double red[MAXSPLIT][WAVE][4];
+  __shared__ double red[MAXSPLIT][WAVE][6];
   const int w = threadIdx.x / WAVE;
   const int nw = blockDim.x / WAVE;
   Lane L;
@@ -312,75 +315,90 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
   const Box B = box_of(V, L.s);
 
   // loop bookkeeping: clear the other parity's cells, count this step's rebuild, rebuild rows
-  bool rebuild = false;
   if (A.manage) {
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < G.ncells; c += gridDim.x * blockDim.x)
       A.cellOther[c] = 0;
-    rebuild = A.flagCur[L.s] != 0;
+    const bool rebuild = A.flagCur[L.s] != 0;
     if (rebuild && L.tile == V.stile[L.s] && threadIdx.x == 0) A.updateCount[L.s] += 1;
     if (rebuild && w == 0 && L.act) build_row(V, P, G, A.cellCur, L.v, L.s, B, A.err, A.maxcnt);
     if (rebuild) __syncthreads();
   }
 
   const double x = V.x[L.v], y = V.y[L.v];
-  const double radv = V.rad[L.v];
-  double fx = 0, fy = 0, e = 0, es = 0;
-
-  // ---- interparticle gather, entries k = w, w + nw, ...
-  if constexpr (INTER == INT_VV) {
-    // kernelCalcVertexInteraction, cudaKernel.cuh:690-722
-    const int c = L.act ? V.ncnt[L.v] : 0;
-    for (int k = w; k < c; k += nw) {
-      const int j = V.nbr[(long)k * V.nv + L.v];
-      e += pair_force<POT>(P, pbx(x - V.x[j], B), pby(y - V.y[j], B), radv + V.rad[j], fx, fy);
-    }
-  } else if constexpr (INTER == INT_SMOOTH) {
-    // kernelCalcSmoothInteraction, cudaKernel.cuh:922-975, as a gather over three roles:
-    // v as `this` and as `other` against every u in list(v), and v as `previous` of
-    // next(v) against every u in list(next(v)).
-    // ring neighbours of v (inactive lanes past the tile end use v itself: always in bounds)
-    const int vn = L.act ? L.v0 + L.nl : L.v;
-    const int vp = L.act ? L.v0 + L.pl : L.v;
-    const int vpp = L.act ? L.v0 + ((L.pl > L.segS) ? L.pl - 1 : L.segE) : L.v;
-    const double xn = V.x[vn], yn = V.y[vn];
-    const double xp = V.x[vp], yp = V.y[vp];
-    const double xpp = V.x[vpp], ypp = V.y[vpp];
-    const double radn = V.rad[vn];
-    const int c = L.act ? V.ncnt[L.v] : 0;
-    for (int k = w; k < c; k += nw) {
-      const long idx = (long)k * V.nv + L.v;
-      const int u = V.nbr[idx];
-      const int2 pp = V.nbrp[idx];
-      const double xu = V.x[u], yu = V.y[u], ru = V.rad[u];
-      const double radSum = radv + ru;
-      es += smooth_pair<POT, ROLE_THIS>(P, B, x, y, xu, yu, V.x[pp.x], V.y[pp.x], V.x[pp.y], V.y[pp.y], radSum,
-                                        fx, fy);
-      es += smooth_pair<POT, ROLE_OTHER>(P, B, xu, yu, x, y, xp, yp, xpp, ypp, radSum, fx, fy);
-    }
-    const int cn = L.act ? V.ncnt[vn] : 0;
-    for (int k = w; k < cn; k += nw) {
-      const int u = V.nbr[(long)k * V.nv + vn];
-      smooth_pair<POT, ROLE_PREV>(P, B, V.x[u], V.y[u], xn, yn, x, y, xp, yp, V.rad[u] + radn, fx, fy);
-    }
-  } else {
-    // kernelCalcAllToAllVertexInteraction, cudaKernel.cuh:725-757 (test mode)
-    if (L.act) {
-      const int b0 = V.svoff[L.s], b1 = V.svoff[L.s + 1];
-      for (int j = b0 + w; j < b1; j += nw) {
-        if (V.vring[j] == L.r) continue;
+  // gather partials: force on v, vertex-vertex energy, smooth energy, share for prev(v)
+  double fx = 0, fy = 0, e = 0, es = 0, qx = 0, qy = 0;
+  const int g0 = nw > 1 ? 1 : 0;
+  const int ng = nw - g0;
+  if (w >= g0) {
+    const double radv = V.rad[L.v];
+    if constexpr (INTER == INT_VV) {
+      // kernelCalcVertexInteraction, cudaKernel.cuh:690-722
+      const int c = L.act ? V.ncnt[L.v] : 0;
+      for (int k = w - g0; k < c; k += ng) {
+        const int j = V.nbr[(long)k * V.nv + L.v];
         e += pair_force<POT>(P, pbx(x - V.x[j], B), pby(y - V.y[j], B), radv + V.rad[j], fx, fy);
+      }
+    } else if constexpr (INTER == INT_SMOOTH) {
+      // kernelCalcSmoothInteraction, cudaKernel.cuh:922-975, as a gather: for every u in list(v),
+      // (v, segment of u) gives v's `this` share, and (u, segment (prev(v), v)) gives v's
+      // `other` share and prev(v)'s `previous` share (handed to prev(v)'s lane below).
+      // Ring neighbours of v; inactive lanes past the tile end use v itself (in bounds).
+      const int vp = L.act ? L.v0 + L.pl : L.v;
+      const int vpp = L.act ? L.v0 + ((L.pl > L.segS) ? L.pl - 1 : L.segE) : L.v;
+      const double xp = V.x[vp], yp = V.y[vp];
+      const double xpp = V.x[vpp], ypp = V.y[vpp];
+      const int c = L.act ? V.ncnt[L.v] : 0;
+      for (int k = w - g0; k < c; k += ng) {
+        const long idx = (long)k * V.nv + L.v;
+        const int u = V.nbr[idx];
+        const int2 pp = V.nbrp[idx];
+        const double xu = V.x[u], yu = V.y[u];
+        const double radSum = radv + V.rad[u];
+        es += smooth_pair<POT, ROLE_THIS>(P, B, x, y, xu, yu, V.x[pp.x], V.y[pp.x], V.x[pp.y], V.y[pp.y], radSum,
+                                          fx, fy);
+        es += smooth_pair_op<POT>(P, B, xu, yu, x, y, xp, yp, xpp, ypp, radSum, fx, fy, qx, qy);
+      }
+    } else {
+      // kernelCalcAllToAllVertexInteraction, cudaKernel.cuh:725-757 (test mode)
+      if (L.act) {
+        const int b0 = V.svoff[L.s], b1 = V.svoff[L.s + 1];
+        for (int j = b0 + w - g0; j < b1; j += ng) {
+          if (V.vring[j] == L.r) continue;
+          e += pair_force<POT>(P, pbx(x - V.x[j], B), pby(y - V.y[j], B), radv + V.rad[j], fx, fy);
+        }
       }
     }
   }
+  if (nw > 1 && w > 0) {
+    red[w][L.lane][0] = fx;
+    red[w][L.lane][1] = fy;
+    red[w][L.lane][2] = e;
+    red[w][L.lane][3] = es;
+    red[w][L.lane][4] = qx;
+    red[w][L.lane][5] = qy;
+  }
 
-  // ---- reduce the waves' partials in wave order (deterministic)
-  if (nw > 1) {
-    if (w > 0) {
-      red[w][L.lane][0] = fx;
-      red[w][L.lane][1] = fy;
-      red[w][L.lane][2] = e;
-      red[w][L.lane][3] = es;
+  // ---- wave 0: work that does not need the gather, overlapped with it
+  ShapeOut so{0, 0, 0};
+  double zt0 = 0, zt1 = 0, ang = 0;
+  if (w == 0) {
+    so = ring_shape(V, P, L, B, x, y, true);
+    if constexpr ((MODE & MODE_LANGEVIN) != 0)  // Langevin::updateThermalVel, Simulator.cu:54-77
+      gauss_pair(P.seed, (uint32_t)L.s, TAG_THERMAL, A.ctr[0] - 1, (uint32_t)(L.r - V.sroff[L.s]), zt0, zt1);
+    if (P.particle == PART_ACTIVE) {  // addSelfPropulsion, DPM2D.cu:1995-2035
+      const double amplitude = (P.taup != 0) ? sqrt(2.0 * P.dt / P.taup) : 0.0;
+      double z0, z1;
+      gauss_pair(P.seed, (uint32_t)L.s, TAG_ACTIVE, A.ctr[1] - 1, (uint32_t)(L.r - V.sroff[L.s]), z0, z1);
+      ang = V.ang[L.r];
+      ang += amplitude * z0;
+      ang = ang + kPI;
+      ang = ang - 2.0 * kPI * floor(ang / (2.0 * kPI));
+      ang = ang - kPI;
+      // the store depends on every lane's load of the old angle, which has returned
+      if (L.act && L.lane == L.segS) V.ang[L.r] = ang;
     }
+  }
+  if (nw > 1) {
     __syncthreads();
     if (w > 0) return;
     for (int q = 1; q < nw; q++) {
@@ -388,35 +406,27 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
       fy += red[q][L.lane][1];
       e += red[q][L.lane][2];
       es += red[q][L.lane][3];
+      qx += red[q][L.lane][4];
+      qy += red[q][L.lane][5];
     }
   }
 
-  // ---- wave 0: shape force first, then the interparticle sum (the reference's order)
-  const ShapeOut so = ring_shape(V, P, L, B, x, y, true);
-  fx = so.fx + fx;
-  fy = so.fy + fy;
-  e = so.e + e;
+  // ---- wave 0: assemble F = shape + interparticle (+ shares from the next lane)
   if constexpr (INTER == INT_SMOOTH) {
+    fx += shfl_d(qx, L.nl);
+    fy += shfl_d(qy, L.nl);
     const double pe = seg_sum(L.act ? es : 0.0, L.lane, L.segS, L.segE);
     if (L.act) {
       V.es[L.v] = es;
       if (L.lane == L.segS) V.pe[L.r] = pe;
     }
   }
-  // active self propulsion, addSelfPropulsion DPM2D.cu:1995-2035
+  fx = so.fx + fx;
+  fy = so.fy + fy;
+  e = so.e + e;
   if (P.particle == PART_ACTIVE) {
-    const double amplitude = (P.taup != 0) ? sqrt(2.0 * P.dt / P.taup) : 0.0;
-    double z0, z1;
-    gauss_pair(P.seed, (uint32_t)L.s, TAG_ACTIVE, A.ctr[1] - 1, (uint32_t)(L.r - V.sroff[L.s]), z0, z1);
-    double a = V.ang[L.r];
-    a += amplitude * z0;
-    a = a + kPI;
-    a = a - 2.0 * kPI * floor(a / (2.0 * kPI));
-    a = a - kPI;
-    fx += P.driving * ((1 - 0) * cos(a) + 0 * sin(a));
-    fy += P.driving * ((1 - 1) * cos(a) + 1 * sin(a));
-    // the store depends on every lane's load of the old angle, which has returned
-    if (L.act && L.lane == L.segS) V.ang[L.r] = a;
+    fx += P.driving * ((1 - 0) * cos(ang) + 0 * sin(ang));
+    fy += P.driving * ((1 - 1) * cos(ang) + 1 * sin(ang));
   }
 
   double vx = 0, vy = 0;
@@ -425,11 +435,8 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
     vy = V.vy[L.v];
   }
   if constexpr ((MODE & MODE_LANGEVIN) != 0) {
-    // Langevin::updateThermalVel, Simulator.cu:54-77: one Gaussian pair per ring
-    double z0, z1;
-    gauss_pair(P.seed, (uint32_t)L.s, TAG_THERMAL, A.ctr[0] - 1, (uint32_t)(L.r - V.sroff[L.s]), z0, z1);
-    fx += P.noise * z0 - P.gamma * vx;
-    fy += P.noise * z1 - P.gamma * vy;
+    fx += P.noise * zt0 - P.gamma * vx;
+    fy += P.noise * zt1 - P.gamma * vy;
   }
   if constexpr ((MODE & MODE_FIRE) != 0) {
     const FireState fs = A.fire[L.s];
