@@ -82,14 +82,35 @@ def cpu_baseline(st, dt, steps):
                       f"(incl. its all-pairs neighbour rebuilds), one thread, {el:.1f} s"}
 
 
+COPY_ELEMS = 1 << 27  # float64 elements of the calibration copy: 1 GiB read + 1 GiB written
+
+
+def traffic_run(s, torch):
+    """Launch sequence for `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE`: three copies of
+    COPY_ELEMS doubles (the calibration kernel: 8 * COPY_ELEMS bytes read and written each), then
+    50 back-to-back launches of the step kernel on the prepared state."""
+    src = torch.rand(COPY_ELEMS, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    us, kbar = s.time_step_kernel(50)
+    s.synchronize()
+    print(json.dumps({"traffic_run": True, "copy_bytes_each_way": 8 * COPY_ELEMS, "step_launches": 50,
+                      "kernel_us": us, "mean_neighbors": kbar,
+                      "alg_bytes_per_launch": alg_bytes_per_step(s.nv, s.np, kbar)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=2000)
     ap.add_argument("--prep-iters", type=int, default=1000)
-    ap.add_argument("--cpu-steps", type=int, default=300)
+    ap.add_argument("--cpu-steps", type=int, default=1200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic", action="store_true",
+                    help="counter-collection run: a copy of known size, then step-kernel launches only")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the path is single-GPU (systems shard across GPUs "
@@ -99,6 +120,9 @@ def main():
     import torch  # device plumbing: synchronisation around the timed region
     s, st, dt, prep_s, tuned = prepare(args)
     nv, np_ = s.nv, s.np
+    if args.traffic:
+        traffic_run(s, torch)
+        return
 
     done = 0
     while done < args.warmup:

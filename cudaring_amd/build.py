@@ -19,6 +19,16 @@ LIB = os.path.join(HERE, "libdpmring.so")
 
 SOURCES = ["dpm_kernels.hip", "dpm_api.hip"]
 
+# C++ class layer (DPM2D / FIRE / Simulator / ioDPMFile) and the drivers built on it: plain host
+# C++ over the C ABI, linked against libdpmring.so (found next to bin/ through the rpath)
+HOST = os.path.join(HERE, "host")
+DRIVERS = os.path.join(HERE, "drivers")
+BIN = os.path.join(HERE, "bin")
+HOST_SOURCES = ["DPM2D.cpp", "FIRE.cpp", "Simulator.cpp"]
+DRIVER_SOURCES = ["compressFIRE.cpp", "runDPM.cpp"]
+CXX = os.environ.get("CXX", "g++")
+CXXFLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-variable"]
+
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
@@ -34,6 +44,35 @@ def _compile(src):
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
     return obj
+
+
+def _cxx(src, obj):
+    r = subprocess.run([CXX, *CXXFLAGS, "-c", src, "-o", obj], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{CXX} failed for {src}:\n{r.stdout}\n{r.stderr}")
+    return obj
+
+
+def build_host(verbose=False):
+    """Compile the class layer and link one executable per driver into bin/."""
+    os.makedirs(BUILD, exist_ok=True)
+    os.makedirs(BIN, exist_ok=True)
+    jobs = [(os.path.join(HOST, s), os.path.join(BUILD, "host_" + os.path.splitext(s)[0] + ".o"))
+            for s in HOST_SOURCES]
+    jobs += [(os.path.join(DRIVERS, s), os.path.join(BUILD, "drv_" + os.path.splitext(s)[0] + ".o"))
+             for s in DRIVER_SOURCES]
+    with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+        objs = list(ex.map(lambda j: _cxx(*j), jobs))
+    host_objs = objs[:len(HOST_SOURCES)]
+    for src, obj in zip(DRIVER_SOURCES, objs[len(HOST_SOURCES):]):
+        exe = os.path.join(BIN, os.path.splitext(src)[0])
+        cmd = [CXX, "-o", exe, obj, *host_objs, f"-L{HERE}", "-ldpmring", "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"link failed for {src}:\n{r.stdout}\n{r.stderr}")
+        if verbose:
+            print(f"built {exe}")
+    return BIN
 
 
 def build(verbose=False):
@@ -53,4 +92,5 @@ def build(verbose=False):
 
 if __name__ == "__main__":
     build(verbose=True)
+    build_host(verbose=True)
     sys.exit(0)

@@ -815,6 +815,48 @@ int dpm_calc_force_energy(dpm_set_t h) {
   return DPM_OK;
 }
 
+int dpm_get_interaction_forces(dpm_set_t h, double* host, long n) {
+  CHECK_HANDLE(h);
+  if (!host || n != 2L * S->nv) return fail(S, DPM_ERR_ARG, "interaction forces: n must be 2 * numVertices");
+  int rc = require_list(S);
+  if (rc) return rc;
+  const size_t nv = S->nv, np = S->np;
+  // keep the stored forces and energies: this call must not change the state
+  std::vector<double> fx(nv), fy(nv), ev(nv), pe(np);
+  HIPCHK(hipMemcpyAsync(fx.data(), S->V.fx, nv * sizeof(double), hipMemcpyDeviceToHost, S->stream));
+  HIPCHK(hipMemcpyAsync(fy.data(), S->V.fy, nv * sizeof(double), hipMemcpyDeviceToHost, S->stream));
+  HIPCHK(hipMemcpyAsync(ev.data(), S->V.ev, nv * sizeof(double), hipMemcpyDeviceToHost, S->stream));
+  HIPCHK(hipMemcpyAsync(pe.data(), S->V.pe, np * sizeof(double), hipMemcpyDeviceToHost, S->stream));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  // the same step kernel with the shape costs and self-propulsion switched off
+  const Params keep = S->P;
+  S->P.ea = S->P.el = S->P.eb = 0;
+  S->P.particle = PART_PASSIVE;
+  HIPCHK(launch_fill_int(S->d_any + S->cur, 0, 1, S->stream));
+  HIPCHK(launch_bin(S, 0, 0));
+  StepArgs A{};
+  A.ctr = S->d_ctr;
+  A.err = S->d_err;
+  A.manage = 0;
+  hipError_t e = launch_step(S, MODE_FORCE, A);
+  S->P = keep;
+  HIPCHK(e);
+  std::vector<double> ix(nv), iy(nv);
+  HIPCHK(hipMemcpyAsync(ix.data(), S->V.fx, nv * sizeof(double), hipMemcpyDeviceToHost, S->stream));
+  HIPCHK(hipMemcpyAsync(iy.data(), S->V.fy, nv * sizeof(double), hipMemcpyDeviceToHost, S->stream));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  for (size_t v = 0; v < nv; v++) {
+    host[2 * v] = ix[v];
+    host[2 * v + 1] = iy[v];
+  }
+  HIPCHK(hipMemcpyAsync(S->V.fx, fx.data(), nv * sizeof(double), hipMemcpyHostToDevice, S->stream));
+  HIPCHK(hipMemcpyAsync(S->V.fy, fy.data(), nv * sizeof(double), hipMemcpyHostToDevice, S->stream));
+  HIPCHK(hipMemcpyAsync(S->V.ev, ev.data(), nv * sizeof(double), hipMemcpyHostToDevice, S->stream));
+  HIPCHK(hipMemcpyAsync(S->V.pe, pe.data(), np * sizeof(double), hipMemcpyHostToDevice, S->stream));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
 int dpm_init_langevin(dpm_set_t h, double T, double gamma, int readState, int zeroMomentum) {
   CHECK_HANDLE(h);
   S->simKind = 1;

@@ -52,6 +52,7 @@ SIGNATURES = {
     "dpm_reset_update_count": (_i, [_vp]),
     "dpm_calc_particle_shape": (_i, [_vp]),
     "dpm_calc_force_energy": (_i, [_vp]),
+    "dpm_get_interaction_forces": (_i, [_vp, _vp, _l]),
     "dpm_init_langevin": (_i, [_vp, _d, _d, _i, _i]),
     "dpm_langevin_loop": (_i, [_vp, _l]),
     "dpm_init_nve": (_i, [_vp, _d, _i, _i]),

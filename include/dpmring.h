@@ -128,6 +128,11 @@ int dpm_reset_update_count(dpm_set_t h);
 int dpm_calc_particle_shape(dpm_set_t h);
 /* calcForceEnergy(), DPM2D.h:547 / DPM2D.cu:1962-2104 */
 int dpm_calc_force_energy(dpm_set_t h);
+/* getInteractionForces (DPM2D.h:575; written to forces.dat by savePacking, FileIO.h:424): the
+ * forces of the vertex interaction alone at the current positions, AoS 2V.  Stored forces and
+ * energies are left as they were.  (The reference's vertexVertex branch writes per-vertex
+ * energies past the end of a per-ring array, DPM2D.cu:2221-2251; this returns the forces it means.) */
+int dpm_get_interaction_forces(dpm_set_t h, double* host, long n);
 
 /* ---- integrators ---------------------------------------------------------- */
 /* initLangevin(T, gamma, readState) / langevinLoop(), DPM2D.h:680-682.  zeroMomentum

@@ -1,0 +1,90 @@
+"""The C++ class layer and drivers end to end on the GPU, checked against the oracle.
+
+compressFIRE makes and compresses a small packing (soft disks on the host, then vertex FIRE on
+the device) and saves it with ioDPMFile; runDPM reads it back and runs Langevin NVT.  The oracle
+loads the same files and must reproduce the saved forces and the NVT trajectory.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+BIN = os.path.join(ROOT, "cudaring_amd", "bin")
+
+
+def load(path, cols=1):
+    a = np.loadtxt(path, ndmin=1)
+    return a.reshape(-1) if cols == 1 else a.reshape(-1, cols)
+
+
+def state_from_dir(d):
+    nvp = load(os.path.join(d, "numVertexInParticleList.dat")).astype(np.int64)
+    return {"nvp": nvp, "pos": load(os.path.join(d, "positions.dat"), 2).ravel(),
+            "rad": load(os.path.join(d, "radii.dat")), "l0": load(os.path.join(d, "restLengths.dat")),
+            "theta0": load(os.path.join(d, "restAngles.dat")), "a0": load(os.path.join(d, "restAreas.dat")),
+            "box": load(os.path.join(d, "boxSize.dat"))}
+
+
+def run(args, timeout=110):
+    print("running", " ".join(args), flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    return r.stdout
+
+
+@pytest.fixture(scope="module")
+def packing_dir(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("cf"))
+    run([os.path.join(BIN, "compressFIRE"), out, "16", "16", "1.1", "0.1", "0.4", "5", "0.05"])
+    return os.path.join(out, "dp")
+
+
+def test_compress_fire_packing(packing_dir, oracle_mod):
+    params = dict(ln.split("\t") for ln in open(os.path.join(packing_dir, "params.dat")).read().splitlines())
+    assert abs(float(params["phi"]) - 0.4) < 0.01
+    st = state_from_dir(packing_dir)
+    o = oracle_mod.Oracle(st["nvp"])
+    o.load_state(st)
+    o.set_types("harmonic", "vertexVertex")
+    o.set_energy_costs(1e3, 1, 0, 1)
+    o.calc_neighbor_list(1.0)
+    o.calc_force_energy()
+    assert o.max_unbalanced_force() < 1e-7  # FIRE stopped at 1e-10 / sigma; files keep 14 digits
+    # forces.dat holds the interaction forces alone
+    o.set_energy_costs(0, 0, 0, 1)
+    o.calc_force_energy()
+    fint = load(os.path.join(packing_dir, "forces.dat"), 2).ravel()
+    np.testing.assert_allclose(fint, o.get("force"), rtol=0, atol=1e-9)
+    shapes = load(os.path.join(packing_dir, "shapes.dat"), 3)
+    assert shapes.shape == (16, 3) and np.all(shapes[:, 2] >= 1.0)
+
+
+def test_run_nvt_matches_oracle(packing_dir, tmp_path, oracle_mod):
+    out = str(tmp_path / "nvt")
+    steps, dt_units, T, damping, seed = 200, 2e-4, 1e-3, 1.0, 9
+    run([os.path.join(BIN, "runDPM"), packing_dir, out, "nvt", str(dt_units), str(T), str(damping),
+         str(steps), "16", str(seed)])
+    st = state_from_dir(packing_dir)
+    o = oracle_mod.Oracle(st["nvp"])
+    o.load_state(st)
+    o.set_types("harmonic", "vertexVertex")
+    o.set_energy_costs(1e5, 20, 10, 1)
+    sigma = 2 * np.sqrt(np.mean(st["a0"]) / np.pi)
+    o.set_dt(dt_units * sigma)
+    o.set_rng(seed, 0)
+    cut = o.set_displacement_cutoff(0.5, 2 * np.mean(st["rad"]))
+    o.calc_neighbor_list(cut)
+    o.calc_force_energy()
+    o.init_langevin(T, damping / sigma)
+    o.langevin(steps)
+    pos = load(os.path.join(out, "positions.dat"), 2).ravel()
+    np.testing.assert_allclose(pos, o.get("pos"), rtol=0, atol=1e-10)
+    vel = load(os.path.join(out, "velocities.dat"), 2).ravel()
+    np.testing.assert_allclose(vel, o.get("vel"), rtol=0, atol=1e-9)
+    e = load(os.path.join(out, "energy.dat"), 8)
+    assert e.shape[0] == 100 and int(e[-1, 0]) == steps
+    np.testing.assert_allclose(e[-1, 2], o.potential_energy() / len(st["rad"]), rtol=1e-9)
