@@ -82,6 +82,184 @@ def cpu_baseline(st, dt, steps):
                       f"(incl. its all-pairs neighbour rebuilds), one thread, {el:.1f} s"}
 
 
+FIRE_SYSTEMS, FIRE_SEED = 512, 20261202 + 3
+FIRE_PARAMS = (0.1, 0.5, 1.1, 0.99, 0.01, 0.1, 0.1)  # a_start, f_dec, f_inc, f_a, dt0, dt_max, a
+FIRE_TOL, FIRE_DPHI, FIRE_MAXIT = 1e-12, 1e-3, 200000
+
+
+def fire_systems():
+    """512 dilute packings at preferred phi 0.5: N ~ U{16..256} bidisperse rings, calA0 ~ U[1.0, 1.2]
+    per packing, centres on a jittered lattice (random, no ring crossings)."""
+    from cudaring_amd import packing
+    rng = np.random.default_rng(FIRE_SEED)
+    sizes = rng.integers(16, 257, FIRE_SYSTEMS)
+    calA0 = rng.uniform(1.0, 1.2, FIRE_SYSTEMS)
+    return [packing.lattice_system(int(n), 0.5, FIRE_SEED + i, calA0=float(c), perturb=0.15)
+            for i, (n, c) in enumerate(zip(sizes, calA0))]
+
+
+def fire_setup(s, size):
+    """harmonic vertex-vertex, cfg1 costs, skin 0.3 of `size` (the same for every path)."""
+    s.set_types("harmonic", "vertexVertex")
+    s.set_energy_costs(*COSTS)
+    return s.set_displacement_cutoff(0.3, size)
+
+
+def fire_step(s, cut, plain):
+    """One compression step of every system in `s` (compressFIRE.cpp:104-131, 153): grow by
+    sqrt((phi + dphi) / phi), rebuild, FIRE to max|F_c| <= tol.  Returns iterations per system."""
+    phi = s.phi()
+    s.scale_vertices(np.sqrt((phi + FIRE_DPHI) / phi))
+    s.calc_neighbor_list(cut)
+    s.calc_force_energy()
+    s.reset_last_positions()
+    s.init_fire(FIRE_PARAMS, 20, 0)
+    if not plain:
+        return s.fire_minimize(FIRE_TOL, FIRE_MAXIT)
+    it, f = 0, s.max_unbalanced_force()[0]
+    while f > FIRE_TOL and it < FIRE_MAXIT:  # the driver loop: one iteration, one host check
+        s.fire(1)
+        f = s.max_unbalanced_force()[0]
+        it += 1
+    return np.array([it])
+
+
+def batched_fire(args):
+    """cfg3: packings/s of the batched set against the plain one-system path."""
+    from cudaring_amd import DPMSet, packing
+    systems = fire_systems()
+    st = packing.concat_systems(systems)
+    s = DPMSet(st["nvp"], rings_per_system=st["rings_per_system"])
+    s.load_state(st)
+    size = 2 * float(np.mean(st["rad"]))
+    cut = fire_setup(s, size)
+    s.calc_neighbor_list(cut)
+    s.calc_force_energy()
+    s.init_fire(FIRE_PARAMS, 20, 0)
+    t0 = time.perf_counter()
+    it0 = s.fire_minimize(FIRE_TOL, FIRE_MAXIT)  # relax the dilute start (untimed)
+    relax_s = time.perf_counter() - t0
+    start = {k: s.get(k) for k in ("pos", "rad", "l0", "a0")}
+    nv_sys = np.array([int(np.sum(x["nvp"])) for x in systems])
+
+    s.synchronize()
+    t0 = time.perf_counter()
+    iters = np.zeros(FIRE_SYSTEMS, dtype=np.int64)
+    for _ in range(args.fire_steps):
+        iters += fire_step(s, cut, plain=False)
+    s.synchronize()
+    t_batch = time.perf_counter() - t0
+    done = FIRE_SYSTEMS * args.fire_steps
+    converged = int(np.sum(s.max_unbalanced_force() <= FIRE_TOL))
+
+    # plain path on a seeded sample of the same systems from the same relaxed start
+    rng = np.random.default_rng(FIRE_SEED + 1)
+    pick = np.sort(rng.choice(FIRE_SYSTEMS, args.fire_plain, replace=False))
+    voff = np.concatenate([[0], np.cumsum(nv_sys)])
+    roff = np.concatenate([[0], np.cumsum(st["rings_per_system"])])
+    t_plain, plain_iters = 0.0, 0
+    for i in pick:
+        sub = dict(systems[i])
+        sl, rl = slice(voff[i], voff[i + 1]), slice(roff[i], roff[i + 1])
+        sub["pos"] = start["pos"][2 * voff[i]:2 * voff[i + 1]]
+        sub["rad"], sub["l0"], sub["a0"] = start["rad"][sl], start["l0"][sl], start["a0"][rl]
+        p = DPMSet(sub["nvp"])
+        p.load_state(sub)
+        pcut = fire_setup(p, size)
+        p.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.fire_steps):
+            plain_iters += int(fire_step(p, pcut, plain=True)[0])
+        p.synchronize()
+        t_plain += time.perf_counter() - t0
+        p.close()
+    plain_rate = args.fire_plain * args.fire_steps / t_plain
+    batch_rate = done / t_batch
+    vit = float(np.sum(iters * nv_sys))
+    s.close()
+    return {"packings_per_s": round(batch_rate, 2), "plain_packings_per_s": round(plain_rate, 3),
+            "speedup_vs_plain": round(batch_rate / plain_rate, 2),
+            "systems": FIRE_SYSTEMS, "vertices": int(nv_sys.sum()), "phi_steps": args.fire_steps,
+            "dphi": FIRE_DPHI, "tol": FIRE_TOL, "converged_systems": converged,
+            "batched_s": round(t_batch, 3), "iterations_mean": round(float(iters.mean()), 1),
+            "iterations_max": int(iters.max()), "vertex_fire_iterations_per_s": round(vit / t_batch, 1),
+            "plain_sample": f"{args.fire_plain} of the {FIRE_SYSTEMS} systems, same start and phi steps, "
+                            f"one DPM2D-style set each, host check per iteration: {plain_iters} iterations "
+                            f"in {t_plain:.2f} s",
+            "start_relax_iterations_max": int(it0.max()), "start_relax_s": round(relax_s, 2)}
+
+
+ADJ_RINGS, ADJ_PHI, ADJ_SEED, ADJ_T = 256, 0.85, 20261202 + 4, 1e-3
+
+
+def adjoint_bench(args, torch):
+    """cfg4: 256 rings at phi 0.85, harmonic vertex-vertex, NVE at dt = 0.01 sigma; the reverse pass
+    (gradients of a linear functional of (x_N, v_N) with respect to x_0, v_0, per-ring ka, kl, kb
+    and l0) against the checkpointed forward pass and against plain NVE steps."""
+    from cudaring_amd import DPMSet, protocols
+    s, st = protocols.prepared_state(lambda nvp: DPMSet(nvp), ADJ_RINGS, ADJ_PHI, ADJ_SEED, costs=COSTS,
+                                     iters=args.prep_iters)
+    sigma = float(2 * np.sqrt(np.mean(s.get("a0")) / np.pi))
+    s.set_dt(0.01 * sigma)
+    s.set_seed(ADJ_SEED)
+    nr = s.np
+    s.set_ring_costs(np.full(nr, COSTS[0]), np.full(nr, COSTS[1]), np.full(nr, COSTS[2]))
+    size = 2 * float(np.mean(s.get("rad")))
+    cut = s.set_displacement_cutoff(0.3, size)
+    s.calc_neighbor_list(cut)
+    s.calc_force_energy()
+    s.init_nve(ADJ_T, zero_momentum=True)
+    x0, v0 = s.get("pos"), s.get("vel")
+    steps, every = args.adj_steps, 100
+
+    def restart():
+        s.set("pos", x0)
+        s.set("vel", v0)
+        s.calc_neighbor_list(cut)
+        s.calc_force_energy()
+        s.synchronize()
+
+    restart()
+    s.nve(steps)  # capture the graphs of both paths once
+    restart()
+    s.nve_forward(steps, every)
+    restart()
+    t0 = time.perf_counter()
+    s.nve(steps)
+    s.synchronize()
+    t_plain = time.perf_counter() - t0
+    restart()
+    t0 = time.perf_counter()
+    s.nve_forward(steps, every)
+    s.synchronize()
+    t_fwd = time.perf_counter() - t0
+    rng = np.random.default_rng(ADJ_SEED)
+    nv = s.nv
+    ax = torch.as_tensor(rng.standard_normal(2 * nv), device="cuda")
+    av = torch.as_tensor(rng.standard_normal(2 * nv), device="cuda")
+    outs = [torch.empty(2 * nv, dtype=torch.float64, device="cuda") for _ in range(2)]
+    gring = torch.empty(3 * nr, dtype=torch.float64, device="cuda")
+    gl0 = torch.empty(nv, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    s.nve_backward(ax.data_ptr(), av.data_ptr(), outs[0].data_ptr(), outs[1].data_ptr(), gring.data_ptr(),
+                   gl0.data_ptr())  # first call captures the reverse-pass graphs
+    restart()
+    s.nve_forward(steps, every)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.nve_backward(ax.data_ptr(), av.data_ptr(), outs[0].data_ptr(), outs[1].data_ptr(), gring.data_ptr(),
+                   gl0.data_ptr())
+    s.synchronize()
+    t_bwd = time.perf_counter() - t0
+    finite = bool(torch.isfinite(outs[0]).all() and torch.isfinite(gring).all())
+    s.close()
+    return {"adjoint_over_forward": round(t_bwd / t_fwd, 3), "adjoint_over_plain_nve": round(t_bwd / t_plain, 3),
+            "forward_s": round(t_fwd, 4), "backward_s": round(t_bwd, 4), "plain_nve_s": round(t_plain, 4),
+            "rings": nr, "vertices": nv, "steps": steps, "checkpoint_every": every, "finite": finite,
+            "workload": f"NVE harmonic vertexVertex, {nr} rings 16/22 at phi {ADJ_PHI}, dt 0.01 sigma, T {ADJ_T}; "
+                        "gradients wrt x0, v0, per-ring ka/kl/kb and l0"}
+
+
 COPY_ELEMS = 1 << 27  # float64 elements of the calibration copy: 1 GiB read + 1 GiB written
 
 
@@ -109,6 +287,13 @@ def main():
     ap.add_argument("--prep-iters", type=int, default=1000)
     ap.add_argument("--cpu-steps", type=int, default=1200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fire-steps", type=int, default=5, help="phi steps of the batched FIRE compression")
+    ap.add_argument("--fire-plain", type=int, default=8, help="systems timed on the plain one-system path")
+    ap.add_argument("--no-fire", action="store_true")
+    ap.add_argument("--adj-steps", type=int, default=2000)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--fire-only", action="store_true")
+    ap.add_argument("--adjoint-only", action="store_true")
     ap.add_argument("--traffic", action="store_true",
                     help="counter-collection run: a copy of known size, then step-kernel launches only")
     args = ap.parse_args()
@@ -118,6 +303,12 @@ def main():
         sys.exit(2)
 
     import torch  # device plumbing: synchronisation around the timed region
+    if args.fire_only:
+        print(json.dumps({"batched_fire": batched_fire(args)}), flush=True)
+        return
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_bench(args, torch)}), flush=True)
+        return
     s, st, dt, prep_s, tuned = prepare(args)
     nv, np_ = s.nv, s.np
     if args.traffic:
@@ -155,9 +346,12 @@ def main():
             "kernel": "k_step<vertexSmooth,WCA,kick|langevin|start>", "kernel_us": round(us, 3),
             "alg_bytes_per_launch": int(alg), "mean_neighbors": round(kbar, 3),
             "step_us": round(1e6 * el / args.steps, 3)}
+    s.close()
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(st, dt, args.cpu_steps)
+    fire = None if args.no_fire else batched_fire(args)
+    adjoint = None if args.no_adjoint else adjoint_bench(args, torch)
     out = {
         "metric": METRIC, "value": value, "unit": "vertex-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True,
@@ -168,6 +362,8 @@ def main():
                    "T_end": temp, "prep_s": round(prep_s, 1), "autotune": tuned},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batched_fire": fire,
+        "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
 

@@ -246,13 +246,15 @@ int require_list(dpm_set_s* S) {
 
 // One loop step's launches at position parity cur: bin (normally idle) + step.  The FIRE step
 // also needs the per-system FIRE state and the tile partial buffer.
-int loop_step(dpm_set_s* S, int mode, int bumpT) {
+int loop_step(dpm_set_s* S, int mode, int bumpT, double* recx = nullptr, double* recy = nullptr) {
   HIPCHK(launch_bin(S, bumpT, S->P.particle == PART_ACTIVE ? 1 : 0));
   if (mode == MODE_FIRE) {
     HIPCHK(launch_fire_step(S));
     return DPM_OK;
   }
   StepArgs A = loop_args(S);
+  A.recx = recx;
+  A.recy = recy;
   HIPCHK(launch_step(S, mode, A));
   return DPM_OK;
 }
@@ -262,16 +264,20 @@ int loop_step(dpm_set_s* S, int mode, int bumpT) {
 //   per step: bin, step [rebuild rows if flagged; F(x'); thermostat; v += dt/2 F;
 //             unless last: v += dt/2 F, x'' = x' + dt v, displacement test]
 // The fused form performs the reference's operations in the reference's order.
-int verlet_sequence(dpm_set_s* S, long steps, bool langevin) {
+// With `rec` (slots of 2 nv doubles, x then y), the position after step i is also written to
+// slot i + 1 (the adjoint's recomputed segment).
+int verlet_sequence(dpm_set_s* S, long steps, bool langevin, double* rec = nullptr) {
   const double h = 0.5 * S->P.dt;
+  const long nv = S->nv;
   HIPCHK(launch_fill_int(S->d_flag[1 - S->cur], 0, S->nsys, S->stream));
   HIPCHK(launch_fill_int(S->d_any + 1 - S->cur, 0, 1, S->stream));
-  HIPCHK(launch_start(S, h, S->P.dt));
+  HIPCHK(launch_start(S, h, S->P.dt, rec ? rec + 2 * nv : nullptr, rec ? rec + 3 * nv : nullptr));
   swap_pos(S);
   for (long i = 0; i < steps; i++) {
     const bool more = i + 1 < steps;
     const int mode = MODE_KICK | (langevin ? MODE_LANGEVIN : 0) | (more ? MODE_START : 0);
-    int rc = loop_step(S, mode, langevin ? 1 : 0);
+    double* rx = (rec && more) ? rec + (i + 2) * 2 * nv : nullptr;
+    int rc = loop_step(S, mode, langevin ? 1 : 0, rx, rx ? rx + nv : nullptr);
     if (rc) return rc;
     if (more) swap_pos(S);
   }
@@ -372,6 +378,137 @@ int temperature_scale(dpm_set_s* S, double T) {
   HIPCHK(hipMemcpyAsync(S->d_scale, sc.data(), sizeof(double) * S->nsys, hipMemcpyHostToDevice, S->stream));
   HIPCHK(launch_scale_vel(S));
   HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
+// ------------------------------------------------------------------ adjoint ---
+// Flags, any-flags and both parities' cell counts cleared: the state the forward loops and the
+// adjoint's own rebuild protocol both start from.
+int clean_protocol(dpm_set_s* S) {
+  HIPCHK(launch_fill_int(S->d_flag[0], 0, S->nsys, S->stream));
+  HIPCHK(launch_fill_int(S->d_flag[1], 0, S->nsys, S->stream));
+  HIPCHK(launch_fill_int(S->d_any, 0, 2, S->stream));
+  HIPCHK(launch_fill_int(S->G.count[0], 0, S->G.ncells, S->stream));
+  HIPCHK(launch_fill_int(S->G.count[1], 0, S->G.ncells, S->stream));
+  return DPM_OK;
+}
+
+int adjoint_supported(dpm_set_s* S) {
+  if (S->P.interaction != INT_VV)
+    return fail(S, DPM_ERR_UNSUPPORTED, "adjoint: vertexVertex interaction only");
+  if (S->P.monomer != MONO_HARMONIC) return fail(S, DPM_ERR_UNSUPPORTED, "adjoint: harmonic monomers only");
+  if (S->P.particle != PART_PASSIVE) return fail(S, DPM_ERR_UNSUPPORTED, "adjoint: passive particles only");
+  return require_list(S);
+}
+
+// work buffers: w0, w1, ax (2nv each), partials g (4nv), ring area and dA (np each), ring sums (3np)
+int adj_alloc(dpm_set_s* S) {
+  if (S->d_adj && S->adjNv == S->nv) return DPM_OK;
+  HIPCHK(hipStreamSynchronize(S->stream));
+  hipFree(S->d_adj);
+  S->d_adj = nullptr;
+  HIPCHK(dalloc(&S->d_adj, 10 * S->nv + 5 * S->np));
+  S->adjNv = S->nv;
+  drop_graphs(S);  // captured sequences hold the old addresses
+  return DPM_OK;
+}
+double* adj_w(dpm_set_s* S, int i) { return S->d_adj + (long)i * 2 * S->nv; }
+double* adj_ax(dpm_set_s* S) { return S->d_adj + 4 * S->nv; }
+double* adj_g(dpm_set_s* S) { return S->d_adj + 6 * S->nv; }
+double* adj_A(dpm_set_s* S) { return S->d_adj + 10 * S->nv; }
+double* adj_dA(dpm_set_s* S) { return S->d_adj + 10 * S->nv + S->np; }
+double* adj_ring(dpm_set_s* S) { return S->d_adj + 10 * S->nv + 2 * S->np; }
+
+View view_at(const dpm_set_s* S, double* x, double* y) {
+  View V = S->V;
+  V.x = x;
+  V.y = y;
+  return V;
+}
+
+// one H w evaluation at the positions of Vn with the rebuild protocol of parity p
+int hvp_launches(dpm_set_s* S, const View& Vn, int p, AdjArgs A) {
+  HIPCHK(launch_adj_flag(S, Vn, p));
+  HIPCHK(launch_bin_view(S, Vn, p));
+  HIPCHK(launch_adj_area(S, Vn, A.wx, A.wy, adj_A(S), adj_dA(S)));
+  A.flag = S->d_flag[p];
+  A.count = S->G.count[p];
+  A.A = adj_A(S);
+  A.dA = adj_dA(S);
+  HIPCHK(launch_adj_hvp(S, Vn, A));
+  return DPM_OK;
+}
+
+// state copies: x, y (current buffer), vx, vy, fx, fy <-> 6 nv doubles at `dst` / `src`
+int save_state(dpm_set_s* S, double* dst) {
+  const long nv = S->nv;
+  const double* src[6] = {S->V.x, S->V.y, S->V.vx, S->V.vy, S->V.fx, S->V.fy};
+  for (int q = 0; q < 6; q++) HIPCHK(launch_copy_d(dst + q * nv, src[q], nv, S->stream));
+  return DPM_OK;
+}
+int load_state(dpm_set_s* S, const double* src) {
+  const long nv = S->nv;
+  double* dst[6] = {S->V.x, S->V.y, S->V.vx, S->V.vy, S->V.fx, S->V.fy};
+  for (int q = 0; q < 6; q++) HIPCHK(launch_copy_d(dst[q], src + q * nv, nv, S->stream));
+  return DPM_OK;
+}
+
+// positions x_a .. x_{a+len} of a recomputed segment into d_seg slots 0..len
+int record_segment(dpm_set_s* S, long len) {
+  const long nv = S->nv;
+  HIPCHK(launch_copy_d(S->d_seg, S->V.x, nv, S->stream));
+  HIPCHK(launch_copy_d(S->d_seg + nv, S->V.y, nv, S->stream));
+  HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
+  int rc;
+  if (len < 4 || !graphs_enabled()) {
+    rc = verlet_sequence(S, len, false, S->d_seg);
+    return rc;
+  }
+  rc = run_graph(S, std::make_tuple(4, len, S->cur), [&]() { return verlet_sequence(S, len, false, S->d_seg); });
+  if (rc) return rc;
+  if ((len % 2) == 1) swap_pos(S);
+  return DPM_OK;
+}
+
+// reverse pass over positions n = top .. a of the recorded segment starting at step a
+int backward_segment(dpm_set_s* S, long a, long top, long N) {
+  const long nv = S->nv;
+  const long count = top - a + 1;
+  const bool hasTop = (top == N), hasZero = (a == 0);
+  auto seq = [&]() -> int {
+    int p = 0;
+    for (long n = top; n >= a; n--) {
+      double* sx = S->d_seg + (n - a) * 2 * nv;
+      const View Vn = view_at(S, sx, sx + nv);
+      AdjArgs A{};
+      A.wx = adj_w(S, S->wCur);
+      A.wy = A.wx + nv;
+      A.ax = adj_ax(S);
+      A.ay = A.ax + nv;
+      A.nwx = adj_w(S, 1 - S->wCur);
+      A.nwy = A.nwx + nv;
+      A.g = adj_g(S);
+      A.cd = ((n == N || n == 0) ? 0.5 : 1.0) * S->tapeDt;
+      A.dt = S->tapeDt;
+      A.mode = 1;
+      A.update = n != 0;
+      int rc = hvp_launches(S, Vn, p, A);
+      if (rc) return rc;
+      p ^= 1;
+      S->wCur ^= 1;
+    }
+    return clean_protocol(S);
+  };
+  const int w0 = S->wCur;
+  if (!graphs_enabled()) return seq();
+  const int kind = 5 | (hasTop ? 16 : 0) | (hasZero ? 32 : 0);
+  int rc = run_graph(S, std::make_tuple(kind, count, w0), [&]() {
+    int r = seq();
+    S->wCur = w0;
+    return r;
+  });
+  if (rc) return rc;
+  if (count % 2) S->wCur ^= 1;
   return DPM_OK;
 }
 
@@ -520,7 +657,8 @@ int dpm_destroy(dpm_set_t h) {
                   V.ncnt, V.nbr, V.nbrp, V.tile_v0, V.tile_v1, V.tile_sys, V.stile, S->G.ncx, S->G.ncy,
                   S->G.coff, S->G.brute, S->G.count[0], S->G.count[1], S->G.slots, S->G.cellSys, S->d_flag[0],
                   S->d_flag[1], S->d_any, S->d_updateCount, S->d_err, S->d_maxcnt, S->d_ctr, S->d_notDone,
-                  S->d_fire[0], S->d_fire[1], S->d_part, S->d_red, S->d_scale};
+                  S->d_fire[0], S->d_fire[1], S->d_part, S->d_red, S->d_scale, V.kr,
+                  S->d_ck, S->d_fin, S->d_seg, S->d_adj};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (S->ownStream) hipStreamDestroy(S->ownStream);
@@ -567,6 +705,9 @@ int dpm_set_field(dpm_set_t h, int field, const double* host, long n) {
   int nc;
   if (!host || !field_ptrs(S, field, &c0, &c1, &cnt, &nc)) return fail(S, DPM_ERR_ARG, "bad field");
   if (n != cnt * nc) return fail(S, DPM_ERR_ARG, "field size mismatch");
+  if (field == DPM_FIELD_RAD || field == DPM_FIELD_L0 || field == DPM_FIELD_THETA0 || field == DPM_FIELD_A0 ||
+      field == DPM_FIELD_BOX)
+    S->paramSerial++;
   if (nc == 1) {
     HIPCHK(hipMemcpyAsync(c0, host, sizeof(double) * cnt, hipMemcpyHostToDevice, S->stream));
   } else {
@@ -642,6 +783,34 @@ int dpm_set_types(dpm_set_t h, int potential, int interaction, int monomer, int 
 int dpm_set_energy_costs(dpm_set_t h, double ea, double el, double eb, double ec) {
   CHECK_HANDLE(h);
   S->P.ea = ea; S->P.el = el; S->P.eb = eb; S->P.ec = ec;
+  bump(S);
+  return DPM_OK;
+}
+
+int dpm_set_ring_costs(dpm_set_t h, const double* ka, const double* kl, const double* kb) {
+  CHECK_HANDLE(h);
+  if (!ka && !kl && !kb) {
+    if (S->V.kr) {
+      HIPCHK(hipStreamSynchronize(S->stream));
+      hipFree(S->V.kr);
+      S->V.kr = nullptr;
+      S->paramSerial++;
+      bump(S);
+    }
+    return DPM_OK;
+  }
+  if (!ka || !kl || !kb) return fail(S, DPM_ERR_ARG, "ring costs: pass all three arrays or none");
+  const long np = S->np;
+  if (!S->V.kr) HIPCHK(dalloc(&S->V.kr, 3 * np));
+  std::vector<double> k(3 * np);
+  for (long r = 0; r < np; r++) {
+    k[r] = ka[r];
+    k[np + r] = kl[r];
+    k[2 * np + r] = kb[r];
+  }
+  HIPCHK(hipMemcpyAsync(S->V.kr, k.data(), sizeof(double) * 3 * np, hipMemcpyHostToDevice, S->stream));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  S->paramSerial++;
   bump(S);
   return DPM_OK;
 }
@@ -830,8 +999,10 @@ int dpm_get_interaction_forces(dpm_set_t h, double* host, long n) {
   HIPCHK(hipStreamSynchronize(S->stream));
   // the same step kernel with the shape costs and self-propulsion switched off
   const Params keep = S->P;
+  double* keepKr = S->V.kr;
   S->P.ea = S->P.el = S->P.eb = 0;
   S->P.particle = PART_PASSIVE;
+  S->V.kr = nullptr;
   HIPCHK(launch_fill_int(S->d_any + S->cur, 0, 1, S->stream));
   HIPCHK(launch_bin(S, 0, 0));
   StepArgs A{};
@@ -840,6 +1011,7 @@ int dpm_get_interaction_forces(dpm_set_t h, double* host, long n) {
   A.manage = 0;
   hipError_t e = launch_step(S, MODE_FORCE, A);
   S->P = keep;
+  S->V.kr = keepKr;
   HIPCHK(e);
   std::vector<double> ix(nv), iy(nv);
   HIPCHK(hipMemcpyAsync(ix.data(), S->V.fx, nv * sizeof(double), hipMemcpyDeviceToHost, S->stream));
@@ -1157,7 +1329,136 @@ int dpm_time_step_kernel(dpm_set_t h, long reps, double* avgUs, double* meanNeig
 
 int dpm_force_hvp(dpm_set_t h, const double* ux, const double* uy, double* outx, double* outy) {
   CHECK_HANDLE(h);
-  return fail(S, DPM_ERR_UNSUPPORTED, "dpm_force_hvp: not available in this build");
+  if (!ux || !uy || !outx || !outy) return fail(S, DPM_ERR_ARG, "dpm_force_hvp: null vector");
+  int rc = adjoint_supported(S);
+  if (rc) return rc;
+  rc = adj_alloc(S);
+  if (rc) return rc;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
+    rc = clean_protocol(S);
+    if (rc) return rc;
+    AdjArgs A{};
+    A.wx = ux;
+    A.wy = uy;
+    A.hx = outx;
+    A.hy = outy;
+    A.mode = 0;
+    rc = hvp_launches(S, S->V, 0, A);
+    if (rc) return rc;
+    rc = clean_protocol(S);
+    if (rc) return rc;
+    int err = 0;
+    HIPCHK(hipMemcpyAsync(&err, S->d_err, sizeof(int), hipMemcpyDeviceToHost, S->stream));
+    HIPCHK(hipStreamSynchronize(S->stream));
+    if (!err) return DPM_OK;
+    std::vector<int> all(S->nsys, 1);  // rows outgrew the list: widen, then evaluate again
+    rc = rebuild(S, all, 0);
+    if (rc) return rc;
+  }
+  return fail(S, DPM_ERR_OVERFLOW, "dpm_force_hvp: neighbour list overflow");
+}
+
+int dpm_energy_param_grad(dpm_set_t h, double* gring, double* gl0) {
+  CHECK_HANDLE(h);
+  int rc = adjoint_supported(S);
+  if (rc) return rc;
+  rc = adj_alloc(S);
+  if (rc) return rc;
+  double* tmp = adj_g(S);
+  HIPCHK(launch_energy_param_partials(S, tmp));
+  if (gring) HIPCHK(launch_ring_sum3(S, tmp, gring));
+  if (gl0) HIPCHK(launch_copy_d(gl0, tmp + 3 * S->nv, S->nv, S->stream));
+  HIPCHK(launch_fill_d(tmp, 0.0, 4 * S->nv, S->stream));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
+int dpm_nve_forward(dpm_set_t h, long steps, long every) {
+  CHECK_HANDLE(h);
+  if (steps < 1 || every < 1) return fail(S, DPM_ERR_ARG, "dpm_nve_forward: steps and every must be >= 1");
+  int rc = adjoint_supported(S);
+  if (rc) return rc;
+  const long nv = S->nv;
+  const long nck = (steps + every - 1) / every;
+  if (nck > S->tapeNck || every > S->tapeEvery || !S->d_fin) {
+    HIPCHK(hipStreamSynchronize(S->stream));
+    hipFree(S->d_ck);
+    hipFree(S->d_seg);
+    hipFree(S->d_fin);
+    S->d_ck = S->d_seg = S->d_fin = nullptr;
+    HIPCHK(dalloc(&S->d_ck, nck * 6 * nv));
+    HIPCHK(dalloc(&S->d_seg, (every + 1) * 2 * nv));
+    HIPCHK(dalloc(&S->d_fin, 6 * nv));
+    drop_graphs(S);
+  }
+  S->tapeSteps = 0;
+  for (long k = 0; k < nck; k++) {
+    const long len = std::min(every, steps - k * every);
+    rc = save_state(S, S->d_ck + k * 6 * nv);
+    if (rc) return rc;
+    rc = run_verlet(S, len, false);
+    if (rc) return rc;
+  }
+  rc = save_state(S, S->d_fin);
+  if (rc) return rc;
+  rc = check_err_after_loop(S);
+  if (rc) return rc;
+  S->tapeSteps = steps;
+  S->tapeEvery = every;
+  S->tapeNck = nck;
+  S->tapeDt = S->P.dt;
+  S->tapeP = S->P;
+  S->tapeSerial = S->paramSerial;
+  return DPM_OK;
+}
+
+int dpm_nve_backward(dpm_set_t h, const double* ax, const double* av, double* gx0, double* gv0, double* gring,
+                     double* gl0) {
+  CHECK_HANDLE(h);
+  if (!ax || !av) return fail(S, DPM_ERR_ARG, "dpm_nve_backward: null adjoint");
+  if (S->tapeSteps < 1) return fail(S, DPM_ERR_STATE, "dpm_nve_backward: no trajectory recorded");
+  if (memcmp(&S->P, &S->tapeP, sizeof(Params)) != 0 || S->paramSerial != S->tapeSerial)
+    return fail(S, DPM_ERR_STATE, "dpm_nve_backward: parameters changed since dpm_nve_forward");
+  int rc = adjoint_supported(S);
+  if (rc) return rc;
+  rc = adj_alloc(S);
+  if (rc) return rc;
+  const long nv = S->nv, N = S->tapeSteps, every = S->tapeEvery;
+  S->wCur = 0;
+  HIPCHK(launch_copy_d(adj_w(S, 0), av, 2 * nv, S->stream));
+  HIPCHK(launch_copy_d(adj_ax(S), ax, 2 * nv, S->stream));
+  HIPCHK(launch_fill_d(adj_g(S), 0.0, 4 * nv, S->stream));
+  HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
+  std::vector<int> all(S->nsys, 1);
+  for (long k = S->tapeNck - 1; k >= 0; k--) {
+    const long a = k * every, b = std::min(a + every, N);
+    rc = load_state(S, S->d_ck + k * 6 * nv);
+    if (rc) return rc;
+    rc = rebuild(S, all, 0);
+    if (rc) return rc;
+    rc = record_segment(S, b - a);
+    if (rc) return rc;
+    rc = clean_protocol(S);
+    if (rc) return rc;
+    rc = backward_segment(S, a, (k == S->tapeNck - 1) ? b : b - 1, N);
+    if (rc) return rc;
+    int err = 0;
+    HIPCHK(hipMemcpyAsync(&err, S->d_err, sizeof(int), hipMemcpyDeviceToHost, S->stream));
+    HIPCHK(hipStreamSynchronize(S->stream));
+    if (err) return fail(S, DPM_ERR_OVERFLOW, "dpm_nve_backward: neighbour list overflow");
+  }
+  if (gx0) HIPCHK(launch_copy_d(gx0, adj_ax(S), 2 * nv, S->stream));
+  if (gv0) HIPCHK(launch_copy_d(gv0, adj_w(S, S->wCur), 2 * nv, S->stream));
+  if (gring) HIPCHK(launch_ring_sum3(S, adj_g(S), gring));
+  if (gl0) HIPCHK(launch_copy_d(gl0, adj_g(S) + 3 * nv, nv, S->stream));
+  // leave the set in the state the forward pass ended in
+  rc = load_state(S, S->d_fin);
+  if (rc) return rc;
+  rc = rebuild(S, all, 0);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
 }
 
 }  // extern "C"

@@ -47,6 +47,7 @@ struct View {
   double *a0, *area, *perim, *cx, *cy, *pe, *ang;
   double *bx, *by, *ibx, *iby;  // per-system box and its reciprocal
   int *svoff, *sroff;  // per-system vertex / ring offsets (nsys + 1)
+  double* kr;          // per-ring shape costs [ka(np) | kl(np) | kb(np)], or null for the globals
   int *nbr, *ncnt;     // ELL neighbour list nbr[k * nv + v], count[v]
   int2* nbrp;          // beside each entry u: (prev(u), prev(prev(u))) in u's ring
   int *tile_v0, *tile_v1, *tile_sys;  // ring-aligned tiles (<= 64 vertices, one system)

@@ -51,6 +51,8 @@ struct StepArgs {
   double dt;                      // drift time step
   double* xout;                   // drift destination (MODE_START)
   double* yout;
+  double* recx;                   // optional second copy of the drift destination (adjoint tape)
+  double* recy;
   int* flagCur;                   // rebuild flags of this step's positions
   int* flagOut;                   // rebuild flags of the drift destination
   int* anyOut;                    // any-flag of the drift destination
@@ -63,6 +65,29 @@ struct StepArgs {
   int* maxcnt;
   int* updateCount;
   int manage;                     // inside loops: rebuild flagged rows, clear cells, count updates
+};
+
+// Arguments of the H w kernel (dpm_adjoint.hip).
+struct AdjArgs {
+  const int* flag;      // rebuild flags of this parity (rows rebuilt first when set), or null
+  const int* count;     // cell counts of this parity
+  int* err;
+  int* maxcnt;
+  const double* A;      // ring area and its derivative along w
+  const double* dA;
+  const double* wx;     // the vector w
+  const double* wy;
+  double* hx;           // mode 0: H w
+  double* hy;
+  double* ax;           // mode 1: adjoint of positions, updated in place
+  double* ay;
+  double* nwx;          // mode 1: next vector (w + dt ax when update, else w)
+  double* nwy;
+  double* g;            // mode 1: per-vertex parameter partials [ka | kl | kb | l0] x nv, accumulated
+  double cd;            // c dt
+  double dt;
+  int mode;
+  int update;
 };
 
 }  // namespace dpm
@@ -115,6 +140,19 @@ struct dpm_set_s {
   double* d_scale = nullptr; // per-system scalars
   std::vector<double> h_red;
 
+  // adjoint tape (dpm_nve_forward / dpm_nve_backward)
+  long tapeSteps = 0, tapeEvery = 0, tapeNck = 0;
+  double tapeDt = 0;
+  dpm::Params tapeP{};        // parameters the trajectory was recorded with
+  long tapeSerial = -1;
+  long paramSerial = 0;       // bumped by set_ring_costs and by set_field of rad, l0, theta0, a0, box
+  double* d_ck = nullptr;     // checkpoints: tapeNck x 6 x nv (x, y, vx, vy, fx, fy)
+  double* d_fin = nullptr;    // final state 6 x nv
+  double* d_seg = nullptr;    // one segment's positions: (tapeEvery + 1) x 2 x nv
+  double* d_adj = nullptr;    // work: w[2] (2nv each), ax (2nv), partials (4nv), ring area and dA (2np)
+  long adjNv = 0;
+  int adjCur = 0, wCur = 0;
+
   // graph cache: key = (kind, steps, cur)
   std::map<std::tuple<int, long, int>, hipGraphExec_t> graphs;
   long version = 0;  // bumped when launch arguments change
@@ -126,7 +164,7 @@ namespace dpm {
 // launchers (dpm_kernels.hip); all return hipError_t of the last launch
 hipError_t launch_shape_only(dpm_set_s* S);
 hipError_t launch_step(dpm_set_s* S, int mode, const StepArgs& A);
-hipError_t launch_start(dpm_set_s* S, double h, double dt);
+hipError_t launch_start(dpm_set_s* S, double h, double dt, double* recx = nullptr, double* recy = nullptr);
 hipError_t launch_bin(dpm_set_s* S, int bumpThermal, int bumpActive);
 hipError_t launch_build_rows(dpm_set_s* S, int countUpdates);
 hipError_t launch_fill_int(int* p, int value, long n, hipStream_t st);
@@ -142,5 +180,13 @@ hipError_t launch_fire_partials(dpm_set_s* S);
 hipError_t launch_fire_step(dpm_set_s* S);
 hipError_t launch_fire_done(dpm_set_s* S, int* notDone);
 StepArgs loop_args(dpm_set_s* S);
+hipError_t launch_bin_view(dpm_set_s* S, const View& Vn, int parity);
+hipError_t launch_adj_area(dpm_set_s* S, const View& Vn, const double* wx, const double* wy, double* A, double* dA);
+hipError_t launch_adj_flag(dpm_set_s* S, const View& Vn, int parity);
+hipError_t launch_adj_hvp(dpm_set_s* S, const View& Vn, AdjArgs A);
+hipError_t launch_energy_param_partials(dpm_set_s* S, double* out);
+hipError_t launch_ring_sum3(dpm_set_s* S, const double* in, double* out);
+hipError_t launch_fill_d(double* p, double value, long n, hipStream_t st);
+hipError_t launch_copy_d(double* dst, const double* src, long n, hipStream_t st);
 
 }  // namespace dpm

@@ -20,6 +20,7 @@
 // double buffered by position-buffer parity, so a step costs two launches
 // (k_bin, normally idle, and k_step) with no host synchronisation.
 #include "dpm_internal.h"
+#include "dpm_rows.h"
 
 namespace dpm {
 
@@ -56,7 +57,6 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-__device__ __forceinline__ Box box_of(const View& V, int s) { return Box{V.bx[s], V.by[s], V.ibx[s], V.iby[s]}; }
 
 // Lane context of a ring-aligned tile.
 struct Lane {
@@ -139,12 +139,16 @@ __device__ __forceinline__ ShapeOut ring_shape(const View& V, const Params& P, c
   const double rnx = pbx(xn - comx, B), rny = pby(yn - comy, B);
   const double rpx = pbx(xp - comx, B), rpy = pby(yp - comy, B);
   const double a0 = V.a0[L.r];
+  // shape costs: the global ea, el, eb, or per-ring values when the caller set them
+  const double ea = V.kr ? V.kr[L.r] : P.ea;
+  const double el = V.kr ? V.kr[V.np + L.r] : P.el;
+  const double eb = V.kr ? V.kr[2 * V.np + L.r] : P.eb;
   // area term (calcAreaForceEnergy 429-435)
   const double deltaA = (area / a0) - 1.;
-  const double gmA = P.ea * deltaA / a0;
+  const double gmA = ea * deltaA / a0;
   double fx = 0.5 * gmA * (rpy - rny);
   double fy = 0.5 * gmA * (rnx - rpx);
-  double e = (0.5 * P.ea * deltaA * deltaA) / nd;
+  double e = (0.5 * ea * deltaA * deltaA) / nd;
   // perimeter term (calcPerimeterForceEnergy 437-447 / FENE 449-455)
   const double nsx = rnx - rvx, nsy = rny - rvy;
   const double qsx = rvx - rpx, qsy = rvy - rpy;
@@ -155,116 +159,31 @@ __device__ __forceinline__ ShapeOut ring_shape(const View& V, const Params& P, c
   if (P.monomer == MONO_HARMONIC) {
     const double pDelta = (pLen / l0p) - 1.;
     const double nDelta = (nLen / l0v) - 1.;
-    fx += P.el * ((nDelta * (rnx - rvx) / (l0v * nLen)) - (pDelta * (rvx - rpx) / (l0p * pLen)));
-    fy += P.el * ((nDelta * (rny - rvy) / (l0v * nLen)) - (pDelta * (rvy - rpy) / (l0p * pLen)));
-    e += (0.5 * P.el * pDelta * pDelta);
+    fx += el * ((nDelta * (rnx - rvx) / (l0v * nLen)) - (pDelta * (rvx - rpx) / (l0p * pLen)));
+    fy += el * ((nDelta * (rny - rvy) / (l0v * nLen)) - (pDelta * (rvy - rpy) / (l0p * pLen)));
+    e += (0.5 * el * pDelta * pDelta);
   } else {
-    const double c = 2 * P.el * P.stiff * P.extSq;
+    const double c = 2 * el * P.stiff * P.extSq;
     fx += c * ((nLen / (P.extSq * l0v * l0v - nLen * nLen)) * (rnx - rvx) / nLen -
                (pLen / (P.extSq * l0p * l0p - pLen * pLen)) * (rvx - rpx) / pLen);
     fy += c * ((nLen / (P.extSq * l0v * l0v - nLen * nLen)) * (rny - rvy) / nLen -
                (pLen / (P.extSq * l0p * l0p - pLen * pLen)) * (rvy - rpy) / pLen);
-    e += P.el * P.stiff * P.extSq * log(1 - pLen * pLen / (P.extSq * l0p * l0p));
+    e += el * P.stiff * P.extSq * log(1 - pLen * pLen / (P.extSq * l0p * l0p));
   }
   // bending term (calcBendingForceEnergy 457-471)
   const double D = theta - V.th0[L.v];
   const double Dp = shfl_d(D, L.pl), Dn = shfl_d(D, L.nl);
   const double preVar = (D - Dp) / (qsx * qsx + qsy * qsy);
   const double nextVar = (D - Dn) / (nsx * nsx + nsy * nsy);
-  fx += P.eb * (preVar * qsy + nextVar * nsy);
-  fy += P.eb * (preVar * (-qsx) + nextVar * (-nsx));
-  e += (0.5 * P.eb * D * D);
+  fx += eb * (preVar * qsy + nextVar * nsy);
+  fy += eb * (preVar * (-qsx) + nextVar * (-nsx));
+  e += (0.5 * eb * D * D);
   o.fx = fx; o.fy = fy; o.e = e;
   return o;
 }
 
 // --------------------------------------------------------- neighbours ---
-__device__ __forceinline__ void cell_of(double x, double y, const Box& B, int ncx, int ncy, int& cx, int& cy) {
-  const double wx = x - B.Lx * floor(x * B.iLx), wy = y - B.Ly * floor(y * B.iLy);
-  // clamp in floating point first (fmax maps NaN to 0), so the conversion is always defined
-  const double fx = fmin(fmax(wx * B.iLx * ncx, 0.0), ncx - 1.0);
-  const double fy = fmin(fmax(wy * B.iLy * ncy, 0.0), ncy - 1.0);
-  cx = (int)fx;
-  cy = (int)fy;
-}
-
-// previous and second previous vertex of u in its ring (getPreviousId 233-243)
-__device__ __forceinline__ int2 prev_pair(const View& V, int u) {
-  const int ring = V.vring[u];
-  const int f = V.rfirst[ring], n = V.rnv[ring];
-  const int p1 = (u == f) ? f + n - 1 : u - 1;
-  const int p2 = (p1 == f) ? f + n - 1 : p1 - 1;
-  return make_int2(p1, p2);
-}
-
-// insert j (with its prev pair) into the ascending ELL row of v that holds c entries
-__device__ __forceinline__ void row_insert(const View& V, int v, int c, int j, int2 pj) {
-  const long nv = V.nv;
-  int k = c;
-  while (k > 0) {
-    const int prev = V.nbr[(long)(k - 1) * nv + v];
-    if (prev <= j) break;
-    V.nbr[(long)k * nv + v] = prev;
-    V.nbrp[(long)k * nv + v] = V.nbrp[(long)(k - 1) * nv + v];
-    k--;
-  }
-  V.nbr[(long)k * nv + v] = j;
-  V.nbrp[(long)k * nv + v] = pj;
-}
-
-// Rebuild the row of vertex v.  kernelCalcNeighborList predicate (cudaKernel.cuh:1735-1763):
-// j in list(v) iff rings differ and -(1 - d/radSum) < cutDistance; rows stay ascending as in
-// the reference.  Candidates come from the 3x3 cell block (or the whole system when the box
-// is narrower than three cells).  Also resets lastPos (resetLastPositions).
-__device__ void build_row(const View& V, const Params& P, const CellGrid& G, const int* count, int v, int s,
-                          const Box& B, int* err, int* maxcnt) {
-  const double x = V.x[v], y = V.y[v], rv = V.rad[v];
-  const int r = V.vring[v];
-  const int W = V.W;
-  int c = 0;
-  if (G.brute[s]) {
-    const int b0 = V.svoff[s], b1 = V.svoff[s + 1];
-    for (int j = b0; j < b1; j++) {
-      if (V.vring[j] == r) continue;
-      const double dx = pbx(x - V.x[j], B), dy = pby(y - V.y[j], B);
-      const double d = sqrt(dx * dx + dy * dy);
-      if (-(1 - d / (rv + V.rad[j])) < P.cutDistance) {
-        if (c < W) {
-          V.nbr[(long)c * V.nv + v] = j;
-          V.nbrp[(long)c * V.nv + v] = prev_pair(V, j);
-        }
-        c++;
-      }
-    }
-  } else {
-    const int ncx = G.ncx[s], ncy = G.ncy[s];
-    int cx, cy;
-    cell_of(x, y, B, ncx, ncy, cx, cy);
-    for (int oy = -1; oy <= 1; oy++) {
-      const int yy = (cy + oy + ncy) % ncy;
-      for (int ox = -1; ox <= 1; ox++) {
-        const int xx = (cx + ox + ncx) % ncx;
-        const int cell = G.coff[s] + yy * ncx + xx;
-        const int m = min(count[cell], G.cap);
-        for (int q = 0; q < m; q++) {
-          const int j = G.slots[(long)cell * G.cap + q];
-          if (V.vring[j] == r) continue;
-          const double dx = pbx(x - V.x[j], B), dy = pby(y - V.y[j], B);
-          const double d = sqrt(dx * dx + dy * dy);
-          if (-(1 - d / (rv + V.rad[j])) < P.cutDistance) {
-            if (c < W) row_insert(V, v, c, j, prev_pair(V, j));
-            c++;
-          }
-        }
-      }
-    }
-  }
-  if (c > W) atomicOr(err, ERR_LIST_OVERFLOW);
-  atomicMax(maxcnt, c);
-  V.ncnt[v] = c < W ? c : W;
-  V.lx[v] = x;
-  V.ly[v] = y;
-}
+// cell_of, prev_pair, row_insert and build_row live in dpm_rows.h (shared with the adjoint).
 
 // Bin the positions of flagged systems into the cells of this parity.  Thread 0 first bumps
 // the RNG call counters of the step and clears the other parity's flags (they were consumed
@@ -473,6 +392,10 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
     const double xn = x + A.dt * vx, yn = y + A.dt * vy;
     A.xout[L.v] = xn;
     A.yout[L.v] = yn;
+    if (A.recx) {
+      A.recx[L.v] = xn;
+      A.recy[L.v] = yn;
+    }
     const double ddx = pbx(xn - V.lx[L.v], B), ddy = pby(yn - V.ly[L.v], B);
     if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) {
       atomicOr(&A.flagOut[L.s], 1);
@@ -491,7 +414,8 @@ __global__ void k_shape(View V, Params P) {
 }
 
 // first half step of a velocity-Verlet call: v += h F; x' = x + dt v; displacement test
-__global__ void k_start(View V, Params P, double h, double dt, double* xo, double* yo, int* flagOut, int* anyOut) {
+__global__ void k_start(View V, Params P, double h, double dt, double* xo, double* yo, int* flagOut, int* anyOut,
+                        double* rx, double* ry) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= V.nv) return;
   const int s = V.vsys[v];
@@ -501,6 +425,10 @@ __global__ void k_start(View V, Params P, double h, double dt, double* xo, doubl
   const double xn = V.x[v] + dt * vx, yn = V.y[v] + dt * vy;
   V.vx[v] = vx; V.vy[v] = vy;
   xo[v] = xn; yo[v] = yn;
+  if (rx) {
+    rx[v] = xn;
+    ry[v] = yn;
+  }
   const double ddx = pbx(xn - V.lx[v], B), ddy = pby(yn - V.ly[v], B);
   if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) {
     atomicOr(&flagOut[s], 1);
@@ -817,10 +745,10 @@ hipError_t launch_shape_only(dpm_set_s* S) {
   return hipGetLastError();
 }
 
-hipError_t launch_start(dpm_set_s* S, double h, double dt) {
+hipError_t launch_start(dpm_set_s* S, double h, double dt, double* recx, double* recy) {
   const int nb = 1 - S->cur;
   k_start<<<grid_v(S->nv), 256, 0, S->stream>>>(S->V, S->P, h, dt, S->pos[nb][0], S->pos[nb][1],
-                                                S->d_flag[nb], S->d_any + nb);
+                                                S->d_flag[nb], S->d_any + nb, recx, recy);
   return hipGetLastError();
 }
 
@@ -830,6 +758,15 @@ hipError_t launch_bin(dpm_set_s* S, int bumpThermal, int bumpActive) {
   k_bin<<<grid_v(n), 256, 0, S->stream>>>(S->V, S->G, S->G.count[b], S->d_flag[b], S->d_any + b,
                                           S->d_flag[1 - b], S->d_any + 1 - b, S->d_ctr, bumpThermal, bumpActive,
                                           S->d_err);
+  return hipGetLastError();
+}
+
+// k_bin on positions other than the set's current buffer (the adjoint's recorded positions),
+// with the flags and cells of `parity`; no RNG counter bump.
+hipError_t launch_bin_view(dpm_set_s* S, const View& Vn, int parity) {
+  long n = S->nv > S->nsys ? S->nv : S->nsys;
+  k_bin<<<grid_v(n), 256, 0, S->stream>>>(Vn, S->G, S->G.count[parity], S->d_flag[parity], S->d_any + parity,
+                                          S->d_flag[1 - parity], S->d_any + 1 - parity, S->d_ctr, 0, 0, S->d_err);
   return hipGetLastError();
 }
 

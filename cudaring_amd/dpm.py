@@ -37,6 +37,7 @@ SIGNATURES = {
     "dpm_device_ptr": (_i, [_vp, _i, _i, C.POINTER(_vp)]),
     "dpm_set_types": (_i, [_vp, _i, _i, _i, _i]),
     "dpm_set_energy_costs": (_i, [_vp, _d, _d, _d, _d]),
+    "dpm_set_ring_costs": (_i, [_vp, _vp, _vp, _vp]),
     "dpm_set_attraction_constants": (_i, [_vp, _d, _d]),
     "dpm_set_lj_cutoff": (_i, [_vp, _d]),
     "dpm_set_fene_constants": (_i, [_vp, _d, _d]),
@@ -74,6 +75,9 @@ SIGNATURES = {
     "dpm_get_particle_shapes": (_i, [_vp, _vp]),
     "dpm_scale_vertices": (_i, [_vp, _vp]),
     "dpm_force_hvp": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "dpm_energy_param_grad": (_i, [_vp, _vp, _vp]),
+    "dpm_nve_forward": (_i, [_vp, _l, _l]),
+    "dpm_nve_backward": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "dpm_time_step_kernel": (_i, [_vp, _l, C.POINTER(_d), C.POINTER(_d)]),
     "dpm_set_launch_config": (_i, [_vp, _i]),
 }
@@ -185,6 +189,15 @@ class DPMSet:
 
     def set_energy_costs(self, ea, el, eb, ec):
         self._chk(self.L.dpm_set_energy_costs(self.h, ea, el, eb, ec), "set_energy_costs")
+
+    def set_ring_costs(self, ka=None, kl=None, kb=None):
+        """Per-ring ka, kl, kb (arrays of numParticles); no arguments returns to the global costs."""
+        if ka is None:
+            self._chk(self.L.dpm_set_ring_costs(self.h, None, None, None), "set_ring_costs")
+            return
+        arrs = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (self.np,)))
+                for a in (ka, kl, kb)]
+        self._chk(self.L.dpm_set_ring_costs(self.h, *[_p(a) for a in arrs]), "set_ring_costs")
 
     def set_attraction(self, l1, l2):
         self._chk(self.L.dpm_set_attraction_constants(self.h, l1, l2), "set_attraction")
@@ -337,6 +350,24 @@ class DPMSet:
         self._chk(self.L.dpm_time_step_kernel(self.h, reps, C.byref(us), C.byref(kbar)), "time_step_kernel")
         return us.value, kbar.value
 
+    # ---------------------------------------------------------------- adjoint
+    # Vectors are device pointers (ints) of float64 arrays on the set's stream; "2V SoA" = the V x
+    # components followed by the V y components.
     def force_hvp(self, ux_ptr, uy_ptr, ox_ptr, oy_ptr):
+        """out = H u at the current positions."""
         self._chk(self.L.dpm_force_hvp(self.h, C.c_void_p(ux_ptr), C.c_void_p(uy_ptr),
                                        C.c_void_p(ox_ptr), C.c_void_p(oy_ptr)), "force_hvp")
+
+    def energy_param_grad(self, gring_ptr, gl0_ptr):
+        """dU/d(ka, kl, kb) per ring (3R) and dU/dl0 (V) at the current shape variables."""
+        self._chk(self.L.dpm_energy_param_grad(self.h, C.c_void_p(gring_ptr), C.c_void_p(gl0_ptr)),
+                  "energy_param_grad")
+
+    def nve_forward(self, steps, every):
+        """NVE steps with checkpoints every `every` steps for nve_backward."""
+        self._chk(self.L.dpm_nve_forward(self.h, steps, every), "nve_forward")
+
+    def nve_backward(self, ax_ptr, av_ptr, gx_ptr, gv_ptr, gring_ptr, gl0_ptr):
+        """Reverse pass of the last nve_forward (2V SoA adjoints in, gradients out)."""
+        p = [C.c_void_p(x) if x else None for x in (ax_ptr, av_ptr, gx_ptr, gv_ptr, gring_ptr, gl0_ptr)]
+        self._chk(self.L.dpm_nve_backward(self.h, *p), "nve_backward")

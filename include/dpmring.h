@@ -93,6 +93,10 @@ int dpm_device_ptr(dpm_set_t h, int field, int component, void** ptr);
 int dpm_set_types(dpm_set_t h, int potential, int interaction, int monomer, int particle);
 /* setEnergyCosts(ea, el, eb, ec), DPM2D.h:525 */
 int dpm_set_energy_costs(dpm_set_t h, double ea, double el, double eb, double ec);
+/* Per-ring shape costs ka[R], kl[R], kb[R] replacing the global ea, el, eb of setEnergyCosts in the
+ * area, perimeter and bending terms; all three NULL returns to the globals.  No reference
+ * counterpart: the reference's costs are device-global scalars (cudaKernel.cuh:39-42). */
+int dpm_set_ring_costs(dpm_set_t h, const double* ka, const double* kl, const double* kb);
 /* setAttractionConstants(l1, l2), DPM2D.h:535 */
 int dpm_set_attraction_constants(dpm_set_t h, double l1, double l2);
 /* setLJcutoff, DPM2D.h:539 (energy and force shifts use the current ec) */
@@ -187,9 +191,25 @@ int dpm_set_launch_config(dpm_set_t h, int wavesPerTile);
 int dpm_time_step_kernel(dpm_set_t h, long reps, double* avgMicroseconds, double* meanNeighbors);
 
 /* ---- adjoint (no reference counterpart: differentiation is an addition) ---- */
-/* Hessian-vector product of the potential energy: out = H u (device SoA arrays of length V,
- * caller-owned, on the handle's stream), evaluated at the current positions. */
+/* Supported for the vertexVertex interaction with harmonic monomers and passive particles, all
+ * four potentials.  Vectors are device arrays owned by the caller and used on the handle's
+ * stream; a "2V SoA" vector holds the V x components followed by the V y components. */
+/* Hessian-vector product of the potential energy at the current positions: out = H u
+ * (ux, uy, outx, outy: device arrays of V). */
 int dpm_force_hvp(dpm_set_t h, const double* ux, const double* uy, double* outx, double* outy);
+/* Gradient of the potential energy at the current shape variables (after calcForceEnergy)
+ * with respect to the per-ring costs, gring = [dU/dka(R) | dU/dkl(R) | dU/dkb(R)], and to the
+ * rest lengths, gl0[V]; either may be NULL. */
+int dpm_energy_param_grad(dpm_set_t h, double* gring, double* gl0);
+/* NVELoop() x steps (DPM2D.h:686) from the current state, keeping a checkpoint of positions,
+ * velocities and forces every `every` steps for dpm_nve_backward. */
+int dpm_nve_forward(dpm_set_t h, long steps, long every);
+/* Reverse pass of the last dpm_nve_forward: given ax = dL/dx_N and av = dL/dv_N (2V SoA), returns
+ * gx0 = dL/dx_0, gv0 = dL/dv_0 (2V SoA), gring = dL/d(ka, kl, kb) per ring (3R) and
+ * gl0 = dL/dl0 (V) through the dynamics; outputs may be NULL.  Segments between checkpoints are
+ * recomputed; the set is left in the state the forward pass ended in. */
+int dpm_nve_backward(dpm_set_t h, const double* ax, const double* av, double* gx0, double* gv0, double* gring,
+                     double* gl0);
 
 #ifdef __cplusplus
 }

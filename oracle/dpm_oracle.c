@@ -29,6 +29,7 @@ struct orc_state {
   double *pos, *vel, *force, *energy, *lastPos;
   double *rad, *l0, *theta0, *theta, *length;
   double *a0, *area, *perimeter, *particlePos, *particleEnergy, *particleAngle;
+  double* kr; /* per-ring [ka | kl | kb] or NULL */
   long *neighborList, neighborListSize, *maxNeighborList, maxNeighbors;
   double cutDistance, cutoff;
   long updateCount;
@@ -123,6 +124,7 @@ void orc_destroy(orc_state* s) {
   free(s->a0); free(s->area); free(s->perimeter); free(s->particlePos);
   free(s->particleEnergy); free(s->particleAngle); free(s->maxNeighborList);
   free(s->neighborList);
+  free(s->kr);
   free(s);
 }
 
@@ -173,6 +175,13 @@ int orc_get_field(const orc_state* s, int f, double* data, long n) {
 /* ----------------------------------------------------------- parameters --- */
 void orc_set_types(orc_state* s, int pot, int inter, int mono, int part) {
   s->potentialType = pot; s->interactionType = inter; s->monomerType = mono; s->particleType = part;
+}
+/* per-ring shape costs (an addition); NULL ka clears them */
+void orc_set_ring_costs(orc_state* s, const double* ka, const double* kl, const double* kb) {
+  long np = s->numParticles;
+  if (!ka) { free(s->kr); s->kr = NULL; return; }
+  if (!s->kr) s->kr = (double*)malloc(sizeof(double) * 3 * np);
+  for (long p = 0; p < np; p++) { s->kr[p] = ka[p]; s->kr[np + p] = kl[p]; s->kr[2 * np + p] = kb[p]; }
 }
 /* setEnergyCosts, DPM2D.cu:1875-1884 */
 void orc_set_energy_costs(orc_state* s, double ea, double el, double eb, double ec) {
@@ -284,47 +293,47 @@ void orc_calc_particle_positions(orc_state* s) {
 
 /* ---------------------------------------------------------- shape force --- */
 /* calcAreaForceEnergy, cudaKernel.cuh:429-435 */
-static double area_force_energy(const orc_state* s, double pA0, double pA, const double* nPos,
+static double area_force_energy(double ea, double pA0, double pA, const double* nPos,
                                 const double* pPos, double* f) {
   double deltaA = (pA / pA0) - 1.;
-  double gradMultiple = s->ea * deltaA / pA0;
+  double gradMultiple = ea * deltaA / pA0;
   f[0] += 0.5 * gradMultiple * (pPos[1] - nPos[1]);
   f[1] += 0.5 * gradMultiple * (nPos[0] - pPos[0]);
-  return (0.5 * s->ea * deltaA * deltaA);
+  return (0.5 * ea * deltaA * deltaA);
 }
 
 /* calcPerimeterForceEnergy, cudaKernel.cuh:437-447 */
-static double perimeter_force_energy(const orc_state* s, double tL0, double pL0, double nLength,
+static double perimeter_force_energy(double el, double tL0, double pL0, double nLength,
                                      double pLength, const double* vPos, const double* nPos,
                                      const double* pPos, double* f) {
   double pDelta = (pLength / pL0) - 1.;
   double nDelta = (nLength / tL0) - 1.;
   for (int dim = 0; dim < 2; dim++)
-    f[dim] += s->el * ((nDelta * (nPos[dim] - vPos[dim]) / (tL0 * nLength)) -
-                       (pDelta * (vPos[dim] - pPos[dim]) / (pL0 * pLength)));
-  return (0.5 * s->el * pDelta * pDelta);
+    f[dim] += el * ((nDelta * (nPos[dim] - vPos[dim]) / (tL0 * nLength)) -
+                    (pDelta * (vPos[dim] - pPos[dim]) / (pL0 * pLength)));
+  return (0.5 * el * pDelta * pDelta);
 }
 
 /* calcFENEPerimeterForceEnergy, cudaKernel.cuh:449-455 (energy sign kept as in the reference) */
-static double fene_force_energy(const orc_state* s, double tL0, double pL0, double nLength,
+static double fene_force_energy(const orc_state* s, double el, double tL0, double pL0, double nLength,
                                 double pLength, const double* vPos, const double* nPos,
                                 const double* pPos, double* f) {
   for (int dim = 0; dim < 2; dim++)
-    f[dim] += 2 * s->el * s->stiff * s->extSq *
+    f[dim] += 2 * el * s->stiff * s->extSq *
               ((nLength / (s->extSq * tL0 * tL0 - nLength * nLength)) * (nPos[dim] - vPos[dim]) / nLength -
                (pLength / (s->extSq * pL0 * pL0 - pLength * pLength)) * (vPos[dim] - pPos[dim]) / pLength);
-  return s->el * s->stiff * s->extSq * log(1 - pLength * pLength / (s->extSq * pL0 * pL0));
+  return el * s->stiff * s->extSq * log(1 - pLength * pLength / (s->extSq * pL0 * pL0));
 }
 
 /* calcBendingForceEnergy, cudaKernel.cuh:457-471 */
-static double bending_force_energy(const orc_state* s, const double* preSeg, const double* nextSeg,
+static double bending_force_energy(double eb, const double* preSeg, const double* nextSeg,
                                    double thisD, double nextD, double preD, double* f) {
   double preN[2] = {preSeg[1], -preSeg[0]};
   double nextN[2] = {nextSeg[1], -nextSeg[0]};
   double preVar = (thisD - preD) / (preSeg[0] * preSeg[0] + preSeg[1] * preSeg[1]);
   double nextVar = (thisD - nextD) / (nextSeg[0] * nextSeg[0] + nextSeg[1] * nextSeg[1]);
-  for (int dim = 0; dim < 2; dim++) f[dim] += s->eb * (preVar * preN[dim] + nextVar * nextN[dim]);
-  return (0.5 * s->eb * thisD * thisD);
+  for (int dim = 0; dim < 2; dim++) f[dim] += eb * (preVar * preN[dim] + nextVar * nextN[dim]);
+  return (0.5 * eb * thisD * thisD);
 }
 
 /* kernelCalcShapeForceEnergy, cudaKernel.cuh:474-529: zeroes F and overwrites energy */
@@ -334,6 +343,10 @@ void orc_calc_shape_force_energy(orc_state* s) {
     double vertexPos[2], nextPos[2], previousPos[2], partPos[2], nextSegment[2], previousSegment[2];
     double particleArea = s->area[p];
     double shapeEnergy = 0.0;
+    /* global costs, or per-ring ones (an addition: the reference's are device-global scalars) */
+    const double ea = s->kr ? s->kr[p] : s->ea;
+    const double el = s->kr ? s->kr[s->numParticles + p] : s->el;
+    const double eb = s->kr ? s->kr[2 * s->numParticles + p] : s->eb;
     long nextId = orc_next_id(s, v, p);
     long previousId = orc_previous_id(s, v, p);
     double* f = &s->force[v * 2];
@@ -346,7 +359,7 @@ void orc_calc_shape_force_energy(orc_state* s) {
       nextPos[dim] = pbcd(s, s->pos[nextId * 2 + dim], partPos[dim], dim);
       previousPos[dim] = pbcd(s, s->pos[previousId * 2 + dim], partPos[dim], dim);
     }
-    shapeEnergy += area_force_energy(s, s->a0[p], particleArea, nextPos, previousPos, f) /
+    shapeEnergy += area_force_energy(ea, s->a0[p], particleArea, nextPos, previousPos, f) /
                    s->numVertexInParticleList[p];
     for (int dim = 0; dim < 2; dim++) {
       nextSegment[dim] = nextPos[dim] - vertexPos[dim];
@@ -355,15 +368,15 @@ void orc_calc_shape_force_energy(orc_state* s) {
     double previousLength = sqrt(previousSegment[0] * previousSegment[0] + previousSegment[1] * previousSegment[1]);
     double nextLength = sqrt(nextSegment[0] * nextSegment[0] + nextSegment[1] * nextSegment[1]);
     if (s->monomerType == ORC_MONO_HARMONIC)
-      shapeEnergy += perimeter_force_energy(s, s->l0[v], s->l0[previousId], nextLength, previousLength,
+      shapeEnergy += perimeter_force_energy(el, s->l0[v], s->l0[previousId], nextLength, previousLength,
                                             vertexPos, nextPos, previousPos, f);
     else
-      shapeEnergy += fene_force_energy(s, s->l0[v], s->l0[previousId], nextLength, previousLength,
+      shapeEnergy += fene_force_energy(s, el, s->l0[v], s->l0[previousId], nextLength, previousLength,
                                        vertexPos, nextPos, previousPos, f);
     double previousAngleDelta = s->theta[previousId] - s->theta0[previousId];
     double thisAngleDelta = s->theta[v] - s->theta0[v];
     double nextAngleDelta = s->theta[nextId] - s->theta0[nextId];
-    shapeEnergy += bending_force_energy(s, previousSegment, nextSegment, thisAngleDelta, nextAngleDelta,
+    shapeEnergy += bending_force_energy(eb, previousSegment, nextSegment, thisAngleDelta, nextAngleDelta,
                                         previousAngleDelta, f);
     s->energy[v] = shapeEnergy;
   }

@@ -60,6 +60,7 @@ int orc_get_field(const orc_state* s, int field, double* data, long n);
 
 /* parameters (DPM2D.cu:1875-1960) */
 void orc_set_types(orc_state* s, int potentialType, int interactionType, int monomerType, int particleType);
+void orc_set_ring_costs(orc_state* s, const double* ka, const double* kl, const double* kb);
 void orc_set_energy_costs(orc_state* s, double ea, double el, double eb, double ec);
 void orc_set_attraction_constants(orc_state* s, double l1, double l2);
 void orc_set_lj_cutoff(orc_state* s, double LJcutoff);

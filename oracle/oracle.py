@@ -46,6 +46,7 @@ def lib():
             "orc_get_field": (i, [vp, i, vp, l]),
             "orc_set_types": (None, [vp, i, i, i, i]),
             "orc_set_energy_costs": (None, [vp, d, d, d, d]),
+            "orc_set_ring_costs": (None, [vp, vp, vp, vp]),
             "orc_set_attraction_constants": (None, [vp, d, d]),
             "orc_set_lj_cutoff": (None, [vp, d]),
             "orc_set_fene_constants": (None, [vp, d, d]),
@@ -151,6 +152,14 @@ class Oracle:
 
     def set_energy_costs(self, ea, el, eb, ec):
         self.L.orc_set_energy_costs(self.h, ea, el, eb, ec)
+
+    def set_ring_costs(self, ka=None, kl=None, kb=None):
+        if ka is None:
+            self.L.orc_set_ring_costs(self.h, None, None, None)
+            return
+        arrs = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (self.np,)))
+                for a in (ka, kl, kb)]
+        self.L.orc_set_ring_costs(self.h, *[_ptr(a) for a in arrs])
 
     def set_attraction(self, l1, l2):
         self.L.orc_set_attraction_constants(self.h, l1, l2)

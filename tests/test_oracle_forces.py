@@ -189,3 +189,30 @@ def test_interaction_force_is_gradient(oracle_mod, pot, inter):
     # skip coordinates whose finite-difference stencil crosses a projection branch
     ok = err / np.abs(F).max() < 1e-7
     assert ok.mean() > 0.9, (err / np.abs(F).max())
+
+
+def test_ring_costs_gradient_and_uniform_limit(oracle_mod):
+    """Per-ring ka, kl, kb: F = -grad U with different costs on each ring, and uniform per-ring
+    costs reproduce the global ones exactly."""
+    st = packing.random_system(3, 0.35, 4)
+    rng = np.random.default_rng(5)
+    pos = st["pos"].reshape(-1, 2) + 0.02 * (rng.random((len(st["rad"]), 2)) - 0.5)
+    st["pos"] = pos.ravel()
+    st["a0"] = st["a0"] * np.array([1.05, 0.97, 1.02])
+    o = oracle_from_state(oracle_mod, st, costs=(1, 1, 0.01, 1), interaction="all")
+    o.calc_force_energy()
+    f_glob, u_glob = o.get("force"), o.potential_energy()
+    o.set_ring_costs([1, 1, 1], [1, 1, 1], [0.01, 0.01, 0.01])
+    o.calc_force_energy()
+    assert np.array_equal(o.get("force"), f_glob) and o.potential_energy() == u_glob
+    o.set_ring_costs([3.0, 0.5, 1.5], [2.0, 0.7, 1.1], [0.3, 0.02, 0.1])
+    p = st["pos"].copy()
+    o.set("pos", p)
+    o.calc_force_energy()
+    F = o.get("force")
+    assert np.max(np.abs(F - f_glob)) > 1e-3
+    idx = list(range(0, len(p), 5))
+    assert np.max(np.abs(F[idx] - fd_forces(o, p, idx))) / np.abs(F).max() < 1e-7
+    o.set_ring_costs()
+    o.calc_force_energy()
+    assert np.array_equal(o.get("force"), f_glob)
