@@ -260,6 +260,66 @@ def adjoint_bench(args, torch):
                         "gradients wrt x0, v0, per-ring ka/kl/kb and l0"}
 
 
+CFG5_SEED = 20261202 + 5
+
+
+def cfg5_bench(args, torch, batched=False):
+    """Large active + adhesive: rings of 32 vertices in two sizes (ratio 1.4), vertex-segment
+    interaction with the adhesive potential (l1 = 0.1, l2 = 0.2), active Brownian driving
+    f0 = v0 gamma with v0 = 0.1 and taup = 1/Dr = 100, phi 0.90.  One system of 16384 rings, or
+    (batched) 64 systems with N ~ U{2048..8192} in one set.  Compressed with harmonic FIRE from
+    phi 0.5 before timing."""
+    from cudaring_amd import DPMSet, autotune, packing, protocols
+    rng = np.random.default_rng(CFG5_SEED)
+    sizes = rng.integers(2048, 8193, 64) if batched else [16384]
+    systems = [packing.size_bidisperse_system(int(n), 0.5, CFG5_SEED + i) for i, n in enumerate(sizes)]
+    st = packing.concat_systems(systems)
+    t0 = time.perf_counter()
+    s = DPMSet(st["nvp"], rings_per_system=st["rings_per_system"])
+    s.load_state(st)
+    s.set_types("harmonic", "vertexVertex")
+    s.set_energy_costs(*COSTS)
+    protocols.compress(s, 0.5, 0.90, dphi=args.cfg5_dphi, iters=args.cfg5_iters)
+    sigma = float(2 * np.sqrt(np.mean(s.get("a0")) / np.pi))
+    s.set_types("adhesive", "vertexSmooth", particle="active")
+    s.set_attraction(0.1, 0.2)
+    s.set_self_propulsion(0.1 * GAMMA, 100.0 * sigma)
+    s.set("pangle", rng.uniform(-np.pi, np.pi, s.np))
+    s.set_dt(5e-3 * sigma)
+    s.set_seed(CFG5_SEED)
+    size = 2 * float(np.mean(s.get("rad")))
+    cut = s.set_displacement_cutoff(SKIN, size)
+    s.calc_neighbor_list(cut)
+    s.calc_force_energy()
+    s.init_langevin(1e-4, GAMMA)
+    waves, wtimes = autotune.tune_waves_per_tile(s, reps=20)
+    prep = time.perf_counter() - t0
+    for _ in range(2):
+        s.langevin(CHUNK)
+    s.synchronize()
+    s.reset_update_count()
+    t0 = time.perf_counter()
+    done = 0
+    while done < args.cfg5_steps:
+        n = min(CHUNK, args.cfg5_steps - done)
+        s.langevin(n)
+        done += n
+    s.synchronize()
+    el = time.perf_counter() - t0
+    us, kbar = s.time_step_kernel(50)
+    alg = alg_bytes_per_step(s.nv, s.np, kbar)
+    out = {"vertex_steps_per_s": s.nv * args.cfg5_steps / el, "vertices": s.nv, "rings": s.np,
+           "systems": len(sizes), "steps": args.cfg5_steps, "step_us": round(1e6 * el / args.cfg5_steps, 2),
+           "kernel_us": round(us, 2), "mean_neighbors": round(kbar, 3),
+           "achieved_GBps": round(alg / (us * 1e-6) / 1e9, 1),
+           "frac_hbm": round(alg / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+           "rebuilds": [int(x) for x in s.update_count()[:4]], "T_end": float(np.mean(s.temperature())),
+           "prep_s": round(prep, 1), "waves_per_tile": waves,
+           "finite": bool(np.all(np.isfinite(s.get("pos"))))}
+    s.close()
+    return out
+
+
 COPY_ELEMS = 1 << 27  # float64 elements of the calibration copy: 1 GiB read + 1 GiB written
 
 
@@ -294,6 +354,11 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--fire-only", action="store_true")
     ap.add_argument("--adjoint-only", action="store_true")
+    ap.add_argument("--cfg5", action="store_true", help="large active + adhesive system (16384 rings)")
+    ap.add_argument("--cfg5-batched", action="store_true", help="64 active + adhesive systems in one set")
+    ap.add_argument("--cfg5-steps", type=int, default=2000)
+    ap.add_argument("--cfg5-dphi", type=float, default=0.05)
+    ap.add_argument("--cfg5-iters", type=int, default=400)
     ap.add_argument("--traffic", action="store_true",
                     help="counter-collection run: a copy of known size, then step-kernel launches only")
     args = ap.parse_args()
@@ -308,6 +373,9 @@ def main():
         return
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_bench(args, torch)}), flush=True)
+        return
+    if args.cfg5 or args.cfg5_batched:
+        print(json.dumps({"active_adhesive": cfg5_bench(args, torch, batched=args.cfg5_batched)}), flush=True)
         return
     s, st, dt, prep_s, tuned = prepare(args)
     nv, np_ = s.nv, s.np

@@ -87,6 +87,29 @@ def lattice_system(nrings, phi, seed, n_small=16, n_large=22, calA0=1.10, pertur
             "theta0": theta0, "a0": a0}
 
 
+def size_bidisperse_system(nrings, phi, seed, n=32, ratio=1.4, calA0=1.10, perturb=0.1):
+    """Rings of equal vertex count n in two sizes (area ratio ratio^2, 50/50), so l0 and the vertex
+    radius differ between the classes; jittered square lattice at preferred phi."""
+    rng = np.random.default_rng(seed)
+    nvp = np.full(nrings, n, dtype=np.int64)
+    big = rng.permutation(nrings) < nrings // 2
+    a0 = np.where(big, ratio * ratio, 1.0)
+    c = calA0 * n * np.tan(np.pi / n) / np.pi
+    l0r = 2.0 * np.sqrt(np.pi * c * a0) / n
+    l0 = np.repeat(l0r, n)
+    theta0 = np.full(nrings * n, 2.0 * np.pi / n)
+    rad = 0.5 * l0
+    L = np.sqrt(preferred_phi_area(nvp, a0, rad) / phi)
+    m = int(np.ceil(np.sqrt(nrings)))
+    a = L / m
+    idx = rng.permutation(m * m)[:nrings]
+    centers = np.stack([(idx % m + 0.5) * a, (idx // m + 0.5) * a], axis=1)
+    centers += perturb * a * (rng.random((nrings, 2)) - 0.5)
+    pos = place_rings(nvp, a0, centers, rng=rng, jitter=1e-3, l0=l0)
+    return {"nvp": nvp, "box": np.array([L, L]), "pos": pos.ravel(), "rad": rad, "l0": l0,
+            "theta0": theta0, "a0": a0}
+
+
 def concat_systems(systems):
     """Stack systems into one set: nvp, rings_per_system and concatenated fields."""
     out = {"nvp": np.concatenate([s["nvp"] for s in systems]),

@@ -23,6 +23,13 @@
  *   integrators         NVE time reversibility and O(dt^2) energy drift
  *   FIRE                convergence to |F|max < 1e-12 on a harmonic packing
  *   RNG                 Philox4x32-10 known-answer vectors (Random123)
+ *   per-ring costs      (an addition) finite differences with distinct costs per ring;
+ *                       uniform per-ring costs equal the global ones bit for bit
+ *   Langevin            per-ring noise pinned through the shared Philox stream and
+ *                       <K> against T; the reference's own Thrust/time(0) streams are
+ *                       not reproducible even by the reference: parity unpinned by design
+ * The set-up draws (drand48 order of setPolySizeDistribution, setRandomParticles,
+ * initVerticesOnParticles) are restated in oracle/setup_oracle.py.
  */
 #ifndef DPM_ORACLE_H
 #define DPM_ORACLE_H

@@ -219,3 +219,27 @@ def test_batched_fire_minimize(oracle_mod):
         ito = o.fire_minimize(1e-10, 100000)
         assert abs(int(it[s]) - ito) <= max(5, 0.02 * ito), (it[s], ito)
         assert U[s] == pytest.approx(o.potential_energy(), rel=1e-9)
+
+
+def test_cfg1_nve_full_length(oracle_mod):
+    """The plain NVE configuration at full length: 32 bidisperse rings (16/22, calA0 1.10,
+    ka = kl = 1, kb = 0.01), harmonic vertex-vertex, phi 0.80 with uniform random centres,
+    T = 1e-3 with zero net momentum, dt = 0.01 sigma, skin 0.3 sigma, 1e4 steps.  The first 100
+    steps match the oracle step for step; over 1e4 steps both conserve energy to the same
+    O(dt^2) level."""
+    print("case cfg1_nve", flush=True)
+    st = packing.random_system(32, 0.80, 20261202 + 1)
+    size = 2 * float(np.mean(st["rad"]))
+    sigma = 2 * np.sqrt(np.mean(st["a0"]) / np.pi)
+    o, g = prepare(oracle_mod, st, "harmonic", "vertexVertex", skin=0.3, dt=0.01 * sigma)
+    o.calc_force_energy(); g.calc_force_energy()
+    o.init_nve(1e-3, zero_momentum=True); g.init_nve(1e-3, zero_momentum=True)
+    e0 = o.potential_energy() + o.kinetic_energy()
+    o.nve(100); g.nve(100)
+    assert rel_err(g.get("pos"), o.get("pos"), scale=1.0) < 1e-11
+    o.nve(9900); g.nve(9900)
+    eo = o.potential_energy() + o.kinetic_energy()
+    eg = g.potential_energy()[0] + g.kinetic_energy()[0]
+    # the random start releases its overlap energy into heat; E stays within 1% of E0
+    assert abs(eo - e0) < 1e-2 * abs(e0) and abs(eg - e0) < 1e-2 * abs(e0)
+    assert np.all(np.isfinite(g.get("pos")))
