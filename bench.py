@@ -141,6 +141,8 @@ def batched_fire(args):
     relax_s = time.perf_counter() - t0
     start = {k: s.get(k) for k in ("pos", "rad", "l0", "a0")}
     nv_sys = np.array([int(np.sum(x["nvp"])) for x in systems])
+    from cudaring_amd import autotune
+    fire_kernel, fire_tune = autotune.tune_fire(s, FIRE_PARAMS, cut)
 
     s.synchronize()
     t0 = time.perf_counter()
@@ -186,7 +188,9 @@ def batched_fire(args):
             "plain_sample": f"{args.fire_plain} of the {FIRE_SYSTEMS} systems, same start and phi steps, "
                             f"one DPM2D-style set each, host check per iteration: {plain_iters} iterations "
                             f"in {t_plain:.2f} s",
-            "start_relax_iterations_max": int(it0.max()), "start_relax_s": round(relax_s, 2)}
+            "start_relax_iterations_max": int(it0.max()), "start_relax_s": round(relax_s, 2),
+            "fire_kernel": {"persistent": bool(fire_kernel[0]), "waves": int(fire_kernel[1])},
+            "fire_tune_s": fire_tune}
 
 
 ADJ_RINGS, ADJ_PHI, ADJ_SEED, ADJ_T = 256, 0.85, 20261202 + 4, 1e-3

@@ -46,6 +46,33 @@ def tune_skin(s, size, candidates=(0.2, 0.3, 0.5, 0.8), steps=400, kind="langevi
     return best, times
 
 
+def tune_fire(s, params7, cut, candidates=((1, 2), (1, 4), (1, 8), (0, 0)), iters=200):
+    """Pick the fire_minimize kernel (persistent workgroup per system with 2/4/8 waves, or graphs
+    of the split step kernels) by timing `iters` iterations of each from the caller's state,
+    which is restored after every trial and at the end."""
+    keep = {k: s.get(k) for k in ("pos", "vel")}
+    times = {}
+    for persistent, waves in candidates:
+        for k, v in keep.items():
+            s.set(k, v)
+        s.calc_neighbor_list(cut)
+        s.calc_force_energy()
+        s.init_fire(params7, 20, 0)
+        s.set_fire_kernel(persistent, waves)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.fire_minimize(0.0, iters)
+        s.synchronize()
+        times[(persistent, waves)] = time.perf_counter() - t0
+    best = min(times, key=times.get)
+    for k, v in keep.items():
+        s.set(k, v)
+    s.calc_neighbor_list(cut)
+    s.calc_force_energy()
+    s.set_fire_kernel(*best)
+    return best, {f"{'persistent' if p else 'graph'}{w or ''}": round(t, 4) for (p, w), t in times.items()}
+
+
 def summary(times):
     return {str(k): round(float(v), 4) for k, v in sorted(times.items())}
 

@@ -1157,6 +1157,29 @@ int dpm_fire_minimize(dpm_set_t h, double tol, long maxIter, long* iterations) {
   }
   HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
   long done = 0;
+  if (S->fireKernel == 1 && S->P.particle == PART_PASSIVE) {
+    // one workgroup per system, at most `pchunk` iterations per launch
+    rc = clean_protocol(S);
+    if (rc) return rc;
+    int waves = S->fireWaves;
+    if (waves <= 0) {
+      int maxTiles = 1;
+      for (long s = 0; s < S->nsys; s++) maxTiles = std::max(maxTiles, S->h_stile[s + 1] - S->h_stile[s]);
+      waves = std::min(8, maxTiles);
+    }
+    const long pchunk = 2048;
+    while (done <= maxIter) {
+      HIPCHK(launch_fire_sys(S, (int)pchunk, waves));
+      done += pchunk;
+      int notDone = 0;
+      HIPCHK(launch_fill_int(S->d_notDone, 0, 1, S->stream));
+      HIPCHK(launch_fire_done(S, S->d_notDone));
+      HIPCHK(hipMemcpyAsync(&notDone, S->d_notDone, sizeof(int), hipMemcpyDeviceToHost, S->stream));
+      HIPCHK(hipStreamSynchronize(S->stream));
+      if (!notDone) break;
+    }
+    done = maxIter + 1;  // the graph path below is skipped
+  }
   const long chunk = 256;
   while (done <= maxIter) {
     rc = run_fire(S, chunk);
@@ -1177,6 +1200,15 @@ int dpm_fire_minimize(dpm_set_t h, double tol, long maxIter, long* iterations) {
     for (long s = 0; s < S->nsys; s++) iterations[s] = fs[s].iters;
   }
   return check_err_after_loop(S);
+}
+
+int dpm_set_fire_kernel(dpm_set_t h, int persistent, int wavesPerSystem) {
+  CHECK_HANDLE(h);
+  if (persistent < 0 || persistent > 1 || wavesPerSystem < 0 || wavesPerSystem > 8)
+    return fail(S, DPM_ERR_ARG, "dpm_set_fire_kernel: persistent in {0, 1}, waves in 0..8");
+  S->fireKernel = persistent;
+  S->fireWaves = wavesPerSystem;
+  return DPM_OK;
 }
 
 int dpm_get_fire_state(dpm_set_t h, double* out) {

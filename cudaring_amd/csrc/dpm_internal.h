@@ -133,6 +133,8 @@ struct dpm_set_s {
   dpm::FireParams fp{};
   double* d_part = nullptr;  // per-tile partials [ntiles * 4]
   bool fireInit = false;
+  int fireKernel = 1;   // dpm_fire_minimize: 1 = one persistent workgroup per system, 0 = graph of step kernels
+  int fireWaves = 0;    // waves per system workgroup (0 = from the largest system's tile count)
   int* d_notDone = nullptr;
 
   // reductions
@@ -181,6 +183,7 @@ hipError_t launch_fire_step(dpm_set_s* S);
 hipError_t launch_fire_done(dpm_set_s* S, int* notDone);
 StepArgs loop_args(dpm_set_s* S);
 hipError_t launch_bin_view(dpm_set_s* S, const View& Vn, int parity);
+hipError_t launch_fire_sys(dpm_set_s* S, int chunk, int waves);
 hipError_t launch_adj_area(dpm_set_s* S, const View& Vn, const double* wx, const double* wy, double* A, double* dA);
 hipError_t launch_adj_flag(dpm_set_s* S, const View& Vn, int parity);
 hipError_t launch_adj_hvp(dpm_set_s* S, const View& Vn, AdjArgs A);

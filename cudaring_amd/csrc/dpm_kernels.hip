@@ -221,6 +221,53 @@ __global__ void k_build_rows(View V, Params P, CellGrid G, const int* count, con
   build_row(V, P, G, count, L.v, L.s, box_of(V, L.s), err, maxcnt);
 }
 
+// Interparticle gather of lane L's vertex over entries k0, k0 + kstep, ... of its row:
+// accumulates the force on v, the vertex-vertex energy e, the smooth energy es and the share
+// (qx, qy) that v's `other` evaluation owes prev(v) (handed to prev(v)'s lane by the caller).
+template <int INTER, int POT>
+__device__ __forceinline__ void gather_pairs(const View& V, const Params& P, const Lane& L, const Box& B, double x,
+                                             double y, int k0, int kstep, double& fx, double& fy, double& e,
+                                             double& es, double& qx, double& qy) {
+  const double radv = V.rad[L.v];
+  if constexpr (INTER == INT_VV) {
+    // kernelCalcVertexInteraction, cudaKernel.cuh:690-722
+    const int c = L.act ? V.ncnt[L.v] : 0;
+    for (int k = k0; k < c; k += kstep) {
+      const int j = V.nbr[(long)k * V.nv + L.v];
+      e += pair_force<POT>(P, pbx(x - V.x[j], B), pby(y - V.y[j], B), radv + V.rad[j], fx, fy);
+    }
+  } else if constexpr (INTER == INT_SMOOTH) {
+    // kernelCalcSmoothInteraction, cudaKernel.cuh:922-975, as a gather: for every u in list(v),
+    // (v, segment of u) gives v's `this` share, and (u, segment (prev(v), v)) gives v's
+    // `other` share and prev(v)'s `previous` share (handed to prev(v)'s lane by the caller).
+    // Ring neighbours of v; inactive lanes past the tile end use v itself (in bounds).
+    const int vp = L.act ? L.v0 + L.pl : L.v;
+    const int vpp = L.act ? L.v0 + ((L.pl > L.segS) ? L.pl - 1 : L.segE) : L.v;
+    const double xp = V.x[vp], yp = V.y[vp];
+    const double xpp = V.x[vpp], ypp = V.y[vpp];
+    const int c = L.act ? V.ncnt[L.v] : 0;
+    for (int k = k0; k < c; k += kstep) {
+      const long idx = (long)k * V.nv + L.v;
+      const int u = V.nbr[idx];
+      const int2 pp = V.nbrp[idx];
+      const double xu = V.x[u], yu = V.y[u];
+      const double radSum = radv + V.rad[u];
+      es += smooth_pair<POT, ROLE_THIS>(P, B, x, y, xu, yu, V.x[pp.x], V.y[pp.x], V.x[pp.y], V.y[pp.y], radSum,
+                                        fx, fy);
+      es += smooth_pair_op<POT>(P, B, xu, yu, x, y, xp, yp, xpp, ypp, radSum, fx, fy, qx, qy);
+    }
+  } else {
+    // kernelCalcAllToAllVertexInteraction, cudaKernel.cuh:725-757 (test mode)
+    if (L.act) {
+      const int b0 = V.svoff[L.s], b1 = V.svoff[L.s + 1];
+      for (int j = b0 + k0; j < b1; j += kstep) {
+        if (V.vring[j] == L.r) continue;
+        e += pair_force<POT>(P, pbx(x - V.x[j], B), pby(y - V.y[j], B), radv + V.rad[j], fx, fy);
+      }
+    }
+  }
+}
+
 // -------------------------------------------------------------- step ---
 // Waves of a tile: with nw > 1, wave 0 does the ring geometry, shape force and random draws
 // while waves 1..nw-1 share the neighbour gather; with nw == 1 wave 0 does both in turn.
@@ -248,46 +295,7 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
   double fx = 0, fy = 0, e = 0, es = 0, qx = 0, qy = 0;
   const int g0 = nw > 1 ? 1 : 0;
   const int ng = nw - g0;
-  if (w >= g0) {
-    const double radv = V.rad[L.v];
-    if constexpr (INTER == INT_VV) {
-      // kernelCalcVertexInteraction, cudaKernel.cuh:690-722
-      const int c = L.act ? V.ncnt[L.v] : 0;
-      for (int k = w - g0; k < c; k += ng) {
-        const int j = V.nbr[(long)k * V.nv + L.v];
-        e += pair_force<POT>(P, pbx(x - V.x[j], B), pby(y - V.y[j], B), radv + V.rad[j], fx, fy);
-      }
-    } else if constexpr (INTER == INT_SMOOTH) {
-      // kernelCalcSmoothInteraction, cudaKernel.cuh:922-975, as a gather: for every u in list(v),
-      // (v, segment of u) gives v's `this` share, and (u, segment (prev(v), v)) gives v's
-      // `other` share and prev(v)'s `previous` share (handed to prev(v)'s lane below).
-      // Ring neighbours of v; inactive lanes past the tile end use v itself (in bounds).
-      const int vp = L.act ? L.v0 + L.pl : L.v;
-      const int vpp = L.act ? L.v0 + ((L.pl > L.segS) ? L.pl - 1 : L.segE) : L.v;
-      const double xp = V.x[vp], yp = V.y[vp];
-      const double xpp = V.x[vpp], ypp = V.y[vpp];
-      const int c = L.act ? V.ncnt[L.v] : 0;
-      for (int k = w - g0; k < c; k += ng) {
-        const long idx = (long)k * V.nv + L.v;
-        const int u = V.nbr[idx];
-        const int2 pp = V.nbrp[idx];
-        const double xu = V.x[u], yu = V.y[u];
-        const double radSum = radv + V.rad[u];
-        es += smooth_pair<POT, ROLE_THIS>(P, B, x, y, xu, yu, V.x[pp.x], V.y[pp.x], V.x[pp.y], V.y[pp.y], radSum,
-                                          fx, fy);
-        es += smooth_pair_op<POT>(P, B, xu, yu, x, y, xp, yp, xpp, ypp, radSum, fx, fy, qx, qy);
-      }
-    } else {
-      // kernelCalcAllToAllVertexInteraction, cudaKernel.cuh:725-757 (test mode)
-      if (L.act) {
-        const int b0 = V.svoff[L.s], b1 = V.svoff[L.s + 1];
-        for (int j = b0 + w - g0; j < b1; j += ng) {
-          if (V.vring[j] == L.r) continue;
-          e += pair_force<POT>(P, pbx(x - V.x[j], B), pby(y - V.y[j], B), radv + V.rad[j], fx, fy);
-        }
-      }
-    }
-  }
+  if (w >= g0) gather_pairs<INTER, POT>(V, P, L, B, x, y, w - g0, ng, fx, fy, e, es, qx, qy);
   if (nw > 1 && w > 0) {
     red[w][L.lane][0] = fx;
     red[w][L.lane][1] = fy;
@@ -676,6 +684,205 @@ __global__ void k_fire_done(const FireState* fs, int nsys, int* notDone) {
   if (s < nsys && !fs[s].converged) atomicOr(notDone, 1);
 }
 
+// ------------------------------------------------------- persistent FIRE ---
+// One workgroup per system runs whole FIRE iterations (FIRE::minimizerVertexLoop,
+// FIRE.cu:397-411) back to back, separated by workgroup barriers only: systems share nothing,
+// so no grid-wide synchronisation is needed and a converged system's workgroup simply exits.
+// Each wave walks the system's ring-aligned tiles; the force phase is the step kernel's
+// (ring_shape + gather_pairs).  At most `chunk` iterations per launch; the loop exit is
+// decided by thread 0 and read by every thread after a barrier, so all waves leave together.
+#define FS_WAVES 8
+
+// workgroup sum of a and maximum of b (fixed order: lanes by butterfly, waves by index)
+__device__ __forceinline__ void block_sum_max(double a, double b, double (*red)[2], double& sa, double& mb) {
+  const int w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+  a = wave_sum(a);
+  b = wave_max(b);
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    red[w][0] = a;
+    red[w][1] = b;
+  }
+  __syncthreads();
+  double s = 0, m = -1;
+  for (int q = 0; q < nw; q++) {
+    s += red[q][0];
+    m = fmax(m, red[q][1]);
+  }
+  sa = s;
+  mb = m;
+  __syncthreads();
+}
+
+// P = v.F and max F_c^2 over the system's vertices
+__device__ __forceinline__ void fire_power(const View& V, int t0, int t1, double (*red)[2], double& P, double& Fm) {
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+  double p = 0, m = -1;
+  for (int t = t0 + w; t < t1; t += nw) {
+    const int v = V.tile_v0[t] + lane;
+    if (v < V.tile_v1[t]) {
+      const double fx = V.fx[v], fy = V.fy[v];
+      p += V.vx[v] * fx + V.vy[v] * fy;
+      m = fmax(m, fmax(fx * fx, fy * fy));
+    }
+  }
+  block_sum_max(p, m, red, P, Fm);
+}
+
+template <int INTER, int POT>
+__global__ __launch_bounds__(WAVE* FS_WAVES) void k_fire_sys(View V, Params P, FireParams fp, CellGrid G,
+                                                             FireState* fst, int chunk, int* updateCount, int* err,
+                                                             int* maxcnt) {
+  __shared__ double red[FS_WAVES][2];
+  __shared__ FireState sfs;
+  __shared__ int shStop, shReset;
+  const int s = blockIdx.x;
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+  const int t0 = V.stile[s], t1 = V.stile[s + 1];
+  const Box B = box_of(V, s);
+  if (threadIdx.x == 0) sfs = fst[s];
+  double Pv, Fm;
+  fire_power(V, t0, t1, red, Pv, Fm);  // also publishes sfs (barrier inside)
+  for (int it = 0; it < chunk; it++) {
+    // ---- checkFIREParams (FIRE.cu:338-371) and the stopping rule, thread 0
+    if (threadIdx.x == 0) {
+      FireState fs = sfs;
+      int stop = fs.converged;
+      if (!stop && (sqrt(Fm) <= fp.tol || fs.iters >= fp.maxIter)) {
+        fs.converged = 1;
+        stop = 1;
+      }
+      int reset = 0;
+      if (!stop) {
+        fs.P = Pv;
+        if (Pv > 0) {
+          fs.numStep++;
+          if (fs.numStep > fp.minStep) {
+            fs.dt = fmin(fs.dt * fp.f_inc, fp.dt_max);
+            fs.a *= fp.f_a;
+          }
+        } else {
+          fs.numStep = 0;
+          fs.dt = fmax(fs.dt * fp.f_dec, fp.dt_max / 10);
+          fs.a = fp.a_start;
+          reset = 1;
+        }
+        fs.iters++;
+      }
+      sfs = fs;
+      shStop = stop;
+      shReset = reset;
+    }
+    __syncthreads();
+    if (shStop) break;
+    const double dt = sfs.dt, h = 0.5 * dt, fa = sfs.a;
+    const int reset = shReset;
+    // ---- back-step on reset (with the decreased dt), half kick, |v|^2 and |F|^2
+    double vv = 0, ff = 0;
+    for (int t = t0 + w; t < t1; t += nw) {
+      const int v = V.tile_v0[t] + lane;
+      if (v < V.tile_v1[t]) {
+        double vx = V.vx[v], vy = V.vy[v];
+        const double fx = V.fx[v], fy = V.fy[v];
+        if (reset) {
+          V.x[v] -= h * vx;
+          V.y[v] -= h * vy;
+          vx = 0;
+          vy = 0;
+        }
+        vx += h * fx;
+        vy += h * fy;
+        V.vx[v] = vx;
+        V.vy[v] = vy;
+        vv += vx * vx + vy * vy;
+        ff += fx * fx + fy * fy;
+      }
+    }
+    double VV, FF, dummy;
+    block_sum_max(vv, -1.0, red, VV, dummy);
+    block_sum_max(ff, -1.0, red, FF, dummy);
+    // ---- bendVertexVelocityTowardsForce (FIRE.cu:374-394), drift, displacement test
+    int moved = 0;
+    for (int t = t0 + w; t < t1; t += nw) {
+      const int v = V.tile_v0[t] + lane;
+      if (v < V.tile_v1[t]) {
+        double vx = V.vx[v], vy = V.vy[v];
+        if (FF > 0) {
+          const double ratio = sqrt(VV / FF);
+          vx = (1 - fa) * vx + fa * V.fx[v] * ratio;
+          vy = (1 - fa) * vy + fa * V.fy[v] * ratio;
+        }
+        const double xn = V.x[v] + dt * vx, yn = V.y[v] + dt * vy;
+        V.vx[v] = vx;
+        V.vy[v] = vy;
+        V.x[v] = xn;
+        V.y[v] = yn;
+        const double ddx = pbx(xn - V.lx[v], B), ddy = pby(yn - V.ly[v], B);
+        if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) moved = 1;
+      }
+    }
+    // ---- checkNeighbors: rebuild the system's rows from cells binned here (DPM2D.cu:1408-1423)
+    if (__syncthreads_or(moved)) {
+      const int ncx = G.ncx[s], ncy = G.ncy[s];
+      int* count = G.count[0];
+      if (!G.brute[s]) {
+        for (int t = t0 + w; t < t1; t += nw) {
+          const int v = V.tile_v0[t] + lane;
+          if (v < V.tile_v1[t]) {
+            int cx, cy;
+            cell_of(V.x[v], V.y[v], B, ncx, ncy, cx, cy);
+            const int c = G.coff[s] + cy * ncx + cx;
+            const int slot = atomicAdd(&count[c], 1);
+            if (slot < G.cap) G.slots[(long)c * G.cap + slot] = v;
+            else atomicOr(err, ERR_CELL_OVERFLOW);
+          }
+        }
+      }
+      __syncthreads();
+      for (int t = t0 + w; t < t1; t += nw) {
+        const int v = V.tile_v0[t] + lane;
+        if (v < V.tile_v1[t]) build_row(V, P, G, count, v, s, B, err, maxcnt);
+      }
+      __syncthreads();
+      if (!G.brute[s])
+        for (int c = threadIdx.x; c < ncx * ncy; c += blockDim.x) count[G.coff[s] + c] = 0;
+      if (threadIdx.x == 0) updateCount[s] += 1;
+    }
+    // ---- calcForceEnergy (shape + interaction) and the second half kick
+    double p = 0, m = -1;
+    for (int t = t0 + w; t < t1; t += nw) {
+      Lane L;
+      lane_ctx(V, t, lane, L);
+      const double x = V.x[L.v], y = V.y[L.v];
+      double fx = 0, fy = 0, e = 0, es = 0, qx = 0, qy = 0;
+      gather_pairs<INTER, POT>(V, P, L, B, x, y, 0, 1, fx, fy, e, es, qx, qy);
+      const ShapeOut so = ring_shape(V, P, L, B, x, y, true);
+      if constexpr (INTER == INT_SMOOTH) {
+        fx += shfl_d(qx, L.nl);
+        fy += shfl_d(qy, L.nl);
+        const double pe = seg_sum(L.act ? es : 0.0, L.lane, L.segS, L.segE);
+        if (L.act) {
+          V.es[L.v] = es;
+          if (L.lane == L.segS) V.pe[L.r] = pe;
+        }
+      }
+      fx = so.fx + fx;
+      fy = so.fy + fy;
+      if (L.act) {
+        const double vx = V.vx[L.v] + h * fx, vy = V.vy[L.v] + h * fy;
+        V.fx[L.v] = fx;
+        V.fy[L.v] = fy;
+        V.ev[L.v] = so.e + e;
+        V.vx[L.v] = vx;
+        V.vy[L.v] = vy;
+        p += vx * fx + vy * fy;
+        m = fmax(m, fmax(fx * fx, fy * fy));
+      }
+    }
+    block_sum_max(p, m, red, Pv, Fm);
+  }
+  if (threadIdx.x == 0) fst[s] = sfs;
+}
+
 // --------------------------------------------------------- launchers ---
 static inline int grid_v(long n) { return (int)((n + 255) / 256); }
 static inline int grid_w(int ntiles) { return (ntiles + 3) / 4; }
@@ -832,6 +1039,32 @@ hipError_t launch_fire_mid(dpm_set_s* S) {
 hipError_t launch_fire_partials(dpm_set_s* S) {
   k_fire_partials<<<grid_w(S->V.ntiles), 256, 0, S->stream>>>(S->V, S->d_part);
   return hipGetLastError();
+}
+
+template <int INTER, int POT>
+static hipError_t fire_sys_t(dpm_set_s* S, int chunk, int threads) {
+  k_fire_sys<INTER, POT><<<(unsigned)S->nsys, threads, 0, S->stream>>>(
+      S->V, S->P, S->fp, S->G, S->d_fire[S->fireCur], chunk, S->d_updateCount, S->d_err, S->d_maxcnt);
+  return hipGetLastError();
+}
+
+template <int INTER>
+static hipError_t fire_sys_pot(dpm_set_s* S, int chunk, int threads) {
+  switch (S->P.potential) {
+    case POT_HARMONIC: return fire_sys_t<INTER, POT_HARMONIC>(S, chunk, threads);
+    case POT_LJ: return fire_sys_t<INTER, POT_LJ>(S, chunk, threads);
+    case POT_ADHESIVE: return fire_sys_t<INTER, POT_ADHESIVE>(S, chunk, threads);
+    default: return fire_sys_t<INTER, POT_WCA>(S, chunk, threads);
+  }
+}
+
+hipError_t launch_fire_sys(dpm_set_s* S, int chunk, int waves) {
+  const int threads = WAVE * (waves < 1 ? 1 : (waves > FS_WAVES ? FS_WAVES : waves));
+  switch (S->P.interaction) {
+    case INT_VV: return fire_sys_pot<INT_VV>(S, chunk, threads);
+    case INT_SMOOTH: return fire_sys_pot<INT_SMOOTH>(S, chunk, threads);
+    default: return fire_sys_pot<INT_ALL>(S, chunk, threads);
+  }
 }
 
 hipError_t launch_fire_done(dpm_set_s* S, int* notDone) {

@@ -65,6 +65,7 @@ SIGNATURES = {
     "dpm_fire_loop": (_i, [_vp, _l]),
     "dpm_get_fire_state": (_i, [_vp, _vp]),
     "dpm_fire_minimize": (_i, [_vp, _d, _l, _vp]),
+    "dpm_set_fire_kernel": (_i, [_vp, _i, _i]),
     "dpm_get_potential_energy": (_i, [_vp, _vp]),
     "dpm_get_kinetic_energy": (_i, [_vp, _vp]),
     "dpm_get_temperature": (_i, [_vp, _vp]),
@@ -294,6 +295,10 @@ class DPMSet:
         o = o.reshape(self.nsys, 5)
         return {"dt": o[:, 0], "alpha": o[:, 1], "numStep": o[:, 2].astype(np.int64), "P": o[:, 3],
                 "iters": o[:, 4].astype(np.int64)}
+
+    def set_fire_kernel(self, persistent=True, waves=0):
+        """fire_minimize on one persistent workgroup per system, or on graphs of step kernels."""
+        self._chk(self.L.dpm_set_fire_kernel(self.h, int(bool(persistent)), int(waves)), "set_fire_kernel")
 
     def fire_minimize(self, tol, max_iter):
         it = np.zeros(self.nsys, dtype=np.int64)

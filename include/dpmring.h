@@ -158,6 +158,11 @@ int dpm_adjust_temperature(dpm_set_t h, double T);
 int dpm_init_fire(dpm_set_t h, const double* params7, long minStep, long numStep);
 /* vertexFIRELoop() x iterations, DPM2D.h:669 / FIRE.cu:397-411 (all systems advance) */
 int dpm_fire_loop(dpm_set_t h, long iterations);
+/* Kernel of dpm_fire_minimize: persistent = 1 (default) runs one workgroup per system that
+ * iterates with workgroup barriers only (wavesPerSystem 1..8, 0 = from the largest system);
+ * persistent = 0 replays graphs of the split FIRE step kernels with per-system freezing.
+ * No reference counterpart (launch configuration, chosen by the autotuner). */
+int dpm_set_fire_kernel(dpm_set_t h, int persistent, int wavesPerSystem);
 /* FIRE state per system: out[5*s + {0..4}] = fire_dt, a, numStep, last P = v.F, iterations */
 int dpm_get_fire_state(dpm_set_t h, double* out);
 /* The compressFIRE.cpp:121-131 loop for every system at once: iterate each system until

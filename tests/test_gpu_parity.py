@@ -192,8 +192,9 @@ def test_split_invariance(oracle_mod, inter):
         assert rel_err(g.get("force"), ref) < TOL_F, split
 
 
-def test_batched_fire_minimize(oracle_mod):
-    print("case batched_fire_minimize", flush=True)
+@pytest.mark.parametrize("persistent", [True, False])
+def test_batched_fire_minimize(oracle_mod, persistent):
+    print(f"case batched_fire_minimize persistent={persistent}", flush=True)
     # square ring counts on a dilute lattice: unjammed, every system converges
     systems = [packing.lattice_system(n, 0.5, 300 + n, perturb=0.05) for n in (4, 9, 16, 25)]
     cat = packing.concat_systems(systems)
@@ -206,6 +207,7 @@ def test_batched_fire_minimize(oracle_mod):
     dt = 0.01 * 2.0
     prm = [0.2, 0.5, 1.1, 0.99, dt, 10 * dt, 0.2]
     g.init_fire(prm, 20, 0)
+    g.set_fire_kernel(persistent)
     it = g.fire_minimize(1e-10, 100000)
     fmax = g.max_unbalanced_force()
     assert np.all(fmax <= 1e-10), fmax
@@ -243,3 +245,35 @@ def test_cfg1_nve_full_length(oracle_mod):
     # the random start releases its overlap energy into heat; E stays within 1% of E0
     assert abs(eo - e0) < 1e-2 * abs(e0) and abs(eg - e0) < 1e-2 * abs(e0)
     assert np.all(np.isfinite(g.get("pos")))
+
+
+@pytest.mark.parametrize("pot,inter", [("harmonic", "vertexVertex"), ("wca", "vertexSmooth")])
+def test_persistent_fire_matches_step_kernels(pot, inter):
+    """fire_minimize on one workgroup per system (tol 0, 60 iterations) against 60 iterations of
+    the split FIRE step kernels, on a batched set with rebuilds along the way."""
+    print(f"case persistent_fire {pot} {inter}", flush=True)
+    systems = [relaxed_state(n, 0.8, 500 + n, final=(pot, inter)) for n in (9, 16)]
+    cat = packing.concat_systems(systems)
+    sets = []
+    for _ in range(2):
+        g = gpu_from_state(cat, potential=pot, interaction=inter, costs=(1, 1, 0.1, 1),
+                           rings_per_system=cat["rings_per_system"])
+        cut = g.set_displacement_cutoff(0.1, 2 * np.mean(cat["rad"]))
+        g.calc_neighbor_list(cut)
+        g.calc_force_energy()
+        dt = 0.02 * 2 * np.sqrt(np.mean(cat["a0"]) / np.pi)
+        g.init_fire([0.2, 0.5, 1.1, 0.99, dt, 10 * dt, 0.2], 20, 0)
+        sets.append(g)
+    a, b = sets
+    a.set_fire_kernel(True)
+    it = a.fire_minimize(0.0, 60)
+    b.fire(60)
+    assert list(it) == [60, 60]
+    fa, fb = a.fire_state(), b.fire_state()
+    assert list(fa["numStep"]) == list(fb["numStep"])
+    np.testing.assert_allclose(fa["dt"], fb["dt"], rtol=1e-13)
+    np.testing.assert_allclose(fa["alpha"], fb["alpha"], rtol=1e-13)
+    assert rel_err(a.get("pos"), b.get("pos"), scale=1.0) < 1e-11
+    assert rel_err(a.get("vel"), b.get("vel")) < 1e-8
+    assert rel_err(a.get("force"), b.get("force")) < 1e-8
+    assert int(a.update_count().sum()) == int(b.update_count().sum()) and int(a.update_count().sum()) > 0
