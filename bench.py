@@ -58,6 +58,15 @@ def prepare(args):
     return s, st, dt, time.perf_counter() - t0, {"waves_per_tile": waves, "kernel_us": autotune.summary(wtimes)}
 
 
+def run_steps(s, steps):
+    """`steps` Langevin steps as graph replays of CHUNK steps."""
+    done = 0
+    while done < steps:
+        n = min(CHUNK, steps - done)
+        s.langevin(n)
+        done += n
+
+
 def cpu_baseline(st, dt, steps):
     """The oracle (single thread) on `steps` Langevin steps of the same prepared state."""
     from oracle import oracle
@@ -84,7 +93,11 @@ def cpu_baseline(st, dt, steps):
 
 FIRE_SYSTEMS, FIRE_SEED = 512, 20261202 + 3
 FIRE_PARAMS = (0.1, 0.5, 1.1, 0.99, 0.01, 0.1, 0.1)  # a_start, f_dec, f_inc, f_a, dt0, dt_max, a
-FIRE_TOL, FIRE_DPHI, FIRE_MAXIT = 1e-12, 1e-3, 200000
+FIRE_TOL, FIRE_DPHI, FIRE_MAXIT = 1e-12, 1e-3, 100000
+# ka = kl = 1 as in the NVE configuration; kb = 0.1: with kb = 0.01 and calA0 > 1 the rings'
+# soft bending modes keep FIRE (alpha decaying without bound, FIRE.cu:338-347) above 1e-12
+# for more than 1e5 iterations, on the oracle as on the GPU
+FIRE_COSTS = (1.0, 1.0, 0.1, 1.0)
 
 
 def fire_systems():
@@ -99,9 +112,9 @@ def fire_systems():
 
 
 def fire_setup(s, size):
-    """harmonic vertex-vertex, cfg1 costs, skin 0.3 of `size` (the same for every path)."""
+    """harmonic vertex-vertex, FIRE_COSTS, skin 0.3 of `size` (the same for every path)."""
     s.set_types("harmonic", "vertexVertex")
-    s.set_energy_costs(*COSTS)
+    s.set_energy_costs(*FIRE_COSTS)
     return s.set_displacement_cutoff(0.3, size)
 
 
@@ -327,15 +340,18 @@ def cfg5_bench(args, torch, batched=False):
 COPY_ELEMS = 1 << 27  # float64 elements of the calibration copy: 1 GiB read + 1 GiB written
 
 
-def traffic_run(s, torch):
+def traffic_run(s, torch, args):
     """Launch sequence for `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE`: three copies of
-    COPY_ELEMS doubles (the calibration kernel: 8 * COPY_ELEMS bytes read and written each), then
-    50 back-to-back launches of the step kernel on the prepared state."""
+    COPY_ELEMS doubles (the calibration kernel: 8 * COPY_ELEMS bytes read and written each), the
+    benchmark's warm-up and timed steps, then 50 back-to-back launches of the step kernel on the
+    state the roofline figure is measured on."""
     src = torch.rand(COPY_ELEMS, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
     for _ in range(3):
         dst.copy_(src)
     torch.cuda.synchronize()
+    del src, dst
+    run_steps(s, args.warmup + args.steps)
     us, kbar = s.time_step_kernel(50)
     s.synchronize()
     print(json.dumps({"traffic_run": True, "copy_bytes_each_way": 8 * COPY_ELEMS, "step_launches": 50,
@@ -349,7 +365,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=2000)
     ap.add_argument("--prep-iters", type=int, default=1000)
-    ap.add_argument("--cpu-steps", type=int, default=1200)
+    ap.add_argument("--cpu-steps", type=int, default=600)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fire-steps", type=int, default=5, help="phi steps of the batched FIRE compression")
     ap.add_argument("--fire-plain", type=int, default=8, help="systems timed on the plain one-system path")
@@ -384,23 +400,15 @@ def main():
     s, st, dt, prep_s, tuned = prepare(args)
     nv, np_ = s.nv, s.np
     if args.traffic:
-        traffic_run(s, torch)
+        traffic_run(s, torch, args)
         return
 
-    done = 0
-    while done < args.warmup:
-        n = min(CHUNK, args.warmup - done)
-        s.langevin(n)
-        done += n
+    run_steps(s, args.warmup)
     s.synchronize()
     torch.cuda.synchronize()
     s.reset_update_count()
     t0 = time.perf_counter()
-    done = 0
-    while done < args.steps:
-        n = min(CHUNK, args.steps - done)
-        s.langevin(n)
-        done += n
+    run_steps(s, args.steps)
     s.synchronize()
     torch.cuda.synchronize()
     el = time.perf_counter() - t0

@@ -138,6 +138,39 @@ __global__ void k_adj_flag(View V, Params P, int* flag, int* any, int* cellOther
   }
 }
 
+// the two passes above in one launch: thread t tests vertex t, clears cell t, and walks ring t
+__global__ void k_adj_prep(View V, Params P, const double* wx, const double* wy, double* A, double* dA, int* flag,
+                           int* any, int* cellOther, int ncells) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int c = t; c < ncells; c += gridDim.x * blockDim.x) cellOther[c] = 0;
+  if (t < V.nv) {
+    const int s = V.vsys[t];
+    const Box B = box_of(V, s);
+    const double ddx = pbx(V.x[t] - V.lx[t], B), ddy = pby(V.y[t] - V.ly[t], B);
+    if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) {
+      atomicOr(&flag[s], 1);
+      atomicOr(any, 1);
+    }
+  }
+  if (t < V.np) {
+    const int f = V.rfirst[t], n = V.rnv[t];
+    const Box B = box_of(V, V.rsys[t]);
+    Dual ux = mk(0, wx[f]), uy = mk(0, wy[f]);
+    Dual S = mk(0, 0);
+    for (int i = 0; i < n; i++) {
+      const int a = f + i, b = (i + 1 < n) ? a + 1 : f;
+      const Dual dx = mk(pbx(V.x[b] - V.x[a], B), wx[b] - wx[a]);
+      const Dual dy = mk(pby(V.y[b] - V.y[a], B), wy[b] - wy[a]);
+      S = S + (ux * dy - dx * uy);
+      ux = ux + dx;
+      uy = uy + dy;
+    }
+    const double sg = S.v < 0 ? -1.0 : 1.0;
+    A[t] = 0.5 * fabs(S.v);
+    dA[t] = 0.5 * sg * S.t;
+  }
+}
+
 // ------------------------------------------------------------ H w per vertex ---
 // AdjArgs (dpm_internal.h): mode 0 writes H w; mode 1 applies the reverse-pass update.
 template <int POT>
@@ -280,6 +313,15 @@ static inline unsigned grid_n(long n) { return (unsigned)((n + 255) / 256); }
 
 hipError_t launch_adj_area(dpm_set_s* S, const View& Vn, const double* wx, const double* wy, double* A, double* dA) {
   k_adj_area<<<grid_n(S->np), 256, 0, S->stream>>>(Vn, wx, wy, A, dA);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_prep(dpm_set_s* S, const View& Vn, int parity, const double* wx, const double* wy, double* A,
+                           double* dA) {
+  long n = S->nv > S->G.ncells ? S->nv : S->G.ncells;
+  if (S->np > n) n = S->np;
+  k_adj_prep<<<grid_n(n), 256, 0, S->stream>>>(Vn, S->P, wx, wy, A, dA, S->d_flag[parity], S->d_any + parity,
+                                               S->G.count[1 - parity], S->G.ncells);
   return hipGetLastError();
 }
 

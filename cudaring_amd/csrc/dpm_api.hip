@@ -272,6 +272,12 @@ int verlet_sequence(dpm_set_s* S, long steps, bool langevin, double* rec = nullp
   HIPCHK(launch_fill_int(S->d_flag[1 - S->cur], 0, S->nsys, S->stream));
   HIPCHK(launch_fill_int(S->d_any + 1 - S->cur, 0, 1, S->stream));
   HIPCHK(launch_start(S, h, S->P.dt, rec ? rec + 2 * nv : nullptr, rec ? rec + 3 * nv : nullptr));
+  if (rec) {
+    // a recomputed segment starts from a restored checkpoint without a valid list: every
+    // system rebuilds its rows in the first step
+    HIPCHK(launch_fill_int(S->d_flag[1 - S->cur], 1, S->nsys, S->stream));
+    HIPCHK(launch_fill_int(S->d_any + 1 - S->cur, 1, 1, S->stream));
+  }
   swap_pos(S);
   for (long i = 0; i < steps; i++) {
     const bool more = i + 1 < steps;
@@ -428,9 +434,8 @@ View view_at(const dpm_set_s* S, double* x, double* y) {
 
 // one H w evaluation at the positions of Vn with the rebuild protocol of parity p
 int hvp_launches(dpm_set_s* S, const View& Vn, int p, AdjArgs A) {
-  HIPCHK(launch_adj_flag(S, Vn, p));
+  HIPCHK(launch_adj_prep(S, Vn, p, A.wx, A.wy, adj_A(S), adj_dA(S)));
   HIPCHK(launch_bin_view(S, Vn, p));
-  HIPCHK(launch_adj_area(S, Vn, A.wx, A.wy, adj_A(S), adj_dA(S)));
   A.flag = S->d_flag[p];
   A.count = S->G.count[p];
   A.A = adj_A(S);
@@ -458,7 +463,6 @@ int record_segment(dpm_set_s* S, long len) {
   const long nv = S->nv;
   HIPCHK(launch_copy_d(S->d_seg, S->V.x, nv, S->stream));
   HIPCHK(launch_copy_d(S->d_seg + nv, S->V.y, nv, S->stream));
-  HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
   int rc;
   if (len < 4 || !graphs_enabled()) {
     rc = verlet_sequence(S, len, false, S->d_seg);
@@ -1465,9 +1469,11 @@ int dpm_nve_backward(dpm_set_t h, const double* ax, const double* av, double* gx
   std::vector<int> all(S->nsys, 1);
   for (long k = S->tapeNck - 1; k >= 0; k--) {
     const long a = k * every, b = std::min(a + every, N);
+    // restore the checkpoint, recompute the segment (its first step rebuilds every row), then
+    // the reverse pass over it; no host synchronisation until the end
     rc = load_state(S, S->d_ck + k * 6 * nv);
     if (rc) return rc;
-    rc = rebuild(S, all, 0);
+    rc = clean_protocol(S);
     if (rc) return rc;
     rc = record_segment(S, b - a);
     if (rc) return rc;
@@ -1475,6 +1481,8 @@ int dpm_nve_backward(dpm_set_t h, const double* ax, const double* av, double* gx
     if (rc) return rc;
     rc = backward_segment(S, a, (k == S->tapeNck - 1) ? b : b - 1, N);
     if (rc) return rc;
+  }
+  {
     int err = 0;
     HIPCHK(hipMemcpyAsync(&err, S->d_err, sizeof(int), hipMemcpyDeviceToHost, S->stream));
     HIPCHK(hipStreamSynchronize(S->stream));

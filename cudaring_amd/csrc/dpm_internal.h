@@ -186,6 +186,8 @@ hipError_t launch_bin_view(dpm_set_s* S, const View& Vn, int parity);
 hipError_t launch_fire_sys(dpm_set_s* S, int chunk, int waves);
 hipError_t launch_adj_area(dpm_set_s* S, const View& Vn, const double* wx, const double* wy, double* A, double* dA);
 hipError_t launch_adj_flag(dpm_set_s* S, const View& Vn, int parity);
+hipError_t launch_adj_prep(dpm_set_s* S, const View& Vn, int parity, const double* wx, const double* wy, double* A,
+                           double* dA);
 hipError_t launch_adj_hvp(dpm_set_s* S, const View& Vn, AdjArgs A);
 hipError_t launch_energy_param_partials(dpm_set_s* S, double* out);
 hipError_t launch_ring_sum3(dpm_set_s* S, const double* in, double* out);

@@ -3,13 +3,15 @@
 // soft-disk FIRE, vertices on the disks, then vertex FIRE at growing packing fraction).
 //
 // usage: compressFIRE outDir numParticles numVertexPerParticle calA0 polydispersity phiTarget seed
-//                     [deltaPhi=2e-3] [phi0=0.2] [bidisperse nSmall nLarge]
+//                     [deltaPhi=2e-3] [phi0=0.2] [costs=ea,el,eb,ec] [maxIter=5000000] [bi=nSmall,nLarge]
 //   phiTarget <= 0 stops after the set-up (no device is used) and writes the set-up files.
 // Each compression step: scaleVertices(sqrt((phi + deltaPhi) / phi)), then FIRE until the
 // largest force component is below 1e-10 / (mean particle size), as compressFIRE.cpp:104-131.
+// The default costs are compressFIRE.cpp:30's (1e3, 1, 0, 1).
 #include <sys/stat.h>
 
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <iostream>
 #include <string>
@@ -22,7 +24,7 @@
 int main(int argc, char** argv) {
   if (argc < 8) {
     std::cerr << "usage: compressFIRE outDir numParticles numVertexPerParticle calA0 polydispersity "
-                 "phiTarget seed [deltaPhi] [phi0] [nSmall nLarge]" << std::endl;
+                 "phiTarget seed [deltaPhi=] [phi0=] [costs=ea,el,eb,ec] [maxIter=] [bi=nSmall,nLarge]" << std::endl;
     return 2;
   }
   std::string outDir = argv[1];
@@ -30,16 +32,29 @@ int main(int argc, char** argv) {
   long numParticles = std::atol(argv[2]), nvp = std::atol(argv[3]);
   double calA0 = std::atof(argv[4]), poly = std::atof(argv[5]), phiTarget = std::atof(argv[6]);
   unsigned long seed = std::strtoul(argv[7], nullptr, 10);
-  double deltaPhi = argc > 8 ? std::atof(argv[8]) : 2e-3;
-  double phi0 = argc > 9 ? std::atof(argv[9]) : 0.2;
-  long nSmall = argc > 11 ? std::atol(argv[10]) : 0, nLarge = argc > 11 ? std::atol(argv[11]) : 0;
-  const long minStep = 20, maxIterations = 5000000;
+  double deltaPhi = 2e-3, phi0 = 0.2, ea = 1e3, el = 1, eb = 0, ec = 1;
+  long nSmall = 0, nLarge = 0, maxIterations = 5000000;
+  for (int i = 8; i < argc; i++) {
+    std::string a = argv[i];
+    const size_t eq = a.find('=');
+    const std::string key = a.substr(0, eq), val = eq == std::string::npos ? "" : a.substr(eq + 1);
+    if (key == "deltaPhi") deltaPhi = std::atof(val.c_str());
+    else if (key == "phi0") phi0 = std::atof(val.c_str());
+    else if (key == "maxIter") maxIterations = std::atol(val.c_str());
+    else if (key == "costs") std::sscanf(val.c_str(), "%lf,%lf,%lf,%lf", &ea, &el, &eb, &ec);
+    else if (key == "bi") std::sscanf(val.c_str(), "%ld,%ld", &nSmall, &nLarge);
+    else {
+      std::cerr << "compressFIRE: unknown option " << a << std::endl;
+      return 2;
+    }
+  }
+  const long minStep = 20;
   const double cutDistance = 1, dt = 1e-2, forceTolerance0 = 1e-10;
   mkdir(outDir.c_str(), 0755);
 
   DPM2D dpm(numParticles, 2, nvp);
   dpm.setSeed(seed);
-  dpm.setEnergyCosts(1e3, 1, 0, 1);  // compressFIRE.cpp:30
+  dpm.setEnergyCosts(ea, el, eb, ec);
   ioDPMFile io(&dpm);
 
   // ---- soft disks

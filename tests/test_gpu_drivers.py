@@ -39,7 +39,8 @@ def run(args, timeout=110):
 @pytest.fixture(scope="module")
 def packing_dir(tmp_path_factory):
     out = str(tmp_path_factory.mktemp("cf"))
-    run([os.path.join(BIN, "compressFIRE"), out, "16", "16", "1.1", "0.1", "0.4", "5", "0.05"])
+    run([os.path.join(BIN, "compressFIRE"), out, "16", "16", "1.1", "0.1", "0.4", "5", "deltaPhi=0.05",
+         "costs=1,1,0.1,1", "maxIter=200000"])
     return os.path.join(out, "dp")
 
 
@@ -50,7 +51,7 @@ def test_compress_fire_packing(packing_dir, oracle_mod):
     o = oracle_mod.Oracle(st["nvp"])
     o.load_state(st)
     o.set_types("harmonic", "vertexVertex")
-    o.set_energy_costs(1e3, 1, 0, 1)
+    o.set_energy_costs(1, 1, 0.1, 1)
     o.calc_neighbor_list(1.0)
     o.calc_force_energy()
     assert o.max_unbalanced_force() < 1e-7  # FIRE stopped at 1e-10 / sigma; files keep 14 digits

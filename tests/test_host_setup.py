@@ -111,7 +111,7 @@ def test_file_format(driver, tmp_path):
 
 
 def test_bidisperse_sizes(driver, tmp_path):
-    out = run_setup(driver, str(tmp_path), 40, 16, 1.1, 0.0, 5, extra=("2e-3", "0.2", "16", "22"))
+    out = run_setup(driver, str(tmp_path), 40, 16, 1.1, 0.0, 5, extra=("bi=16,22",))
     n = [int(x) for x in load1(os.path.join(out, "setup", "numVertexInParticleList.dat"))]
     assert sorted(set(n)) == [16, 22] and n.count(22) == 20
     a0 = load1(os.path.join(out, "setup", "restAreas.dat"))
