@@ -45,6 +45,14 @@ def prepare(args):
     s, st = protocols.prepared_state(lambda nvp: DPMSet(nvp), NRINGS, PHI, SEED, costs=COSTS,
                                      final=("wca", "vertexSmooth"), iters=args.prep_iters,
                                      final_iters=3 * args.prep_iters)
+    if args.ring_order == "spatial":
+        from cudaring_amd import packing
+        s.close()
+        st, _ = packing.spatial_order(st)
+        s = DPMSet(st["nvp"])
+        s.load_state(st)
+        s.set_types("wca", "vertexSmooth")
+        s.set_energy_costs(*COSTS)
     sigma = float(2 * np.sqrt(np.mean(s.get("a0")) / np.pi))  # getMeanParticleSize
     dt = DT_UNITS * sigma
     s.set_dt(dt)
@@ -367,6 +375,8 @@ def main():
     ap.add_argument("--prep-iters", type=int, default=1000)
     ap.add_argument("--cpu-steps", type=int, default=600)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ring-order", choices=("random", "spatial"), default="random",
+                    help="ring numbering of the prepared packing: as generated, or along a Morton curve")
     ap.add_argument("--fire-steps", type=int, default=5, help="phi steps of the batched FIRE compression")
     ap.add_argument("--fire-plain", type=int, default=8, help="systems timed on the plain one-system path")
     ap.add_argument("--no-fire", action="store_true")

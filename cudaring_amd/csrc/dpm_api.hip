@@ -643,6 +643,7 @@ int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPer
   S->P.seed = 0;
   S->fp.tol = -1;
   if (const char* e = getenv("DPM_SPLIT")) S->split = std::max(1, std::min(8, atoi(e)));
+  if (const char* e = getenv("DPM_XCD_MAP")) S->xcdMap = atoi(e) != 0;
   if (alloc_list(S, 8) != DPM_OK) { dpm_destroy(S); return DPM_ERR_OOM; }
   *out = S;
   return DPM_OK;

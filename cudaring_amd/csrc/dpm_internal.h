@@ -65,6 +65,7 @@ struct StepArgs {
   int* maxcnt;
   int* updateCount;
   int manage;                     // inside loops: rebuild flagged rows, clear cells, count updates
+  int xcd;                        // XCD-aware workgroup -> tile mapping
 };
 
 // Arguments of the H w kernel (dpm_adjoint.hip).
@@ -99,6 +100,7 @@ struct dpm_set_s {
   long nsys = 0, np = 0, nv = 0;
   std::string err;
   int split = 4;  // wavefronts per tile in the step kernel (1..8)
+  int xcdMap = 1; // XCD-aware workgroup -> tile mapping in the step kernel
 
   // host copies of topology
   std::vector<int> h_rnv, h_rfirst, h_rsys, h_vring, h_vsys, h_svoff, h_sroff;

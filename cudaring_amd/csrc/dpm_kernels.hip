@@ -82,6 +82,15 @@ __device__ __forceinline__ void lane_ctx(const View& V, int tile, int lane, Lane
   L.pl = (lane > L.segS) ? lane - 1 : L.segE;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b mod 8), each
+// with its own L2.  This maps the workgroups of XCD k onto the k-th contiguous eighth of the tiles,
+// so that rings stored near each other (spatially sorted sets) share one L2.  A bijection of
+// [0, nb).
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int q = nb >> 3, r = nb & 7, k = b & 7;
+  return k * q + (k < r ? k : r) + (b >> 3);
+}
+
 // wave-per-tile kernels (4 tiles per 256-thread block)
 __device__ __forceinline__ bool wave_tile_ctx(const View& V, Lane& L) {
   const int tile = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
@@ -277,7 +286,7 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
   const int w = threadIdx.x / WAVE;
   const int nw = blockDim.x / WAVE;
   Lane L;
-  lane_ctx(V, blockIdx.x, threadIdx.x & (WAVE - 1), L);
+  lane_ctx(V, A.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x, threadIdx.x & (WAVE - 1), L);
   const Box B = box_of(V, L.s);
 
   // loop bookkeeping: clear the other parity's cells, count this step's rebuild, rebuild rows
@@ -1089,6 +1098,7 @@ StepArgs loop_args(dpm_set_s* S) {
   A.maxcnt = S->d_maxcnt;
   A.updateCount = S->d_updateCount;
   A.manage = 1;
+  A.xcd = S->xcdMap;
   return A;
 }
 

@@ -110,6 +110,50 @@ def size_bidisperse_system(nrings, phi, seed, n=32, ratio=1.4, calA0=1.10, pertu
             "theta0": theta0, "a0": a0}
 
 
+def _morton2(ix, iy):
+    """Interleave the bits of two non-negative int arrays (16 bits each)."""
+    def spread(v):
+        v = v.astype(np.uint64) & 0xFFFF
+        v = (v | (v << 8)) & 0x00FF00FF
+        v = (v | (v << 4)) & 0x0F0F0F0F
+        v = (v | (v << 2)) & 0x33333333
+        v = (v | (v << 1)) & 0x55555555
+        return v
+    return spread(ix) | (spread(iy) << np.uint64(1))
+
+
+def spatial_order(st, ncell=None):
+    """The same system with its rings renumbered along a Morton curve of their centres, so that
+    rings close in space are close in memory.  Returns (state, ring permutation)."""
+    nvp = np.asarray(st["nvp"])
+    L = np.asarray(st["box"], dtype=np.float64)
+    first = np.concatenate([[0], np.cumsum(nvp)])
+    pos = np.asarray(st["pos"]).reshape(-1, 2)
+    centres = np.empty((len(nvp), 2))
+    for p in range(len(nvp)):
+        seg = pos[first[p]:first[p + 1]]
+        d = seg - seg[0]
+        d -= L * np.round(d / L)
+        centres[p] = seg[0] + d.mean(0)
+    centres -= L * np.floor(centres / L)
+    m = ncell or int(2 ** np.ceil(np.log2(max(2, np.sqrt(len(nvp))))))
+    cell = np.minimum((centres / L * m).astype(np.int64), m - 1)
+    order = np.argsort(_morton2(cell[:, 0], cell[:, 1]), kind="stable")
+    vidx = np.concatenate([np.arange(first[p], first[p + 1]) for p in order])
+    out = dict(st)
+    out["nvp"] = nvp[order]
+    out["pos"] = pos[vidx].ravel()
+    for k in ("rad", "l0", "theta0"):
+        out[k] = np.asarray(st[k])[vidx]
+    out["a0"] = np.asarray(st["a0"])[order]
+    for k in ("vel",):
+        if k in st and st[k] is not None:
+            out[k] = np.asarray(st[k]).reshape(-1, 2)[vidx].ravel()
+    if "pangle" in st and st["pangle"] is not None:
+        out["pangle"] = np.asarray(st["pangle"])[order]
+    return out, order
+
+
 def concat_systems(systems):
     """Stack systems into one set: nvp, rings_per_system and concatenated fields."""
     out = {"nvp": np.concatenate([s["nvp"] for s in systems]),
