@@ -288,6 +288,9 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
   Lane L;
   lane_ctx(V, A.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x, threadIdx.x & (WAVE - 1), L);
   const Box B = box_of(V, L.s);
+  // own position first: the row rebuild below does not change it, so its load overlaps the
+  // rebuild-flag load instead of waiting behind it
+  const double x = V.x[L.v], y = V.y[L.v];
 
   // loop bookkeeping: clear the other parity's cells, count this step's rebuild, rebuild rows
   if (A.manage) {
@@ -299,7 +302,6 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
     if (rebuild) __syncthreads();
   }
 
-  const double x = V.x[L.v], y = V.y[L.v];
   // gather partials: force on v, vertex-vertex energy, smooth energy, share for prev(v)
   double fx = 0, fy = 0, e = 0, es = 0, qx = 0, qy = 0;
   const int g0 = nw > 1 ? 1 : 0;
