@@ -586,7 +586,7 @@ int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPer
   for (int b = 0; b < 2; b++) for (int c = 0; c < 2; c++) A(S->pos[b][c], nv);
   A(V.vx, nv); A(V.vy, nv); A(V.fx, nv); A(V.fy, nv); A(V.rad, nv); A(V.l0, nv); A(V.th0, nv);
   A(V.th, nv); A(V.len, nv); A(V.ev, nv); A(V.es, nv); A(V.lx, nv); A(V.ly, nv);
-  A(V.vring, nv); A(V.vsys, nv);
+  A(V.vring, nv); A(V.vsys, nv); A(V.vinfo, nv);
   A(V.rfirst, np); A(V.rnv, np); A(V.rsys, np);
   A(V.a0, np); A(V.area, np); A(V.perim, np); A(V.cx, np); A(V.cy, np); A(V.pe, np); A(V.ang, np);
   A(V.bx, numSystems); A(V.by, numSystems); A(V.ibx, numSystems); A(V.iby, numSystems);
@@ -603,7 +603,13 @@ int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPer
 #undef A
   set_pos_view(S);
   hipStream_t st = S->stream;
-  bool ok = up(V.vring, S->h_vring, st) == hipSuccess && up(V.vsys, S->h_vsys, st) == hipSuccess &&
+  std::vector<int4> vinfo(nv);
+  for (long v = 0; v < nv; v++) {
+    const int r = S->h_vring[v];
+    vinfo[v] = make_int4(r, S->h_rfirst[r], S->h_rnv[r], S->h_vsys[v]);
+  }
+  bool ok = up(V.vinfo, vinfo, st) == hipSuccess &&
+            up(V.vring, S->h_vring, st) == hipSuccess && up(V.vsys, S->h_vsys, st) == hipSuccess &&
             up(V.rfirst, S->h_rfirst, st) == hipSuccess && up(V.rnv, S->h_rnv, st) == hipSuccess &&
             up(V.rsys, S->h_rsys, st) == hipSuccess && up(V.svoff, S->h_svoff, st) == hipSuccess &&
             up(V.sroff, S->h_sroff, st) == hipSuccess && up(V.tile_v0, S->h_tile_v0, st) == hipSuccess &&
@@ -663,7 +669,7 @@ int dpm_destroy(dpm_set_t h) {
                   S->G.coff, S->G.brute, S->G.count[0], S->G.count[1], S->G.slots, S->G.cellSys, S->d_flag[0],
                   S->d_flag[1], S->d_any, S->d_updateCount, S->d_err, S->d_maxcnt, S->d_ctr, S->d_notDone,
                   S->d_fire[0], S->d_fire[1], S->d_part, S->d_red, S->d_scale, V.kr,
-                  S->d_ck, S->d_fin, S->d_seg, S->d_adj};
+                  S->d_ck, S->d_fin, S->d_seg, S->d_adj, V.vinfo};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (S->ownStream) hipStreamDestroy(S->ownStream);

@@ -43,6 +43,7 @@ struct View {
   double *rad, *l0, *th0, *th, *len, *ev, *es;
   double *lx, *ly;
   int *vring, *vsys;
+  int4* vinfo;  // per vertex in one load: ring, ring's first vertex, ring size, system
   int *rfirst, *rnv, *rsys;
   double *a0, *area, *perim, *cx, *cy, *pe, *ang;
   double *bx, *by, *ibx, *iby;  // per-system box and its reciprocal

@@ -72,9 +72,10 @@ __device__ __forceinline__ void lane_ctx(const View& V, int tile, int lane, Lane
   L.s = V.tile_sys[tile];
   L.act = (L.v0 + lane) < v1;
   L.v = L.act ? L.v0 + lane : L.v0;
-  L.r = V.vring[L.v];
-  const int first = V.rfirst[L.r];
-  L.n = V.rnv[L.r];
+  const int4 vi = V.vinfo[L.v];
+  L.r = vi.x;
+  const int first = vi.y;
+  L.n = vi.z;
   L.segS = L.act ? first - L.v0 : lane;
   const int segN = L.act ? L.n : 1;
   L.segE = L.segS + segN - 1;
