@@ -298,6 +298,8 @@ def cfg5_bench(args, torch, batched=False):
     rng = np.random.default_rng(CFG5_SEED)
     sizes = rng.integers(2048, 8193, 64) if batched else [16384]
     systems = [packing.size_bidisperse_system(int(n), 0.5, CFG5_SEED + i) for i, n in enumerate(sizes)]
+    if args.ring_order == "spatial":
+        systems = [packing.spatial_order(x)[0] for x in systems]
     st = packing.concat_systems(systems)
     t0 = time.perf_counter()
     s = DPMSet(st["nvp"], rings_per_system=st["rings_per_system"])

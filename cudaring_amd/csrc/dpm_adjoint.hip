@@ -171,6 +171,23 @@ __global__ void k_adj_prep(View V, Params P, const double* wx, const double* wy,
   }
 }
 
+// One Verlet list for a whole recomputed segment: the rows built at the reference positions
+// (lastPos) stay valid while every vertex of every recorded slot is within the displacement
+// bound of checkDisplacement (3 |x - x_ref| <= cutoff); `bad` is set otherwise.
+__global__ void k_adj_seg_check(View V, Params P, const double* seg, int slots, int* bad) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V.nv) return;
+  const Box B = box_of(V, V.vsys[v]);
+  const double rx = V.lx[v], ry = V.ly[v];
+  int out = 0;
+  for (int n = 0; n < slots; n++) {
+    const double* sx = seg + (long)n * 2 * V.nv;
+    const double dx = pbx(sx[v] - rx, B), dy = pby(sx[V.nv + v] - ry, B);
+    if (3 * sqrt(dx * dx + dy * dy) > P.cutoff) out = 1;
+  }
+  if (out) atomicOr(bad, 1);
+}
+
 // ------------------------------------------------------------ H w per vertex ---
 // AdjArgs (dpm_internal.h): mode 0 writes H w; mode 1 applies the reverse-pass update.
 template <int POT>
@@ -202,8 +219,23 @@ __global__ void k_adj_hvp(View V, Params P, CellGrid G, AdjArgs A) {
   const double kl = V.kr ? V.kr[V.np + r] : P.el;
   const double kb = V.kr ? V.kr[2 * V.np + r] : P.eb;
   const double a0 = V.a0[r];
-  // area term per unit ka
-  const Dual area = mk(A.A[r], A.dA[r]);
+  // area term per unit ka: the ring's area and its derivative along w, from the precomputed
+  // pass or walked here by this thread (n <= 64 minimum-image hops)
+  Dual area;
+  if (A.A) {
+    area = mk(A.A[r], A.dA[r]);
+  } else {
+    Dual ux = mk(0, A.wx[f]), uy = mk(0, A.wy[f]), S2 = mk(0, 0);
+    for (int i = 0; i < n; i++) {
+      const int a = f + i, b = (i + 1 < n) ? a + 1 : f;
+      const Dual dx = mk(pbx(V.x[b] - V.x[a], B), A.wx[b] - A.wx[a]);
+      const Dual dy = mk(pby(V.y[b] - V.y[a], B), A.wy[b] - A.wy[a]);
+      S2 = S2 + (ux * dy - dx * uy);
+      ux = ux + dx;
+      uy = uy + dy;
+    }
+    area = mk(0.5 * fabs(S2.v), (S2.v < 0 ? -0.5 : 0.5) * S2.t);
+  }
   const Dual gA = 0.5 * ((area / a0) - 1.0) / a0;
   const D2 Fa = {gA * (Rm1.y - Rp1.y), gA * (Rp1.x - Rm1.x)};
   // perimeter term per unit kl
@@ -342,6 +374,11 @@ hipError_t launch_adj_hvp(dpm_set_s* S, const View& Vn, AdjArgs A) {
     case POT_ADHESIVE: k_adj_hvp<POT_ADHESIVE><<<gr, 256, 0, S->stream>>>(Vn, S->P, S->G, A); break;
     default: k_adj_hvp<POT_WCA><<<gr, 256, 0, S->stream>>>(Vn, S->P, S->G, A); break;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_seg_check(dpm_set_s* S, const double* seg, int slots, int* bad) {
+  k_adj_seg_check<<<grid_n(S->nv), 256, 0, S->stream>>>(S->V, S->P, seg, slots, bad);
   return hipGetLastError();
 }
 

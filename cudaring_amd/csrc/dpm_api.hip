@@ -479,6 +479,21 @@ int backward_segment(dpm_set_s* S, long a, long top, long N) {
   const long nv = S->nv;
   const long count = top - a + 1;
   const bool hasTop = (top == N), hasZero = (a == 0);
+  // One list for the whole segment when it can serve every recorded position: rows built at
+  // the middle slot, checked against all slots.  Otherwise each position runs the rebuild
+  // protocol (displacement flags, binning, in-kernel row rebuild).
+  const long mid = (count - 1) / 2;
+  HIPCHK(launch_copy_d(S->V.x, S->d_seg + mid * 2 * nv, nv, S->stream));
+  HIPCHK(launch_copy_d(S->V.y, S->d_seg + mid * 2 * nv + nv, nv, S->stream));
+  std::vector<int> all(S->nsys, 1);
+  int rc = rebuild(S, all, 0);
+  if (rc) return rc;
+  int bad = 0;
+  HIPCHK(launch_fill_int(S->d_notDone, 0, 1, S->stream));
+  HIPCHK(launch_adj_seg_check(S, S->d_seg, (int)count, S->d_notDone));
+  HIPCHK(hipMemcpyAsync(&bad, S->d_notDone, sizeof(int), hipMemcpyDeviceToHost, S->stream));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  const bool single = !bad;
   auto seq = [&]() -> int {
     int p = 0;
     for (long n = top; n >= a; n--) {
@@ -496,17 +511,21 @@ int backward_segment(dpm_set_s* S, long a, long top, long N) {
       A.dt = S->tapeDt;
       A.mode = 1;
       A.update = n != 0;
-      int rc = hvp_launches(S, Vn, p, A);
-      if (rc) return rc;
+      if (single) {
+        HIPCHK(launch_adj_hvp(S, Vn, A));  // ring areas walked in the kernel, rows as built
+      } else {
+        int r2 = hvp_launches(S, Vn, p, A);
+        if (r2) return r2;
+      }
       p ^= 1;
       S->wCur ^= 1;
     }
-    return clean_protocol(S);
+    return single ? DPM_OK : clean_protocol(S);
   };
   const int w0 = S->wCur;
   if (!graphs_enabled()) return seq();
-  const int kind = 5 | (hasTop ? 16 : 0) | (hasZero ? 32 : 0);
-  int rc = run_graph(S, std::make_tuple(kind, count, w0), [&]() {
+  const int kind = 5 | (hasTop ? 16 : 0) | (hasZero ? 32 : 0) | (single ? 64 : 0);
+  rc = run_graph(S, std::make_tuple(kind, count, w0), [&]() {
     int r = seq();
     S->wCur = w0;
     return r;

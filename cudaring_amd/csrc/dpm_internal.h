@@ -188,6 +188,7 @@ hipError_t launch_bin_view(dpm_set_s* S, const View& Vn, int parity);
 hipError_t launch_fire_sys(dpm_set_s* S, int chunk, int waves);
 hipError_t launch_adj_area(dpm_set_s* S, const View& Vn, const double* wx, const double* wy, double* A, double* dA);
 hipError_t launch_adj_flag(dpm_set_s* S, const View& Vn, int parity);
+hipError_t launch_adj_seg_check(dpm_set_s* S, const double* seg, int slots, int* bad);
 hipError_t launch_adj_prep(dpm_set_s* S, const View& Vn, int parity, const double* wx, const double* wy, double* A,
                            double* dA);
 hipError_t launch_adj_hvp(dpm_set_s* S, const View& Vn, AdjArgs A);

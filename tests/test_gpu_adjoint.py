@@ -140,12 +140,15 @@ def oracle_final(oracle_mod, st, ka, kl, kb, dt, size, cut, steps, x0, v0, l0):
     return o.get("pos"), o.get("vel"), o.potential_energy()
 
 
-def test_nve_backward_matches_oracle_fd(oracle_mod):
+@pytest.mark.parametrize("vscale", [0.05, 0.5])
+def test_nve_backward_matches_oracle_fd(oracle_mod, vscale):
+    """vscale 0.05: one Verlet list serves each recomputed segment; 0.5: vertices move past the
+    skin within a segment, so every position runs the rebuild protocol."""
     st = soft_state(seed=12)
     nr, dt, steps, every = len(st["nvp"]), 5e-3, 120, 50
     ka, kl, kb = ring_costs(nr, 5)
     rng = np.random.default_rng(2)
-    v0 = 0.05 * rng.standard_normal(2 * len(st["rad"]))
+    v0 = vscale * rng.standard_normal(2 * len(st["rad"]))
     g, cut, size = nve_setup(st, ka, kl, kb, dt)
     g.set("vel", v0)
     g.calc_neighbor_list(cut)

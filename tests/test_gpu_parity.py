@@ -261,7 +261,7 @@ def test_persistent_fire_matches_step_kernels(pot, inter):
         cut = g.set_displacement_cutoff(0.1, 2 * np.mean(cat["rad"]))
         g.calc_neighbor_list(cut)
         g.calc_force_energy()
-        dt = 0.02 * 2 * np.sqrt(np.mean(cat["a0"]) / np.pi)
+        dt = 0.01 * 2 * np.sqrt(np.mean(cat["a0"]) / np.pi)  # dt_max = 10 dt stays stable
         g.init_fire([0.2, 0.5, 1.1, 0.99, dt, 10 * dt, 0.2], 20, 0)
         sets.append(g)
     a, b = sets
