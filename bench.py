@@ -30,6 +30,10 @@ COSTS = (1.0, 1.0, 0.01, 1.0)  # ka, kl, kb, ec
 SKIN = 0.5                     # setDisplacementCutoff(0.5, 2 <r_v>), runNVT.cpp:37,136
 DT_UNITS = 1e-3                # dt = 1e-3 * sigma / sqrt(ec), sigma = mean particle size
 CHUNK = 250                    # steps per graph replay
+# HBM bytes per launch of the step kernel on this workload, from the FETCH_SIZE and WRITE_SIZE
+# counter runs of `bench.py --traffic` (each scaled by its reading for a 1 GiB device copy:
+# 2.000 and 1.000): 2 x 3026.2 KiB + 1585.1 KiB.  Re-measure after changing the step kernel.
+TRAFFIC_BYTES_PER_LAUNCH = 7820500
 
 
 def alg_bytes_per_step(nv, np_, kbar):
@@ -434,7 +438,7 @@ def main():
     alg = alg_bytes_per_step(nv, np_, kbar)
     achieved = alg / (us * 1e-6) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": TRAFFIC_BYTES_PER_LAUNCH,
             "kernel": "k_step<vertexSmooth,WCA,kick|langevin|start>", "kernel_us": round(us, 3),
             "alg_bytes_per_launch": int(alg), "mean_neighbors": round(kbar, 3),
             "step_us": round(1e6 * el / args.steps, 3)}
