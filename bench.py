@@ -310,16 +310,23 @@ def cfg5_bench(args, torch, batched=False):
     s.load_state(st)
     s.set_types("harmonic", "vertexVertex")
     s.set_energy_costs(*COSTS)
-    protocols.compress(s, 0.5, 0.90, dphi=args.cfg5_dphi, iters=args.cfg5_iters)
+    protocols.compress(s, 0.5, 0.90, dphi=args.cfg5_dphi, iters=args.cfg5_iters,
+                       progress=lambda phi: print(f"# compressed to phi {phi:.3f}", file=sys.stderr, flush=True))
     sigma = float(2 * np.sqrt(np.mean(s.get("a0")) / np.pi))
-    s.set_types("adhesive", "vertexSmooth", particle="active")
+    # relax with the production interaction before the dynamics (no deep overlaps left)
+    s.set_types("adhesive", "vertexSmooth")
     s.set_attraction(0.1, 0.2)
+    size = 2 * float(np.mean(s.get("rad")))
+    s.calc_neighbor_list(s.set_displacement_cutoff(0.3, size))
+    s.calc_force_energy()
+    protocols.fire_relax(s, 0.01 * size, args.cfg5_iters)
+    s.set_types("adhesive", "vertexSmooth", particle="active")
     s.set_self_propulsion(0.1 * GAMMA, 100.0 * sigma)
     s.set("pangle", rng.uniform(-np.pi, np.pi, s.np))
     s.set_dt(5e-3 * sigma)
     s.set_seed(CFG5_SEED)
     size = 2 * float(np.mean(s.get("rad")))
-    cut = s.set_displacement_cutoff(SKIN, size)
+    cut = s.set_displacement_cutoff(0.3, size)
     s.calc_neighbor_list(cut)
     s.calc_force_energy()
     s.init_langevin(1e-4, GAMMA)
@@ -393,8 +400,8 @@ def main():
     ap.add_argument("--cfg5", action="store_true", help="large active + adhesive system (16384 rings)")
     ap.add_argument("--cfg5-batched", action="store_true", help="64 active + adhesive systems in one set")
     ap.add_argument("--cfg5-steps", type=int, default=2000)
-    ap.add_argument("--cfg5-dphi", type=float, default=0.05)
-    ap.add_argument("--cfg5-iters", type=int, default=400)
+    ap.add_argument("--cfg5-dphi", type=float, default=0.02)
+    ap.add_argument("--cfg5-iters", type=int, default=2000)
     ap.add_argument("--traffic", action="store_true",
                     help="counter-collection run: a copy of known size, then step-kernel launches only")
     args = ap.parse_args()

@@ -25,8 +25,9 @@ def fire_relax(sys_, dt, iters, chunk=500, tol=0.0):
             break
 
 
-def compress(sys_, phi_start, phi_target, dphi=0.02, iters=2000, skin=0.3):
-    """Grow every ring of every system until the preferred phi reaches phi_target."""
+def compress(sys_, phi_start, phi_target, dphi=0.02, iters=2000, skin=0.3, progress=None):
+    """Grow every ring of every system until the preferred phi reaches phi_target;
+    progress(phi), if given, is called after each relaxed compression step."""
     phi = phi_start
     while phi < phi_target - 1e-12:
         nxt = min(phi + dphi, phi_target)
@@ -39,6 +40,8 @@ def compress(sys_, phi_start, phi_target, dphi=0.02, iters=2000, skin=0.3):
         dt = 0.01 * size * 8
         fire_relax(sys_, dt, iters)
         phi = nxt
+        if progress is not None:
+            progress(phi)
     return phi
 
 
