@@ -15,6 +15,7 @@
 // with c = 1/2 at the two ends of the trajectory and 1 inside.
 #include "dpm_internal.h"
 #include "dpm_rows.h"
+#include "dpm_tile.h"
 
 namespace dpm {
 
@@ -97,30 +98,6 @@ __device__ __forceinline__ bool gm_dual(const Params& P, Dual d, double radSum, 
   }
 }
 
-// ------------------------------------------------------- ring areas (dual) ---
-// Per ring: signed shoelace sum over the minimum-image walk from the first vertex, with tangents
-// w; area = |S| / 2 and its directional derivative.
-__global__ void k_adj_area(View V, const double* wx, const double* wy, double* A, double* dA) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= V.np) return;
-  const int f = V.rfirst[r], n = V.rnv[r];
-  const int s = V.rsys[r];
-  const Box B = box_of(V, s);
-  Dual ux = mk(0, wx[f]), uy = mk(0, wy[f]);
-  Dual S = mk(0, 0);
-  for (int i = 0; i < n; i++) {
-    const int a = f + i, b = (i + 1 < n) ? a + 1 : f;
-    const Dual dx = mk(pbx(V.x[b] - V.x[a], B), wx[b] - wx[a]);
-    const Dual dy = mk(pby(V.y[b] - V.y[a], B), wy[b] - wy[a]);
-    S = S + (ux * dy - dx * uy);
-    ux = ux + dx;
-    uy = uy + dy;
-  }
-  const double sg = S.v < 0 ? -1.0 : 1.0;
-  A[r] = 0.5 * fabs(S.v);
-  dA[r] = 0.5 * sg * S.t;
-}
-
 // --------------------------------------------------------- displacement flag ---
 // Verlet-list validity at the positions in V.x: flags systems whose vertices moved more than a
 // third of the skin since the rows were built (checkDisplacement, DPM2D.cu:1390-1406), and
@@ -135,39 +112,6 @@ __global__ void k_adj_flag(View V, Params P, int* flag, int* any, int* cellOther
   if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) {
     atomicOr(&flag[s], 1);
     atomicOr(any, 1);
-  }
-}
-
-// the two passes above in one launch: thread t tests vertex t, clears cell t, and walks ring t
-__global__ void k_adj_prep(View V, Params P, const double* wx, const double* wy, double* A, double* dA, int* flag,
-                           int* any, int* cellOther, int ncells) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  for (int c = t; c < ncells; c += gridDim.x * blockDim.x) cellOther[c] = 0;
-  if (t < V.nv) {
-    const int s = V.vsys[t];
-    const Box B = box_of(V, s);
-    const double ddx = pbx(V.x[t] - V.lx[t], B), ddy = pby(V.y[t] - V.ly[t], B);
-    if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) {
-      atomicOr(&flag[s], 1);
-      atomicOr(any, 1);
-    }
-  }
-  if (t < V.np) {
-    const int f = V.rfirst[t], n = V.rnv[t];
-    const Box B = box_of(V, V.rsys[t]);
-    Dual ux = mk(0, wx[f]), uy = mk(0, wy[f]);
-    Dual S = mk(0, 0);
-    for (int i = 0; i < n; i++) {
-      const int a = f + i, b = (i + 1 < n) ? a + 1 : f;
-      const Dual dx = mk(pbx(V.x[b] - V.x[a], B), wx[b] - wx[a]);
-      const Dual dy = mk(pby(V.y[b] - V.y[a], B), wy[b] - wy[a]);
-      S = S + (ux * dy - dx * uy);
-      ux = ux + dx;
-      uy = uy + dy;
-    }
-    const double sg = S.v < 0 ? -1.0 : 1.0;
-    A[t] = 0.5 * fabs(S.v);
-    dA[t] = 0.5 * sg * S.t;
   }
 }
 
@@ -188,114 +132,138 @@ __global__ void k_adj_seg_check(View V, Params P, const double* seg, int slots, 
   if (out) atomicOr(bad, 1);
 }
 
-// ------------------------------------------------------------ H w per vertex ---
+// ------------------------------------------------------------ H w per tile ---
+// One workgroup per ring-aligned tile, as the step kernel: wave 0 evaluates the shape force and
+// its tangent with the ring walks done as segmented wave scans (the unwrapped position of a
+// vertex relative to the ring's first vertex is a scan of minimum-image segments; its tangent is
+// w_v - w_first), while waves 1..nw-1 share the pair gather, of which only the tangent is needed.
 // AdjArgs (dpm_internal.h): mode 0 writes H w; mode 1 applies the reverse-pass update.
 template <int POT>
-__global__ void k_adj_hvp(View V, Params P, CellGrid G, AdjArgs A) {
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= V.nv) return;
-  const int r = V.vring[v], s = V.vsys[v];
-  const Box B = box_of(V, s);
-  if (A.flag && A.flag[s]) build_row(V, P, G, A.count, v, s, B, A.err, A.maxcnt);
-
-  const int f = V.rfirst[r], n = V.rnv[r];
-  const int im1 = (v == f) ? f + n - 1 : v - 1;
-  const int im2 = (im1 == f) ? f + n - 1 : im1 - 1;
-  const int ip1 = (v == f + n - 1) ? f : v + 1;
-  const int ip2 = (ip1 == f + n - 1) ? f : ip1 + 1;
-  const double x0 = V.x[v], y0 = V.y[v];
-  // ring neighbourhood unwrapped by minimum-image hops from v, with tangents w
-  const D2 R0 = {mk(0, A.wx[v]), mk(0, A.wy[v])};
-  const double hm1x = -pbx(x0 - V.x[im1], B), hm1y = -pby(y0 - V.y[im1], B);
-  const D2 Rm1 = {mk(hm1x, A.wx[im1]), mk(hm1y, A.wy[im1])};
-  const D2 Rm2 = {mk(hm1x - pbx(V.x[im1] - V.x[im2], B), A.wx[im2]),
-                  mk(hm1y - pby(V.y[im1] - V.y[im2], B), A.wy[im2])};
-  const double hp1x = pbx(V.x[ip1] - x0, B), hp1y = pby(V.y[ip1] - y0, B);
-  const D2 Rp1 = {mk(hp1x, A.wx[ip1]), mk(hp1y, A.wy[ip1])};
-  const D2 Rp2 = {mk(hp1x + pbx(V.x[ip2] - V.x[ip1], B), A.wx[ip2]),
-                  mk(hp1y + pby(V.y[ip2] - V.y[ip1], B), A.wy[ip2])};
-
-  const double ka = V.kr ? V.kr[r] : P.ea;
-  const double kl = V.kr ? V.kr[V.np + r] : P.el;
-  const double kb = V.kr ? V.kr[2 * V.np + r] : P.eb;
-  const double a0 = V.a0[r];
-  // area term per unit ka: the ring's area and its derivative along w, from the precomputed
-  // pass or walked here by this thread (n <= 64 minimum-image hops)
-  Dual area;
-  if (A.A) {
-    area = mk(A.A[r], A.dA[r]);
-  } else {
-    Dual ux = mk(0, A.wx[f]), uy = mk(0, A.wy[f]), S2 = mk(0, 0);
-    for (int i = 0; i < n; i++) {
-      const int a = f + i, b = (i + 1 < n) ? a + 1 : f;
-      const Dual dx = mk(pbx(V.x[b] - V.x[a], B), A.wx[b] - A.wx[a]);
-      const Dual dy = mk(pby(V.y[b] - V.y[a], B), A.wy[b] - A.wy[a]);
-      S2 = S2 + (ux * dy - dx * uy);
-      ux = ux + dx;
-      uy = uy + dy;
-    }
-    area = mk(0.5 * fabs(S2.v), (S2.v < 0 ? -0.5 : 0.5) * S2.t);
+__global__ __launch_bounds__(WAVE* MAXSPLIT) void k_adj_hvp(View V, Params P, CellGrid G, AdjArgs A) {
+  __shared__ double red[MAXSPLIT][WAVE][2];
+  const int w = threadIdx.x / WAVE;
+  const int nw = blockDim.x / WAVE;
+  Lane L;
+  lane_ctx(V, A.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x, threadIdx.x & (WAVE - 1), L);
+  const Box B = box_of(V, L.s);
+  const double x = V.x[L.v], y = V.y[L.v];
+  const double wvx = A.wx[L.v], wvy = A.wy[L.v];
+  if (A.cellOther)
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < A.ncells; c += gridDim.x * blockDim.x)
+      A.cellOther[c] = 0;
+  if (A.flag) {
+    const bool rebuild = A.flag[L.s] != 0;
+    if (rebuild && w == 0 && L.act) build_row(V, P, G, A.count, L.v, L.s, B, A.err, A.maxcnt);
+    if (rebuild) __syncthreads();
   }
-  const Dual gA = 0.5 * ((area / a0) - 1.0) / a0;
-  const D2 Fa = {gA * (Rm1.y - Rp1.y), gA * (Rp1.x - Rm1.x)};
-  // perimeter term per unit kl
-  const D2 nS = Rp1 - R0, pS = R0 - Rm1;
-  const Dual nLen = dsqrt(dot(nS, nS)), pLen = dsqrt(dot(pS, pS));
-  const double l0v = V.l0[v], l0p = V.l0[im1];
-  const Dual nDelta = nLen / l0v - 1.0, pDelta = pLen / l0p - 1.0;
-  const D2 Fp = (nDelta / (l0v * nLen)) * nS - (pDelta / (l0p * pLen)) * pS;
-  // bending term per unit kb: turning angles of im1, v, ip1 (calcAngle, cudaKernel.cuh:423-427)
-  const D2 pm = Rm1 - Rm2, np1 = Rp2 - Rp1;
-  const Dual thm = datan2(cross(pS, pm), dot(pS, pm));
-  const Dual th = datan2(cross(nS, pS), dot(nS, pS));
-  const Dual thp = datan2(cross(np1, nS), dot(np1, nS));
-  const Dual Dm = thm - V.th0[im1], D = th - V.th0[v], Dp = thp - V.th0[ip1];
-  const Dual preVar = (D - Dm) / dot(pS, pS), nextVar = (D - Dp) / dot(nS, nS);
-  const D2 Fb = {preVar * pS.y + nextVar * nS.y, -(preVar * pS.x) - nextVar * nS.x};
 
-  Dual fx = ka * Fa.x + kl * Fp.x + kb * Fb.x;
-  Dual fy = ka * Fa.y + kl * Fp.y + kb * Fb.y;
-  // vertex-vertex pairs over the row (kernelCalcVertexInteraction, cudaKernel.cuh:690-722)
-  const double radv = V.rad[v];
-  const int c = V.ncnt[v];
-  for (int k = 0; k < c; k++) {
-    const int j = V.nbr[(long)k * V.nv + v];
-    const Dual dx = mk(pbx(x0 - V.x[j], B), A.wx[v] - A.wx[j]);
-    const Dual dy = mk(pby(y0 - V.y[j], B), A.wy[v] - A.wy[j]);
-    const Dual d = dsqrt(dx * dx + dy * dy);
-    Dual g;
-    if (gm_dual<POT>(P, d, radv + V.rad[j], g)) {
-      fx = fx + g * dx / d;
-      fy = fy + g * dy / d;
+  // pair tangent: d/de of sum_j g(d) (r_v - r_j) / d along w (kernelCalcVertexInteraction,
+  // cudaKernel.cuh:690-722)
+  double tx = 0, ty = 0;
+  const int g0 = nw > 1 ? 1 : 0;
+  if (w >= g0 && L.act) {
+    const double radv = V.rad[L.v];
+    const int c = V.ncnt[L.v];
+    for (int k = w - g0; k < c; k += nw - g0) {
+      const int j = V.nbr[(long)k * V.nv + L.v];
+      const Dual dx = mk(pbx(x - V.x[j], B), wvx - A.wx[j]);
+      const Dual dy = mk(pby(y - V.y[j], B), wvy - A.wy[j]);
+      const Dual d = dsqrt(dx * dx + dy * dy);
+      Dual g;
+      if (gm_dual<POT>(P, d, radv + V.rad[j], g)) {
+        const Dual q = g / d;
+        tx += (q * dx).t;
+        ty += (q * dy).t;
+      }
     }
   }
-  const double hwx = -fx.t, hwy = -fy.t;  // H w = -dF
+  if (nw > 1 && w > 0) {
+    red[w][L.lane][0] = tx;
+    red[w][L.lane][1] = ty;
+  }
+
+  // ---- wave 0: shape force of the lane's vertex and its tangent
+  D2 Fa{}, Fp{}, Fb{};
+  double wnx = 0, wny = 0, nSx = 0, nSy = 0, nl = 1, l0v = 1, ka = 0, kl = 0, kb = 0;
+  if (w == 0) {
+    const double xn = shfl_d(x, L.nl), yn = shfl_d(y, L.nl);
+    const double xp = shfl_d(x, L.pl), yp = shfl_d(y, L.pl);
+    wnx = shfl_d(wvx, L.nl);
+    wny = shfl_d(wvy, L.nl);
+    const double wpx = shfl_d(wvx, L.pl), wpy = shfl_d(wvy, L.pl);
+    const double wfx = shfl_d(wvx, L.segS), wfy = shfl_d(wvy, L.segS);
+    // segments (v, next) and (prev, v) of the reference's unwrapped walk
+    const D2 nS = {mk(pbx(xn - x, B), wnx - wvx), mk(pby(yn - y, B), wny - wvy)};
+    const D2 pS = {mk(-pbx(xp - x, B), wvx - wpx), mk(-pby(yp - y, B), wvy - wpy)};
+    // area (calcAreaForceEnergy, cudaKernel.cuh:429-435): shoelace over the unwrapped walk
+    const double Ix = seg_scan(L.act ? nS.x.v : 0.0, L.lane, L.segS);
+    const double Iy = seg_scan(L.act ? nS.y.v : 0.0, L.lane, L.segS);
+    const D2 u = {mk(Ix - nS.x.v, wvx - wfx), mk(Iy - nS.y.v, wvy - wfy)};
+    const Dual shoe = cross(u, nS);
+    const double Sv = seg_sum(L.act ? shoe.v : 0.0, L.lane, L.segS, L.segE);
+    const double St = seg_sum(L.act ? shoe.t : 0.0, L.lane, L.segS, L.segE);
+    const Dual area = mk(0.5 * fabs(Sv), (Sv < 0 ? -0.5 : 0.5) * St);
+    const double a0 = V.a0[L.r];
+    const Dual gA = 0.5 * ((area / a0) - 1.0) / a0;
+    Fa = D2{gA * (-(pS.y + nS.y)), gA * (pS.x + nS.x)};
+    // perimeter (calcPerimeterForceEnergy 437-447)
+    const Dual nLen = dsqrt(dot(nS, nS)), pLen = dsqrt(dot(pS, pS));
+    l0v = V.l0[L.v];
+    const double l0p = shfl_d(l0v, L.pl);
+    Fp = (((nLen / l0v) - 1.0) / (l0v * nLen)) * nS - (((pLen / l0p) - 1.0) / (l0p * pLen)) * pS;
+    // bending (calcBendingForceEnergy 457-471; calcAngle 423-427)
+    const Dual D = datan2(cross(nS, pS), dot(nS, pS)) - V.th0[L.v];
+    const Dual Dp = mk(shfl_d(D.v, L.pl), shfl_d(D.t, L.pl));
+    const Dual Dn = mk(shfl_d(D.v, L.nl), shfl_d(D.t, L.nl));
+    const Dual preVar = (D - Dp) / dot(pS, pS), nextVar = (D - Dn) / dot(nS, nS);
+    Fb = D2{preVar * pS.y + nextVar * nS.y, -(preVar * pS.x) - nextVar * nS.x};
+    ka = V.kr ? V.kr[L.r] : P.ea;
+    kl = V.kr ? V.kr[V.np + L.r] : P.el;
+    kb = V.kr ? V.kr[2 * V.np + L.r] : P.eb;
+    nSx = nS.x.v;
+    nSy = nS.y.v;
+    nl = nLen.v;
+  }
+  if (nw > 1) {
+    __syncthreads();
+    if (w > 0) return;
+    for (int q = 1; q < nw; q++) {
+      tx += red[q][L.lane][0];
+      ty += red[q][L.lane][1];
+    }
+  }
+  if (!L.act) return;
+  // Verlet-list validity at the positions processed next (checkDisplacement, DPM2D.cu:1390-1406);
+  // a row rebuilt above was rebuilt by this lane, so lastPos is this lane's own write
+  if (A.px) {
+    const double ddx = pbx(A.px[L.v] - V.lx[L.v], B), ddy = pby(A.py[L.v] - V.ly[L.v], B);
+    if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) {
+      atomicOr(&A.flagOut[L.s], 1);
+      atomicOr(A.anyOut, 1);
+    }
+  }
+  const double hwx = -(ka * Fa.x.t + kl * Fp.x.t + kb * Fb.x.t + tx);  // H w = -dF
+  const double hwy = -(ka * Fa.y.t + kl * Fp.y.t + kb * Fb.y.t + ty);
   if (A.mode == 0) {
-    A.hx[v] = hwx;
-    A.hy[v] = hwy;
+    A.hx[L.v] = hwx;
+    A.hy[L.v] = hwy;
     return;
   }
   // reverse-pass update and parameter partials (values of the unit-cost forces)
-  const double wvx = A.wx[v], wvy = A.wy[v];
   const long nv = V.nv;
-  A.g[v] += A.cd * (wvx * Fa.x.v + wvy * Fa.y.v);
-  A.g[nv + v] += A.cd * (wvx * Fp.x.v + wvy * Fp.y.v);
-  A.g[2 * nv + v] += A.cd * (wvx * Fb.x.v + wvy * Fb.y.v);
-  // rest length l0_v of the segment (v, ip1): its force is +f on v and -f on ip1 with
-  // f = kl (1 / l0^2 - 1 / (l0 |n|)) n, so d/dl0 contracts with (w_v - w_ip1)
-  const double nl = nLen.v;
+  const double cd = A.cd;
+  A.g[L.v] += cd * (wvx * Fa.x.v + wvy * Fa.y.v);
+  A.g[nv + L.v] += cd * (wvx * Fp.x.v + wvy * Fp.y.v);
+  A.g[2 * nv + L.v] += cd * (wvx * Fb.x.v + wvy * Fb.y.v);
+  // rest length l0_v of the segment (v, next): its force is +f on v and -f on next with
+  // f = kl (1 / l0^2 - 1 / (l0 |n|)) n, so d/dl0 contracts with (w_v - w_next)
   const double dg = -2.0 / (l0v * l0v * l0v) + 1.0 / (l0v * l0v * nl);
-  A.g[3 * nv + v] += A.cd * kl * dg * ((wvx - A.wx[ip1]) * nS.x.v + (wvy - A.wy[ip1]) * nS.y.v);
-  const double ax = A.ax[v] - A.cd * hwx, ay = A.ay[v] - A.cd * hwy;
-  A.ax[v] = ax;
-  A.ay[v] = ay;
-  if (A.update) {
-    A.nwx[v] = wvx + A.dt * ax;
-    A.nwy[v] = wvy + A.dt * ay;
-  } else {
-    A.nwx[v] = wvx;
-    A.nwy[v] = wvy;
-  }
+  A.g[3 * nv + L.v] += cd * kl * dg * ((wvx - wnx) * nSx + (wvy - wny) * nSy);
+  const double ax = A.ax[L.v] - cd * hwx, ay = A.ay[L.v] - cd * hwy;
+  A.ax[L.v] = ax;
+  A.ay[L.v] = ay;
+  A.nwx[L.v] = A.update ? wvx + A.dt * ax : wvx;
+  A.nwy[L.v] = A.update ? wvy + A.dt * ay : wvy;
 }
 
 // ------------------------------------------------- energy parameter partials ---
@@ -343,20 +311,6 @@ __global__ void k_copy_d(double* dst, const double* src, long n) {
 // ------------------------------------------------------------------ launchers ---
 static inline unsigned grid_n(long n) { return (unsigned)((n + 255) / 256); }
 
-hipError_t launch_adj_area(dpm_set_s* S, const View& Vn, const double* wx, const double* wy, double* A, double* dA) {
-  k_adj_area<<<grid_n(S->np), 256, 0, S->stream>>>(Vn, wx, wy, A, dA);
-  return hipGetLastError();
-}
-
-hipError_t launch_adj_prep(dpm_set_s* S, const View& Vn, int parity, const double* wx, const double* wy, double* A,
-                           double* dA) {
-  long n = S->nv > S->G.ncells ? S->nv : S->G.ncells;
-  if (S->np > n) n = S->np;
-  k_adj_prep<<<grid_n(n), 256, 0, S->stream>>>(Vn, S->P, wx, wy, A, dA, S->d_flag[parity], S->d_any + parity,
-                                               S->G.count[1 - parity], S->G.ncells);
-  return hipGetLastError();
-}
-
 hipError_t launch_adj_flag(dpm_set_s* S, const View& Vn, int parity) {
   const long n = S->nv > S->G.ncells ? S->nv : S->G.ncells;
   k_adj_flag<<<grid_n(n), 256, 0, S->stream>>>(Vn, S->P, S->d_flag[parity], S->d_any + parity,
@@ -367,12 +321,14 @@ hipError_t launch_adj_flag(dpm_set_s* S, const View& Vn, int parity) {
 hipError_t launch_adj_hvp(dpm_set_s* S, const View& Vn, AdjArgs A) {
   A.err = S->d_err;
   A.maxcnt = S->d_maxcnt;
-  const unsigned gr = grid_n(S->nv);
+  A.xcd = S->xcdMap;
+  const unsigned g = (unsigned)S->V.ntiles, b = WAVE * S->split;
+  hipStream_t st = S->stream;
   switch (S->P.potential) {
-    case POT_HARMONIC: k_adj_hvp<POT_HARMONIC><<<gr, 256, 0, S->stream>>>(Vn, S->P, S->G, A); break;
-    case POT_LJ: k_adj_hvp<POT_LJ><<<gr, 256, 0, S->stream>>>(Vn, S->P, S->G, A); break;
-    case POT_ADHESIVE: k_adj_hvp<POT_ADHESIVE><<<gr, 256, 0, S->stream>>>(Vn, S->P, S->G, A); break;
-    default: k_adj_hvp<POT_WCA><<<gr, 256, 0, S->stream>>>(Vn, S->P, S->G, A); break;
+    case POT_HARMONIC: k_adj_hvp<POT_HARMONIC><<<g, b, 0, st>>>(Vn, S->P, S->G, A); break;
+    case POT_LJ: k_adj_hvp<POT_LJ><<<g, b, 0, st>>>(Vn, S->P, S->G, A); break;
+    case POT_ADHESIVE: k_adj_hvp<POT_ADHESIVE><<<g, b, 0, st>>>(Vn, S->P, S->G, A); break;
+    default: k_adj_hvp<POT_WCA><<<g, b, 0, st>>>(Vn, S->P, S->G, A); break;
   }
   return hipGetLastError();
 }

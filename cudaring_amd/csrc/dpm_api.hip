@@ -407,13 +407,13 @@ int adjoint_supported(dpm_set_s* S) {
   return require_list(S);
 }
 
-// work buffers: w0, w1, ax (2nv each), partials g (4nv), ring area and dA (np each), ring sums (3np)
+// work buffers: w0, w1, ax (2nv each), partials g (4nv), ring sums (3np)
 int adj_alloc(dpm_set_s* S) {
   if (S->d_adj && S->adjNv == S->nv) return DPM_OK;
   HIPCHK(hipStreamSynchronize(S->stream));
   hipFree(S->d_adj);
   S->d_adj = nullptr;
-  HIPCHK(dalloc(&S->d_adj, 10 * S->nv + 5 * S->np));
+  HIPCHK(dalloc(&S->d_adj, 10 * S->nv + 3 * S->np));
   S->adjNv = S->nv;
   drop_graphs(S);  // captured sequences hold the old addresses
   return DPM_OK;
@@ -421,9 +421,7 @@ int adj_alloc(dpm_set_s* S) {
 double* adj_w(dpm_set_s* S, int i) { return S->d_adj + (long)i * 2 * S->nv; }
 double* adj_ax(dpm_set_s* S) { return S->d_adj + 4 * S->nv; }
 double* adj_g(dpm_set_s* S) { return S->d_adj + 6 * S->nv; }
-double* adj_A(dpm_set_s* S) { return S->d_adj + 10 * S->nv; }
-double* adj_dA(dpm_set_s* S) { return S->d_adj + 10 * S->nv + S->np; }
-double* adj_ring(dpm_set_s* S) { return S->d_adj + 10 * S->nv + 2 * S->np; }
+double* adj_ring(dpm_set_s* S) { return S->d_adj + 10 * S->nv; }
 
 View view_at(const dpm_set_s* S, double* x, double* y) {
   View V = S->V;
@@ -434,12 +432,10 @@ View view_at(const dpm_set_s* S, double* x, double* y) {
 
 // one H w evaluation at the positions of Vn with the rebuild protocol of parity p
 int hvp_launches(dpm_set_s* S, const View& Vn, int p, AdjArgs A) {
-  HIPCHK(launch_adj_prep(S, Vn, p, A.wx, A.wy, adj_A(S), adj_dA(S)));
+  HIPCHK(launch_adj_flag(S, Vn, p));
   HIPCHK(launch_bin_view(S, Vn, p));
   A.flag = S->d_flag[p];
   A.count = S->G.count[p];
-  A.A = adj_A(S);
-  A.dA = adj_dA(S);
   HIPCHK(launch_adj_hvp(S, Vn, A));
   return DPM_OK;
 }
@@ -496,6 +492,10 @@ int backward_segment(dpm_set_s* S, long a, long top, long N) {
   const bool single = !bad;
   auto seq = [&]() -> int {
     int p = 0;
+    if (!single) {  // list validity at the first position processed, x_top
+      double* st = S->d_seg + (top - a) * 2 * nv;
+      HIPCHK(launch_adj_flag(S, view_at(S, st, st + nv), 0));
+    }
     for (long n = top; n >= a; n--) {
       double* sx = S->d_seg + (n - a) * 2 * nv;
       const View Vn = view_at(S, sx, sx + nv);
@@ -511,12 +511,22 @@ int backward_segment(dpm_set_s* S, long a, long top, long N) {
       A.dt = S->tapeDt;
       A.mode = 1;
       A.update = n != 0;
-      if (single) {
-        HIPCHK(launch_adj_hvp(S, Vn, A));  // ring areas walked in the kernel, rows as built
-      } else {
-        int r2 = hvp_launches(S, Vn, p, A);
-        if (r2) return r2;
+      if (!single) {
+        // rebuild protocol of the step loop: bin the flagged systems, rebuild their rows in the
+        // kernel, which also tests the next position and clears the other parity's cells
+        HIPCHK(launch_bin_view(S, Vn, p));
+        A.flag = S->d_flag[p];
+        A.count = S->G.count[p];
+        A.cellOther = S->G.count[1 - p];
+        A.ncells = S->G.ncells;
+        if (n > a) {
+          A.px = sx - 2 * nv;
+          A.py = sx - nv;
+          A.flagOut = S->d_flag[1 - p];
+          A.anyOut = S->d_any + 1 - p;
+        }
       }
+      HIPCHK(launch_adj_hvp(S, Vn, A));
       p ^= 1;
       S->wCur ^= 1;
     }

@@ -74,8 +74,6 @@ struct AdjArgs {
   const int* count;     // cell counts of this parity
   int* err;
   int* maxcnt;
-  const double* A;      // ring area and its derivative along w
-  const double* dA;
   const double* wx;     // the vector w
   const double* wy;
   double* hx;           // mode 0: H w
@@ -87,8 +85,15 @@ struct AdjArgs {
   double* g;            // mode 1: per-vertex parameter partials [ka | kl | kb | l0] x nv, accumulated
   double cd;            // c dt
   double dt;
+  const double* px;     // positions processed next: their Verlet-list validity goes to flagOut/anyOut
+  const double* py;     //   (null: no test)
+  int* flagOut;
+  int* anyOut;
+  int* cellOther;       // cell counts of the other parity, cleared here (null: left alone)
+  int ncells;
   int mode;
   int update;
+  int xcd;              // XCD-aware workgroup -> tile mapping
 };
 
 }  // namespace dpm
@@ -186,11 +191,8 @@ hipError_t launch_fire_done(dpm_set_s* S, int* notDone);
 StepArgs loop_args(dpm_set_s* S);
 hipError_t launch_bin_view(dpm_set_s* S, const View& Vn, int parity);
 hipError_t launch_fire_sys(dpm_set_s* S, int chunk, int waves);
-hipError_t launch_adj_area(dpm_set_s* S, const View& Vn, const double* wx, const double* wy, double* A, double* dA);
 hipError_t launch_adj_flag(dpm_set_s* S, const View& Vn, int parity);
 hipError_t launch_adj_seg_check(dpm_set_s* S, const double* seg, int slots, int* bad);
-hipError_t launch_adj_prep(dpm_set_s* S, const View& Vn, int parity, const double* wx, const double* wy, double* A,
-                           double* dA);
 hipError_t launch_adj_hvp(dpm_set_s* S, const View& Vn, AdjArgs A);
 hipError_t launch_energy_param_partials(dpm_set_s* S, double* out);
 hipError_t launch_ring_sum3(dpm_set_s* S, const double* in, double* out);
