@@ -290,14 +290,16 @@ def adjoint_bench(args, torch):
 
 
 CFG5_SEED = 20261202 + 5
+CFG5_COSTS = (1e5, 20.0, 10.0, 1.0)  # ea, el, eb, ec of the active driver, runActive.cpp:39
 
 
 def cfg5_bench(args, torch, batched=False):
     """Large active + adhesive: rings of 32 vertices in two sizes (ratio 1.4), vertex-segment
     interaction with the adhesive potential (l1 = 0.1, l2 = 0.2), active Brownian driving
-    f0 = v0 gamma with v0 = 0.1 and taup = 1/Dr = 100, phi 0.90.  One system of 16384 rings, or
-    (batched) 64 systems with N ~ U{2048..8192} in one set.  Compressed with harmonic FIRE from
-    phi 0.5 before timing."""
+    f0 = v0 gamma with v0 = 0.1 and taup = 1/Dr = 100, phi 0.90, shape costs of the reference's
+    active driver.  One system of 16384 rings, or (batched) 64 systems with N ~ U{2048..8192} in
+    one set.  Compressed with harmonic vertex-vertex FIRE from phi 0.5 (ka = kl = 1), then relaxed
+    with the production interaction and costs, before timing."""
     from cudaring_amd import DPMSet, autotune, packing, protocols
     rng = np.random.default_rng(CFG5_SEED)
     sizes = rng.integers(2048, 8193, 64) if batched else [16384]
@@ -313,23 +315,25 @@ def cfg5_bench(args, torch, batched=False):
     protocols.compress(s, 0.5, 0.90, dphi=args.cfg5_dphi, iters=args.cfg5_iters,
                        progress=lambda phi: print(f"# compressed to phi {phi:.3f}", file=sys.stderr, flush=True))
     sigma = float(2 * np.sqrt(np.mean(s.get("a0")) / np.pi))
-    # relax with the production interaction before the dynamics (no deep overlaps left)
+    # the active driver's shape costs and the production interaction, relaxed before the dynamics
+    # (FIRE dt well inside the stability limit of the ea = 1e5 area mode)
     s.set_types("adhesive", "vertexSmooth")
     s.set_attraction(0.1, 0.2)
+    s.set_energy_costs(*CFG5_COSTS)
     size = 2 * float(np.mean(s.get("rad")))
-    s.calc_neighbor_list(s.set_displacement_cutoff(0.3, size))
-    s.calc_force_energy()
-    protocols.fire_relax(s, 0.01 * size, args.cfg5_iters)
-    s.set_types("adhesive", "vertexSmooth", particle="active")
-    s.set_self_propulsion(0.1 * GAMMA, 100.0 * sigma)
-    s.set("pangle", rng.uniform(-np.pi, np.pi, s.np))
-    s.set_dt(5e-3 * sigma)
-    s.set_seed(CFG5_SEED)
-    size = 2 * float(np.mean(s.get("rad")))
-    cut = s.set_displacement_cutoff(0.3, size)
+    cut = s.set_displacement_cutoff(SKIN, size)  # cutoff 0.5, runActive.cpp:37,130
     s.calc_neighbor_list(cut)
     s.calc_force_energy()
-    s.init_langevin(1e-4, GAMMA)
+    protocols.fire_relax(s, 0.002 * size, 2 * args.cfg5_iters)
+    s.set_types("adhesive", "vertexSmooth", particle="active")
+    # units of runActive.cpp:113-123: driving in ec / sigma, taup and 1 / damping in sigma / sqrt(ec)
+    s.set_self_propulsion(0.1 * GAMMA / sigma, 100.0 * sigma)
+    s.set("pangle", rng.uniform(-np.pi, np.pi, s.np))
+    s.set_dt(DT_UNITS * sigma)
+    s.set_seed(CFG5_SEED)
+    s.calc_neighbor_list(cut)
+    s.calc_force_energy()
+    s.init_langevin(1e-4, GAMMA / sigma)
     waves, wtimes = autotune.tune_waves_per_tile(s, reps=20)
     prep = time.perf_counter() - t0
     for _ in range(2):
