@@ -140,7 +140,7 @@ __global__ void k_adj_seg_check(View V, Params P, const double* seg, int slots, 
 // AdjArgs (dpm_internal.h): mode 0 writes H w; mode 1 applies the reverse-pass update.
 template <int POT>
 __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_adj_hvp(View V, Params P, CellGrid G, AdjArgs A) {
-  __shared__ double red[MAXSPLIT][WAVE][2];
+  extern __shared__ double red[];  // red[((w - 1) * 2 + q) * WAVE + lane], sized by the launch
   const int w = threadIdx.x / WAVE;
   const int nw = blockDim.x / WAVE;
   Lane L;
@@ -178,8 +178,8 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_adj_hvp(View V, Params P, Ce
     }
   }
   if (nw > 1 && w > 0) {
-    red[w][L.lane][0] = tx;
-    red[w][L.lane][1] = ty;
+    red[(w - 1) * 2 * WAVE + L.lane] = tx;
+    red[((w - 1) * 2 + 1) * WAVE + L.lane] = ty;
   }
 
   // ---- wave 0: shape force of the lane's vertex and its tangent
@@ -228,8 +228,8 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_adj_hvp(View V, Params P, Ce
     __syncthreads();
     if (w > 0) return;
     for (int q = 1; q < nw; q++) {
-      tx += red[q][L.lane][0];
-      ty += red[q][L.lane][1];
+      tx += red[(q - 1) * 2 * WAVE + L.lane];
+      ty += red[((q - 1) * 2 + 1) * WAVE + L.lane];
     }
   }
   if (!L.act) return;
@@ -323,12 +323,13 @@ hipError_t launch_adj_hvp(dpm_set_s* S, const View& Vn, AdjArgs A) {
   A.maxcnt = S->d_maxcnt;
   A.xcd = S->xcdMap;
   const unsigned g = (unsigned)S->V.ntiles, b = WAVE * S->split;
+  const size_t sh = (size_t)(S->split - 1) * 2 * WAVE * sizeof(double);
   hipStream_t st = S->stream;
   switch (S->P.potential) {
-    case POT_HARMONIC: k_adj_hvp<POT_HARMONIC><<<g, b, 0, st>>>(Vn, S->P, S->G, A); break;
-    case POT_LJ: k_adj_hvp<POT_LJ><<<g, b, 0, st>>>(Vn, S->P, S->G, A); break;
-    case POT_ADHESIVE: k_adj_hvp<POT_ADHESIVE><<<g, b, 0, st>>>(Vn, S->P, S->G, A); break;
-    default: k_adj_hvp<POT_WCA><<<g, b, 0, st>>>(Vn, S->P, S->G, A); break;
+    case POT_HARMONIC: k_adj_hvp<POT_HARMONIC><<<g, b, sh, st>>>(Vn, S->P, S->G, A); break;
+    case POT_LJ: k_adj_hvp<POT_LJ><<<g, b, sh, st>>>(Vn, S->P, S->G, A); break;
+    case POT_ADHESIVE: k_adj_hvp<POT_ADHESIVE><<<g, b, sh, st>>>(Vn, S->P, S->G, A); break;
+    default: k_adj_hvp<POT_WCA><<<g, b, sh, st>>>(Vn, S->P, S->G, A); break;
   }
   return hipGetLastError();
 }

@@ -172,12 +172,18 @@ __device__ __forceinline__ void gather_pairs(const View& V, const Params& P, con
                                              double y, int k0, int kstep, double& fx, double& fy, double& e,
                                              double& es, double& qx, double& qy) {
   const double radv = V.rad[L.v];
+  // Row entries are fetched one iteration ahead, the first one before the row length is known:
+  // rows are W entries wide (unused entries hold -1), so k < W keeps every fetch in bounds, and
+  // an entry is only used once k < ncnt is known.
   if constexpr (INTER == INT_VV) {
     // kernelCalcVertexInteraction, cudaKernel.cuh:690-722
+    int j = k0 < V.W ? V.nbr[(long)k0 * V.nv + L.v] : 0;
     const int c = L.act ? V.ncnt[L.v] : 0;
     for (int k = k0; k < c; k += kstep) {
-      const int j = V.nbr[(long)k * V.nv + L.v];
+      const int kn = k + kstep;
+      const int jn = kn < V.W ? V.nbr[(long)kn * V.nv + L.v] : 0;
       e += pair_force<POT>(P, pbx(x - V.x[j], B), pby(y - V.y[j], B), radv + V.rad[j], fx, fy);
+      j = jn;
     }
   } else if constexpr (INTER == INT_SMOOTH) {
     // kernelCalcSmoothInteraction, cudaKernel.cuh:922-975, as a gather: for every u in list(v),
@@ -188,16 +194,28 @@ __device__ __forceinline__ void gather_pairs(const View& V, const Params& P, con
     const int vpp = L.act ? L.v0 + ((L.pl > L.segS) ? L.pl - 1 : L.segE) : L.v;
     const double xp = V.x[vp], yp = V.y[vp];
     const double xpp = V.x[vpp], ypp = V.y[vpp];
+    int u = 0;
+    int2 pp = make_int2(0, 0);
+    if (k0 < V.W) {
+      u = V.nbr[(long)k0 * V.nv + L.v];
+      pp = V.nbrp[(long)k0 * V.nv + L.v];
+    }
     const int c = L.act ? V.ncnt[L.v] : 0;
     for (int k = k0; k < c; k += kstep) {
-      const long idx = (long)k * V.nv + L.v;
-      const int u = V.nbr[idx];
-      const int2 pp = V.nbrp[idx];
+      const int kn = k + kstep;
+      int un = 0;
+      int2 ppn = make_int2(0, 0);
+      if (kn < V.W) {
+        un = V.nbr[(long)kn * V.nv + L.v];
+        ppn = V.nbrp[(long)kn * V.nv + L.v];
+      }
       const double xu = V.x[u], yu = V.y[u];
       const double radSum = radv + V.rad[u];
       es += smooth_pair<POT, ROLE_THIS>(P, B, x, y, xu, yu, V.x[pp.x], V.y[pp.x], V.x[pp.y], V.y[pp.y], radSum,
                                         fx, fy);
       es += smooth_pair_op<POT>(P, B, xu, yu, x, y, xp, yp, xpp, ypp, radSum, fx, fy, qx, qy);
+      u = un;
+      pp = ppn;
     }
   } else {
     // kernelCalcAllToAllVertexInteraction, cudaKernel.cuh:725-757 (test mode)
@@ -216,7 +234,8 @@ __device__ __forceinline__ void gather_pairs(const View& V, const Params& P, con
 // while waves 1..nw-1 share the neighbour gather; with nw == 1 wave 0 does both in turn.
 template <int INTER, int POT, int MODE>
 __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepArgs A, CellGrid G) {
-  __shared__ double red[MAXSPLIT][WAVE][6];
+  // partials of gather waves 1..nw-1: red[((w - 1) * 6 + q) * WAVE + lane], sized by the launch
+  extern __shared__ double red[];
   const int w = threadIdx.x / WAVE;
   const int nw = blockDim.x / WAVE;
   Lane L;
@@ -235,6 +254,18 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
     if (rebuild && w == 0 && L.act) build_row(V, P, G, A.cellCur, L.v, L.s, B, A.err, A.maxcnt);
     if (rebuild) __syncthreads();
   }
+  // wave 0's epilogue operands, fetched before the force work (lastPos after the rebuild above)
+  double vx = 0, vy = 0, lx = 0, ly = 0;
+  if (w == 0) {
+    if constexpr (MODE != MODE_FORCE) {
+      vx = V.vx[L.v];
+      vy = V.vy[L.v];
+    }
+    if constexpr ((MODE & MODE_START) != 0) {
+      lx = V.lx[L.v];
+      ly = V.ly[L.v];
+    }
+  }
 
   // gather partials: force on v, vertex-vertex energy, smooth energy, share for prev(v)
   double fx = 0, fy = 0, e = 0, es = 0, qx = 0, qy = 0;
@@ -242,12 +273,13 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
   const int ng = nw - g0;
   if (w >= g0) gather_pairs<INTER, POT>(V, P, L, B, x, y, w - g0, ng, fx, fy, e, es, qx, qy);
   if (nw > 1 && w > 0) {
-    red[w][L.lane][0] = fx;
-    red[w][L.lane][1] = fy;
-    red[w][L.lane][2] = e;
-    red[w][L.lane][3] = es;
-    red[w][L.lane][4] = qx;
-    red[w][L.lane][5] = qy;
+    double* r = red + (w - 1) * 6 * WAVE + L.lane;
+    r[0 * WAVE] = fx;
+    r[1 * WAVE] = fy;
+    r[2 * WAVE] = e;
+    r[3 * WAVE] = es;
+    r[4 * WAVE] = qx;
+    r[5 * WAVE] = qy;
   }
 
   // ---- wave 0: work that does not need the gather, overlapped with it
@@ -274,12 +306,13 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
     __syncthreads();
     if (w > 0) return;
     for (int q = 1; q < nw; q++) {
-      fx += red[q][L.lane][0];
-      fy += red[q][L.lane][1];
-      e += red[q][L.lane][2];
-      es += red[q][L.lane][3];
-      qx += red[q][L.lane][4];
-      qy += red[q][L.lane][5];
+      const double* r = red + (q - 1) * 6 * WAVE + L.lane;
+      fx += r[0 * WAVE];
+      fy += r[1 * WAVE];
+      e += r[2 * WAVE];
+      es += r[3 * WAVE];
+      qx += r[4 * WAVE];
+      qy += r[5 * WAVE];
     }
   }
 
@@ -301,11 +334,6 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
     fy += P.driving * ((1 - 1) * cos(ang) + 1 * sin(ang));
   }
 
-  double vx = 0, vy = 0;
-  if constexpr (MODE != MODE_FORCE) {
-    vx = V.vx[L.v];
-    vy = V.vy[L.v];
-  }
   if constexpr ((MODE & MODE_LANGEVIN) != 0) {
     fx += P.noise * zt0 - P.gamma * vx;
     fy += P.noise * zt1 - P.gamma * vy;
@@ -349,7 +377,7 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
       A.recx[L.v] = xn;
       A.recy[L.v] = yn;
     }
-    const double ddx = pbx(xn - V.lx[L.v], B), ddy = pby(yn - V.ly[L.v], B);
+    const double ddx = pbx(xn - lx, B), ddy = pby(yn - ly, B);
     if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) {
       atomicOr(&A.flagOut[L.s], 1);
       atomicOr(A.anyOut, 1);
@@ -832,24 +860,28 @@ __global__ __launch_bounds__(WAVE* FS_WAVES) void k_fire_sys(View V, Params P, F
 static inline int grid_v(long n) { return (int)((n + 255) / 256); }
 static inline int grid_w(int ntiles) { return (ntiles + 3) / 4; }
 
+// LDS of the step kernel: six partials per lane of each gather wave
+static inline size_t step_lds(int split) { return (size_t)(split - 1) * 6 * WAVE * sizeof(double); }
+
 template <int INTER, int POT>
 static hipError_t step_mode(dpm_set_s* S, int mode, const StepArgs& A) {
   const int g = S->V.ntiles;
   const int b = WAVE * S->split;
+  const size_t sh = step_lds(S->split);
   hipStream_t st = S->stream;
   switch (mode) {
-    case MODE_FORCE: k_step<INTER, POT, MODE_FORCE><<<g, b, 0, st>>>(S->V, S->P, A, S->G); break;
-    case MODE_KICK: k_step<INTER, POT, MODE_KICK><<<g, b, 0, st>>>(S->V, S->P, A, S->G); break;
+    case MODE_FORCE: k_step<INTER, POT, MODE_FORCE><<<g, b, sh, st>>>(S->V, S->P, A, S->G); break;
+    case MODE_KICK: k_step<INTER, POT, MODE_KICK><<<g, b, sh, st>>>(S->V, S->P, A, S->G); break;
     case MODE_KICK | MODE_LANGEVIN:
-      k_step<INTER, POT, MODE_KICK | MODE_LANGEVIN><<<g, b, 0, st>>>(S->V, S->P, A, S->G);
+      k_step<INTER, POT, MODE_KICK | MODE_LANGEVIN><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
       break;
     case MODE_KICK | MODE_START:
-      k_step<INTER, POT, MODE_KICK | MODE_START><<<g, b, 0, st>>>(S->V, S->P, A, S->G);
+      k_step<INTER, POT, MODE_KICK | MODE_START><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
       break;
     case MODE_KICK | MODE_LANGEVIN | MODE_START:
-      k_step<INTER, POT, MODE_KICK | MODE_LANGEVIN | MODE_START><<<g, b, 0, st>>>(S->V, S->P, A, S->G);
+      k_step<INTER, POT, MODE_KICK | MODE_LANGEVIN | MODE_START><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
       break;
-    case MODE_FIRE: k_step<INTER, POT, MODE_FIRE><<<g, b, 0, st>>>(S->V, S->P, A, S->G); break;
+    case MODE_FIRE: k_step<INTER, POT, MODE_FIRE><<<g, b, sh, st>>>(S->V, S->P, A, S->G); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -879,12 +911,13 @@ hipError_t launch_step(dpm_set_s* S, int mode, const StepArgs& A) {
     case INT_ALL: {
       if (mode != MODE_FORCE) return hipErrorInvalidValue;
       const int g = S->V.ntiles, b = WAVE * S->split;
+      const size_t sh = step_lds(S->split);
       hipStream_t st = S->stream;
       switch (S->P.potential) {
-        case POT_HARMONIC: k_step<INT_ALL, POT_HARMONIC, MODE_FORCE><<<g, b, 0, st>>>(S->V, S->P, A, S->G); break;
-        case POT_LJ: k_step<INT_ALL, POT_LJ, MODE_FORCE><<<g, b, 0, st>>>(S->V, S->P, A, S->G); break;
-        case POT_ADHESIVE: k_step<INT_ALL, POT_ADHESIVE, MODE_FORCE><<<g, b, 0, st>>>(S->V, S->P, A, S->G); break;
-        case POT_WCA: k_step<INT_ALL, POT_WCA, MODE_FORCE><<<g, b, 0, st>>>(S->V, S->P, A, S->G); break;
+        case POT_HARMONIC: k_step<INT_ALL, POT_HARMONIC, MODE_FORCE><<<g, b, sh, st>>>(S->V, S->P, A, S->G); break;
+        case POT_LJ: k_step<INT_ALL, POT_LJ, MODE_FORCE><<<g, b, sh, st>>>(S->V, S->P, A, S->G); break;
+        case POT_ADHESIVE: k_step<INT_ALL, POT_ADHESIVE, MODE_FORCE><<<g, b, sh, st>>>(S->V, S->P, A, S->G); break;
+        case POT_WCA: k_step<INT_ALL, POT_WCA, MODE_FORCE><<<g, b, sh, st>>>(S->V, S->P, A, S->G); break;
       }
       return hipGetLastError();
     }
