@@ -56,8 +56,10 @@ struct View {
   int ntiles;
 };
 
-// Box of one system with its reciprocal: the minimum image multiplies by 1/L instead of
-// dividing; round(d * (1/L)) differs from round(d / L) only at exact half-box ties.
+// Box of one system with its reciprocal: the minimum image multiplies by 1/L instead of dividing
+// and rounds to nearest-even (one instruction) instead of half away from zero; both differ from
+// the reference's round(d / L) only when d is an exact odd multiple of half the box, where the
+// minimum image itself is ambiguous (and far beyond every interaction range).
 struct Box {
   double Lx, Ly, iLx, iLy;
 };
@@ -67,8 +69,8 @@ struct Box {
 __device__ __forceinline__ double pbc(double delta, double size) {
   return delta - size * round(delta / size);
 }
-__device__ __forceinline__ double pbx(double delta, const Box& b) { return delta - b.Lx * round(delta * b.iLx); }
-__device__ __forceinline__ double pby(double delta, const Box& b) { return delta - b.Ly * round(delta * b.iLy); }
+__device__ __forceinline__ double pbx(double delta, const Box& b) { return delta - b.Lx * rint(delta * b.iLx); }
+__device__ __forceinline__ double pby(double delta, const Box& b) { return delta - b.Ly * rint(delta * b.iLy); }
 
 // ----------------------------------------------------------- Philox RNG ---
 // Philox4x32-10 (Salmon et al., SC'11): 10 rounds, 9 key bumps.
@@ -109,26 +111,29 @@ __device__ __forceinline__ void gauss_pair(uint64_t seed, uint32_t sys, uint32_t
 template <int POT>
 __device__ __forceinline__ double pair_force(const Params& P, double dx, double dy, double radSum,
                                              double& fx, double& fy) {
+  // the force formulas multiply by 1/distance (one division) where the reference divides
   double distance = sqrt(dx * dx + dy * dy);
   if constexpr (POT == POT_HARMONIC) {
     double overlap = 1 - distance / radSum;
     if (overlap > 0) {
       double gm = P.ec * overlap / radSum;
-      fx += gm * dx / distance;
-      fy += gm * dy / distance;
+      const double id = 1.0 / distance;
+      fx += gm * dx * id;
+      fy += gm * dy * id;
       return (0.5 * P.ec * overlap * overlap) * 0.5;
     }
     return 0.0;
   } else if constexpr (POT == POT_LJ) {
-    double ratio = radSum / distance;
-    double r2 = ratio * ratio;
-    double ratio6 = r2 * r2 * r2;
-    double ratio12 = ratio6 * ratio6;
     if (distance <= (P.LJcutoff * radSum)) {
+      const double id = 1.0 / distance;
+      double ratio = radSum * id;
+      double r2 = ratio * ratio;
+      double ratio6 = r2 * r2 * r2;
+      double ratio12 = ratio6 * ratio6;
       double forceShift = P.LJfshift / radSum;
-      double gm = 24 * P.ec * (2 * ratio12 - ratio6) / distance - forceShift;
-      fx += gm * dx / distance;
-      fy += gm * dy / distance;
+      double gm = 24 * P.ec * (2 * ratio12 - ratio6) * id - forceShift;
+      fx += gm * dx * id;
+      fy += gm * dy * id;
       return 0.5 * (4 * P.ec * (ratio12 - ratio6) - P.LJecut - fabs(forceShift) * (distance - P.LJcutoff * radSum));
     }
     return 0.0;
@@ -143,20 +148,22 @@ __device__ __forceinline__ double pair_force(const Params& P, double dx, double 
         gm = -(P.ec * P.l1 / (P.l2 - P.l1)) * (overlap + P.l2) / radSum;
         epot = -(0.5 * (P.ec * P.l1 / (P.l2 - P.l1)) * (overlap + P.l2) * (overlap + P.l2)) * 0.5;
       }
-      fx += gm * dx / distance;
-      fy += gm * dy / distance;
+      const double id = 1.0 / distance;
+      fx += gm * dx * id;
+      fy += gm * dy * id;
       return epot;
     }
     return 0.0;
   } else {  // WCA
-    double ratio = radSum / distance;
-    double r2 = ratio * ratio;
-    double ratio6 = r2 * r2 * r2;
-    double ratio12 = ratio6 * ratio6;
     if (distance <= (kWCAcut * radSum)) {
-      double gm = 24 * P.ec * (2 * ratio12 - ratio6) / distance;
-      fx += gm * dx / distance;
-      fy += gm * dy / distance;
+      const double id = 1.0 / distance;
+      double ratio = radSum * id;
+      double r2 = ratio * ratio;
+      double ratio6 = r2 * r2 * r2;
+      double ratio12 = ratio6 * ratio6;
+      double gm = 24 * P.ec * (2 * ratio12 - ratio6) * id;
+      fx += gm * dx * id;
+      fy += gm * dy * id;
       return 0.5 * P.ec * (4 * (ratio12 - ratio6) + 1);
     }
     return 0.0;
@@ -174,14 +181,15 @@ __device__ __forceinline__ double grad_multiple(const Params& P, double distance
     }
     return 0;
   } else if constexpr (POT == POT_LJ) {
-    double ratio = radSum / distance;
-    double r2 = ratio * ratio;
-    double ratio6 = r2 * r2 * r2;
-    double ratio12 = ratio6 * ratio6;
     if (distance <= (P.LJcutoff * radSum)) {
+      const double id = 1.0 / distance;
+      double ratio = radSum * id;
+      double r2 = ratio * ratio;
+      double ratio6 = r2 * r2 * r2;
+      double ratio12 = ratio6 * ratio6;
       double forceShift = P.LJfshift / radSum;
       epot = 0.5 * (4 * P.ec * (ratio12 - ratio6) - P.LJecut - fabs(forceShift) * (distance - P.LJcutoff * radSum));
-      return 24 * P.ec * (2 * ratio12 - ratio6) / distance - forceShift;
+      return 24 * P.ec * (2 * ratio12 - ratio6) * id - forceShift;
     }
     return 0;
   } else if constexpr (POT == POT_ADHESIVE) {
@@ -195,13 +203,14 @@ __device__ __forceinline__ double grad_multiple(const Params& P, double distance
     }
     return 0;
   } else {
-    double ratio = radSum / distance;
-    double r2 = ratio * ratio;
-    double ratio6 = r2 * r2 * r2;
-    double ratio12 = ratio6 * ratio6;
     if (distance <= (kWCAcut * radSum)) {
+      const double id = 1.0 / distance;
+      double ratio = radSum * id;
+      double r2 = ratio * ratio;
+      double ratio6 = r2 * r2 * r2;
+      double ratio12 = ratio6 * ratio6;
       epot = 0.5 * P.ec * (4 * (ratio12 - ratio6) + 1);
-      return 4 * P.ec * (12 * ratio12 - 6 * ratio6) / distance;
+      return 4 * P.ec * (12 * ratio12 - 6 * ratio6) * id;
     }
     return 0;
   }
@@ -219,6 +228,11 @@ __device__ __forceinline__ double grad_multiple(const Params& P, double distance
 // and ROLE_OTHER.  Inputs: ti = r_i, tj = r_j, tp = r_prev(j), tq = r_prev(prev(j)).
 enum { ROLE_THIS = 0, ROLE_OTHER = 1, ROLE_PREV = 2 };
 
+// Shared by both: with sg = r_j - r_p, ip = r_i - r_p, ps = r_q - r_p and jt = r_j - r_i (each one
+// minimum image), every difference the reference forms is one of these or its exact negative
+// (the minimum image is odd), the sign cross/|cross| is a comparison, and the divisions by the
+// segment length are multiplications by its reciprocal.  The two projections, which select the
+// branches, are formed exactly as the reference forms them.
 template <int POT, int ROLE>
 __device__ __forceinline__ double smooth_pair(const Params& P, const Box& B, double tix, double tiy, double tjx,
                                               double tjy, double tpx, double tpy, double tqx, double tqy,
@@ -228,58 +242,60 @@ __device__ __forceinline__ double smooth_pair(const Params& P, const Box& B, dou
   const double length = sqrt(sgx * sgx + sgy * sgy);
   const double ipx = pbx(tix - tpx, B), ipy = pby(tiy - tpy, B);  // this - previous
   const double proj = (ipx * sgx + ipy * sgy) / (length * length);
-  // previous segment = secondPrevious - previous
+  // previous segment = secondPrevious - previous; previous - secondPrevious = -ps
   const double psx = pbx(tqx - tpx, B), psy = pby(tqy - tpy, B);
   const double previousLength = sqrt(psx * psx + psy * psy);
-  const double previousProj = (pbx(tix - tqx, B) * pbx(tpx - tqx, B) + pby(tiy - tqy, B) * pby(tpy - tqy, B)) /
-                              (previousLength * previousLength);
+  const double previousProj =
+      (pbx(tix - tqx, B) * -psx + pby(tiy - tqy, B) * -psy) / (previousLength * previousLength);
   double energy = 0.0;
   if (proj >= 0 && proj < 1) {
-    const double reduced = fmax(0.0, fmin(1.0, proj));
-    const double qx = tpx + reduced * sgx, qy = tpy + reduced * sgy;
+    const double qx = tpx + proj * sgx, qy = tpy + proj * sgy;  // projection point (proj in [0, 1))
     const double dqx = pbx(tix - qx, B), dqy = pby(tiy - qy, B);
     double epot = 0.0;
     const double gm = grad_multiple<POT>(P, sqrt(dqx * dqx + dqy * dqy), radSum, epot);
     if (gm != 0) {
       // calcCross: (prev - other)_x (other - this)_y - (other - this)_x (prev - other)_y
-      const double pox = pbx(tpx - tjx, B), poy = pby(tpy - tjy, B);
-      const double cross = pox * pby(tjy - tiy, B) - pbx(tjx - tix, B) * poy;
+      const double jtx = pbx(tjx - tix, B), jty = pby(tjy - tiy, B);
+      const double cross = -sgx * jty - jtx * -sgy;
       const double absCross = fabs(cross);
-      const double sign = absCross > 0 ? cross / absCross : 0.0;
+      const double sign = cross > 0 ? 1.0 : (cross < 0 ? -1.0 : 0.0);
+      const double il = 1.0 / length;
       if constexpr (ROLE == ROLE_THIS) {
-        fx += gm * sign * pby(tpy - tjy, B) / length;
-        fy += gm * sign * pbx(tjx - tpx, B) / length;
+        fx += gm * sign * -sgy * il;
+        fy += gm * sign * sgx * il;
       } else if constexpr (ROLE == ROLE_OTHER) {
-        fx += gm * (sign * pby(tiy - tpy, B) + absCross * pox / (length * length)) / length;
-        fy += gm * (sign * pbx(tpx - tix, B) + absCross * poy / (length * length)) / length;
+        const double c = absCross * il * il;
+        fx += gm * (sign * ipy + c * -sgx) * il;
+        fy += gm * (sign * -ipx + c * -sgy) * il;
       } else {
-        fx += gm * (sign * pby(tjy - tiy, B) - absCross * pox / (length * length)) / length;
-        fy += gm * (sign * pbx(tix - tjx, B) - absCross * poy / (length * length)) / length;
+        const double c = absCross * il * il;
+        fx += gm * (sign * jty - c * -sgx) * il;
+        fy += gm * (sign * -jtx - c * -sgy) * il;
       }
       energy += epot;
     }
     if (previousProj >= 0 && previousProj < 1) {
       // concave: subtract the pair (prev(j), i); calcVertexVertexInteraction(previousPos, thisPos)
-      const double dx = pbx(tpx - tix, B), dy = pby(tpy - tiy, B);
+      const double dx = -ipx, dy = -ipy;
       const double d = sqrt(dx * dx + dy * dy);
       double ep = 0.0;
       const double g = grad_multiple<POT>(P, d, radSum, ep);
       if (g != 0) {
-        if constexpr (ROLE == ROLE_THIS) { fx += g * dx / d; fy += g * dy / d; }
-        if constexpr (ROLE == ROLE_PREV) { fx += -g * dx / d; fy += -g * dy / d; }
+        if constexpr (ROLE == ROLE_THIS) { fx += g * dx * (1.0 / d); fy += g * dy * (1.0 / d); }
+        if constexpr (ROLE == ROLE_PREV) { fx += -g * dx * (1.0 / d); fy += -g * dy * (1.0 / d); }
         energy -= ep;
       }
     }
   } else if (proj < 0) {
     if (previousProj >= 1) {
       // convex corner: add the pair (i, prev(j)); calcVertexVertexInteraction(thisPos, previousPos)
-      const double dx = pbx(tix - tpx, B), dy = pby(tiy - tpy, B);
+      const double dx = ipx, dy = ipy;
       const double d = sqrt(dx * dx + dy * dy);
       double ep = 0.0;
       const double g = grad_multiple<POT>(P, d, radSum, ep);
       if (g != 0) {
-        if constexpr (ROLE == ROLE_THIS) { fx += g * dx / d; fy += g * dy / d; }
-        if constexpr (ROLE == ROLE_PREV) { fx += -g * dx / d; fy += -g * dy / d; }
+        if constexpr (ROLE == ROLE_THIS) { fx += g * dx * (1.0 / d); fy += g * dy * (1.0 / d); }
+        if constexpr (ROLE == ROLE_PREV) { fx += -g * dx * (1.0 / d); fy += -g * dy * (1.0 / d); }
         energy += ep;
       }
     }
@@ -300,46 +316,49 @@ __device__ __forceinline__ double smooth_pair_op(const Params& P, const Box& B, 
   const double proj = (ipx * sgx + ipy * sgy) / (length * length);
   const double psx = pbx(tqx - tpx, B), psy = pby(tqy - tpy, B);
   const double previousLength = sqrt(psx * psx + psy * psy);
-  const double previousProj = (pbx(tix - tqx, B) * pbx(tpx - tqx, B) + pby(tiy - tqy, B) * pby(tpy - tqy, B)) /
-                              (previousLength * previousLength);
+  const double previousProj =
+      (pbx(tix - tqx, B) * -psx + pby(tiy - tqy, B) * -psy) / (previousLength * previousLength);
   double energy = 0.0;
   if (proj >= 0 && proj < 1) {
-    const double reduced = fmax(0.0, fmin(1.0, proj));
-    const double qx = tpx + reduced * sgx, qy = tpy + reduced * sgy;
+    const double qx = tpx + proj * sgx, qy = tpy + proj * sgy;
     const double dqx = pbx(tix - qx, B), dqy = pby(tiy - qy, B);
     double epot = 0.0;
     const double gm = grad_multiple<POT>(P, sqrt(dqx * dqx + dqy * dqy), radSum, epot);
     if (gm != 0) {
-      const double pox = pbx(tpx - tjx, B), poy = pby(tpy - tjy, B);
-      const double cross = pox * pby(tjy - tiy, B) - pbx(tjx - tix, B) * poy;
+      const double jtx = pbx(tjx - tix, B), jty = pby(tjy - tiy, B);
+      const double cross = -sgx * jty - jtx * -sgy;
       const double absCross = fabs(cross);
-      const double sign = absCross > 0 ? cross / absCross : 0.0;
-      ox += gm * (sign * pby(tiy - tpy, B) + absCross * pox / (length * length)) / length;
-      oy += gm * (sign * pbx(tpx - tix, B) + absCross * poy / (length * length)) / length;
-      px += gm * (sign * pby(tjy - tiy, B) - absCross * pox / (length * length)) / length;
-      py += gm * (sign * pbx(tix - tjx, B) - absCross * poy / (length * length)) / length;
+      const double sign = cross > 0 ? 1.0 : (cross < 0 ? -1.0 : 0.0);
+      const double il = 1.0 / length;
+      const double c = absCross * il * il;
+      ox += gm * (sign * ipy + c * -sgx) * il;
+      oy += gm * (sign * -ipx + c * -sgy) * il;
+      px += gm * (sign * jty - c * -sgx) * il;
+      py += gm * (sign * -jtx - c * -sgy) * il;
       energy += epot;
     }
     if (previousProj >= 0 && previousProj < 1) {
-      const double dx = pbx(tpx - tix, B), dy = pby(tpy - tiy, B);
+      const double dx = -ipx, dy = -ipy;
       const double d = sqrt(dx * dx + dy * dy);
       double ep = 0.0;
       const double g = grad_multiple<POT>(P, d, radSum, ep);
       if (g != 0) {
-        px += -g * dx / d;
-        py += -g * dy / d;
+        const double id = 1.0 / d;
+        px += -g * dx * id;
+        py += -g * dy * id;
         energy -= ep;
       }
     }
   } else if (proj < 0) {
     if (previousProj >= 1) {
-      const double dx = pbx(tix - tpx, B), dy = pby(tiy - tpy, B);
+      const double dx = ipx, dy = ipy;
       const double d = sqrt(dx * dx + dy * dy);
       double ep = 0.0;
       const double g = grad_multiple<POT>(P, d, radSum, ep);
       if (g != 0) {
-        px += -g * dx / d;
-        py += -g * dy / d;
+        const double id = 1.0 / d;
+        px += -g * dx * id;
+        py += -g * dy * id;
         energy += ep;
       }
     }

@@ -64,9 +64,9 @@ __device__ __forceinline__ ShapeOut ring_shape(const View& V, const Params& P, c
   const double shoeSum = seg_sum(shoe, L.lane, L.segS, L.segE);
   const double sux = seg_sum(ux, L.lane, L.segS, L.segE);
   const double suy = seg_sum(uy, L.lane, L.segS, L.segE);
-  const double nd = (double)L.n;
+  const double nd = (double)L.n, ind = 1.0 / nd;
   const double area = fabs(shoeSum) * 0.5;
-  const double comx = xf + sux / nd, comy = yf + suy / nd;
+  const double comx = xf + sux * ind, comy = yf + suy * ind;
   if (write && L.act) {
     V.th[L.v] = theta;
     V.len[L.v] = seglen;
@@ -91,7 +91,7 @@ __device__ __forceinline__ ShapeOut ring_shape(const View& V, const Params& P, c
   const double gmA = ea * deltaA / a0;
   double fx = 0.5 * gmA * (rpy - rny);
   double fy = 0.5 * gmA * (rnx - rpx);
-  double e = (0.5 * ea * deltaA * deltaA) / nd;
+  double e = (0.5 * ea * deltaA * deltaA) * ind;
   // perimeter term (calcPerimeterForceEnergy 437-447 / FENE 449-455)
   const double nsx = rnx - rvx, nsy = rny - rvy;
   const double qsx = rvx - rpx, qsy = rvy - rpy;
@@ -102,8 +102,9 @@ __device__ __forceinline__ ShapeOut ring_shape(const View& V, const Params& P, c
   if (P.monomer == MONO_HARMONIC) {
     const double pDelta = (pLen / l0p) - 1.;
     const double nDelta = (nLen / l0v) - 1.;
-    fx += el * ((nDelta * (rnx - rvx) / (l0v * nLen)) - (pDelta * (rvx - rpx) / (l0p * pLen)));
-    fy += el * ((nDelta * (rny - rvy) / (l0v * nLen)) - (pDelta * (rvy - rpy) / (l0p * pLen)));
+    const double cn = nDelta / (l0v * nLen), cp = pDelta / (l0p * pLen);
+    fx += el * ((cn * (rnx - rvx)) - (cp * (rvx - rpx)));
+    fy += el * ((cn * (rny - rvy)) - (cp * (rvy - rpy)));
     e += (0.5 * el * pDelta * pDelta);
   } else {
     const double c = 2 * el * P.stiff * P.extSq;

@@ -249,8 +249,11 @@ def test_cfg1_nve_full_length(oracle_mod):
 
 @pytest.mark.parametrize("pot,inter", [("harmonic", "vertexVertex"), ("wca", "vertexSmooth")])
 def test_persistent_fire_matches_step_kernels(pot, inter):
-    """fire_minimize on one workgroup per system (tol 0, 60 iterations) against 60 iterations of
-    the split FIRE step kernels, on a batched set with rebuilds along the way."""
+    """fire_minimize on one workgroup per system (tol 0, 30 iterations) against 30 iterations of
+    the split FIRE step kernels, on a batched set with rebuilds along the way.  The two paths add
+    the pair forces in different orders; at dt_max the stiff WCA trajectory amplifies that
+    roundoff about 1e3-fold per 10 iterations (1e-16 at iteration 20, 7e-13 in the velocities at
+    30, 2e-9 at 40), so the positions are compared at 30 iterations."""
     print(f"case persistent_fire {pot} {inter}", flush=True)
     systems = [relaxed_state(n, 0.8, 500 + n, final=(pot, inter)) for n in (9, 16)]
     cat = packing.concat_systems(systems)
@@ -266,9 +269,9 @@ def test_persistent_fire_matches_step_kernels(pot, inter):
         sets.append(g)
     a, b = sets
     a.set_fire_kernel(True)
-    it = a.fire_minimize(0.0, 60)
-    b.fire(60)
-    assert list(it) == [60, 60]
+    it = a.fire_minimize(0.0, 30)
+    b.fire(30)
+    assert list(it) == [30, 30]
     fa, fb = a.fire_state(), b.fire_state()
     assert list(fa["numStep"]) == list(fb["numStep"])
     np.testing.assert_allclose(fa["dt"], fb["dt"], rtol=1e-13)
