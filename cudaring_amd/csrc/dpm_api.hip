@@ -239,6 +239,8 @@ int reduce_host(dpm_set_s* S) {
 }
 
 int require_list(dpm_set_s* S) {
+  if (S->P.particle == PART_ACTIVE_SHAPE && !S->V.l0r)
+    return fail(S, DPM_ERR_STATE, "activeShape: dpm_set_active_tension not called");
   if (S->P.interaction == INT_ALL) return DPM_OK;
   if (!S->listBuilt) return fail(S, DPM_ERR_STATE, "neighbour list not built: call dpm_calc_neighbor_list first");
   return DPM_OK;
@@ -247,7 +249,7 @@ int require_list(dpm_set_s* S) {
 // One loop step's launches at position parity cur: bin (normally idle) + step.  The FIRE step
 // also needs the per-system FIRE state and the tile partial buffer.
 int loop_step(dpm_set_s* S, int mode, int bumpT, double* recx = nullptr, double* recy = nullptr) {
-  HIPCHK(launch_bin(S, bumpT, S->P.particle == PART_ACTIVE ? 1 : 0));
+  HIPCHK(launch_bin(S, bumpT, S->P.particle != PART_PASSIVE ? 1 : 0));
   if (mode == MODE_FIRE) {
     HIPCHK(launch_fire_step(S));
     return DPM_OK;
@@ -286,6 +288,39 @@ int verlet_sequence(dpm_set_s* S, long steps, bool langevin, double* rec = nullp
     int rc = loop_step(S, mode, langevin ? 1 : 0, rx, rx ? rx + nv : nullptr);
     if (rc) return rc;
     if (more) swap_pos(S);
+  }
+  return DPM_OK;
+}
+
+// BAOAB loop (BAOAB::integrate, Simulator.cu:112-120): the first call's B A O A from the stored
+// force, then per step bin + step [F(x'); v += dt/2 F; unless last: B A O A].  The O step of
+// call k draws at counter k (k_start_baoab reads the counter before the first bump).
+int baoab_sequence(dpm_set_s* S, long steps) {
+  const double h = 0.5 * S->P.dt;
+  HIPCHK(launch_fill_int(S->d_flag[1 - S->cur], 0, S->nsys, S->stream));
+  HIPCHK(launch_fill_int(S->d_any + 1 - S->cur, 0, 1, S->stream));
+  HIPCHK(launch_start_baoab(S, h));
+  swap_pos(S);
+  for (long i = 0; i < steps; i++) {
+    const bool more = i + 1 < steps;
+    int rc = loop_step(S, MODE_KICK | MODE_BAOAB | (more ? MODE_START : 0), 1);
+    if (rc) return rc;
+    if (more) swap_pos(S);
+  }
+  return DPM_OK;
+}
+
+// Overdamped loop (Brownian::integrate 147-152, DrivenBrownian::integrate 182-187): the list
+// validity of the current positions first, then per step bin + step [rebuild if flagged;
+// F(x); v = (F + noise xi) / gamma or F / gamma; x' = x + dt v; displacement test].
+int brown_sequence(dpm_set_s* S, long steps, bool noisy) {
+  HIPCHK(launch_fill_int(S->d_flag[S->cur], 0, S->nsys, S->stream));
+  HIPCHK(launch_fill_int(S->d_any + S->cur, 0, 1, S->stream));
+  HIPCHK(launch_adj_flag(S, S->V, S->cur));
+  for (long i = 0; i < steps; i++) {
+    int rc = loop_step(S, MODE_BROWN | MODE_START | (noisy ? MODE_LANGEVIN : 0), noisy ? 1 : 0);
+    if (rc) return rc;
+    swap_pos(S);
   }
   return DPM_OK;
 }
@@ -331,11 +366,21 @@ int run_verlet(dpm_set_s* S, long steps, bool langevin) {
   int rc = require_list(S);
   if (rc) return rc;
   HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
-  if (steps < 4 || !graphs_enabled()) return verlet_sequence(S, steps, langevin);
-  rc = run_graph(S, std::make_tuple(langevin ? 1 : 2, steps, S->cur),
-                 [&]() { return verlet_sequence(S, steps, langevin); });
+  const int noise = langevin ? S->noiseType : -1;
+  auto seq = [&]() -> int {
+    switch (noise) {
+      case NOISE_BAOAB: return baoab_sequence(S, steps);
+      case NOISE_BROWNIAN: return brown_sequence(S, steps, true);
+      case NOISE_DRIVEN_BROWNIAN: return brown_sequence(S, steps, false);
+      default: return verlet_sequence(S, steps, langevin);
+    }
+  };
+  // every sequence swaps the position buffers once per step
+  if (steps < 4 || !graphs_enabled()) return seq();
+  const int kind = noise < 0 ? 2 : (noise == NOISE_LANGEVIN ? 1 : 10 + noise);
+  rc = run_graph(S, std::make_tuple(kind, steps, S->cur), seq);
   if (rc) return rc;
-  if ((steps % 2) == 1) swap_pos(S);  // the sequence swaps the buffers once per step
+  if ((steps % 2) == 1) swap_pos(S);
   return DPM_OK;
 }
 
@@ -362,7 +407,7 @@ int fire_sequence(dpm_set_s* S, long iterations) {
 int run_fire(dpm_set_s* S, long iterations) {
   if (iterations <= 0) return DPM_OK;
   if (iterations < 4 || !graphs_enabled()) return fire_sequence(S, iterations);
-  int rc = run_graph(S, std::make_tuple(3 + 2 * S->fireCur, iterations, S->cur),
+  int rc = run_graph(S, std::make_tuple(8 + S->fireCur, iterations, S->cur),
                      [&]() { return fire_sequence(S, iterations); });
   if (rc) return rc;
   if ((iterations % 2) == 1) {
@@ -534,7 +579,7 @@ int backward_segment(dpm_set_s* S, long a, long top, long N) {
   };
   const int w0 = S->wCur;
   if (!graphs_enabled()) return seq();
-  const int kind = 5 | (hasTop ? 16 : 0) | (hasZero ? 32 : 0) | (single ? 64 : 0);
+  const int kind = 256 | (hasTop ? 1 : 0) | (hasZero ? 2 : 0) | (single ? 4 : 0);
   rc = run_graph(S, std::make_tuple(kind, count, w0), [&]() {
     int r = seq();
     S->wCur = w0;
@@ -698,7 +743,7 @@ int dpm_destroy(dpm_set_t h) {
                   S->G.coff, S->G.brute, S->G.count[0], S->G.count[1], S->G.slots, S->G.cellSys, S->d_flag[0],
                   S->d_flag[1], S->d_any, S->d_updateCount, S->d_err, S->d_maxcnt, S->d_ctr, S->d_notDone,
                   S->d_fire[0], S->d_fire[1], S->d_part, S->d_red, S->d_scale, V.kr,
-                  S->d_ck, S->d_fin, S->d_seg, S->d_adj, V.vinfo};
+                  S->d_ck, S->d_fin, S->d_seg, S->d_adj, V.vinfo, V.l0r};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (S->ownStream) hipStreamDestroy(S->ownStream);
@@ -810,7 +855,7 @@ int dpm_set_types(dpm_set_t h, int potential, int interaction, int monomer, int 
   CHECK_HANDLE(h);
   if (potential < 0 || potential > 3 ||
       (interaction != INT_VV && interaction != INT_SMOOTH && interaction != INT_ALL) || monomer < 0 ||
-      monomer > 1 || particle < 0 || particle > 1)
+      monomer > 1 || particle < 0 || particle > 2)
     return fail(S, DPM_ERR_ARG, "bad type");
   S->P.potential = potential;
   S->P.interaction = interaction;
@@ -1014,7 +1059,7 @@ int dpm_calc_force_energy(dpm_set_t h) {
   // a standalone force call counts as one active-noise draw (addSelfPropulsion runs inside
   // calcForceEnergy in the reference); with the any-flag cleared the bin launch only bumps it
   HIPCHK(launch_fill_int(S->d_any + S->cur, 0, 1, S->stream));
-  HIPCHK(launch_bin(S, 0, S->P.particle == PART_ACTIVE ? 1 : 0));
+  HIPCHK(launch_bin(S, 0, S->P.particle != PART_PASSIVE ? 1 : 0));
   StepArgs A{};
   A.ctr = S->d_ctr;
   A.err = S->d_err;
@@ -1069,12 +1114,40 @@ int dpm_get_interaction_forces(dpm_set_t h, double* host, long n) {
   return DPM_OK;
 }
 
+// setNoiseType, DPM2D.cu:410-423 (simControlStruct::noiseEnum order)
+int dpm_set_noise_type(dpm_set_t h, int noiseType) {
+  CHECK_HANDLE(h);
+  if (noiseType < NOISE_LANGEVIN || noiseType > NOISE_DRIVEN_BROWNIAN) return fail(S, DPM_ERR_ARG, "bad noise type");
+  S->noiseType = noiseType;
+  bump(S);
+  return DPM_OK;
+}
+
+// setActiveTensionParams, DPM2D.cu:1898-1903: the rest lengths become the current l0
+int dpm_set_active_tension(dpm_set_t h, double l0Tau, double l0Diff) {
+  CHECK_HANDLE(h);
+  if (!(l0Tau != 0) || l0Diff < 0) return fail(S, DPM_ERR_ARG, "l0Tau must be non-zero and l0Diff >= 0");
+  if (!S->V.l0r) HIPCHK(dalloc(&S->V.l0r, S->nv));
+  HIPCHK(launch_copy_d(S->V.l0r, S->V.l0, S->nv, S->stream));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  S->P.l0Tau = l0Tau;
+  S->P.l0Diff = l0Diff;
+  bump(S);  // captured sequences hold the old view
+  return DPM_OK;
+}
+
 int dpm_init_langevin(dpm_set_t h, double T, double gamma, int readState, int zeroMomentum) {
   CHECK_HANDLE(h);
   S->simKind = 1;
   S->Tinject = T;
-  S->P.gamma = gamma;
-  S->P.noise = sqrt(2. * T * gamma / S->P.dt);
+  // initLangevin, DPM2D.cu:3713-3753: BAOAB keeps exp(-gamma dt) and its O-step amplitude
+  if (S->noiseType == NOISE_BAOAB) {
+    S->P.gamma = exp(-gamma * S->P.dt);
+    S->P.noise = sqrt(1 - exp(-2 * gamma * S->P.dt)) * sqrt(T);
+  } else {
+    S->P.gamma = gamma;
+    S->P.noise = sqrt(2. * T * gamma / S->P.dt);
+  }
   bump(S);
   HIPCHK(launch_copy_lastpos(S));
   if (!readState) {

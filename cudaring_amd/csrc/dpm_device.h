@@ -23,14 +23,15 @@ enum Monomer : int { MONO_HARMONIC = 0, MONO_FENE = 1 };
 enum ParticleKind : int { PART_PASSIVE = 0, PART_ACTIVE = 1, PART_ACTIVE_SHAPE = 2 };
 
 // Philox stream tags (counter word 3, high byte)
-enum RngTag : uint32_t { TAG_THERMAL = 1, TAG_INJECT = 2, TAG_ACTIVE = 3 };
+enum RngTag : uint32_t { TAG_THERMAL = 1, TAG_INJECT = 2, TAG_ACTIVE = 3, TAG_SHAPE = 4 };
 
 struct Params {
   int potential, interaction, monomer, particle;
   double ea, el, eb, ec, l1, l2, LJcutoff, LJecut, LJfshift, stiff, extSq;
   double dt;
   double driving, taup;
-  double gamma, noise;
+  double gamma, noise;         // of the noise type (initLangevin, DPM2D.cu:3713-3753)
+  double l0Tau, l0Diff;        // activeShape rest-length dynamics (setActiveTensionParams)
   double cutDistance, cutoff;  // neighbour list radius factor and rebuild threshold
   uint64_t seed;
 };
@@ -41,6 +42,7 @@ struct View {
   double *x, *y, *xn, *yn;
   double *vx, *vy, *fx, *fy;
   double *rad, *l0, *th0, *th, *len, *ev, *es;
+  double* l0r;  // activeShape rest lengths, or null
   double *lx, *ly;
   int *vring, *vsys;
   int4* vinfo;  // per vertex in one load: ring, ring's first vertex, ring size, system

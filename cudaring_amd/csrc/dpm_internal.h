@@ -14,7 +14,13 @@
 namespace dpm {
 
 // step-kernel epilogue bits
-enum : int { MODE_FORCE = 0, MODE_KICK = 1, MODE_LANGEVIN = 2, MODE_START = 4, MODE_FIRE = 8 };
+// (MODE_BAOAB: the next step's first half is B A O A; MODE_BROWN: overdamped update, with
+// thermal noise when MODE_LANGEVIN is also set)
+enum : int { MODE_FORCE = 0, MODE_KICK = 1, MODE_LANGEVIN = 2, MODE_START = 4, MODE_FIRE = 8, MODE_BAOAB = 16,
+             MODE_BROWN = 32 };
+
+// noise types, simControlStruct::noiseEnum order (include/DPM2D.h:27)
+enum : int { NOISE_LANGEVIN = 0, NOISE_BAOAB = 1, NOISE_BROWNIAN = 2, NOISE_DRIVEN_BROWNIAN = 3 };
 
 // error bits written by kernels into Set::d_err
 enum : int { ERR_LIST_OVERFLOW = 1, ERR_CELL_OVERFLOW = 2 };
@@ -131,6 +137,7 @@ struct dpm_set_s {
 
   // integrators
   int simKind = 0;  // 1 langevin, 2 nve, 3 nve rescale
+  int noiseType = 0;  // integrator of the Langevin loop (NOISE_*)
   double Tinject = 0;
   unsigned long long injectCalls = 0;
 
@@ -174,6 +181,7 @@ namespace dpm {
 hipError_t launch_shape_only(dpm_set_s* S);
 hipError_t launch_step(dpm_set_s* S, int mode, const StepArgs& A);
 hipError_t launch_start(dpm_set_s* S, double h, double dt, double* recx = nullptr, double* recy = nullptr);
+hipError_t launch_start_baoab(dpm_set_s* S, double h);
 hipError_t launch_bin(dpm_set_s* S, int bumpThermal, int bumpActive);
 hipError_t launch_build_rows(dpm_set_s* S, int countUpdates);
 hipError_t launch_fill_int(int* p, int value, long n, hipStream_t st);

@@ -288,8 +288,19 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
   double zt0 = 0, zt1 = 0, ang = 0;
   if (w == 0) {
     so = ring_shape(V, P, L, B, x, y, true);
-    if constexpr ((MODE & MODE_LANGEVIN) != 0)  // Langevin::updateThermalVel, Simulator.cu:54-77
+    // one thermal draw per integrate call and ring: Langevin::updateThermalVel (Simulator.cu:54-77)
+    // and Brownian::updateThermalVel (156-179) after this call's force, counter = call index;
+    // BAOAB::updateThermalVel (122-144) in the next call's first half, whose index is one more
+    if constexpr ((MODE & MODE_LANGEVIN) != 0)
       gauss_pair(P.seed, (uint32_t)L.s, TAG_THERMAL, A.ctr[0] - 1, (uint32_t)(L.r - V.sroff[L.s]), zt0, zt1);
+    if constexpr ((MODE & MODE_BAOAB) != 0 && (MODE & MODE_START) != 0)
+      gauss_pair(P.seed, (uint32_t)L.s, TAG_THERMAL, A.ctr[0], (uint32_t)(L.r - V.sroff[L.s]), zt0, zt1);
+    if (P.particle == PART_ACTIVE_SHAPE && L.act) {  // updateRestLength, DPM2D.cu:2037-2056
+      double z0, z1;
+      gauss_pair(P.seed, (uint32_t)L.s, TAG_SHAPE, A.ctr[1] - 1, (uint32_t)(L.r - V.sroff[L.s]), z0, z1);
+      const double l0v = V.l0[L.v];
+      V.l0[L.v] = l0v + ((l0v - V.l0r[L.v]) / P.l0Tau + sqrt(2. * P.l0Diff / P.dt) * z0) * P.dt;
+    }
     if (P.particle == PART_ACTIVE) {  // addSelfPropulsion, DPM2D.cu:1995-2035
       const double amplitude = (P.taup != 0) ? sqrt(2.0 * P.dt / P.taup) : 0.0;
       double z0, z1;
@@ -335,7 +346,7 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
     fy += P.driving * ((1 - 1) * cos(ang) + 1 * sin(ang));
   }
 
-  if constexpr ((MODE & MODE_LANGEVIN) != 0) {
+  if constexpr ((MODE & MODE_LANGEVIN) != 0 && (MODE & MODE_BROWN) == 0) {
     fx += P.noise * zt0 - P.gamma * vx;
     fy += P.noise * zt1 - P.gamma * vy;
   }
@@ -364,14 +375,40 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
   V.fy[L.v] = fy;
   V.ev[L.v] = e;
   if constexpr (MODE == MODE_FORCE) return;
+  if constexpr ((MODE & MODE_BROWN) != 0) {
+    // overdamped velocity (Brownian: (F + noise xi) / gamma; driven: F / gamma), then the drift
+    if constexpr ((MODE & MODE_LANGEVIN) != 0) {
+      vx = (fx + P.noise * zt0) / P.gamma;
+      vy = (fy + P.noise * zt1) / P.gamma;
+    } else {
+      vx = fx / P.gamma;
+      vy = fy / P.gamma;
+    }
+  }
   if constexpr ((MODE & MODE_KICK) != 0) {
     vx += A.h * fx;
     vy += A.h * fy;
   }
   if constexpr ((MODE & MODE_START) != 0) {
-    vx += A.h * fx;
-    vy += A.h * fy;
-    const double xn = x + A.dt * vx, yn = y + A.dt * vy;
+    double xn, yn;
+    if constexpr ((MODE & MODE_BROWN) != 0) {
+      xn = x + vx * A.dt;
+      yn = y + vy * A.dt;
+    } else if constexpr ((MODE & MODE_BAOAB) != 0) {
+      // B, A (half), O, A (half)
+      vx += A.h * fx;
+      vy += A.h * fy;
+      const double xa = x + vx * A.h, ya = y + vy * A.h;
+      vx = P.gamma * vx + P.noise * zt0;
+      vy = P.gamma * vy + P.noise * zt1;
+      xn = xa + vx * A.h;
+      yn = ya + vy * A.h;
+    } else {
+      vx += A.h * fx;
+      vy += A.h * fy;
+      xn = x + A.dt * vx;
+      yn = y + A.dt * vy;
+    }
     A.xout[L.v] = xn;
     A.yout[L.v] = yn;
     if (A.recx) {
@@ -411,6 +448,32 @@ __global__ void k_start(View V, Params P, double h, double dt, double* xo, doubl
     rx[v] = xn;
     ry[v] = yn;
   }
+  const double ddx = pbx(xn - V.lx[v], B), ddy = pby(yn - V.ly[v], B);
+  if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) {
+    atomicOr(&flagOut[s], 1);
+    atomicOr(anyOut, 1);
+  }
+}
+
+// first half of a BAOAB call with the stored force: B (v += h F), A (x += h v), O (v = gamma v +
+// noise xi_ring, the call's draw at counter ctr[0]), A (x += h v); displacement test
+// (BAOAB::integrate, Simulator.cu:112-120, up to its force evaluation)
+__global__ void k_start_baoab(View V, Params P, double h, const unsigned long long* ctr, double* xo, double* yo,
+                              int* flagOut, int* anyOut) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V.nv) return;
+  const int s = V.vsys[v], r = V.vring[v];
+  const Box B = box_of(V, s);
+  double z0, z1;
+  gauss_pair(P.seed, (uint32_t)s, TAG_THERMAL, ctr[0], (uint32_t)(r - V.sroff[s]), z0, z1);
+  double vx = V.vx[v] + h * V.fx[v];
+  double vy = V.vy[v] + h * V.fy[v];
+  const double xa = V.x[v] + vx * h, ya = V.y[v] + vy * h;
+  vx = P.gamma * vx + P.noise * z0;
+  vy = P.gamma * vy + P.noise * z1;
+  const double xn = xa + vx * h, yn = ya + vy * h;
+  V.vx[v] = vx; V.vy[v] = vy;
+  xo[v] = xn; yo[v] = yn;
   const double ddx = pbx(xn - V.lx[v], B), ddy = pby(yn - V.ly[v], B);
   if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) {
     atomicOr(&flagOut[s], 1);
@@ -883,6 +946,18 @@ static hipError_t step_mode(dpm_set_s* S, int mode, const StepArgs& A) {
       k_step<INTER, POT, MODE_KICK | MODE_LANGEVIN | MODE_START><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
       break;
     case MODE_FIRE: k_step<INTER, POT, MODE_FIRE><<<g, b, sh, st>>>(S->V, S->P, A, S->G); break;
+    case MODE_KICK | MODE_BAOAB:
+      k_step<INTER, POT, MODE_KICK | MODE_BAOAB><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
+      break;
+    case MODE_KICK | MODE_BAOAB | MODE_START:
+      k_step<INTER, POT, MODE_KICK | MODE_BAOAB | MODE_START><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
+      break;
+    case MODE_BROWN | MODE_START:
+      k_step<INTER, POT, MODE_BROWN | MODE_START><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
+      break;
+    case MODE_BROWN | MODE_LANGEVIN | MODE_START:
+      k_step<INTER, POT, MODE_BROWN | MODE_LANGEVIN | MODE_START><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -902,6 +977,7 @@ static hipError_t step_pot(dpm_set_s* S, int mode, const StepArgs& A) {
 hipError_t launch_step(dpm_set_s* S, int mode, const StepArgs& A) {
   // refuse argument sets the chosen epilogue would dereference as null
   if (!A.ctr) return hipErrorInvalidValue;
+  if (S->P.particle == PART_ACTIVE_SHAPE && !S->V.l0r) return hipErrorInvalidValue;
   if ((mode & MODE_FIRE) && (!A.fire || !A.part)) return hipErrorInvalidValue;
   if ((mode & MODE_START) && (!A.xout || !A.yout || !A.flagOut || !A.anyOut)) return hipErrorInvalidValue;
   if (A.manage && (!A.flagCur || !A.cellCur || !A.cellOther || !A.err || !A.maxcnt || !A.updateCount))
@@ -935,6 +1011,13 @@ hipError_t launch_start(dpm_set_s* S, double h, double dt, double* recx, double*
   const int nb = 1 - S->cur;
   k_start<<<grid_v(S->nv), 256, 0, S->stream>>>(S->V, S->P, h, dt, S->pos[nb][0], S->pos[nb][1],
                                                 S->d_flag[nb], S->d_any + nb, recx, recy);
+  return hipGetLastError();
+}
+
+hipError_t launch_start_baoab(dpm_set_s* S, double h) {
+  const int b = 1 - S->cur;
+  k_start_baoab<<<grid_v(S->nv), 256, 0, S->stream>>>(S->V, S->P, h, S->d_ctr, S->pos[b][0], S->pos[b][1],
+                                                      S->d_flag[b], S->d_any + b);
   return hipGetLastError();
 }
 

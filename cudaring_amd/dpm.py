@@ -19,7 +19,8 @@ FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 POT = {"harmonic": 0, "lj": 1, "adhesive": 2, "wca": 3}
 INTER = {"vertexVertex": 0, "vertexSmooth": 1, "all": 3}
 MONO = {"harmonic": 0, "FENE": 1}
-PART = {"passive": 0, "active": 1}
+PART = {"passive": 0, "active": 1, "activeShape": 2}
+NOISE = {"langevin": 0, "baoab": 1, "brownian": 2, "drivenBrownian": 3}
 ERRORS = {-1: "bad argument", -2: "HIP error", -3: "out of memory", -4: "unsupported",
           -5: "overflow", -6: "bad state"}
 
@@ -43,6 +44,8 @@ SIGNATURES = {
     "dpm_set_fene_constants": (_i, [_vp, _d, _d]),
     "dpm_set_time_step": (_i, [_vp, _d]),
     "dpm_set_self_propulsion": (_i, [_vp, _d, _d]),
+    "dpm_set_active_tension": (_i, [_vp, _d, _d]),
+    "dpm_set_noise_type": (_i, [_vp, _i]),
     "dpm_set_seed": (_i, [_vp, C.c_uint64]),
     "dpm_set_displacement_cutoff": (_i, [_vp, _d, _d, C.POINTER(_d)]),
     "dpm_calc_neighbor_list": (_i, [_vp, _d]),
@@ -214,6 +217,14 @@ class DPMSet:
 
     def set_self_propulsion(self, driving, taup):
         self._chk(self.L.dpm_set_self_propulsion(self.h, driving, taup), "set_self_propulsion")
+
+    def set_active_tension(self, l0_tau, l0_diff):
+        """activeShape rest-length dynamics; the rest lengths become the current l0."""
+        self._chk(self.L.dpm_set_active_tension(self.h, l0_tau, l0_diff), "set_active_tension")
+
+    def set_noise_type(self, noise):
+        """Integrator of the Langevin loop: langevin, baoab, brownian or drivenBrownian."""
+        self._chk(self.L.dpm_set_noise_type(self.h, NOISE[noise]), "set_noise_type")
 
     def set_seed(self, seed):
         self._chk(self.L.dpm_set_seed(self.h, seed), "set_seed")

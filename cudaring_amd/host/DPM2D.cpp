@@ -132,10 +132,9 @@ void DPM2D::push_params() {
     case simControlStruct::interactionEnum::all: inter = DPM_INT_ALL; break;
     default: throw std::invalid_argument("DPM2D: cellSmooth interaction is not provided");
   }
-  int part = simControl.particleType == simControlStruct::particleEnum::active ? DPM_PART_ACTIVE : DPM_PART_PASSIVE;
-  if (simControl.particleType == simControlStruct::particleEnum::activeShape)
-    throw std::invalid_argument("DPM2D: activeShape particles are not provided");
+  const int part = (int)simControl.particleType;  // passive, active, activeShape = DPM_PART_*
   check(dpm_set_types(h_, pot, inter, (int)simControl.monomerType, part), "setTypes");
+  check(dpm_set_noise_type(h_, (int)simControl.noiseType), "setNoiseType");
   check(dpm_set_energy_costs(h_, ea, el, eb, ec), "setEnergyCosts");
   check(dpm_set_attraction_constants(h_, l1, l2), "setAttractionConstants");
   if (LJcutoff > 0) check(dpm_set_lj_cutoff(h_, LJcutoff), "setLJcutoff");
@@ -175,9 +174,18 @@ void DPM2D::setParticleType(simControlStruct::particleEnum t) {
   push_params();
 }
 void DPM2D::setNoiseType(simControlStruct::noiseEnum t) {
-  if (t != simControlStruct::noiseEnum::langevin)
-    throw std::invalid_argument("DPM2D: only the langevin noise type is provided");
-  simControl.noiseType = t;
+  simControl.noiseType = t;  // langevin, baoab, brownian, drivenBrownian = DPM_NOISE_*
+  push_params();
+}
+// setActiveTensionParams, DPM2D.cu:1898-1903: the rest lengths become the current l0
+void DPM2D::setActiveTensionParams(double l0Tau_, double l0Diff_) {
+  l0Tau = l0Tau_;
+  l0Diff = l0Diff_;
+  check(dpm_set_active_tension(handle(), l0Tau, l0Diff), "setActiveTensionParams");
+}
+void DPM2D::getActiveTensionParams(double& l0Tau_, double& l0Diff_) {
+  l0Tau_ = l0Tau;
+  l0Diff_ = l0Diff;
 }
 void DPM2D::setPotentialType(simControlStruct::potentialEnum t) {
   simControl.potentialType = t;

@@ -42,6 +42,7 @@ class DPM2D {
 
   long nDim, numParticles, numVertexPerParticle, numVertices;
   double dt, rho0, calA0, ea, el, eb, ec, driving, taup, l1, l2, LJcutoff, LJecut, LJfshift, stiff, extSq;
+  double l0Tau = 1, l0Diff = 0;  // activeShape rest-length dynamics
   double cutoff, cutDistance;
   uint64_t seed;
 
@@ -108,6 +109,8 @@ class DPM2D {
   void setEnergyCosts(double ea_, double el_, double eb_, double ec_);
   void setSelfPropulsionParams(double driving_, double taup_);
   void getSelfPropulsionParams(double& driving_, double& taup_) { driving_ = driving; taup_ = taup; }
+  void setActiveTensionParams(double l0Tau_, double l0Diff_);
+  void getActiveTensionParams(double& l0Tau_, double& l0Diff_);
   void setAttractionConstants(double l1_, double l2_);
   void setLJcutoff(double LJcutoff_);
   void setFENEconstants(double stiff_, double ext_);

@@ -65,7 +65,8 @@ enum {
 enum { DPM_POT_HARMONIC = 0, DPM_POT_LJ = 1, DPM_POT_ADHESIVE = 2, DPM_POT_WCA = 3 };
 enum { DPM_INT_VERTEX_VERTEX = 0, DPM_INT_VERTEX_SMOOTH = 1, DPM_INT_ALL = 3 };
 enum { DPM_MONO_HARMONIC = 0, DPM_MONO_FENE = 1 };
-enum { DPM_PART_PASSIVE = 0, DPM_PART_ACTIVE = 1 };
+enum { DPM_PART_PASSIVE = 0, DPM_PART_ACTIVE = 1, DPM_PART_ACTIVE_SHAPE = 2 };
+enum { DPM_NOISE_LANGEVIN = 0, DPM_NOISE_BAOAB = 1, DPM_NOISE_BROWNIAN = 2, DPM_NOISE_DRIVEN_BROWNIAN = 3 };
 
 /* ---- lifecycle ------------------------------------------------------------ */
 /* Replaces DPM2D(long nParticles, long dim, long nVertexPerParticle) (DPM2D.h:47) followed by
@@ -107,6 +108,10 @@ int dpm_set_fene_constants(dpm_set_t h, double stiff, double ext);
 int dpm_set_time_step(dpm_set_t h, double dt);
 /* setSelfPropulsionParams(driving, taup), DPM2D.h:527 */
 int dpm_set_self_propulsion(dpm_set_t h, double driving, double taup);
+/* setActiveTensionParams(l0Tau, l0Diff), DPM2D.h:531 / DPM2D.cu:1898-1903: the rest lengths of the
+ * activeShape particle type become the current l0; each force call then moves
+ * l0 += ((l0 - l0Rest) / l0Tau + sqrt(2 l0Diff / dt) xi_ring) dt (updateRestLength, DPM2D.cu:2037-2056) */
+int dpm_set_active_tension(dpm_set_t h, double l0Tau, double l0Diff);
 /* seed of the Philox streams that replace thrust::default_random_engine + lrand48
  * (defs.h:30-79, DPM2D.cu:43); resets the per-stream call counters */
 int dpm_set_seed(dpm_set_t h, uint64_t seed);
@@ -139,8 +144,12 @@ int dpm_calc_force_energy(dpm_set_t h);
 int dpm_get_interaction_forces(dpm_set_t h, double* host, long n);
 
 /* ---- integrators ---------------------------------------------------------- */
-/* initLangevin(T, gamma, readState) / langevinLoop(), DPM2D.h:680-682.  zeroMomentum
- * removes the mean velocity after injection (an addition; the reference does not). */
+/* setNoiseType, DPM2D.h:237 / DPM2D.cu:410-423: the integrator of langevinLoop, in
+ * simControlStruct::noiseEnum order (0 langevin, 1 baoab, 2 brownian, 3 drivenBrownian;
+ * Simulator.cu:23-30, 112-144, 147-179, 182-203) */
+int dpm_set_noise_type(dpm_set_t h, int noiseType);
+/* initLangevin(T, gamma, readState) / langevinLoop(), DPM2D.h:680-682, DPM2D.cu:3713-3753.
+ * zeroMomentum removes the mean velocity after injection (an addition; the reference does not). */
 int dpm_init_langevin(dpm_set_t h, double T, double gamma, int readState, int zeroMomentum);
 int dpm_langevin_loop(dpm_set_t h, long steps);
 /* initNVE(T, readState) / NVELoop(), DPM2D.h:684-686 */

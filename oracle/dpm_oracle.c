@@ -17,7 +17,7 @@
 static const double PI = 3.141592653589793238462643383279502884197; /* defs.h:14 */
 static const double WCAcut = 1.122462048;                            /* defs.h:17 */
 
-enum { TAG_THERMAL = 1, TAG_INJECT = 2, TAG_ACTIVE = 3 };
+enum { TAG_THERMAL = 1, TAG_INJECT = 2, TAG_ACTIVE = 3, TAG_SHAPE = 4 };
 
 struct orc_state {
   long numParticles, numVertices;
@@ -36,6 +36,9 @@ struct orc_state {
   int shift;
   int simKind;
   double Tinject, gamma, noise;
+  int noiseType;            /* simControlStruct::noiseEnum order */
+  double l0Tau, l0Diff;     /* activeShape (setActiveTensionParams) */
+  double* l0Rest;
   double a_start, a, f_dec, f_inc, f_a, fire_dt_max, fire_dt, lastP;
   long minStep, numStep;
   uint64_t seed;
@@ -117,6 +120,7 @@ orc_state* orc_create(long numParticles, const long* nvp) {
 }
 
 void orc_destroy(orc_state* s) {
+  free(s->l0Rest);
   if (!s) return;
   free(s->numVertexInParticleList); free(s->firstVertexInParticleId); free(s->particleIdList);
   free(s->pos); free(s->vel); free(s->force); free(s->lastPos); free(s->energy);
@@ -661,6 +665,27 @@ void orc_add_self_propulsion(orc_state* s) {
   }
 }
 
+/* setActiveTensionParams, DPM2D.cu:1898-1903: the rest lengths become the current l0 */
+void orc_set_active_tension(orc_state* s, double l0Tau, double l0Diff) {
+  s->l0Tau = l0Tau;
+  s->l0Diff = l0Diff;
+  if (!s->l0Rest) s->l0Rest = (double*)malloc(sizeof(double) * s->numVertices);
+  memcpy(s->l0Rest, s->l0, sizeof(double) * s->numVertices);
+}
+
+/* updateRestLength, DPM2D.cu:2037-2056 (one Gaussian per ring per force call):
+ * l0 += ((l0 - l0Rest) / l0Tau + sqrt(2 l0Diff / dt) xi_ring) dt */
+void orc_update_rest_length(orc_state* s) {
+  uint64_t call = s->activeCalls++;
+  double noise = sqrt(2. * s->l0Diff / s->dt);
+  for (long v = 0; v < s->numVertices; v++) {
+    long p = s->particleIdList[v];
+    double z0, z1;
+    orc_gauss_pair(s->seed, s->systemId, TAG_SHAPE, call, (uint32_t)p, &z0, &z1);
+    s->l0[v] += ((s->l0[v] - s->l0Rest[v]) / s->l0Tau + noise * z0) * s->dt;
+  }
+}
+
 /* calcForceEnergyGPU, DPM2D.cu:2058-2104 with calcShapeForceEnergy 1976-1993 */
 void orc_calc_force_energy(orc_state* s) {
   orc_calc_particle_shape(s);
@@ -674,6 +699,7 @@ void orc_calc_force_energy(orc_state* s) {
     default: orc_calc_smooth_interaction(s); break;
   }
   if (s->particleType == ORC_ACTIVE) orc_add_self_propulsion(s);
+  if (s->particleType == ORC_ACTIVE_SHAPE) orc_update_rest_length(s);
 }
 
 /* ----------------------------------------------------------- neighbours --- */
@@ -794,12 +820,19 @@ void orc_inject_kinetic_energy(orc_state* s, int zeroMomentum) {
   }
 }
 
-/* initLangevin, DPM2D.cu:3713-3753 (langevin noise type) */
+void orc_set_noise_type(orc_state* s, int noiseType) { s->noiseType = noiseType; }
+
+/* initLangevin, DPM2D.cu:3713-3753: the (gamma, noise) pair of each noise type */
 void orc_init_langevin(orc_state* s, double T, double gamma, int readState, int zeroMomentum) {
   s->simKind = ORC_SIM_LANGEVIN;
   s->Tinject = T;
-  s->gamma = gamma;
-  s->noise = sqrt(2. * T * gamma / s->dt);
+  if (s->noiseType == ORC_NOISE_BAOAB) {
+    s->gamma = exp(-gamma * s->dt);                         /* the O step's velocity factor */
+    s->noise = sqrt(1 - exp(-2 * gamma * s->dt)) * sqrt(T);  /* and its noise amplitude */
+  } else {
+    s->gamma = gamma;
+    s->noise = sqrt(2. * T * gamma / s->dt);
+  }
   orc_reset_last_positions(s);
   if (!readState) orc_inject_kinetic_energy(s, zeroMomentum);
 }
@@ -846,15 +879,60 @@ static void thermal_forces(orc_state* s) {
   }
 }
 
-/* Langevin::integrate, Simulator.cu:23-30 */
+/* BAOAB::updateThermalVel, Simulator.cu:122-144: v = gamma v + noise xi_ring */
+static void baoab_thermal_velocity(orc_state* s) {
+  uint64_t call = s->thermalCalls++;
+  for (long v = 0; v < s->numVertices; v++) {
+    long p = s->particleIdList[v];
+    double z[2];
+    orc_gauss_pair(s->seed, s->systemId, TAG_THERMAL, call, (uint32_t)p, &z[0], &z[1]);
+    for (int dim = 0; dim < 2; dim++) s->vel[v * 2 + dim] = s->gamma * s->vel[v * 2 + dim] + s->noise * z[dim];
+  }
+}
+
+/* Brownian::updateThermalVel, Simulator.cu:156-179: v = (F + noise xi_ring) / gamma;
+ * DrivenBrownian::updateThermalVel, Simulator.cu:189-203: v = F / gamma (no draw) */
+static void overdamped_velocity(orc_state* s, int noisy) {
+  uint64_t call = noisy ? s->thermalCalls++ : 0;
+  for (long v = 0; v < s->numVertices; v++) {
+    long p = s->particleIdList[v];
+    double z[2] = {0, 0};
+    if (noisy) orc_gauss_pair(s->seed, s->systemId, TAG_THERMAL, call, (uint32_t)p, &z[0], &z[1]);
+    for (int dim = 0; dim < 2; dim++)
+      s->vel[v * 2 + dim] = noisy ? (s->force[v * 2 + dim] + s->noise * z[dim]) / s->gamma
+                                  : s->force[v * 2 + dim] / s->gamma;
+  }
+}
+
+/* Langevin::integrate, Simulator.cu:23-30; BAOAB::integrate 112-120; Brownian::integrate
+ * 147-152; DrivenBrownian::integrate 182-187 -- chosen by the noise type */
 void orc_langevin_loop(orc_state* s, long steps) {
   for (long n = 0; n < steps; n++) {
-    update_velocity(s, 0.5 * s->dt);
-    update_position(s, s->dt);
-    orc_check_neighbors(s);
-    orc_calc_force_energy(s);
-    thermal_forces(s);
-    update_velocity(s, 0.5 * s->dt);
+    switch (s->noiseType) {
+      case ORC_NOISE_BAOAB:
+        update_velocity(s, 0.5 * s->dt);
+        update_position(s, 0.5 * s->dt);
+        baoab_thermal_velocity(s);
+        update_position(s, 0.5 * s->dt);
+        orc_check_neighbors(s);
+        orc_calc_force_energy(s);
+        update_velocity(s, 0.5 * s->dt);
+        break;
+      case ORC_NOISE_BROWNIAN:
+      case ORC_NOISE_DRIVEN_BROWNIAN:
+        orc_check_neighbors(s);
+        orc_calc_force_energy(s);
+        overdamped_velocity(s, s->noiseType == ORC_NOISE_BROWNIAN);
+        update_position(s, s->dt);
+        break;
+      default:
+        update_velocity(s, 0.5 * s->dt);
+        update_position(s, s->dt);
+        orc_check_neighbors(s);
+        orc_calc_force_energy(s);
+        thermal_forces(s);
+        update_velocity(s, 0.5 * s->dt);
+    }
   }
 }
 

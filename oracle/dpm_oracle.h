@@ -46,6 +46,7 @@ enum { ORC_VERTEX_VERTEX = 0, ORC_VERTEX_SMOOTH = 1, ORC_CELL_SMOOTH = 2, ORC_AL
 enum { ORC_MONO_HARMONIC = 0, ORC_MONO_FENE = 1 };
 enum { ORC_PASSIVE = 0, ORC_ACTIVE = 1, ORC_ACTIVE_SHAPE = 2 };
 enum { ORC_SIM_NONE = 0, ORC_SIM_LANGEVIN = 1, ORC_SIM_NVE = 2, ORC_SIM_NVE_RESCALE = 3 };
+enum { ORC_NOISE_LANGEVIN = 0, ORC_NOISE_BAOAB = 1, ORC_NOISE_BROWNIAN = 2, ORC_NOISE_DRIVEN_BROWNIAN = 3 };
 
 /* field ids shared by orc_set_field / orc_get_field */
 enum {
@@ -104,7 +105,10 @@ void orc_reset_last_positions(orc_state* s);
 double orc_get_cut_distance(const orc_state* s);
 
 /* integrators (Simulator.cu:23-89, 206-238; DPM2D.cu:3713-3817) */
+void orc_set_noise_type(orc_state* s, int noiseType);
 void orc_init_langevin(orc_state* s, double T, double gamma, int readState, int zeroMomentum);
+void orc_set_active_tension(orc_state* s, double l0Tau, double l0Diff);
+void orc_update_rest_length(orc_state* s);
 void orc_init_nve(orc_state* s, double T, int readState, int zeroMomentum);
 void orc_init_nve_rescale(orc_state* s, double T);
 void orc_inject_kinetic_energy(orc_state* s, int zeroMomentum);

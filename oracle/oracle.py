@@ -16,6 +16,7 @@ POT = {"harmonic": 0, "lj": 1, "adhesive": 2, "wca": 3}
 INTER = {"vertexVertex": 0, "vertexSmooth": 1, "cellSmooth": 2, "all": 3}
 MONO = {"harmonic": 0, "FENE": 1}
 PART = {"passive": 0, "active": 1, "activeShape": 2}
+NOISE = {"langevin": 0, "baoab": 1, "brownian": 2, "drivenBrownian": 3}
 FIELDS = ["pos", "vel", "force", "energy", "rad", "l0", "theta0", "theta", "length",
           "a0", "area", "perimeter", "ppos", "penergy", "pangle", "lastpos", "box"]
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
@@ -75,6 +76,9 @@ def lib():
             "orc_reset_last_positions": (None, [vp]),
             "orc_get_cut_distance": (d, [vp]),
             "orc_init_langevin": (None, [vp, d, d, i, i]),
+            "orc_set_noise_type": (None, [vp, i]),
+            "orc_set_active_tension": (None, [vp, d, d]),
+            "orc_update_rest_length": (None, [vp]),
             "orc_init_nve": (None, [vp, d, i, i]),
             "orc_init_nve_rescale": (None, [vp, d]),
             "orc_inject_kinetic_energy": (None, [vp, i]),
@@ -221,6 +225,12 @@ class Oracle:
 
     def mean_vertex_radius(self):
         return self.L.orc_get_mean_vertex_radius(self.h)
+
+    def set_noise_type(self, noise):
+        self.L.orc_set_noise_type(self.h, NOISE[noise])
+
+    def set_active_tension(self, l0_tau, l0_diff):
+        self.L.orc_set_active_tension(self.h, l0_tau, l0_diff)
 
     def init_langevin(self, T, gamma, read_state=False, zero_momentum=False):
         self.L.orc_init_langevin(self.h, T, gamma, int(read_state), int(zero_momentum))
