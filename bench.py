@@ -362,6 +362,29 @@ def cfg5_bench(args, torch, batched=False):
     return out
 
 
+def integrators_bench(s, dt, steps):
+    """The benchmark workload under each integrator of the Langevin loop (Simulator.cu:23-30,
+    112-203): vertex-steps/s over `steps` steps after 1,000 untimed ones, from the same state.
+    The overdamped schemes run at dt/100: x += dt F / gamma is stable only for dt k / gamma < 2,
+    and the WCA contact stiffness k ~ 456 ec / sigma_v^2 puts dt = 1e-3 sigma far above that."""
+    out = {}
+    for noise in ("langevin", "baoab", "brownian", "drivenBrownian"):
+        s.set_noise_type(noise)
+        s.set_dt(dt if noise in ("langevin", "baoab") else dt / 100)
+        s.init_langevin(T, GAMMA, read_state=True)
+        run_steps(s, 1000)
+        s.synchronize()
+        t0 = time.perf_counter()
+        run_steps(s, steps)
+        s.synchronize()
+        el = time.perf_counter() - t0
+        out[noise] = {"vertex_steps_per_s": round(s.nv * steps / el, 1), "step_us": round(1e6 * el / steps, 3),
+                      "T_end": float(s.temperature()[0]), "finite": bool(np.all(np.isfinite(s.get("pos"))))}
+    s.set_noise_type("langevin")
+    s.set_dt(dt)
+    return out
+
+
 COPY_ELEMS = 1 << 27  # float64 elements of the calibration copy: 1 GiB read + 1 GiB written
 
 
@@ -406,6 +429,8 @@ def main():
     ap.add_argument("--cfg5-steps", type=int, default=2000)
     ap.add_argument("--cfg5-dphi", type=float, default=0.02)
     ap.add_argument("--cfg5-iters", type=int, default=2000)
+    ap.add_argument("--integrators", action="store_true",
+                    help="the benchmark workload under each Langevin-loop integrator (5,000 steps each)")
     ap.add_argument("--traffic", action="store_true",
                     help="counter-collection run: a copy of known size, then step-kernel launches only")
     args = ap.parse_args()
@@ -428,6 +453,9 @@ def main():
     nv, np_ = s.nv, s.np
     if args.traffic:
         traffic_run(s, torch, args)
+        return
+    if args.integrators:
+        print(json.dumps({"integrators": integrators_bench(s, dt, 5000)}), flush=True)
         return
 
     run_steps(s, args.warmup)

@@ -88,18 +88,27 @@ bool field_ptrs(dpm_set_s* S, int f, double** c0, double** c1, long* n, int* nco
     case DPM_FIELD_PANGLE: *c0 = V.ang; *n = S->np; return true;
     case DPM_FIELD_LASTPOS: *c0 = V.lx; *c1 = V.ly; *n = S->nv; *ncomp = 2; return true;
     case DPM_FIELD_BOX: *c0 = V.bx; *c1 = V.by; *n = S->nsys; *ncomp = 2; return true;
+    case DPM_FIELD_WALLFORCE: *c0 = V.wfx; *c1 = V.wfy; *n = S->nv; *ncomp = 2; return true;
   }
   return false;
 }
 
+// box reciprocals, and the image multipliers of the geometry (periodic x for normal and
+// fixedSides, periodic y for normal only)
 int upload_box(dpm_set_s* S) {
-  std::vector<double> ix(S->nsys), iy(S->nsys);
+  std::vector<double> ix(S->nsys), iy(S->nsys), mx(S->nsys), my(S->nsys);
+  const bool px = S->P.geometry == GEOM_NORMAL || S->P.geometry == GEOM_FIXED_SIDES;
+  const bool py = S->P.geometry == GEOM_NORMAL;
   for (long s = 0; s < S->nsys; s++) {
     ix[s] = 1.0 / S->h_box[2 * s];
     iy[s] = 1.0 / S->h_box[2 * s + 1];
+    mx[s] = px ? ix[s] : 0.0;
+    my[s] = py ? iy[s] : 0.0;
   }
   HIPCHK(up(S->V.ibx, ix, S->stream));
   HIPCHK(up(S->V.iby, iy, S->stream));
+  HIPCHK(up(S->V.mbx, mx, S->stream));
+  HIPCHK(up(S->V.mby, my, S->stream));
   HIPCHK(hipStreamSynchronize(S->stream));
   return DPM_OK;
 }
@@ -227,6 +236,12 @@ int check_err_after_loop(dpm_set_s* S) {
   if (err) return fail(S, DPM_ERR_OVERFLOW, err & ERR_LIST_OVERFLOW ? "neighbour list overflow during loop"
                                                                     : "cell overflow during loop");
   return DPM_OK;
+}
+
+// area of system s's domain: the box, or pi R^2 for the round box (getPhi, DPM2D.cu:1208-1246)
+double box_area(const dpm_set_s* S, long s) {
+  if (S->P.geometry == GEOM_ROUND_BOX) return kPI * S->P.boxRadius * S->P.boxRadius;
+  return S->h_box[2 * s] * S->h_box[2 * s + 1];
 }
 
 int reduce_host(dpm_set_s* S) {
@@ -449,6 +464,7 @@ int adjoint_supported(dpm_set_s* S) {
     return fail(S, DPM_ERR_UNSUPPORTED, "adjoint: vertexVertex interaction only");
   if (S->P.monomer != MONO_HARMONIC) return fail(S, DPM_ERR_UNSUPPORTED, "adjoint: harmonic monomers only");
   if (S->P.particle != PART_PASSIVE) return fail(S, DPM_ERR_UNSUPPORTED, "adjoint: passive particles only");
+  if (S->P.geometry != GEOM_NORMAL) return fail(S, DPM_ERR_UNSUPPORTED, "adjoint: periodic geometry only");
   return require_list(S);
 }
 
@@ -664,6 +680,7 @@ int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPer
   A(V.rfirst, np); A(V.rnv, np); A(V.rsys, np);
   A(V.a0, np); A(V.area, np); A(V.perim, np); A(V.cx, np); A(V.cy, np); A(V.pe, np); A(V.ang, np);
   A(V.bx, numSystems); A(V.by, numSystems); A(V.ibx, numSystems); A(V.iby, numSystems);
+  A(V.mbx, numSystems); A(V.mby, numSystems); A(V.wfx, nv); A(V.wfy, nv);
   A(V.svoff, numSystems + 1); A(V.sroff, numSystems + 1);
   A(V.ncnt, nv);
   A(V.tile_v0, V.ntiles); A(V.tile_v1, V.ntiles); A(V.tile_sys, V.ntiles);
@@ -691,9 +708,10 @@ int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPer
             up(V.stile, S->h_stile, st) == hipSuccess;
   std::vector<double> ones(numSystems, 1.0);
   ok = ok && up(V.bx, ones, st) == hipSuccess && up(V.by, ones, st) == hipSuccess &&
-       up(V.ibx, ones, st) == hipSuccess && up(V.iby, ones, st) == hipSuccess;
+       up(V.ibx, ones, st) == hipSuccess && up(V.iby, ones, st) == hipSuccess &&
+       up(V.mbx, ones, st) == hipSuccess && up(V.mby, ones, st) == hipSuccess;
   double* zd[] = {S->pos[0][0], S->pos[0][1], S->pos[1][0], S->pos[1][1], V.vx, V.vy, V.fx, V.fy, V.rad, V.l0,
-                  V.th0, V.th, V.len, V.ev, V.es, V.lx, V.ly};
+                  V.th0, V.th, V.len, V.ev, V.es, V.lx, V.ly, V.wfx, V.wfy};
   for (double* p : zd) ok = ok && hipMemsetAsync(p, 0, sizeof(double) * nv, st) == hipSuccess;
   double* zp[] = {V.a0, V.area, V.perim, V.cx, V.cy, V.pe, V.ang};
   for (double* p : zp) ok = ok && hipMemsetAsync(p, 0, sizeof(double) * np, st) == hipSuccess;
@@ -719,6 +737,7 @@ int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPer
   S->P.ea = 1e5; S->P.el = 20; S->P.eb = 10; S->P.ec = 1;
   S->P.gamma = 1; S->P.noise = 0;
   S->P.cutDistance = 1;
+  S->P.ew = 1e3;  // DPM2D.cu:70
   S->P.cutoff = 0;
   S->P.seed = 0;
   S->fp.tol = -1;
@@ -743,7 +762,7 @@ int dpm_destroy(dpm_set_t h) {
                   S->G.coff, S->G.brute, S->G.count[0], S->G.count[1], S->G.slots, S->G.cellSys, S->d_flag[0],
                   S->d_flag[1], S->d_any, S->d_updateCount, S->d_err, S->d_maxcnt, S->d_ctr, S->d_notDone,
                   S->d_fire[0], S->d_fire[1], S->d_part, S->d_red, S->d_scale, V.kr,
-                  S->d_ck, S->d_fin, S->d_seg, S->d_adj, V.vinfo, V.l0r};
+                  S->d_ck, S->d_fin, S->d_seg, S->d_adj, V.vinfo, V.l0r, V.mbx, V.mby, V.wfx, V.wfy};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (S->ownStream) hipStreamDestroy(S->ownStream);
@@ -862,6 +881,43 @@ int dpm_set_types(dpm_set_t h, int potential, int interaction, int monomer, int 
   S->P.monomer = monomer;
   S->P.particle = particle;
   bump(S);
+  return DPM_OK;
+}
+
+// setGeometryType, DPM2D.cu:338-358
+int dpm_set_geometry(dpm_set_t h, int geometry) {
+  CHECK_HANDLE(h);
+  if (geometry < GEOM_NORMAL || geometry > GEOM_ROUND_BOX) return fail(S, DPM_ERR_ARG, "bad geometry");
+  S->P.geometry = geometry;
+  HIPCHK(launch_fill_d(S->V.wfx, 0.0, S->nv, S->stream));
+  HIPCHK(launch_fill_d(S->V.wfy, 0.0, S->nv, S->stream));
+  int rc = upload_box(S);
+  if (rc) return rc;
+  bump(S);
+  return DPM_OK;
+}
+
+// setBoxEnergyCost, DPM2D.cu:1911-1914
+int dpm_set_box_energy_cost(dpm_set_t h, double ew) {
+  CHECK_HANDLE(h);
+  S->P.ew = ew;
+  bump(S);
+  return DPM_OK;
+}
+
+// setBoxRadius / getBoxRadius, DPM2D.cu:627-664
+int dpm_set_box_radius(dpm_set_t h, double boxRadius) {
+  CHECK_HANDLE(h);
+  if (!(boxRadius > 0)) return fail(S, DPM_ERR_ARG, "boxRadius must be positive");
+  S->P.boxRadius = boxRadius;
+  bump(S);
+  return DPM_OK;
+}
+
+int dpm_get_box_radius(dpm_set_t h, double* boxRadius) {
+  CHECK_HANDLE(h);
+  if (!boxRadius) return fail(S, DPM_ERR_ARG, "null output");
+  *boxRadius = S->P.boxRadius;
   return DPM_OK;
 }
 
@@ -1270,7 +1326,7 @@ int dpm_fire_minimize(dpm_set_t h, double tol, long maxIter, long* iterations) {
   }
   HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
   long done = 0;
-  if (S->fireKernel == 1 && S->P.particle == PART_PASSIVE) {
+  if (S->fireKernel == 1 && S->P.particle == PART_PASSIVE && S->P.geometry == GEOM_NORMAL) {
     // one workgroup per system, at most `pchunk` iterations per launch
     rc = clean_protocol(S);
     if (rc) return rc;
@@ -1392,7 +1448,7 @@ int dpm_get_phi(dpm_set_t h, double* out) {
   int rc = reduce_host(S);
   if (rc) return rc;
   for (long s = 0; s < S->nsys; s++)
-    out[s] = (S->h_red[s * 8 + 5] + kPI * S->h_red[s * 8 + 7]) / (S->h_box[2 * s] * S->h_box[2 * s + 1]);
+    out[s] = (S->h_red[s * 8 + 5] + kPI * S->h_red[s * 8 + 7]) / box_area(S, s);
   return DPM_OK;
 }
 
@@ -1401,7 +1457,7 @@ int dpm_get_preferred_phi(dpm_set_t h, double* out) {
   int rc = reduce_host(S);
   if (rc) return rc;
   for (long s = 0; s < S->nsys; s++)
-    out[s] = (S->h_red[s * 8 + 6] + kPI * S->h_red[s * 8 + 7]) / (S->h_box[2 * s] * S->h_box[2 * s + 1]);
+    out[s] = (S->h_red[s * 8 + 6] + kPI * S->h_red[s * 8 + 7]) / box_area(S, s);
   return DPM_OK;
 }
 

@@ -21,12 +21,14 @@ enum Potential : int { POT_HARMONIC = 0, POT_LJ = 1, POT_ADHESIVE = 2, POT_WCA =
 enum Interaction : int { INT_VV = 0, INT_SMOOTH = 1, INT_CELL_SMOOTH = 2, INT_ALL = 3 };
 enum Monomer : int { MONO_HARMONIC = 0, MONO_FENE = 1 };
 enum ParticleKind : int { PART_PASSIVE = 0, PART_ACTIVE = 1, PART_ACTIVE_SHAPE = 2 };
+enum Geometry : int { GEOM_NORMAL = 0, GEOM_FIXED_BOX = 1, GEOM_FIXED_SIDES = 2, GEOM_ROUND_BOX = 3 };
 
 // Philox stream tags (counter word 3, high byte)
 enum RngTag : uint32_t { TAG_THERMAL = 1, TAG_INJECT = 2, TAG_ACTIVE = 3, TAG_SHAPE = 4 };
 
 struct Params {
-  int potential, interaction, monomer, particle;
+  int potential, interaction, monomer, particle, geometry;
+  double ew, boxRadius;  // wall energy scale (setWallEnergyScale), round-box radius (setBoxRadius)
   double ea, el, eb, ec, l1, l2, LJcutoff, LJecut, LJfshift, stiff, extSq;
   double dt;
   double driving, taup;
@@ -43,6 +45,7 @@ struct View {
   double *vx, *vy, *fx, *fy;
   double *rad, *l0, *th0, *th, *len, *ev, *es;
   double* l0r;  // activeShape rest lengths, or null
+  double *wfx, *wfy;  // force on the walls per vertex (getWallForce), written with walls
   double *lx, *ly;
   int *vring, *vsys;
   int4* vinfo;  // per vertex in one load: ring, ring's first vertex, ring size, system
@@ -50,6 +53,7 @@ struct View {
   double *a0, *area, *perim, *cx, *cy, *pe, *ang;
   double *bx, *by, *ibx, *iby;  // per-system box and its reciprocal
   int *svoff, *sroff;  // per-system vertex / ring offsets (nsys + 1)
+  double *mbx, *mby;   // per-system image multipliers (Box::mx, Box::my)
   double* kr;          // per-ring shape costs [ka(np) | kl(np) | kb(np)], or null for the globals
   int *nbr, *ncnt;     // ELL neighbour list nbr[k * nv + v], count[v]
   int2* nbrp;          // beside each entry u: (prev(u), prev(prev(u))) in u's ring
@@ -64,6 +68,7 @@ struct View {
 // minimum image itself is ambiguous (and far beyond every interaction range).
 struct Box {
   double Lx, Ly, iLx, iLy;
+  double mx, my;  // image multipliers: 1/L along a periodic direction, 0 along a walled one
 };
 
 // ------------------------------------------------------------ geometry ---
@@ -71,8 +76,10 @@ struct Box {
 __device__ __forceinline__ double pbc(double delta, double size) {
   return delta - size * round(delta / size);
 }
-__device__ __forceinline__ double pbx(double delta, const Box& b) { return delta - b.Lx * rint(delta * b.iLx); }
-__device__ __forceinline__ double pby(double delta, const Box& b) { return delta - b.Ly * rint(delta * b.iLy); }
+// boxDistance, cudaKernel.cuh:76-94: with a zero multiplier (a walled direction) the difference
+// is returned as it is
+__device__ __forceinline__ double pbx(double delta, const Box& b) { return delta - b.Lx * rint(delta * b.mx); }
+__device__ __forceinline__ double pby(double delta, const Box& b) { return delta - b.Ly * rint(delta * b.my); }
 
 // ----------------------------------------------------------- Philox RNG ---
 // Philox4x32-10 (Salmon et al., SC'11): 10 rounds, 9 key bumps.
@@ -216,6 +223,51 @@ __device__ __forceinline__ double grad_multiple(const Params& P, double distance
     }
     return 0;
   }
+}
+
+// ------------------------------------------------------------------ walls ---
+// calcWallWCAInteraction, cudaKernel.cuh:1221-1237: WCA (energy scale ew) between a vertex of
+// radius r and its image (wx, wy) on the wall; the wall takes the opposite force.
+__device__ __forceinline__ double wall_wca(const Params& P, const Box& B, double x, double y, double r, double wx,
+                                           double wy, double& fx, double& fy, double& wfx, double& wfy) {
+  const double dx = pbx(x - wx, B), dy = pby(y - wy, B);
+  const double distance = sqrt(dx * dx + dy * dy);
+  if (distance < (kWCAcut * r)) {
+    const double ratio = r / distance;
+    const double ratio6 = pow(ratio, 6);
+    const double ratio12 = ratio6 * ratio6;
+    const double gm = 24 * P.ew * (2 * ratio12 - ratio6) / distance;
+    fx += gm * dx / distance;
+    fy += gm * dy / distance;
+    wfx -= gm * dx / distance;
+    wfy -= gm * dy / distance;
+    return P.ew * (4 * (ratio12 - ratio6) + 1);
+  }
+  return 0.0;
+}
+
+// kernelCalcVertexSquareWallInteraction (fixedBox), kernelCalcVertexSidesInteraction2D
+// (fixedSides) and kernelCalcVertexRoundWallInteraction (roundBox, centred on the origin),
+// cudaKernel.cuh:1324-1410.  Adds to (fx, fy), returns the wall energy, writes the wall force.
+__device__ __forceinline__ double wall_forces(const Params& P, const Box& B, double x, double y, double r, double& fx,
+                                              double& fy, double& wfx, double& wfy) {
+  wfx = wfy = 0.0;
+  double e = 0.0;
+  const double rc = kWCAcut * r;
+  if (P.geometry == GEOM_FIXED_BOX) {
+    if (x < rc) e += wall_wca(P, B, x, y, r, 0.0, y, fx, fy, wfx, wfy);
+    else if ((B.Lx - x) < rc) e += wall_wca(P, B, x, y, r, B.Lx, y, fx, fy, wfx, wfy);
+  }
+  if (P.geometry == GEOM_FIXED_BOX || P.geometry == GEOM_FIXED_SIDES) {
+    if (y < rc) e += wall_wca(P, B, x, y, r, x, 0.0, fx, fy, wfx, wfy);
+    else if ((B.Ly - y) < rc) e += wall_wca(P, B, x, y, r, x, B.Ly, fx, fy, wfx, wfy);
+  }
+  if (P.geometry == GEOM_ROUND_BOX) {
+    const double rr = sqrt(x * x + y * y), th = atan2(y, x);
+    if ((P.boxRadius - rr) < rc)
+      e += wall_wca(P, B, x, y, r, P.boxRadius * cos(th), P.boxRadius * sin(th), fx, fy, wfx, wfy);
+  }
+  return e;
 }
 
 // ------------------------------------------------- smooth vertex-segment ---

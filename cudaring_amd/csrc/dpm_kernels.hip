@@ -295,6 +295,14 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
       gauss_pair(P.seed, (uint32_t)L.s, TAG_THERMAL, A.ctr[0] - 1, (uint32_t)(L.r - V.sroff[L.s]), zt0, zt1);
     if constexpr ((MODE & MODE_BAOAB) != 0 && (MODE & MODE_START) != 0)
       gauss_pair(P.seed, (uint32_t)L.s, TAG_THERMAL, A.ctr[0], (uint32_t)(L.r - V.sroff[L.s]), zt0, zt1);
+    if (P.geometry != GEOM_NORMAL) {  // walls, after the interactions (DPM2D.cu:2090-2103)
+      double wfx, wfy;
+      so.e += wall_forces(P, B, x, y, V.rad[L.v], so.fx, so.fy, wfx, wfy);
+      if (L.act) {
+        V.wfx[L.v] = wfx;
+        V.wfy[L.v] = wfy;
+      }
+    }
     if (P.particle == PART_ACTIVE_SHAPE && L.act) {  // updateRestLength, DPM2D.cu:2037-2056
       double z0, z1;
       gauss_pair(P.seed, (uint32_t)L.s, TAG_SHAPE, A.ctr[1] - 1, (uint32_t)(L.r - V.sroff[L.s]), z0, z1);

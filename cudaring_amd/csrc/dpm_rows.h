@@ -5,7 +5,9 @@
 
 namespace dpm {
 
-__device__ __forceinline__ Box box_of(const View& V, int s) { return Box{V.bx[s], V.by[s], V.ibx[s], V.iby[s]}; }
+__device__ __forceinline__ Box box_of(const View& V, int s) {
+  return Box{V.bx[s], V.by[s], V.ibx[s], V.iby[s], V.mbx[s], V.mby[s]};
+}
 
 __device__ __forceinline__ void cell_of(double x, double y, const Box& B, int ncx, int ncy, int& cx, int& cy) {
   const double wx = x - B.Lx * floor(x * B.iLx), wy = y - B.Ly * floor(y * B.iLy);

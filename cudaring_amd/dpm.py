@@ -14,13 +14,14 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libdpmring.so")
 
 FIELDS = ["pos", "vel", "force", "energy", "rad", "l0", "theta0", "theta", "length", "a0", "area",
-          "perimeter", "ppos", "penergy", "pangle", "lastpos", "box"]
+          "perimeter", "ppos", "penergy", "pangle", "lastpos", "box", "wallforce"]
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 POT = {"harmonic": 0, "lj": 1, "adhesive": 2, "wca": 3}
 INTER = {"vertexVertex": 0, "vertexSmooth": 1, "all": 3}
 MONO = {"harmonic": 0, "FENE": 1}
 PART = {"passive": 0, "active": 1, "activeShape": 2}
 NOISE = {"langevin": 0, "baoab": 1, "brownian": 2, "drivenBrownian": 3}
+GEOM = {"normal": 0, "fixedBox": 1, "fixedSides": 2, "roundBox": 3}
 ERRORS = {-1: "bad argument", -2: "HIP error", -3: "out of memory", -4: "unsupported",
           -5: "overflow", -6: "bad state"}
 
@@ -46,6 +47,10 @@ SIGNATURES = {
     "dpm_set_self_propulsion": (_i, [_vp, _d, _d]),
     "dpm_set_active_tension": (_i, [_vp, _d, _d]),
     "dpm_set_noise_type": (_i, [_vp, _i]),
+    "dpm_set_geometry": (_i, [_vp, _i]),
+    "dpm_set_box_energy_cost": (_i, [_vp, _d]),
+    "dpm_set_box_radius": (_i, [_vp, _d]),
+    "dpm_get_box_radius": (_i, [_vp, _vp]),
     "dpm_set_seed": (_i, [_vp, C.c_uint64]),
     "dpm_set_displacement_cutoff": (_i, [_vp, _d, _d, C.POINTER(_d)]),
     "dpm_calc_neighbor_list": (_i, [_vp, _d]),
@@ -158,7 +163,7 @@ class DPMSet:
     def field_size(self, name):
         n = {"v": self.nv, "r": self.np, "s": self.nsys}
         kind = {"pos": ("v", 2), "vel": ("v", 2), "force": ("v", 2), "lastpos": ("v", 2),
-                "ppos": ("r", 2), "box": ("s", 2)}
+                "wallforce": ("v", 2), "ppos": ("r", 2), "box": ("s", 2)}
         if name in kind:
             k, c = kind[name]
             return n[k] * c
@@ -221,6 +226,15 @@ class DPMSet:
     def set_active_tension(self, l0_tau, l0_diff):
         """activeShape rest-length dynamics; the rest lengths become the current l0."""
         self._chk(self.L.dpm_set_active_tension(self.h, l0_tau, l0_diff), "set_active_tension")
+
+    def set_geometry(self, geometry, ew=None, box_radius=None):
+        """Walls: normal (periodic), fixedBox, fixedSides or roundBox; wall energy scale ew and
+        round-box radius when given."""
+        self._chk(self.L.dpm_set_geometry(self.h, GEOM[geometry]), "set_geometry")
+        if ew is not None:
+            self._chk(self.L.dpm_set_box_energy_cost(self.h, ew), "set_box_energy_cost")
+        if box_radius is not None:
+            self._chk(self.L.dpm_set_box_radius(self.h, box_radius), "set_box_radius")
 
     def set_noise_type(self, noise):
         """Integrator of the Langevin loop: langevin, baoab, brownian or drivenBrownian."""

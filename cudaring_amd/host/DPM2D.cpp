@@ -135,6 +135,9 @@ void DPM2D::push_params() {
   const int part = (int)simControl.particleType;  // passive, active, activeShape = DPM_PART_*
   check(dpm_set_types(h_, pot, inter, (int)simControl.monomerType, part), "setTypes");
   check(dpm_set_noise_type(h_, (int)simControl.noiseType), "setNoiseType");
+  check(dpm_set_geometry(h_, (int)simControl.geometryType), "setGeometryType");
+  check(dpm_set_box_energy_cost(h_, ew), "setBoxEnergyCost");
+  if (boxRadius > 0) check(dpm_set_box_radius(h_, boxRadius), "setBoxRadius");
   check(dpm_set_energy_costs(h_, ea, el, eb, ec), "setEnergyCosts");
   check(dpm_set_attraction_constants(h_, l1, l2), "setAttractionConstants");
   if (LJcutoff > 0) check(dpm_set_lj_cutoff(h_, LJcutoff), "setLJcutoff");
@@ -165,9 +168,16 @@ void DPM2D::setSimulationType(simControlStruct::simulationEnum t) {
   simControl.simulationType = t;
 }
 void DPM2D::setGeometryType(simControlStruct::geometryEnum t) {
-  if (t != simControlStruct::geometryEnum::normal)
-    throw std::invalid_argument("DPM2D: only the periodic (normal) geometry is provided");
-  simControl.geometryType = t;
+  simControl.geometryType = t;  // normal, fixedBox, fixedSides, roundBox = DPM_GEOM_*
+  push_params();
+}
+void DPM2D::setBoxEnergyCost(double ew_) {
+  ew = ew_;
+  push_params();
+}
+void DPM2D::setBoxRadius(double boxRadius_) {
+  boxRadius = boxRadius_;
+  push_params();
 }
 void DPM2D::setParticleType(simControlStruct::particleEnum t) {
   simControl.particleType = t;

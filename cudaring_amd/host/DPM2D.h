@@ -43,6 +43,7 @@ class DPM2D {
   long nDim, numParticles, numVertexPerParticle, numVertices;
   double dt, rho0, calA0, ea, el, eb, ec, driving, taup, l1, l2, LJcutoff, LJecut, LJfshift, stiff, extSq;
   double l0Tau = 1, l0Diff = 0;  // activeShape rest-length dynamics
+  double ew = 1e3, boxRadius = 0;  // wall energy scale (DPM2D.cu:70), round-box radius
   double cutoff, cutDistance;
   uint64_t seed;
 
@@ -110,6 +111,9 @@ class DPM2D {
   void setSelfPropulsionParams(double driving_, double taup_);
   void getSelfPropulsionParams(double& driving_, double& taup_) { driving_ = driving; taup_ = taup; }
   void setActiveTensionParams(double l0Tau_, double l0Diff_);
+  void setBoxEnergyCost(double ew_);
+  void setBoxRadius(double boxRadius_);
+  double getBoxRadius() const { return boxRadius; }
   void getActiveTensionParams(double& l0Tau_, double& l0Diff_);
   void setAttractionConstants(double l1_, double l2_);
   void setLJcutoff(double LJcutoff_);

@@ -58,7 +58,8 @@ enum {
   DPM_FIELD_PANGLE = 14,   /* R   setParticleAngles / getParticleAngles, DPM2D.h:326-327         */
   DPM_FIELD_LASTPOS = 15,  /* 2V  d_lastPos (neighbour-list reference positions)               */
   DPM_FIELD_BOX = 16,      /* 2B  setBoxSize / getBoxSize, DPM2D.h:275-276 (per system)        */
-  DPM_FIELD_COUNT = 17
+  DPM_FIELD_WALLFORCE = 17,/* 2V  d_wallForce (force on the walls per vertex), DPM2D.h:146       */
+  DPM_FIELD_COUNT = 18
 };
 
 /* potential / interaction / monomer / particle enums keep simControlStruct's order, DPM2D.h:22-32 */
@@ -67,6 +68,7 @@ enum { DPM_INT_VERTEX_VERTEX = 0, DPM_INT_VERTEX_SMOOTH = 1, DPM_INT_ALL = 3 };
 enum { DPM_MONO_HARMONIC = 0, DPM_MONO_FENE = 1 };
 enum { DPM_PART_PASSIVE = 0, DPM_PART_ACTIVE = 1, DPM_PART_ACTIVE_SHAPE = 2 };
 enum { DPM_NOISE_LANGEVIN = 0, DPM_NOISE_BAOAB = 1, DPM_NOISE_BROWNIAN = 2, DPM_NOISE_DRIVEN_BROWNIAN = 3 };
+enum { DPM_GEOM_NORMAL = 0, DPM_GEOM_FIXED_BOX = 1, DPM_GEOM_FIXED_SIDES = 2, DPM_GEOM_ROUND_BOX = 3 };
 
 /* ---- lifecycle ------------------------------------------------------------ */
 /* Replaces DPM2D(long nParticles, long dim, long nVertexPerParticle) (DPM2D.h:47) followed by
@@ -92,6 +94,16 @@ int dpm_device_ptr(dpm_set_t h, int field, int component, void** ptr);
 /* ---- parameters ----------------------------------------------------------- */
 /* setPotentialType / setInteractionType / setMonomerType / setParticleType, DPM2D.h:225-250 */
 int dpm_set_types(dpm_set_t h, int potential, int interaction, int monomer, int particle);
+/* setGeometryType, DPM2D.h:228 / DPM2D.cu:338-358 (simControlStruct::geometryEnum order): walls
+ * (WCA with energy scale ew) at x = 0, Lx and y = 0, Ly (fixedBox), at y = 0, Ly with x periodic
+ * (fixedSides), or on the circle of radius boxRadius about the origin (roundBox); distances lose
+ * their periodic image along walled directions (boxDistance, cudaKernel.cuh:76-94) */
+int dpm_set_geometry(dpm_set_t h, int geometry);
+/* setBoxEnergyCost(ew), DPM2D.h:537 / DPM2D.cu:1911-1914 (default 1e3, DPM2D.cu:70) */
+int dpm_set_box_energy_cost(dpm_set_t h, double ew);
+/* setBoxRadius / getBoxRadius, DPM2D.h:278-280 / DPM2D.cu:627-664 (roundBox) */
+int dpm_set_box_radius(dpm_set_t h, double boxRadius);
+int dpm_get_box_radius(dpm_set_t h, double* boxRadius);
 /* setEnergyCosts(ea, el, eb, ec), DPM2D.h:525 */
 int dpm_set_energy_costs(dpm_set_t h, double ea, double el, double eb, double ec);
 /* Per-ring shape costs ka[R], kl[R], kb[R] replacing the global ea, el, eb of setEnergyCosts in the

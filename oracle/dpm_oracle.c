@@ -37,6 +37,9 @@ struct orc_state {
   int simKind;
   double Tinject, gamma, noise;
   int noiseType;            /* simControlStruct::noiseEnum order */
+  int geometryType;         /* simControlStruct::geometryEnum order */
+  double ew, boxRadius;     /* wall energy scale, round-box radius */
+  double* wallForce;        /* [numVertices * 2], rewritten by each force call with walls */
   double l0Tau, l0Diff;     /* activeShape (setActiveTensionParams) */
   double* l0Rest;
   double a_start, a, f_dec, f_inc, f_a, fire_dt_max, fire_dt, lastP;
@@ -116,10 +119,13 @@ orc_state* orc_create(long numParticles, const long* nvp) {
   s->dt = 1e-3; s->ea = 1e5; s->el = 20; s->eb = 10; s->ec = 1;
   s->cutDistance = 1; s->updateCount = 0; s->shift = 0;
   s->gamma = 1; s->noise = 0;
+  s->ew = 1e3; /* DPM2D.cu:70 */
+  s->wallForce = calloc(n2, 8);
   return s;
 }
 
 void orc_destroy(orc_state* s) {
+  free(s->wallForce);
   free(s->l0Rest);
   if (!s) return;
   free(s->numVertexInParticleList); free(s->firstVertexInParticleId); free(s->particleIdList);
@@ -145,13 +151,15 @@ static double* field_ptr(const orc_state* s, int f) {
     case ORC_F_PPOS: return s->particlePos; case ORC_F_PENERGY: return s->particleEnergy;
     case ORC_F_PANGLE: return s->particleAngle; case ORC_F_LASTPOS: return s->lastPos;
     case ORC_F_BOX: return (double*)s->boxSize;
+    case ORC_F_WALLFORCE: return s->wallForce;
   }
   return NULL;
 }
 
 long orc_field_size(const orc_state* s, int f) {
   switch (f) {
-    case ORC_F_POS: case ORC_F_VEL: case ORC_F_FORCE: case ORC_F_LASTPOS: return 2 * s->numVertices;
+    case ORC_F_POS: case ORC_F_VEL: case ORC_F_FORCE: case ORC_F_LASTPOS: case ORC_F_WALLFORCE:
+      return 2 * s->numVertices;
     case ORC_F_ENERGY: case ORC_F_RAD: case ORC_F_L0: case ORC_F_THETA0: case ORC_F_THETA:
     case ORC_F_LENGTH: return s->numVertices;
     case ORC_F_A0: case ORC_F_AREA: case ORC_F_PERIMETER: case ORC_F_PENERGY:
@@ -232,9 +240,17 @@ long orc_previous_id(const orc_state* s, long vertexId, long particleId) {
   return previousId + s->numVertexInParticleList[particleId];
 }
 
+/* boxDistance, cudaKernel.cuh:76-94: periodic image only along the periodic directions
+ * (both for normal, x for fixedSides, none for fixedBox and roundBox) */
 static inline double pbcd(const orc_state* s, double x1, double x2, int dim) {
-  return orc_pbc_distance(x1, x2, s->boxSize[dim]);
+  if (s->geometryType == ORC_GEOM_NORMAL || (s->geometryType == ORC_GEOM_FIXED_SIDES && dim == 0))
+    return orc_pbc_distance(x1, x2, s->boxSize[dim]);
+  return x1 - x2;
 }
+
+void orc_set_geometry(orc_state* s, int geometryType) { s->geometryType = geometryType; }
+void orc_set_wall_energy_scale(orc_state* s, double ew) { s->ew = ew; }
+void orc_set_box_radius(orc_state* s, double boxRadius) { s->boxRadius = boxRadius; }
 
 /* calcAngle, cudaKernel.cuh:423-427 */
 static double calc_angle(const double* n, const double* p) {
@@ -686,6 +702,50 @@ void orc_update_rest_length(orc_state* s) {
   }
 }
 
+/* calcWallWCAInteraction, cudaKernel.cuh:1221-1237: WCA of the vertex (radius = its own) with
+ * its image on the wall; the wall takes the opposite force */
+static double wall_wca(orc_state* s, long v, double wx, double wy) {
+  double d[2] = {pbcd(s, s->pos[2 * v], wx, 0), pbcd(s, s->pos[2 * v + 1], wy, 1)};
+  double radSum = s->rad[v];
+  double distance = sqrt(d[0] * d[0] + d[1] * d[1]);
+  double ratio = radSum / distance;
+  double ratio6 = pow(ratio, 6);
+  double ratio12 = ratio6 * ratio6;
+  if (distance < (WCAcut * radSum)) {
+    double gradMultiple = 24 * s->ew * (2 * ratio12 - ratio6) / distance;
+    for (int dim = 0; dim < 2; dim++) {
+      s->force[2 * v + dim] += gradMultiple * d[dim] / distance;
+      s->wallForce[2 * v + dim] -= gradMultiple * d[dim] / distance;
+    }
+    return s->ew * (4 * (ratio12 - ratio6) + 1);
+  }
+  return 0.0;
+}
+
+/* kernelCalcVertexSquareWallInteraction, kernelCalcVertexSidesInteraction2D and
+ * kernelCalcVertexRoundWallInteraction, cudaKernel.cuh:1324-1410 (the round box is centred on
+ * the origin) */
+void orc_calc_wall_forces(orc_state* s) {
+  const double* L = s->boxSize;
+  for (long v = 0; v < s->numVertices; v++) {
+    double x = s->pos[2 * v], y = s->pos[2 * v + 1], rc = WCAcut * s->rad[v];
+    s->wallForce[2 * v] = s->wallForce[2 * v + 1] = 0.;
+    if (s->geometryType == ORC_GEOM_FIXED_BOX) {
+      if (x < rc) s->energy[v] += wall_wca(s, v, 0, y);
+      else if ((L[0] - x) < rc) s->energy[v] += wall_wca(s, v, L[0], y);
+    }
+    if (s->geometryType == ORC_GEOM_FIXED_BOX || s->geometryType == ORC_GEOM_FIXED_SIDES) {
+      if (y < rc) s->energy[v] += wall_wca(s, v, x, 0);
+      else if ((L[1] - y) < rc) s->energy[v] += wall_wca(s, v, x, L[1]);
+    }
+    if (s->geometryType == ORC_GEOM_ROUND_BOX) {
+      double r = sqrt(x * x + y * y), theta = atan2(y, x);
+      if ((s->boxRadius - r) < rc)
+        s->energy[v] += wall_wca(s, v, s->boxRadius * cos(theta), s->boxRadius * sin(theta));
+    }
+  }
+}
+
 /* calcForceEnergyGPU, DPM2D.cu:2058-2104 with calcShapeForceEnergy 1976-1993 */
 void orc_calc_force_energy(orc_state* s) {
   orc_calc_particle_shape(s);
@@ -700,6 +760,7 @@ void orc_calc_force_energy(orc_state* s) {
   }
   if (s->particleType == ORC_ACTIVE) orc_add_self_propulsion(s);
   if (s->particleType == ORC_ACTIVE_SHAPE) orc_update_rest_length(s);
+  if (s->geometryType != ORC_GEOM_NORMAL) orc_calc_wall_forces(s);
 }
 
 /* ----------------------------------------------------------- neighbours --- */

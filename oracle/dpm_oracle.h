@@ -47,12 +47,13 @@ enum { ORC_MONO_HARMONIC = 0, ORC_MONO_FENE = 1 };
 enum { ORC_PASSIVE = 0, ORC_ACTIVE = 1, ORC_ACTIVE_SHAPE = 2 };
 enum { ORC_SIM_NONE = 0, ORC_SIM_LANGEVIN = 1, ORC_SIM_NVE = 2, ORC_SIM_NVE_RESCALE = 3 };
 enum { ORC_NOISE_LANGEVIN = 0, ORC_NOISE_BAOAB = 1, ORC_NOISE_BROWNIAN = 2, ORC_NOISE_DRIVEN_BROWNIAN = 3 };
+enum { ORC_GEOM_NORMAL = 0, ORC_GEOM_FIXED_BOX = 1, ORC_GEOM_FIXED_SIDES = 2, ORC_GEOM_ROUND_BOX = 3 };
 
 /* field ids shared by orc_set_field / orc_get_field */
 enum {
   ORC_F_POS = 0, ORC_F_VEL, ORC_F_FORCE, ORC_F_ENERGY, ORC_F_RAD, ORC_F_L0,
   ORC_F_THETA0, ORC_F_THETA, ORC_F_LENGTH, ORC_F_A0, ORC_F_AREA, ORC_F_PERIMETER,
-  ORC_F_PPOS, ORC_F_PENERGY, ORC_F_PANGLE, ORC_F_LASTPOS, ORC_F_BOX,
+  ORC_F_PPOS, ORC_F_PENERGY, ORC_F_PANGLE, ORC_F_LASTPOS, ORC_F_BOX, ORC_F_WALLFORCE,
   ORC_F_COUNT
 };
 
@@ -106,6 +107,10 @@ double orc_get_cut_distance(const orc_state* s);
 
 /* integrators (Simulator.cu:23-89, 206-238; DPM2D.cu:3713-3817) */
 void orc_set_noise_type(orc_state* s, int noiseType);
+void orc_set_geometry(orc_state* s, int geometryType);
+void orc_set_wall_energy_scale(orc_state* s, double ew);
+void orc_set_box_radius(orc_state* s, double boxRadius);
+void orc_calc_wall_forces(orc_state* s);
 void orc_init_langevin(orc_state* s, double T, double gamma, int readState, int zeroMomentum);
 void orc_set_active_tension(orc_state* s, double l0Tau, double l0Diff);
 void orc_update_rest_length(orc_state* s);

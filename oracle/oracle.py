@@ -17,8 +17,9 @@ INTER = {"vertexVertex": 0, "vertexSmooth": 1, "cellSmooth": 2, "all": 3}
 MONO = {"harmonic": 0, "FENE": 1}
 PART = {"passive": 0, "active": 1, "activeShape": 2}
 NOISE = {"langevin": 0, "baoab": 1, "brownian": 2, "drivenBrownian": 3}
+GEOM = {"normal": 0, "fixedBox": 1, "fixedSides": 2, "roundBox": 3}
 FIELDS = ["pos", "vel", "force", "energy", "rad", "l0", "theta0", "theta", "length",
-          "a0", "area", "perimeter", "ppos", "penergy", "pangle", "lastpos", "box"]
+          "a0", "area", "perimeter", "ppos", "penergy", "pangle", "lastpos", "box", "wallforce"]
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 
 _lib = None
@@ -77,6 +78,9 @@ def lib():
             "orc_get_cut_distance": (d, [vp]),
             "orc_init_langevin": (None, [vp, d, d, i, i]),
             "orc_set_noise_type": (None, [vp, i]),
+            "orc_set_geometry": (None, [vp, i]),
+            "orc_set_wall_energy_scale": (None, [vp, d]),
+            "orc_set_box_radius": (None, [vp, d]),
             "orc_set_active_tension": (None, [vp, d, d]),
             "orc_update_rest_length": (None, [vp]),
             "orc_init_nve": (None, [vp, d, i, i]),
@@ -225,6 +229,13 @@ class Oracle:
 
     def mean_vertex_radius(self):
         return self.L.orc_get_mean_vertex_radius(self.h)
+
+    def set_geometry(self, geometry, ew=None, box_radius=None):
+        self.L.orc_set_geometry(self.h, GEOM[geometry])
+        if ew is not None:
+            self.L.orc_set_wall_energy_scale(self.h, ew)
+        if box_radius is not None:
+            self.L.orc_set_box_radius(self.h, box_radius)
 
     def set_noise_type(self, noise):
         self.L.orc_set_noise_type(self.h, NOISE[noise])
