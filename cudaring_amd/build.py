@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdpmring.so")
 
-SOURCES = ["dpm_kernels.hip", "dpm_adjoint.hip", "dpm_api.hip"]
+SOURCES = ["dpm_kernels.hip", "dpm_adjoint.hip", "dpm_observe.hip", "dpm_api.hip"]
 
 # C++ class layer (DPM2D / FIRE / Simulator / ioDPMFile) and the drivers built on it: plain host
 # C++ over the C ABI, linked against libdpmring.so (found next to bin/ through the rpath)

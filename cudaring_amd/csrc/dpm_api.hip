@@ -606,6 +606,56 @@ int backward_segment(dpm_set_s* S, long a, long top, long N) {
   return DPM_OK;
 }
 
+// scratch of at least n doubles for the observables
+int obs_scratch(dpm_set_s* S, long n) {
+  if (S->obsLen >= n) return DPM_OK;
+  HIPCHK(hipStreamSynchronize(S->stream));
+  hipFree(S->d_obs);
+  S->d_obs = nullptr;
+  HIPCHK(dalloc(&S->d_obs, n));
+  S->obsLen = n;
+  return DPM_OK;
+}
+
+// setInitialPositions + setInitialParticlePositions, DPM2D.cu:921-930
+int store_initial_positions(dpm_set_s* S) {
+  if (!S->d_x0) HIPCHK(dalloc(&S->d_x0, 2 * S->nv + 2 * S->np));
+  HIPCHK(launch_shape_only(S));
+  HIPCHK(launch_copy_d(S->d_x0, S->V.x, S->nv, S->stream));
+  HIPCHK(launch_copy_d(S->d_x0 + S->nv, S->V.y, S->nv, S->stream));
+  HIPCHK(launch_copy_d(S->d_x0 + 2 * S->nv, S->V.cx, S->np, S->stream));
+  HIPCHK(launch_copy_d(S->d_x0 + 2 * S->nv + S->np, S->V.cy, S->np, S->stream));
+  return DPM_OK;
+}
+
+// per-system sums of the displacement kernel's two outputs: (|dr|^2, sin(k|dr|)/(k|dr|))
+int displacement_sums(dpm_set_s* S, bool rings, double waveNum, std::vector<double>& d2, std::vector<double>& sf) {
+  if (!S->d_x0) return fail(S, DPM_ERR_STATE, "initial positions not set (init a simulation or call "
+                                               "dpm_set_initial_positions)");
+  const long n = rings ? S->np : S->nv;
+  int rc = obs_scratch(S, 2 * n);
+  if (rc) return rc;
+  if (rings) HIPCHK(launch_shape_only(S));
+  const double* x = rings ? S->V.cx : S->V.x;
+  const double* y = rings ? S->V.cy : S->V.y;
+  const double* x0 = S->d_x0 + (rings ? 2 * S->nv : 0);
+  const double* y0 = x0 + (rings ? S->np : S->nv);
+  HIPCHK(launch_displacement(S, x, y, x0, y0, n, rings ? 1 : 0, waveNum, S->d_obs, S->d_obs + n));
+  std::vector<double> h(2 * n);
+  HIPCHK(hipMemcpyAsync(h.data(), S->d_obs, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, S->stream));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  d2.assign(S->nsys, 0.0);
+  sf.assign(S->nsys, 0.0);
+  for (long s = 0; s < S->nsys; s++) {
+    const long a = rings ? S->h_sroff[s] : S->h_svoff[s], b = rings ? S->h_sroff[s + 1] : S->h_svoff[s + 1];
+    for (long i = a; i < b; i++) {
+      d2[s] += h[i];
+      sf[s] += h[n + i];
+    }
+  }
+  return DPM_OK;
+}
+
 }  // namespace
 
 // ======================================================================= ABI
@@ -762,7 +812,8 @@ int dpm_destroy(dpm_set_t h) {
                   S->G.coff, S->G.brute, S->G.count[0], S->G.count[1], S->G.slots, S->G.cellSys, S->d_flag[0],
                   S->d_flag[1], S->d_any, S->d_updateCount, S->d_err, S->d_maxcnt, S->d_ctr, S->d_notDone,
                   S->d_fire[0], S->d_fire[1], S->d_part, S->d_red, S->d_scale, V.kr,
-                  S->d_ck, S->d_fin, S->d_seg, S->d_adj, V.vinfo, V.l0r, V.mbx, V.mby, V.wfx, V.wfy};
+                  S->d_ck, S->d_fin, S->d_seg, S->d_adj, V.vinfo, V.l0r, V.mbx, V.mby, V.wfx, V.wfy,
+                  S->d_contacts, S->d_ccount, S->d_x0, S->d_obs};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (S->ownStream) hipStreamDestroy(S->ownStream);
@@ -1206,6 +1257,7 @@ int dpm_init_langevin(dpm_set_t h, double T, double gamma, int readState, int ze
   }
   bump(S);
   HIPCHK(launch_copy_lastpos(S));
+  { int r0 = store_initial_positions(S); if (r0) return r0; }
   if (!readState) {
     HIPCHK(launch_inject(S, sqrt(T), S->injectCalls++));
     if (zeroMomentum) HIPCHK(launch_sub_mean(S));
@@ -1220,6 +1272,7 @@ int dpm_init_nve(dpm_set_t h, double T, int readState, int zeroMomentum) {
   S->Tinject = T;
   bump(S);
   HIPCHK(launch_copy_lastpos(S));
+  { int r0 = store_initial_positions(S); if (r0) return r0; }
   if (!readState) {
     HIPCHK(launch_inject(S, sqrt(T), S->injectCalls++));
     if (zeroMomentum) HIPCHK(launch_sub_mean(S));
@@ -1250,6 +1303,7 @@ int dpm_init_nve_rescale(dpm_set_t h, double T) {
   S->Tinject = T;
   bump(S);
   HIPCHK(launch_copy_lastpos(S));
+  { int r0 = store_initial_positions(S); if (r0) return r0; }
   HIPCHK(launch_inject(S, sqrt(T), S->injectCalls++));
   return temperature_scale(S, T);
 }
@@ -1458,6 +1512,166 @@ int dpm_get_preferred_phi(dpm_set_t h, double* out) {
   if (rc) return rc;
   for (long s = 0; s < S->nsys; s++)
     out[s] = (S->h_red[s * 8 + 6] + kPI * S->h_red[s * 8 + 7]) / box_area(S, s);
+  return DPM_OK;
+}
+
+// setInitialPositions + setInitialParticlePositions, DPM2D.cu:921-930 (also done by the inits)
+int dpm_set_initial_positions(dpm_set_t h) {
+  CHECK_HANDLE(h);
+  int rc = store_initial_positions(S);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
+// calcContacts, DPM2D.cu:3408-3428: rows widen (doubling from 16) until every ring's contacts fit
+int dpm_calc_contacts(dpm_set_t h) {
+  CHECK_HANDLE(h);
+  int rc = require_list(S);
+  if (rc) return rc;
+  if (!S->d_ccount) HIPCHK(dalloc(&S->d_ccount, S->np));
+  int width = S->contactWidth > 0 ? S->contactWidth : 16;
+  for (;;) {
+    if (width != S->contactWidth || !S->d_contacts) {
+      HIPCHK(hipStreamSynchronize(S->stream));
+      hipFree(S->d_contacts);
+      S->d_contacts = nullptr;
+      HIPCHK(dalloc(&S->d_contacts, (long)S->np * width));
+      S->contactWidth = width;
+    }
+    HIPCHK(launch_fill_int(S->d_notDone, 0, 1, S->stream));
+    HIPCHK(launch_contacts(S, S->d_contacts, S->d_ccount, width, S->d_notDone));
+    int over = 0;
+    HIPCHK(hipMemcpyAsync(&over, S->d_notDone, sizeof(int), hipMemcpyDeviceToHost, S->stream));
+    HIPCHK(hipStreamSynchronize(S->stream));
+    if (!over) return DPM_OK;
+    if (width >= (1 << 20)) return fail(S, DPM_ERR_OVERFLOW, "contact rows do not fit");
+    width *= 2;
+  }
+}
+
+// getContacts, DPM2D.cu:1019-1024: rows of `cap` ring ids (-1 padded) and counts; the device
+// row width in *width.  Before any dpm_calc_contacts every ring has no contact.
+int dpm_get_contacts(dpm_set_t h, long* list, long* counts, long cap, long* width) {
+  CHECK_HANDLE(h);
+  const long np = S->np, W = S->contactWidth;
+  std::vector<int> rows((size_t)np * (W > 0 ? W : 1), -1), cnt(np, 0);
+  if (S->d_contacts) {
+    HIPCHK(hipMemcpyAsync(rows.data(), S->d_contacts, sizeof(int) * np * W, hipMemcpyDeviceToHost, S->stream));
+    HIPCHK(hipMemcpyAsync(cnt.data(), S->d_ccount, sizeof(int) * np, hipMemcpyDeviceToHost, S->stream));
+    HIPCHK(hipStreamSynchronize(S->stream));
+  }
+  for (long p = 0; p < np; p++) {
+    if (counts) counts[p] = cnt[p];
+    if (list)
+      for (long k = 0; k < cap; k++) list[p * cap + k] = (k < cnt[p] && k < W) ? rows[p * W + k] : -1;
+  }
+  if (width) *width = W;
+  return DPM_OK;
+}
+
+// getParticleHexaticOrder, DPM2D.cu:1505-1510: mean over each system's rings of psi6 from the
+// contacts of the last dpm_calc_contacts (0 for rings without contacts)
+int dpm_get_hexatic_order(dpm_set_t h, double* perSystem) {
+  CHECK_HANDLE(h);
+  if (!perSystem) return fail(S, DPM_ERR_ARG, "null output");
+  int rc = obs_scratch(S, S->np);
+  if (rc) return rc;
+  std::vector<double> psi(S->np, 0.0);
+  if (S->d_contacts) {
+    HIPCHK(launch_shape_only(S));
+    HIPCHK(launch_hexatic(S, S->d_contacts, S->d_ccount, S->contactWidth, S->d_obs));
+    HIPCHK(hipMemcpyAsync(psi.data(), S->d_obs, sizeof(double) * S->np, hipMemcpyDeviceToHost, S->stream));
+    HIPCHK(hipStreamSynchronize(S->stream));
+  }
+  for (long s = 0; s < S->nsys; s++) {
+    double t = 0;
+    for (long p = S->h_sroff[s]; p < S->h_sroff[s + 1]; p++) t += psi[p];
+    perSystem[s] = t / (S->h_sroff[s + 1] - S->h_sroff[s]);
+  }
+  return DPM_OK;
+}
+
+// getVertexMSD / getParticleMSD, DPM2D.cu:1284-1298: sum of squared displacements over the
+// system's vertices (ring centres) / (count * Lx * Ly)
+int dpm_get_vertex_msd(dpm_set_t h, double* perSystem) {
+  CHECK_HANDLE(h);
+  std::vector<double> d2, sf;
+  int rc = displacement_sums(S, false, 1.0, d2, sf);
+  if (rc) return rc;
+  for (long s = 0; s < S->nsys; s++)
+    perSystem[s] = d2[s] / ((S->h_svoff[s + 1] - S->h_svoff[s]) * S->h_box[2 * s] * S->h_box[2 * s + 1]);
+  return DPM_OK;
+}
+
+int dpm_get_particle_msd(dpm_set_t h, double* perSystem) {
+  CHECK_HANDLE(h);
+  std::vector<double> d2, sf;
+  int rc = displacement_sums(S, true, 1.0, d2, sf);
+  if (rc) return rc;
+  for (long s = 0; s < S->nsys; s++)
+    perSystem[s] = d2[s] / ((S->h_sroff[s + 1] - S->h_sroff[s]) * S->h_box[2 * s] * S->h_box[2 * s + 1]);
+  return DPM_OK;
+}
+
+// getVertexISF (wave number pi / (2 r) of the system's first vertex) / getParticleISF(k),
+// DPM2D.cu:1486-1503: mean of sin(k d) / (k d)
+int dpm_get_vertex_isf(dpm_set_t h, double* perSystem) {
+  CHECK_HANDLE(h);
+  std::vector<double> d2, sf;
+  int rc = displacement_sums(S, false, 0.0, d2, sf);
+  if (rc) return rc;
+  for (long s = 0; s < S->nsys; s++) perSystem[s] = sf[s] / (S->h_svoff[s + 1] - S->h_svoff[s]);
+  return DPM_OK;
+}
+
+int dpm_get_particle_isf(dpm_set_t h, double waveNumber, double* perSystem) {
+  CHECK_HANDLE(h);
+  if (!(waveNumber > 0)) return fail(S, DPM_ERR_ARG, "waveNumber must be positive");
+  std::vector<double> d2, sf;
+  int rc = displacement_sums(S, true, waveNumber, d2, sf);
+  if (rc) return rc;
+  for (long s = 0; s < S->nsys; s++) perSystem[s] = sf[s] / (S->h_sroff[s + 1] - S->h_sroff[s]);
+  return DPM_OK;
+}
+
+// getPerParticleStressTensor, DPM2D.cu:962-967 / 2189-2198: 4 per ring (xx, xy, yx, yy)
+int dpm_get_per_particle_stress(dpm_set_t h, double* out) {
+  CHECK_HANDLE(h);
+  int rc = require_list(S);
+  if (rc) return rc;
+  rc = obs_scratch(S, 4 * S->np);
+  if (rc) return rc;
+  HIPCHK(launch_shape_only(S));
+  HIPCHK(launch_particle_stress(S, S->d_obs));
+  HIPCHK(hipMemcpyAsync(out, S->d_obs, sizeof(double) * 4 * S->np, hipMemcpyDeviceToHost, S->stream));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
+// getStressTensor, DPM2D.cu:969-974 / 2181-2187: per system the sum of its rings' tensors (the
+// reference's kernelCalcStressTensor adds ring 0's entries once per ring without atomics;
+// the sum is what it is meant to form); getPressure, DPM2D.cu:976-983: trace / (2 Nv)
+int dpm_get_stress_tensor(dpm_set_t h, double* out4) {
+  CHECK_HANDLE(h);
+  std::vector<double> per(4 * S->np);
+  int rc = dpm_get_per_particle_stress(h, per.data());
+  if (rc) return rc;
+  for (long s = 0; s < S->nsys; s++) {
+    for (int c = 0; c < 4; c++) out4[4 * s + c] = 0;
+    for (long p = S->h_sroff[s]; p < S->h_sroff[s + 1]; p++)
+      for (int c = 0; c < 4; c++) out4[4 * s + c] += per[4 * p + c];
+  }
+  return DPM_OK;
+}
+
+int dpm_get_pressure(dpm_set_t h, double* perSystem) {
+  CHECK_HANDLE(h);
+  std::vector<double> st(4 * S->nsys);
+  int rc = dpm_get_stress_tensor(h, st.data());
+  if (rc) return rc;
+  for (long s = 0; s < S->nsys; s++)
+    perSystem[s] = (st[4 * s] + st[4 * s + 3]) / (2.0 * (S->h_svoff[s + 1] - S->h_svoff[s]));
   return DPM_OK;
 }
 

@@ -138,6 +138,12 @@ struct dpm_set_s {
   // integrators
   int simKind = 0;  // 1 langevin, 2 nve, 3 nve rescale
   int noiseType = 0;  // integrator of the Langevin loop (NOISE_*)
+  // observables: contact rows (np x contactWidth, -1 padded) and counts, initial positions of
+  // vertices and ring centres, per-ring / per-vertex scratch
+  int *d_contacts = nullptr, *d_ccount = nullptr;
+  int contactWidth = 0;
+  double *d_x0 = nullptr, *d_obs = nullptr;  // d_x0: x0, y0 (nv each), cx0, cy0 (np each)
+  long obsLen = 0;
   double Tinject = 0;
   unsigned long long injectCalls = 0;
 
@@ -205,6 +211,11 @@ hipError_t launch_adj_hvp(dpm_set_s* S, const View& Vn, AdjArgs A);
 hipError_t launch_energy_param_partials(dpm_set_s* S, double* out);
 hipError_t launch_ring_sum3(dpm_set_s* S, const double* in, double* out);
 hipError_t launch_fill_d(double* p, double value, long n, hipStream_t st);
+hipError_t launch_contacts(dpm_set_s* S, int* list, int* count, int width, int* overflow);
+hipError_t launch_hexatic(dpm_set_s* S, const int* list, const int* count, int width, double* psi6);
+hipError_t launch_displacement(dpm_set_s* S, const double* x, const double* y, const double* x0, const double* y0,
+                               long n, int rings, double waveNum, double* d2, double* sf);
+hipError_t launch_particle_stress(dpm_set_s* S, double* out);
 hipError_t launch_copy_d(double* dst, const double* src, long n, hipStream_t st);
 
 }  // namespace dpm

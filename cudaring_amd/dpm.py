@@ -48,6 +48,17 @@ SIGNATURES = {
     "dpm_set_active_tension": (_i, [_vp, _d, _d]),
     "dpm_set_noise_type": (_i, [_vp, _i]),
     "dpm_set_geometry": (_i, [_vp, _i]),
+    "dpm_set_initial_positions": (_i, [_vp]),
+    "dpm_get_vertex_msd": (_i, [_vp, _vp]),
+    "dpm_get_particle_msd": (_i, [_vp, _vp]),
+    "dpm_get_vertex_isf": (_i, [_vp, _vp]),
+    "dpm_get_particle_isf": (_i, [_vp, _d, _vp]),
+    "dpm_calc_contacts": (_i, [_vp]),
+    "dpm_get_contacts": (_i, [_vp, _vp, _vp, _l, _vp]),
+    "dpm_get_hexatic_order": (_i, [_vp, _vp]),
+    "dpm_get_per_particle_stress": (_i, [_vp, _vp]),
+    "dpm_get_stress_tensor": (_i, [_vp, _vp]),
+    "dpm_get_pressure": (_i, [_vp, _vp]),
     "dpm_set_box_energy_cost": (_i, [_vp, _d]),
     "dpm_set_box_radius": (_i, [_vp, _d]),
     "dpm_get_box_radius": (_i, [_vp, _vp]),
@@ -361,6 +372,51 @@ class DPMSet:
         o = np.zeros(3 * self.np)
         self._chk(self.L.dpm_get_particle_shapes(self.h, _p(o)), "shapes")
         return o.reshape(self.np, 3)
+
+    # ----------------------------------------------------------- observables
+    def set_initial_positions(self):
+        self._chk(self.L.dpm_set_initial_positions(self.h), "set_initial_positions")
+
+    def vertex_msd(self):
+        return self._per_sys(self.L.dpm_get_vertex_msd, "vertex_msd")
+
+    def particle_msd(self):
+        return self._per_sys(self.L.dpm_get_particle_msd, "particle_msd")
+
+    def vertex_isf(self):
+        return self._per_sys(self.L.dpm_get_vertex_isf, "vertex_isf")
+
+    def particle_isf(self, k):
+        o = np.zeros(self.nsys)
+        self._chk(self.L.dpm_get_particle_isf(self.h, k, _p(o)), "particle_isf")
+        return o
+
+    def contacts(self):
+        """calcContacts, then (rows [np, width] -1 padded, counts [np])."""
+        self._chk(self.L.dpm_calc_contacts(self.h), "calc_contacts")
+        w = C.c_long()
+        self._chk(self.L.dpm_get_contacts(self.h, None, None, 0, C.byref(w)), "contacts")
+        cap = max(w.value, 1)
+        lst = np.empty(self.np * cap, dtype=np.int64)
+        cnt = np.empty(self.np, dtype=np.int64)
+        self._chk(self.L.dpm_get_contacts(self.h, _p(lst), _p(cnt), cap, C.byref(w)), "contacts")
+        return lst.reshape(self.np, cap), cnt
+
+    def hexatic_order(self):
+        return self._per_sys(self.L.dpm_get_hexatic_order, "hexatic_order")
+
+    def per_particle_stress(self):
+        o = np.zeros(4 * self.np)
+        self._chk(self.L.dpm_get_per_particle_stress(self.h, _p(o)), "per_particle_stress")
+        return o.reshape(self.np, 4)
+
+    def stress_tensor(self):
+        o = np.zeros(4 * self.nsys)
+        self._chk(self.L.dpm_get_stress_tensor(self.h, _p(o)), "stress_tensor")
+        return o.reshape(self.nsys, 4)
+
+    def pressure(self):
+        return self._per_sys(self.L.dpm_get_pressure, "pressure")
 
     def scale_vertices(self, scale):
         s = np.ascontiguousarray(np.broadcast_to(np.asarray(scale, dtype=np.float64), (self.nsys,)))

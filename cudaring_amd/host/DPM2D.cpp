@@ -363,6 +363,39 @@ double DPM2D::getKineticEnergy() { return one(dpm_get_kinetic_energy, this, "get
 double DPM2D::getTemperature() { return one(dpm_get_temperature, this, "getTemperature"); }
 double DPM2D::getPhi() { return one(dpm_get_phi, this, "getPhi"); }
 
+// ---------------------------------------------------------------- observables
+void DPM2D::setInitialPositions() { check(dpm_set_initial_positions(handle()), "setInitialPositions"); }
+double DPM2D::getVertexMSD() { return one(dpm_get_vertex_msd, this, "getVertexMSD"); }
+double DPM2D::getParticleMSD() { return one(dpm_get_particle_msd, this, "getParticleMSD"); }
+double DPM2D::getVertexISF() { return one(dpm_get_vertex_isf, this, "getVertexISF"); }
+double DPM2D::getParticleISF(double waveNumber) {
+  double v = 0;
+  check(dpm_get_particle_isf(handle(), waveNumber, &v), "getParticleISF");
+  return v;
+}
+double DPM2D::getParticleHexaticOrder() { return one(dpm_get_hexatic_order, this, "getParticleHexaticOrder"); }
+double DPM2D::getPressure() { return one(dpm_get_pressure, this, "getPressure"); }
+void DPM2D::calcContacts() { check(dpm_calc_contacts(handle()), "calcContacts"); }
+std::vector<long> DPM2D::getContacts() {
+  calcContacts();
+  long w = 0;
+  check(dpm_get_contacts(handle(), nullptr, nullptr, 0, &w), "getContacts");
+  contactListSize = w > 0 ? w : 1;
+  std::vector<long> rows((size_t)numParticles * contactListSize);
+  check(dpm_get_contacts(handle(), rows.data(), nullptr, contactListSize, &w), "getContacts");
+  return rows;
+}
+std::vector<double> DPM2D::getPerParticleStressTensor() {
+  std::vector<double> out(4 * numParticles);
+  check(dpm_get_per_particle_stress(handle(), out.data()), "getPerParticleStressTensor");
+  return out;
+}
+std::vector<double> DPM2D::getStressTensor() {
+  std::vector<double> out(4);
+  check(dpm_get_stress_tensor(handle(), out.data()), "getStressTensor");
+  return out;
+}
+
 double DPM2D::getPreferredPhi() {
   if (h_) return one(dpm_get_preferred_phi, this, "getPreferredPhi");
   // host form for set-up before a device set exists (DPM2D.cu:1222-1238, cudaKernel.cuh:1636-1642)

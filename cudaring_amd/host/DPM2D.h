@@ -132,6 +132,19 @@ class DPM2D {
   double getParticleKineticEnergy();
   double getShapeCOMEnergyRatio() { return getShapeKineticEnergy() / getParticleKineticEnergy(); }
   double getPhi();
+  // observables (DPM2D.h:375-477 of the reference)
+  void setInitialPositions();
+  double getVertexMSD();
+  double getParticleMSD();
+  double getVertexISF();
+  double getParticleISF(double waveNumber);
+  double getParticleHexaticOrder();
+  double getPressure();
+  void calcContacts();
+  std::vector<long> getContacts();  // numParticles x contactListSize, -1 padded
+  std::vector<double> getPerParticleStressTensor();
+  std::vector<double> getStressTensor();
+  long contactListSize = 0;
   double getPreferredPhi();
   double getMeanParticleSize();
   double getMeanVertexRadius();

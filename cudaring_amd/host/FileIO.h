@@ -63,7 +63,8 @@ class ioDPMFile {
     double ekin = dpm_->getKineticEnergy() / numVertices;
     energyFile << step + 1 << "\t" << (step + 1) * timeStep << "\t";
     energyFile << std::setprecision(precision) << epot << "\t" << ekin << "\t" << epot + ekin << "\t";
-    energyFile << dpm_->getShapeCOMEnergyRatio() << "\t" << "nan" << "\t" << dpm_->getPhi() << std::endl;
+    energyFile << dpm_->getShapeCOMEnergyRatio() << "\t" << dpm_->getParticleHexaticOrder() << "\t";
+    energyFile << dpm_->getPhi() << std::endl;
   }
 
   // ---- scalar, 1-D and 2-D files

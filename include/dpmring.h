@@ -155,6 +155,29 @@ int dpm_calc_force_energy(dpm_set_t h);
  * energies past the end of a per-ring array, DPM2D.cu:2221-2251; this returns the forces it means.) */
 int dpm_get_interaction_forces(dpm_set_t h, double* host, long n);
 
+/* ---- observables ---------------------------------------------------------- */
+/* setInitialPositions + setInitialParticlePositions, DPM2D.cu:921-930: the MSD / ISF reference
+ * positions (vertices and ring centres); the Langevin, NVE and NVE-rescale inits also store them */
+int dpm_set_initial_positions(dpm_set_t h);
+/* getVertexMSD / getParticleMSD, DPM2D.h:441-443 / DPM2D.cu:1284-1298 (per system) */
+int dpm_get_vertex_msd(dpm_set_t h, double* perSystem);
+int dpm_get_particle_msd(dpm_set_t h, double* perSystem);
+/* getVertexISF / getParticleISF(waveNumber), DPM2D.h:473-475 / DPM2D.cu:1486-1503 (per system) */
+int dpm_get_vertex_isf(dpm_set_t h, double* perSystem);
+int dpm_get_particle_isf(dpm_set_t h, double waveNumber, double* perSystem);
+/* calcContacts / getContacts, DPM2D.h:389, 632 / DPM2D.cu:1019-1024, 3408-3428: per ring the rings
+ * owning an entry of its vertices' rows, in order of first appearance; rows of `cap` (-1 padded),
+ * counts[R], device row width in *width */
+int dpm_calc_contacts(dpm_set_t h);
+int dpm_get_contacts(dpm_set_t h, long* list, long* counts, long cap, long* width);
+/* getParticleHexaticOrder, DPM2D.h:477 / DPM2D.cu:1505-1510 (from the last dpm_calc_contacts) */
+int dpm_get_hexatic_order(dpm_set_t h, double* perSystem);
+/* getPerParticleStressTensor / getStressTensor / getPressure, DPM2D.h:375-379 / DPM2D.cu:962-983,
+ * 2181-2198 (harmonic contacts): 4 per ring, 4 per system, 1 per system */
+int dpm_get_per_particle_stress(dpm_set_t h, double* perRing4);
+int dpm_get_stress_tensor(dpm_set_t h, double* perSystem4);
+int dpm_get_pressure(dpm_set_t h, double* perSystem);
+
 /* ---- integrators ---------------------------------------------------------- */
 /* setNoiseType, DPM2D.h:237 / DPM2D.cu:410-423: the integrator of langevinLoop, in
  * simControlStruct::noiseEnum order (0 langevin, 1 baoab, 2 brownian, 3 drivenBrownian;

@@ -40,6 +40,8 @@ struct orc_state {
   int geometryType;         /* simControlStruct::geometryEnum order */
   double ew, boxRadius;     /* wall energy scale, round-box radius */
   double* wallForce;        /* [numVertices * 2], rewritten by each force call with walls */
+  long *contactList, contactListSize, *maxContactList; /* per particle, -1 padded */
+  double *initialPos, *particleInitPos;                /* MSD / ISF references */
   double l0Tau, l0Diff;     /* activeShape (setActiveTensionParams) */
   double* l0Rest;
   double a_start, a, f_dec, f_inc, f_a, fire_dt_max, fire_dt, lastP;
@@ -121,10 +123,16 @@ orc_state* orc_create(long numParticles, const long* nvp) {
   s->gamma = 1; s->noise = 0;
   s->ew = 1e3; /* DPM2D.cu:70 */
   s->wallForce = calloc(n2, 8);
+  s->contactListSize = 0;
+  s->contactList = NULL;
+  s->maxContactList = (long*)calloc(np, sizeof(long));
+  s->initialPos = calloc(n2, 8);
+  s->particleInitPos = calloc(2 * np, 8);
   return s;
 }
 
 void orc_destroy(orc_state* s) {
+  free(s->contactList); free(s->maxContactList); free(s->initialPos); free(s->particleInitPos);
   free(s->wallForce);
   free(s->l0Rest);
   if (!s) return;
@@ -895,6 +903,7 @@ void orc_init_langevin(orc_state* s, double T, double gamma, int readState, int 
     s->noise = sqrt(2. * T * gamma / s->dt);
   }
   orc_reset_last_positions(s);
+  orc_set_initial_positions(s);
   if (!readState) orc_inject_kinetic_energy(s, zeroMomentum);
 }
 
@@ -903,6 +912,7 @@ void orc_init_nve(orc_state* s, double T, int readState, int zeroMomentum) {
   s->simKind = ORC_SIM_NVE;
   s->Tinject = T;
   orc_reset_last_positions(s);
+  orc_set_initial_positions(s);
   s->shift = 1;
   if (!readState) orc_inject_kinetic_energy(s, zeroMomentum);
 }
@@ -922,6 +932,7 @@ void orc_init_nve_rescale(orc_state* s, double T) {
   s->simKind = ORC_SIM_NVE_RESCALE;
   s->Tinject = T;
   orc_reset_last_positions(s);
+  orc_set_initial_positions(s);
   s->shift = 1;
   orc_inject_kinetic_energy(s, 0);
   double scale = sqrt(T / orc_get_temperature(s));
@@ -1189,4 +1200,176 @@ void orc_get_particle_shapes(orc_state* s, double* shape) {
     shape[p * 3 + 1] = s->perimeter[p];
     shape[p * 3 + 2] = s->perimeter[p] * s->perimeter[p] / (4 * PI * s->area[p]);
   }
+}
+
+/* ---------------------------------------------------------- observables --- */
+/* calcContacts + kernelCalcContacts, DPM2D.cu:3408-3428, cudaKernel.cuh:1791-1822: the particles
+ * owning any entry of the particle's vertices' rows, in order of first appearance; the row width
+ * doubles until every particle's contacts fit */
+void orc_calc_contacts(orc_state* s) {
+  long width = s->contactListSize > 0 ? s->contactListSize : 16;
+  for (;;) {
+    free(s->contactList);
+    s->contactList = (long*)malloc(sizeof(long) * s->numParticles * width);
+    for (long i = 0; i < s->numParticles * width; i++) s->contactList[i] = -1;
+    int fits = 1;
+    for (long p = 0; p < s->numParticles && fits; p++) {
+      long added = 0, f = s->firstVertexInParticleId[p];
+      for (long v = f; v < f + s->numVertexInParticleList[p]; v++) {
+        for (long k = 0; k < s->maxNeighborList[v]; k++) {
+          long other = s->neighborList[v * s->neighborListSize + k];
+          if (other == v || other == -1) continue;
+          long c = s->particleIdList[other];
+          int isNew = 1;
+          for (long q = 0; q < added; q++) if (s->contactList[p * width + q] == c) isNew = 0;
+          if (!isNew) continue;
+          if (added == width) { fits = 0; break; }
+          s->contactList[p * width + added++] = c;
+        }
+        if (!fits) break;
+      }
+      s->maxContactList[p] = added;
+    }
+    if (fits) break;
+    width *= 2;
+  }
+  s->contactListSize = width;
+}
+
+long orc_contact_list_size(const orc_state* s) { return s->contactListSize; }
+
+void orc_get_contacts(const orc_state* s, long* list, long* counts) {
+  if (list) memcpy(list, s->contactList, sizeof(long) * s->numParticles * s->contactListSize);
+  if (counts) memcpy(counts, s->maxContactList, sizeof(long) * s->numParticles);
+}
+
+/* getParticleHexaticOrder + kernelCalcHexaticOrderParameter, DPM2D.cu:1505-1510,
+ * cudaKernel.cuh:1951-1976: per particle |<exp(6 i theta)>| over its contacts (0 without), mean */
+double orc_get_hexatic_order(orc_state* s) {
+  orc_calc_particle_positions(s);
+  double sum = 0;
+  for (long p = 0; p < s->numParticles; p++) {
+    double re = 0, im = 0;
+    long n = s->maxContactList[p];
+    if (n > 0) {
+      for (long q = 0; q < n; q++) {
+        long o = s->contactList[p * s->contactListSize + q];
+        if (o == p || o == -1) continue;
+        double dx = pbcd(s, s->particlePos[2 * p], s->particlePos[2 * o], 0);
+        double dy = pbcd(s, s->particlePos[2 * p + 1], s->particlePos[2 * o + 1], 1);
+        double a = atan2(dy, dx);
+        re += cos(6.0 * a);
+        im += sin(6.0 * a);
+      }
+      re /= n;
+      im /= n;
+    }
+    sum += hypot(re, im);
+  }
+  return sum / s->numParticles;
+}
+
+/* setInitialPositions + setInitialParticlePositions, DPM2D.cu:921-930 */
+void orc_set_initial_positions(orc_state* s) {
+  memcpy(s->initialPos, s->pos, sizeof(double) * 2 * s->numVertices);
+  orc_calc_particle_positions(s);
+  memcpy(s->particleInitPos, s->particlePos, sizeof(double) * 2 * s->numParticles);
+}
+
+/* getVertexMSD / getParticleMSD, DPM2D.cu:1284-1298 (normalised by the box area) */
+double orc_get_vertex_msd(orc_state* s) {
+  double sum = 0;
+  for (long v = 0; v < s->numVertices; v++)
+    for (int d = 0; d < 2; d++) {
+      double dd = pbcd(s, s->pos[2 * v + d], s->initialPos[2 * v + d], d);
+      sum += dd * dd;
+    }
+  return sum / (s->numVertices * s->boxSize[0] * s->boxSize[1]);
+}
+
+double orc_get_particle_msd(orc_state* s) {
+  orc_calc_particle_positions(s);
+  double sum = 0;
+  for (long p = 0; p < s->numParticles; p++)
+    for (int d = 0; d < 2; d++) {
+      double dd = pbcd(s, s->particlePos[2 * p + d], s->particleInitPos[2 * p + d], d);
+      sum += dd * dd;
+    }
+  return sum / (s->numParticles * s->boxSize[0] * s->boxSize[1]);
+}
+
+/* getVertexISF / getParticleISF, DPM2D.cu:1486-1503, cudaKernel.cuh:1935-1949: isotropic
+ * sin(k d) / (k d), vertex wave number pi / (2 rad[0]) */
+double orc_get_vertex_isf(orc_state* s) {
+  double k = PI / (2 * s->rad[0]), sum = 0;
+  for (long v = 0; v < s->numVertices; v++) {
+    double d = calc_distance(s, &s->pos[2 * v], &s->initialPos[2 * v]);
+    sum += sin(k * d) / (k * d);
+  }
+  return sum / s->numVertices;
+}
+
+double orc_get_particle_isf(orc_state* s, double k) {
+  orc_calc_particle_positions(s);
+  double sum = 0;
+  for (long p = 0; p < s->numParticles; p++) {
+    double d = calc_distance(s, &s->particlePos[2 * p], &s->particleInitPos[2 * p]);
+    sum += sin(k * d) / (k * d);
+  }
+  return sum / s->numParticles;
+}
+
+/* kernelCalcPerParticleStressTensor, cudaKernel.cuh:1525-1565 (harmonic contacts): for each
+ * overlapping listed pair of the particle's vertices, (lever) (x) f with f = ec overlap / radSum
+ * * delta / (distance radSum), delta = other - this, and the lever = the vertex position
+ * relative to the particle centre plus delta / 2 of this and every earlier overlap of the same
+ * vertex (the reference accumulates it).  The rows are the vertices' own. */
+void orc_calc_per_particle_stress(orc_state* s, double* out) {
+  orc_calc_particle_positions(s);
+  for (long p = 0; p < s->numParticles; p++) {
+    double* st = out + 4 * p;
+    st[0] = st[1] = st[2] = st[3] = 0;
+    long f = s->firstVertexInParticleId[p];
+    for (long v = f; v < f + s->numVertexInParticleList[p]; v++) {
+      double rel[2] = {pbcd(s, s->pos[2 * v], s->particlePos[2 * p], 0),
+                       pbcd(s, s->pos[2 * v + 1], s->particlePos[2 * p + 1], 1)};
+      for (long k = 0; k < s->maxNeighborList[v]; k++) {
+        long o = s->neighborList[v * s->neighborListSize + k];
+        if (o == v || o == -1) continue;
+        double radSum = s->rad[v] + s->rad[o];
+        double distance = calc_distance(s, &s->pos[2 * v], &s->pos[2 * o]);
+        double overlap = 1 - distance / radSum;
+        if (overlap > 0) {
+          double gm = s->ec * overlap / radSum, delta[2], force[2];
+          for (int d = 0; d < 2; d++) {
+            delta[d] = pbcd(s, s->pos[2 * o + d], s->pos[2 * v + d], d);
+            force[d] = gm * delta[d] / (distance * radSum);
+            rel[d] += delta[d] * 0.5;
+          }
+          st[0] += rel[0] * force[0];
+          st[3] += rel[1] * force[1];
+          st[1] += rel[0] * force[1];
+          st[2] += rel[1] * force[0];
+        }
+      }
+    }
+  }
+}
+
+/* calcStressTensor, DPM2D.cu:2181-2187: the sum of the per-particle tensors (the reference's
+ * kernelCalcStressTensor adds particle 0's entries once per particle, without atomics); and
+ * getPressure, DPM2D.cu:976-983: trace / (nDim numVertices) */
+void orc_calc_stress_tensor(orc_state* s, double* out4) {
+  double* per = (double*)malloc(sizeof(double) * 4 * s->numParticles);
+  orc_calc_per_particle_stress(s, per);
+  out4[0] = out4[1] = out4[2] = out4[3] = 0;
+  for (long p = 0; p < s->numParticles; p++)
+    for (int c = 0; c < 4; c++) out4[c] += per[4 * p + c];
+  free(per);
+}
+
+double orc_get_pressure(orc_state* s) {
+  double st[4];
+  orc_calc_stress_tensor(s, st);
+  return (st[0] + st[3]) / (2 * s->numVertices);
 }

@@ -111,6 +111,19 @@ void orc_set_geometry(orc_state* s, int geometryType);
 void orc_set_wall_energy_scale(orc_state* s, double ew);
 void orc_set_box_radius(orc_state* s, double boxRadius);
 void orc_calc_wall_forces(orc_state* s);
+/* observables */
+void orc_calc_contacts(orc_state* s);
+long orc_contact_list_size(const orc_state* s);
+void orc_get_contacts(const orc_state* s, long* list, long* counts);
+double orc_get_hexatic_order(orc_state* s);
+void orc_set_initial_positions(orc_state* s);
+double orc_get_vertex_msd(orc_state* s);
+double orc_get_particle_msd(orc_state* s);
+double orc_get_vertex_isf(orc_state* s);
+double orc_get_particle_isf(orc_state* s, double waveNumber);
+void orc_calc_per_particle_stress(orc_state* s, double* out4np);
+void orc_calc_stress_tensor(orc_state* s, double* out4);
+double orc_get_pressure(orc_state* s);
 void orc_init_langevin(orc_state* s, double T, double gamma, int readState, int zeroMomentum);
 void orc_set_active_tension(orc_state* s, double l0Tau, double l0Diff);
 void orc_update_rest_length(orc_state* s);

@@ -78,6 +78,18 @@ def lib():
             "orc_get_cut_distance": (d, [vp]),
             "orc_init_langevin": (None, [vp, d, d, i, i]),
             "orc_set_noise_type": (None, [vp, i]),
+            "orc_calc_contacts": (None, [vp]),
+            "orc_contact_list_size": (l, [vp]),
+            "orc_get_contacts": (None, [vp, vp, vp]),
+            "orc_get_hexatic_order": (d, [vp]),
+            "orc_set_initial_positions": (None, [vp]),
+            "orc_get_vertex_msd": (d, [vp]),
+            "orc_get_particle_msd": (d, [vp]),
+            "orc_get_vertex_isf": (d, [vp]),
+            "orc_get_particle_isf": (d, [vp, d]),
+            "orc_calc_per_particle_stress": (None, [vp, vp]),
+            "orc_calc_stress_tensor": (None, [vp, vp]),
+            "orc_get_pressure": (d, [vp]),
             "orc_set_geometry": (None, [vp, i]),
             "orc_set_wall_energy_scale": (None, [vp, d]),
             "orc_set_box_radius": (None, [vp, d]),
@@ -236,6 +248,47 @@ class Oracle:
             self.L.orc_set_wall_energy_scale(self.h, ew)
         if box_radius is not None:
             self.L.orc_set_box_radius(self.h, box_radius)
+
+    # ---- observables
+    def contacts(self):
+        """(list [np, width] -1 padded, counts [np]) after calcContacts."""
+        self.L.orc_calc_contacts(self.h)
+        w = self.L.orc_contact_list_size(self.h)
+        lst = np.empty(self.np * w, dtype=np.int64)
+        cnt = np.empty(self.np, dtype=np.int64)
+        self.L.orc_get_contacts(self.h, _ptr(lst), _ptr(cnt))
+        return lst.reshape(self.np, w), cnt
+
+    def hexatic_order(self):
+        return self.L.orc_get_hexatic_order(self.h)
+
+    def set_initial_positions(self):
+        self.L.orc_set_initial_positions(self.h)
+
+    def vertex_msd(self):
+        return self.L.orc_get_vertex_msd(self.h)
+
+    def particle_msd(self):
+        return self.L.orc_get_particle_msd(self.h)
+
+    def vertex_isf(self):
+        return self.L.orc_get_vertex_isf(self.h)
+
+    def particle_isf(self, k):
+        return self.L.orc_get_particle_isf(self.h, k)
+
+    def per_particle_stress(self):
+        a = np.empty(4 * self.np)
+        self.L.orc_calc_per_particle_stress(self.h, _ptr(a))
+        return a.reshape(self.np, 4)
+
+    def stress_tensor(self):
+        a = np.empty(4)
+        self.L.orc_calc_stress_tensor(self.h, _ptr(a))
+        return a
+
+    def pressure(self):
+        return self.L.orc_get_pressure(self.h)
 
     def set_noise_type(self, noise):
         self.L.orc_set_noise_type(self.h, NOISE[noise])
