@@ -153,7 +153,7 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_adj_hvp(View V, Params P, Ce
       A.cellOther[c] = 0;
   if (A.flag) {
     const bool rebuild = A.flag[L.s] != 0;
-    if (rebuild && w == 0 && L.act) build_row(V, P, G, A.count, L.v, L.s, B, A.err, A.maxcnt);
+    if (rebuild && w == 0 && L.act) build_row(V, P, G, A.count, G.slots, L.v, L.s, B, A.err, A.maxcnt);
     if (rebuild) __syncthreads();
   }
 

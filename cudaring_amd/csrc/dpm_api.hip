@@ -113,6 +113,27 @@ int upload_box(dpm_set_s* S) {
   return DPM_OK;
 }
 
+// the step loop's three cell lists (counts + slots at the current capacity) and four flag rows
+int alloc_md_cells(dpm_set_s* S) {
+  HIPCHK(hipStreamSynchronize(S->stream));
+  const long C = S->G.ncells;
+  for (int i = 0; i < 3; i++) {
+    hipFree(S->mdCount[i]);
+    hipFree(S->mdSlots[i]);
+    S->mdCount[i] = S->mdSlots[i] = nullptr;
+    HIPCHK(dalloc(&S->mdCount[i], C > 0 ? C : 1));
+    HIPCHK(dalloc(&S->mdSlots[i], (C > 0 ? C : 1) * S->G.cap));
+    HIPCHK(launch_fill_int(S->mdCount[i], 0, C, S->stream));
+  }
+  for (int i = 0; i < 4; i++)
+    if (!S->mdFlag[i]) {
+      HIPCHK(dalloc(&S->mdFlag[i], S->nsys));
+      HIPCHK(launch_fill_int(S->mdFlag[i], 0, S->nsys, S->stream));
+    }
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
 // (Re)configure the per-system cell grids for the current cutDistance, boxes and radii.
 int configure_cells(dpm_set_s* S) {
   const long B = S->nsys;
@@ -120,7 +141,9 @@ int configure_cells(dpm_set_s* S) {
   long C = 0;
   double maxMean = 1;
   for (long s = 0; s < B; s++) {
-    const double rc = (1 + S->P.cutDistance) * 2 * S->h_maxrad[s];
+    // list radius, plus the cutoff / 3 a vertex may move between the step loop's binning and
+    // its rebuild
+    const double rc = (1 + S->P.cutDistance) * 2 * S->h_maxrad[s] + S->P.cutoff / 3;
     const double Lx = S->h_box[2 * s], Ly = S->h_box[2 * s + 1];
     int nx = rc > 0 ? (int)floor(Lx / rc) : 0;
     int ny = rc > 0 ? (int)floor(Ly / rc) : 0;
@@ -161,6 +184,8 @@ int configure_cells(dpm_set_s* S) {
   HIPCHK(hipStreamSynchronize(S->stream));
   S->G.ncells = (int)C;
   S->G.cap = cap;
+  int rc = alloc_md_cells(S);
+  if (rc) return rc;
   bump(S);
   return DPM_OK;
 }
@@ -212,6 +237,8 @@ int rebuild(dpm_set_s* S, const std::vector<int>& flags, int countUpdates) {
       hipFree(S->G.slots);
       S->G.slots = nullptr;
       HIPCHK(dalloc(&S->G.slots, (long)S->G.ncells * S->G.cap));
+      int rc = alloc_md_cells(S);
+      if (rc) return rc;
       bump(S);
       continue;
     }
@@ -233,6 +260,9 @@ int check_err_after_loop(dpm_set_s* S) {
   int err = 0;
   HIPCHK(hipMemcpyAsync(&err, S->d_err, sizeof(int), hipMemcpyDeviceToHost, S->stream));
   HIPCHK(hipStreamSynchronize(S->stream));
+  if (err & ERR_STALE_BINS)
+    return fail(S, DPM_ERR_STATE, "a vertex moved more than cutoff / 3 in one step: the neighbour-list skin "
+                                  "does not cover the time step");
   if (err) return fail(S, DPM_ERR_OVERFLOW, err & ERR_LIST_OVERFLOW ? "neighbour list overflow during loop"
                                                                     : "cell overflow during loop");
   return DPM_OK;
@@ -276,67 +306,112 @@ int loop_step(dpm_set_s* S, int mode, int bumpT, double* recx = nullptr, double*
   return DPM_OK;
 }
 
+// Arguments of the step-loop kernel of rotation index r: it reads the rebuild flags F[r] of its
+// positions, rebuilds flagged rows from cells C[r - 1] (the previous positions, one step stale:
+// the cells are widened by the cutoff / 3 the displacement test allows), bins its own positions
+// into C[r], writes the flags F[r + 1] of its drift destination and clears C[r + 1] and F[r + 2]
+// for the kernels after it.  Its draws use the call indices ctr[k] + offset.  The loop needs no
+// binning launch.
+StepArgs md_args(dpm_set_s* S, long r, long long offT, long long offA) {
+  StepArgs A = loop_args(S);
+  const int a = (int)(r % 3), b = (int)((r + 1) % 3), c = (int)((r + 2) % 3);
+  A.flagCur = S->mdFlag[a];
+  A.cellCur = S->mdCount[c];  // r - 1
+  A.slotsCur = S->mdSlots[c];
+  A.cellOut = S->mdCount[a];
+  A.slotsOut = S->mdSlots[a];
+  A.cellOther = S->mdCount[b];
+  A.flagOut = S->mdFlag[b];
+  A.flagClear = S->mdFlag[c];
+  A.staleCheck = 1;
+  A.anyOut = S->d_any + 2;  // not read by the loop
+  A.ctrOffT = offT;
+  A.ctrOffA = offA;
+  return A;
+}
+
+int md_clear_slot(dpm_set_s* S, int r) {
+  HIPCHK(launch_fill_int(S->mdCount[r], 0, S->G.ncells, S->stream));
+  HIPCHK(launch_fill_int(S->mdFlag[r], 0, S->nsys, S->stream));
+  return DPM_OK;
+}
+
+long long active_draws(const dpm_set_s* S, long steps) { return S->P.particle != PART_PASSIVE ? steps : 0; }
+
 // Launch sequence of a velocity-Verlet loop (NVE / Langevin):
-//   start: v += dt/2 F, x' = x + dt v, displacement test          (buffer b -> 1-b)
-//   per step: bin, step [rebuild rows if flagged; F(x'); thermostat; v += dt/2 F;
-//             unless last: v += dt/2 F, x'' = x' + dt v, displacement test]
+//   start: v += dt/2 F, x' = x + dt v, bin + displacement test of x'  (buffer b -> 1-b)
+//   per step: step [rebuild rows if flagged; F(x'); thermostat; v += dt/2 F;
+//             unless last: v += dt/2 F, x'' = x' + dt v, bin + displacement test of x'']
+//   end: RNG call counters advanced by the loop's draws.
 // The fused form performs the reference's operations in the reference's order.
 // With `rec` (slots of 2 nv doubles, x then y), the position after step i is also written to
 // slot i + 1 (the adjoint's recomputed segment).
 int verlet_sequence(dpm_set_s* S, long steps, bool langevin, double* rec = nullptr) {
   const double h = 0.5 * S->P.dt;
   const long nv = S->nv;
-  HIPCHK(launch_fill_int(S->d_flag[1 - S->cur], 0, S->nsys, S->stream));
-  HIPCHK(launch_fill_int(S->d_any + 1 - S->cur, 0, 1, S->stream));
-  HIPCHK(launch_start(S, h, S->P.dt, rec ? rec + 2 * nv : nullptr, rec ? rec + 3 * nv : nullptr));
+  HIPCHK(launch_fill_int(S->mdCount[0], 0, S->G.ncells, S->stream));
+  HIPCHK(launch_fill_int(S->mdFlag[1], 0, S->nsys, S->stream));
+  StepArgs A0 = md_args(S, 0, 0, 0);
+  A0.recx = rec ? rec + 2 * nv : nullptr;
+  A0.recy = rec ? rec + 3 * nv : nullptr;
+  HIPCHK(launch_start_md(S, h, S->P.dt, A0));
   if (rec) {
     // a recomputed segment starts from a restored checkpoint without a valid list: every
     // system rebuilds its rows in the first step
-    HIPCHK(launch_fill_int(S->d_flag[1 - S->cur], 1, S->nsys, S->stream));
-    HIPCHK(launch_fill_int(S->d_any + 1 - S->cur, 1, 1, S->stream));
+    HIPCHK(launch_fill_int(S->mdFlag[1], 1, S->nsys, S->stream));
   }
   swap_pos(S);
-  for (long i = 0; i < steps; i++) {
-    const bool more = i + 1 < steps;
+  for (long i = 1; i <= steps; i++) {
+    const bool more = i < steps;
     const int mode = MODE_KICK | (langevin ? MODE_LANGEVIN : 0) | (more ? MODE_START : 0);
-    double* rx = (rec && more) ? rec + (i + 2) * 2 * nv : nullptr;
-    int rc = loop_step(S, mode, langevin ? 1 : 0, rx, rx ? rx + nv : nullptr);
-    if (rc) return rc;
+    StepArgs A = md_args(S, i, i - 1, i - 1);
+    if (rec && more) {
+      A.recx = rec + (i + 1) * 2 * nv;
+      A.recy = A.recx + nv;
+    }
+    HIPCHK(launch_step(S, mode, A));
     if (more) swap_pos(S);
   }
+  HIPCHK(launch_ctr_add(S, langevin ? steps : 0, active_draws(S, steps)));
   return DPM_OK;
 }
 
 // BAOAB loop (BAOAB::integrate, Simulator.cu:112-120): the first call's B A O A from the stored
-// force, then per step bin + step [F(x'); v += dt/2 F; unless last: B A O A].  The O step of
-// call k draws at counter k (k_start_baoab reads the counter before the first bump).
+// force, then per step [F(x'); v += dt/2 F; unless last: B A O A].  The O step of call k of the
+// loop draws at counter ctr[0] + k.
 int baoab_sequence(dpm_set_s* S, long steps) {
   const double h = 0.5 * S->P.dt;
-  HIPCHK(launch_fill_int(S->d_flag[1 - S->cur], 0, S->nsys, S->stream));
-  HIPCHK(launch_fill_int(S->d_any + 1 - S->cur, 0, 1, S->stream));
-  HIPCHK(launch_start_baoab(S, h));
+  HIPCHK(launch_fill_int(S->mdCount[0], 0, S->G.ncells, S->stream));
+  HIPCHK(launch_fill_int(S->mdFlag[1], 0, S->nsys, S->stream));
+  HIPCHK(launch_start_baoab(S, h, md_args(S, 0, 0, 0)));
   swap_pos(S);
-  for (long i = 0; i < steps; i++) {
-    const bool more = i + 1 < steps;
-    int rc = loop_step(S, MODE_KICK | MODE_BAOAB | (more ? MODE_START : 0), 1);
-    if (rc) return rc;
+  for (long i = 1; i <= steps; i++) {
+    const bool more = i < steps;
+    HIPCHK(launch_step(S, MODE_KICK | MODE_BAOAB | (more ? MODE_START : 0), md_args(S, i, i, i - 1)));
     if (more) swap_pos(S);
   }
+  HIPCHK(launch_ctr_add(S, steps, active_draws(S, steps)));
   return DPM_OK;
 }
 
-// Overdamped loop (Brownian::integrate 147-152, DrivenBrownian::integrate 182-187): the list
-// validity of the current positions first, then per step bin + step [rebuild if flagged;
-// F(x); v = (F + noise xi) / gamma or F / gamma; x' = x + dt v; displacement test].
+// Overdamped loop (Brownian::integrate 147-152, DrivenBrownian::integrate 182-187): the cells
+// and list validity of the current positions first, then per step [rebuild if flagged; F(x);
+// v = (F + noise xi) / gamma or F / gamma; x' = x + dt v; bin + displacement test of x'].
 int brown_sequence(dpm_set_s* S, long steps, bool noisy) {
-  HIPCHK(launch_fill_int(S->d_flag[S->cur], 0, S->nsys, S->stream));
-  HIPCHK(launch_fill_int(S->d_any + S->cur, 0, 1, S->stream));
-  HIPCHK(launch_adj_flag(S, S->V, S->cur));
-  for (long i = 0; i < steps; i++) {
-    int rc = loop_step(S, MODE_BROWN | MODE_START | (noisy ? MODE_LANGEVIN : 0), noisy ? 1 : 0);
-    if (rc) return rc;
+  // step i has rotation index i - 1; step 1 rebuilds from exact cells of x_0 (C[2])
+  int rc = md_clear_slot(S, 2);
+  if (!rc) rc = md_clear_slot(S, 0);
+  if (rc) return rc;
+  HIPCHK(launch_fill_int(S->mdFlag[1], 0, S->nsys, S->stream));
+  HIPCHK(launch_bin_flag(S, S->mdCount[2], S->mdSlots[2], S->mdFlag[0]));
+  for (long i = 1; i <= steps; i++) {
+    const int mode = MODE_BROWN | MODE_START | (noisy ? MODE_LANGEVIN : 0);
+    StepArgs A = md_args(S, i - 1, i - 1, i - 1);
+    A.staleCheck = i > 1;
+    HIPCHK(launch_step(S, mode, A));
     swap_pos(S);
   }
+  HIPCHK(launch_ctr_add(S, noisy ? steps : 0, active_draws(S, steps)));
   return DPM_OK;
 }
 
@@ -736,7 +811,7 @@ int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPer
   A(V.tile_v0, V.ntiles); A(V.tile_v1, V.ntiles); A(V.tile_sys, V.ntiles);
   A(V.stile, numSystems + 1);
   A(S->G.ncx, numSystems); A(S->G.ncy, numSystems); A(S->G.coff, numSystems); A(S->G.brute, numSystems);
-  A(S->d_flag[0], numSystems); A(S->d_flag[1], numSystems); A(S->d_any, 2); A(S->d_updateCount, numSystems);
+  A(S->d_flag[0], numSystems); A(S->d_flag[1], numSystems); A(S->d_any, 3); A(S->d_updateCount, numSystems);
   A(S->d_err, 1); A(S->d_maxcnt, 1); A(S->d_ctr, 4); A(S->d_notDone, 1);
   A(S->d_fire[0], numSystems); A(S->d_fire[1], numSystems);
   A(S->d_part, (long)V.ntiles * 4);
@@ -813,7 +888,9 @@ int dpm_destroy(dpm_set_t h) {
                   S->d_flag[1], S->d_any, S->d_updateCount, S->d_err, S->d_maxcnt, S->d_ctr, S->d_notDone,
                   S->d_fire[0], S->d_fire[1], S->d_part, S->d_red, S->d_scale, V.kr,
                   S->d_ck, S->d_fin, S->d_seg, S->d_adj, V.vinfo, V.l0r, V.mbx, V.mby, V.wfx, V.wfy,
-                  S->d_contacts, S->d_ccount, S->d_x0, S->d_obs};
+                  S->d_contacts, S->d_ccount, S->d_x0, S->d_obs, S->mdCount[0], S->mdCount[1], S->mdCount[2],
+                  S->mdSlots[0], S->mdSlots[1], S->mdSlots[2], S->mdFlag[0], S->mdFlag[1], S->mdFlag[2],
+                  S->mdFlag[3]};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (S->ownStream) hipStreamDestroy(S->ownStream);
@@ -1169,6 +1246,7 @@ int dpm_calc_force_energy(dpm_set_t h) {
   HIPCHK(launch_bin(S, 0, S->P.particle != PART_PASSIVE ? 1 : 0));
   StepArgs A{};
   A.ctr = S->d_ctr;
+  A.ctrOffT = A.ctrOffA = -1;  // the bin launch has bumped the counters
   A.err = S->d_err;
   A.manage = 0;
   HIPCHK(launch_step(S, MODE_FORCE, A));
@@ -1199,6 +1277,7 @@ int dpm_get_interaction_forces(dpm_set_t h, double* host, long n) {
   HIPCHK(launch_bin(S, 0, 0));
   StepArgs A{};
   A.ctr = S->d_ctr;
+  A.ctrOffT = A.ctrOffA = -1;
   A.err = S->d_err;
   A.manage = 0;
   hipError_t e = launch_step(S, MODE_FORCE, A);
@@ -1709,17 +1788,25 @@ int dpm_time_step_kernel(dpm_set_t h, long reps, double* avgUs, double* meanNeig
   HIPCHK(dalloc(&svy, S->nv));
   HIPCHK(hipMemcpyAsync(svx, S->V.vx, sizeof(double) * S->nv, hipMemcpyDeviceToDevice, S->stream));
   HIPCHK(hipMemcpyAsync(svy, S->V.vy, sizeof(double) * S->nv, hipMemcpyDeviceToDevice, S->stream));
-  // the Langevin step kernel as launched inside loops, with the current parity's rebuild flags
-  // clear (no rebuild); the drift writes into the other position buffer only
-  HIPCHK(launch_fill_int(S->d_flag[S->cur], 0, S->nsys, S->stream));
-  StepArgs A = loop_args(S);
+  // the Langevin step kernel as launched inside loops (binning its drift destination through
+  // the rotating cell lists), with rebuild flags that stay clear: no rebuild, and the drift
+  // writes into the other position buffer only
+  HIPCHK(launch_fill_int(S->mdFlag[3], 0, S->nsys, S->stream));
+  for (int r = 0; r < 3; r++) HIPCHK(launch_fill_int(S->mdCount[r], 0, S->G.ncells, S->stream));
+  auto args = [&](long i) {
+    StepArgs A = md_args(S, i, 0, 0);
+    A.flagCur = S->mdFlag[3];
+    A.flagOut = S->d_flag[1 - S->cur];
+    A.flagClear = nullptr;
+    return A;
+  };
   const int mode = MODE_KICK | MODE_LANGEVIN | MODE_START;
-  HIPCHK(launch_step(S, mode, A));  // warm caches and code
+  HIPCHK(launch_step(S, mode, args(0)));  // warm caches and code
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, S->stream));
-  for (long i = 0; i < reps; i++) HIPCHK(launch_step(S, mode, A));
+  for (long i = 0; i < reps; i++) HIPCHK(launch_step(S, mode, args(i + 1)));
   HIPCHK(hipEventRecord(e1, S->stream));
   HIPCHK(hipEventSynchronize(e1));
   float ms = 0;

@@ -23,7 +23,7 @@ enum : int { MODE_FORCE = 0, MODE_KICK = 1, MODE_LANGEVIN = 2, MODE_START = 4, M
 enum : int { NOISE_LANGEVIN = 0, NOISE_BAOAB = 1, NOISE_BROWNIAN = 2, NOISE_DRIVEN_BROWNIAN = 3 };
 
 // error bits written by kernels into Set::d_err
-enum : int { ERR_LIST_OVERFLOW = 1, ERR_CELL_OVERFLOW = 2 };
+enum : int { ERR_LIST_OVERFLOW = 1, ERR_CELL_OVERFLOW = 2, ERR_STALE_BINS = 4 };
 
 // per-system FIRE scalars (double buffered by iteration parity)
 struct FireState {
@@ -62,9 +62,15 @@ struct StepArgs {
   int* flagCur;                   // rebuild flags of this step's positions
   int* flagOut;                   // rebuild flags of the drift destination
   int* anyOut;                    // any-flag of the drift destination
-  int* cellCur;                   // cell counts binned from this step's positions
-  int* cellOther;                 // cell counts of the other parity, cleared here
-  const unsigned long long* ctr;  // [0] thermal calls, [1] active calls (already bumped)
+  int* cellCur;                   // cells the rows are rebuilt from (this step's positions, or
+  int* slotsCur;                  //   in the step loop the previous step's) and their slots
+  int* cellOther;                 // cell counts cleared here (the next binning's target)
+  int* flagClear;                 // rebuild flags cleared here, or null
+  int* cellOut;                   // step loop: this step's positions are binned into these, or null
+  int* slotsOut;
+  int staleCheck;                 // rows from the previous step's cells: check its displacement
+  const unsigned long long* ctr;  // [0] thermal calls, [1] active calls
+  long long ctrOffT, ctrOffA;     // call index of this step's draws = ctr[k] + offset
   FireState* fire;                // MODE_FIRE: per-system state of this iteration
   double* part;                   // MODE_FIRE: per-tile partial sums [ntiles * 4]
   int* err;
@@ -127,6 +133,12 @@ struct dpm_set_s {
   // neighbour machinery
   dpm::CellGrid G{};
   int* d_flag[2] = {nullptr, nullptr};  // per-system rebuild flags, by position parity
+  // Step-loop cells and flags, rotated over three buffers so that a step kernel rebuilds from
+  // one, bins its drift destination into the next and clears the one after (no separate
+  // binning launch); mdFlag[3] is a sink for launches whose flags are not wanted.
+  int* mdCount[3] = {nullptr, nullptr, nullptr};
+  int* mdSlots[3] = {nullptr, nullptr, nullptr};
+  int* mdFlag[4] = {nullptr, nullptr, nullptr, nullptr};
   int* d_any = nullptr;                 // [2] any-flag by parity
   int* d_updateCount = nullptr;
   int* d_err = nullptr;
@@ -186,8 +198,10 @@ namespace dpm {
 // launchers (dpm_kernels.hip); all return hipError_t of the last launch
 hipError_t launch_shape_only(dpm_set_s* S);
 hipError_t launch_step(dpm_set_s* S, int mode, const StepArgs& A);
-hipError_t launch_start(dpm_set_s* S, double h, double dt, double* recx = nullptr, double* recy = nullptr);
-hipError_t launch_start_baoab(dpm_set_s* S, double h);
+hipError_t launch_start_baoab(dpm_set_s* S, double h, const StepArgs& A);
+hipError_t launch_start_md(dpm_set_s* S, double h, double dt, const StepArgs& A);
+hipError_t launch_bin_flag(dpm_set_s* S, int* count, int* slots, int* flag);
+hipError_t launch_ctr_add(dpm_set_s* S, long long addT, long long addA);
 hipError_t launch_bin(dpm_set_s* S, int bumpThermal, int bumpActive);
 hipError_t launch_build_rows(dpm_set_s* S, int countUpdates);
 hipError_t launch_fill_int(int* p, int value, long n, hipStream_t st);

@@ -162,7 +162,7 @@ __global__ void k_build_rows(View V, Params P, CellGrid G, const int* count, con
   if (!flagCur[L.s]) return;
   if (countUpdates && L.tile == V.stile[L.s] && L.lane == 0) updateCount[L.s] += 1;
   if (!L.act) return;
-  build_row(V, P, G, count, L.v, L.s, box_of(V, L.s), err, maxcnt);
+  build_row(V, P, G, count, G.slots, L.v, L.s, box_of(V, L.s), err, maxcnt);
 }
 
 // Interparticle gather of lane L's vertex over entries k0, k0 + kstep, ... of its row:
@@ -248,11 +248,25 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
 
   // loop bookkeeping: clear the other parity's cells, count this step's rebuild, rebuild rows
   if (A.manage) {
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < G.ncells; c += gridDim.x * blockDim.x)
-      A.cellOther[c] = 0;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int c = gid; c < G.ncells; c += gridDim.x * blockDim.x) A.cellOther[c] = 0;
+    if (A.flagClear)
+      for (int t = gid; t < V.nsys; t += gridDim.x * blockDim.x) A.flagClear[t] = 0;
+    // the step loop bins these positions for the next step's rebuild (off the critical path:
+    // wave 0 waits for the gather waves anyway)
+    if (A.cellOut && w == 0 && L.act) bin_vertex(G, B, A.cellOut, A.slotsOut, L.v, L.s, x, y, A.err);
     const bool rebuild = A.flagCur[L.s] != 0;
     if (rebuild && L.tile == V.stile[L.s] && threadIdx.x == 0) A.updateCount[L.s] += 1;
-    if (rebuild && w == 0 && L.act) build_row(V, P, G, A.cellCur, L.v, L.s, B, A.err, A.maxcnt);
+    if (rebuild && w == 0 && L.act) {
+      // rows from the previous positions' cells: complete while no vertex moved more than
+      // cutoff / 3 since them, the margin the cells are widened by (the previous position is
+      // this lane's own, in the buffer the drift below overwrites)
+      if (A.staleCheck) {
+        const double ddx = pbx(x - A.xout[L.v], B), ddy = pby(y - A.yout[L.v], B);
+        if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) atomicOr(A.err, ERR_STALE_BINS);
+      }
+      build_row(V, P, G, A.cellCur, A.slotsCur, L.v, L.s, B, A.err, A.maxcnt);
+    }
     if (rebuild) __syncthreads();
   }
   // wave 0's epilogue operands, fetched before the force work (lastPos after the rebuild above)
@@ -291,10 +305,10 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
     // one thermal draw per integrate call and ring: Langevin::updateThermalVel (Simulator.cu:54-77)
     // and Brownian::updateThermalVel (156-179) after this call's force, counter = call index;
     // BAOAB::updateThermalVel (122-144) in the next call's first half, whose index is one more
-    if constexpr ((MODE & MODE_LANGEVIN) != 0)
-      gauss_pair(P.seed, (uint32_t)L.s, TAG_THERMAL, A.ctr[0] - 1, (uint32_t)(L.r - V.sroff[L.s]), zt0, zt1);
-    if constexpr ((MODE & MODE_BAOAB) != 0 && (MODE & MODE_START) != 0)
-      gauss_pair(P.seed, (uint32_t)L.s, TAG_THERMAL, A.ctr[0], (uint32_t)(L.r - V.sroff[L.s]), zt0, zt1);
+    const unsigned long long callT = A.ctr[0] + (unsigned long long)A.ctrOffT;
+    const unsigned long long callA = A.ctr[1] + (unsigned long long)A.ctrOffA;
+    if constexpr ((MODE & MODE_LANGEVIN) != 0 || ((MODE & MODE_BAOAB) != 0 && (MODE & MODE_START) != 0))
+      gauss_pair(P.seed, (uint32_t)L.s, TAG_THERMAL, callT, (uint32_t)(L.r - V.sroff[L.s]), zt0, zt1);
     if (P.geometry != GEOM_NORMAL) {  // walls, after the interactions (DPM2D.cu:2090-2103)
       double wfx, wfy;
       so.e += wall_forces(P, B, x, y, V.rad[L.v], so.fx, so.fy, wfx, wfy);
@@ -305,14 +319,14 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
     }
     if (P.particle == PART_ACTIVE_SHAPE && L.act) {  // updateRestLength, DPM2D.cu:2037-2056
       double z0, z1;
-      gauss_pair(P.seed, (uint32_t)L.s, TAG_SHAPE, A.ctr[1] - 1, (uint32_t)(L.r - V.sroff[L.s]), z0, z1);
+      gauss_pair(P.seed, (uint32_t)L.s, TAG_SHAPE, callA, (uint32_t)(L.r - V.sroff[L.s]), z0, z1);
       const double l0v = V.l0[L.v];
       V.l0[L.v] = l0v + ((l0v - V.l0r[L.v]) / P.l0Tau + sqrt(2. * P.l0Diff / P.dt) * z0) * P.dt;
     }
     if (P.particle == PART_ACTIVE) {  // addSelfPropulsion, DPM2D.cu:1995-2035
       const double amplitude = (P.taup != 0) ? sqrt(2.0 * P.dt / P.taup) : 0.0;
       double z0, z1;
-      gauss_pair(P.seed, (uint32_t)L.s, TAG_ACTIVE, A.ctr[1] - 1, (uint32_t)(L.r - V.sroff[L.s]), z0, z1);
+      gauss_pair(P.seed, (uint32_t)L.s, TAG_ACTIVE, callA, (uint32_t)(L.r - V.sroff[L.s]), z0, z1);
       ang = V.ang[L.r];
       ang += amplitude * z0;
       ang = ang + kPI;
@@ -440,52 +454,69 @@ __global__ void k_shape(View V, Params P) {
   ring_shape(V, P, L, box_of(V, L.s), V.x[L.v], V.y[L.v], true);
 }
 
-// first half step of a velocity-Verlet call: v += h F; x' = x + dt v; displacement test
-__global__ void k_start(View V, Params P, double h, double dt, double* xo, double* yo, int* flagOut, int* anyOut,
-                        double* rx, double* ry) {
+// The step loop's first half step (velocity Verlet, or BAOAB's B A O A with the draw of call
+// ctr[0] + ctrOffT) from the stored force, with the loop's cell protocol: the start positions
+// are binned into A.cellOut / A.slotsOut (the first step rebuilds from them), the destination
+// is flagged into A.flagOut, and A.cellOther / A.flagClear are cleared (BAOAB::integrate,
+// Simulator.cu:112-120, up to its force evaluation).
+template <bool BAOAB>
+__global__ void k_start_loop(View V, Params P, CellGrid G, double h, double dt, StepArgs A) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int c = v; c < G.ncells; c += gridDim.x * blockDim.x) A.cellOther[c] = 0;
+  for (int t = v; t < V.nsys; t += gridDim.x * blockDim.x) A.flagClear[t] = 0;
+  if (v >= V.nv) return;
+  const int s = V.vsys[v];
+  const Box B = box_of(V, s);
+  double vx = V.vx[v] + h * V.fx[v];
+  double vy = V.vy[v] + h * V.fy[v];
+  double xn, yn;
+  if constexpr (BAOAB) {
+    double z0, z1;
+    gauss_pair(P.seed, (uint32_t)s, TAG_THERMAL, A.ctr[0] + (unsigned long long)A.ctrOffT,
+               (uint32_t)(V.vring[v] - V.sroff[s]), z0, z1);
+    const double xa = V.x[v] + vx * h, ya = V.y[v] + vy * h;
+    vx = P.gamma * vx + P.noise * z0;
+    vy = P.gamma * vy + P.noise * z1;
+    xn = xa + vx * h;
+    yn = ya + vy * h;
+  } else {
+    xn = V.x[v] + dt * vx;
+    yn = V.y[v] + dt * vy;
+  }
+  V.vx[v] = vx;
+  V.vy[v] = vy;
+  A.xout[v] = xn;
+  A.yout[v] = yn;
+  if (A.recx) {
+    A.recx[v] = xn;
+    A.recy[v] = yn;
+  }
+  bin_vertex(G, B, A.cellOut, A.slotsOut, v, s, V.x[v], V.y[v], A.err);  // the start positions' cells
+  const double ddx = pbx(xn - V.lx[v], B), ddy = pby(yn - V.ly[v], B);
+  if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) {
+    atomicOr(&A.flagOut[s], 1);
+    atomicOr(A.anyOut, 1);
+  }
+}
+
+// the current positions into a cell list with their displacement test (the overdamped loops'
+// prologue: their first step rebuilds from these when flagged)
+__global__ void k_bin_flag(View V, Params P, CellGrid G, int* count, int* slots, int* flag, int* err) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= V.nv) return;
   const int s = V.vsys[v];
   const Box B = box_of(V, s);
-  const double vx = V.vx[v] + h * V.fx[v];
-  const double vy = V.vy[v] + h * V.fy[v];
-  const double xn = V.x[v] + dt * vx, yn = V.y[v] + dt * vy;
-  V.vx[v] = vx; V.vy[v] = vy;
-  xo[v] = xn; yo[v] = yn;
-  if (rx) {
-    rx[v] = xn;
-    ry[v] = yn;
-  }
-  const double ddx = pbx(xn - V.lx[v], B), ddy = pby(yn - V.ly[v], B);
-  if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) {
-    atomicOr(&flagOut[s], 1);
-    atomicOr(anyOut, 1);
-  }
+  const double x = V.x[v], y = V.y[v];
+  bin_vertex(G, B, count, slots, v, s, x, y, err);
+  const double ddx = pbx(x - V.lx[v], B), ddy = pby(y - V.ly[v], B);
+  if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) atomicOr(&flag[s], 1);
 }
 
-// first half of a BAOAB call with the stored force: B (v += h F), A (x += h v), O (v = gamma v +
-// noise xi_ring, the call's draw at counter ctr[0]), A (x += h v); displacement test
-// (BAOAB::integrate, Simulator.cu:112-120, up to its force evaluation)
-__global__ void k_start_baoab(View V, Params P, double h, const unsigned long long* ctr, double* xo, double* yo,
-                              int* flagOut, int* anyOut) {
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= V.nv) return;
-  const int s = V.vsys[v], r = V.vring[v];
-  const Box B = box_of(V, s);
-  double z0, z1;
-  gauss_pair(P.seed, (uint32_t)s, TAG_THERMAL, ctr[0], (uint32_t)(r - V.sroff[s]), z0, z1);
-  double vx = V.vx[v] + h * V.fx[v];
-  double vy = V.vy[v] + h * V.fy[v];
-  const double xa = V.x[v] + vx * h, ya = V.y[v] + vy * h;
-  vx = P.gamma * vx + P.noise * z0;
-  vy = P.gamma * vy + P.noise * z1;
-  const double xn = xa + vx * h, yn = ya + vy * h;
-  V.vx[v] = vx; V.vy[v] = vy;
-  xo[v] = xn; yo[v] = yn;
-  const double ddx = pbx(xn - V.lx[v], B), ddy = pby(yn - V.ly[v], B);
-  if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) {
-    atomicOr(&flagOut[s], 1);
-    atomicOr(anyOut, 1);
+// RNG call counters advanced by the draws of a finished loop
+__global__ void k_ctr_add(unsigned long long* ctr, long long addT, long long addA) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctr[0] += (unsigned long long)addT;
+    ctr[1] += (unsigned long long)addA;
   }
 }
 
@@ -885,7 +916,7 @@ __global__ __launch_bounds__(WAVE* FS_WAVES) void k_fire_sys(View V, Params P, F
       __syncthreads();
       for (int t = t0 + w; t < t1; t += nw) {
         const int v = V.tile_v0[t] + lane;
-        if (v < V.tile_v1[t]) build_row(V, P, G, count, v, s, B, err, maxcnt);
+        if (v < V.tile_v1[t]) build_row(V, P, G, count, G.slots, v, s, B, err, maxcnt);
       }
       __syncthreads();
       if (!G.brute[s])
@@ -1015,17 +1046,25 @@ hipError_t launch_shape_only(dpm_set_s* S) {
   return hipGetLastError();
 }
 
-hipError_t launch_start(dpm_set_s* S, double h, double dt, double* recx, double* recy) {
-  const int nb = 1 - S->cur;
-  k_start<<<grid_v(S->nv), 256, 0, S->stream>>>(S->V, S->P, h, dt, S->pos[nb][0], S->pos[nb][1],
-                                                S->d_flag[nb], S->d_any + nb, recx, recy);
+hipError_t launch_start_baoab(dpm_set_s* S, double h, const StepArgs& A) {
+  k_start_loop<true><<<grid_v(S->nv > S->G.ncells ? S->nv : S->G.ncells), 256, 0, S->stream>>>(S->V, S->P, S->G, h,
+                                                                                           S->P.dt, A);
   return hipGetLastError();
 }
 
-hipError_t launch_start_baoab(dpm_set_s* S, double h) {
-  const int b = 1 - S->cur;
-  k_start_baoab<<<grid_v(S->nv), 256, 0, S->stream>>>(S->V, S->P, h, S->d_ctr, S->pos[b][0], S->pos[b][1],
-                                                      S->d_flag[b], S->d_any + b);
+hipError_t launch_start_md(dpm_set_s* S, double h, double dt, const StepArgs& A) {
+  k_start_loop<false><<<grid_v(S->nv > S->G.ncells ? S->nv : S->G.ncells), 256, 0, S->stream>>>(S->V, S->P, S->G, h,
+                                                                                            dt, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_bin_flag(dpm_set_s* S, int* count, int* slots, int* flag) {
+  k_bin_flag<<<grid_v(S->nv), 256, 0, S->stream>>>(S->V, S->P, S->G, count, slots, flag, S->d_err);
+  return hipGetLastError();
+}
+
+hipError_t launch_ctr_add(dpm_set_s* S, long long addT, long long addA) {
+  k_ctr_add<<<1, 64, 0, S->stream>>>(S->d_ctr, addT, addA);
   return hipGetLastError();
 }
 
@@ -1153,8 +1192,11 @@ StepArgs loop_args(dpm_set_s* S) {
   A.flagOut = S->d_flag[1 - b];
   A.anyOut = S->d_any + 1 - b;
   A.cellCur = S->G.count[b];
+  A.slotsCur = S->G.slots;
   A.cellOther = S->G.count[1 - b];
   A.ctr = S->d_ctr;
+  A.ctrOffT = -1;  // k_bin has bumped the counters of this step
+  A.ctrOffA = -1;
   A.err = S->d_err;
   A.maxcnt = S->d_maxcnt;
   A.updateCount = S->d_updateCount;

@@ -46,8 +46,8 @@ __device__ __forceinline__ void row_insert(const View& V, int v, int c, int j, i
 // j in list(v) iff rings differ and -(1 - d/radSum) < cutDistance; rows stay ascending as in
 // the reference.  Candidates come from the 3x3 cell block (or the whole system when the box
 // is narrower than three cells).  Also resets lastPos (resetLastPositions).
-__device__ void build_row(const View& V, const Params& P, const CellGrid& G, const int* count, int v, int s,
-                          const Box& B, int* err, int* maxcnt) {
+__device__ void build_row(const View& V, const Params& P, const CellGrid& G, const int* count, const int* slots, int v,
+                          int s, const Box& B, int* err, int* maxcnt) {
   const double x = V.x[v], y = V.y[v], rv = V.rad[v];
   const int r = V.vring[v];
   const int W = V.W;
@@ -77,7 +77,7 @@ __device__ void build_row(const View& V, const Params& P, const CellGrid& G, con
         const int cell = G.coff[s] + yy * ncx + xx;
         const int m = min(count[cell], G.cap);
         for (int q = 0; q < m; q++) {
-          const int j = G.slots[(long)cell * G.cap + q];
+          const int j = slots[(long)cell * G.cap + q];
           if (V.vring[j] == r) continue;
           const double dx = pbx(x - V.x[j], B), dy = pby(y - V.y[j], B);
           const double d = sqrt(dx * dx + dy * dy);
@@ -94,6 +94,19 @@ __device__ void build_row(const View& V, const Params& P, const CellGrid& G, con
   V.ncnt[v] = c < W ? c : W;
   V.lx[v] = x;
   V.ly[v] = y;
+}
+
+// Bin vertex v at (x, y) of system s into a cell list (counts + slots of capacity G.cap);
+// systems searched by brute force have no cells.
+__device__ __forceinline__ void bin_vertex(const CellGrid& G, const Box& B, int* count, int* slots, int v, int s,
+                                           double x, double y, int* err) {
+  if (G.brute[s]) return;
+  int cx, cy;
+  cell_of(x, y, B, G.ncx[s], G.ncy[s], cx, cy);
+  const int c = G.coff[s] + cy * G.ncx[s] + cx;
+  const int slot = atomicAdd(&count[c], 1);
+  if (slot < G.cap) slots[(long)c * G.cap + slot] = v;
+  else atomicOr(err, ERR_CELL_OVERFLOW);
 }
 
 }  // namespace dpm
