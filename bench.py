@@ -32,8 +32,8 @@ DT_UNITS = 1e-3                # dt = 1e-3 * sigma / sqrt(ec), sigma = mean part
 CHUNK = 250                    # steps per graph replay
 # HBM bytes per launch of the step kernel on this workload, from the FETCH_SIZE and WRITE_SIZE
 # counter runs of `bench.py --traffic` (each scaled by its reading for a 1 GiB device copy:
-# 2.000 and 1.000): 2 x 3026.2 KiB + 1585.1 KiB.  Re-measure after changing the step kernel.
-TRAFFIC_BYTES_PER_LAUNCH = 7820500
+# 2.000 and 1.000): 2 x 3873.0 KiB + 2252.1 KiB.  Re-measure after changing the step kernel.
+TRAFFIC_BYTES_PER_LAUNCH = 10238114
 
 
 def alg_bytes_per_step(nv, np_, kbar):
