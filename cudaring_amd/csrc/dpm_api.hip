@@ -1778,6 +1778,42 @@ int dpm_scale_vertices(dpm_set_t h, const double* scale) {
   return DPM_OK;
 }
 
+// scaleBox, DPM2D.cu:1693-1719.  Ring centres are those of the current positions (recomputed
+// here; the reference holds those of its last force evaluation, the same after any loop or
+// force call).  The round box takes one radius for all systems, so its scales must agree.
+int dpm_scale_box(dpm_set_t h, const double* scale) {
+  CHECK_HANDLE(h);
+  if (!scale) return fail(S, DPM_ERR_ARG, "null scale");
+  for (long s = 0; s < S->nsys; s++)
+    if (!(scale[s] > 0)) return fail(S, DPM_ERR_ARG, "scale must be positive");
+  const bool round = S->P.geometry == GEOM_ROUND_BOX;
+  if (round)
+    for (long s = 1; s < S->nsys; s++)
+      if (scale[s] != scale[0]) return fail(S, DPM_ERR_ARG, "roundBox: one scale for all systems");
+  HIPCHK(launch_shape_only(S));
+  if (round) {
+    S->P.boxRadius /= scale[0];
+  } else {
+    for (long s = 0; s < S->nsys; s++) {
+      S->h_box[2 * s] /= scale[s];
+      S->h_box[2 * s + 1] /= scale[s];
+    }
+    std::vector<double> bx(S->nsys), by(S->nsys);
+    for (long s = 0; s < S->nsys; s++) { bx[s] = S->h_box[2 * s]; by[s] = S->h_box[2 * s + 1]; }
+    HIPCHK(up(S->V.bx, bx, S->stream));
+    HIPCHK(up(S->V.by, by, S->stream));
+    int rc = upload_box(S);
+    if (rc) return rc;
+  }
+  HIPCHK(hipMemcpyAsync(S->d_scale, scale, sizeof(double) * S->nsys, hipMemcpyHostToDevice, S->stream));
+  HIPCHK(launch_scale_box(S));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  S->paramSerial++;
+  bump(S);
+  if (S->cutSet && !round) return configure_cells(S);
+  return DPM_OK;
+}
+
 int dpm_time_step_kernel(dpm_set_t h, long reps, double* avgUs, double* meanNeighbors) {
   CHECK_HANDLE(h);
   int rc = require_list(S);

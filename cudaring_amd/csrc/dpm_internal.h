@@ -210,6 +210,7 @@ hipError_t launch_inject(dpm_set_s* S, double amplitude, unsigned long long call
 hipError_t launch_sub_mean(dpm_set_s* S);
 hipError_t launch_scale_vel(dpm_set_s* S);
 hipError_t launch_scale_vertices(dpm_set_s* S);
+hipError_t launch_scale_box(dpm_set_s* S);
 hipError_t launch_copy_lastpos(dpm_set_s* S);
 hipError_t launch_fire_begin(dpm_set_s* S);
 hipError_t launch_fire_mid(dpm_set_s* S);

@@ -629,6 +629,28 @@ __global__ void k_scale_vertices(View V, const double* scale) {
   }
 }
 
+// scaleBox, DPM2D.cu:1693-1719, with translateVertices (kernelTranslateVertices,
+// cudaKernel.cuh:1678-1693): every vertex moves by the shift of its ring's centre to
+// centre / scale, a box distance in the new box (uploaded before this launch); the centres
+// themselves are divided in k_scale_centres, launched after
+__global__ void k_scale_box(View V, const double* scale) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V.nv) return;
+  const int r = V.vring[v], s = V.vsys[v];
+  const Box B = box_of(V, s);
+  const double cx = V.cx[r], cy = V.cy[r], sc = scale[s];
+  V.x[v] += pbx(cx / sc - cx, B);
+  V.y[v] += pby(cy / sc - cy, B);
+}
+
+__global__ void k_scale_centres(View V, const double* scale) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= V.np) return;
+  const double sc = scale[V.rsys[r]];
+  V.cx[r] /= sc;
+  V.cy[r] /= sc;
+}
+
 // --------------------------------------------------------------- FIRE ---
 // sum of the per-tile partials q of system s, all 64 lanes striding over the tiles; the
 // xor-butterfly leaves the same value in every lane (addition is commutative)
@@ -1123,6 +1145,12 @@ hipError_t launch_scale_vel(dpm_set_s* S) {
 hipError_t launch_scale_vertices(dpm_set_s* S) {
   long n = S->nv > S->np ? S->nv : S->np;
   k_scale_vertices<<<grid_v(n), 256, 0, S->stream>>>(S->V, S->d_scale);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale_box(dpm_set_s* S) {
+  k_scale_box<<<grid_v(S->nv), 256, 0, S->stream>>>(S->V, S->d_scale);
+  k_scale_centres<<<grid_v(S->np), 256, 0, S->stream>>>(S->V, S->d_scale);
   return hipGetLastError();
 }
 

@@ -94,6 +94,7 @@ SIGNATURES = {
     "dpm_get_preferred_phi": (_i, [_vp, _vp]),
     "dpm_get_particle_shapes": (_i, [_vp, _vp]),
     "dpm_scale_vertices": (_i, [_vp, _vp]),
+    "dpm_scale_box": (_i, [_vp, _vp]),
     "dpm_force_hvp": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "dpm_energy_param_grad": (_i, [_vp, _vp, _vp]),
     "dpm_nve_forward": (_i, [_vp, _l, _l]),
@@ -421,6 +422,16 @@ class DPMSet:
     def scale_vertices(self, scale):
         s = np.ascontiguousarray(np.broadcast_to(np.asarray(scale, dtype=np.float64), (self.nsys,)))
         self._chk(self.L.dpm_scale_vertices(self.h, _p(s)), "scale_vertices")
+
+    def box_radius(self):
+        r = C.c_double()
+        self._chk(self.L.dpm_get_box_radius(self.h, C.byref(r)), "get_box_radius")
+        return r.value
+
+    def scale_box(self, scale):
+        """scaleBox (DPM2D.cu:1693-1719) per system: box (or round-box radius) / scale."""
+        s = np.ascontiguousarray(np.broadcast_to(np.asarray(scale, dtype=np.float64), (self.nsys,)))
+        self._chk(self.L.dpm_scale_box(self.h, _p(s)), "scale_box")
 
     def synchronize(self):
         self._chk(self.L.dpm_synchronize(self.h), "synchronize")

@@ -623,6 +623,23 @@ void DPM2D::scaleVertices(double scale) {
   check(dpm_scale_vertices(handle(), &scale), "scaleVertices");
 }
 
+// getDeformableWaveNumber, DPM2D.cu:1453-1462
+double DPM2D::getDeformableWaveNumber() {
+  if (simControl.geometryType == simControlStruct::geometryEnum::roundBox)
+    return PI / (2. * std::sqrt(boxRadius * boxRadius * getPreferredPhi() / numParticles));
+  return PI / (2. * std::sqrt(boxSize[0] * boxSize[1] * getPreferredPhi() / (PI * numParticles)));
+}
+
+// scaleBox, DPM2D.cu:1693-1719 (the library moves each ring with its centre)
+void DPM2D::scaleBox(double scale) {
+  check(dpm_scale_box(handle(), &scale), "scaleBox");
+  if (simControl.geometryType == simControlStruct::geometryEnum::roundBox) {
+    boxRadius /= scale;
+  } else {
+    for (auto& L : boxSize) L /= scale;
+  }
+}
+
 // ---------------------------------------------------------------- soft disks (set-up only)
 // Host restatement of the soft-disk path the compression driver runs before vertices exist:
 // kernelCalcParticleNeighborList (cudaKernel.cuh:1765-1787, list width grown as a power of two,

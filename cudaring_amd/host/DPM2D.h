@@ -138,6 +138,7 @@ class DPM2D {
   double getParticleMSD();
   double getVertexISF();
   double getParticleISF(double waveNumber);
+  double getDeformableWaveNumber();  // DPM2D.cu:1453-1462
   double getParticleHexaticOrder();
   double getPressure();
   void calcContacts();
@@ -178,6 +179,7 @@ class DPM2D {
   void setRandomParticles(double phi0, double extraRad);
   void initVerticesOnParticles();
   void scaleVertices(double scale);
+  void scaleBox(double scale);
 
   // ---- soft-disk (particle) set-up path, host side (FIRE.cu:43-171, DPM2D.cu:3521-3600)
   std::vector<double> particlePos, particleRad, particleVel, particleForce, particleEnergy, particleLastPos;

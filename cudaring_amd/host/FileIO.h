@@ -55,8 +55,7 @@ class ioDPMFile {
   void closeEnergyFile() { energyFile.close(); }
 
   // Energy log (FileIO.h:58-70): step+1, time, U/Nv, K/Nv, E/Nv, shape/COM kinetic energy ratio,
-  // hexatic order, phi.  The hexatic column needs the contact list, which this engine does not
-  // build; it is written as nan so the column count stays the reference's.
+  // hexatic order (from the ring contacts), phi.
   void saveEnergy(long step, double timeStep, long numParticles, long numVertices) {
     (void)numParticles;
     double epot = dpm_->getPotentialEnergy() / numVertices;

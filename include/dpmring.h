@@ -229,6 +229,11 @@ int dpm_get_preferred_phi(dpm_set_t h, double* perSystem);
 int dpm_get_particle_shapes(dpm_set_t h, double* shape3);
 /* scaleVertices(scale) per system, DPM2D.h:492 / DPM2D.cu:1802-1811 */
 int dpm_scale_vertices(dpm_set_t h, const double* scalePerSystem);
+/* scaleBox(scale) per system, DPM2D.h:498 / DPM2D.cu:1693-1719: ring centres and the box divide
+ * by scale and each ring moves with its centre (translateVertices, DPM2D.cu:1834-1839); in the
+ * round box the radius divides instead, with one scale for every system.  The neighbour list
+ * is not rebuilt (call dpm_calc_neighbors, as the reference's drivers do). */
+int dpm_scale_box(dpm_set_t h, const double* scalePerSystem);
 
 /* ---- launch configuration and measurement (no reference counterpart) ------- */
 /* Wavefronts per ring tile in the step kernel (1..8): the gather over a tile's neighbour rows

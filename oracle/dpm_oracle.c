@@ -259,6 +259,7 @@ static inline double pbcd(const orc_state* s, double x1, double x2, int dim) {
 void orc_set_geometry(orc_state* s, int geometryType) { s->geometryType = geometryType; }
 void orc_set_wall_energy_scale(orc_state* s, double ew) { s->ew = ew; }
 void orc_set_box_radius(orc_state* s, double boxRadius) { s->boxRadius = boxRadius; }
+double orc_get_box_radius(orc_state* s) { return s->boxRadius; }
 
 /* calcAngle, cudaKernel.cuh:423-427 */
 static double calc_angle(const double* n, const double* p) {
@@ -1191,6 +1192,32 @@ void orc_scale_vertices(orc_state* s, double scale) {
   }
   for (long p = 0; p < s->numParticles; p++) { s->a0[p] *= scale * scale; s->area[p] *= scale * scale; }
   for (long v = 0; v < s->numVertices; v++) { s->l0[v] *= scale; s->rad[v] *= scale; }
+}
+
+/* scaleBox, DPM2D.cu:1693-1719: ring centres and the box (the round box's radius) shrink by
+ * 1/scale; translateVertices (DPM2D.cu:1834-1839, kernelTranslateVertices cudaKernel.cuh:1678-1693)
+ * then moves each ring by the shift of its centre, a box distance in the new box.  The centres
+ * are those of the last force evaluation. */
+void orc_scale_box(orc_state* s, double scale) {
+  long np = s->numParticles;
+  double* last = (double*)malloc(sizeof(double) * 2 * np);
+  memcpy(last, s->particlePos, sizeof(double) * 2 * np);
+  for (long i = 0; i < 2 * np; i++) s->particlePos[i] /= scale;
+  if (s->geometryType == ORC_GEOM_ROUND_BOX) {
+    s->boxRadius /= scale;
+  } else {
+    s->boxSize[0] /= scale;
+    s->boxSize[1] /= scale;
+  }
+  for (long p = 0; p < np; p++) {
+    double delta[2];
+    for (int dim = 0; dim < 2; dim++) delta[dim] = pbcd(s, s->particlePos[p * 2 + dim], last[p * 2 + dim], dim);
+    long first = s->firstVertexInParticleId[p];
+    long end = first + s->numVertexInParticleList[p];
+    for (long v = first; v < end; v++)
+      for (int dim = 0; dim < 2; dim++) s->pos[v * 2 + dim] += delta[dim];
+  }
+  free(last);
 }
 
 /* getParticleShapes, DPM2D.cu:739-756 */

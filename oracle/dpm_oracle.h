@@ -110,6 +110,7 @@ void orc_set_noise_type(orc_state* s, int noiseType);
 void orc_set_geometry(orc_state* s, int geometryType);
 void orc_set_wall_energy_scale(orc_state* s, double ew);
 void orc_set_box_radius(orc_state* s, double boxRadius);
+double orc_get_box_radius(orc_state* s);
 void orc_calc_wall_forces(orc_state* s);
 /* observables */
 void orc_calc_contacts(orc_state* s);
@@ -153,6 +154,7 @@ double orc_get_mean_particle_size(const orc_state* s);
 double orc_get_mean_vertex_radius(const orc_state* s);
 void orc_adjust_temperature(orc_state* s, double targetTemp);
 void orc_scale_vertices(orc_state* s, double scale);
+void orc_scale_box(orc_state* s, double scale);
 void orc_get_particle_shapes(orc_state* s, double* shape3);
 
 /* RNG: Philox4x32-10 (Salmon et al., SC'11) and the Box-Muller pair built on it */

@@ -93,6 +93,7 @@ def lib():
             "orc_set_geometry": (None, [vp, i]),
             "orc_set_wall_energy_scale": (None, [vp, d]),
             "orc_set_box_radius": (None, [vp, d]),
+            "orc_get_box_radius": (d, [vp]),
             "orc_set_active_tension": (None, [vp, d, d]),
             "orc_update_rest_length": (None, [vp]),
             "orc_init_nve": (None, [vp, d, i, i]),
@@ -116,6 +117,7 @@ def lib():
             "orc_get_mean_vertex_radius": (d, [vp]),
             "orc_adjust_temperature": (None, [vp, d]),
             "orc_scale_vertices": (None, [vp, d]),
+            "orc_scale_box": (None, [vp, d]),
             "orc_get_particle_shapes": (None, [vp, vp]),
             "orc_philox4x32_10": (None, [vp, vp, vp]),
             "orc_gauss_pair": (None, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, dp, dp]),
@@ -331,6 +333,12 @@ class Oracle:
 
     def scale_vertices(self, s):
         self.L.orc_scale_vertices(self.h, s)
+
+    def scale_box(self, s):
+        self.L.orc_scale_box(self.h, s)
+
+    def box_radius(self):
+        return self.L.orc_get_box_radius(self.h)
 
     def shapes(self):
         o = np.zeros(3 * self.np)

@@ -78,3 +78,29 @@ def test_wall_langevin_trajectory(oracle_mod, geometry):
         assert rel_err(g.get("pos"), o.get("pos"), scale=1.0) < 1e-11
         assert rel_err(g.get("vel"), o.get("vel")) < 1e-9
     g.close()
+
+
+@pytest.mark.parametrize("geometry", ["normal", "fixedBox", "fixedSides", "roundBox"])
+def test_scale_box_parity(oracle_mod, geometry):
+    """scaleBox (DPM2D.cu:1693-1719) then a neighbour rebuild and forces, as compressNVE.cpp does
+    between compression steps"""
+    print(f"case scale_box {geometry}", flush=True)
+    o, g, st = prepare(oracle_mod, geometry)
+    o.calc_force_energy()
+    g.calc_force_energy()
+    for s in (1.01, 1.02):
+        o.scale_box(s)
+        g.scale_box(s)
+        if geometry == "roundBox":
+            assert abs(g.box_radius() - o.box_radius()) == 0.0
+        else:
+            assert np.array_equal(g.get("box"), o.get("box"))
+        assert rel_err(g.get("ppos"), o.get("ppos"), scale=1.0) < 1e-13
+        assert rel_err(g.get("pos"), o.get("pos"), scale=1.0) < 1e-13
+        size = 2 * st["rad"].mean()
+        o.calc_neighbor_list(o.set_displacement_cutoff(0.5, size))
+        g.calc_neighbor_list(g.set_displacement_cutoff(0.5, size))
+        o.calc_force_energy()
+        g.calc_force_energy()
+        assert rel_err(g.get("force"), o.get("force")) < 1e-12
+    g.close()
