@@ -4,7 +4,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <initializer_list>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/dpmring.h"
@@ -89,6 +91,12 @@ bool field_ptrs(dpm_set_s* S, int f, double** c0, double** c1, long* n, int* nco
     case DPM_FIELD_LASTPOS: *c0 = V.lx; *c1 = V.ly; *n = S->nv; *ncomp = 2; return true;
     case DPM_FIELD_BOX: *c0 = V.bx; *c1 = V.by; *n = S->nsys; *ncomp = 2; return true;
     case DPM_FIELD_WALLFORCE: *c0 = V.wfx; *c1 = V.wfy; *n = S->nv; *ncomp = 2; return true;
+    case DPM_FIELD_PVEL: *c0 = V.pvx; *c1 = V.pvy; *n = S->np; *ncomp = 2; return true;
+    case DPM_FIELD_PANGVEL: *c0 = V.pw; *n = S->np; return true;
+    case DPM_FIELD_PFORCE: *c0 = V.pfx; *c1 = V.pfy; *n = S->np; *ncomp = 2; return true;
+    case DPM_FIELD_PTORQUE: *c0 = V.ptq; *n = S->np; return true;
+    case DPM_FIELD_VTORQUE: *c0 = V.vtq; *n = S->nv; return true;
+    case DPM_FIELD_PRAD: *c0 = V.prad; *n = S->np; return true;
   }
   return false;
 }
@@ -731,6 +739,78 @@ int displacement_sums(dpm_set_s* S, bool rings, double waveNum, std::vector<doub
   return DPM_OK;
 }
 
+// ------------------------------------------------------------- rigid rings ---
+int rigid_supported(dpm_set_s* S) {
+  if (S->P.interaction != INT_VV && S->P.interaction != INT_SMOOTH)
+    return fail(S, DPM_ERR_UNSUPPORTED, "rigid rings: vertexVertex or vertexSmooth interaction only");
+  if (S->P.geometry != GEOM_NORMAL) return fail(S, DPM_ERR_UNSUPPORTED, "rigid rings: periodic geometry only");
+  if (S->P.particle != PART_PASSIVE) return fail(S, DPM_ERR_UNSUPPORTED, "rigid rings: passive particles only");
+  return require_list(S);
+}
+
+// rotational state cleared (initRotationalVariables / initDeltaVariables, DPM2D.cu:197-221)
+int rigid_clear(dpm_set_s* S) {
+  double* rp[] = {S->V.ang, S->V.pw, S->V.ptq, S->V.moi, S->V.pla};
+  for (double* p : rp) HIPCHK(launch_fill_d(p, 0.0, S->np, S->stream));
+  HIPCHK(launch_fill_d(S->V.vtq, 0.0, S->nv, S->stream));
+  return DPM_OK;
+}
+
+// Launch sequence of a rigid loop (RigidNVE / RigidLangevin), the velocity-Verlet sequence's
+// cell and flag protocol with ring updates:
+//   start: half kick of the rings, centres and angles advanced, vertices moved (buffer b -> 1-b)
+//   per step: step [rebuild rows if flagged; interaction F(x'); ring force, torque, moment of
+//             inertia; half kick; unless last: the next step's half kick and move]
+//   end: thermal call counter advanced by the loop's draws.
+int rigid_sequence(dpm_set_s* S, long steps, bool langevin) {
+  HIPCHK(launch_fill_int(S->mdCount[0], 0, S->G.ncells, S->stream));
+  HIPCHK(launch_fill_int(S->mdFlag[1], 0, S->nsys, S->stream));
+  HIPCHK(launch_rigid_start(S, langevin, md_args(S, 0, 0, 0)));
+  swap_pos(S);
+  for (long i = 1; i <= steps; i++) {
+    const bool more = i < steps;
+    StepArgs A = md_args(S, i, i - 1, 0);
+    A.rigidLangevin = langevin ? 1 : 0;
+    HIPCHK(launch_step(S, MODE_RIGID | MODE_KICK | (more ? MODE_START : 0), A));
+    if (more) swap_pos(S);
+  }
+  HIPCHK(launch_ctr_add(S, langevin ? steps : 0, 0));
+  return DPM_OK;
+}
+
+int run_rigid(dpm_set_s* S, long steps, bool langevin) {
+  if (steps <= 0) return DPM_OK;
+  int rc = rigid_supported(S);
+  if (rc) return rc;
+  HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
+  auto seq = [&]() -> int { return rigid_sequence(S, steps, langevin); };
+  if (steps < 4 || !graphs_enabled()) {
+    rc = seq();
+  } else {
+    rc = run_graph(S, std::make_tuple(langevin ? 20 : 21, steps, S->cur), seq);
+    if (!rc && (steps % 2) == 1) swap_pos(S);
+  }
+  if (rc) return rc;
+  return check_err_after_loop(S);
+}
+
+// rigid FIRE iterations: per iteration the ring update and vertex move of one workgroup per
+// system (into the other position buffer, with its flags), the bin launch and the step kernel
+// (rebuild of flagged rows, forces, ring sums, the second half kick)
+int rigid_fire_sequence(dpm_set_s* S, long iterations) {
+  HIPCHK(launch_fill_int(S->d_flag[1 - S->cur], 0, S->nsys, S->stream));
+  HIPCHK(launch_fill_int(S->d_any + 1 - S->cur, 0, 1, S->stream));
+  for (long it = 0; it < iterations; it++) {
+    HIPCHK(launch_rigid_fire_update(S));
+    swap_pos(S);
+    HIPCHK(launch_bin(S, 0, 0));
+    StepArgs A = loop_args(S);
+    A.fire = S->d_rfire;
+    HIPCHK(launch_step(S, MODE_RIGID | MODE_FIRE, A));
+  }
+  return DPM_OK;
+}
+
 }  // namespace
 
 // ======================================================================= ABI
@@ -806,6 +886,9 @@ int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPer
   A(V.a0, np); A(V.area, np); A(V.perim, np); A(V.cx, np); A(V.cy, np); A(V.pe, np); A(V.ang, np);
   A(V.bx, numSystems); A(V.by, numSystems); A(V.ibx, numSystems); A(V.iby, numSystems);
   A(V.mbx, numSystems); A(V.mby, numSystems); A(V.wfx, nv); A(V.wfy, nv);
+  A(V.pvx, np); A(V.pvy, np); A(V.pw, np); A(V.pfx, np); A(V.pfy, np); A(V.ptq, np); A(V.moi, np);
+  A(V.prad, np); A(V.plx, np); A(V.ply, np); A(V.pla, np); A(V.vtq, nv);
+  A(S->d_rfire, numSystems);
   A(V.svoff, numSystems + 1); A(V.sroff, numSystems + 1);
   A(V.ncnt, nv);
   A(V.tile_v0, V.ntiles); A(V.tile_v1, V.ntiles); A(V.tile_sys, V.ntiles);
@@ -836,9 +919,10 @@ int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPer
        up(V.ibx, ones, st) == hipSuccess && up(V.iby, ones, st) == hipSuccess &&
        up(V.mbx, ones, st) == hipSuccess && up(V.mby, ones, st) == hipSuccess;
   double* zd[] = {S->pos[0][0], S->pos[0][1], S->pos[1][0], S->pos[1][1], V.vx, V.vy, V.fx, V.fy, V.rad, V.l0,
-                  V.th0, V.th, V.len, V.ev, V.es, V.lx, V.ly, V.wfx, V.wfy};
+                  V.th0, V.th, V.len, V.ev, V.es, V.lx, V.ly, V.wfx, V.wfy, V.vtq};
   for (double* p : zd) ok = ok && hipMemsetAsync(p, 0, sizeof(double) * nv, st) == hipSuccess;
-  double* zp[] = {V.a0, V.area, V.perim, V.cx, V.cy, V.pe, V.ang};
+  double* zp[] = {V.a0, V.area, V.perim, V.cx, V.cy, V.pe, V.ang, V.pvx, V.pvy, V.pw, V.pfx, V.pfy, V.ptq,
+                  V.moi, V.prad, V.plx, V.ply, V.pla};
   for (double* p : zp) ok = ok && hipMemsetAsync(p, 0, sizeof(double) * np, st) == hipSuccess;
   ok = ok && hipMemsetAsync(V.ncnt, 0, sizeof(int) * nv, st) == hipSuccess;
   ok = ok && hipMemsetAsync(S->d_flag[0], 0, sizeof(int) * numSystems, st) == hipSuccess;
@@ -890,7 +974,8 @@ int dpm_destroy(dpm_set_t h) {
                   S->d_ck, S->d_fin, S->d_seg, S->d_adj, V.vinfo, V.l0r, V.mbx, V.mby, V.wfx, V.wfy,
                   S->d_contacts, S->d_ccount, S->d_x0, S->d_obs, S->mdCount[0], S->mdCount[1], S->mdCount[2],
                   S->mdSlots[0], S->mdSlots[1], S->mdSlots[2], S->mdFlag[0], S->mdFlag[1], S->mdFlag[2],
-                  S->mdFlag[3]};
+                  S->mdFlag[3], V.pvx, V.pvy, V.pw, V.pfx, V.pfy, V.ptq, V.moi, V.prad, V.plx, V.ply,
+                  V.pla, V.vtq, S->d_rfire};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (S->ownStream) hipStreamDestroy(S->ownStream);
@@ -2000,6 +2085,163 @@ int dpm_nve_backward(dpm_set_t h, const double* ax, const double* av, double* gx
   rc = rebuild(S, all, 0);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
+// ------------------------------------------------------------- rigid rings ---
+// calcRigidForceEnergy, DPM2D.cu:2128-2142
+int dpm_calc_rigid_force_energy(dpm_set_t h) {
+  CHECK_HANDLE(h);
+  int rc = rigid_supported(S);
+  if (rc) return rc;
+  StepArgs A{};
+  A.ctr = S->d_ctr;
+  A.err = S->d_err;
+  A.manage = 0;
+  HIPCHK(launch_step(S, MODE_RIGID, A));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
+// initRigidLangevin / initRigidNVE, DPM2D.cu:3823-3868
+static int rigid_init(dpm_set_s* S, double T, double gamma, bool langevin, int readState) {
+  S->rigidKind = langevin ? 1 : 2;
+  S->Tinject = T;
+  if (langevin) {
+    S->P.gamma = gamma;
+    S->P.noise = sqrt(2. * T * gamma);
+  }
+  bump(S);
+  int rc = rigid_clear(S);
+  if (rc) return rc;
+  HIPCHK(launch_copy_lastpos(S));
+  rc = store_initial_positions(S);
+  if (rc) return rc;
+  if (!readState) HIPCHK(launch_rigid_inject(S, sqrt(T), S->injectCalls++));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
+int dpm_init_rigid_langevin(dpm_set_t h, double T, double gamma, int readState) {
+  CHECK_HANDLE(h);
+  if (!(gamma > 0)) return fail(S, DPM_ERR_ARG, "gamma must be positive");
+  return rigid_init(S, T, gamma, true, readState);
+}
+
+int dpm_init_rigid_nve(dpm_set_t h, double T, int readState) {
+  CHECK_HANDLE(h);
+  return rigid_init(S, T, 0.0, false, readState);
+}
+
+int dpm_rigid_langevin_loop(dpm_set_t h, long steps) {
+  CHECK_HANDLE(h);
+  if (S->rigidKind != 1) return fail(S, DPM_ERR_STATE, "dpm_init_rigid_langevin not called");
+  return run_rigid(S, steps, true);
+}
+
+int dpm_rigid_nve_loop(dpm_set_t h, long steps) {
+  CHECK_HANDLE(h);
+  if (S->rigidKind != 2) return fail(S, DPM_ERR_STATE, "dpm_init_rigid_nve not called");
+  return run_rigid(S, steps, false);
+}
+
+// initRigidFIRE, DPM2D.cu:3700-3706 (FIRE parameters as dpm_init_fire's)
+int dpm_init_rigid_fire(dpm_set_t h, const double* p, long minStep, long numStep) {
+  CHECK_HANDLE(h);
+  if (!p) return fail(S, DPM_ERR_ARG, "null FIRE params");
+  S->fp.a_start = p[0]; S->fp.f_dec = p[1]; S->fp.f_inc = p[2]; S->fp.f_a = p[3];
+  S->fp.dt_max = p[5];
+  S->fp.minStep = minStep;
+  S->fp.tol = -1;
+  S->fp.maxIter = 0;
+  std::vector<FireState> fs(S->nsys);
+  for (auto& f : fs) {
+    f.dt = p[4]; f.a = p[6]; f.P = 0; f.numStep = numStep; f.iters = 0; f.converged = 0; f.pad = 0;
+  }
+  HIPCHK(hipMemcpyAsync(S->d_rfire, fs.data(), sizeof(FireState) * S->nsys, hipMemcpyHostToDevice, S->stream));
+  int rc = rigid_clear(S);
+  if (rc) return rc;
+  S->rigidFireInit = true;
+  bump(S);
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
+// rigidFIRELoop, DPM2D.cu:3708-3710 (FIRE::minimizerRigidLoop, FIRE.cu:279-298), `iterations` times
+int dpm_rigid_fire_loop(dpm_set_t h, long iterations) {
+  CHECK_HANDLE(h);
+  if (!S->rigidFireInit) return fail(S, DPM_ERR_STATE, "dpm_init_rigid_fire not called");
+  int rc = rigid_supported(S);
+  if (rc) return rc;
+  if (iterations <= 0) return DPM_OK;
+  HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
+  if (iterations < 4 || !graphs_enabled()) {
+    rc = rigid_fire_sequence(S, iterations);
+  } else {
+    rc = run_graph(S, std::make_tuple(22, iterations, S->cur), [&]() { return rigid_fire_sequence(S, iterations); });
+    if (!rc && (iterations % 2) == 1) swap_pos(S);
+  }
+  if (rc) return rc;
+  return check_err_after_loop(S);
+}
+
+// FIRE scalars of the rigid minimiser per system: dt, a, numStep, last power, iterations
+int dpm_get_rigid_fire_state(dpm_set_t h, double* out5) {
+  CHECK_HANDLE(h);
+  if (!out5) return fail(S, DPM_ERR_ARG, "null output");
+  std::vector<FireState> fs(S->nsys);
+  HIPCHK(hipMemcpyAsync(fs.data(), S->d_rfire, sizeof(FireState) * S->nsys, hipMemcpyDeviceToHost, S->stream));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  for (long s = 0; s < S->nsys; s++) {
+    out5[5 * s] = fs[s].dt;
+    out5[5 * s + 1] = fs[s].a;
+    out5[5 * s + 2] = (double)fs[s].numStep;
+    out5[5 * s + 3] = fs[s].P;
+    out5[5 * s + 4] = (double)fs[s].iters;
+  }
+  return DPM_OK;
+}
+
+static int ring_fields(dpm_set_s* S, std::initializer_list<std::pair<const double*, std::vector<double>*>> f) {
+  for (auto& kv : f) {
+    kv.second->resize(S->np);
+    HIPCHK(hipMemcpyAsync(kv.second->data(), kv.first, sizeof(double) * S->np, hipMemcpyDeviceToHost, S->stream));
+  }
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
+// getRigidMaxUnbalancedForce, DPM2D.cu:3643-3652, per system
+int dpm_get_rigid_max_unbalanced_force(dpm_set_t h, double* perSystem) {
+  CHECK_HANDLE(h);
+  if (!perSystem) return fail(S, DPM_ERR_ARG, "null output");
+  std::vector<double> fx, fy, tq;
+  int rc = ring_fields(S, {{S->V.pfx, &fx}, {S->V.pfy, &fy}, {S->V.ptq, &tq}});
+  if (rc) return rc;
+  for (long s = 0; s < S->nsys; s++) {
+    double mf = -1, mt = -1;
+    for (int r = S->h_sroff[s]; r < S->h_sroff[s + 1]; r++) {
+      mf = std::max(mf, std::max(fx[r] * fx[r], fy[r] * fy[r]));
+      mt = std::max(mt, tq[r] * tq[r]);
+    }
+    perSystem[s] = std::max(sqrt(mf), sqrt(mt));
+  }
+  return DPM_OK;
+}
+
+// getRigidKineticEnergy, DPM2D.cu:1149-1159, per system
+int dpm_get_rigid_kinetic_energy(dpm_set_t h, double* perSystem) {
+  CHECK_HANDLE(h);
+  if (!perSystem) return fail(S, DPM_ERR_ARG, "null output");
+  std::vector<double> vx, vy, w, rad;
+  int rc = ring_fields(S, {{S->V.pvx, &vx}, {S->V.pvy, &vy}, {S->V.pw, &w}, {S->V.prad, &rad}});
+  if (rc) return rc;
+  for (long s = 0; s < S->nsys; s++) {
+    double k = 0, r2 = 0;
+    for (int r = S->h_sroff[s]; r < S->h_sroff[s + 1]; r++) k += vx[r] * vx[r] + vy[r] * vy[r];
+    for (int r = S->h_sroff[s]; r < S->h_sroff[s + 1]; r++) r2 += (w[r] * rad[r]) * (w[r] * rad[r]);
+    perSystem[s] = 0.5 * k + 0.5 * r2;
+  }
   return DPM_OK;
 }
 

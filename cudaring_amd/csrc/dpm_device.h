@@ -60,6 +60,10 @@ struct View {
   int *tile_v0, *tile_v1, *tile_sys;  // ring-aligned tiles (<= 64 vertices, one system)
   int* stile;          // per-system first tile (nsys + 1)
   int ntiles;
+  // rigid rings, per ring: velocity, angular velocity, force, torque / moment of inertia (an
+  // angular acceleration), moment of inertia, particle radius, last centre and angle; per
+  // vertex: torque about the ring centre
+  double *pvx, *pvy, *pw, *pfx, *pfy, *ptq, *moi, *prad, *plx, *ply, *pla, *vtq;
 };
 
 // Box of one system with its reciprocal: the minimum image multiplies by 1/L instead of dividing

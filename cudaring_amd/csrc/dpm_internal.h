@@ -15,9 +15,11 @@ namespace dpm {
 
 // step-kernel epilogue bits
 // (MODE_BAOAB: the next step's first half is B A O A; MODE_BROWN: overdamped update, with
-// thermal noise when MODE_LANGEVIN is also set)
+// thermal noise when MODE_LANGEVIN is also set; MODE_RIGID: rigid rings, whose ring force and
+// torque drive ring velocities, with the rigid Langevin kick when MODE_LANGEVIN is also set and
+// the per-system FIRE step when MODE_FIRE is)
 enum : int { MODE_FORCE = 0, MODE_KICK = 1, MODE_LANGEVIN = 2, MODE_START = 4, MODE_FIRE = 8, MODE_BAOAB = 16,
-             MODE_BROWN = 32 };
+             MODE_BROWN = 32, MODE_RIGID = 64 };
 
 // noise types, simControlStruct::noiseEnum order (include/DPM2D.h:27)
 enum : int { NOISE_LANGEVIN = 0, NOISE_BAOAB = 1, NOISE_BROWNIAN = 2, NOISE_DRIVEN_BROWNIAN = 3 };
@@ -77,6 +79,7 @@ struct StepArgs {
   int* maxcnt;
   int* updateCount;
   int manage;                     // inside loops: rebuild flagged rows, clear cells, count updates
+  int rigidLangevin;              // MODE_RIGID: the RigidLangevin kick (else RigidNVE's)
   int xcd;                        // XCD-aware workgroup -> tile mapping
 };
 
@@ -149,6 +152,9 @@ struct dpm_set_s {
 
   // integrators
   int simKind = 0;  // 1 langevin, 2 nve, 3 nve rescale
+  int rigidKind = 0;  // rigid rings: 1 langevin, 2 nve
+  dpm::FireState* d_rfire = nullptr;  // rigid FIRE state per system
+  bool rigidFireInit = false;
   int noiseType = 0;  // integrator of the Langevin loop (NOISE_*)
   // observables: contact rows (np x contactWidth, -1 padded) and counts, initial positions of
   // vertices and ring centres, per-ring / per-vertex scratch
@@ -232,5 +238,8 @@ hipError_t launch_displacement(dpm_set_s* S, const double* x, const double* y, c
                                long n, int rings, double waveNum, double* d2, double* sf);
 hipError_t launch_particle_stress(dpm_set_s* S, double* out);
 hipError_t launch_copy_d(double* dst, const double* src, long n, hipStream_t st);
+hipError_t launch_rigid_start(dpm_set_s* S, bool langevin, const StepArgs& A);
+hipError_t launch_rigid_inject(dpm_set_s* S, double amplitude, unsigned long long call);
+hipError_t launch_rigid_fire_update(dpm_set_s* S);
 
 }  // namespace dpm

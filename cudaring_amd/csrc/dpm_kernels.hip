@@ -39,6 +39,7 @@ __device__ __forceinline__ bool wave_tile_ctx(const View& V, Lane& L) {
 // terms of 429-471.  Writes theta, length and the ring area/perimeter/centre.
 struct ShapeOut {
   double fx, fy, e;
+  double cx, cy;  // ring centre (calcParticlePos, cudaKernel.cuh:365-388)
 };
 
 __device__ __forceinline__ ShapeOut ring_shape(const View& V, const Params& P, const Lane& L, const Box& B,
@@ -67,6 +68,8 @@ __device__ __forceinline__ ShapeOut ring_shape(const View& V, const Params& P, c
   const double nd = (double)L.n, ind = 1.0 / nd;
   const double area = fabs(shoeSum) * 0.5;
   const double comx = xf + sux * ind, comy = yf + suy * ind;
+  o.cx = comx;
+  o.cy = comy;
   if (write && L.act) {
     V.th[L.v] = theta;
     V.len[L.v] = seglen;
@@ -233,6 +236,105 @@ __device__ __forceinline__ void gather_pairs(const View& V, const Params& P, con
 // -------------------------------------------------------------- step ---
 // Waves of a tile: with nw > 1, wave 0 does the ring geometry, shape force and random draws
 // while waves 1..nw-1 share the neighbour gather; with nw == 1 wave 0 does both in turn.
+// ---------------------------------------------------------- rigid rings ---
+// Half kick of a rigid ring: RigidNVE::updateVelocity (Simulator.cu:348-366), or
+// RigidLangevin::updateVelocity (263-286) with its unit Gaussian pair (z0, z1) of the step and
+// d_thermalVel zero; the angular velocity takes the torque / moment of inertia.
+__device__ __forceinline__ void rigid_kick(bool langevin, double h, double gamma, double Fx, double Fy, double T,
+                                           double z0, double z1, double& vx, double& vy, double& w) {
+  if (langevin) {
+    vx += h * (Fx - vx * gamma);
+    vx -= 0.5 * h * h * (Fx - vx * gamma) / gamma;
+    vx += z0;
+    vy += h * (Fy - vy * gamma);
+    vy -= 0.5 * h * h * (Fy - vy * gamma) / gamma;
+    vy += z1;
+  } else {
+    vx += h * Fx;
+    vy += h * Fy;
+  }
+  w += h * T;
+}
+
+// A ring's vertex after the ring moved from centre (cx, cy) to (ncx, ncy) and turned by dth:
+// translateVertices then rotateVertices (kernelTranslateVertices / kernelRotateVertices,
+// cudaKernel.cuh:1678-1713), each difference a box distance.
+__device__ __forceinline__ void rigid_move(const Box& B, double x, double y, double cx, double cy, double ncx,
+                                           double ncy, double c, double sn, double& xn, double& yn) {
+  const double x1 = x + pbx(ncx - cx, B), y1 = y + pby(ncy - cy, B);
+  const double dx = pbx(x1 - ncx, B), dy = pby(y1 - ncy, B);
+  xn = x1 + ((dx * c - dy * sn) - dx);
+  yn = y1 + ((dx * sn + dy * c) - dy);
+}
+
+// Step-kernel epilogue of rigid rings (wave 0, one ring per lane segment): the interaction
+// forces' transfer to the ring (calcRigidForceEnergy, DPM2D.cu:2128-2142: torque r x f about
+// the centre, moment of inertia sum |r|^2, ring force sum f, ring torque sum (r x f) / I, ring
+// energy sum e for vertexVertex), then the second half kick of this step and, with MODE_START,
+// the next step's first half: last centre and angle stored, half kick with the next step's
+// draw, centre and angle advanced, vertices moved (RigidNVE / RigidLangevin::integrate,
+// Simulator.cu:241-252, 316-326).  With MODE_FIRE the half kick takes the system's FIRE step.
+template <int MODE, bool VV>
+__device__ __forceinline__ void rigid_epilogue(const View& V, const Params& P, const StepArgs& A, const Lane& L,
+                                               const Box& B, double x, double y, double fx, double fy, double e,
+                                               double cx, double cy, double lx, double ly, double z0, double z1) {
+  const double rx = pbx(x - cx, B), ry = pby(y - cy, B);
+  const double tq = rx * fy - ry * fx;
+  const double I = seg_sum(L.act ? rx * rx + ry * ry : 0.0, L.lane, L.segS, L.segE);
+  const double Fx = seg_sum(L.act ? fx : 0.0, L.lane, L.segS, L.segE);
+  const double Fy = seg_sum(L.act ? fy : 0.0, L.lane, L.segS, L.segE);
+  const double T = seg_sum(L.act ? tq : 0.0, L.lane, L.segS, L.segE) / I;
+  double E = 0;
+  if constexpr (VV) E = seg_sum(L.act ? e : 0.0, L.lane, L.segS, L.segE);
+  if (!L.act) return;
+  const bool head = L.lane == L.segS;
+  V.fx[L.v] = fx;
+  V.fy[L.v] = fy;
+  V.ev[L.v] = e;
+  V.vtq[L.v] = tq;
+  if (head) {
+    V.pfx[L.r] = Fx;
+    V.pfy[L.r] = Fy;
+    V.ptq[L.r] = T;
+    V.moi[L.r] = I;
+    if constexpr (VV) V.pe[L.r] = E;
+  }
+  if constexpr (MODE == MODE_RIGID) return;
+  double vx = V.pvx[L.r], vy = V.pvy[L.r], w = V.pw[L.r];
+  double h = A.h;
+  if constexpr ((MODE & MODE_FIRE) != 0) h = 0.5 * A.fire[L.s].dt;
+  rigid_kick(A.rigidLangevin, h, P.gamma, Fx, Fy, T, z0, z1, vx, vy, w);
+  if constexpr ((MODE & MODE_START) != 0) {
+    double n0 = 0, n1 = 0;
+    if (A.rigidLangevin)
+      gauss_pair(P.seed, (uint32_t)L.s, TAG_THERMAL, A.ctr[0] + (unsigned long long)A.ctrOffT + 1ull,
+                 (uint32_t)(L.r - V.sroff[L.s]), n0, n1);
+    rigid_kick(A.rigidLangevin, A.h, P.gamma, Fx, Fy, T, n0, n1, vx, vy, w);
+    const double ang = V.ang[L.r];
+    const double ncx = cx + A.dt * vx, ncy = cy + A.dt * vy, nang = ang + A.dt * w;
+    const double dth = nang - ang;
+    double xn, yn;
+    rigid_move(B, x, y, cx, cy, ncx, ncy, cos(dth), sin(dth), xn, yn);
+    A.xout[L.v] = xn;
+    A.yout[L.v] = yn;
+    const double ddx = pbx(xn - lx, B), ddy = pby(yn - ly, B);
+    if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) {
+      atomicOr(&A.flagOut[L.s], 1);
+      atomicOr(A.anyOut, 1);
+    }
+    if (head) {
+      V.ang[L.r] = nang;
+      V.cx[L.r] = ncx;
+      V.cy[L.r] = ncy;
+    }
+  }
+  if (head) {
+    V.pvx[L.r] = vx;
+    V.pvy[L.r] = vy;
+    V.pw[L.r] = w;
+  }
+}
+
 template <int INTER, int POT, int MODE>
 __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepArgs A, CellGrid G) {
   // partials of gather waves 1..nw-1: red[((w - 1) * 6 + q) * WAVE + lane], sized by the launch
@@ -309,6 +411,10 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
     const unsigned long long callA = A.ctr[1] + (unsigned long long)A.ctrOffA;
     if constexpr ((MODE & MODE_LANGEVIN) != 0 || ((MODE & MODE_BAOAB) != 0 && (MODE & MODE_START) != 0))
       gauss_pair(P.seed, (uint32_t)L.s, TAG_THERMAL, callT, (uint32_t)(L.r - V.sroff[L.s]), zt0, zt1);
+    // rigid Langevin: this step's pair (RigidLangevin::integrate draws it before its first half)
+    if constexpr ((MODE & MODE_RIGID) != 0 && (MODE & MODE_KICK) != 0)
+      if (A.rigidLangevin)
+        gauss_pair(P.seed, (uint32_t)L.s, TAG_THERMAL, callT, (uint32_t)(L.r - V.sroff[L.s]), zt0, zt1);
     if (P.geometry != GEOM_NORMAL) {  // walls, after the interactions (DPM2D.cu:2090-2103)
       double wfx, wfy;
       so.e += wall_forces(P, B, x, y, V.rad[L.v], so.fx, so.fy, wfx, wfy);
@@ -359,6 +465,10 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
       V.es[L.v] = es;
       if (L.lane == L.segS) V.pe[L.r] = pe;
     }
+  }
+  if constexpr ((MODE & MODE_RIGID) != 0) {
+    rigid_epilogue<MODE, INTER == INT_VV>(V, P, A, L, B, x, y, fx, fy, e, so.cx, so.cy, lx, ly, zt0, zt1);
+    return;
   }
   fx = so.fx + fx;
   fy = so.fy + fy;
@@ -496,6 +606,201 @@ __global__ void k_start_loop(View V, Params P, CellGrid G, double h, double dt, 
   if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) {
     atomicOr(&A.flagOut[s], 1);
     atomicOr(A.anyOut, 1);
+  }
+}
+
+// The rigid loop's first half step from the stored ring force and torque, one thread per ring
+// (RigidNVE / RigidLangevin::integrate, Simulator.cu:241-252, 316-326, up to the vertex move):
+// last centre and angle stored, half kick with the draw of call ctr[0] + ctrOffT, centre and
+// angle advanced, and each vertex binned at its start position into A.cellOut, moved into
+// A.xout and tested into A.flagOut; A.cellOther / A.flagClear are cleared.
+__global__ void k_rigid_start(View V, Params P, CellGrid G, StepArgs A) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int c = t; c < G.ncells; c += gridDim.x * blockDim.x) A.cellOther[c] = 0;
+  for (int q = t; q < V.nsys; q += gridDim.x * blockDim.x) A.flagClear[q] = 0;
+  if (t >= V.np) return;
+  const int r = t, s = V.rsys[r];
+  const Box B = box_of(V, s);
+  double vx = V.pvx[r], vy = V.pvy[r], w = V.pw[r];
+  double z0 = 0, z1 = 0;
+  if (A.rigidLangevin)
+    gauss_pair(P.seed, (uint32_t)s, TAG_THERMAL, A.ctr[0] + (unsigned long long)A.ctrOffT, (uint32_t)(r - V.sroff[s]),
+               z0, z1);
+  rigid_kick(A.rigidLangevin, A.h, P.gamma, V.pfx[r], V.pfy[r], V.ptq[r], z0, z1, vx, vy, w);
+  const double cx = V.cx[r], cy = V.cy[r], ang = V.ang[r];
+  const double ncx = cx + A.dt * vx, ncy = cy + A.dt * vy, nang = ang + A.dt * w;
+  const double dth = nang - ang, c = cos(dth), sn = sin(dth);
+  bool flag = false;
+  for (int v = V.rfirst[r]; v < V.rfirst[r] + V.rnv[r]; v++) {
+    const double x = V.x[v], y = V.y[v];
+    bin_vertex(G, B, A.cellOut, A.slotsOut, v, s, x, y, A.err);
+    double xn, yn;
+    rigid_move(B, x, y, cx, cy, ncx, ncy, c, sn, xn, yn);
+    A.xout[v] = xn;
+    A.yout[v] = yn;
+    const double ddx = pbx(xn - V.lx[v], B), ddy = pby(yn - V.ly[v], B);
+    flag = flag || 3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff;
+  }
+  if (flag) {
+    atomicOr(&A.flagOut[s], 1);
+    atomicOr(A.anyOut, 1);
+  }
+  V.plx[r] = cx;
+  V.ply[r] = cy;
+  V.pla[r] = ang;
+  V.pvx[r] = vx;
+  V.pvy[r] = vy;
+  V.pw[r] = w;
+  V.ang[r] = nang;
+  V.cx[r] = ncx;
+  V.cy[r] = ncy;
+}
+
+// RigidLangevin::injectKineticEnergy (Simulator.cu:254-261), one workgroup per system: a
+// Gaussian pair of the given amplitude per ring (counter `call`), minus the system's mean ring
+// velocity (kernelConserveParticleMomentum, cudaKernel.cuh:1998-2027, over every ring)
+__global__ __launch_bounds__(256) void k_rigid_inject(View V, Params P, double amplitude, unsigned long long call) {
+  __shared__ double red[2][256 / WAVE];
+  const int s = blockIdx.x, tid = threadIdx.x;
+  const int r0 = V.sroff[s], r1 = V.sroff[s + 1];
+  double mx = 0, my = 0;
+  for (int r = r0 + tid; r < r1; r += blockDim.x) {
+    double z0, z1;
+    gauss_pair(P.seed, (uint32_t)s, TAG_INJECT, call, (uint32_t)(r - r0), z0, z1);
+    V.pvx[r] = amplitude * z0;
+    V.pvy[r] = amplitude * z1;
+    mx += V.pvx[r];
+    my += V.pvy[r];
+  }
+  mx = wave_sum(mx);
+  my = wave_sum(my);
+  if ((tid & (WAVE - 1)) == 0) {
+    red[0][tid / WAVE] = mx;
+    red[1][tid / WAVE] = my;
+  }
+  __syncthreads();
+  mx = 0;
+  my = 0;
+  for (int q = 0; q < (int)(blockDim.x / WAVE); q++) {
+    mx += red[0][q];
+    my += red[1][q];
+  }
+  mx /= (r1 - r0);
+  my /= (r1 - r0);
+  for (int r = r0 + tid; r < r1; r += blockDim.x) {
+    V.pvx[r] -= mx;
+    V.pvy[r] -= my;
+  }
+}
+
+// workgroup sums of two values (every thread gets both)
+__device__ __forceinline__ void block_sum2(double a, double b, double (*red)[256 / WAVE], double& sa, double& sb) {
+  const int tid = threadIdx.x;
+  a = wave_sum(a);
+  b = wave_sum(b);
+  __syncthreads();
+  if ((tid & (WAVE - 1)) == 0) {
+    red[0][tid / WAVE] = a;
+    red[1][tid / WAVE] = b;
+  }
+  __syncthreads();
+  sa = 0;
+  sb = 0;
+  for (int q = 0; q < (int)(blockDim.x / WAVE); q++) {
+    sa += red[0][q];
+    sb += red[1][q];
+  }
+}
+
+// One rigid FIRE iteration up to the vertex move, one workgroup per system (FIRE::
+// minimizerRigidLoop, FIRE.cu:279-291, with checkRigidFIREParams 217-254 and
+// bendRigidVelocityTowardsForce 256-277): power v.F over the ring velocities and forces, the
+// FIRE scalars' update, on a restart the step back of centres and angles by half the
+// already-decreased step with both velocities cleared, the half kick, the bend of the ring
+// velocities, the move of centres and angles, then every vertex moved into (xo, yo) with its
+// displacement test.  The last centres and angles are stored at the start of the iteration.
+__global__ __launch_bounds__(256) void k_rigid_fire_update(View V, Params P, FireParams fp, FireState* fs,
+                                                           double* xo, double* yo, int* flagOut, int* anyOut) {
+  __shared__ double red[2][256 / WAVE];
+  const int s = blockIdx.x, tid = threadIdx.x;
+  const int r0 = V.sroff[s], r1 = V.sroff[s + 1];
+  double p = 0;
+  for (int r = r0 + tid; r < r1; r += blockDim.x) p += V.pvx[r] * V.pfx[r] + V.pvy[r] * V.pfy[r];
+  double P0, unused;
+  block_sum2(p, 0.0, red, P0, unused);
+  FireState f = fs[s];
+  const bool restart = !(P0 > 0);
+  if (!restart) {
+    f.numStep++;
+    if (f.numStep > fp.minStep) {
+      f.dt = fmin(f.dt * fp.f_inc, fp.dt_max);
+      f.a *= fp.f_a;
+    }
+  } else {
+    f.numStep = 0;
+    f.dt = fmax(f.dt * fp.f_dec, fp.dt_max / 10);
+    f.a = fp.a_start;
+  }
+  const double h = 0.5 * f.dt;
+  double vv = 0, ff = 0;
+  for (int r = r0 + tid; r < r1; r += blockDim.x) {
+    double cx = V.cx[r], cy = V.cy[r], ang = V.ang[r];
+    double vx = V.pvx[r], vy = V.pvy[r], w = V.pw[r];
+    V.plx[r] = cx;
+    V.ply[r] = cy;
+    V.pla[r] = ang;
+    if (restart) {
+      cx -= h * vx;
+      cy -= h * vy;
+      ang -= h * w;
+      vx = vy = w = 0;
+    }
+    rigid_kick(false, h, 0.0, V.pfx[r], V.pfy[r], V.ptq[r], 0.0, 0.0, vx, vy, w);
+    V.cx[r] = cx;
+    V.cy[r] = cy;
+    V.ang[r] = ang;
+    V.pvx[r] = vx;
+    V.pvy[r] = vy;
+    V.pw[r] = w;
+    vv += vx * vx + vy * vy;
+    ff += V.pfx[r] * V.pfx[r] + V.pfy[r] * V.pfy[r];
+  }
+  double VV, FF;
+  block_sum2(vv, ff, red, VV, FF);
+  for (int r = r0 + tid; r < r1; r += blockDim.x) {
+    double vx = V.pvx[r], vy = V.pvy[r];
+    if (FF > 0) {
+      const double ratio = sqrt(VV / FF);
+      vx = (1 - f.a) * vx + f.a * V.pfx[r] * ratio;
+      vy = (1 - f.a) * vy + f.a * V.pfy[r] * ratio;
+    }
+    V.pvx[r] = vx;
+    V.pvy[r] = vy;
+    V.cx[r] += f.dt * vx;
+    V.cy[r] += f.dt * vy;
+    V.ang[r] += f.dt * V.pw[r];
+  }
+  __syncthreads();
+  const Box B = box_of(V, s);
+  bool flag = false;
+  for (int v = V.svoff[s] + tid; v < V.svoff[s + 1]; v += blockDim.x) {
+    const int r = V.vring[v];
+    const double dth = V.ang[r] - V.pla[r];
+    double xn, yn;
+    rigid_move(B, V.x[v], V.y[v], V.plx[r], V.ply[r], V.cx[r], V.cy[r], cos(dth), sin(dth), xn, yn);
+    xo[v] = xn;
+    yo[v] = yn;
+    const double ddx = pbx(xn - V.lx[v], B), ddy = pby(yn - V.ly[v], B);
+    flag = flag || 3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff;
+  }
+  if (flag) {
+    atomicOr(&flagOut[s], 1);
+    atomicOr(anyOut, 1);
+  }
+  if (tid == 0) {
+    f.P = P0;
+    f.iters++;
+    fs[s] = f;
   }
 }
 
@@ -1019,6 +1324,16 @@ static hipError_t step_mode(dpm_set_s* S, int mode, const StepArgs& A) {
     case MODE_BROWN | MODE_LANGEVIN | MODE_START:
       k_step<INTER, POT, MODE_BROWN | MODE_LANGEVIN | MODE_START><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
       break;
+    case MODE_RIGID: k_step<INTER, POT, MODE_RIGID><<<g, b, sh, st>>>(S->V, S->P, A, S->G); break;
+    case MODE_RIGID | MODE_KICK:
+      k_step<INTER, POT, MODE_RIGID | MODE_KICK><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
+      break;
+    case MODE_RIGID | MODE_KICK | MODE_START:
+      k_step<INTER, POT, MODE_RIGID | MODE_KICK | MODE_START><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
+      break;
+    case MODE_RIGID | MODE_FIRE:
+      k_step<INTER, POT, MODE_RIGID | MODE_FIRE><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -1039,7 +1354,8 @@ hipError_t launch_step(dpm_set_s* S, int mode, const StepArgs& A) {
   // refuse argument sets the chosen epilogue would dereference as null
   if (!A.ctr) return hipErrorInvalidValue;
   if (S->P.particle == PART_ACTIVE_SHAPE && !S->V.l0r) return hipErrorInvalidValue;
-  if ((mode & MODE_FIRE) && (!A.fire || !A.part)) return hipErrorInvalidValue;
+  if ((mode & MODE_FIRE) && (!A.fire || (!A.part && !(mode & MODE_RIGID)))) return hipErrorInvalidValue;
+  if ((mode & MODE_RIGID) && S->P.interaction == INT_ALL) return hipErrorInvalidValue;
   if ((mode & MODE_START) && (!A.xout || !A.yout || !A.flagOut || !A.anyOut)) return hipErrorInvalidValue;
   if (A.manage && (!A.flagCur || !A.cellCur || !A.cellOther || !A.err || !A.maxcnt || !A.updateCount))
     return hipErrorInvalidValue;
@@ -1151,6 +1467,30 @@ hipError_t launch_scale_vertices(dpm_set_s* S) {
 hipError_t launch_scale_box(dpm_set_s* S) {
   k_scale_box<<<grid_v(S->nv), 256, 0, S->stream>>>(S->V, S->d_scale);
   k_scale_centres<<<grid_v(S->np), 256, 0, S->stream>>>(S->V, S->d_scale);
+  return hipGetLastError();
+}
+
+hipError_t launch_rigid_start(dpm_set_s* S, bool langevin, const StepArgs& A) {
+  if (!A.ctr || !A.cellOut || !A.slotsOut || !A.cellOther || !A.flagClear || !A.flagOut || !A.anyOut || !A.xout ||
+      !A.yout)
+    return hipErrorInvalidValue;
+  StepArgs B = A;
+  B.rigidLangevin = langevin ? 1 : 0;
+  long n = S->np > S->G.ncells ? S->np : S->G.ncells;
+  if (n < S->nsys) n = S->nsys;
+  k_rigid_start<<<grid_v(n), 256, 0, S->stream>>>(S->V, S->P, S->G, B);
+  return hipGetLastError();
+}
+
+hipError_t launch_rigid_inject(dpm_set_s* S, double amplitude, unsigned long long call) {
+  k_rigid_inject<<<(int)S->nsys, 256, 0, S->stream>>>(S->V, S->P, amplitude, call);
+  return hipGetLastError();
+}
+
+hipError_t launch_rigid_fire_update(dpm_set_s* S) {
+  const int nb = 1 - S->cur;
+  k_rigid_fire_update<<<(int)S->nsys, 256, 0, S->stream>>>(S->V, S->P, S->fp, S->d_rfire, S->pos[nb][0],
+                                                            S->pos[nb][1], S->d_flag[nb], S->d_any + nb);
   return hipGetLastError();
 }
 

@@ -14,7 +14,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libdpmring.so")
 
 FIELDS = ["pos", "vel", "force", "energy", "rad", "l0", "theta0", "theta", "length", "a0", "area",
-          "perimeter", "ppos", "penergy", "pangle", "lastpos", "box", "wallforce"]
+          "perimeter", "ppos", "penergy", "pangle", "lastpos", "box", "wallforce",
+          "pvel", "pangvel", "pforce", "ptorque", "vtorque", "prad"]
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 POT = {"harmonic": 0, "lj": 1, "adhesive": 2, "wca": 3}
 INTER = {"vertexVertex": 0, "vertexSmooth": 1, "all": 3}
@@ -83,6 +84,16 @@ SIGNATURES = {
     "dpm_init_fire": (_i, [_vp, _vp, _l, _l]),
     "dpm_fire_loop": (_i, [_vp, _l]),
     "dpm_get_fire_state": (_i, [_vp, _vp]),
+    "dpm_calc_rigid_force_energy": (_i, [_vp]),
+    "dpm_init_rigid_langevin": (_i, [_vp, _d, _d, _i]),
+    "dpm_rigid_langevin_loop": (_i, [_vp, _l]),
+    "dpm_init_rigid_nve": (_i, [_vp, _d, _i]),
+    "dpm_rigid_nve_loop": (_i, [_vp, _l]),
+    "dpm_init_rigid_fire": (_i, [_vp, _vp, _l, _l]),
+    "dpm_rigid_fire_loop": (_i, [_vp, _l]),
+    "dpm_get_rigid_fire_state": (_i, [_vp, _vp]),
+    "dpm_get_rigid_max_unbalanced_force": (_i, [_vp, _vp]),
+    "dpm_get_rigid_kinetic_energy": (_i, [_vp, _vp]),
     "dpm_fire_minimize": (_i, [_vp, _d, _l, _vp]),
     "dpm_set_fire_kernel": (_i, [_vp, _i, _i]),
     "dpm_get_potential_energy": (_i, [_vp, _vp]),
@@ -175,11 +186,11 @@ class DPMSet:
     def field_size(self, name):
         n = {"v": self.nv, "r": self.np, "s": self.nsys}
         kind = {"pos": ("v", 2), "vel": ("v", 2), "force": ("v", 2), "lastpos": ("v", 2),
-                "wallforce": ("v", 2), "ppos": ("r", 2), "box": ("s", 2)}
+                "wallforce": ("v", 2), "ppos": ("r", 2), "box": ("s", 2), "pvel": ("r", 2), "pforce": ("r", 2)}
         if name in kind:
             k, c = kind[name]
             return n[k] * c
-        if name in ("a0", "area", "perimeter", "penergy", "pangle"):
+        if name in ("a0", "area", "perimeter", "penergy", "pangle", "pangvel", "ptorque", "prad"):
             return self.np
         return self.nv
 
@@ -332,6 +343,43 @@ class DPMSet:
         o = o.reshape(self.nsys, 5)
         return {"dt": o[:, 0], "alpha": o[:, 1], "numStep": o[:, 2].astype(np.int64), "P": o[:, 3],
                 "iters": o[:, 4].astype(np.int64)}
+
+    # ---------------------------------------------------------- rigid rings
+    def calc_rigid_force_energy(self):
+        """calcRigidForceEnergy (DPM2D.cu:2128-2142): ring force, torque / inertia, energies."""
+        self._chk(self.L.dpm_calc_rigid_force_energy(self.h), "calc_rigid_force_energy")
+
+    def init_rigid_langevin(self, T, gamma, read_state=False):
+        self._chk(self.L.dpm_init_rigid_langevin(self.h, T, gamma, int(read_state)), "init_rigid_langevin")
+
+    def init_rigid_nve(self, T, read_state=False):
+        self._chk(self.L.dpm_init_rigid_nve(self.h, T, int(read_state)), "init_rigid_nve")
+
+    def rigid_langevin(self, n):
+        self._chk(self.L.dpm_rigid_langevin_loop(self.h, n), "rigid_langevin_loop")
+
+    def rigid_nve(self, n):
+        self._chk(self.L.dpm_rigid_nve_loop(self.h, n), "rigid_nve_loop")
+
+    def init_rigid_fire(self, params7, min_step=20, num_step=0):
+        p = np.ascontiguousarray(params7, dtype=np.float64)
+        self._chk(self.L.dpm_init_rigid_fire(self.h, _p(p), min_step, num_step), "init_rigid_fire")
+
+    def rigid_fire(self, n):
+        self._chk(self.L.dpm_rigid_fire_loop(self.h, n), "rigid_fire_loop")
+
+    def rigid_fire_state(self):
+        o = np.zeros(5 * self.nsys)
+        self._chk(self.L.dpm_get_rigid_fire_state(self.h, _p(o)), "rigid_fire_state")
+        o = o.reshape(self.nsys, 5)
+        return {"dt": o[:, 0], "alpha": o[:, 1], "numStep": o[:, 2].astype(np.int64), "P": o[:, 3],
+                "iters": o[:, 4].astype(np.int64)}
+
+    def rigid_max_unbalanced_force(self):
+        return self._per_sys(self.L.dpm_get_rigid_max_unbalanced_force, "rigid_max_unbalanced_force")
+
+    def rigid_kinetic_energy(self):
+        return self._per_sys(self.L.dpm_get_rigid_kinetic_energy, "rigid_kinetic_energy")
 
     def set_fire_kernel(self, persistent=True, waves=0):
         """fire_minimize on one persistent workgroup per system, or on graphs of step kernels."""

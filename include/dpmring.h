@@ -59,7 +59,13 @@ enum {
   DPM_FIELD_LASTPOS = 15,  /* 2V  d_lastPos (neighbour-list reference positions)               */
   DPM_FIELD_BOX = 16,      /* 2B  setBoxSize / getBoxSize, DPM2D.h:275-276 (per system)        */
   DPM_FIELD_WALLFORCE = 17,/* 2V  d_wallForce (force on the walls per vertex), DPM2D.h:146       */
-  DPM_FIELD_COUNT = 18
+  DPM_FIELD_PVEL = 18,     /* 2R  setParticleVelocities / getParticleVelocities, DPM2D.h:339-340  */
+  DPM_FIELD_PANGVEL = 19,  /* R   getParticleAngularVelocities (rigid rings), DPM2D.h:351         */
+  DPM_FIELD_PFORCE = 20,   /* 2R  getParticleForces (rigid ring force), DPM2D.h:343               */
+  DPM_FIELD_PTORQUE = 21,  /* R   getParticleTorques (ring torque / moment of inertia), DPM2D.h:354 */
+  DPM_FIELD_VTORQUE = 22,  /* V   getVertexTorques (about the ring centre), DPM2D.h:373           */
+  DPM_FIELD_PRAD = 23,     /* R   setParticleRadii / getParticleRadii, DPM2D.h:327-328           */
+  DPM_FIELD_COUNT = 24
 };
 
 /* potential / interaction / monomer / particle enums keep simControlStruct's order, DPM2D.h:22-32 */
@@ -234,6 +240,34 @@ int dpm_scale_vertices(dpm_set_t h, const double* scalePerSystem);
  * round box the radius divides instead, with one scale for every system.  The neighbour list
  * is not rebuilt (call dpm_calc_neighbors, as the reference's drivers do). */
 int dpm_scale_box(dpm_set_t h, const double* scalePerSystem);
+
+/* ---- rigid rings (shapeEnum::rigid) ------------------------------------------------------ *
+ * Each ring moves as a rigid body: its centre (DPM_FIELD_PPOS), velocity (PVEL), angle (PANGLE)
+ * and angular velocity (PANGVEL) are driven by the sum of its vertices' interaction forces
+ * (PFORCE) and their torque about the centre divided by the moment of inertia sum |r|^2 (PTORQUE,
+ * an angular acceleration).  No shape forces, walls or activity; vertexVertex or vertexSmooth,
+ * periodic boxes. */
+/* calcRigidForceEnergy, DPM2D.h:567 / DPM2D.cu:2128-2142 */
+int dpm_calc_rigid_force_energy(dpm_set_t h);
+/* initRigidLangevin(T, gamma, readState) / rigidLangevinLoop(), DPM2D.h:693-695 /
+ * DPM2D.cu:3823-3847; Simulator.cu:241-313 */
+int dpm_init_rigid_langevin(dpm_set_t h, double T, double gamma, int readState);
+int dpm_rigid_langevin_loop(dpm_set_t h, long steps);
+/* initRigidNVE(T, readState) / rigidNVELoop(), DPM2D.h:697-699 / DPM2D.cu:3850-3868;
+ * Simulator.cu:316-371 */
+int dpm_init_rigid_nve(dpm_set_t h, double T, int readState);
+int dpm_rigid_nve_loop(dpm_set_t h, long steps);
+/* initRigidFIRE(params7, minStep, numStep, ...) / rigidFIRELoop(), DPM2D.h:675-677 /
+ * DPM2D.cu:3700-3710; FIRE.cu:173-298 (iterations per call) */
+int dpm_init_rigid_fire(dpm_set_t h, const double* params7, long minStep, long numStep);
+int dpm_rigid_fire_loop(dpm_set_t h, long iterations);
+/* rigid FIRE scalars per system: dt, a, numStep, last power v.F, iterations (FIRE.h:30-40);
+ * no reference counterpart (the reference keeps them in its FIRE object) */
+int dpm_get_rigid_fire_state(dpm_set_t h, double* out5PerSystem);
+/* getRigidMaxUnbalancedForce, DPM2D.h:660 / DPM2D.cu:3643-3652 (per system) */
+int dpm_get_rigid_max_unbalanced_force(dpm_set_t h, double* perSystem);
+/* getRigidKineticEnergy, DPM2D.h:425 / DPM2D.cu:1149-1159 (per system; needs DPM_FIELD_PRAD) */
+int dpm_get_rigid_kinetic_energy(dpm_set_t h, double* perSystem);
 
 /* ---- launch configuration and measurement (no reference counterpart) ------- */
 /* Wavefronts per ring tile in the step kernel (1..8): the gather over a tile's neighbour rows

@@ -49,6 +49,10 @@ struct orc_state {
   uint64_t seed;
   uint32_t systemId;
   uint64_t thermalCalls, injectCalls, activeCalls;
+  /* rigid rings (initRotationalVariables / initDeltaVariables, DPM2D.cu:197-221) */
+  double *particleVel, *particleAngvel, *particleForce, *particleTorque, *momentOfInertia;
+  double *particleLastPos, *particleLastAngle, *torque, *particleRad;
+  int rigidKind;            /* 1 Langevin, 2 NVE */
 };
 
 /* ------------------------------------------------------------------ RNG --- */
@@ -128,14 +132,21 @@ orc_state* orc_create(long numParticles, const long* nvp) {
   s->maxContactList = (long*)calloc(np, sizeof(long));
   s->initialPos = calloc(n2, 8);
   s->particleInitPos = calloc(2 * np, 8);
+  s->particleVel = calloc(2 * np, 8); s->particleAngvel = calloc(np, 8);
+  s->particleForce = calloc(2 * np, 8); s->particleTorque = calloc(np, 8);
+  s->momentOfInertia = calloc(np, 8); s->particleLastPos = calloc(2 * np, 8);
+  s->particleLastAngle = calloc(np, 8); s->torque = calloc(nv, 8); s->particleRad = calloc(np, 8);
   return s;
 }
 
 void orc_destroy(orc_state* s) {
+  if (!s) return;
   free(s->contactList); free(s->maxContactList); free(s->initialPos); free(s->particleInitPos);
   free(s->wallForce);
   free(s->l0Rest);
-  if (!s) return;
+  free(s->particleVel); free(s->particleAngvel); free(s->particleForce); free(s->particleTorque);
+  free(s->momentOfInertia); free(s->particleLastPos); free(s->particleLastAngle); free(s->torque);
+  free(s->particleRad);
   free(s->numVertexInParticleList); free(s->firstVertexInParticleId); free(s->particleIdList);
   free(s->pos); free(s->vel); free(s->force); free(s->lastPos); free(s->energy);
   free(s->rad); free(s->l0); free(s->theta0); free(s->theta); free(s->length);
@@ -160,6 +171,9 @@ static double* field_ptr(const orc_state* s, int f) {
     case ORC_F_PANGLE: return s->particleAngle; case ORC_F_LASTPOS: return s->lastPos;
     case ORC_F_BOX: return (double*)s->boxSize;
     case ORC_F_WALLFORCE: return s->wallForce;
+    case ORC_F_PVEL: return s->particleVel; case ORC_F_PANGVEL: return s->particleAngvel;
+    case ORC_F_PFORCE: return s->particleForce; case ORC_F_PTORQUE: return s->particleTorque;
+    case ORC_F_VTORQUE: return s->torque; case ORC_F_PRAD: return s->particleRad;
   }
   return NULL;
 }
@@ -169,10 +183,10 @@ long orc_field_size(const orc_state* s, int f) {
     case ORC_F_POS: case ORC_F_VEL: case ORC_F_FORCE: case ORC_F_LASTPOS: case ORC_F_WALLFORCE:
       return 2 * s->numVertices;
     case ORC_F_ENERGY: case ORC_F_RAD: case ORC_F_L0: case ORC_F_THETA0: case ORC_F_THETA:
-    case ORC_F_LENGTH: return s->numVertices;
+    case ORC_F_LENGTH: case ORC_F_VTORQUE: return s->numVertices;
     case ORC_F_A0: case ORC_F_AREA: case ORC_F_PERIMETER: case ORC_F_PENERGY:
-    case ORC_F_PANGLE: return s->numParticles;
-    case ORC_F_PPOS: return 2 * s->numParticles;
+    case ORC_F_PANGLE: case ORC_F_PANGVEL: case ORC_F_PTORQUE: case ORC_F_PRAD: return s->numParticles;
+    case ORC_F_PPOS: case ORC_F_PVEL: case ORC_F_PFORCE: return 2 * s->numParticles;
     case ORC_F_BOX: return 2;
   }
   return -1;
@@ -1103,6 +1117,244 @@ long orc_fire_minimize(orc_state* s, double tol, long maxIter) {
     it++;
   }
   return it;
+}
+
+/* ---------------------------------------------------------- rigid rings --- */
+/* calcRigidForceEnergy, DPM2D.cu:2128-2142: ring centres, then the interaction forces alone (no
+ * shape terms, no walls) and their transfer to the rings -- per-vertex torque r x f about the
+ * centre, moment of inertia sum |r|^2 (unit vertex masses), ring force sum f, ring torque
+ * sum (r x f) / I (an angular acceleration) and, for vertexVertex, the ring energy sum e
+ * (kernelCalcVertexForceTorque, kernelCalcTorqueAndMomentOfInertia,
+ * kernelCalcParticleRigidForceEnergy: cudaKernel.cuh:1412-1492).  vertexSmooth takes the
+ * deformable path's vertex-segment interaction (kernelCalcSmoothInteraction 922-975) with its
+ * ring energies, where the reference calls its unfinished kernelCalcVertexSmoothInteraction
+ * (1054-1141). */
+void orc_calc_rigid_force_energy(orc_state* s) {
+  long nv = s->numVertices, np = s->numParticles;
+  orc_calc_particle_positions(s);
+  memset(s->force, 0, sizeof(double) * 2 * nv);
+  memset(s->energy, 0, sizeof(double) * nv);
+  memset(s->particleEnergy, 0, sizeof(double) * np);
+  int smooth = s->interactionType == ORC_VERTEX_SMOOTH;
+  if (smooth) orc_calc_smooth_interaction(s);
+  else orc_calc_vertex_interaction(s);
+  for (long p = 0; p < np; p++) {
+    long first = s->firstVertexInParticleId[p], end = first + s->numVertexInParticleList[p];
+    double I = 0, F[2] = {0, 0}, T = 0, E = 0;
+    for (long v = first; v < end; v++) {
+      double r[2];
+      for (int dim = 0; dim < 2; dim++) r[dim] = pbcd(s, s->pos[v * 2 + dim], s->particlePos[p * 2 + dim], dim);
+      s->torque[v] = r[0] * s->force[v * 2 + 1] - r[1] * s->force[v * 2];
+      I += r[0] * r[0] + r[1] * r[1];
+    }
+    for (long v = first; v < end; v++) {
+      F[0] += s->force[v * 2];
+      F[1] += s->force[v * 2 + 1];
+      T += s->torque[v];
+      E += s->energy[v];
+    }
+    s->momentOfInertia[p] = I;
+    s->particleForce[p * 2] = F[0];
+    s->particleForce[p * 2 + 1] = F[1];
+    s->particleTorque[p] = T / I;
+    if (!smooth) s->particleEnergy[p] = E;
+  }
+}
+
+/* translateVertices + rotateVertices, DPM2D.cu:1834-1848 (kernelTranslateVertices
+ * cudaKernel.cuh:1678-1693, kernelRotateVertices 1695-1713): each ring moves by the box distance
+ * from its last centre to its centre, then turns about its centre by its angle change. */
+static void rigid_move_vertices(orc_state* s) {
+  long np = s->numParticles;
+  for (long p = 0; p < np; p++) {
+    double d[2];
+    for (int dim = 0; dim < 2; dim++) d[dim] = pbcd(s, s->particlePos[p * 2 + dim], s->particleLastPos[p * 2 + dim], dim);
+    long first = s->firstVertexInParticleId[p], end = first + s->numVertexInParticleList[p];
+    for (long v = first; v < end; v++)
+      for (int dim = 0; dim < 2; dim++) s->pos[v * 2 + dim] += d[dim];
+  }
+  for (long p = 0; p < np; p++) {
+    double dth = s->particleAngle[p] - s->particleLastAngle[p];
+    long first = s->firstVertexInParticleId[p], end = first + s->numVertexInParticleList[p];
+    for (long v = first; v < end; v++) {
+      double d[2], n[2];
+      for (int dim = 0; dim < 2; dim++) d[dim] = pbcd(s, s->pos[v * 2 + dim], s->particlePos[p * 2 + dim], dim);
+      n[0] = d[0] * cos(dth) - d[1] * sin(dth);
+      n[1] = d[0] * sin(dth) + d[1] * cos(dth);
+      for (int dim = 0; dim < 2; dim++) s->pos[v * 2 + dim] += (n[dim] - d[dim]);
+    }
+  }
+}
+
+/* initRigidLangevin / initRigidNVE, DPM2D.cu:3823-3868: angles, angular velocities, torques and
+ * last angles cleared (initRotationalVariables / initDeltaVariables, 197-221), last and initial
+ * positions stored, and unless the state is read, RigidLangevin::injectKineticEnergy
+ * (Simulator.cu:254-261): one Gaussian pair of amplitude sqrt(T) per ring, minus the mean ring
+ * velocity (kernelConserveParticleMomentum, cudaKernel.cuh:1998-2027, whose one block covers at
+ * most its block size of rings; the mean here is over all of them). */
+void orc_init_rigid(orc_state* s, double T, double gamma, int langevin, int readState) {
+  long np = s->numParticles;
+  s->rigidKind = langevin ? 1 : 2;
+  s->Tinject = T;
+  s->gamma = gamma;
+  s->noise = sqrt(2. * T * gamma);
+  memset(s->particleAngle, 0, sizeof(double) * np);
+  memset(s->particleAngvel, 0, sizeof(double) * np);
+  memset(s->particleTorque, 0, sizeof(double) * np);
+  memset(s->momentOfInertia, 0, sizeof(double) * np);
+  memset(s->particleLastAngle, 0, sizeof(double) * np);
+  memset(s->torque, 0, sizeof(double) * s->numVertices);
+  orc_reset_last_positions(s);
+  orc_set_initial_positions(s);
+  if (!readState) {
+    double amplitude = sqrt(T), m[2] = {0, 0};
+    uint64_t call = s->injectCalls++;
+    for (long p = 0; p < np; p++) {
+      double z0, z1;
+      orc_gauss_pair(s->seed, s->systemId, TAG_INJECT, call, (uint32_t)p, &z0, &z1);
+      s->particleVel[p * 2] = amplitude * z0;
+      s->particleVel[p * 2 + 1] = amplitude * z1;
+      m[0] += s->particleVel[p * 2];
+      m[1] += s->particleVel[p * 2 + 1];
+    }
+    m[0] /= np; m[1] /= np;
+    for (long p = 0; p < np; p++) { s->particleVel[p * 2] -= m[0]; s->particleVel[p * 2 + 1] -= m[1]; }
+  }
+}
+
+/* RigidLangevin::updateVelocity, Simulator.cu:263-286, with d_thermalVel zero; z is the step's
+ * unit Gaussian pair per ring.  RigidNVE::updateVelocity 348-366 when z is NULL. */
+static void rigid_kick(orc_state* s, double h, const double* z) {
+  long np = s->numParticles;
+  for (long p = 0; p < np; p++) {
+    for (int dim = 0; dim < 2; dim++) {
+      double* v = &s->particleVel[p * 2 + dim];
+      double F = s->particleForce[p * 2 + dim];
+      if (z) {
+        *v += h * (F - *v * s->gamma);
+        *v -= 0.5 * h * h * (F - *v * s->gamma) / s->gamma;
+        *v += z[p * 2 + dim] - 0.0;
+      } else {
+        *v += h * F;
+      }
+    }
+    s->particleAngvel[p] += h * s->particleTorque[p];
+  }
+}
+
+/* RigidLangevin::integrate / RigidNVE::integrate, Simulator.cu:241-252, 316-326.  The draw of
+ * Langevin::updateThermalVel (54-77) lands in the vertex forces, which the rigid force call
+ * overwrites; the rings feel only its unit Gaussian pair, added unscaled in both half kicks
+ * (RigidLangevin::updatePosition 288-307 adds d_rando, which stays zero). */
+void orc_rigid_loop(orc_state* s, long steps) {
+  long np = s->numParticles;
+  double* z = s->rigidKind == 1 ? (double*)malloc(sizeof(double) * 2 * np) : NULL;
+  for (long n = 0; n < steps; n++) {
+    memcpy(s->particleLastPos, s->particlePos, sizeof(double) * 2 * np);
+    memcpy(s->particleLastAngle, s->particleAngle, sizeof(double) * np);
+    if (z) {
+      uint64_t call = s->thermalCalls++;
+      for (long p = 0; p < np; p++)
+        orc_gauss_pair(s->seed, s->systemId, TAG_THERMAL, call, (uint32_t)p, &z[p * 2], &z[p * 2 + 1]);
+    }
+    rigid_kick(s, 0.5 * s->dt, z);
+    for (long p = 0; p < np; p++) {
+      for (int dim = 0; dim < 2; dim++) s->particlePos[p * 2 + dim] += s->dt * s->particleVel[p * 2 + dim];
+      s->particleAngle[p] += s->dt * s->particleAngvel[p];
+    }
+    rigid_move_vertices(s);
+    orc_check_neighbors(s);
+    orc_calc_rigid_force_energy(s);
+    rigid_kick(s, 0.5 * s->dt, z);
+  }
+  free(z);
+}
+
+/* initRigidFIRE, DPM2D.cu:3700-3706: initFIRE, rotational variables cleared, last centres
+ * stored */
+void orc_init_rigid_fire(orc_state* s, const double* p, long minStep, long numStep) {
+  long np = s->numParticles;
+  orc_init_fire(s, p, minStep, numStep);
+  memset(s->particleAngle, 0, sizeof(double) * np);
+  memset(s->particleAngvel, 0, sizeof(double) * np);
+  memset(s->particleTorque, 0, sizeof(double) * np);
+  memset(s->momentOfInertia, 0, sizeof(double) * np);
+  memset(s->particleLastAngle, 0, sizeof(double) * np);
+  memset(s->torque, 0, sizeof(double) * s->numVertices);
+  memcpy(s->particleLastPos, s->particlePos, sizeof(double) * 2 * np);
+}
+
+/* FIRE::minimizerRigidLoop, FIRE.cu:279-298, with checkRigidFIREParams (217-254) and
+ * bendRigidVelocityTowardsForce (256-277): power and bending over the ring velocities and forces
+ * only; a restart steps the centres and angles back by half the already-decreased step and
+ * clears both velocities.  The last centres and angles are stored at the start of every
+ * iteration, so the vertices follow that iteration's own move (the reference stores them only
+ * in initRigidFIRE, so each later iteration would re-apply every earlier move). */
+void orc_rigid_fire_loop(orc_state* s, long iterations) {
+  long np = s->numParticles;
+  for (long it = 0; it < iterations; it++) {
+    memcpy(s->particleLastPos, s->particlePos, sizeof(double) * 2 * np);
+    memcpy(s->particleLastAngle, s->particleAngle, sizeof(double) * np);
+    double vDotF = 0;
+    for (long i = 0; i < 2 * np; i++) vDotF += s->particleVel[i] * s->particleForce[i];
+    s->lastP = vDotF;
+    if (vDotF > 0) {
+      s->numStep++;
+      if (s->numStep > s->minStep) {
+        s->fire_dt = fmin(s->fire_dt * s->f_inc, s->fire_dt_max);
+        s->a *= s->f_a;
+      }
+    } else {
+      s->numStep = 0;
+      s->fire_dt = fmax(s->fire_dt * s->f_dec, s->fire_dt_max / 10);
+      s->a = s->a_start;
+      double h = 0.5 * s->fire_dt;
+      for (long p = 0; p < np; p++) {
+        for (int dim = 0; dim < 2; dim++) s->particlePos[p * 2 + dim] -= h * s->particleVel[p * 2 + dim];
+        s->particleAngle[p] -= h * s->particleAngvel[p];
+      }
+      memset(s->particleVel, 0, sizeof(double) * 2 * np);
+      memset(s->particleAngvel, 0, sizeof(double) * np);
+    }
+    rigid_kick(s, 0.5 * s->fire_dt, NULL);
+    double vv = 0, ff = 0;
+    for (long i = 0; i < 2 * np; i++) vv += s->particleVel[i] * s->particleVel[i];
+    for (long i = 0; i < 2 * np; i++) ff += s->particleForce[i] * s->particleForce[i];
+    if (ff > 0) {
+      double ratio = sqrt(vv / ff), fa = s->a;
+      for (long i = 0; i < 2 * np; i++)
+        s->particleVel[i] = (1 - fa) * s->particleVel[i] + fa * s->particleForce[i] * ratio;
+    }
+    for (long p = 0; p < np; p++) {
+      for (int dim = 0; dim < 2; dim++) s->particlePos[p * 2 + dim] += s->fire_dt * s->particleVel[p * 2 + dim];
+      s->particleAngle[p] += s->fire_dt * s->particleAngvel[p];
+    }
+    rigid_move_vertices(s);
+    orc_check_neighbors(s);
+    orc_calc_rigid_force_energy(s);
+    rigid_kick(s, 0.5 * s->fire_dt, NULL);
+  }
+}
+
+/* getRigidMaxUnbalancedForce, DPM2D.cu:3643-3652: the larger of the square roots of the largest
+ * squared ring force component and the largest squared ring torque */
+double orc_get_rigid_max_unbalanced_force(const orc_state* s) {
+  double mf = -1, mt = -1;
+  for (long i = 0; i < 2 * s->numParticles; i++) mf = fmax(mf, s->particleForce[i] * s->particleForce[i]);
+  for (long p = 0; p < s->numParticles; p++) mt = fmax(mt, s->particleTorque[p] * s->particleTorque[p]);
+  return fmax(sqrt(mf), sqrt(mt));
+}
+
+/* getRigidKineticEnergy, DPM2D.cu:1149-1159: 0.5 sum v^2 + 0.5 sum (omega R)^2 with R the ring's
+ * particle radius */
+double orc_get_rigid_kinetic_energy(const orc_state* s) {
+  double k = 0, r = 0;
+  for (long i = 0; i < 2 * s->numParticles; i++) k += s->particleVel[i] * s->particleVel[i];
+  for (long p = 0; p < s->numParticles; p++) {
+    double w = s->particleAngvel[p] * s->particleRad[p];
+    r += w * w;
+  }
+  return 0.5 * k + 0.5 * r;
 }
 
 /* ---------------------------------------------------------- observables --- */

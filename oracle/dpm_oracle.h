@@ -54,6 +54,7 @@ enum {
   ORC_F_POS = 0, ORC_F_VEL, ORC_F_FORCE, ORC_F_ENERGY, ORC_F_RAD, ORC_F_L0,
   ORC_F_THETA0, ORC_F_THETA, ORC_F_LENGTH, ORC_F_A0, ORC_F_AREA, ORC_F_PERIMETER,
   ORC_F_PPOS, ORC_F_PENERGY, ORC_F_PANGLE, ORC_F_LASTPOS, ORC_F_BOX, ORC_F_WALLFORCE,
+  ORC_F_PVEL, ORC_F_PANGVEL, ORC_F_PFORCE, ORC_F_PTORQUE, ORC_F_VTORQUE, ORC_F_PRAD,
   ORC_F_COUNT
 };
 
@@ -155,6 +156,14 @@ double orc_get_mean_vertex_radius(const orc_state* s);
 void orc_adjust_temperature(orc_state* s, double targetTemp);
 void orc_scale_vertices(orc_state* s, double scale);
 void orc_scale_box(orc_state* s, double scale);
+/* rigid rings */
+void orc_calc_rigid_force_energy(orc_state* s);
+void orc_init_rigid(orc_state* s, double T, double gamma, int langevin, int readState);
+void orc_rigid_loop(orc_state* s, long steps);
+void orc_init_rigid_fire(orc_state* s, const double* p, long minStep, long numStep);
+void orc_rigid_fire_loop(orc_state* s, long iterations);
+double orc_get_rigid_max_unbalanced_force(const orc_state* s);
+double orc_get_rigid_kinetic_energy(const orc_state* s);
 void orc_get_particle_shapes(orc_state* s, double* shape3);
 
 /* RNG: Philox4x32-10 (Salmon et al., SC'11) and the Box-Muller pair built on it */

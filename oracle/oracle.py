@@ -19,7 +19,8 @@ PART = {"passive": 0, "active": 1, "activeShape": 2}
 NOISE = {"langevin": 0, "baoab": 1, "brownian": 2, "drivenBrownian": 3}
 GEOM = {"normal": 0, "fixedBox": 1, "fixedSides": 2, "roundBox": 3}
 FIELDS = ["pos", "vel", "force", "energy", "rad", "l0", "theta0", "theta", "length",
-          "a0", "area", "perimeter", "ppos", "penergy", "pangle", "lastpos", "box", "wallforce"]
+          "a0", "area", "perimeter", "ppos", "penergy", "pangle", "lastpos", "box", "wallforce",
+          "pvel", "pangvel", "pforce", "ptorque", "vtorque", "prad"]
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 
 _lib = None
@@ -118,6 +119,13 @@ def lib():
             "orc_adjust_temperature": (None, [vp, d]),
             "orc_scale_vertices": (None, [vp, d]),
             "orc_scale_box": (None, [vp, d]),
+            "orc_calc_rigid_force_energy": (None, [vp]),
+            "orc_init_rigid": (None, [vp, d, d, i, i]),
+            "orc_rigid_loop": (None, [vp, l]),
+            "orc_init_rigid_fire": (None, [vp, vp, l, l]),
+            "orc_rigid_fire_loop": (None, [vp, l]),
+            "orc_get_rigid_max_unbalanced_force": (d, [vp]),
+            "orc_get_rigid_kinetic_energy": (d, [vp]),
             "orc_get_particle_shapes": (None, [vp, vp]),
             "orc_philox4x32_10": (None, [vp, vp, vp]),
             "orc_gauss_pair": (None, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, dp, dp]),
@@ -339,6 +347,32 @@ class Oracle:
 
     def box_radius(self):
         return self.L.orc_get_box_radius(self.h)
+
+    # ---- rigid rings
+    def calc_rigid_force_energy(self):
+        self.L.orc_calc_rigid_force_energy(self.h)
+
+    def init_rigid_langevin(self, T, gamma, read_state=False):
+        self.L.orc_init_rigid(self.h, T, gamma, 1, int(read_state))
+
+    def init_rigid_nve(self, T, read_state=False):
+        self.L.orc_init_rigid(self.h, T, 0.0, 0, int(read_state))
+
+    def rigid_loop(self, steps):
+        self.L.orc_rigid_loop(self.h, steps)
+
+    def init_rigid_fire(self, params, min_step=20, num_step=0):
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        self.L.orc_init_rigid_fire(self.h, _ptr(p), min_step, num_step)
+
+    def rigid_fire(self, iterations):
+        self.L.orc_rigid_fire_loop(self.h, iterations)
+
+    def rigid_max_unbalanced_force(self):
+        return self.L.orc_get_rigid_max_unbalanced_force(self.h)
+
+    def rigid_kinetic_energy(self):
+        return self.L.orc_get_rigid_kinetic_energy(self.h)
 
     def shapes(self):
         o = np.zeros(3 * self.np)
