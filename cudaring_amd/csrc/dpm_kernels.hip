@@ -612,12 +612,13 @@ __global__ void k_start_loop(View V, Params P, CellGrid G, double h, double dt, 
 // The rigid loop's first half step from the stored ring force and torque, one thread per ring
 // (RigidNVE / RigidLangevin::integrate, Simulator.cu:241-252, 316-326, up to the vertex move):
 // last centre and angle stored, half kick with the draw of call ctr[0] + ctrOffT, centre and
-// angle advanced, and each vertex binned at its start position into A.cellOut, moved into
-// A.xout and tested into A.flagOut; A.cellOther / A.flagClear are cleared.
+// angle advanced, and each vertex moved into A.xout and tested into A.flagOut (binned at its
+// start position into A.cellOut when given); A.cellOther (and A.flagClear when given) cleared.
 __global__ void k_rigid_start(View V, Params P, CellGrid G, StepArgs A) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   for (int c = t; c < G.ncells; c += gridDim.x * blockDim.x) A.cellOther[c] = 0;
-  for (int q = t; q < V.nsys; q += gridDim.x * blockDim.x) A.flagClear[q] = 0;
+  if (A.flagClear)
+    for (int q = t; q < V.nsys; q += gridDim.x * blockDim.x) A.flagClear[q] = 0;
   if (t >= V.np) return;
   const int r = t, s = V.rsys[r];
   const Box B = box_of(V, s);
@@ -633,7 +634,7 @@ __global__ void k_rigid_start(View V, Params P, CellGrid G, StepArgs A) {
   bool flag = false;
   for (int v = V.rfirst[r]; v < V.rfirst[r] + V.rnv[r]; v++) {
     const double x = V.x[v], y = V.y[v];
-    bin_vertex(G, B, A.cellOut, A.slotsOut, v, s, x, y, A.err);
+    if (A.cellOut) bin_vertex(G, B, A.cellOut, A.slotsOut, v, s, x, y, A.err);
     double xn, yn;
     rigid_move(B, x, y, cx, cy, ncx, ncy, c, sn, xn, yn);
     A.xout[v] = xn;
@@ -1471,8 +1472,7 @@ hipError_t launch_scale_box(dpm_set_s* S) {
 }
 
 hipError_t launch_rigid_start(dpm_set_s* S, bool langevin, const StepArgs& A) {
-  if (!A.ctr || !A.cellOut || !A.slotsOut || !A.cellOther || !A.flagClear || !A.flagOut || !A.anyOut || !A.xout ||
-      !A.yout)
+  if (!A.ctr || (A.cellOut && !A.slotsOut) || !A.cellOther || !A.flagOut || !A.anyOut || !A.xout || !A.yout)
     return hipErrorInvalidValue;
   StepArgs B = A;
   B.rigidLangevin = langevin ? 1 : 0;
