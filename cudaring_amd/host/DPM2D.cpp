@@ -768,3 +768,83 @@ void DPM2D::initNVERescale(double Temp) {
   cout << "DPM2D::initNVERescale:: current temperature: " << std::setprecision(12) << getTemperature() << endl;
 }
 void DPM2D::NVERescaleLoop() { sim_->integrate(); }
+
+// ---------------------------------------------------------------- rigid rings
+// setShapeType, DPM2D.cu:391-402
+void DPM2D::setShapeType(simControlStruct::shapeEnum t) { simControl.shapeType = t; }
+void DPM2D::setParticleRadii(std::vector<double>& r) {
+  need(r, numParticles, "setParticleRadii");
+  particleRad = r;
+  set_field(DPM_FIELD_PRAD, r);
+}
+std::vector<double> DPM2D::getParticleRadii() { return field(DPM_FIELD_PRAD, numParticles); }
+void DPM2D::setParticlePositions(std::vector<double>& p) {
+  need(p, 2 * numParticles, "setParticlePositions");
+  set_field(DPM_FIELD_PPOS, p);
+}
+void DPM2D::setParticleVelocities(std::vector<double>& v) {
+  need(v, 2 * numParticles, "setParticleVelocities");
+  set_field(DPM_FIELD_PVEL, v);
+}
+std::vector<double> DPM2D::getParticleVelocities() { return field(DPM_FIELD_PVEL, 2 * numParticles); }
+std::vector<double> DPM2D::getParticleForces() { return field(DPM_FIELD_PFORCE, 2 * numParticles); }
+std::vector<double> DPM2D::getParticleAngularVelocities() { return field(DPM_FIELD_PANGVEL, numParticles); }
+void DPM2D::setParticleAngularVelocities(std::vector<double>& w) {
+  need(w, numParticles, "setParticleAngularVelocities");
+  set_field(DPM_FIELD_PANGVEL, w);
+}
+std::vector<double> DPM2D::getParticleTorques() { return field(DPM_FIELD_PTORQUE, numParticles); }
+std::vector<double> DPM2D::getVertexTorques() { return field(DPM_FIELD_VTORQUE, numVertices); }
+
+void DPM2D::calcRigidForceEnergy() { check(dpm_calc_rigid_force_energy(handle()), "calcRigidForceEnergy"); }
+double DPM2D::getRigidMaxUnbalancedForce() {
+  return one(dpm_get_rigid_max_unbalanced_force, this, "getRigidMaxUnbalancedForce");
+}
+double DPM2D::getRigidKineticEnergy() { return one(dpm_get_rigid_kinetic_energy, this, "getRigidKineticEnergy"); }
+// getParticleTemperature, DPM2D.cu:1112-1114
+double DPM2D::getParticleTemperature() {
+  std::vector<double> v = getParticleVelocities();
+  double k = 0;
+  for (double x : v) k += x * x;
+  return 2 * (0.5 * k) / (numParticles * nDim);
+}
+// getRigidPhi, DPM2D.cu:1240-1250: rest areas over the box (or the round box)
+double DPM2D::getRigidPhi() {
+  std::vector<double> a0 = getRestAreas();
+  double a = std::accumulate(a0.begin(), a0.end(), 0.0);
+  if (simControl.geometryType == simControlStruct::geometryEnum::roundBox) return a / (PI * boxRadius * boxRadius);
+  return a / (boxSize[0] * boxSize[1]);
+}
+// getRigidWaveNumber, DPM2D.cu:1464-1466
+double DPM2D::getRigidWaveNumber() {
+  return PI / (2. * std::sqrt(boxSize[0] * boxSize[1] * getPreferredPhi() / (PI * numParticles)));
+}
+// initRigidLangevin / rigidLangevinLoop, DPM2D.cu:3823-3848
+void DPM2D::initRigidLangevin(double Temp, double gamma, bool readState) {
+  check(dpm_init_rigid_langevin(handle(), Temp, gamma, readState ? 1 : 0), "initRigidLangevin");
+  cout << "DPM2D::initRigidLangevin:: current temperature: " << std::setprecision(12) << getParticleTemperature()
+       << endl;
+}
+void DPM2D::rigidLangevinLoop() { rigidLangevinLoop(1); }
+void DPM2D::rigidLangevinLoop(long steps) { check(dpm_rigid_langevin_loop(handle(), steps), "rigidLangevinLoop"); }
+// initRigidNVE / rigidNVELoop, DPM2D.cu:3850-3872
+void DPM2D::initRigidNVE(double Temp, bool readState) {
+  check(dpm_init_rigid_nve(handle(), Temp, readState ? 1 : 0), "initRigidNVE");
+  cout << "DPM2D::initRigidNVE:: current temperature: " << std::setprecision(12) << getParticleTemperature()
+       << endl;
+}
+void DPM2D::rigidNVELoop() { rigidNVELoop(1); }
+void DPM2D::rigidNVELoop(long steps) { check(dpm_rigid_nve_loop(handle(), steps), "rigidNVELoop"); }
+// initRigidFIRE / rigidFIRELoop, DPM2D.cu:3700-3710 (numDOF and cutDist are kept by the
+// reference's FIRE object only for its own bookkeeping)
+void DPM2D::initRigidFIRE(std::vector<double>& p, long minStep_, long numStep_, long numDOF_, double cutDist_) {
+  (void)numDOF_;
+  (void)cutDist_;
+  if (p.size() != 7) {
+    cout << "DPM2D::initRigidFIRE: wrong number of FIRE parameters, must be 7" << endl;
+    return;
+  }
+  check(dpm_init_rigid_fire(handle(), p.data(), minStep_, numStep_), "initRigidFIRE");
+}
+void DPM2D::rigidFIRELoop() { rigidFIRELoop(1); }
+void DPM2D::rigidFIRELoop(long iterations) { check(dpm_rigid_fire_loop(handle(), iterations), "rigidFIRELoop"); }

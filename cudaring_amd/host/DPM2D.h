@@ -214,6 +214,34 @@ class DPM2D {
   void initNVERescale(double Temp);
   void NVERescaleLoop();
 
+  // ---- rigid rings (DPM2D.h:327-373, 425-469, 567, 660-699)
+  void setShapeType(simControlStruct::shapeEnum t);
+  void setParticleRadii(std::vector<double>& particleRad_);
+  std::vector<double> getParticleRadii();
+  void setParticlePositions(std::vector<double>& particlePos_);
+  void setParticleVelocities(std::vector<double>& particleVel_);
+  std::vector<double> getParticleVelocities();
+  std::vector<double> getParticleForces();
+  void setParticleAngularVelocities(std::vector<double>& particleAngvel_);
+  std::vector<double> getParticleAngularVelocities();
+  std::vector<double> getParticleTorques();
+  std::vector<double> getVertexTorques();
+  void calcRigidForceEnergy();
+  double getRigidMaxUnbalancedForce();
+  double getRigidKineticEnergy();
+  double getParticleTemperature();
+  double getRigidPhi();
+  double getRigidWaveNumber();
+  void initRigidLangevin(double Temp, double gamma, bool readState);
+  void rigidLangevinLoop();
+  void rigidLangevinLoop(long steps);
+  void initRigidNVE(double Temp, bool readState);
+  void rigidNVELoop();
+  void rigidNVELoop(long steps);
+  void initRigidFIRE(std::vector<double>& FIREparams, long minStep, long numStep, long numDOF, double cutDist);
+  void rigidFIRELoop();
+  void rigidFIRELoop(long iterations);
+
   long neighborListSize = 0;
 
  private:

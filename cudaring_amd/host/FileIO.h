@@ -227,6 +227,60 @@ class ioDPMFile {
     save1DFile(dir + "particleAngles.dat", dpm_->getParticleAngles());
   }
 
+  // ---- rigid rings: packing read (FileIO.h:328-381), packing save (437-458), state save (517-525)
+  void readRigidPackingFromDirectory(const std::string& dir, long numParticles_, long nDim_) {
+    std::vector<long> n = read1DIndexFile(dir + "numVertexInParticleList.dat", numParticles_);
+    dpm_->setNumVertexInParticleList(n);
+    long nv = std::accumulate(n.begin(), n.end(), 0L);
+    std::cout << "readRigidPackingFromDirectory:: numVertices: " << nv << " on device: " << dpm_->getNumVertices()
+              << std::endl;
+    std::vector<double> pos = read2DFile(dir + "positions.dat", nv);
+    dpm_->setVertexPositions(pos);
+    std::vector<double> rad = read1DFile(dir + "radii.dat", nv);
+    dpm_->setVertexRadii(rad);
+    std::vector<double> a0 = read1DFile(dir + "restAreas.dat", numParticles_);
+    dpm_->setRestAreas(a0);
+    std::vector<double> prad = read1DFile(dir + "particleRad.dat", numParticles_);
+    dpm_->setParticleRadii(prad);
+    std::vector<double> ppos = read2DFile(dir + "particlePos.dat", numParticles_);
+    dpm_->setParticlePositions(ppos);
+    std::vector<double> ang = read1DFile(dir + "particleAngles.dat", numParticles_);
+    dpm_->setParticleAngles(ang);
+    dpm_->setLengthScaleToOne();
+    if (dpm_->simControl.geometryType == simControlStruct::geometryEnum::roundBox) {
+      dpm_->setBoxRadius(read0DFile(dir + "boxSize.dat"));
+    } else {
+      std::vector<double> box = read1DFile(dir + "boxSize.dat", nDim_);
+      dpm_->setBoxSize(box);
+    }
+  }
+  void saveRigidPacking(const std::string& dir) {
+    std::ofstream p((dir + "params.dat").c_str());
+    p << "numParticles" << "\t" << dpm_->getNumParticles() << std::endl;
+    p << "phi" << "\t" << dpm_->getPhi() << std::endl;
+    p << "epot" << "\t" << dpm_->getPotentialEnergy() << std::endl;
+    p << "temperature" << "\t" << dpm_->getTemperature() << std::endl;
+    p.close();
+    save1DIndexFile(dir + "numVertexInParticleList.dat", dpm_->getNumVertexInParticleList());
+    save1DFile(dir + "radii.dat", dpm_->getVertexRadii());
+    save2DFile(dir + "forces.dat", dpm_->getVertexForces(), 2);
+    save2DFile(dir + "positions.dat", dpm_->getVertexPositions(), 2);
+    save1DFile(dir + "restAreas.dat", dpm_->getRestAreas());
+    save1DFile(dir + "particleRad.dat", dpm_->getParticleRadii());
+    save2DFile(dir + "particlePos.dat", dpm_->getParticlePositions(), 2);
+    save2DFile(dir + "particleVel.dat", dpm_->getParticleVelocities(), 2);
+    save1DFile(dir + "particleAngles.dat", dpm_->getParticleAngles());
+    save1DFile(dir + "particleAngvel.dat", dpm_->getParticleAngularVelocities());
+  }
+  void saveRigidState(const std::string& dir) {
+    save2DFile(dir + "positions.dat", dpm_->getVertexPositions(), 2);
+    save2DFile(dir + "velocities.dat", dpm_->getVertexVelocities(), 2);
+    save2DFile(dir + "particlePos.dat", dpm_->getParticlePositions(), 2);
+    save2DFile(dir + "particleVel.dat", dpm_->getParticleVelocities(), 2);
+    save1DFile(dir + "particleAngles.dat", dpm_->getParticleAngles());
+    save1DFile(dir + "particleAngvel.dat", dpm_->getParticleAngularVelocities());
+  }
+
   // ---- cell format (FileIO.h:611-704): NUMCL / PACKF / BOXSZ / STRSS header, then per ring
   // "CINFO n . . a0 area p0" and per vertex "VINFO . . x y rad l0 theta0 fx fy"
   void readPackingFromCellFormat(const std::string& name, long skiplines) {

@@ -120,3 +120,40 @@ def test_compress_nve_scale_box(packing_dir, tmp_path):
         assert e.shape[0] == 100 and int(e[0, 0]) == 1 and int(e[-1, 0]) == 199
         etot = e[:, 4]
         assert np.abs(etot - etot[0]).max() < 1e-2 * abs(etot[0]), d
+
+
+@pytest.mark.parametrize("mode", ["nvt", "fire"])
+def test_run_rigid_matches_oracle(packing_dir, tmp_path, oracle_mod, mode):
+    """runRigid reads the packing as rigid rings; the oracle replays the same calls"""
+    out = str(tmp_path / ("rigid_" + mode))
+    steps, dt_units, T, damping, seed = 100, 2e-4, 1e-3, 1.0, 9
+    log = run([os.path.join(BIN, "runRigid"), packing_dir, out, mode, str(dt_units), str(T), str(damping),
+               str(steps), "16", str(seed)])
+    st = state_from_dir(packing_dir)
+    o = oracle_mod.Oracle(st["nvp"])
+    o.load_state(st)
+    o.set("prad", load(os.path.join(packing_dir, "particleRad.dat")))
+    o.set_types("harmonic", "vertexVertex")
+    o.set_energy_costs(0, 0, 0, 1)
+    sigma = 2 * np.sqrt(np.mean(st["a0"]) / np.pi)
+    dt = dt_units * sigma
+    o.set_dt(dt)
+    o.set_rng(seed, 0)
+    o.calc_neighbor_list(o.set_displacement_cutoff(0.5, 2 * np.mean(st["rad"])))
+    o.calc_rigid_force_energy()
+    if mode == "nvt":
+        o.init_rigid_langevin(T, damping / sigma)
+        o.calc_rigid_force_energy()
+        o.rigid_loop(steps)
+    else:
+        o.init_rigid_fire([0.2, 0.5, 1.1, 0.99, dt, 10 * dt, 0.2], 20, 0)
+        o.calc_rigid_force_energy()
+        o.rigid_fire(steps)
+    pos = load(os.path.join(out, "positions.dat"), 2).ravel()
+    np.testing.assert_allclose(pos, o.get("pos"), rtol=0, atol=1e-10)
+    pvel = load(os.path.join(out, "particleVel.dat"), 2).ravel()
+    np.testing.assert_allclose(pvel, o.get("pvel"), rtol=1e-9, atol=1e-12)
+    e = load(os.path.join(out, "energy.dat"), 6)
+    assert e.shape[0] == 100 and int(e[-1, 0]) == steps
+    if mode == "nvt":
+        assert "T: " in log
