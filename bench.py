@@ -385,6 +385,53 @@ def integrators_bench(s, dt, steps):
     return out
 
 
+def rigid_bench(s, dt, steps):
+    """The benchmark packing as rigid rings (RigidNVE, RigidLangevin and rigid FIRE; Simulator.cu:
+    241-371, FIRE.cu:173-298): vertex-steps/s over `steps` steps (iterations) after 500 untimed
+    ones.  The rigid Langevin kick adds unit Gaussian draws to the ring velocities in both half
+    kicks whatever T and dt (per step v -> c^2 v + (1 + c) z, c = (1 - h gamma)(1 + h^2 / 2)), so
+    it runs at gamma = 1 / dt, where the ring speeds settle near 1.5 instead of ~sqrt(2 / (dt
+    gamma)) = 45 at gamma = 1."""
+    a0 = s.get("a0")
+    s.set("prad", np.sqrt(a0 / np.pi))
+    pos0 = s.get("pos")
+    cut = s.set_displacement_cutoff(SKIN, 2 * s.get("rad").mean())
+    out = {}
+    for kind in ("nve", "langevin", "fire"):
+        # every run from the benchmark packing at rest
+        s.set("pos", pos0)
+        s.set("pvel", np.zeros(2 * s.np))
+        s.calc_neighbor_list(cut)
+        s.calc_rigid_force_energy()
+        if kind == "nve":
+            s.init_rigid_nve(T)
+            loop = s.rigid_nve
+        elif kind == "langevin":
+            s.init_rigid_langevin(T, 1.0 / dt)
+            loop = s.rigid_langevin
+        else:
+            s.init_rigid_fire([0.2, 0.5, 1.1, 0.99, dt, 10 * dt, 0.2], 20, 0)
+            loop = s.rigid_fire
+        s.calc_rigid_force_energy()
+        try:
+            loop(500)
+            s.synchronize()
+            s.reset_update_count()
+            t0 = time.perf_counter()
+            loop(steps)
+            s.synchronize()
+            el = time.perf_counter() - t0
+        except Exception as exc:  # a run the list width cannot hold: report it, time the others
+            out[kind] = {"error": str(exc)}
+            continue
+        out[kind] = {"vertex_steps_per_s": round(s.nv * steps / el, 1), "step_us": round(1e6 * el / steps, 3),
+                     "rebuilds": int(s.update_count()[0]), "steps": steps,
+                     "finite": bool(np.all(np.isfinite(s.get("pos"))))}
+        if kind == "fire":
+            out[kind]["max_force_torque"] = float(s.rigid_max_unbalanced_force()[0])
+    return out
+
+
 COPY_ELEMS = 1 << 27  # float64 elements of the calibration copy: 1 GiB read + 1 GiB written
 
 
@@ -431,6 +478,8 @@ def main():
     ap.add_argument("--cfg5-iters", type=int, default=2000)
     ap.add_argument("--integrators", action="store_true",
                     help="the benchmark workload under each Langevin-loop integrator (5,000 steps each)")
+    ap.add_argument("--rigid", action="store_true",
+                    help="the benchmark packing as rigid rings: rigid NVE, Langevin and FIRE throughput")
     ap.add_argument("--traffic", action="store_true",
                     help="counter-collection run: a copy of known size, then step-kernel launches only")
     args = ap.parse_args()
@@ -456,6 +505,9 @@ def main():
         return
     if args.integrators:
         print(json.dumps({"integrators": integrators_bench(s, dt, 5000)}), flush=True)
+        return
+    if args.rigid:
+        print(json.dumps({"rigid": rigid_bench(s, dt, 5000)}), flush=True)
         return
 
     run_steps(s, args.warmup)

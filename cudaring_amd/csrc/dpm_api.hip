@@ -436,13 +436,14 @@ int run_graph(dpm_set_s* S, std::tuple<int, long, int> key, F seq) {
   if (S->graphVersion != S->version) drop_graphs(S);
   auto it = S->graphs.find(key);
   if (it == S->graphs.end()) {
-    const int cur0 = S->cur, f0 = S->fireCur;
+    const int cur0 = S->cur, f0 = S->fireCur, rf0 = S->rfireCur;
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
     int r2 = seq();
     hipError_t e = hipStreamEndCapture(S->stream, &g);
     S->cur = cur0;
     S->fireCur = f0;
+    S->rfireCur = rf0;
     set_pos_view(S);
     if (r2) return r2;
     HIPCHK(e);
@@ -801,18 +802,22 @@ int run_rigid(dpm_set_s* S, long steps, bool langevin) {
   return check_err_after_loop(S);
 }
 
-// rigid FIRE iterations: per iteration the ring update and vertex move of one workgroup per
-// system (into the other position buffer, with its flags), the bin launch and the step kernel
-// (rebuild of flagged rows, forces, ring sums, the second half kick)
+// rigid FIRE iterations: the tile partials of the current state, then per iteration the ring
+// update and vertex move (into the other position buffer, with its flags; the FIRE state flips
+// buffers), the bin launch and the step kernel (rebuild of flagged rows, forces, ring sums, the
+// second half kick, the next iteration's partials)
 int rigid_fire_sequence(dpm_set_s* S, long iterations) {
   HIPCHK(launch_fill_int(S->d_flag[1 - S->cur], 0, S->nsys, S->stream));
   HIPCHK(launch_fill_int(S->d_any + 1 - S->cur, 0, 1, S->stream));
+  HIPCHK(launch_rigid_fire_partials(S));
   for (long it = 0; it < iterations; it++) {
-    HIPCHK(launch_rigid_fire_update(S));
+    HIPCHK(launch_rigid_fire_move(S));
+    S->rfireCur = 1 - S->rfireCur;
     swap_pos(S);
     HIPCHK(launch_bin(S, 0, 0));
     StepArgs A = loop_args(S);
-    A.fire = S->d_rfire;
+    A.fire = S->d_rfire[S->rfireCur];
+    A.part = S->d_part;
     HIPCHK(launch_step(S, MODE_RIGID | MODE_FIRE, A));
   }
   return DPM_OK;
@@ -895,7 +900,7 @@ int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPer
   A(V.mbx, numSystems); A(V.mby, numSystems); A(V.wfx, nv); A(V.wfy, nv);
   A(V.pvx, np); A(V.pvy, np); A(V.pw, np); A(V.pfx, np); A(V.pfy, np); A(V.ptq, np); A(V.moi, np);
   A(V.prad, np); A(V.plx, np); A(V.ply, np); A(V.pla, np); A(V.vtq, nv);
-  A(S->d_rfire, numSystems);
+  A(S->d_rfire[0], numSystems); A(S->d_rfire[1], numSystems);
   A(V.svoff, numSystems + 1); A(V.sroff, numSystems + 1);
   A(V.ncnt, nv);
   A(V.tile_v0, V.ntiles); A(V.tile_v1, V.ntiles); A(V.tile_sys, V.ntiles);
@@ -982,7 +987,7 @@ int dpm_destroy(dpm_set_t h) {
                   S->d_contacts, S->d_ccount, S->d_x0, S->d_obs, S->mdCount[0], S->mdCount[1], S->mdCount[2],
                   S->mdSlots[0], S->mdSlots[1], S->mdSlots[2], S->mdFlag[0], S->mdFlag[1], S->mdFlag[2],
                   S->mdFlag[3], V.pvx, V.pvy, V.pw, V.pfx, V.pfy, V.ptq, V.moi, V.prad, V.plx, V.ply,
-                  V.pla, V.vtq, S->d_rfire};
+                  V.pla, V.vtq, S->d_rfire[0], S->d_rfire[1]};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (S->ownStream) hipStreamDestroy(S->ownStream);
@@ -2165,7 +2170,9 @@ int dpm_init_rigid_fire(dpm_set_t h, const double* p, long minStep, long numStep
   for (auto& f : fs) {
     f.dt = p[4]; f.a = p[6]; f.P = 0; f.numStep = numStep; f.iters = 0; f.converged = 0; f.pad = 0;
   }
-  HIPCHK(hipMemcpyAsync(S->d_rfire, fs.data(), sizeof(FireState) * S->nsys, hipMemcpyHostToDevice, S->stream));
+  for (int b = 0; b < 2; b++)
+    HIPCHK(hipMemcpyAsync(S->d_rfire[b], fs.data(), sizeof(FireState) * S->nsys, hipMemcpyHostToDevice, S->stream));
+  S->rfireCur = 0;
   int rc = rigid_clear(S);
   if (rc) return rc;
   S->rigidFireInit = true;
@@ -2185,8 +2192,13 @@ int dpm_rigid_fire_loop(dpm_set_t h, long iterations) {
   if (iterations < 4 || !graphs_enabled()) {
     rc = rigid_fire_sequence(S, iterations);
   } else {
-    rc = run_graph(S, std::make_tuple(22, iterations, S->cur), [&]() { return rigid_fire_sequence(S, iterations); });
-    if (!rc && (iterations % 2) == 1) swap_pos(S);
+    // the sequence flips the position and FIRE-state buffers once per iteration
+    rc = run_graph(S, std::make_tuple(22 + S->rfireCur, iterations, S->cur),
+                   [&]() { return rigid_fire_sequence(S, iterations); });
+    if (!rc && (iterations % 2) == 1) {
+      swap_pos(S);
+      S->rfireCur = 1 - S->rfireCur;
+    }
   }
   if (rc) return rc;
   return check_err_after_loop(S);
@@ -2197,7 +2209,8 @@ int dpm_get_rigid_fire_state(dpm_set_t h, double* out5) {
   CHECK_HANDLE(h);
   if (!out5) return fail(S, DPM_ERR_ARG, "null output");
   std::vector<FireState> fs(S->nsys);
-  HIPCHK(hipMemcpyAsync(fs.data(), S->d_rfire, sizeof(FireState) * S->nsys, hipMemcpyDeviceToHost, S->stream));
+  HIPCHK(hipMemcpyAsync(fs.data(), S->d_rfire[S->rfireCur], sizeof(FireState) * S->nsys, hipMemcpyDeviceToHost,
+                        S->stream));
   HIPCHK(hipStreamSynchronize(S->stream));
   for (long s = 0; s < S->nsys; s++) {
     out5[5 * s] = fs[s].dt;

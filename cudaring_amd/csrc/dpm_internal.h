@@ -153,7 +153,8 @@ struct dpm_set_s {
   // integrators
   int simKind = 0;  // 1 langevin, 2 nve, 3 nve rescale
   int rigidKind = 0;  // rigid rings: 1 langevin, 2 nve
-  dpm::FireState* d_rfire = nullptr;  // rigid FIRE state per system
+  dpm::FireState* d_rfire[2] = {nullptr, nullptr};  // rigid FIRE state per system, by iteration parity
+  int rfireCur = 0;
   bool rigidFireInit = false;
   int noiseType = 0;  // integrator of the Langevin loop (NOISE_*)
   // observables: contact rows (np x contactWidth, -1 padded) and counts, initial positions of
@@ -240,6 +241,7 @@ hipError_t launch_particle_stress(dpm_set_s* S, double* out);
 hipError_t launch_copy_d(double* dst, const double* src, long n, hipStream_t st);
 hipError_t launch_rigid_start(dpm_set_s* S, bool langevin, const StepArgs& A);
 hipError_t launch_rigid_inject(dpm_set_s* S, double amplitude, unsigned long long call);
-hipError_t launch_rigid_fire_update(dpm_set_s* S);
+hipError_t launch_rigid_fire_partials(dpm_set_s* S);
+hipError_t launch_rigid_fire_move(dpm_set_s* S);
 
 }  // namespace dpm

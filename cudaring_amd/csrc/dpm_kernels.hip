@@ -267,6 +267,20 @@ __device__ __forceinline__ void rigid_move(const Box& B, double x, double y, dou
   yn = y1 + ((dx * sn + dy * c) - dy);
 }
 
+// Per-tile partials of the rigid FIRE sums over the rings whose first vertex is in the tile:
+// [0] v.F, [1] |v|^2, [2] |F|^2 (ring velocities and forces)
+__device__ __forceinline__ void rigid_fire_partials(const Lane& L, bool head, double vx, double vy, double Fx,
+                                                    double Fy, double* part) {
+  const double p0 = wave_sum(head ? vx * Fx + vy * Fy : 0.0);
+  const double p1 = wave_sum(head ? vx * vx + vy * vy : 0.0);
+  const double p2 = wave_sum(head ? Fx * Fx + Fy * Fy : 0.0);
+  if (L.lane == 0) {
+    part[L.tile * 4 + 0] = p0;
+    part[L.tile * 4 + 1] = p1;
+    part[L.tile * 4 + 2] = p2;
+  }
+}
+
 // Step-kernel epilogue of rigid rings (wave 0, one ring per lane segment): the interaction
 // forces' transfer to the ring (calcRigidForceEnergy, DPM2D.cu:2128-2142: torque r x f about
 // the centre, moment of inertia sum |r|^2, ring force sum f, ring torque sum (r x f) / I, ring
@@ -286,8 +300,22 @@ __device__ __forceinline__ void rigid_epilogue(const View& V, const Params& P, c
   const double T = seg_sum(L.act ? tq : 0.0, L.lane, L.segS, L.segE) / I;
   double E = 0;
   if constexpr (VV) E = seg_sum(L.act ? e : 0.0, L.lane, L.segS, L.segE);
+  const bool head = L.act && L.lane == L.segS;
+  double vx = 0, vy = 0, w = 0;
+  if constexpr (MODE != MODE_RIGID) {
+    if (L.act) {
+      vx = V.pvx[L.r];
+      vy = V.pvy[L.r];
+      w = V.pw[L.r];
+    }
+  }
+  if constexpr ((MODE & MODE_FIRE) != 0) {
+    // the second half kick with the system's FIRE step, then the tile partials of the next
+    // iteration's sums (every lane takes part in the wave sums)
+    if (L.act) rigid_kick(false, 0.5 * A.fire[L.s].dt, 0.0, Fx, Fy, T, 0.0, 0.0, vx, vy, w);
+    rigid_fire_partials(L, head, vx, vy, Fx, Fy, A.part);
+  }
   if (!L.act) return;
-  const bool head = L.lane == L.segS;
   V.fx[L.v] = fx;
   V.fy[L.v] = fy;
   V.ev[L.v] = e;
@@ -300,32 +328,31 @@ __device__ __forceinline__ void rigid_epilogue(const View& V, const Params& P, c
     if constexpr (VV) V.pe[L.r] = E;
   }
   if constexpr (MODE == MODE_RIGID) return;
-  double vx = V.pvx[L.r], vy = V.pvy[L.r], w = V.pw[L.r];
-  double h = A.h;
-  if constexpr ((MODE & MODE_FIRE) != 0) h = 0.5 * A.fire[L.s].dt;
-  rigid_kick(A.rigidLangevin, h, P.gamma, Fx, Fy, T, z0, z1, vx, vy, w);
-  if constexpr ((MODE & MODE_START) != 0) {
-    double n0 = 0, n1 = 0;
-    if (A.rigidLangevin)
-      gauss_pair(P.seed, (uint32_t)L.s, TAG_THERMAL, A.ctr[0] + (unsigned long long)A.ctrOffT + 1ull,
-                 (uint32_t)(L.r - V.sroff[L.s]), n0, n1);
-    rigid_kick(A.rigidLangevin, A.h, P.gamma, Fx, Fy, T, n0, n1, vx, vy, w);
-    const double ang = V.ang[L.r];
-    const double ncx = cx + A.dt * vx, ncy = cy + A.dt * vy, nang = ang + A.dt * w;
-    const double dth = nang - ang;
-    double xn, yn;
-    rigid_move(B, x, y, cx, cy, ncx, ncy, cos(dth), sin(dth), xn, yn);
-    A.xout[L.v] = xn;
-    A.yout[L.v] = yn;
-    const double ddx = pbx(xn - lx, B), ddy = pby(yn - ly, B);
-    if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) {
-      atomicOr(&A.flagOut[L.s], 1);
-      atomicOr(A.anyOut, 1);
-    }
-    if (head) {
-      V.ang[L.r] = nang;
-      V.cx[L.r] = ncx;
-      V.cy[L.r] = ncy;
+  if constexpr ((MODE & MODE_FIRE) == 0) {
+    rigid_kick(A.rigidLangevin, A.h, P.gamma, Fx, Fy, T, z0, z1, vx, vy, w);
+    if constexpr ((MODE & MODE_START) != 0) {
+      double n0 = 0, n1 = 0;
+      if (A.rigidLangevin)
+        gauss_pair(P.seed, (uint32_t)L.s, TAG_THERMAL, A.ctr[0] + (unsigned long long)A.ctrOffT + 1ull,
+                   (uint32_t)(L.r - V.sroff[L.s]), n0, n1);
+      rigid_kick(A.rigidLangevin, A.h, P.gamma, Fx, Fy, T, n0, n1, vx, vy, w);
+      const double ang = V.ang[L.r];
+      const double ncx = cx + A.dt * vx, ncy = cy + A.dt * vy, nang = ang + A.dt * w;
+      const double dth = nang - ang;
+      double xn, yn;
+      rigid_move(B, x, y, cx, cy, ncx, ncy, cos(dth), sin(dth), xn, yn);
+      A.xout[L.v] = xn;
+      A.yout[L.v] = yn;
+      const double ddx = pbx(xn - lx, B), ddy = pby(yn - ly, B);
+      if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) {
+        atomicOr(&A.flagOut[L.s], 1);
+        atomicOr(A.anyOut, 1);
+      }
+      if (head) {
+        V.ang[L.r] = nang;
+        V.cx[L.r] = ncx;
+        V.cy[L.r] = ncy;
+      }
     }
   }
   if (head) {
@@ -694,117 +721,6 @@ __global__ __launch_bounds__(256) void k_rigid_inject(View V, Params P, double a
   }
 }
 
-// workgroup sums of two values (every thread gets both)
-__device__ __forceinline__ void block_sum2(double a, double b, double (*red)[256 / WAVE], double& sa, double& sb) {
-  const int tid = threadIdx.x;
-  a = wave_sum(a);
-  b = wave_sum(b);
-  __syncthreads();
-  if ((tid & (WAVE - 1)) == 0) {
-    red[0][tid / WAVE] = a;
-    red[1][tid / WAVE] = b;
-  }
-  __syncthreads();
-  sa = 0;
-  sb = 0;
-  for (int q = 0; q < (int)(blockDim.x / WAVE); q++) {
-    sa += red[0][q];
-    sb += red[1][q];
-  }
-}
-
-// One rigid FIRE iteration up to the vertex move, one workgroup per system (FIRE::
-// minimizerRigidLoop, FIRE.cu:279-291, with checkRigidFIREParams 217-254 and
-// bendRigidVelocityTowardsForce 256-277): power v.F over the ring velocities and forces, the
-// FIRE scalars' update, on a restart the step back of centres and angles by half the
-// already-decreased step with both velocities cleared, the half kick, the bend of the ring
-// velocities, the move of centres and angles, then every vertex moved into (xo, yo) with its
-// displacement test.  The last centres and angles are stored at the start of the iteration.
-__global__ __launch_bounds__(256) void k_rigid_fire_update(View V, Params P, FireParams fp, FireState* fs,
-                                                           double* xo, double* yo, int* flagOut, int* anyOut) {
-  __shared__ double red[2][256 / WAVE];
-  const int s = blockIdx.x, tid = threadIdx.x;
-  const int r0 = V.sroff[s], r1 = V.sroff[s + 1];
-  double p = 0;
-  for (int r = r0 + tid; r < r1; r += blockDim.x) p += V.pvx[r] * V.pfx[r] + V.pvy[r] * V.pfy[r];
-  double P0, unused;
-  block_sum2(p, 0.0, red, P0, unused);
-  FireState f = fs[s];
-  const bool restart = !(P0 > 0);
-  if (!restart) {
-    f.numStep++;
-    if (f.numStep > fp.minStep) {
-      f.dt = fmin(f.dt * fp.f_inc, fp.dt_max);
-      f.a *= fp.f_a;
-    }
-  } else {
-    f.numStep = 0;
-    f.dt = fmax(f.dt * fp.f_dec, fp.dt_max / 10);
-    f.a = fp.a_start;
-  }
-  const double h = 0.5 * f.dt;
-  double vv = 0, ff = 0;
-  for (int r = r0 + tid; r < r1; r += blockDim.x) {
-    double cx = V.cx[r], cy = V.cy[r], ang = V.ang[r];
-    double vx = V.pvx[r], vy = V.pvy[r], w = V.pw[r];
-    V.plx[r] = cx;
-    V.ply[r] = cy;
-    V.pla[r] = ang;
-    if (restart) {
-      cx -= h * vx;
-      cy -= h * vy;
-      ang -= h * w;
-      vx = vy = w = 0;
-    }
-    rigid_kick(false, h, 0.0, V.pfx[r], V.pfy[r], V.ptq[r], 0.0, 0.0, vx, vy, w);
-    V.cx[r] = cx;
-    V.cy[r] = cy;
-    V.ang[r] = ang;
-    V.pvx[r] = vx;
-    V.pvy[r] = vy;
-    V.pw[r] = w;
-    vv += vx * vx + vy * vy;
-    ff += V.pfx[r] * V.pfx[r] + V.pfy[r] * V.pfy[r];
-  }
-  double VV, FF;
-  block_sum2(vv, ff, red, VV, FF);
-  for (int r = r0 + tid; r < r1; r += blockDim.x) {
-    double vx = V.pvx[r], vy = V.pvy[r];
-    if (FF > 0) {
-      const double ratio = sqrt(VV / FF);
-      vx = (1 - f.a) * vx + f.a * V.pfx[r] * ratio;
-      vy = (1 - f.a) * vy + f.a * V.pfy[r] * ratio;
-    }
-    V.pvx[r] = vx;
-    V.pvy[r] = vy;
-    V.cx[r] += f.dt * vx;
-    V.cy[r] += f.dt * vy;
-    V.ang[r] += f.dt * V.pw[r];
-  }
-  __syncthreads();
-  const Box B = box_of(V, s);
-  bool flag = false;
-  for (int v = V.svoff[s] + tid; v < V.svoff[s + 1]; v += blockDim.x) {
-    const int r = V.vring[v];
-    const double dth = V.ang[r] - V.pla[r];
-    double xn, yn;
-    rigid_move(B, V.x[v], V.y[v], V.plx[r], V.ply[r], V.cx[r], V.cy[r], cos(dth), sin(dth), xn, yn);
-    xo[v] = xn;
-    yo[v] = yn;
-    const double ddx = pbx(xn - V.lx[v], B), ddy = pby(yn - V.ly[v], B);
-    flag = flag || 3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff;
-  }
-  if (flag) {
-    atomicOr(&flagOut[s], 1);
-    atomicOr(anyOut, 1);
-  }
-  if (tid == 0) {
-    f.P = P0;
-    f.iters++;
-    fs[s] = f;
-  }
-}
-
 // the current positions into a cell list with their displacement test (the overdamped loops'
 // prologue: their first step rebuilds from these when flagged)
 __global__ void k_bin_flag(View V, Params P, CellGrid G, int* count, int* slots, int* flag, int* err) {
@@ -970,6 +886,103 @@ __device__ __forceinline__ double sys_max(const double* part, int q, int t0, int
   double acc = -1;
   for (int t = t0 + lane; t < t1; t += WAVE) acc = fmax(acc, part[t * 4 + q]);
   return wave_max(acc);
+}
+
+// --------------------------------------------------------- rigid FIRE ---
+__global__ void k_rigid_fire_partials(View V, double* part) {
+  Lane L;
+  if (!wave_tile_ctx(V, L)) return;
+  const bool head = L.act && L.lane == L.segS;
+  double vx = 0, vy = 0, Fx = 0, Fy = 0;
+  if (head) {
+    vx = V.pvx[L.r];
+    vy = V.pvy[L.r];
+    Fx = V.pfx[L.r];
+    Fy = V.pfy[L.r];
+  }
+  rigid_fire_partials(L, head, vx, vy, Fx, Fy, part);
+}
+
+// One rigid FIRE iteration up to the vertex move, one wave per tile (FIRE::minimizerRigidLoop,
+// FIRE.cu:279-291, with checkRigidFIREParams 217-254 and bendRigidVelocityTowardsForce 256-277).
+// Every wave derives its system's FIRE update from the tile partials of the state after the
+// previous iteration (power P = sum v.F decides; on a restart the centres and angles step back by
+// half the already-decreased step and both velocities clear), and the bend ratio from
+// |v'|^2 = h^2 |F|^2 after a restart, else |v|^2 + 2 h P + h^2 |F|^2, the half-kicked velocities'
+// norm.  Each lane moves its ring redundantly and its vertex with it; the ring's first lane
+// stores the ring, the system's first tile the new FIRE state (fout; fin stays as it was).
+// The last centres and angles are the ones at the start of the iteration.
+__global__ void k_rigid_fire_move(View V, Params P, FireParams fp, const FireState* fin, FireState* fout,
+                                  const double* part, double* xo, double* yo, int* flagOut, int* anyOut) {
+  Lane L;
+  if (!wave_tile_ctx(V, L)) return;
+  const int t0 = V.stile[L.s], t1 = V.stile[L.s + 1];
+  const double P0 = sys_sum(part, 0, t0, t1, L.lane);
+  const double VV = sys_sum(part, 1, t0, t1, L.lane);
+  const double FF = sys_sum(part, 2, t0, t1, L.lane);
+  FireState f = fin[L.s];
+  const bool restart = !(P0 > 0);
+  if (!restart) {
+    f.numStep++;
+    if (f.numStep > fp.minStep) {
+      f.dt = fmin(f.dt * fp.f_inc, fp.dt_max);
+      f.a *= fp.f_a;
+    }
+  } else {
+    f.numStep = 0;
+    f.dt = fmax(f.dt * fp.f_dec, fp.dt_max / 10);
+    f.a = fp.a_start;
+  }
+  const double h = 0.5 * f.dt;
+  if (L.tile == t0 && L.lane == 0) {
+    FireState o = f;
+    o.P = P0;
+    o.iters = f.iters + 1;
+    fout[L.s] = o;
+  }
+  if (!L.act) return;
+  const double VV1 = restart ? h * h * FF : VV + 2 * h * P0 + h * h * FF;
+  double cx = V.cx[L.r], cy = V.cy[L.r], ang = V.ang[L.r];
+  double vx = V.pvx[L.r], vy = V.pvy[L.r], w = V.pw[L.r];
+  const double Fx = V.pfx[L.r], Fy = V.pfy[L.r];
+  const double lcx = cx, lcy = cy, lang = ang;
+  if (restart) {
+    cx -= h * vx;
+    cy -= h * vy;
+    ang -= h * w;
+    vx = vy = w = 0;
+  }
+  rigid_kick(false, h, 0.0, Fx, Fy, V.ptq[L.r], 0.0, 0.0, vx, vy, w);
+  if (FF > 0) {
+    const double ratio = sqrt(VV1 / FF);
+    vx = (1 - f.a) * vx + f.a * Fx * ratio;
+    vy = (1 - f.a) * vy + f.a * Fy * ratio;
+  }
+  cx += f.dt * vx;
+  cy += f.dt * vy;
+  ang += f.dt * w;
+  const Box B = box_of(V, L.s);
+  const double dth = ang - lang;
+  double xn, yn;
+  rigid_move(B, V.x[L.v], V.y[L.v], lcx, lcy, cx, cy, cos(dth), sin(dth), xn, yn);
+  xo[L.v] = xn;
+  yo[L.v] = yn;
+  const double ddx = pbx(xn - V.lx[L.v], B), ddy = pby(yn - V.ly[L.v], B);
+  if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) {
+    atomicOr(&flagOut[L.s], 1);
+    atomicOr(anyOut, 1);
+  }
+  if (L.lane == L.segS) {
+    V.plx[L.r] = lcx;
+    V.ply[L.r] = lcy;
+    V.pla[L.r] = lang;
+    V.cx[L.r] = cx;
+    V.cy[L.r] = cy;
+    V.ang[L.r] = ang;
+    V.pvx[L.r] = vx;
+    V.pvy[L.r] = vy;
+    V.pw[L.r] = w;
+  }
 }
 
 // FIRE::minimizerVertexLoop, FIRE.cu:397-411, split at its two global reductions.
@@ -1355,7 +1368,7 @@ hipError_t launch_step(dpm_set_s* S, int mode, const StepArgs& A) {
   // refuse argument sets the chosen epilogue would dereference as null
   if (!A.ctr) return hipErrorInvalidValue;
   if (S->P.particle == PART_ACTIVE_SHAPE && !S->V.l0r) return hipErrorInvalidValue;
-  if ((mode & MODE_FIRE) && (!A.fire || (!A.part && !(mode & MODE_RIGID)))) return hipErrorInvalidValue;
+  if ((mode & MODE_FIRE) && (!A.fire || !A.part)) return hipErrorInvalidValue;
   if ((mode & MODE_RIGID) && S->P.interaction == INT_ALL) return hipErrorInvalidValue;
   if ((mode & MODE_START) && (!A.xout || !A.yout || !A.flagOut || !A.anyOut)) return hipErrorInvalidValue;
   if (A.manage && (!A.flagCur || !A.cellCur || !A.cellOther || !A.err || !A.maxcnt || !A.updateCount))
@@ -1487,10 +1500,16 @@ hipError_t launch_rigid_inject(dpm_set_s* S, double amplitude, unsigned long lon
   return hipGetLastError();
 }
 
-hipError_t launch_rigid_fire_update(dpm_set_s* S) {
+hipError_t launch_rigid_fire_partials(dpm_set_s* S) {
+  k_rigid_fire_partials<<<grid_w(S->V.ntiles), 256, 0, S->stream>>>(S->V, S->d_part);
+  return hipGetLastError();
+}
+
+hipError_t launch_rigid_fire_move(dpm_set_s* S) {
   const int nb = 1 - S->cur;
-  k_rigid_fire_update<<<(int)S->nsys, 256, 0, S->stream>>>(S->V, S->P, S->fp, S->d_rfire, S->pos[nb][0],
-                                                            S->pos[nb][1], S->d_flag[nb], S->d_any + nb);
+  k_rigid_fire_move<<<grid_w(S->V.ntiles), 256, 0, S->stream>>>(
+      S->V, S->P, S->fp, S->d_rfire[S->rfireCur], S->d_rfire[1 - S->rfireCur], S->d_part, S->pos[nb][0],
+      S->pos[nb][1], S->d_flag[nb], S->d_any + nb);
   return hipGetLastError();
 }
 
