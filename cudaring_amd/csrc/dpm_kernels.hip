@@ -129,6 +129,29 @@ __device__ __forceinline__ ShapeOut ring_shape(const View& V, const Params& P, c
   return o;
 }
 
+// The ring centre alone (calcParticlePos, cudaKernel.cuh:365-388), as ring_shape computes it,
+// stored by the ring's first lane: the rigid rings' shape pass
+__device__ __forceinline__ ShapeOut ring_centre(const View& V, const Lane& L, const Box& B, double x, double y) {
+  ShapeOut o{0, 0, 0, 0, 0};
+  const double xn = shfl_d(x, L.nl), yn = shfl_d(y, L.nl);
+  const double xf = shfl_d(x, L.segS), yf = shfl_d(y, L.segS);
+  const double dnx = pbx(xn - x, B), dny = pby(yn - y, B);
+  const double Ix = seg_scan(L.act ? dnx : 0.0, L.lane, L.segS);
+  const double Iy = seg_scan(L.act ? dny : 0.0, L.lane, L.segS);
+  const double ux = L.act ? Ix - dnx : 0.0;
+  const double uy = L.act ? Iy - dny : 0.0;
+  const double sux = seg_sum(ux, L.lane, L.segS, L.segE);
+  const double suy = seg_sum(uy, L.lane, L.segS, L.segE);
+  const double ind = 1.0 / (double)L.n;
+  o.cx = xf + sux * ind;
+  o.cy = yf + suy * ind;
+  if (L.act && L.lane == L.segS) {
+    V.cx[L.r] = o.cx;
+    V.cy[L.r] = o.cy;
+  }
+  return o;
+}
+
 // --------------------------------------------------------- neighbours ---
 // cell_of, prev_pair, row_insert and build_row live in dpm_rows.h (shared with the adjoint).
 
@@ -401,7 +424,7 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
   // wave 0's epilogue operands, fetched before the force work (lastPos after the rebuild above)
   double vx = 0, vy = 0, lx = 0, ly = 0;
   if (w == 0) {
-    if constexpr (MODE != MODE_FORCE) {
+    if constexpr (MODE != MODE_FORCE && (MODE & MODE_RIGID) == 0) {
       vx = V.vx[L.v];
       vy = V.vy[L.v];
     }
@@ -430,7 +453,8 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
   ShapeOut so{0, 0, 0};
   double zt0 = 0, zt1 = 0, ang = 0;
   if (w == 0) {
-    so = ring_shape(V, P, L, B, x, y, true);
+    if constexpr ((MODE & MODE_RIGID) != 0) so = ring_centre(V, L, B, x, y);
+    else so = ring_shape(V, P, L, B, x, y, true);
     // one thermal draw per integrate call and ring: Langevin::updateThermalVel (Simulator.cu:54-77)
     // and Brownian::updateThermalVel (156-179) after this call's force, counter = call index;
     // BAOAB::updateThermalVel (122-144) in the next call's first half, whose index is one more
