@@ -91,12 +91,12 @@ bool field_ptrs(dpm_set_s* S, int f, double** c0, double** c1, long* n, int* nco
     case DPM_FIELD_LASTPOS: *c0 = V.lx; *c1 = V.ly; *n = S->nv; *ncomp = 2; return true;
     case DPM_FIELD_BOX: *c0 = V.bx; *c1 = V.by; *n = S->nsys; *ncomp = 2; return true;
     case DPM_FIELD_WALLFORCE: *c0 = V.wfx; *c1 = V.wfy; *n = S->nv; *ncomp = 2; return true;
-    case DPM_FIELD_PVEL: *c0 = V.pvx; *c1 = V.pvy; *n = S->np; *ncomp = 2; return true;
-    case DPM_FIELD_PANGVEL: *c0 = V.pw; *n = S->np; return true;
-    case DPM_FIELD_PFORCE: *c0 = V.pfx; *c1 = V.pfy; *n = S->np; *ncomp = 2; return true;
-    case DPM_FIELD_PTORQUE: *c0 = V.ptq; *n = S->np; return true;
-    case DPM_FIELD_VTORQUE: *c0 = V.vtq; *n = S->nv; return true;
-    case DPM_FIELD_PRAD: *c0 = V.prad; *n = S->np; return true;
+    case DPM_FIELD_PVEL: *c0 = S->R.pvx; *c1 = S->R.pvy; *n = S->np; *ncomp = 2; return true;
+    case DPM_FIELD_PANGVEL: *c0 = S->R.pw; *n = S->np; return true;
+    case DPM_FIELD_PFORCE: *c0 = S->R.pfx; *c1 = S->R.pfy; *n = S->np; *ncomp = 2; return true;
+    case DPM_FIELD_PTORQUE: *c0 = S->R.ptq; *n = S->np; return true;
+    case DPM_FIELD_VTORQUE: *c0 = S->R.vtq; *n = S->nv; return true;
+    case DPM_FIELD_PRAD: *c0 = S->R.prad; *n = S->np; return true;
   }
   return false;
 }
@@ -751,9 +751,9 @@ int rigid_supported(dpm_set_s* S) {
 
 // rotational state cleared (initRotationalVariables / initDeltaVariables, DPM2D.cu:197-221)
 int rigid_clear(dpm_set_s* S) {
-  double* rp[] = {S->V.ang, S->V.pw, S->V.ptq, S->V.moi, S->V.pla};
+  double* rp[] = {S->V.ang, S->R.pw, S->R.ptq, S->R.moi, S->R.pla};
   for (double* p : rp) HIPCHK(launch_fill_d(p, 0.0, S->np, S->stream));
-  HIPCHK(launch_fill_d(S->V.vtq, 0.0, S->nv, S->stream));
+  HIPCHK(launch_fill_d(S->R.vtq, 0.0, S->nv, S->stream));
   return DPM_OK;
 }
 
@@ -911,8 +911,10 @@ int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPer
   A(V.a0, np); A(V.area, np); A(V.perim, np); A(V.cx, np); A(V.cy, np); A(V.pe, np); A(V.ang, np);
   A(V.bx, numSystems); A(V.by, numSystems); A(V.ibx, numSystems); A(V.iby, numSystems);
   A(V.mbx, numSystems); A(V.mby, numSystems); A(V.wfx, nv); A(V.wfy, nv);
-  A(V.pvx, np); A(V.pvy, np); A(V.pw, np); A(V.pfx, np); A(V.pfy, np); A(V.ptq, np); A(V.moi, np);
-  A(V.prad, np); A(V.plx, np); A(V.ply, np); A(V.pla, np); A(V.vtq, nv);
+  A(S->R.pvx, np); A(S->R.pvy, np); A(S->R.pw, np); A(S->R.pfx, np); A(S->R.pfy, np); A(S->R.ptq, np);
+  A(S->R.moi, np); A(S->R.prad, np); A(S->R.plx, np); A(S->R.ply, np); A(S->R.pla, np); A(S->R.vtq, nv);
+  if (hipMalloc(&S->d_rg, sizeof(dpm::Rigid)) != hipSuccess) { dpm_destroy(S); return DPM_ERR_OOM; }
+  V.rg = S->d_rg;
   A(S->d_rfire[0], numSystems); A(S->d_rfire[1], numSystems);
   A(V.svoff, numSystems + 1); A(V.sroff, numSystems + 1);
   A(V.ncnt, nv);
@@ -944,11 +946,12 @@ int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPer
        up(V.ibx, ones, st) == hipSuccess && up(V.iby, ones, st) == hipSuccess &&
        up(V.mbx, ones, st) == hipSuccess && up(V.mby, ones, st) == hipSuccess;
   double* zd[] = {S->pos[0][0], S->pos[0][1], S->pos[1][0], S->pos[1][1], V.vx, V.vy, V.fx, V.fy, V.rad, V.l0,
-                  V.th0, V.th, V.len, V.ev, V.es, V.lx, V.ly, V.wfx, V.wfy, V.vtq};
+                  V.th0, V.th, V.len, V.ev, V.es, V.lx, V.ly, V.wfx, V.wfy, S->R.vtq};
   for (double* p : zd) ok = ok && hipMemsetAsync(p, 0, sizeof(double) * nv, st) == hipSuccess;
-  double* zp[] = {V.a0, V.area, V.perim, V.cx, V.cy, V.pe, V.ang, V.pvx, V.pvy, V.pw, V.pfx, V.pfy, V.ptq,
-                  V.moi, V.prad, V.plx, V.ply, V.pla};
+  double* zp[] = {V.a0, V.area, V.perim, V.cx, V.cy, V.pe, V.ang, S->R.pvx, S->R.pvy, S->R.pw, S->R.pfx,
+                  S->R.pfy, S->R.ptq, S->R.moi, S->R.prad, S->R.plx, S->R.ply, S->R.pla};
   for (double* p : zp) ok = ok && hipMemsetAsync(p, 0, sizeof(double) * np, st) == hipSuccess;
+  ok = ok && hipMemcpyAsync(S->d_rg, &S->R, sizeof(dpm::Rigid), hipMemcpyHostToDevice, st) == hipSuccess;
   ok = ok && hipMemsetAsync(V.ncnt, 0, sizeof(int) * nv, st) == hipSuccess;
   ok = ok && hipMemsetAsync(S->d_flag[0], 0, sizeof(int) * numSystems, st) == hipSuccess;
   ok = ok && hipMemsetAsync(S->d_flag[1], 0, sizeof(int) * numSystems, st) == hipSuccess;
@@ -999,8 +1002,8 @@ int dpm_destroy(dpm_set_t h) {
                   S->d_ck, S->d_fin, S->d_seg, S->d_adj, V.vinfo, V.l0r, V.mbx, V.mby, V.wfx, V.wfy,
                   S->d_contacts, S->d_ccount, S->d_x0, S->d_obs, S->mdCount[0], S->mdCount[1], S->mdCount[2],
                   S->mdSlots[0], S->mdSlots[1], S->mdSlots[2], S->mdFlag[0], S->mdFlag[1], S->mdFlag[2],
-                  S->mdFlag[3], V.pvx, V.pvy, V.pw, V.pfx, V.pfy, V.ptq, V.moi, V.prad, V.plx, V.ply,
-                  V.pla, V.vtq, S->d_rfire[0], S->d_rfire[1]};
+                  S->mdFlag[3], S->R.pvx, S->R.pvy, S->R.pw, S->R.pfx, S->R.pfy, S->R.ptq, S->R.moi,
+                  S->R.prad, S->R.plx, S->R.ply, S->R.pla, S->R.vtq, S->d_rfire[0], S->d_rfire[1], S->d_rg};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (S->ownStream) hipStreamDestroy(S->ownStream);
@@ -2249,7 +2252,7 @@ int dpm_get_rigid_max_unbalanced_force(dpm_set_t h, double* perSystem) {
   CHECK_HANDLE(h);
   if (!perSystem) return fail(S, DPM_ERR_ARG, "null output");
   std::vector<double> fx, fy, tq;
-  int rc = ring_fields(S, {{S->V.pfx, &fx}, {S->V.pfy, &fy}, {S->V.ptq, &tq}});
+  int rc = ring_fields(S, {{S->R.pfx, &fx}, {S->R.pfy, &fy}, {S->R.ptq, &tq}});
   if (rc) return rc;
   for (long s = 0; s < S->nsys; s++) {
     double mf = -1, mt = -1;
@@ -2267,7 +2270,7 @@ int dpm_get_rigid_kinetic_energy(dpm_set_t h, double* perSystem) {
   CHECK_HANDLE(h);
   if (!perSystem) return fail(S, DPM_ERR_ARG, "null output");
   std::vector<double> vx, vy, w, rad;
-  int rc = ring_fields(S, {{S->V.pvx, &vx}, {S->V.pvy, &vy}, {S->V.pw, &w}, {S->V.prad, &rad}});
+  int rc = ring_fields(S, {{S->R.pvx, &vx}, {S->R.pvy, &vy}, {S->R.pw, &w}, {S->R.prad, &rad}});
   if (rc) return rc;
   for (long s = 0; s < S->nsys; s++) {
     double k = 0, r2 = 0;

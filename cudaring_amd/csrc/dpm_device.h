@@ -60,9 +60,14 @@ struct View {
   int *tile_v0, *tile_v1, *tile_sys;  // ring-aligned tiles (<= 64 vertices, one system)
   int* stile;          // per-system first tile (nsys + 1)
   int ntiles;
-  // rigid rings, per ring: velocity, angular velocity, force, torque / moment of inertia (an
-  // angular acceleration), moment of inertia, particle radius, last centre and angle; per
-  // vertex: torque about the ring centre
+  const struct Rigid* rg;  // rigid-ring arrays (device copy of the pointers; one kernel argument)
+};
+
+// Rigid rings, per ring: velocity, angular velocity, force, torque / moment of inertia (an
+// angular acceleration), moment of inertia, particle radius, last centre and angle; per vertex:
+// torque about the ring centre.  Reached through View::rg so that the step kernel's argument
+// block stays the size it has without them.
+struct Rigid {
   double *pvx, *pvy, *pw, *pfx, *pfy, *ptq, *moi, *prad, *plx, *ply, *pla, *vtq;
 };
 

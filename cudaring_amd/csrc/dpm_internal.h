@@ -129,6 +129,8 @@ struct dpm_set_s {
   std::vector<double> h_maxrad; // per system
 
   dpm::View V{};
+  dpm::Rigid R{};                 // rigid-ring arrays (host copy of the pointers)
+  dpm::Rigid* d_rg = nullptr;     // their device copy, V.rg
   dpm::Params P{};
   double* pos[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [buffer][component]
   int cur = 0;

@@ -315,6 +315,7 @@ template <int MODE, bool VV>
 __device__ __forceinline__ void rigid_epilogue(const View& V, const Params& P, const StepArgs& A, const Lane& L,
                                                const Box& B, double x, double y, double fx, double fy, double e,
                                                double cx, double cy, double lx, double ly, double z0, double z1) {
+  const Rigid R = *V.rg;
   const double rx = pbx(x - cx, B), ry = pby(y - cy, B);
   const double tq = rx * fy - ry * fx;
   const double I = seg_sum(L.act ? rx * rx + ry * ry : 0.0, L.lane, L.segS, L.segE);
@@ -327,9 +328,9 @@ __device__ __forceinline__ void rigid_epilogue(const View& V, const Params& P, c
   double vx = 0, vy = 0, w = 0;
   if constexpr (MODE != MODE_RIGID) {
     if (L.act) {
-      vx = V.pvx[L.r];
-      vy = V.pvy[L.r];
-      w = V.pw[L.r];
+      vx = R.pvx[L.r];
+      vy = R.pvy[L.r];
+      w = R.pw[L.r];
     }
   }
   if constexpr ((MODE & MODE_FIRE) != 0) {
@@ -342,12 +343,12 @@ __device__ __forceinline__ void rigid_epilogue(const View& V, const Params& P, c
   V.fx[L.v] = fx;
   V.fy[L.v] = fy;
   V.ev[L.v] = e;
-  V.vtq[L.v] = tq;
+  R.vtq[L.v] = tq;
   if (head) {
-    V.pfx[L.r] = Fx;
-    V.pfy[L.r] = Fy;
-    V.ptq[L.r] = T;
-    V.moi[L.r] = I;
+    R.pfx[L.r] = Fx;
+    R.pfy[L.r] = Fy;
+    R.ptq[L.r] = T;
+    R.moi[L.r] = I;
     if constexpr (VV) V.pe[L.r] = E;
   }
   if constexpr (MODE == MODE_RIGID) return;
@@ -379,9 +380,9 @@ __device__ __forceinline__ void rigid_epilogue(const View& V, const Params& P, c
     }
   }
   if (head) {
-    V.pvx[L.r] = vx;
-    V.pvy[L.r] = vy;
-    V.pw[L.r] = w;
+    R.pvx[L.r] = vx;
+    R.pvy[L.r] = vy;
+    R.pw[L.r] = w;
   }
 }
 
@@ -666,6 +667,7 @@ __global__ void k_start_loop(View V, Params P, CellGrid G, double h, double dt, 
 // angle advanced, and each vertex moved into A.xout and tested into A.flagOut (binned at its
 // start position into A.cellOut when given); A.cellOther (and A.flagClear when given) cleared.
 __global__ void k_rigid_start(View V, Params P, CellGrid G, StepArgs A) {
+  const Rigid R = *V.rg;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   for (int c = t; c < G.ncells; c += gridDim.x * blockDim.x) A.cellOther[c] = 0;
   if (A.flagClear)
@@ -673,12 +675,12 @@ __global__ void k_rigid_start(View V, Params P, CellGrid G, StepArgs A) {
   if (t >= V.np) return;
   const int r = t, s = V.rsys[r];
   const Box B = box_of(V, s);
-  double vx = V.pvx[r], vy = V.pvy[r], w = V.pw[r];
+  double vx = R.pvx[r], vy = R.pvy[r], w = R.pw[r];
   double z0 = 0, z1 = 0;
   if (A.rigidLangevin)
     gauss_pair(P.seed, (uint32_t)s, TAG_THERMAL, A.ctr[0] + (unsigned long long)A.ctrOffT, (uint32_t)(r - V.sroff[s]),
                z0, z1);
-  rigid_kick(A.rigidLangevin, A.h, P.gamma, V.pfx[r], V.pfy[r], V.ptq[r], z0, z1, vx, vy, w);
+  rigid_kick(A.rigidLangevin, A.h, P.gamma, R.pfx[r], R.pfy[r], R.ptq[r], z0, z1, vx, vy, w);
   const double cx = V.cx[r], cy = V.cy[r], ang = V.ang[r];
   const double ncx = cx + A.dt * vx, ncy = cy + A.dt * vy, nang = ang + A.dt * w;
   const double dth = nang - ang, c = cos(dth), sn = sin(dth);
@@ -697,12 +699,12 @@ __global__ void k_rigid_start(View V, Params P, CellGrid G, StepArgs A) {
     atomicOr(&A.flagOut[s], 1);
     atomicOr(A.anyOut, 1);
   }
-  V.plx[r] = cx;
-  V.ply[r] = cy;
-  V.pla[r] = ang;
-  V.pvx[r] = vx;
-  V.pvy[r] = vy;
-  V.pw[r] = w;
+  R.plx[r] = cx;
+  R.ply[r] = cy;
+  R.pla[r] = ang;
+  R.pvx[r] = vx;
+  R.pvy[r] = vy;
+  R.pw[r] = w;
   V.ang[r] = nang;
   V.cx[r] = ncx;
   V.cy[r] = ncy;
@@ -712,6 +714,7 @@ __global__ void k_rigid_start(View V, Params P, CellGrid G, StepArgs A) {
 // Gaussian pair of the given amplitude per ring (counter `call`), minus the system's mean ring
 // velocity (kernelConserveParticleMomentum, cudaKernel.cuh:1998-2027, over every ring)
 __global__ __launch_bounds__(256) void k_rigid_inject(View V, Params P, double amplitude, unsigned long long call) {
+  const Rigid R = *V.rg;
   __shared__ double red[2][256 / WAVE];
   const int s = blockIdx.x, tid = threadIdx.x;
   const int r0 = V.sroff[s], r1 = V.sroff[s + 1];
@@ -719,10 +722,10 @@ __global__ __launch_bounds__(256) void k_rigid_inject(View V, Params P, double a
   for (int r = r0 + tid; r < r1; r += blockDim.x) {
     double z0, z1;
     gauss_pair(P.seed, (uint32_t)s, TAG_INJECT, call, (uint32_t)(r - r0), z0, z1);
-    V.pvx[r] = amplitude * z0;
-    V.pvy[r] = amplitude * z1;
-    mx += V.pvx[r];
-    my += V.pvy[r];
+    R.pvx[r] = amplitude * z0;
+    R.pvy[r] = amplitude * z1;
+    mx += R.pvx[r];
+    my += R.pvy[r];
   }
   mx = wave_sum(mx);
   my = wave_sum(my);
@@ -740,8 +743,8 @@ __global__ __launch_bounds__(256) void k_rigid_inject(View V, Params P, double a
   mx /= (r1 - r0);
   my /= (r1 - r0);
   for (int r = r0 + tid; r < r1; r += blockDim.x) {
-    V.pvx[r] -= mx;
-    V.pvy[r] -= my;
+    R.pvx[r] -= mx;
+    R.pvy[r] -= my;
   }
 }
 
@@ -914,15 +917,16 @@ __device__ __forceinline__ double sys_max(const double* part, int q, int t0, int
 
 // --------------------------------------------------------- rigid FIRE ---
 __global__ void k_rigid_fire_partials(View V, double* part) {
+  const Rigid R = *V.rg;
   Lane L;
   if (!wave_tile_ctx(V, L)) return;
   const bool head = L.act && L.lane == L.segS;
   double vx = 0, vy = 0, Fx = 0, Fy = 0;
   if (head) {
-    vx = V.pvx[L.r];
-    vy = V.pvy[L.r];
-    Fx = V.pfx[L.r];
-    Fy = V.pfy[L.r];
+    vx = R.pvx[L.r];
+    vy = R.pvy[L.r];
+    Fx = R.pfx[L.r];
+    Fy = R.pfy[L.r];
   }
   rigid_fire_partials(L, head, vx, vy, Fx, Fy, part);
 }
@@ -938,6 +942,7 @@ __global__ void k_rigid_fire_partials(View V, double* part) {
 // The last centres and angles are the ones at the start of the iteration.
 __global__ void k_rigid_fire_move(View V, Params P, FireParams fp, const FireState* fin, FireState* fout,
                                   const double* part, double* xo, double* yo, int* flagOut, int* anyOut) {
+  const Rigid R = *V.rg;
   Lane L;
   if (!wave_tile_ctx(V, L)) return;
   const int t0 = V.stile[L.s], t1 = V.stile[L.s + 1];
@@ -967,8 +972,8 @@ __global__ void k_rigid_fire_move(View V, Params P, FireParams fp, const FireSta
   if (!L.act) return;
   const double VV1 = restart ? h * h * FF : VV + 2 * h * P0 + h * h * FF;
   double cx = V.cx[L.r], cy = V.cy[L.r], ang = V.ang[L.r];
-  double vx = V.pvx[L.r], vy = V.pvy[L.r], w = V.pw[L.r];
-  const double Fx = V.pfx[L.r], Fy = V.pfy[L.r];
+  double vx = R.pvx[L.r], vy = R.pvy[L.r], w = R.pw[L.r];
+  const double Fx = R.pfx[L.r], Fy = R.pfy[L.r];
   const double lcx = cx, lcy = cy, lang = ang;
   if (restart) {
     cx -= h * vx;
@@ -976,7 +981,7 @@ __global__ void k_rigid_fire_move(View V, Params P, FireParams fp, const FireSta
     ang -= h * w;
     vx = vy = w = 0;
   }
-  rigid_kick(false, h, 0.0, Fx, Fy, V.ptq[L.r], 0.0, 0.0, vx, vy, w);
+  rigid_kick(false, h, 0.0, Fx, Fy, R.ptq[L.r], 0.0, 0.0, vx, vy, w);
   if (FF > 0) {
     const double ratio = sqrt(VV1 / FF);
     vx = (1 - f.a) * vx + f.a * Fx * ratio;
@@ -997,15 +1002,15 @@ __global__ void k_rigid_fire_move(View V, Params P, FireParams fp, const FireSta
     atomicOr(anyOut, 1);
   }
   if (L.lane == L.segS) {
-    V.plx[L.r] = lcx;
-    V.ply[L.r] = lcy;
-    V.pla[L.r] = lang;
+    R.plx[L.r] = lcx;
+    R.ply[L.r] = lcy;
+    R.pla[L.r] = lang;
     V.cx[L.r] = cx;
     V.cy[L.r] = cy;
     V.ang[L.r] = ang;
-    V.pvx[L.r] = vx;
-    V.pvy[L.r] = vy;
-    V.pw[L.r] = w;
+    R.pvx[L.r] = vx;
+    R.pvy[L.r] = vy;
+    R.pw[L.r] = w;
   }
 }
 
