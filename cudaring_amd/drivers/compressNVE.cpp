@@ -1,15 +1,21 @@
-// compressNVE -- quasistatic compression of a saved packing by NVE runs between box shrinks
-// (the flow of compressNVE.cpp:143-231 in farceri/cudaRing, from a packing on disk).
+// compressNVE -- quasistatic compression by NVE runs between box shrinks (compressNVE.cpp in
+// farceri/cudaRing, same arguments and directory layout).
 //
-// usage: compressNVE inDir outDir timeStep Tinject maxStep numParticles phiTarget
-//                    [potential=wca|lj|harmonic] [box=periodic|square|sides] [deltaPhi=2e-3] [dyn=nve|scalevel]
-//   At each packing fraction: neighbours and forces, the energy injected by the last box shrink
-//   removed (adjustKineticEnergy to the previous total energy), maxStep NVE steps with the energy
-//   log every maxStep/100 steps, the packing saved under outDir/<phi>/, then
+// usage: compressNVE outDir inDir timeStep Tinject shape0 numParticles maxStep lx ly potType boxType
+//                    [phiTarget=0.96] [deltaPhi=2e-3] [seed] [dyn=nve|scalevel]
+//   inDir "0": a new packing -- polydisperse sizes (calA0 = shape0, polydispersity 0.1), soft disks
+//   at phi 0.1 in an lx x ly box (boxType "circle": a round box of radius lx), soft-disk FIRE,
+//   vertices on the disks, lengths in units of the mean particle size; written to
+//   outDir/<N>-box<lx><ly>-A<shape0>/{fire,initial}/ (circle: outDir/circle/<N>-A<shape0>/).
+//   Otherwise the packing and velocities of outDir/inDir/ are read.
+//   boxType: square (fixedBox), sides (fixedSides), circle (roundBox), anything else periodic;
+//   walls have energy scale ew = 10.  potType: wca, lj or harmonic, vertex-segment interaction.
+//   Then at each packing fraction: neighbours and forces, the energy injected by the last box
+//   shrink removed (adjustKineticEnergy to the previous total energy), maxStep NVE steps with the
+//   energy log every maxStep/100 steps, the packing saved under <dir>/<phi>/, then
 //   scaleBox(sqrt((phi + deltaPhi) / phi)) until phi reaches phiTarget.  timeStep is in units of
-//   sigma / sqrt(ec) with sigma the mean particle size; costs are compressNVE.cpp:38's
-//   (1e5, 20, 10, 1) with walls of energy scale ew = 10.  The packing's rings must lie inside the
-//   box for the walled geometries.
+//   sigma / sqrt(ec), sigma the mean particle size; costs (1e5, 20, 10, 1) as compressNVE.cpp:38.
+//   maxStep 0 stops after the set-up (no device is used).
 #include <sys/stat.h>
 
 #include <cmath>
@@ -19,50 +25,111 @@
 #include <string>
 
 #include "../host/DPM2D.h"
+#include "../host/FIRE.h"
 #include "../host/FileIO.h"
 
 int main(int argc, char** argv) {
-  if (argc < 8) {
-    std::cerr << "usage: compressNVE inDir outDir timeStep Tinject maxStep numParticles phiTarget "
-                 "[potential] [box] [deltaPhi] [nve|scalevel]" << std::endl;
+  if (argc < 12) {
+    std::cerr << "usage: compressNVE outDir inDir timeStep Tinject shape0 numParticles maxStep lx ly potType "
+                 "boxType [phiTarget] [deltaPhi] [seed] [nve|scalevel]" << std::endl;
     return 2;
   }
-  std::string inDir = argv[1], outDir = argv[2];
-  if (inDir.back() != '/') inDir += "/";
+  std::string outDir = argv[1], inDir = argv[2];
   if (outDir.back() != '/') outDir += "/";
-  double timeStep = std::atof(argv[3]), Tinject = std::atof(argv[4]);
-  long maxStep = std::atol(argv[5]), numParticles = std::atol(argv[6]);
-  double phiTh = std::atof(argv[7]);
-  std::string pot = argc > 8 ? argv[8] : "wca", boxType = argc > 9 ? argv[9] : "periodic";
-  double deltaPhi = argc > 10 ? std::atof(argv[10]) : 2e-3;
-  std::string dynType = argc > 11 ? argv[11] : "nve";
-  const double ea = 1e5, el = 20, eb = 10, ec = 1, ew = 10, LJcut = 1.5, cutoff = 0.5;
-  const long maxSearchStep = 1500;
+  double timeStep = std::atof(argv[3]), Tinject = std::atof(argv[4]), shape0 = std::atof(argv[5]);
+  long numParticles = std::atol(argv[6]), maxStep = std::atol(argv[7]);
+  double lx = std::atof(argv[8]), ly = std::atof(argv[9]);
+  std::string potType = argv[10], boxType = argv[11];
+  double phiTh = argc > 12 ? std::atof(argv[12]) : 0.96, deltaPhi = argc > 13 ? std::atof(argv[13]) : 2e-3;
+  unsigned long seed = argc > 14 ? std::strtoul(argv[14], nullptr, 10) : 0;
+  std::string dynType = argc > 15 ? argv[15] : "nve";
+  const long maxSearchStep = 1500, maxIterations = 5000000, minStep = 20;
+  const double polydispersity = 0.1, cutoff = 0.5, LJcut = 1.5, forceTolerance = 1e-12, FIREStep = 1e-2;
+  const double ea = 1e5, el = 20, eb = 10, ec = 1, ew = 10;
+  double currentPhi = 0.1, prevEnergy = 0;
   mkdir(outDir.c_str(), 0755);
 
   DPM2D dpm(numParticles, 2, 32);
+  if (argc > 14) dpm.setSeed(seed);
   if (boxType == "square") dpm.setGeometryType(simControlStruct::geometryEnum::fixedBox);
   else if (boxType == "sides") dpm.setGeometryType(simControlStruct::geometryEnum::fixedSides);
-  else if (boxType != "periodic") {
-    std::cerr << "compressNVE: box must be periodic, square or sides" << std::endl;
-    return 2;
-  }
-  if (boxType != "periodic") dpm.setBoxEnergyCost(ew);
+  else if (boxType == "circle") dpm.setGeometryType(simControlStruct::geometryEnum::roundBox);
+  else dpm.setGeometryType(simControlStruct::geometryEnum::normal);
+  if (boxType == "square" || boxType == "sides" || boxType == "circle") dpm.setBoxEnergyCost(ew);
   ioDPMFile io(&dpm);
-  io.readPackingFromDirectory(inDir, numParticles, 2);
-  if (boxType != "periodic") {
-    std::vector<double> pos = dpm.getVertexPositions(), L = dpm.getBoxSize();
-    for (size_t i = 0; i < pos.size(); i++) {
-      const double x = pos[i], Lw = L[i % 2];
-      const bool walled = boxType == "square" || i % 2 == 1;
-      if (walled && (x < 0 || x > Lw)) {
-        std::cerr << "compressNVE: vertex " << i / 2 << " lies outside the walled box" << std::endl;
-        return 2;
-      }
+
+  if (inDir != "0") {
+    std::cout << "Reading initial configuration from " << inDir << std::endl;
+    inDir = outDir + inDir + "/";
+    io.readPackingFromDirectory(inDir, numParticles, 2);
+    io.readState(inDir, numParticles, dpm.getNumVertices(), 2);
+  } else {
+    char tag[128];
+    if (boxType == "circle") {
+      outDir += "circle/";
+      mkdir(outDir.c_str(), 0755);
+      std::snprintf(tag, sizeof tag, "%s-A%s/", argv[6], argv[5]);
+    } else {
+      std::snprintf(tag, sizeof tag, "%s-box%s%s-A%s/", argv[6], argv[8], argv[9], argv[5]);
     }
+    outDir += tag;
+    mkdir(outDir.c_str(), 0755);
+    // soft disks with the harmonic contact, relaxed by FIRE (compressNVE.cpp:70-103)
+    dpm.setPotentialType(simControlStruct::potentialEnum::harmonic);
+    dpm.setInteractionType(simControlStruct::interactionEnum::vertexVertex);
+    dpm.setPolySizeDistribution(shape0, polydispersity);
+    if (boxType == "circle") dpm.setRoundScaledRandomParticles(currentPhi, 1.5, lx);
+    else dpm.setScaledRandomParticles(currentPhi, 1.5, lx, ly);
+    dpm.scaleParticlePacking();
+    dpm.setEnergyCosts(0, 0, 0, ec);
+    std::vector<double> fp = {0.2, 0.5, 1.1, 0.99, FIREStep, 10 * FIREStep, 0.2};
+    dpm.initFIRE(fp, minStep, 0, numParticles);
+    dpm.setParticleMassFIRE();
+    double size = 2 * dpm.getMinParticleSigma();
+    // setDisplacementCutoff(0.5, size) of the harmonic contact (DPM2D.cu:1300-1329): list radius
+    // 1 + 0.5 particle diameters, refresh after a displacement of 0.5 size / 3
+    double cutDistance = 1 + cutoff, dispCut = cutoff * size;
+    dpm.calcParticleNeighborList(cutDistance);
+    dpm.calcParticleForceEnergy();
+    dpm.resetParticleLastPositions();
+    long iteration = 0;
+    while (dpm.getParticleMaxUnbalancedForce() > forceTolerance && iteration != maxIterations) {
+      dpm.particleFIRELoop();
+      if (3 * dpm.getParticleMaxDisplacement() > dispCut) {
+        dpm.calcParticleNeighborList(cutDistance);
+        dpm.resetParticleLastPositions();
+      }
+      iteration += 1;
+    }
+    std::cout << "\nFIRE: iteration: " << iteration << " maxUnbalancedForce: " << std::setprecision(precision)
+              << dpm.getParticleMaxUnbalancedForce() << " energy: " << dpm.getParticlePotentialEnergy() << std::endl;
+    std::string fireDir = outDir + "fire/";
+    mkdir(fireDir.c_str(), 0755);
+    io.saveParticlePacking(fireDir);
+    dpm.initVerticesOnParticles();
+    dpm.scalePacking(dpm.getMeanParticleSize());
+    if (maxStep <= 0) {  // set-up only: host files, no device
+      std::string dp = outDir + "initial/";
+      mkdir(dp.c_str(), 0755);
+      io.save1DIndexFile(dp + "numVertexInParticleList.dat", dpm.getNumVertexInParticleList());
+      io.save1DFile(dp + "radii.dat", dpm.getVertexRadii());
+      io.save2DFile(dp + "positions.dat", dpm.getVertexPositions(), 2);
+      io.save1DFile(dp + "restAreas.dat", dpm.getRestAreas());
+      io.save1DFile(dp + "restLengths.dat", dpm.getRestLengths());
+      io.save1DFile(dp + "restAngles.dat", dpm.getRestAngles());
+      io.save1DFile(dp + "particleRad.dat", dpm.particleRad);
+      if (boxType == "circle") io.save0DFile(dp + "boxSize.dat", dpm.getBoxRadius());
+      else io.save1DFile(dp + "boxSize.dat", dpm.getBoxSize());
+      return 0;
+    }
+    std::string initDir = outDir + "initial/";
+    mkdir(initDir.c_str(), 0755);
+    io.savePacking(initDir);
   }
-  if (pot == "wca") dpm.setPotentialType(simControlStruct::potentialEnum::wca);
-  else if (pot == "lj") {
+  if (maxStep <= 0) return 0;
+
+  if (potType == "wca") dpm.setPotentialType(simControlStruct::potentialEnum::wca);
+  else if (potType == "lj") {
     dpm.setPotentialType(simControlStruct::potentialEnum::lennardJones);
     dpm.setLJcutoff(LJcut);
   } else dpm.setPotentialType(simControlStruct::potentialEnum::harmonic);
@@ -71,8 +138,9 @@ int main(int argc, char** argv) {
   dpm.setEnergyCosts(ea, el, eb, ec);
   long numVertices = dpm.getNumVertices();
 
+  // quasistatic isothermal compression (compressNVE.cpp:143-231)
   dpm.calcParticleShape();
-  double currentPhi = dpm.getPhi(), prevEnergy = 0;
+  currentPhi = dpm.getPhi();
   std::cout << "CURRENT PHI: " << currentPhi << " preferred phi: " << dpm.getPreferredPhi() << std::endl;
   double sigma = dpm.getMeanParticleSize(), size = 2 * dpm.getMeanVertexRadius();
   double timeUnit = sigma / std::sqrt(ec);
@@ -80,7 +148,7 @@ int main(int argc, char** argv) {
   std::cout << "Time step: " << timeStep << " sigma: " << sigma << " size: " << size << " Tinject: " << Tinject
             << std::endl;
   if (dynType == "scalevel") dpm.initNVERescale(Tinject);
-  else dpm.initNVE(Tinject, false);
+  else dpm.initNVE(Tinject, inDir != "0");
   long saveFreq = std::max(1L, maxStep / 100);
   for (long searchStep = 0; searchStep < maxSearchStep; searchStep++) {
     char phiName[32];
@@ -125,9 +193,14 @@ int main(int argc, char** argv) {
     double scaleFactor = std::sqrt((currentPhi + deltaPhi) / currentPhi);
     dpm.scaleBox(scaleFactor);
     currentPhi = dpm.getPhi();
-    std::vector<double> L = dpm.getBoxSize();
-    std::cout << "\nNEW PACKING FRACTION: " << currentPhi << " preferred: " << dpm.getPreferredPhi()
-              << " Lx: " << L[0] << " Ly: " << L[1] << " scale: " << scaleFactor << std::endl;
+    std::cout << "\nNEW PACKING FRACTION: " << currentPhi << " preferred: " << dpm.getPreferredPhi();
+    if (boxType == "circle") {
+      std::cout << " boxR: " << dpm.getBoxRadius();
+    } else {
+      std::vector<double> L = dpm.getBoxSize();
+      std::cout << " Lx: " << L[0] << " Ly: " << L[1];
+    }
+    std::cout << " scale: " << scaleFactor << std::endl;
   }
   return 0;
 }

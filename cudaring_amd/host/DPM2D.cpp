@@ -404,6 +404,7 @@ double DPM2D::getPreferredPhi() {
     double r = hRad[firstVertexInParticleId[p]];
     v += r * r * (0.5 * numVertexInParticleList[p] - 1);
   }
+  if (simControl.geometryType == simControlStruct::geometryEnum::roundBox) return (a + PI * v) / (PI * boxRadius * boxRadius);
   return (a + PI * v) / (boxSize[0] * boxSize[1]);
 }
 double DPM2D::getMeanParticleSize() {
@@ -603,6 +604,103 @@ void DPM2D::setRandomParticles(double phi0, double extraRad) {
        << areaSum / (boxSize[0] * boxSize[1]) << endl;
 }
 
+// DPM2D.cu:1631-1653: the box (lx, ly) grown so that the rest shapes fill phi0, disks drawn in
+// 0.95 of it (x then y per disk)
+void DPM2D::setScaledRandomParticles(double phi0, double extraRad, double lx, double ly) {
+  std::vector<double> box = {lx, ly};
+  setBoxSize(box);
+  double scale = std::sqrt(getPreferredPhi() / phi0);
+  box = {lx * scale, ly * scale};
+  setBoxSize(box);
+  std::vector<double> a0 = getRestAreas();
+  double areaSum = 0;
+  particlePos.assign(2 * numParticles, 0.0);
+  particleRad.assign(numParticles, 0.0);
+  for (long p = 0; p < numParticles; p++) {
+    for (long d = 0; d < 2; d++) particlePos[2 * p + d] = 0.95 * boxSize[d] * drand48();
+    double nv = (double)numVertexInParticleList[p];
+    particleRad[p] = extraRad * std::sqrt((2. * a0[p]) / (nv * std::sin(2. * PI / nv)));
+    areaSum += PI * particleRad[p] * particleRad[p];
+  }
+  particleVel.assign(2 * numParticles, 0.0);
+  particleForce.assign(2 * numParticles, 0.0);
+  particleEnergy.assign(numParticles, 0.0);
+  particleLastPos = particlePos;
+  setLengthScaleToOne();
+  cout << "DPM2D::setScaledRandomParticles: packing fraction: " << getPreferredPhi() << " "
+       << areaSum / (boxSize[0] * boxSize[1]) << endl;
+}
+
+// DPM2D.cu:1655-1676: the round box (centred on the origin) grown to phi0, disks drawn at radius
+// 0.95 R sqrt(U) and angle 2 pi U - pi
+void DPM2D::setRoundScaledRandomParticles(double phi0, double extraRad, double boxRadius_) {
+  setBoxRadius(boxRadius_);
+  double scale = std::sqrt(getPreferredPhi() / phi0);
+  setBoxRadius(boxRadius_ * scale);
+  std::vector<double> a0 = getRestAreas();
+  double areaSum = 0;
+  particlePos.assign(2 * numParticles, 0.0);
+  particleRad.assign(numParticles, 0.0);
+  for (long p = 0; p < numParticles; p++) {
+    double r = 0.95 * boxRadius * std::sqrt(drand48());
+    double th = 2 * PI * drand48() - PI;
+    particlePos[2 * p] = r * std::cos(th);
+    particlePos[2 * p + 1] = r * std::sin(th);
+    double nv = (double)numVertexInParticleList[p];
+    particleRad[p] = extraRad * std::sqrt((2. * a0[p]) / (nv * std::sin(2. * PI / nv)));
+    areaSum += PI * particleRad[p] * particleRad[p];
+  }
+  particleVel.assign(2 * numParticles, 0.0);
+  particleForce.assign(2 * numParticles, 0.0);
+  particleEnergy.assign(numParticles, 0.0);
+  particleLastPos = particlePos;
+  setLengthScaleToOne();
+  cout << "DPM2D::setRoundScaledRandomParticles: packing fraction: " << getPreferredPhi() << " "
+       << areaSum / (PI * boxRadius * boxRadius) << endl;
+}
+
+// scalePacking, DPM2D.cu:1728-1763: every length (disks, vertices, rest sizes, box) divided by scale
+void DPM2D::scalePacking(double scale) {
+  for (auto& x : particlePos) x /= scale;
+  for (auto& r : particleRad) r /= scale;
+  std::vector<double> pos = getVertexPositions(), rad = getVertexRadii(), l0 = getRestLengths(), a0 = getRestAreas();
+  for (auto& x : pos) x /= scale;
+  for (auto& r : rad) r /= scale;
+  for (auto& l : l0) l /= scale;
+  for (auto& a : a0) a /= (scale * scale);
+  setVertexPositions(pos);
+  setVertexRadii(rad);
+  setRestLengths(l0);
+  setRestAreas(a0);
+  if (simControl.geometryType == simControlStruct::geometryEnum::roundBox) {
+    setBoxRadius(boxRadius / scale);
+  } else {
+    std::vector<double> box = {boxSize[0] / scale, boxSize[1] / scale};
+    setBoxSize(box);
+  }
+}
+
+// scaleParticlePacking, DPM2D.cu:1765-1800: disks, vertex radii, rest sizes and box divided by the
+// mean disk radius (vertex positions are not, they are placed on the disks afterwards)
+void DPM2D::scaleParticlePacking() {
+  double scale = getMeanParticleSigma();
+  for (auto& r : particleRad) r /= scale;
+  for (auto& x : particlePos) x /= scale;
+  std::vector<double> rad = getVertexRadii(), l0 = getRestLengths(), a0 = getRestAreas();
+  for (auto& r : rad) r /= scale;
+  for (auto& l : l0) l /= scale;
+  for (auto& a : a0) a /= (scale * scale);
+  setVertexRadii(rad);
+  setRestLengths(l0);
+  setRestAreas(a0);
+  if (simControl.geometryType == simControlStruct::geometryEnum::roundBox) {
+    setBoxRadius(boxRadius / scale);
+  } else {
+    std::vector<double> box = {boxSize[0] / scale, boxSize[1] / scale};
+    setBoxSize(box);
+  }
+}
+
 void DPM2D::initVerticesOnParticles() {
   // DPM2D.cu:1678-1691: the angle uses the global vertex id, so ring p starts rotated by
   // 2 pi firstVertex / n; jitter 1e-3 l0 U(0,1), x draw before y draw
@@ -652,8 +750,8 @@ void DPM2D::calcParticleNeighborList(double cutDistance_) {
   for (long i = 0; i < numParticles; i++) {
     for (long j = 0; j < numParticles; j++) {
       if (j == i) continue;
-      double dx = pbc(particlePos[2 * i] - particlePos[2 * j], boxSize[0]);
-      double dy = pbc(particlePos[2 * i + 1] - particlePos[2 * j + 1], boxSize[1]);
+      double dx = boxd(particlePos[2 * i] - particlePos[2 * j], 0);
+      double dy = boxd(particlePos[2 * i + 1] - particlePos[2 * j + 1], 1);
       double radSum = 2 * particleRad[i];
       double overlap = 1 - std::sqrt(dx * dx + dy * dy) / radSum;
       if (-overlap < cutDistance_) rows[i].push_back(j);
@@ -678,8 +776,8 @@ void DPM2D::calcParticleForceEnergy() {
     for (long k = 0; k < partMaxNeighborList[i]; k++) {
       long j = partNeighborList[i * partNeighborListSize + k];
       if (j < 0 || j == i) continue;
-      double dx = pbc(particlePos[2 * i] - particlePos[2 * j], boxSize[0]);
-      double dy = pbc(particlePos[2 * i + 1] - particlePos[2 * j + 1], boxSize[1]);
+      double dx = boxd(particlePos[2 * i] - particlePos[2 * j], 0);
+      double dy = boxd(particlePos[2 * i + 1] - particlePos[2 * j + 1], 1);
       double d = std::sqrt(dx * dx + dy * dy), radSum = particleRad[i] + particleRad[j];
       if (!wca) {
         double ov = 1 - d / radSum;  // calcContactInteraction, cudaKernel.cuh:531-544
@@ -696,6 +794,50 @@ void DPM2D::calcParticleForceEnergy() {
         particleForce[2 * i + 1] += g * dy / d;
         particleEnergy[i] += 0.5 * ec * (4 * (r12 - r6) + 1);
       }
+    }
+  }
+  if (simControl.geometryType != simControlStruct::geometryEnum::normal) calcParticleWallInteraction();
+}
+
+// boxDistance (cudaKernel.cuh:76-94): the minimum image along periodic directions only
+double DPM2D::boxd(double d, int dim) const {
+  using G = simControlStruct::geometryEnum;
+  const G g = simControl.geometryType;
+  if (g == G::normal || (g == G::fixedSides && dim == 0)) return pbc(d, boxSize[dim]);
+  return d;
+}
+
+// calcParticleFixedWallInteraction, DPM2D.cu:3590-3609 (kernelCalcParticleSquareWallInteraction,
+// kernelCalcParticleSidesInteraction2D, kernelCalcParticleRoundWallInteraction with
+// calcWallWCAInteraction, cudaKernel.cuh:1221-1330): WCA of energy scale ew against the
+// particle's image on the wall, radius sum = the particle's radius
+void DPM2D::calcParticleWallInteraction() {
+  using G = simControlStruct::geometryEnum;
+  auto wall = [&](long p, double wx, double wy) {
+    double x = particlePos[2 * p], y = particlePos[2 * p + 1], radSum = particleRad[p];
+    double dx = x - wx, dy = y - wy, d = std::sqrt(dx * dx + dy * dy);
+    if (d < WCAcut * radSum) {
+      double r = radSum / d, r6 = std::pow(r, 6), r12 = r6 * r6;
+      double g = 24 * ew * (2 * r12 - r6) / d;
+      particleForce[2 * p] += g * dx / d;
+      particleForce[2 * p + 1] += g * dy / d;
+      particleEnergy[p] += ew * (4 * (r12 - r6) + 1);
+    }
+  };
+  const G g = simControl.geometryType;
+  for (long p = 0; p < numParticles; p++) {
+    double x = particlePos[2 * p], y = particlePos[2 * p + 1], rc = WCAcut * particleRad[p];
+    if (g == G::fixedBox) {
+      if (x < rc) wall(p, 0, y);
+      else if (boxSize[0] - x < rc) wall(p, boxSize[0], y);
+    }
+    if (g == G::fixedBox || g == G::fixedSides) {
+      if (y < rc) wall(p, x, 0);
+      else if (boxSize[1] - y < rc) wall(p, x, boxSize[1]);
+    }
+    if (g == G::roundBox) {
+      double R = std::sqrt(x * x + y * y), th = std::atan2(y, x);
+      if (boxRadius - R < rc) wall(p, boxRadius * std::cos(th), boxRadius * std::sin(th));
     }
   }
 }
@@ -717,6 +859,7 @@ double DPM2D::getParticlePotentialEnergy() { return std::accumulate(particleEner
 double DPM2D::getParticlePhi() {
   double s = 0;
   for (double r : particleRad) s += r * r;
+  if (simControl.geometryType == simControlStruct::geometryEnum::roundBox) return s / (boxRadius * boxRadius);
   return s * PI / (boxSize[0] * boxSize[1]);
 }
 double DPM2D::getMeanParticleSigma() { return std::accumulate(particleRad.begin(), particleRad.end(), 0.0) / numParticles; }

@@ -177,6 +177,10 @@ class DPM2D {
   // setBiSizeDistribution as a stub (DPM2D.h:484).  Sizing as setPolySizeDistribution.
   void setBiSizeDistribution(double calA0_, long nSmall, long nLarge, double fracLarge);
   void setRandomParticles(double phi0, double extraRad);
+  void setScaledRandomParticles(double phi0, double extraRad, double lx, double ly);
+  void setRoundScaledRandomParticles(double phi0, double extraRad, double boxRadius_);
+  void scalePacking(double scale);
+  void scaleParticlePacking();
   void initVerticesOnParticles();
   void scaleVertices(double scale);
   void scaleBox(double scale);
@@ -187,6 +191,8 @@ class DPM2D {
   long partNeighborListSize = 0;
   void calcParticleNeighborList(double cutDistance_);
   void calcParticleForceEnergy();
+  void calcParticleWallInteraction();
+  double boxd(double d, int dim) const;  // box distance along dim (periodic directions only)
   double getParticleMaxUnbalancedForce();
   double getParticleMaxDisplacement();
   void resetParticleLastPositions() { particleLastPos = particlePos; }

@@ -177,7 +177,10 @@ class ioDPMFile {
     save2DFile(dir + "velocities.dat", dpm_->getVertexVelocities(), 2);
     save1DFile(dir + "particleRad.dat", dpm_->particleRad);
     save2DFile(dir + "shapes.dat", dpm_->getParticleShapes(), 3);
-    save1DFile(dir + "boxSize.dat", dpm_->getBoxSize());
+    if (dpm_->simControl.geometryType == simControlStruct::geometryEnum::roundBox)
+      save0DFile(dir + "boxSize.dat", dpm_->getBoxRadius());  // FileIO.h:389-396
+    else
+      save1DFile(dir + "boxSize.dat", dpm_->getBoxSize());
   }
 
   void readPackingFromDirectory(const std::string& dir, long numParticles_, long nDim_) {
@@ -196,10 +199,16 @@ class ioDPMFile {
     std::vector<double> th0 = read1DFile(dir + "restAngles.dat", nv);
     dpm_->setRestAngles(th0);
     dpm_->setLengthScale();
-    std::vector<double> box = read1DFile(dir + "boxSize.dat", nDim_);
-    dpm_->setBoxSize(box);
-    std::cout << "FileIO::readPackingFromDirectory: preferred phi: " << dpm_->getPreferredPhi()
-              << " box-Lx: " << box[0] << ", Ly: " << box[1] << std::endl;
+    if (dpm_->simControl.geometryType == simControlStruct::geometryEnum::roundBox) {
+      dpm_->setBoxRadius(read0DFile(dir + "boxSize.dat"));  // the round box's radius (FileIO.h:311-316)
+      std::cout << "FileIO::readPackingFromDirectory: preferred phi: " << dpm_->getPreferredPhi()
+                << " boxRadius: " << dpm_->getBoxRadius() << std::endl;
+    } else {
+      std::vector<double> box = read1DFile(dir + "boxSize.dat", nDim_);
+      dpm_->setBoxSize(box);
+      std::cout << "FileIO::readPackingFromDirectory: preferred phi: " << dpm_->getPreferredPhi()
+                << " box-Lx: " << box[0] << ", Ly: " << box[1] << std::endl;
+    }
     dpm_->calcParticleShape();
   }
 

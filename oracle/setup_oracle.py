@@ -75,6 +75,31 @@ def random_particles(n, a0, l0, rad, phi0, extra_rad, rng):
     return a0, [x / scale for x in l0], [x / scale for x in rad], pos, prad
 
 
+def scaled_random_particles(n, a0, rad, phi0, extra_rad, lx, ly, rng):
+    """setScaledRandomParticles, DPM2D.cu:1631-1653: box grown to phi0, disks in 0.95 of it"""
+    scale = math.sqrt(preferred_phi(n, a0, rad, (lx, ly)) / phi0)
+    box = (lx * scale, ly * scale)
+    pos, prad = [], []
+    for p, k in enumerate(n):
+        pos += [0.95 * box[0] * rng(), 0.95 * box[1] * rng()]
+        prad.append(extra_rad * math.sqrt(2.0 * a0[p] / (k * math.sin(2.0 * math.pi / k))))
+    return box, pos, prad
+
+
+def round_scaled_random_particles(n, a0, rad, phi0, extra_rad, radius, rng):
+    """setRoundScaledRandomParticles, DPM2D.cu:1655-1676: round box grown to phi0 (preferred phi
+    over pi R^2), disks at radius 0.95 R sqrt(U), angle 2 pi U - pi"""
+    area = preferred_phi(n, a0, rad) / (math.pi * radius * radius)
+    R = radius * math.sqrt(area / phi0)
+    pos, prad = [], []
+    for p, k in enumerate(n):
+        r = 0.95 * R * math.sqrt(rng())
+        t = 2 * math.pi * rng() - math.pi
+        pos += [r * math.cos(t), r * math.sin(t)]
+        prad.append(extra_rad * math.sqrt(2.0 * a0[p] / (k * math.sin(2.0 * math.pi / k))))
+    return R, pos, prad
+
+
 def vertices_on_particles(n, a0, l0, ppos, rng):
     pos, v = [], 0
     for p, k in enumerate(n):

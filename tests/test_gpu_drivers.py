@@ -95,11 +95,13 @@ def test_compress_nve_scale_box(packing_dir, tmp_path):
     """compressNVE: NVE at each packing fraction, scaleBox between them (compressNVE.cpp:143-231).
     The saved boxes follow the scale factors sqrt((phi + dphi) / phi) of the printed fractions,
     the covered area hardly changes between fractions, and NVE conserves energy at each one."""
+    import shutil
     out = str(tmp_path / "nve")
+    shutil.copytree(packing_dir, os.path.join(out, "pack"))  # inDir is relative to outDir
     L0 = load(os.path.join(packing_dir, "boxSize.dat"))
-    log = run([os.path.join(BIN, "compressNVE"), packing_dir, out, "2e-4", "1e-4", "200", "16", "0.404",
-               "wca", "periodic", "0.002"])
-    dirs = sorted(d for d in os.listdir(out) if os.path.isdir(os.path.join(out, d)))
+    log = run([os.path.join(BIN, "compressNVE"), out, "pack", "2e-4", "1e-4", "1.1", "16", "200", "0", "0", "wca",
+               "periodic", "0.404", "0.002"])
+    dirs = sorted(d for d in os.listdir(out) if os.path.isdir(os.path.join(out, d)) and d != "pack")
     assert len(dirs) >= 2, log[-2000:]
     phis = [float(ln.split("NEW PACKING FRACTION: ")[1].split()[0]) for ln in log.splitlines()
             if "NEW PACKING FRACTION" in ln]

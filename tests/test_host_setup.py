@@ -117,3 +117,51 @@ def test_bidisperse_sizes(driver, tmp_path):
     a0 = load1(os.path.join(out, "setup", "restAreas.dat"))
     ratio = {k: a for k, a in zip(n, a0)}
     assert abs(ratio[22] / ratio[16] - (22 / 16) ** 2) < 1e-12
+
+
+NVE_BIN = os.path.join(ROOT, "cudaring_amd", "bin", "compressNVE")
+
+
+@pytest.mark.parametrize("box_type", ["periodic", "circle"])
+def test_compress_nve_setup_matches_oracle(driver, tmp_path, box_type):
+    """compressNVE's own packing (compressNVE.cpp:70-112) with maxStep 0: sizes, scaled random
+    disks in the box (square, or round with its wall), scaleParticlePacking, soft-disk FIRE,
+    vertices on the disks, scalePacking by the mean particle size -- draw for draw against the
+    set-up oracle"""
+    np_, shape0, lx, ly, seed = 24, 1.1, 2.0, 1.0, 13
+    out = str(tmp_path)
+    r = subprocess.run([NVE_BIN, out, "0", "1e-3", "1e-4", str(shape0), str(np_), "0", str(lx), str(ly), "wca",
+                        box_type, "0.9", "0.002", str(seed)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    fire_line = [ln for ln in r.stdout.splitlines() if ln.startswith("FIRE: iteration")][-1]
+    assert float(fire_line.split("maxUnbalancedForce: ")[1].split()[0]) <= 1e-12
+    base = os.path.join(out, "circle", f"{np_}-A{shape0}") if box_type == "circle" else \
+        os.path.join(out, f"{np_}-box{lx}{ly}-A{shape0}")
+    assert os.path.isdir(base), os.listdir(out)
+    rng = so.Drand48(seed)
+    n = so.poly_sizes(np_, 32, 0.1, rng)
+    d = os.path.join(base, "initial")
+    assert [int(x) for x in load1(os.path.join(d, "numVertexInParticleList.dat"))] == n
+    a0, l0, th0, rad = so.ring_sizing(n, 32, shape0)
+    if box_type == "circle":
+        R, ppos0, prad = so.round_scaled_random_particles(n, a0, rad, 0.1, 1.5, lx, rng)
+    else:
+        box, ppos0, prad = so.scaled_random_particles(n, a0, rad, 0.1, 1.5, lx, ly, rng)
+    # scaleParticlePacking: every length by the mean disk radius
+    s = sum(prad) / len(prad)
+    prad = [x / s for x in prad]
+    l0 = [x / s for x in l0]
+    rad = [x / s for x in rad]
+    a0 = [x / (s * s) for x in a0]
+    ppos = load1(os.path.join(base, "fire", "particlePos.dat"))
+    np.testing.assert_allclose(load1(os.path.join(base, "fire", "particleRad.dat")), prad, rtol=1e-13)
+    pos = so.vertices_on_particles(n, a0, l0, ppos, rng)
+    # scalePacking by the mean particle size 2 sqrt(<a0> / pi)
+    m = 2 * math.sqrt(sum(a0) / len(a0) / math.pi)
+    np.testing.assert_allclose(load1(os.path.join(d, "positions.dat")), [x / m for x in pos], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(load1(os.path.join(d, "restAreas.dat")), [x / (m * m) for x in a0], rtol=1e-13)
+    np.testing.assert_allclose(load1(os.path.join(d, "radii.dat")), [x / m for x in rad], rtol=1e-13)
+    if box_type == "circle":
+        np.testing.assert_allclose(load1(os.path.join(d, "boxSize.dat")), [R / s / m], rtol=1e-13)
+    else:
+        np.testing.assert_allclose(load1(os.path.join(d, "boxSize.dat")), [b / s / m for b in box], rtol=1e-13)
