@@ -874,13 +874,18 @@ int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPer
   S->np = np;
   S->nv = nv;
   // ring-aligned tiles of at most 64 vertices, greedy, never crossing a system
+  const int cap = 64;  // smaller caps (32-48 vertices) measured slower at 1024 rings
   S->h_stile.push_back(0);
   for (long s = 0; s < numSystems; s++) {
     long r = S->h_sroff[s];
     while (r < S->h_sroff[s + 1]) {
       const int v0 = S->h_rfirst[r];
       int v1 = v0;
-      while (r < S->h_sroff[s + 1] && (v1 + S->h_rnv[r] - v0) <= 64) {
+      while (r < S->h_sroff[s + 1] && (v1 + S->h_rnv[r] - v0) <= cap) {
+        v1 += S->h_rnv[r];
+        r++;
+      }
+      if (v1 == v0) {  // a ring larger than the cap gets a tile of its own
         v1 += S->h_rnv[r];
         r++;
       }
