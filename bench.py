@@ -478,6 +478,7 @@ def main():
     ap.add_argument("--cfg5-iters", type=int, default=2000)
     ap.add_argument("--integrators", action="store_true",
                     help="the benchmark workload under each Langevin-loop integrator (5,000 steps each)")
+    ap.add_argument("--no-rigid", action="store_true", help="skip the rigid-ring runs of the default line")
     ap.add_argument("--rigid", action="store_true",
                     help="the benchmark packing as rigid rings: rigid NVE, Langevin and FIRE throughput")
     ap.add_argument("--traffic", action="store_true",
@@ -528,6 +529,7 @@ def main():
     us, kbar = s.time_step_kernel(200)
     alg = alg_bytes_per_step(nv, np_, kbar)
     achieved = alg / (us * 1e-6) / 1e9
+    rigid = None if args.no_rigid else rigid_bench(s, dt, 2000)
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": TRAFFIC_BYTES_PER_LAUNCH,
             "kernel": "k_step<vertexSmooth,WCA,kick|langevin|start>", "kernel_us": round(us, 3),
@@ -551,6 +553,7 @@ def main():
         "cpu_baseline": cpu,
         "batched_fire": fire,
         "adjoint": adjoint,
+        "rigid": rigid,
     }
     print(json.dumps(out), flush=True)
 

@@ -32,7 +32,9 @@ def compare_rings(o, g, tol=1e-12):
 
 
 @pytest.mark.parametrize("pot,inter", [("harmonic", "vertexVertex"), ("wca", "vertexVertex"),
-                                       ("harmonic", "vertexSmooth"), ("wca", "vertexSmooth")])
+                                       ("harmonic", "vertexSmooth"), ("wca", "vertexSmooth"),
+                                       ("lj", "vertexVertex"), ("adhesive", "vertexVertex"),
+                                       ("lj", "vertexSmooth"), ("adhesive", "vertexSmooth")])
 def test_rigid_force_parity(oracle_mod, pot, inter):
     print(f"case rigid_force {pot} {inter}", flush=True)
     st = ring_block(gap=0.6 if pot == "harmonic" else 1.9)

@@ -104,3 +104,41 @@ def test_scale_box_parity(oracle_mod, geometry):
         g.calc_force_energy()
         assert rel_err(g.get("force"), o.get("force")) < 1e-12
     g.close()
+
+
+def test_scale_box_batched_systems(oracle_mod):
+    """two periodic systems in one set, each shrunk by its own scale (dpm_scale_box takes one
+    scale per system), against one oracle each"""
+    print("case scale_box batched", flush=True)
+    from cudaring_amd import DPMSet
+    from tests.test_oracle_rigid import ring_block
+    sts = [ring_block(gap=1.0, seed=2), ring_block(gap=1.4, seed=6)]
+    nvp = np.concatenate([s["nvp"] for s in sts])
+    g = DPMSet(nvp, rings_per_system=[len(s["nvp"]) for s in sts])
+    cat = {k: np.concatenate([s[k] for s in sts]) for k in ("pos", "rad", "l0", "theta0", "a0", "box")}
+    g.load_state(cat)
+    g.set_types("wca", "vertexSmooth", "harmonic", "passive")
+    g.set_energy_costs(1, 1, 0.1, 1)
+    size = 2 * np.mean([s["rad"].mean() for s in sts])
+    g.calc_neighbor_list(g.set_displacement_cutoff(0.5, size))
+    g.calc_force_energy()
+    scales = [1.02, 1.05]
+    g.scale_box(scales)
+    g.calc_neighbor_list(g.set_displacement_cutoff(0.5, size))
+    g.calc_force_energy()
+    pos, force, box = g.get("pos").reshape(-1, 2), g.get("force").reshape(-1, 2), g.get("box").reshape(-1, 2)
+    k = 0
+    for sid, st in enumerate(sts):
+        o = oracle_from_state(oracle_mod, st, potential="wca", interaction="vertexSmooth", costs=(1, 1, 0.1, 1),
+                              system_id=sid)
+        o.calc_neighbor_list(o.set_displacement_cutoff(0.5, size))
+        o.calc_force_energy()
+        o.scale_box(scales[sid])
+        o.calc_neighbor_list(o.set_displacement_cutoff(0.5, size))
+        o.calc_force_energy()
+        n = int(st["nvp"].sum())
+        assert np.array_equal(box[sid], o.get("box"))
+        assert rel_err(pos[k:k + n].ravel(), o.get("pos"), scale=1.0) < 1e-13, sid
+        assert rel_err(force[k:k + n].ravel(), o.get("force")) < 1e-12, sid
+        k += n
+    g.close()
