@@ -2140,10 +2140,7 @@ int dpm_calc_rigid_force_energy(dpm_set_t h) {
 static int rigid_init(dpm_set_s* S, double T, double gamma, bool langevin, int readState) {
   S->rigidKind = langevin ? 1 : 2;
   S->Tinject = T;
-  if (langevin) {
-    S->P.gamma = gamma;
-    S->P.noise = sqrt(2. * T * gamma);
-  }
+  if (langevin) S->P.gamma = gamma;  // the rigid kick adds unit draws: no noise amplitude
   bump(S);
   int rc = rigid_clear(S);
   if (rc) return rc;

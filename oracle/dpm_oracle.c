@@ -1197,7 +1197,6 @@ void orc_init_rigid(orc_state* s, double T, double gamma, int langevin, int read
   s->rigidKind = langevin ? 1 : 2;
   s->Tinject = T;
   s->gamma = gamma;
-  s->noise = sqrt(2. * T * gamma);
   memset(s->particleAngle, 0, sizeof(double) * np);
   memset(s->particleAngvel, 0, sizeof(double) * np);
   memset(s->particleTorque, 0, sizeof(double) * np);
