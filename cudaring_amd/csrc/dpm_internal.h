@@ -27,6 +27,10 @@ enum : int { NOISE_LANGEVIN = 0, NOISE_BAOAB = 1, NOISE_BROWNIAN = 2, NOISE_DRIV
 // error bits written by kernels into Set::d_err
 enum : int { ERR_LIST_OVERFLOW = 1, ERR_CELL_OVERFLOW = 2, ERR_STALE_BINS = 4 };
 
+// rebuild-flag bits (per system): 1 = rebuild, FLAG_FAR = a vertex moved more than cutoff / 3 in
+// the drift that produced these positions
+enum : int { FLAG_FAR = 2 };
+
 // per-system FIRE scalars (double buffered by iteration parity)
 struct FireState {
   double dt, a, P;

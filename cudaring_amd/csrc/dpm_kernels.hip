@@ -259,6 +259,14 @@ __device__ __forceinline__ void gather_pairs(const View& V, const Params& P, con
 // -------------------------------------------------------------- step ---
 // Waves of a tile: with nw > 1, wave 0 does the ring geometry, shape force and random draws
 // while waves 1..nw-1 share the neighbour gather; with nw == 1 wave 0 does both in turn.
+// A drift that moved a vertex more than cutoff / 3 marks its system FLAG_FAR: the next rebuild
+// cannot use cells of the positions before the drift
+__device__ __forceinline__ void far_test(const Params& P, const Box& B, double dx, double dy, int* flag) {
+  dx = pbx(dx, B);
+  dy = pby(dy, B);
+  if (3 * sqrt(dx * dx + dy * dy) > P.cutoff) atomicOr(flag, FLAG_FAR);
+}
+
 // ---------------------------------------------------------- rigid rings ---
 // Half kick of a rigid ring: RigidNVE::updateVelocity (Simulator.cu:348-366), or
 // RigidLangevin::updateVelocity (263-286) with its unit Gaussian pair (z0, z1) of the step and
@@ -372,6 +380,7 @@ __device__ __forceinline__ void rigid_epilogue(const View& V, const Params& P, c
         atomicOr(&A.flagOut[L.s], 1);
         atomicOr(A.anyOut, 1);
       }
+      far_test(P, B, xn - x, yn - y, A.flagOut + L.s);
       if (head) {
         V.ang[L.r] = nang;
         V.cx[L.r] = ncx;
@@ -408,17 +417,21 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
     // the step loop bins these positions for the next step's rebuild (off the critical path:
     // wave 0 waits for the gather waves anyway)
     if (A.cellOut && w == 0 && L.act) bin_vertex(G, B, A.cellOut, A.slotsOut, L.v, L.s, x, y, A.err);
-    const bool rebuild = A.flagCur[L.s] != 0;
+    const int flag = A.flagCur[L.s];
+    const bool rebuild = (flag & 1) != 0;
+    // step loop: rows come from the previous positions' cells, complete while no vertex moved
+    // more than cutoff / 3 in the last step (the margin the cells are widened by); a system
+    // whose previous drift moved a vertex farther carries FLAG_FAR and searches all its
+    // vertices instead (the reference's all-pairs search)
+    const bool allPairs = A.staleCheck && (flag & FLAG_FAR) != 0;
     if (rebuild && L.tile == V.stile[L.s] && threadIdx.x == 0) A.updateCount[L.s] += 1;
     if (rebuild && w == 0 && L.act) {
-      // rows from the previous positions' cells: complete while no vertex moved more than
-      // cutoff / 3 since them, the margin the cells are widened by (the previous position is
-      // this lane's own, in the buffer the drift below overwrites)
-      if (A.staleCheck) {
+      if (A.staleCheck && !allPairs) {  // the previous position is this lane's own, in the
+                                        // buffer the drift below overwrites
         const double ddx = pbx(x - A.xout[L.v], B), ddy = pby(y - A.yout[L.v], B);
         if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) atomicOr(A.err, ERR_STALE_BINS);
       }
-      build_row(V, P, G, A.cellCur, A.slotsCur, L.v, L.s, B, A.err, A.maxcnt);
+      build_row(V, P, G, A.cellCur, A.slotsCur, L.v, L.s, B, A.err, A.maxcnt, allPairs);
     }
     if (rebuild) __syncthreads();
   }
@@ -604,6 +617,7 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
       atomicOr(&A.flagOut[L.s], 1);
       atomicOr(A.anyOut, 1);
     }
+    far_test(P, B, xn - x, yn - y, A.flagOut + L.s);
   }
   V.vx[L.v] = vx;
   V.vy[L.v] = vy;
@@ -659,6 +673,7 @@ __global__ void k_start_loop(View V, Params P, CellGrid G, double h, double dt, 
     atomicOr(&A.flagOut[s], 1);
     atomicOr(A.anyOut, 1);
   }
+  far_test(P, B, xn - V.x[v], yn - V.y[v], A.flagOut + s);
 }
 
 // The rigid loop's first half step from the stored ring force and torque, one thread per ring
@@ -694,6 +709,7 @@ __global__ void k_rigid_start(View V, Params P, CellGrid G, StepArgs A) {
     A.yout[v] = yn;
     const double ddx = pbx(xn - V.lx[v], B), ddy = pby(yn - V.ly[v], B);
     flag = flag || 3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff;
+    far_test(P, B, xn - x, yn - y, A.flagOut + s);
   }
   if (flag) {
     atomicOr(&A.flagOut[s], 1);

@@ -44,15 +44,16 @@ __device__ __forceinline__ void row_insert(const View& V, int v, int c, int j, i
 
 // Rebuild the row of vertex v.  kernelCalcNeighborList predicate (cudaKernel.cuh:1735-1763):
 // j in list(v) iff rings differ and -(1 - d/radSum) < cutDistance; rows stay ascending as in
-// the reference.  Candidates come from the 3x3 cell block (or the whole system when the box
-// is narrower than three cells).  Also resets lastPos (resetLastPositions).
+// the reference.  Candidates come from the 3x3 cell block, or the whole system when the box
+// is narrower than three cells or `allPairs` is set (cells that may miss a vertex).  Also
+// resets lastPos (resetLastPositions).
 __device__ void build_row(const View& V, const Params& P, const CellGrid& G, const int* count, const int* slots, int v,
-                          int s, const Box& B, int* err, int* maxcnt) {
+                          int s, const Box& B, int* err, int* maxcnt, bool allPairs = false) {
   const double x = V.x[v], y = V.y[v], rv = V.rad[v];
   const int r = V.vring[v];
   const int W = V.W;
   int c = 0;
-  if (G.brute[s]) {
+  if (G.brute[s] || allPairs) {
     const int b0 = V.svoff[s], b1 = V.svoff[s + 1];
     for (int j = b0; j < b1; j++) {
       if (V.vring[j] == r) continue;

@@ -1,0 +1,44 @@
+"""Step loops whose drift moves vertices farther than the cells' margin (cutoff / 3) in one step:
+the next rebuild searches all vertices of the system (the reference's all-pairs list build,
+cudaKernel.cuh:1735-1763) instead of the previous positions' cells, and the trajectory still
+matches the oracle, which rebuilds from scratch."""
+import numpy as np
+import pytest
+
+from tests.conftest import rel_err
+from tests.helpers import gpu_from_state, oracle_from_state
+from tests.test_oracle_rigid import ring_block
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("inter", ["vertexVertex", "vertexSmooth"])
+def test_fast_rings_nve(oracle_mod, inter):
+    print(f"case far_moves nve {inter}", flush=True)
+    st = ring_block(gap=1.9, seed=3)
+    dt = 2e-3
+    o = oracle_from_state(oracle_mod, st, potential="wca", interaction=inter, costs=(1, 1, 0.1, 1), dt=dt)
+    g = gpu_from_state(st, potential="wca", interaction=inter, costs=(1, 1, 0.1, 1), dt=dt)
+    size = 2 * st["rad"].mean()
+    cut_o = o.set_displacement_cutoff(0.5, size)
+    cut_g = g.set_displacement_cutoff(0.5, size)
+    o.calc_neighbor_list(cut_o)
+    g.calc_neighbor_list(cut_g)
+    o.calc_force_energy()
+    g.calc_force_energy()
+    o.init_nve(0.0)
+    g.init_nve(0.0)
+    # every ring translates at a speed that covers 0.6 * cutoff per step (cutoff = 0.5 size)
+    rng = np.random.default_rng(1)
+    speed = 0.6 * 0.5 * size / dt
+    ang = rng.uniform(0, 2 * np.pi, len(st["nvp"]))
+    v = np.repeat(np.c_[np.cos(ang), np.sin(ang)] * speed, st["nvp"], axis=0).ravel()
+    o.set("vel", v)
+    g.set("vel", v)
+    for chunk in (1, 2):  # three steps: the rings meet hard soon after
+        o.nve(chunk)
+        g.nve(chunk)
+        assert rel_err(g.get("pos"), o.get("pos"), scale=1.0) < 1e-11, chunk
+        assert rel_err(g.get("force"), o.get("force")) < 1e-9, chunk
+    assert g.update_count()[0] >= 2  # the list was rebuilt (all pairs) in these steps
+    g.close()
