@@ -991,3 +991,58 @@ void DPM2D::initRigidFIRE(std::vector<double>& p, long minStep_, long numStep_, 
 }
 void DPM2D::rigidFIRELoop() { rigidFIRELoop(1); }
 void DPM2D::rigidFIRELoop(long iterations) { check(dpm_rigid_fire_loop(handle(), iterations), "rigidFIRELoop"); }
+
+// ---------------------------------------------------------------- two-ring interaction test
+// setTwoParticleTest, DPM2D.cu:3229-3267: two equal rings (setMonoSizeDistribution) in an
+// lx x ly box at (0.65 lx, y0 ly) and (0.35 lx, y1 ly); ring 1 moves along x at vel1
+void DPM2D::setTwoParticleTest(double lx, double ly, double y0, double y1, double vel1) {
+  setMonoSizeDistribution();
+  std::vector<double> a0 = getRestAreas();
+  particleRad.assign(numParticles, 0.0);
+  for (long p = 0; p < numParticles; p++) {
+    double nv = (double)numVertexInParticleList[p];
+    particleRad[p] = std::sqrt((2. * a0[p]) / (nv * std::sin(2. * PI / nv)));
+  }
+  std::vector<double> box = {lx, ly};
+  setBoxSize(box);
+  particlePos.assign(2 * numParticles, 0.0);
+  particlePos[0] = lx * 0.65;
+  particlePos[1] = ly * y0;
+  particlePos[2] = lx * 0.35;
+  particlePos[3] = ly * y1;
+  initVerticesOnParticles();
+  setInitialPositions();
+  std::vector<double> vel(2 * numVertices, 0.0);
+  if (simControl.shapeType == simControlStruct::shapeEnum::rigid) {
+    std::vector<double> pv(2 * numParticles, 0.0);
+    pv[2] = vel1;
+    setParticleVelocities(pv);
+  } else {
+    for (long v = firstVertexInParticleId[1]; v < firstVertexInParticleId[1] + numVertexInParticleList[1]; v++)
+      vel[2 * v] = vel1;
+  }
+  setVertexVelocities(vel);
+  setLengthScaleToOne();
+}
+
+// testInteraction / testDeformableInteraction, DPM2D.cu:3269-3315, 3374-3383: velocity Verlet of
+// ring 1's vertices only (ring 0 stays put), the list check and the force call between the
+// half kicks.  A test harness: the half kicks run on the host through the field copies.
+void DPM2D::testInteraction(double timeStep) {
+  if (simControl.shapeType == simControlStruct::shapeEnum::rigid)
+    throw std::invalid_argument("DPM2D::testInteraction: deformable rings only");
+  const long v0 = firstVertexInParticleId[1], v1 = v0 + numVertexInParticleList[1];
+  std::vector<double> pos = getVertexPositions(), vel = getVertexVelocities(), f = getVertexForces();
+  for (long v = v0; v < v1; v++)
+    for (int d = 0; d < 2; d++) {
+      vel[2 * v + d] += 0.5 * timeStep * f[2 * v + d];
+      pos[2 * v + d] += timeStep * vel[2 * v + d];
+    }
+  setVertexPositions(pos);
+  checkNeighbors();
+  calcForceEnergy();
+  f = getVertexForces();
+  for (long v = v0; v < v1; v++)
+    for (int d = 0; d < 2; d++) vel[2 * v + d] += 0.5 * timeStep * f[2 * v + d];
+  setVertexVelocities(vel);
+}

@@ -248,6 +248,10 @@ class DPM2D {
   void rigidFIRELoop();
   void rigidFIRELoop(long iterations);
 
+  // ---- two-ring interaction test (DPM2D.h:613-627)
+  void setTwoParticleTest(double lx, double ly, double y0, double y1, double vel1);
+  void testInteraction(double timeStep);
+
   long neighborListSize = 0;
 
  private:

@@ -159,3 +159,39 @@ def test_run_rigid_matches_oracle(packing_dir, tmp_path, oracle_mod, mode):
     assert e.shape[0] == 100 and int(e[-1, 0]) == steps
     if mode == "nvt":
         assert "T: " in log
+
+
+def test_two_ring_interaction(tmp_path, oracle_mod):
+    """testInteraction (testInteraction.cpp; DPM2D.cu:3229-3315): ring 1 passes ring 0 under
+    Lennard-Jones attraction, velocity Verlet on ring 1 only.  Ring 0 does not move, so U + K is
+    conserved; the oracle replays the same steps from the saved start packing."""
+    out = str(tmp_path / "two")
+    steps, dt = 10000, 5e-4
+    run([os.path.join(BIN, "testInteraction"), out, str(dt), str(steps), "lj", "4"])
+    e = load(os.path.join(out, "energy.dat"), 8)
+    etot = e[:, 4]
+    assert np.abs(etot - etot[0]).max() < 1e-3 * np.abs(etot).max()
+    assert np.abs(e[:, 2]).max() > 0  # the rings felt each other
+    st = state_from_dir(os.path.join(out, "start"))
+    end = state_from_dir(os.path.join(out, "end"))
+    assert np.abs(end["pos"][:40] - st["pos"][:40]).max() == 0.0  # ring 0 stayed put
+    o = oracle_mod.Oracle(st["nvp"])
+    o.load_state(st)
+    o.set("vel", load(os.path.join(out, "start", "velocities.dat"), 2).ravel())
+    o.set_types("lj", "vertexSmooth")
+    o.set_energy_costs(1e5, 20, 10, 1)
+    o.set_lj_cutoff(1.5)
+    o.set_dt(dt)
+    o.calc_neighbor_list(o.set_displacement_cutoff(0.5, 2 * st["rad"][0]))
+    o.calc_force_energy()
+    ring1 = slice(40, 80)
+    for _ in range(steps):
+        x, v, f = o.get("pos"), o.get("vel"), o.get("force")
+        v[ring1] += 0.5 * dt * f[ring1]
+        x[ring1] += dt * v[ring1]
+        o.set("pos", x)
+        o.check_neighbors()
+        o.calc_force_energy()
+        v[ring1] += 0.5 * dt * o.get("force")[ring1]
+        o.set("vel", v)
+    np.testing.assert_allclose(end["pos"], o.get("pos"), rtol=0, atol=1e-8)
