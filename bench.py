@@ -485,8 +485,8 @@ def main():
                     help="counter-collection run: a copy of known size, then step-kernel launches only")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: the path is single-GPU (systems shard across GPUs "
-              "with no exchange; see DESIGN.md).  Use --gpus 1.", file=sys.stderr)
+        print("bench.py runs on one GPU only: the engine measures one GPU's share (batched systems "
+              "would shard across GPUs with no exchange).  Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import torch  # device plumbing: synchronisation around the timed region
