@@ -1747,6 +1747,25 @@ int dpm_calc_contacts(dpm_set_t h) {
   }
 }
 
+// calcGeometricContacts, DPM2D.cu:3430-3450: gap 1, rows of 4 * nDim = 8, extra contacts dropped
+int dpm_calc_geometric_contacts(dpm_set_t h) {
+  CHECK_HANDLE(h);
+  int rc = require_list(S);
+  if (rc) return rc;
+  if (!S->d_ccount) HIPCHK(dalloc(&S->d_ccount, S->np));
+  const int width = 8;
+  if (width != S->contactWidth || !S->d_contacts) {
+    HIPCHK(hipStreamSynchronize(S->stream));
+    hipFree(S->d_contacts);
+    S->d_contacts = nullptr;
+    HIPCHK(dalloc(&S->d_contacts, (long)S->np * width));
+    S->contactWidth = width;
+  }
+  HIPCHK(launch_contacts(S, S->d_contacts, S->d_ccount, width, S->d_notDone, 1.0));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
 // getContacts, DPM2D.cu:1019-1024: rows of `cap` ring ids (-1 padded) and counts; the device
 // row width in *width.  Before any dpm_calc_contacts every ring has no contact.
 int dpm_get_contacts(dpm_set_t h, long* list, long* counts, long cap, long* width) {

@@ -239,7 +239,7 @@ hipError_t launch_adj_hvp(dpm_set_s* S, const View& Vn, AdjArgs A);
 hipError_t launch_energy_param_partials(dpm_set_s* S, double* out);
 hipError_t launch_ring_sum3(dpm_set_s* S, const double* in, double* out);
 hipError_t launch_fill_d(double* p, double value, long n, hipStream_t st);
-hipError_t launch_contacts(dpm_set_s* S, int* list, int* count, int width, int* overflow);
+hipError_t launch_contacts(dpm_set_s* S, int* list, int* count, int width, int* overflow, double gap = -1.0);
 hipError_t launch_hexatic(dpm_set_s* S, const int* list, const int* count, int width, double* psi6);
 hipError_t launch_displacement(dpm_set_s* S, const double* x, const double* y, const double* x0, const double* y0,
                                long n, int rings, double waveNum, double* d2, double* sf);

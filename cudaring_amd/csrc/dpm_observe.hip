@@ -8,8 +8,9 @@ namespace dpm {
 
 // kernelCalcContacts, cudaKernel.cuh:1791-1822: the rings owning any entry of the rows of ring
 // p's vertices, in order of first appearance, into a -1 padded row of `width`; `overflow` is set
-// when a ring has more.
-__global__ void k_contacts(View V, int* list, int* count, int width, int* overflow) {
+// when a ring has more.  With gap >= 0, kernelCalcGeometricContacts (1823-1861): only entries
+// within -(1 - d / radSum) < gap count, and contacts beyond the row width are dropped.
+__global__ void k_contacts(View V, int* list, int* count, int width, int* overflow, double gap) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= V.np) return;
   int* row = list + (long)p * width;
@@ -21,6 +22,12 @@ __global__ void k_contacts(View V, int* list, int* count, int width, int* overfl
     for (int k = 0; k < c; k++) {
       const int o = V.nbr[(long)k * V.nv + v];
       if (o == v || o < 0) continue;
+      if (gap >= 0) {
+        const Box B = box_of(V, V.rsys[p]);
+        const double dx = pbx(V.x[v] - V.x[o], B), dy = pby(V.y[v] - V.y[o], B);
+        if (!(-(1 - sqrt(dx * dx + dy * dy) / (V.rad[v] + V.rad[o])) < gap)) continue;
+        if (added == width) continue;
+      }
       const int q = V.vring[o];
       bool isNew = true;
       for (int t = 0; t < added; t++)
@@ -119,8 +126,8 @@ __global__ void k_particle_stress(View V, Params P, double* out) {
 
 static inline unsigned grid_o(long n) { return (unsigned)((n + 255) / 256); }
 
-hipError_t launch_contacts(dpm_set_s* S, int* list, int* count, int width, int* overflow) {
-  k_contacts<<<grid_o(S->np), 256, 0, S->stream>>>(S->V, list, count, width, overflow);
+hipError_t launch_contacts(dpm_set_s* S, int* list, int* count, int width, int* overflow, double gap) {
+  k_contacts<<<grid_o(S->np), 256, 0, S->stream>>>(S->V, list, count, width, overflow, gap);
   return hipGetLastError();
 }
 

@@ -55,6 +55,7 @@ SIGNATURES = {
     "dpm_get_vertex_isf": (_i, [_vp, _vp]),
     "dpm_get_particle_isf": (_i, [_vp, _d, _vp]),
     "dpm_calc_contacts": (_i, [_vp]),
+    "dpm_calc_geometric_contacts": (_i, [_vp]),
     "dpm_get_contacts": (_i, [_vp, _vp, _vp, _l, _vp]),
     "dpm_get_hexatic_order": (_i, [_vp, _vp]),
     "dpm_get_per_particle_stress": (_i, [_vp, _vp]),
@@ -443,6 +444,17 @@ class DPMSet:
     def contacts(self):
         """calcContacts, then (rows [np, width] -1 padded, counts [np])."""
         self._chk(self.L.dpm_calc_contacts(self.h), "calc_contacts")
+        w = C.c_long()
+        self._chk(self.L.dpm_get_contacts(self.h, None, None, 0, C.byref(w)), "contacts")
+        cap = max(w.value, 1)
+        lst = np.empty(self.np * cap, dtype=np.int64)
+        cnt = np.empty(self.np, dtype=np.int64)
+        self._chk(self.L.dpm_get_contacts(self.h, _p(lst), _p(cnt), cap, C.byref(w)), "contacts")
+        return lst.reshape(self.np, cap), cnt
+
+    def geometric_contacts(self):
+        """calcGeometricContacts (gap 1, rows of 8), then (rows [np, 8] -1 padded, counts [np])."""
+        self._chk(self.L.dpm_calc_geometric_contacts(self.h), "calc_geometric_contacts")
         w = C.c_long()
         self._chk(self.L.dpm_get_contacts(self.h, None, None, 0, C.byref(w)), "contacts")
         cap = max(w.value, 1)

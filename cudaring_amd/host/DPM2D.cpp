@@ -376,6 +376,10 @@ double DPM2D::getParticleISF(double waveNumber) {
 double DPM2D::getParticleHexaticOrder() { return one(dpm_get_hexatic_order, this, "getParticleHexaticOrder"); }
 double DPM2D::getPressure() { return one(dpm_get_pressure, this, "getPressure"); }
 void DPM2D::calcContacts() { check(dpm_calc_contacts(handle()), "calcContacts"); }
+void DPM2D::calcGeometricContacts() {
+  check(dpm_calc_geometric_contacts(handle()), "calcGeometricContacts");
+  contactListSize = 8;
+}
 std::vector<long> DPM2D::getContacts() {
   calcContacts();
   long w = 0;

@@ -142,6 +142,7 @@ class DPM2D {
   double getParticleHexaticOrder();
   double getPressure();
   void calcContacts();
+  void calcGeometricContacts();  // DPM2D.h:634
   std::vector<long> getContacts();  // numParticles x contactListSize, -1 padded
   std::vector<double> getPerParticleStressTensor();
   std::vector<double> getStressTensor();

@@ -175,6 +175,9 @@ int dpm_get_particle_isf(dpm_set_t h, double waveNumber, double* perSystem);
  * owning an entry of its vertices' rows, in order of first appearance; rows of `cap` (-1 padded),
  * counts[R], device row width in *width */
 int dpm_calc_contacts(dpm_set_t h);
+/* calcGeometricContacts, DPM2D.h:634 / DPM2D.cu:3430-3450: rings within one radius sum of a
+ * vertex (gap 1), rows of 8 (4 * nDim), contacts beyond 8 dropped; read with dpm_get_contacts */
+int dpm_calc_geometric_contacts(dpm_set_t h);
 int dpm_get_contacts(dpm_set_t h, long* list, long* counts, long cap, long* width);
 /* getParticleHexaticOrder, DPM2D.h:477 / DPM2D.cu:1505-1510 (from the last dpm_calc_contacts) */
 int dpm_get_hexatic_order(dpm_set_t h, double* perSystem);

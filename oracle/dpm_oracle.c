@@ -1514,6 +1514,37 @@ void orc_calc_contacts(orc_state* s) {
   s->contactListSize = width;
 }
 
+/* calcGeometricContacts + kernelCalcGeometricContacts, DPM2D.cu:3430-3450, cudaKernel.cuh:1823-1861:
+ * the rings owning a row entry within a gap of `gap` radius sums of one of the ring's vertices
+ * (-(1 - d / radSum) < gap), in order of first appearance, into rows of 4 * nDim = 8; a ring's
+ * contacts beyond 8 are dropped (the reference's resize loop never triggers).  The row loop is
+ * bounded by the vertex's own row count (the reference bounds it by the ring's). */
+void orc_calc_geometric_contacts(orc_state* s, double gap) {
+  const long width = 8;
+  free(s->contactList);
+  s->contactList = (long*)malloc(sizeof(long) * s->numParticles * width);
+  for (long i = 0; i < s->numParticles * width; i++) s->contactList[i] = -1;
+  for (long p = 0; p < s->numParticles; p++) {
+    long added = 0, f = s->firstVertexInParticleId[p];
+    for (long v = f; v < f + s->numVertexInParticleList[p]; v++) {
+      for (long k = 0; k < s->maxNeighborList[v]; k++) {
+        long other = s->neighborList[v * s->neighborListSize + k];
+        if (other == v || other == -1) continue;
+        double radSum = s->rad[v] + s->rad[other];
+        double d = calc_distance(s, &s->pos[v * 2], &s->pos[other * 2]);
+        if (!(-(1 - d / radSum) < gap)) continue;
+        if (added >= width) continue;
+        long c = s->particleIdList[other];
+        int isNew = 1;
+        for (long q = 0; q < width; q++) if (s->contactList[p * width + q] == c) isNew = 0;
+        if (isNew) s->contactList[p * width + added++] = c;
+      }
+    }
+    s->maxContactList[p] = added;
+  }
+  s->contactListSize = width;
+}
+
 long orc_contact_list_size(const orc_state* s) { return s->contactListSize; }
 
 void orc_get_contacts(const orc_state* s, long* list, long* counts) {

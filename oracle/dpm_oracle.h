@@ -115,6 +115,7 @@ double orc_get_box_radius(orc_state* s);
 void orc_calc_wall_forces(orc_state* s);
 /* observables */
 void orc_calc_contacts(orc_state* s);
+void orc_calc_geometric_contacts(orc_state* s, double gap);
 long orc_contact_list_size(const orc_state* s);
 void orc_get_contacts(const orc_state* s, long* list, long* counts);
 double orc_get_hexatic_order(orc_state* s);

@@ -80,6 +80,7 @@ def lib():
             "orc_init_langevin": (None, [vp, d, d, i, i]),
             "orc_set_noise_type": (None, [vp, i]),
             "orc_calc_contacts": (None, [vp]),
+            "orc_calc_geometric_contacts": (None, [vp, d]),
             "orc_contact_list_size": (l, [vp]),
             "orc_get_contacts": (None, [vp, vp, vp]),
             "orc_get_hexatic_order": (d, [vp]),
@@ -263,6 +264,15 @@ class Oracle:
     def contacts(self):
         """(list [np, width] -1 padded, counts [np]) after calcContacts."""
         self.L.orc_calc_contacts(self.h)
+        w = self.L.orc_contact_list_size(self.h)
+        lst = np.empty(self.np * w, dtype=np.int64)
+        cnt = np.empty(self.np, dtype=np.int64)
+        self.L.orc_get_contacts(self.h, _ptr(lst), _ptr(cnt))
+        return lst.reshape(self.np, w), cnt
+
+    def geometric_contacts(self, gap=1.0):
+        """(list [np, 8] -1 padded, counts [np]) after calcGeometricContacts."""
+        self.L.orc_calc_geometric_contacts(self.h, gap)
         w = self.L.orc_contact_list_size(self.h)
         lst = np.empty(self.np * w, dtype=np.int64)
         cnt = np.empty(self.np, dtype=np.int64)

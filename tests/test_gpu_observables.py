@@ -34,6 +34,20 @@ def test_contacts_and_hexatic(oracle_mod):
         g.close()
 
 
+def test_geometric_contacts(oracle_mod):
+    print("case geometric contacts", flush=True)
+    for st in (packing.lattice_system(20, 0.8, 5, perturb=0.2), lattice_rings("triangular")):
+        o, g = pair(oracle_mod, st)
+        lo, co = o.geometric_contacts(1.0)
+        lg, cg = g.geometric_contacts()
+        assert lg.shape[1] == 8
+        np.testing.assert_array_equal(cg, co)
+        for p in range(len(co)):
+            np.testing.assert_array_equal(lg[p, :cg[p]], lo[p, :co[p]])
+        assert abs(g.hexatic_order()[0] - o.hexatic_order()) < 1e-13  # psi6 from these contacts
+        g.close()
+
+
 def test_msd_isf_after_dynamics(oracle_mod):
     print("case msd_isf", flush=True)
     st = packing.lattice_system(16, 0.8, 9, perturb=0.1)

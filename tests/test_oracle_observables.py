@@ -120,3 +120,29 @@ def test_stress_sums_to_particle_virial(oracle_mod):
     assert noverlap.max() == 1 and noverlap.sum() > 0
     np.testing.assert_allclose(total, expect, rtol=1e-12, atol=1e-15)
     assert abs(o.pressure() - (total[0] + total[3]) / (2 * len(pos))) < 1e-18
+
+
+def test_geometric_contacts_within_one_radius_sum(oracle_mod):
+    """calcGeometricContacts (DPM2D.cu:3430-3450): rings with a listed vertex pair at
+    d < 2 radSum (gap 1), by brute force over positions, at most 8 per ring; a list built with a
+    skin below the gap bounds them, so the test uses skin 1.2 (list radius above 2 radSum)"""
+    st = packing.lattice_system(12, 0.8, 5, perturb=0.2)
+    o, cut = listed(oracle_mod, st, skin=1.2)
+    lst, cnt = o.geometric_contacts(1.0)
+    assert lst.shape[1] == 8
+    nvp = st["nvp"]
+    ring = np.repeat(np.arange(len(nvp)), nvp)
+    pos = st["pos"].reshape(-1, 2)
+    L = st["box"]
+    d = pos[:, None, :] - pos[None, :, :]
+    d -= L * np.round(d / L)
+    dist = np.sqrt((d ** 2).sum(-1))
+    rs = st["rad"][:, None] + st["rad"][None, :]
+    close = -(1 - dist / rs) < 1.0
+    for p in range(len(nvp)):
+        mine = ring == p
+        expect = set(np.unique(ring[np.any(close[mine], axis=0) & ~mine]))
+        got = lst[p, :cnt[p]]
+        assert len(got) == min(len(expect), 8) and set(got) <= expect
+        assert np.all(lst[p, cnt[p]:] == -1)
+    assert cnt.max() > 0
