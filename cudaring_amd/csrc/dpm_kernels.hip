@@ -414,9 +414,8 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
     for (int c = gid; c < G.ncells; c += gridDim.x * blockDim.x) A.cellOther[c] = 0;
     if (A.flagClear)
       for (int t = gid; t < V.nsys; t += gridDim.x * blockDim.x) A.flagClear[t] = 0;
-    // the step loop bins these positions for the next step's rebuild (off the critical path:
-    // wave 0 waits for the gather waves anyway)
-    if (A.cellOut && w == 0 && L.act) bin_vertex(G, B, A.cellOut, A.slotsOut, L.v, L.s, x, y, A.err);
+    // the step loop bins these positions for the next step's rebuild, on the last wave
+    if (A.cellOut && w == nw - 1 && L.act) bin_vertex(G, B, A.cellOut, A.slotsOut, L.v, L.s, x, y, A.err);
     const int flag = A.flagCur[L.s];
     const bool rebuild = (flag & 1) != 0;
     // step loop: rows come from the previous positions' cells, complete while no vertex moved
