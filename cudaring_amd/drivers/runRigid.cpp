@@ -5,11 +5,11 @@
 //                 [potential=harmonic|wca|lj] [interaction=vertex|smooth]
 //   mode: nve | nvt | fire.  timeStep and damping are in units of sigma / sqrt(ec) with sigma the
 //   mean particle size; the skin is setDisplacementCutoff(0.5, 2 <r_v>).  The rings keep the shapes
-//   they have in inDir (a packing saved by savePacking: particleRad.dat gives the lever arm of the
-//   rotational kinetic energy).  nve / nvt: maxStep steps with the energy log (step, time, U/N,
-//   K/N, E/N, phi) every maxStep/100 steps; fire: maxStep FIRE iterations (or until the largest
-//   ring force or torque is below 1e-12), logged the same way.  The rigid packing
-//   (saveRigidPacking) goes to outDir.
+//   they have in inDir: a packing saved by savePacking (its particleRad.dat gives the lever arm of
+//   the rotational kinetic energy) or a rigid packing saved by saveRigidPacking.  nve / nvt:
+//   maxStep steps with the energy log (step, time, U/N, K/N, E/N, phi) every maxStep/100 steps;
+//   fire: maxStep FIRE iterations (or until the largest ring force or torque is below 1e-12),
+//   logged the same way.  The rigid packing (saveRigidPacking) and boxSize.dat go to outDir.
 #include <sys/stat.h>
 
 #include <cmath>
@@ -52,9 +52,14 @@ int main(int argc, char** argv) {
   }
   if (inter == "smooth") dpm.setInteractionType(simControlStruct::interactionEnum::vertexSmooth);
   ioDPMFile io(&dpm);
-  io.readPackingFromDirectory(inDir, numParticles, 2);
-  std::vector<double> prad = io.read1DFile(inDir + "particleRad.dat", numParticles);
-  dpm.setParticleRadii(prad);
+  struct stat st;
+  if (stat((inDir + "particleAngles.dat").c_str(), &st) == 0) {
+    io.readRigidPackingFromDirectory(inDir, numParticles, 2);  // a packing saved by saveRigidPacking
+  } else {
+    io.readPackingFromDirectory(inDir, numParticles, 2);
+    std::vector<double> prad = io.read1DFile(inDir + "particleRad.dat", numParticles);
+    dpm.setParticleRadii(prad);
+  }
   dpm.setEnergyCosts(0, 0, 0, ec);
   double sigma = dpm.getMeanParticleSize(), timeUnit = sigma / std::sqrt(ec);
   timeStep = dpm.setTimeStep(timeStep * timeUnit);
@@ -90,5 +95,7 @@ int main(int argc, char** argv) {
             << " T: " << dpm.getParticleTemperature() << " max force/torque: " << dpm.getRigidMaxUnbalancedForce()
             << " updates: " << dpm.getUpdateCount() << std::endl;
   io.saveRigidPacking(outDir);
+  // saveRigidPacking writes no box (FileIO.h:437-458) and readRigidPackingFromDirectory needs one
+  io.save1DFile(outDir + "boxSize.dat", dpm.getBoxSize());
   return 0;
 }

@@ -195,3 +195,15 @@ def test_two_ring_interaction(tmp_path, oracle_mod):
         v[ring1] += 0.5 * dt * o.get("force")[ring1]
         o.set("vel", v)
     np.testing.assert_allclose(end["pos"], o.get("pos"), rtol=0, atol=1e-8)
+
+
+def test_rigid_packing_round_trip(packing_dir, tmp_path):
+    """saveRigidPacking then readRigidPackingFromDirectory (FileIO.h:328-381, 437-458): a rigid
+    packing read back and saved again without steps keeps its files"""
+    out1, out2 = str(tmp_path / "r1"), str(tmp_path / "r2")
+    run([os.path.join(BIN, "runRigid"), packing_dir, out1, "fire", "2e-4", "0", "1", "50", "16", "3"])
+    run([os.path.join(BIN, "runRigid"), out1, out2, "fire", "2e-4", "0", "1", "0", "16", "3"])
+    for name in ("positions.dat", "radii.dat", "restAreas.dat", "particleRad.dat", "numVertexInParticleList.dat"):
+        assert open(os.path.join(out1, name)).read() == open(os.path.join(out2, name)).read(), name
+    np.testing.assert_allclose(load(os.path.join(out2, "particlePos.dat"), 2),
+                               load(os.path.join(out1, "particlePos.dat"), 2), rtol=0, atol=1e-12)
