@@ -413,6 +413,7 @@ def rigid_bench(s, dt, steps):
             s.init_rigid_fire([0.2, 0.5, 1.1, 0.99, dt, 10 * dt, 0.2], 20, 0)
             loop = s.rigid_fire
         s.calc_rigid_force_energy()
+        f_start = float(s.rigid_max_unbalanced_force()[0])
         try:
             loop(500)
             s.synchronize()
@@ -428,6 +429,7 @@ def rigid_bench(s, dt, steps):
                      "rebuilds": int(s.update_count()[0]), "steps": steps,
                      "finite": bool(np.all(np.isfinite(s.get("pos"))))}
         if kind == "fire":
+            out[kind]["max_force_torque_start"] = f_start
             out[kind]["max_force_torque"] = float(s.rigid_max_unbalanced_force()[0])
     return out
 
