@@ -392,6 +392,9 @@ class Oracle:
     def update_count(self):
         return self.L.orc_get_update_count(self.h)
 
+    def reset_update_count(self):
+        self.L.orc_reset_update_count(self.h)
+
 
 def philox(ctr, key):
     c = np.ascontiguousarray(ctr, dtype=np.uint32)

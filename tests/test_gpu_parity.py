@@ -280,3 +280,27 @@ def test_persistent_fire_matches_step_kernels(pot, inter):
     assert rel_err(a.get("vel"), b.get("vel")) < 1e-8
     assert rel_err(a.get("force"), b.get("force")) < 1e-8
     assert int(a.update_count().sum()) == int(b.update_count().sum()) and int(a.update_count().sum()) > 0
+
+
+def test_nvt_workload_full_size(oracle_mod):
+    """The NVT Langevin workload at its full size: 1024 bidisperse rings (16/22, 19,456 vertices),
+    WCA vertex-segment, phi 0.85, T = 1e-2, gamma = 1, dt = 1e-3 sigma, skin 0.5 sigma, prepared
+    on the GPU (compression + FIRE) and ordered by ring centre.  Forces and energy at the start
+    to 1e-12; 150 Langevin steps, across neighbour-list rebuilds, step for step with the oracle."""
+    print("case nvt_full_size", flush=True)
+    st = relaxed_state(1024, 0.85, 20261202 + 2)
+    st, _ = packing.spatial_order(st)
+    assert int(st["nvp"].sum()) == 19456
+    sigma = 2 * np.sqrt(np.mean(st["a0"]) / np.pi)
+    o, g = prepare(oracle_mod, st, "wca", "vertexSmooth", dt=1e-3 * sigma, seed=11)
+    o.calc_force_energy(); g.calc_force_energy()
+    assert rel_err(g.get("force"), o.get("force")) < TOL_F
+    assert abs(g.potential_energy()[0] - o.potential_energy()) < 1e-12 * abs(o.potential_energy())
+    o.init_langevin(1e-2, 1.0); g.init_langevin(1e-2, 1.0)
+    o.reset_update_count(); g.reset_update_count()
+    for chunk in (50, 100):
+        o.langevin(chunk); g.langevin(chunk)
+        assert rel_err(g.get("pos"), o.get("pos"), scale=1.0) < 1e-11, chunk
+        assert rel_err(g.get("vel"), o.get("vel")) < 1e-9, chunk
+    assert g.update_count()[0] == o.update_count() >= 1
+    g.close()
