@@ -304,3 +304,28 @@ def test_nvt_workload_full_size(oracle_mod):
         assert rel_err(g.get("vel"), o.get("vel")) < 1e-9, chunk
     assert g.update_count()[0] == o.update_count() >= 1
     g.close()
+
+
+@pytest.mark.parametrize("pot,inter", [("harmonic", "vertexVertex"), ("adhesive", "vertexSmooth")])
+def test_large_system_invariants(pot, inter):
+    """Half a million vertices (27,000 bidisperse 16/22 rings at random centres, phi 0.8), beyond
+    what the oracle finishes in seconds: the forces of a passive system sum to zero (every shape
+    and pair term is translation invariant), and translating every vertex by a box fraction
+    changes no force beyond roundoff."""
+    print(f"case large_invariants {pot} {inter}", flush=True)
+    st = packing.random_system(27000, 0.8, 20261202 + 7)
+    assert int(st["nvp"].sum()) > 500000
+    g = gpu_from_state(st, potential=pot, interaction=inter)
+    cut = g.set_displacement_cutoff(0.5, 2 * st["rad"].mean())
+    g.calc_neighbor_list(cut)
+    g.calc_force_energy()
+    f1 = g.get("force").reshape(-1, 2)
+    assert np.abs(f1).max() > 1e-3
+    assert np.all(np.abs(f1.sum(axis=0)) < 1e-12 * np.abs(f1).sum(axis=0))
+    shift = np.array([0.31, 0.17]) * st["box"]
+    g.set("pos", (st["pos"].reshape(-1, 2) + shift).ravel())
+    g.calc_neighbor_list(cut)
+    g.calc_force_energy()
+    f2 = g.get("force").reshape(-1, 2)
+    assert rel_err(f2.ravel(), f1.ravel()) < 1e-9
+    g.close()
