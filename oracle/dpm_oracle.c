@@ -1429,6 +1429,21 @@ void orc_adjust_temperature(orc_state* s, double targetTemp) {
   for (long i = 0; i < 2 * s->numVertices; i++) s->vel[i] *= scale;
 }
 
+/* adjustKineticEnergy, DPM2D.cu:1161-1183: scale the velocities so that K + U returns to
+ * prevEtot, when the kinetic energy covers the change; returns the scale, or 0 when no
+ * adjustment is made (the reference prints and leaves the velocities) */
+double orc_adjust_kinetic_energy(orc_state* s, double prevEtot) {
+  double ekin = orc_get_kinetic_energy(s);
+  double deltaEtot = orc_get_potential_energy(s) + ekin;
+  deltaEtot -= prevEtot;
+  if (ekin > deltaEtot) {
+    double scale = sqrt((ekin - deltaEtot) / ekin);
+    for (long i = 0; i < 2 * s->numVertices; i++) s->vel[i] *= scale;
+    return scale;
+  }
+  return 0.;
+}
+
 /* scaleVertices, DPM2D.cu:1802-1811 with kernelScaleVertexPositions, cudaKernel.cuh:1644-1658:
  * positions are scaled about the stored particlePos (not recomputed, as in the reference). */
 void orc_scale_vertices(orc_state* s, double scale) {

@@ -155,6 +155,7 @@ double orc_get_preferred_phi(const orc_state* s);
 double orc_get_mean_particle_size(const orc_state* s);
 double orc_get_mean_vertex_radius(const orc_state* s);
 void orc_adjust_temperature(orc_state* s, double targetTemp);
+double orc_adjust_kinetic_energy(orc_state* s, double prevEtot);
 void orc_scale_vertices(orc_state* s, double scale);
 void orc_scale_box(orc_state* s, double scale);
 /* rigid rings */

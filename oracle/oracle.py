@@ -118,6 +118,7 @@ def lib():
             "orc_get_mean_particle_size": (d, [vp]),
             "orc_get_mean_vertex_radius": (d, [vp]),
             "orc_adjust_temperature": (None, [vp, d]),
+            "orc_adjust_kinetic_energy": (d, [vp, d]),
             "orc_scale_vertices": (None, [vp, d]),
             "orc_scale_box": (None, [vp, d]),
             "orc_calc_rigid_force_energy": (None, [vp]),
@@ -348,6 +349,21 @@ class Oracle:
 
     def fire_minimize(self, tol, max_iter):
         return self.L.orc_fire_minimize(self.h, tol, max_iter)
+
+    def adjust_temperature(self, T):
+        self.L.orc_adjust_temperature(self.h, T)
+
+    def adjust_kinetic_energy(self, prev_etot):
+        return self.L.orc_adjust_kinetic_energy(self.h, prev_etot)
+
+    def total_force_magnitude(self):
+        return self.L.orc_get_total_force_magnitude(self.h)
+
+    def calc_smooth_interaction(self):
+        self.L.orc_calc_smooth_interaction(self.h)
+
+    def calc_all_to_all_interaction(self):
+        self.L.orc_calc_all_to_all_interaction(self.h)
 
     def scale_vertices(self, s):
         self.L.orc_scale_vertices(self.h, s)
