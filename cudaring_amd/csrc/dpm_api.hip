@@ -1525,6 +1525,26 @@ int dpm_adjust_temperature(dpm_set_t h, double T) {
   return temperature_scale(S, T);
 }
 
+int dpm_adjust_kinetic_energy(dpm_set_t h, const double* prevEtot, double* scaleOut) {
+  CHECK_HANDLE(h);
+  if (!prevEtot) return fail(S, DPM_ERR_ARG, "dpm_adjust_kinetic_energy: null prevEtot");
+  int rc = reduce_host(S);
+  if (rc) return rc;
+  std::vector<double> sc(S->nsys);
+  for (long s = 0; s < S->nsys; s++) {
+    const double ekin = 0.5 * S->h_red[s * 8 + 2];
+    const double epot = S->h_red[s * 8 + 0] + (S->P.interaction == INT_SMOOTH ? S->h_red[s * 8 + 1] : 0.0);
+    const double dE = (epot + ekin) - prevEtot[s];
+    const bool adjust = ekin > dE;
+    sc[s] = adjust ? sqrt((ekin - dE) / ekin) : 1.0;
+    if (scaleOut) scaleOut[s] = adjust ? sc[s] : 0.0;
+  }
+  HIPCHK(hipMemcpyAsync(S->d_scale, sc.data(), sizeof(double) * S->nsys, hipMemcpyHostToDevice, S->stream));
+  HIPCHK(launch_scale_vel(S));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
 int dpm_init_fire(dpm_set_t h, const double* p, long minStep, long numStep) {
   CHECK_HANDLE(h);
   if (!p) return fail(S, DPM_ERR_ARG, "null FIRE params");

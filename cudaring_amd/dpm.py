@@ -82,6 +82,7 @@ SIGNATURES = {
     "dpm_init_nve_rescale": (_i, [_vp, _d]),
     "dpm_nve_rescale_loop": (_i, [_vp, _l]),
     "dpm_adjust_temperature": (_i, [_vp, _d]),
+    "dpm_adjust_kinetic_energy": (_i, [_vp, _vp, _vp]),
     "dpm_init_fire": (_i, [_vp, _vp, _l, _l]),
     "dpm_fire_loop": (_i, [_vp, _l]),
     "dpm_get_fire_state": (_i, [_vp, _vp]),
@@ -329,6 +330,13 @@ class DPMSet:
 
     def adjust_temperature(self, T):
         self._chk(self.L.dpm_adjust_temperature(self.h, T), "adjust_temperature")
+
+    def adjust_kinetic_energy(self, prev_etot):
+        """adjustKineticEnergy per system; returns the scales (0 where nothing was adjusted)."""
+        e = np.ascontiguousarray(np.broadcast_to(np.asarray(prev_etot, dtype=np.float64), (self.nsys,)))
+        out = np.zeros(self.nsys)
+        self._chk(self.L.dpm_adjust_kinetic_energy(self.h, _p(e), _p(out)), "adjust_kinetic_energy")
+        return out
 
     # ------------------------------------------------------------------ FIRE
     def init_fire(self, params7, min_step=20, num_step=0):

@@ -445,14 +445,10 @@ double DPM2D::getParticleKineticEnergy() {
 double DPM2D::getShapeKineticEnergy() { return getKineticEnergy() - getParticleKineticEnergy(); }
 void DPM2D::adjustTemperature(double T) { check(dpm_adjust_temperature(handle(), T), "adjustTemperature"); }
 void DPM2D::adjustKineticEnergy(double prevEtot) {
-  // DPM2D.cu:1161-1183
-  double ekin = getKineticEnergy();
-  double dE = getPotentialEnergy() + ekin - prevEtot;
-  if (ekin > dE) {
-    double scale = std::sqrt((ekin - dE) / ekin);
-    std::vector<double> v = getVertexVelocities();
-    for (double& x : v) x *= scale;
-    setVertexVelocities(v);
+  // DPM2D.cu:1161-1183, on the device
+  double scale = 0;
+  check(dpm_adjust_kinetic_energy(handle(), &prevEtot, &scale), "adjustKineticEnergy");
+  if (scale > 0) {
     cout << "DPM2D::adjustKineticEnergy:: scale: " << scale << endl;
   } else {
     cout << "DPM2D::adjustKineticEnergy:: kinetic energy is less then change in total energy - no adjustment is made" << endl;

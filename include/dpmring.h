@@ -202,8 +202,13 @@ int dpm_nve_loop(dpm_set_t h, long steps);
 /* initNVERescale(T) / NVERescaleLoop(), DPM2D.h:688-690 */
 int dpm_init_nve_rescale(dpm_set_t h, double T);
 int dpm_nve_rescale_loop(dpm_set_t h, long steps);
-/* adjustTemperature(T), DPM2D.h:437 (per system) */
+/* adjustTemperature(T), DPM2D.h:429 / DPM2D.cu:1185-1200 (per system) */
 int dpm_adjust_temperature(dpm_set_t h, double T);
+/* adjustKineticEnergy(prevEtot), DPM2D.h:427 / DPM2D.cu:1161-1183, per system: with K and U the
+ * current kinetic and potential energies and dE = U + K - prevEtot[s], the velocities scale by
+ * sqrt((K - dE) / K) when K > dE and are left alone otherwise; scaleOut[s] (may be NULL) receives
+ * the scale, or 0 where no adjustment was made */
+int dpm_adjust_kinetic_energy(dpm_set_t h, const double* prevEtot, double* scaleOut);
 
 /* ---- FIRE ----------------------------------------------------------------- */
 /* initFIRE(params{a_start, f_dec, f_inc, f_a, dt, dt_max, a}, minStep, numStep, numDOF),

@@ -13,6 +13,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 static const double PI = 3.141592653589793238462643383279502884197; /* defs.h:14 */
 static const double WCAcut = 1.122462048;                            /* defs.h:17 */
@@ -53,6 +56,8 @@ struct orc_state {
   double *particleVel, *particleAngvel, *particleForce, *particleTorque, *momentOfInertia;
   double *particleLastPos, *particleLastAngle, *torque, *particleRad;
   int rigidKind;            /* 1 Langevin, 2 NVE */
+  int listMethod;           /* 1: cell-list row build (default), 0: the reference's all-pairs loop */
+  int threads;              /* OpenMP threads of the per-vertex loops (1: serial, the test default) */
 };
 
 /* ------------------------------------------------------------------ RNG --- */
@@ -126,6 +131,8 @@ orc_state* orc_create(long numParticles, const long* nvp) {
   s->cutDistance = 1; s->updateCount = 0; s->shift = 0;
   s->gamma = 1; s->noise = 0;
   s->ew = 1e3; /* DPM2D.cu:70 */
+  s->listMethod = 1;
+  s->threads = 1;
   s->wallForce = calloc(n2, 8);
   s->contactListSize = 0;
   s->contactList = NULL;
@@ -284,6 +291,7 @@ static double calc_angle(const double* n, const double* p) {
 
 /* kernelCalcParticleShape, cudaKernel.cuh:1571-1606 (one ring walked serially) */
 void orc_calc_particle_shape(orc_state* s) {
+#pragma omp parallel for num_threads(s->threads) if (s->threads > 1)
   for (long pId = 0; pId < s->numParticles; pId++) {
     double tempArea = 0.0;
     s->perimeter[pId] = 0.0;
@@ -317,6 +325,7 @@ void orc_calc_particle_shape(orc_state* s) {
 
 /* calcParticlePos, cudaKernel.cuh:365-388, launched by kernelCalcParticlePositions 1608-1613 */
 void orc_calc_particle_positions(orc_state* s) {
+#pragma omp parallel for num_threads(s->threads) if (s->threads > 1)
   for (long pId = 0; pId < s->numParticles; pId++) {
     double currentPos[2], delta[2], partPos[2];
     long firstId = s->firstVertexInParticleId[pId];
@@ -381,6 +390,7 @@ static double bending_force_energy(double eb, const double* preSeg, const double
 
 /* kernelCalcShapeForceEnergy, cudaKernel.cuh:474-529: zeroes F and overwrites energy */
 void orc_calc_shape_force_energy(orc_state* s) {
+#pragma omp parallel for num_threads(s->threads) if (s->threads > 1)
   for (long v = 0; v < s->numVertices; v++) {
     long p = s->particleIdList[v];
     double vertexPos[2], nextPos[2], previousPos[2], partPos[2], nextSegment[2], previousSegment[2];
@@ -549,6 +559,7 @@ double orc_pair_grad_multiple_and_energy(const orc_state* s, double distance, do
 /* kernelCalcVertexInteraction, cudaKernel.cuh:690-722 (gather over the Verlet row) */
 void orc_calc_vertex_interaction(orc_state* s) {
   long W = s->neighborListSize;
+#pragma omp parallel for num_threads(s->threads) if (s->threads > 1)
   for (long v = 0; v < s->numVertices; v++) {
     double thisPos[2] = {s->pos[v * 2], s->pos[v * 2 + 1]};
     double thisRad = s->rad[v];
@@ -565,6 +576,7 @@ void orc_calc_vertex_interaction(orc_state* s) {
 
 /* kernelCalcAllToAllVertexInteraction, cudaKernel.cuh:725-757 */
 void orc_calc_all_to_all_interaction(orc_state* s) {
+#pragma omp parallel for num_threads(s->threads) if (s->threads > 1)
   for (long v = 0; v < s->numVertices; v++) {
     double thisPos[2] = {s->pos[v * 2], s->pos[v * 2 + 1]};
     double thisRad = s->rad[v];
@@ -593,6 +605,17 @@ static double cross_of(const orc_state* s, const double* thisPos, const double* 
          pbcd(s, otherPos[0], thisPos[0], 0) * pbcd(s, previousPos[1], otherPos[1], 1);
 }
 
+/* a += of the scattered smooth interaction: atomic when the vertex loop runs on several threads
+ * (the reference's atomicAdd), plain in the serial order otherwise */
+static inline void acc(const orc_state* s, double* p, double v) {
+  if (s->threads > 1) {
+#pragma omp atomic
+    *p += v;
+  } else {
+    *p += v;
+  }
+}
+
 /* calcVertexSegmentInteraction, cudaKernel.cuh:877-897.  The reference divides by |cross|;
  * a vertex exactly on the segment line gives 0/0 there, the oracle takes sign 0. */
 static double vertex_segment(orc_state* s, const double* thisPos, const double* projPos,
@@ -605,12 +628,12 @@ static double vertex_segment(orc_state* s, const double* thisPos, const double* 
     double absCross = fabs(cross);
     double sign = absCross > 0 ? cross / absCross : 0.0;
     double L2 = length * length;
-    tF[0] += gm * sign * pbcd(s, previousPos[1], otherPos[1], 1) / length;
-    tF[1] += gm * sign * pbcd(s, otherPos[0], previousPos[0], 0) / length;
-    oF[0] += gm * (sign * pbcd(s, thisPos[1], previousPos[1], 1) + absCross * pbcd(s, previousPos[0], otherPos[0], 0) / L2) / length;
-    oF[1] += gm * (sign * pbcd(s, previousPos[0], thisPos[0], 0) + absCross * pbcd(s, previousPos[1], otherPos[1], 1) / L2) / length;
-    pF[0] += gm * (sign * pbcd(s, otherPos[1], thisPos[1], 1) - absCross * pbcd(s, previousPos[0], otherPos[0], 0) / L2) / length;
-    pF[1] += gm * (sign * pbcd(s, thisPos[0], otherPos[0], 0) - absCross * pbcd(s, previousPos[1], otherPos[1], 1) / L2) / length;
+    acc(s, &tF[0], gm * sign * pbcd(s, previousPos[1], otherPos[1], 1) / length);
+    acc(s, &tF[1], gm * sign * pbcd(s, otherPos[0], previousPos[0], 0) / length);
+    acc(s, &oF[0], gm * (sign * pbcd(s, thisPos[1], previousPos[1], 1) + absCross * pbcd(s, previousPos[0], otherPos[0], 0) / L2) / length);
+    acc(s, &oF[1], gm * (sign * pbcd(s, previousPos[0], thisPos[0], 0) + absCross * pbcd(s, previousPos[1], otherPos[1], 1) / L2) / length);
+    acc(s, &pF[0], gm * (sign * pbcd(s, otherPos[1], thisPos[1], 1) - absCross * pbcd(s, previousPos[0], otherPos[0], 0) / L2) / length);
+    acc(s, &pF[1], gm * (sign * pbcd(s, thisPos[0], otherPos[0], 0) - absCross * pbcd(s, previousPos[1], otherPos[1], 1) / L2) / length);
     return epot;
   }
   return 0.;
@@ -624,8 +647,8 @@ static double vertex_vertex(orc_state* s, const double* thisPos, const double* p
   if (gm != 0) {
     double distance = delta_and_distance(s, thisPos, previousPos, delta);
     for (int dim = 0; dim < 2; dim++) {
-      thisF[dim] += gm * delta[dim] / distance;
-      otherF[dim] += -gm * delta[dim] / distance;
+      acc(s, &thisF[dim], gm * delta[dim] / distance);
+      acc(s, &otherF[dim], -gm * delta[dim] / distance);
     }
     return epot;
   }
@@ -636,6 +659,7 @@ static double vertex_vertex(orc_state* s, const double* thisPos, const double* p
  * atomicAdd; the oracle applies the same updates in vertex order, then list order. */
 void orc_calc_smooth_interaction(orc_state* s) {
   long W = s->neighborListSize;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(s->threads) if (s->threads > 1)
   for (long v = 0; v < s->numVertices; v++) {
     double thisPos[2] = {s->pos[v * 2], s->pos[v * 2 + 1]};
     double thisRad = s->rad[v];
@@ -663,19 +687,19 @@ void orc_calc_smooth_interaction(orc_state* s) {
         double projPos[2] = {previousPos[0] + reduced * segment[0], previousPos[1] + reduced * segment[1]};
         double inter = vertex_segment(s, thisPos, projPos, otherPos, previousPos, length, radSum,
                                       &s->force[v * 2], &s->force[otherId * 2], &s->force[previousId * 2]);
-        s->particleEnergy[particleId] += inter;
-        s->particleEnergy[otherParticleId] += inter;
+        acc(s, &s->particleEnergy[particleId], inter);
+        acc(s, &s->particleEnergy[otherParticleId], inter);
         if (previousProj >= 0 && previousProj < 1) {
           isPreviousVertexSegment = 1;
           inter = vertex_vertex(s, previousPos, thisPos, radSum, &s->force[v * 2], &s->force[previousId * 2]);
-          s->particleEnergy[particleId] += -inter;
-          s->particleEnergy[otherParticleId] += -inter;
+          acc(s, &s->particleEnergy[particleId], -inter);
+          acc(s, &s->particleEnergy[otherParticleId], -inter);
         }
       } else if (proj < 0) {
         if (previousProj >= 1 && isPreviousVertexSegment == 0) {
           double inter = vertex_vertex(s, thisPos, previousPos, radSum, &s->force[v * 2], &s->force[previousId * 2]);
-          s->particleEnergy[particleId] += inter;
-          s->particleEnergy[otherParticleId] += inter;
+          acc(s, &s->particleEnergy[particleId], inter);
+          acc(s, &s->particleEnergy[otherParticleId], inter);
         }
       }
     }
@@ -824,13 +848,98 @@ static void build_list(orc_state* s, double cutDistance) {
   }
 }
 
+/* The same rows from a cell list (the host cell-list concept of DPM2D.cu:3011-3026, 3558-3579):
+ * vertices binned on wrapped coordinates into cells at least as wide as the largest list radius
+ * (1 + cutDistance) * 2 * max radius, so every neighbour of a vertex lies in its 3 x 3 block; the
+ * candidates that pass the reference's predicate are sorted, and the row keeps the smallest W ids
+ * as the all-pairs loop does.  Returns 0 (nothing built) when a box side holds fewer than three
+ * cells or the geometry is walled; the caller then runs the all-pairs loop. */
+static int build_list_cells(orc_state* s, double cutDistance) {
+  if (s->geometryType != ORC_GEOM_NORMAL) return 0;
+  double maxRad = 0;
+  for (long v = 0; v < s->numVertices; v++) if (s->rad[v] > maxRad) maxRad = s->rad[v];
+  double rc = (1 + cutDistance) * 2 * maxRad;
+  if (!(rc > 0)) return 0;
+  long ncx = (long)floor(s->boxSize[0] / rc), ncy = (long)floor(s->boxSize[1] / rc);
+  if (ncx < 3 || ncy < 3) return 0;
+  if (ncx > 4096) ncx = 4096;
+  if (ncy > 4096) ncy = 4096;
+  long nv = s->numVertices, nc = ncx * ncy;
+  long* cell = (long*)malloc(sizeof(long) * nv);
+  long* start = (long*)calloc(nc + 1, sizeof(long));
+  long* item = (long*)malloc(sizeof(long) * nv);
+  for (long v = 0; v < nv; v++) {
+    double wx = s->pos[2 * v] - s->boxSize[0] * floor(s->pos[2 * v] / s->boxSize[0]);
+    double wy = s->pos[2 * v + 1] - s->boxSize[1] * floor(s->pos[2 * v + 1] / s->boxSize[1]);
+    long cx = (long)(wx / s->boxSize[0] * ncx), cy = (long)(wy / s->boxSize[1] * ncy);
+    if (cx < 0) cx = 0;
+    if (cx >= ncx) cx = ncx - 1;
+    if (cy < 0) cy = 0;
+    if (cy >= ncy) cy = ncy - 1;
+    cell[v] = cy * ncx + cx;
+    start[cell[v] + 1]++;
+  }
+  for (long c = 0; c < nc; c++) start[c + 1] += start[c];
+  long* fill = (long*)malloc(sizeof(long) * nc);
+  memcpy(fill, start, sizeof(long) * nc);
+  for (long v = 0; v < nv; v++) item[fill[cell[v]]++] = v; /* ascending within each cell */
+  free(fill);
+  long W = s->neighborListSize;
+#pragma omp parallel num_threads(s->threads) if (s->threads > 1)
+  {
+    long cap = 64, *cand = (long*)malloc(sizeof(long) * cap);
+#pragma omp for schedule(dynamic, 64)
+    for (long v = 0; v < nv; v++) {
+      double thisPos[2] = {s->pos[v * 2], s->pos[v * 2 + 1]};
+      double thisRad = s->rad[v];
+      long particleId = s->particleIdList[v];
+      long cx = cell[v] % ncx, cy = cell[v] / ncx, n = 0;
+      for (long oy = -1; oy <= 1; oy++)
+        for (long ox = -1; ox <= 1; ox++) {
+          long c = ((cy + oy + ncy) % ncy) * ncx + (cx + ox + ncx) % ncx;
+          for (long q = start[c]; q < start[c + 1]; q++) {
+            long o = item[q];
+            if (s->particleIdList[o] == particleId) continue;
+            double otherPos[2] = {s->pos[o * 2], s->pos[o * 2 + 1]};
+            double radSum = thisRad + s->rad[o];
+            if (-(1 - calc_distance(s, thisPos, otherPos) / radSum) < cutDistance) {
+              if (n == cap) {
+                cap *= 2;
+                cand = (long*)realloc(cand, sizeof(long) * cap);
+              }
+              cand[n++] = o;
+            }
+          }
+        }
+      for (long i = 1; i < n; i++) { /* insertion sort: rows are short */
+        long x = cand[i], j = i - 1;
+        while (j >= 0 && cand[j] > x) { cand[j + 1] = cand[j]; j--; }
+        cand[j + 1] = x;
+      }
+      for (long k = 0; k < n && k < W; k++) s->neighborList[v * W + k] = cand[k];
+      s->maxNeighborList[v] = n;
+    }
+    free(cand);
+  }
+  free(cell); free(start); free(item);
+  return 1;
+}
+
+static void build_rows(orc_state* s, double cutDistance) {
+  if (!(s->listMethod == 1 && build_list_cells(s, cutDistance))) build_list(s, cutDistance);
+}
+
+void orc_set_list_method(orc_state* s, int method) { s->listMethod = method; }
+
+void orc_set_threads(orc_state* s, int threads) { s->threads = threads > 0 ? threads : 1; }
+
 /* calcNeighborList, DPM2D.cu:3479-3507: grow-only power-of-two row width */
 void orc_calc_neighbor_list(orc_state* s, double cutDistance) {
   long nv = s->numVertices;
   memset(s->maxNeighborList, 0, sizeof(long) * nv);
   long W = s->neighborListSize;
   for (long i = 0; i < nv * W; i++) s->neighborList[i] = -1;
-  build_list(s, cutDistance);
+  build_rows(s, cutDistance);
   long maxN = -1;
   for (long v = 0; v < nv; v++) if (s->maxNeighborList[v] > maxN) maxN = s->maxNeighborList[v];
   s->maxNeighbors = maxN;
@@ -839,7 +948,7 @@ void orc_calc_neighbor_list(orc_state* s, double cutDistance) {
     W = s->neighborListSize;
     s->neighborList = (long*)realloc(s->neighborList, sizeof(long) * nv * W);
     for (long i = 0; i < nv * W; i++) s->neighborList[i] = -1;
-    build_list(s, cutDistance);
+    build_rows(s, cutDistance);
   }
 }
 
@@ -864,9 +973,10 @@ void orc_reset_update_count(orc_state* s) { s->updateCount = 0; }
  * reference; it is treated as a no-op here. */
 int orc_check_neighbors(orc_state* s) {
   int any = 0;
+#pragma omp parallel for reduction(| : any) num_threads(s->threads) if (s->threads > 1)
   for (long v = 0; v < s->numVertices; v++) {
     double disp = calc_distance(s, &s->pos[v * 2], &s->lastPos[v * 2]);
-    if (3 * disp > s->cutoff) { any = 1; break; }
+    if (3 * disp > s->cutoff) any = 1;
   }
   if (any) {
     orc_calc_neighbor_list(s, s->cutDistance);
@@ -878,9 +988,11 @@ int orc_check_neighbors(orc_state* s) {
 
 /* ---------------------------------------------------------- integrators --- */
 static void update_position(orc_state* s, double ts) {
+#pragma omp parallel for num_threads(s->threads) if (s->threads > 1)
   for (long i = 0; i < 2 * s->numVertices; i++) s->pos[i] += ts * s->vel[i];
 }
 static void update_velocity(orc_state* s, double ts) {
+#pragma omp parallel for num_threads(s->threads) if (s->threads > 1)
   for (long i = 0; i < 2 * s->numVertices; i++) s->vel[i] += ts * s->force[i];
 }
 
@@ -957,6 +1069,7 @@ void orc_init_nve_rescale(orc_state* s, double T) {
 /* Langevin::updateThermalVel, Simulator.cu:54-77: F += noise*xi_ring - gamma*v */
 static void thermal_forces(orc_state* s) {
   uint64_t call = s->thermalCalls++;
+#pragma omp parallel for num_threads(s->threads) if (s->threads > 1)
   for (long v = 0; v < s->numVertices; v++) {
     long p = s->particleIdList[v];
     double z[2];

@@ -105,6 +105,10 @@ long orc_get_update_count(const orc_state* s);
 void orc_reset_update_count(orc_state* s);
 void orc_reset_last_positions(orc_state* s);
 double orc_get_cut_distance(const orc_state* s);
+/* row build of calcNeighborList: 1 = cell list (default; the same rows), 0 = all-pairs loop */
+void orc_set_list_method(orc_state* s, int method);
+/* OpenMP threads of the per-vertex loops (default 1: serial, deterministic order) */
+void orc_set_threads(orc_state* s, int threads);
 
 /* integrators (Simulator.cu:23-89, 206-238; DPM2D.cu:3713-3817) */
 void orc_set_noise_type(orc_state* s, int noiseType);

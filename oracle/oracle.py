@@ -32,6 +32,15 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE, "liboracle.so"], check=True)
 
 
+def use_library(path):
+    """Load the oracle from `path` (a -march=native build for the CPU baseline) instead of the
+    in-tree liboracle.so; must be called before the first Oracle is created."""
+    global _LIB_PATH, _lib
+    if _lib is not None and _LIB_PATH != path:
+        raise RuntimeError("oracle library already loaded from " + _LIB_PATH)
+    _LIB_PATH = path
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -77,6 +86,8 @@ def lib():
             "orc_reset_update_count": (None, [vp]),
             "orc_reset_last_positions": (None, [vp]),
             "orc_get_cut_distance": (d, [vp]),
+            "orc_set_list_method": (None, [vp, i]),
+            "orc_set_threads": (None, [vp, i]),
             "orc_init_langevin": (None, [vp, d, d, i, i]),
             "orc_set_noise_type": (None, [vp, i]),
             "orc_calc_contacts": (None, [vp]),
@@ -213,6 +224,13 @@ class Oracle:
 
     def calc_neighbor_list(self, cut):
         self.L.orc_calc_neighbor_list(self.h, cut)
+
+    def set_list_method(self, cells=True):
+        """Row build of calcNeighborList: cell list (default) or the reference's all-pairs loop."""
+        self.L.orc_set_list_method(self.h, 1 if cells else 0)
+
+    def set_threads(self, n):
+        self.L.orc_set_threads(self.h, n)
 
     def check_neighbors(self):
         return self.L.orc_check_neighbors(self.h)

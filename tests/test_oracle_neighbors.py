@@ -44,3 +44,25 @@ def test_rebuild_on_displacement(oracle_mod):
     assert o.update_count() == 1
     st2 = dict(st); st2["pos"] = pos
     assert o.neighbors()[0] == brute_sets(st2, cut)
+
+
+@pytest.mark.parametrize("pot,skin", [("harmonic", 0.3), ("wca", 0.5)])
+def test_cell_list_rows_equal_all_pairs(oracle_mod, pot, skin):
+    # a box of many cells, with rings translated by whole box lengths (positions are never
+    # wrapped): the cell-list build gives the reference's all-pairs rows, entry for entry
+    st = packing.random_system(300, 0.8, 31)
+    pos = st["pos"].reshape(-1, 2).copy()
+    ring = np.repeat(np.arange(300), st["nvp"])
+    pos += np.stack([(ring % 3 - 1), (ring % 5 - 2)], 1) * st["box"]
+    st["pos"] = pos.ravel()
+    rows = []
+    for cells in (True, False):
+        o = oracle_from_state(oracle_mod, st, potential=pot)
+        o.set_list_method(cells)
+        cut = o.set_displacement_cutoff(skin, 2 * st["rad"].mean())
+        o.calc_neighbor_list(cut)
+        r, W, cnt = o.neighbors()
+        rows.append((r, W, list(cnt)))
+    assert rows[0] == rows[1]
+    assert rows[0][0] == brute_sets(st, cut)
+    assert max(rows[0][2]) > 0
