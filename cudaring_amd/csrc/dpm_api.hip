@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <initializer_list>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -14,6 +15,15 @@
 
 using namespace dpm;
 
+// Scratch of dpm_energy_param_grad_noalloc: the four per-vertex partials (shares of dU/dka,
+// dU/dkl, dU/dkb and dU/dl0) before the per-ring sums, kScratchDoubles doubles = 8 MiB, enough
+// for sets of DPM_NOALLOC_MAX_VERTICES vertices.  Every GPU the library runs on holds its own
+// copy until the process ends; every call on one GPU shares it, so calls that overlap on one GPU
+// race.  In host code `scratch` names a host-side stand-in, not a device address: kernels and
+// copies take the per-device address that scratch_address() looks up.
+constexpr long kScratchDoubles = 4L * DPM_NOALLOC_MAX_VERTICES;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
 namespace {
 
 constexpr double kWCA = 1.122462048;  // defs.h:17
@@ -21,6 +31,24 @@ constexpr double kWCA = 1.122462048;  // defs.h:17
 int fail(dpm_set_s* S, int code, const std::string& msg) {
   if (S) S->err = msg;
   return code;
+}
+
+std::mutex g_scratchMu;
+std::vector<double*> g_scratchAddr;  // device address of `scratch`, per device (null: not looked up)
+
+// Look up (once per device, with `device` current) the device address of `scratch`.
+hipError_t scratch_address(int device, double** out) {
+  std::lock_guard<std::mutex> lock(g_scratchMu);
+  if (device < 0) return hipErrorInvalidDevice;
+  if ((int)g_scratchAddr.size() <= device) g_scratchAddr.resize(device + 1, nullptr);
+  if (!g_scratchAddr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratchAddr[device] = static_cast<double*>(p);
+  }
+  *out = g_scratchAddr[device];
+  return hipSuccess;
 }
 
 #define HIPCHK(expr)                                                                       \
@@ -846,6 +874,10 @@ int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPer
   if (!out || numSystems <= 0 || !ringsPerSystem || !numVertexInParticleList) return DPM_ERR_ARG;
   *out = nullptr;
   if (hipSetDevice(device) != hipSuccess) return DPM_ERR_HIP;
+  // the allocation-free entry point's buffer on this device, looked up once per device (a
+  // failure is reported by that entry point, which looks it up again)
+  double* scratchAddr = nullptr;
+  (void)scratch_address(device, &scratchAddr);
   dpm_set_s* S = new dpm_set_s();
   S->device = device;
   S->nsys = numSystems;
@@ -2065,6 +2097,24 @@ int dpm_energy_param_grad(dpm_set_t h, double* gring, double* gl0) {
   if (gring) HIPCHK(launch_ring_sum3(S, tmp, gring));
   if (gl0) HIPCHK(launch_copy_d(gl0, tmp + 3 * S->nv, S->nv, S->stream));
   HIPCHK(launch_fill_d(tmp, 0.0, 4 * S->nv, S->stream));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
+int dpm_energy_param_grad_noalloc(dpm_set_t h, double* gring, double* gl0) {
+  CHECK_HANDLE(h);
+  if (S->nv > DPM_NOALLOC_MAX_VERTICES)
+    return fail(S, DPM_ERR_ARG, "dpm_energy_param_grad_noalloc: the set has " + std::to_string(S->nv) +
+                                    " vertices, the scratch buffer holds " +
+                                    std::to_string((long)DPM_NOALLOC_MAX_VERTICES));
+  int rc = adjoint_supported(S);
+  if (rc) return rc;
+  double* tmp = nullptr;
+  HIPCHK(scratch_address(S->device, &tmp));
+  // every partial of the set is overwritten before it is read: nothing needs clearing
+  HIPCHK(launch_energy_param_partials(S, tmp));
+  if (gring) HIPCHK(launch_ring_sum3(S, tmp, gring));
+  if (gl0) HIPCHK(launch_copy_d(gl0, tmp + 3 * S->nv, S->nv, S->stream));
   HIPCHK(hipStreamSynchronize(S->stream));
   return DPM_OK;
 }

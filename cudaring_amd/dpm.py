@@ -110,6 +110,7 @@ SIGNATURES = {
     "dpm_scale_box": (_i, [_vp, _vp]),
     "dpm_force_hvp": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "dpm_energy_param_grad": (_i, [_vp, _vp, _vp]),
+    "dpm_energy_param_grad_noalloc": (_i, [_vp, _vp, _vp]),
     "dpm_nve_forward": (_i, [_vp, _l, _l]),
     "dpm_nve_backward": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "dpm_time_step_kernel": (_i, [_vp, _l, C.POINTER(_d), C.POINTER(_d)]),

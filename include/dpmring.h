@@ -297,6 +297,15 @@ int dpm_force_hvp(dpm_set_t h, const double* ux, const double* uy, double* outx,
  * with respect to the per-ring costs, gring = [dU/dka(R) | dU/dkl(R) | dU/dkb(R)], and to the
  * rest lengths, gl0[V]; either may be NULL. */
 int dpm_energy_param_grad(dpm_set_t h, double* gring, double* gl0);
+/* dpm_energy_param_grad without any device allocation, for callers that cannot allocate (code
+ * under HIP graph capture, tight serving or optimisation loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Its per-vertex partials live in a fixed
+ * library buffer of 4 x DPM_NOALLOC_MAX_VERTICES doubles per GPU (8 MiB, held until the process
+ * ends); sets with more vertices get DPM_ERR_ARG.  Calls on one GPU share that buffer: calls that
+ * overlap on one GPU (other streams or threads) race.  Same arguments and results as
+ * dpm_energy_param_grad. */
+#define DPM_NOALLOC_MAX_VERTICES 262144
+int dpm_energy_param_grad_noalloc(dpm_set_t h, double* gring, double* gl0);
 /* NVELoop() x steps (DPM2D.h:686) from the current state, keeping a checkpoint of positions,
  * velocities and forces every `every` steps for dpm_nve_backward. */
 int dpm_nve_forward(dpm_set_t h, long steps, long every);
