@@ -94,12 +94,24 @@ class DPMAdjoint:
 
 
 class _Trajectory(torch.autograd.Function):
+    """The set holds one checkpoint tape, the last nve_forward's: a node whose tape a later
+    trajectory (or a direct nve_forward) has replaced records its own again before the reverse
+    pass, from the inputs it keeps, so several trajectories can share one loss."""
+
     @staticmethod
-    def forward(ctx, x0, v0, ka, kl, kb, l0, adj, steps, every):
+    def _record(adj, x0, v0, ka, kl, kb, l0, steps, every):
         s = adj.s
         adj._prepare(x0, ka, kl, kb, l0)
         s.set("vel", v0.detach().cpu().numpy())
         s.nve_forward(steps, every)
+        return s.tape_serial
+
+    @staticmethod
+    def forward(ctx, x0, v0, ka, kl, kb, l0, adj, steps, every):
+        s = adj.s
+        ctx.tape = _Trajectory._record(adj, x0, v0, ka, kl, kb, l0, steps, every)
+        ctx.inputs = tuple(t.detach().clone() for t in (x0, v0, ka, kl, kb, l0))
+        ctx.steps, ctx.every = steps, every
         ctx.adj = adj
         ctx.device = x0.device
         xN = torch.from_numpy(s.get("pos")).to(x0.device)
@@ -109,6 +121,8 @@ class _Trajectory(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gx, gv):
         s = ctx.adj.s
+        if s.tape_serial != ctx.tape:  # the set's tape is another trajectory's: record this one again
+            ctx.tape = _Trajectory._record(ctx.adj, *ctx.inputs, ctx.steps, ctx.every)
         nv, np_ = s.nv, s.np
         dev = torch.device("cuda")
         z = torch.zeros(2 * nv, dtype=torch.float64, device=dev)

@@ -168,6 +168,7 @@ class DPMSet:
         vfirst = np.concatenate([[0], np.cumsum(nvp)])
         self.vert_off = vfirst[ring_off]
         self.ring_off = ring_off
+        self.tape_serial = 0  # number of nve_forward tapes recorded (nve_backward reads the last)
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
@@ -532,6 +533,7 @@ class DPMSet:
     def nve_forward(self, steps, every):
         """NVE steps with checkpoints every `every` steps for nve_backward."""
         self._chk(self.L.dpm_nve_forward(self.h, steps, every), "nve_forward")
+        self.tape_serial += 1
 
     def nve_backward(self, ax_ptr, av_ptr, gx_ptr, gv_ptr, gring_ptr, gl0_ptr):
         """Reverse pass of the last nve_forward (2V SoA adjoints in, gradients out)."""

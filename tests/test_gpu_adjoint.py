@@ -228,3 +228,35 @@ def test_autograd_energy_and_shape(oracle_mod):
     with torch.no_grad():
         fd = (loss(x0, calA0 + H * e) - loss(x0, calA0 - H * e)) / (2 * H)
     assert abs(float(calA0.grad[2]) - float(fd)) <= 1e-5 * max(abs(float(fd)), 1e-4)
+
+
+def test_two_trajectories_in_one_loss():
+    """Two trajectories of one set in one loss: each node's gradients equal those of its
+    trajectory differentiated alone (the second forward replaced the first one's tape)."""
+    from cudaring_amd.autograd import DPMAdjoint
+    st = soft_state(seed=23)
+    nr, dt, steps = len(st["nvp"]), 5e-3, 60
+    g, cut, size = nve_setup(st, *ring_costs(nr, 5), dt)
+    adj = DPMAdjoint(g, cut)
+    rng = np.random.default_rng(11)
+    ka, kl, kb = (dev(a) for a in ring_costs(nr, 5))
+    l0 = dev(st["l0"])
+    xa = dev(st["pos"])
+    xb = dev(st["pos"] + 0.01 * rng.standard_normal(2 * g.nv))
+    v0 = dev(0.05 * rng.standard_normal(2 * g.nv))
+    wa, wb = dev(rng.standard_normal(2 * g.nv)), dev(rng.standard_normal(2 * g.nv))
+
+    def alone(x, w):
+        x = x.clone().requires_grad_()
+        xN, _ = adj.trajectory(x, v0, ka, kl, kb, l0, steps, every=25)
+        (xN @ w).backward()
+        return x.grad.clone()
+
+    ga, gb = alone(xa, wa), alone(xb, wb)
+    xa_, xb_ = xa.clone().requires_grad_(), xb.clone().requires_grad_()
+    xNa, _ = adj.trajectory(xa_, v0, ka, kl, kb, l0, steps, every=25)
+    xNb, _ = adj.trajectory(xb_, v0, ka, kl, kb, l0, steps, every=25)
+    (xNa @ wa + xNb @ wb).backward()
+    scale = max(float(ga.abs().max()), float(gb.abs().max()))
+    assert float((xa_.grad - ga).abs().max()) <= 1e-12 * scale
+    assert float((xb_.grad - gb).abs().max()) <= 1e-12 * scale
