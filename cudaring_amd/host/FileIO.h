@@ -66,6 +66,11 @@ class ioDPMFile {
     energyFile << dpm_->getPhi() << std::endl;
   }
 
+  // the deformable-ring columns of saveEnergy (FileIO.h:71-82), as the run drivers call it
+  void saveDeformableEnergy(long step, double timeStep, long numVertices) {
+    saveEnergy(step, timeStep, dpm_->getNumParticles(), numVertices);
+  }
+
   // ---- scalar, 1-D and 2-D files
   std::vector<long> read1DIndexFile(const std::string& name, long rows) {
     std::vector<long> data;
@@ -139,6 +144,54 @@ class ioDPMFile {
       outputFile << std::endl;
     }
     outputFile.close();
+  }
+
+  // integer rows of `cols` entries, tab-separated (save2DIndexFile, FileIO.h:224-238)
+  void save2DIndexFile(const std::string& name, const std::vector<long>& data, long cols) {
+    openOutputFile(name);
+    long rows = cols > 0 ? (long)data.size() / cols : 0;
+    for (long r = 0; r < rows; r++) {
+      for (long c = 0; c < cols; c++) outputFile << data[r * cols + c] << "\t";
+      outputFile << std::endl;
+    }
+    outputFile.close();
+  }
+
+  // neighbour rows and ring contacts (saveNeighbors / saveInitialNeighbors, FileIO.h:531-550)
+  void saveNeighbors(const std::string& dir) {
+    save2DIndexFile(dir + "neighborList.dat", dpm_->getNeighbors(), dpm_->neighborListSize);
+  }
+  void saveInitialNeighbors(const std::string& dir, const std::string& file1 = "initialNeighbors.dat",
+                            const std::string& file2 = "initialContacts.dat") {
+    save2DIndexFile(dir + file1, dpm_->getNeighbors(), dpm_->neighborListSize);
+    save2DIndexFile(dir + file2, dpm_->getContacts(), dpm_->contactListSize);
+  }
+  // ring contact rows (saveContacts, FileIO.h:527-529)
+  void saveContacts(const std::string& dir) {
+    save2DIndexFile(dir + "contacts.dat", dpm_->getContacts(), dpm_->contactListSize);
+  }
+  void saveConfiguration(const std::string& dir) {
+    savePacking(dir);
+    saveNeighbors(dir);
+  }
+
+  // driveParams.dat: damping, and taup / f0 of active rings or l0Tau / l0Diff of activeShape
+  // rings (saveDriveParams, FileIO.h:557-574)
+  void saveDriveParams(const std::string& dir, double damping) {
+    std::ofstream p((dir + "driveParams.dat").c_str());
+    p << "damping" << "\t" << damping << std::endl;
+    if (dpm_->simControl.particleType == simControlStruct::particleEnum::active) {
+      double driving, taup;
+      dpm_->getSelfPropulsionParams(driving, taup);
+      p << "taup" << "\t" << taup << std::endl;
+      p << "f0" << "\t" << driving << std::endl;
+    } else if (dpm_->simControl.particleType == simControlStruct::particleEnum::activeShape) {
+      double l0Tau, l0Diff;
+      dpm_->getActiveTensionParams(l0Tau, l0Diff);
+      p << "l0Tau" << "\t" << l0Tau << std::endl;
+      p << "l0Diff" << "\t" << l0Diff << std::endl;
+    }
+    p.close();
   }
 
   // ---- packings (FileIO.h:270-326, 383-435)

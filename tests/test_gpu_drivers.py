@@ -207,3 +207,62 @@ def test_rigid_packing_round_trip(packing_dir, tmp_path):
         assert open(os.path.join(out1, name)).read() == open(os.path.join(out2, name)).read(), name
     np.testing.assert_allclose(load(os.path.join(out2, "particlePos.dat"), 2),
                                load(os.path.join(out1, "particlePos.dat"), 2), rtol=0, atol=1e-12)
+
+
+def test_run_nvt_reference_command_line(packing_dir, tmp_path, oracle_mod):
+    """runNVT with runNVT.cpp's argument list: the default mode writes the sample directory
+    langevinT<T>-damping<g>/ (harmonic: no potential suffix) inside the input directory, the energy
+    log every maxStep/1000 steps, states every maxStep/100; the trajectory is the oracle's.  A "run"
+    continues from the saved state into dynamics/."""
+    import shutil
+    pack = str(tmp_path / "pack") + "/"
+    shutil.copytree(packing_dir, pack)
+    steps, dt_units, T, damping, seed = 200, 2e-4, 1e-3, 1.0, 9
+    args = [os.path.join(BIN, "runNVT"), pack, str(dt_units), str(T), str(damping), str(steps), "0", "16",
+            "langevin", "harmonic"]
+    run(args + ["default", str(seed)])
+    out = pack + f"langevinT{T}-damping{damping}/"
+    assert os.path.isdir(pack + "langevin")
+    e = load(out + "energy.dat", 8)
+    assert e.shape[0] == steps and int(e[-1, 0]) == steps
+    assert sorted(int(d[1:]) for d in os.listdir(out) if d.startswith("t")) == list(range(0, steps, 2))
+    params = dict(ln.split("\t") for ln in open(out + "driveParams.dat").read().splitlines())
+    st = state_from_dir(packing_dir)
+    sigma = 2 * np.sqrt(np.mean(st["a0"]) / np.pi)
+    assert float(params["damping"]) == pytest.approx(damping / sigma, rel=1e-5)
+    o = oracle_mod.Oracle(st["nvp"])
+    o.load_state(st)
+    o.set_types("harmonic", "vertexSmooth")
+    o.set_energy_costs(1e5, 20, 10, 1)
+    o.set_dt(dt_units * sigma)
+    o.set_rng(seed, 0)
+    cut = o.set_displacement_cutoff(0.5, 2 * np.mean(st["rad"]))
+    o.calc_neighbor_list(cut)
+    o.calc_force_energy()
+    o.init_langevin(T, damping / sigma)
+    o.langevin(steps)
+    np.testing.assert_allclose(load(out + "positions.dat", 2).ravel(), o.get("pos"), rtol=0, atol=1e-10)
+    np.testing.assert_allclose(e[-1, 2], o.potential_energy() / len(st["rad"]), rtol=1e-9)
+    run(args + ["run", str(seed)])
+    dyn = load(out + "dynamics/energy.dat", 8)
+    assert dyn.shape[0] == steps and np.all(np.isfinite(dyn))
+
+
+def test_run_nve_and_active_reference_command_lines(packing_dir, tmp_path):
+    """runNVE and runActive with the reference's argument lists and directory names; both read the
+    saved state (as the reference does in every mode)."""
+    import shutil
+    pack = str(tmp_path / "pack") + "/"
+    shutil.copytree(packing_dir, pack)
+    run([os.path.join(BIN, "runNVE"), pack, "1e-4", "1e-3", "200", "0", "16", "wca", "smooth", "0", "default", "3"])
+    e = load(pack + "nve-wca/T1e-3/energy.dat", 8)
+    assert e.shape[0] == 200 and np.all(np.isfinite(e))
+    etot = e[:, 4]
+    assert np.abs(etot - etot[0]).max() < 1e-2 * max(abs(etot[0]), 1e-12)
+    run([os.path.join(BIN, "runActive"), pack, "1e-4", "1e-3", "1", "10", "0.1", "200", "0", "16", "default", "3"])
+    act = pack + "active/tp10-f00.1/"
+    e = load(act + "energy.dat", 8)
+    assert e.shape[0] == 200 and np.all(np.isfinite(e))
+    params = dict(ln.split("\t") for ln in open(act + "driveParams.dat").read().splitlines())
+    assert {"damping", "taup", "f0"} <= set(params)
+    assert os.path.exists(act + "t0/contacts.dat") and os.path.exists(act + "t0/positions.dat")
