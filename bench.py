@@ -10,7 +10,9 @@ timed region.
 
 One JSON line is printed with the roofline of the dominant kernel (the fused
 step kernel, timed live with hipEvents on its stream) and a CPU baseline (the
-oracle, single thread, on a bounded sample of the same workload).
+oracle built for this machine with OpenMP, at one thread and at the process's
+CPU share, on a bounded sample of the same workload), followed by the batched
+FIRE, adjoint, active-adhesive and rigid-ring figures.
 """
 import argparse
 import json
@@ -32,8 +34,11 @@ DT_UNITS = 1e-3                # dt = 1e-3 * sigma / sqrt(ec), sigma = mean part
 CHUNK = 250                    # steps per graph replay
 # HBM bytes per launch of the step kernel on this workload, from the FETCH_SIZE and WRITE_SIZE
 # counter runs of `bench.py --traffic` (each scaled by its reading for a 1 GiB device copy:
-# 2.000 and 1.000): 2 x 3873.0 KiB + 2252.1 KiB.  Re-measure after changing the step kernel.
-TRAFFIC_BYTES_PER_LAUNCH = 10238114
+# 2.000 and 1.000): 2 x 3381.8 KiB + 2249.1 KiB, at mean neighbour count 4.432.  The summary
+# with its commands is profiles/k_step_traffic_nvt1024_wca_smooth.md; re-measure after changing
+# the step kernel.
+TRAFFIC_BYTES_PER_LAUNCH = 9229005
+TRAFFIC_MEAN_NEIGHBORS = 4.432
 
 
 def alg_bytes_per_step(nv, np_, kbar):
@@ -79,28 +84,76 @@ def run_steps(s, steps):
         done += n
 
 
-def cpu_baseline(st, dt, steps):
-    """The oracle (single thread) on `steps` Langevin steps of the same prepared state."""
+def cpu_model():
+    """`lscpu` "Model name" of this machine (from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads():
+    """The CPU share of this process: OMP_NUM_THREADS when set, else the cores it may run on."""
+    n = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return n if n > 0 else len(os.sched_getaffinity(0))
+
+
+def native_oracle():
+    """Build the oracle -O3 -march=native -fopenmp for this machine (outside the tree) and load it;
+    returns the build description, or the in-tree portable build's when no compiler is found."""
+    import subprocess
+    import tempfile
     from oracle import oracle
-    o = oracle.Oracle(st["nvp"])
-    o.load_state(st)
-    o.set_types("wca", "vertexSmooth")
-    o.set_energy_costs(*COSTS)
-    o.set_dt(dt)
-    o.set_rng(SEED, 0)
-    size = 2 * float(np.mean(st["rad"]))
-    cut = o.set_displacement_cutoff(SKIN, size)
-    o.calc_neighbor_list(cut)
-    o.calc_force_energy()
-    o.init_langevin(T, GAMMA)
-    o.langevin(2)
-    t0 = time.perf_counter()
-    o.langevin(steps)
-    el = time.perf_counter() - t0
+    out = tempfile.mkdtemp(prefix="oracle_native_")
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native", f"OUT={out}"],
+                       capture_output=True, text=True)
+    if r.returncode == 0:
+        oracle.use_library(os.path.join(out, "liboracle_native.so"))
+        return "gcc -O3 -march=native -fopenmp -ffp-contract=off, built on this machine"
+    print(f"# native oracle build failed, timing the portable build: {r.stderr.strip()[:200]}", file=sys.stderr)
+    return "gcc -O3 -fopenmp -ffp-contract=off (portable in-tree build)"
+
+
+def cpu_baseline(st, dt, steps, runs=5):
+    """The oracle (the same algorithm: cell-list neighbour rows, rebuild on skin crossing, FP64) on
+    `steps` Langevin steps of the same prepared state, timed at one thread and at every thread of
+    this process's CPU share (OpenMP over vertices): one untimed run, then the median of `runs`."""
+    from oracle import oracle
+    build = native_oracle()
     nv = int(np.sum(st["nvp"]))
-    return {"value": nv * steps / el, "unit": "vertex-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} Langevin steps of the same {NRINGS}-ring WCA vertexSmooth state "
-                      f"(incl. its all-pairs neighbour rebuilds), one thread, {el:.1f} s"}
+    size = 2 * float(np.mean(st["rad"]))
+
+    def one_run(threads):
+        o = oracle.Oracle(st["nvp"])
+        o.load_state(st)
+        o.set_types("wca", "vertexSmooth")
+        o.set_energy_costs(*COSTS)
+        o.set_dt(dt)
+        o.set_rng(SEED, 0)
+        o.set_threads(threads)
+        cut = o.set_displacement_cutoff(SKIN, size)
+        o.calc_neighbor_list(cut)
+        o.calc_force_energy()
+        o.init_langevin(T, GAMMA)
+        t0 = time.perf_counter()
+        o.langevin(steps)
+        return time.perf_counter() - t0
+
+    def median_rate(threads):
+        one_run(threads)  # warm-up
+        return nv * steps / float(np.median([one_run(threads) for _ in range(runs)]))
+
+    n = cpu_threads()
+    single = median_rate(1)
+    multi = median_rate(n) if n > 1 else single
+    return {"value": multi, "unit": "vertex-steps/s", "cores": n, "kind": "port",
+            "single_thread": single, "cpu_model": cpu_model(), "build": build,
+            "sample": f"{steps} Langevin steps of the same {NRINGS}-ring WCA vertexSmooth state (cell-list rows, "
+                      f"rebuilt on skin crossing), median of {runs} after 1 warm-up, at {n} threads and at 1"}
 
 
 FIRE_SYSTEMS, FIRE_SEED = 512, 20261202 + 3
@@ -123,23 +176,29 @@ def fire_systems():
             for i, (n, c) in enumerate(zip(sizes, calA0))]
 
 
-def fire_setup(s, size):
-    """harmonic vertex-vertex, FIRE_COSTS, skin 0.3 of `size` (the same for every path)."""
+def fire_setup(s, size, skin=0.3, cell_scale=1.0):
+    """harmonic vertex-vertex, FIRE_COSTS, the neighbour skin (cutoff in units of `size`) and cell
+    scale (the same for every path)."""
     s.set_types("harmonic", "vertexVertex")
     s.set_energy_costs(*FIRE_COSTS)
-    return s.set_displacement_cutoff(0.3, size)
+    cut = s.set_displacement_cutoff(skin, size)
+    s.set_cell_scale(cell_scale)
+    return cut
 
 
-def fire_step(s, cut, plain):
+def fire_step(s, cut, mode):
     """One compression step of every system in `s` (compressFIRE.cpp:104-131, 153): grow by
-    sqrt((phi + dphi) / phi), rebuild, FIRE to max|F_c| <= tol.  Returns iterations per system."""
+    sqrt((phi + dphi) / phi), rebuild, FIRE to max|F_c| <= tol.  `mode` "engine": the set's own
+    fire_minimize (every system to its own convergence, persistent kernel); "host": the reference
+    driver's loop, one FIRE iteration and one host check of max|F_c| per call.  Returns
+    iterations per system."""
     phi = s.phi()
     s.scale_vertices(np.sqrt((phi + FIRE_DPHI) / phi))
     s.calc_neighbor_list(cut)
     s.calc_force_energy()
     s.reset_last_positions()
     s.init_fire(FIRE_PARAMS, 20, 0)
-    if not plain:
+    if mode == "engine":
         return s.fire_minimize(FIRE_TOL, FIRE_MAXIT)
     it, f = 0, s.max_unbalanced_force()[0]
     while f > FIRE_TOL and it < FIRE_MAXIT:  # the driver loop: one iteration, one host check
@@ -150,8 +209,10 @@ def fire_step(s, cut, plain):
 
 
 def batched_fire(args):
-    """cfg3: packings/s of the batched set against the plain one-system path."""
-    from cudaring_amd import DPMSet, packing
+    """cfg3: packings/s of the batched set against two one-system paths on a stratified sample of
+    the same packings: the engine's own fire_minimize on a set holding one system, and the
+    reference driver's loop (one iteration, one host check per call)."""
+    from cudaring_amd import DPMSet, autotune, packing
     systems = fire_systems()
     st = packing.concat_systems(systems)
     s = DPMSet(st["nvp"], rings_per_system=st["rings_per_system"])
@@ -164,58 +225,69 @@ def batched_fire(args):
     t0 = time.perf_counter()
     it0 = s.fire_minimize(FIRE_TOL, FIRE_MAXIT)  # relax the dilute start (untimed)
     relax_s = time.perf_counter() - t0
-    start = {k: s.get(k) for k in ("pos", "rad", "l0", "a0")}
+    start = {k: s.get(k) for k in ("pos", "vel", "rad", "l0", "a0")}  # FIRE keeps its velocities
     nv_sys = np.array([int(np.sum(x["nvp"])) for x in systems])
-    from cudaring_amd import autotune
     fire_kernel, fire_tune = autotune.tune_fire(s, FIRE_PARAMS, cut)
+    (skin, cell_scale), cut, nbr_tune = autotune.tune_fire_neighbors(s, FIRE_PARAMS, size)
 
     s.synchronize()
     t0 = time.perf_counter()
     iters = np.zeros(FIRE_SYSTEMS, dtype=np.int64)
     for _ in range(args.fire_steps):
-        iters += fire_step(s, cut, plain=False)
+        iters += fire_step(s, cut, "engine")
     s.synchronize()
     t_batch = time.perf_counter() - t0
     done = FIRE_SYSTEMS * args.fire_steps
     converged = int(np.sum(s.max_unbalanced_force() <= FIRE_TOL))
 
-    # plain path on a seeded sample of the same systems from the same relaxed start
-    rng = np.random.default_rng(FIRE_SEED + 1)
-    pick = np.sort(rng.choice(FIRE_SYSTEMS, args.fire_plain, replace=False))
+    # one-system paths on a sample stratified by size (every k-th packing in order of N), from the
+    # same relaxed start through the same phi steps
+    order = np.argsort(nv_sys, kind="stable")
+    pick = order[np.linspace(0, FIRE_SYSTEMS - 1, args.fire_plain).round().astype(int)]
     voff = np.concatenate([[0], np.cumsum(nv_sys)])
     roff = np.concatenate([[0], np.cumsum(st["rings_per_system"])])
-    t_plain, plain_iters = 0.0, 0
+    t_path = {"engine": 0.0, "host": 0.0}
+    it_path = {"engine": 0, "host": 0}
     for i in pick:
-        sub = dict(systems[i])
         sl, rl = slice(voff[i], voff[i + 1]), slice(roff[i], roff[i + 1])
+        sub = dict(systems[i])
         sub["pos"] = start["pos"][2 * voff[i]:2 * voff[i + 1]]
         sub["rad"], sub["l0"], sub["a0"] = start["rad"][sl], start["l0"][sl], start["a0"][rl]
-        p = DPMSet(sub["nvp"])
-        p.load_state(sub)
-        pcut = fire_setup(p, size)
-        p.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.fire_steps):
-            plain_iters += int(fire_step(p, pcut, plain=True)[0])
-        p.synchronize()
-        t_plain += time.perf_counter() - t0
-        p.close()
-    plain_rate = args.fire_plain * args.fire_steps / t_plain
+        for mode in ("engine", "host"):
+            p = DPMSet(sub["nvp"])
+            p.load_state(sub)
+            p.set("vel", start["vel"][2 * voff[i]:2 * voff[i + 1]])
+            pcut = fire_setup(p, size, skin, cell_scale)
+            p.set_fire_kernel(*fire_kernel)
+            p.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.fire_steps):
+                it_path[mode] += int(fire_step(p, pcut, mode)[0])
+            p.synchronize()
+            t_path[mode] += time.perf_counter() - t0
+            p.close()
+    n_sample = len(pick) * args.fire_steps
+    engine_rate, host_rate = n_sample / t_path["engine"], n_sample / t_path["host"]
     batch_rate = done / t_batch
     vit = float(np.sum(iters * nv_sys))
     s.close()
-    return {"packings_per_s": round(batch_rate, 2), "plain_packings_per_s": round(plain_rate, 3),
-            "speedup_vs_plain": round(batch_rate / plain_rate, 2),
+    return {"packings_per_s": round(batch_rate, 2),
+            "single_system_packings_per_s": round(engine_rate, 3),
+            "speedup_vs_single_system": round(batch_rate / engine_rate, 2),
+            "plain_packings_per_s": round(host_rate, 3),
+            "speedup_vs_plain": round(batch_rate / host_rate, 2),
             "systems": FIRE_SYSTEMS, "vertices": int(nv_sys.sum()), "phi_steps": args.fire_steps,
             "dphi": FIRE_DPHI, "tol": FIRE_TOL, "converged_systems": converged,
             "batched_s": round(t_batch, 3), "iterations_mean": round(float(iters.mean()), 1),
             "iterations_max": int(iters.max()), "vertex_fire_iterations_per_s": round(vit / t_batch, 1),
-            "plain_sample": f"{args.fire_plain} of the {FIRE_SYSTEMS} systems, same start and phi steps, "
-                            f"one DPM2D-style set each, host check per iteration: {plain_iters} iterations "
-                            f"in {t_plain:.2f} s",
+            "sample": f"{len(pick)} of the {FIRE_SYSTEMS} packings, every {FIRE_SYSTEMS // len(pick)}-th in order of "
+                      f"size (N {int(nv_sys[pick].min())}..{int(nv_sys[pick].max())} vertices), same start and phi "
+                      f"steps, one set each: the engine's fire_minimize {it_path['engine']} iterations in "
+                      f"{t_path['engine']:.2f} s; the driver loop with a host check per iteration "
+                      f"{it_path['host']} iterations in {t_path['host']:.2f} s",
             "start_relax_iterations_max": int(it0.max()), "start_relax_s": round(relax_s, 2),
             "fire_kernel": {"persistent": bool(fire_kernel[0]), "waves": int(fire_kernel[1])},
-            "fire_tune_s": fire_tune}
+            "fire_tune_s": fire_tune, "skin": skin, "cell_scale": cell_scale, "neighbor_tune_s": nbr_tune}
 
 
 ADJ_RINGS, ADJ_PHI, ADJ_SEED, ADJ_T = 256, 0.85, 20261202 + 4, 1e-3
@@ -448,7 +520,7 @@ def traffic_run(s, torch, args):
         dst.copy_(src)
     torch.cuda.synchronize()
     del src, dst
-    run_steps(s, args.warmup + args.steps)
+    run_steps(s, args.warmup)
     us, kbar = s.time_step_kernel(50)
     s.synchronize()
     print(json.dumps({"traffic_run": True, "copy_bytes_each_way": 8 * COPY_ELEMS, "step_launches": 50,
@@ -462,12 +534,15 @@ def main():
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=2000)
     ap.add_argument("--prep-iters", type=int, default=1000)
-    ap.add_argument("--cpu-steps", type=int, default=600)
+    ap.add_argument("--equil", type=int, default=20000,
+                    help="untimed Langevin steps from the prepared packing before warm-up and timing")
+    ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ring-order", choices=("random", "spatial"), default="random",
                     help="ring numbering of the prepared packing: as generated, or along a Morton curve")
     ap.add_argument("--fire-steps", type=int, default=5, help="phi steps of the batched FIRE compression")
-    ap.add_argument("--fire-plain", type=int, default=8, help="systems timed on the plain one-system path")
+    ap.add_argument("--fire-plain", type=int, default=16,
+                    help="packings (stratified by size) timed on the one-system paths")
     ap.add_argument("--no-fire", action="store_true")
     ap.add_argument("--adj-steps", type=int, default=2000)
     ap.add_argument("--no-adjoint", action="store_true")
@@ -475,6 +550,7 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true")
     ap.add_argument("--cfg5", action="store_true", help="large active + adhesive system (16384 rings)")
     ap.add_argument("--cfg5-batched", action="store_true", help="64 active + adhesive systems in one set")
+    ap.add_argument("--no-cfg5", action="store_true", help="skip the large active + adhesive run of the default line")
     ap.add_argument("--cfg5-steps", type=int, default=2000)
     ap.add_argument("--cfg5-dphi", type=float, default=0.02)
     ap.add_argument("--cfg5-iters", type=int, default=2000)
@@ -503,6 +579,14 @@ def main():
         return
     s, st, dt, prep_s, tuned = prepare(args)
     nv, np_ = s.nv, s.np
+    # untimed: relax the prepared packing to the thermostat's steady state (its neighbour count and
+    # rebuild interval), counting the rebuilds of the equilibration's second half
+    run_steps(s, args.equil // 2)
+    s.synchronize()
+    s.reset_update_count()
+    run_steps(s, args.equil - args.equil // 2)
+    s.synchronize()
+    equil_rebuilds = int(s.update_count()[0])
     if args.traffic:
         traffic_run(s, torch, args)
         return
@@ -513,7 +597,10 @@ def main():
         print(json.dumps({"rigid": rigid_bench(s, dt, 5000)}), flush=True)
         return
 
+    # warm-up, then one replay of every chunk shape the timed loop uses (their graphs captured)
     run_steps(s, args.warmup)
+    for n in sorted({min(CHUNK, args.steps), args.steps % CHUNK} - {0}):
+        run_steps(s, n)
     s.synchronize()
     torch.cuda.synchronize()
     s.reset_update_count()
@@ -534,6 +621,8 @@ def main():
     rigid = None if args.no_rigid else rigid_bench(s, dt, 2000)
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": TRAFFIC_BYTES_PER_LAUNCH,
+            "traffic_mean_neighbors": TRAFFIC_MEAN_NEIGHBORS,
+            "traffic_source": "profiles/k_step_traffic_nvt1024_wca_smooth.md",
             "kernel": "k_step<vertexSmooth,WCA,kick|langevin|start>", "kernel_us": round(us, 3),
             "alg_bytes_per_launch": int(alg), "mean_neighbors": round(kbar, 3),
             "step_us": round(1e6 * el / args.steps, 3)}
@@ -543,6 +632,7 @@ def main():
         cpu = cpu_baseline(st, dt, args.cpu_steps)
     fire = None if args.no_fire else batched_fire(args)
     adjoint = None if args.no_adjoint else adjoint_bench(args, torch)
+    active = None if args.no_cfg5 else cfg5_bench(args, torch)
     out = {
         "metric": METRIC, "value": value, "unit": "vertex-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True,
@@ -550,11 +640,19 @@ def main():
         "config": {"workload": "NVT Langevin, 1024 bidisperse rings 16/22 (19456 vertices), WCA "
                                "vertex-segment, phi=0.85, T=1e-2, gamma=1, dt=1e-3 sigma, skin 0.5",
                    "vertices": nv, "rings": np_, "dt": dt, "rebuilds_in_timed_region": rebuilds,
-                   "T_end": temp, "prep_s": round(prep_s, 1), "autotune": tuned},
+                   "equilibration_steps": args.equil,
+                   "steady_rebuild_interval": (round((args.equil - args.equil // 2) / equil_rebuilds, 1)
+                                               if equil_rebuilds else None),
+                   "T_end": temp, "prep_s": round(prep_s, 1), "autotune": tuned,
+                   "vs_reference_driver": "shape costs ka, kl, kb = 1, 1, 0.01 (runNVT.cpp:38 uses "
+                                          "1e5, 20, 10) and gamma = 1 in simulation units (runNVT.cpp:140 "
+                                          "divides the damping by its time unit); neither changes the "
+                                          "work per step"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "batched_fire": fire,
         "adjoint": adjoint,
+        "active_adhesive": active,
         "rigid": rigid,
     }
     print(json.dumps(out), flush=True)

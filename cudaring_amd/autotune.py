@@ -7,7 +7,8 @@ tiles the set has.  `tune_waves_per_tile` times the fused Langevin step kernel
 for each candidate on the caller's own state (velocities are restored by the
 library after timing) and keeps the fastest.  `tune_skin` picks the neighbour
 skin (setDisplacementCutoff's `cutoff`) that minimises the measured time of a
-short stretch of the real loop, rebuilds included.
+short stretch of the real loop, rebuilds included; `tune_fire_neighbors` picks the skin and the
+cell size of the batched FIRE minimisation the same way.
 """
 import time
 
@@ -71,6 +72,38 @@ def tune_fire(s, params7, cut, candidates=((1, 2), (1, 4), (1, 8), (0, 0)), iter
     s.calc_force_energy()
     s.set_fire_kernel(*best)
     return best, {f"{'persistent' if p else 'graph'}{w or ''}": round(t, 4) for (p, w), t in times.items()}
+
+
+def tune_fire_neighbors(s, params7, size, skins=(0.2, 0.3, 0.5, 0.8), scales=(1.0, 1.5, 2.0), iters=200):
+    """Pick the neighbour skin (setDisplacementCutoff's `cutoff`, in units of `size`) and the
+    cell scale (dpm_set_cell_scale) minimising the time of `iters` fire_minimize iterations from
+    the caller's state, which is restored after every trial and at the end.  The forces, and so
+    the FIRE trajectory, do not depend on either: pairs the wider list adds lie beyond the
+    potential's range."""
+    keep = {k: s.get(k) for k in ("pos", "vel")}
+
+    def restore(skin, scale):
+        for k, v in keep.items():
+            s.set(k, v)
+        cut = s.set_displacement_cutoff(skin, size)
+        s.set_cell_scale(scale)
+        s.calc_neighbor_list(cut)
+        s.calc_force_energy()
+        s.init_fire(params7, 20, 0)
+        return cut
+
+    times = {}
+    for skin in skins:
+        for scale in scales:
+            restore(skin, scale)
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.fire_minimize(0.0, iters)
+            s.synchronize()
+            times[(skin, scale)] = time.perf_counter() - t0
+    best = min(times, key=times.get)
+    cut = restore(*best)
+    return best, cut, {f"skin{k[0]}_cells{k[1]}": round(t, 4) for k, t in times.items()}
 
 
 def summary(times):

@@ -25,7 +25,7 @@ HOST = os.path.join(HERE, "host")
 DRIVERS = os.path.join(HERE, "drivers")
 BIN = os.path.join(HERE, "bin")
 HOST_SOURCES = ["DPM2D.cpp", "FIRE.cpp", "Simulator.cpp"]
-DRIVER_SOURCES = ["compressFIRE.cpp", "compressNVE.cpp", "runDPM.cpp", "runRigid.cpp", "testInteraction.cpp",
+DRIVER_SOURCES = ["compressFIRE.cpp", "makeDPM.cpp", "compressNVE.cpp", "runDPM.cpp", "runRigid.cpp", "testInteraction.cpp",
                   "runNVE.cpp", "runNVT.cpp", "runActive.cpp"]
 CXX = os.environ.get("CXX", "g++")
 CXXFLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-variable"]

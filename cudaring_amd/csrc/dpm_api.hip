@@ -179,7 +179,7 @@ int configure_cells(dpm_set_s* S) {
   for (long s = 0; s < B; s++) {
     // list radius, plus the cutoff / 3 a vertex may move between the step loop's binning and
     // its rebuild
-    const double rc = (1 + S->P.cutDistance) * 2 * S->h_maxrad[s] + S->P.cutoff / 3;
+    const double rc = ((1 + S->P.cutDistance) * 2 * S->h_maxrad[s] + S->P.cutoff / 3) * S->cellScale;
     const double Lx = S->h_box[2 * s], Ly = S->h_box[2 * s + 1];
     int nx = rc > 0 ? (int)floor(Lx / rc) : 0;
     int ny = rc > 0 ? (int)floor(Ly / rc) : 0;
@@ -243,11 +243,18 @@ int alloc_list(dpm_set_s* S, int W) {
   return DPM_OK;
 }
 
-// Rebuild the rows of the systems flagged in `flags` (host array) from the current positions.
-// calcNeighborList (countUpdates = 0) and checkNeighbors (countUpdates = 1) semantics.  A
-// wider row needs new storage: then every system is rebuilt into it before returning.
-int rebuild(dpm_set_s* S, const std::vector<int>& flags, int countUpdates) {
+// Rebuild the rows of the systems flagged in `flags` (host array) from the current positions,
+// or with `atListPositions` from the positions of the last rebuild (lastPos: the rows as they
+// were built then).  calcNeighborList (countUpdates = 0) and checkNeighbors (countUpdates = 1)
+// semantics.  A wider row needs new storage: then every system is rebuilt into it before
+// returning.
+int rebuild(dpm_set_s* S, const std::vector<int>& flags, int countUpdates, bool atListPositions = false) {
   const int b = S->cur;
+  View Vl = S->V;
+  if (atListPositions) {
+    Vl.x = S->V.lx;
+    Vl.y = S->V.ly;
+  }
   for (int attempt = 0; attempt < 8; attempt++) {
     int any = 0;
     for (int f : flags) any |= f;
@@ -256,8 +263,13 @@ int rebuild(dpm_set_s* S, const std::vector<int>& flags, int countUpdates) {
     HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
     HIPCHK(launch_fill_int(S->d_maxcnt, 0, 1, S->stream));
     HIPCHK(launch_fill_int(S->G.count[b], 0, S->G.ncells, S->stream));
-    HIPCHK(launch_bin(S, 0, 0));
-    HIPCHK(launch_build_rows(S, countUpdates));
+    if (atListPositions) {
+      HIPCHK(launch_bin_view(S, Vl, b));
+      HIPCHK(launch_build_rows_view(S, Vl, b, countUpdates));
+    } else {
+      HIPCHK(launch_bin(S, 0, 0));
+      HIPCHK(launch_build_rows(S, countUpdates));
+    }
     int err = 0, maxcnt = 0;
     HIPCHK(hipMemcpyAsync(&err, S->d_err, sizeof(int), hipMemcpyDeviceToHost, S->stream));
     HIPCHK(hipMemcpyAsync(&maxcnt, S->d_maxcnt, sizeof(int), hipMemcpyDeviceToHost, S->stream));
@@ -284,7 +296,7 @@ int rebuild(dpm_set_s* S, const std::vector<int>& flags, int countUpdates) {
       int rc = alloc_list(S, std::max(want, S->V.W));
       if (rc) return rc;
       std::vector<int> all(S->nsys, 1);
-      return rebuild(S, all, 0);
+      return rebuild(S, all, 0, atListPositions);
     }
     S->listBuilt = true;
     return DPM_OK;
@@ -292,16 +304,21 @@ int rebuild(dpm_set_s* S, const std::vector<int>& flags, int countUpdates) {
   return fail(S, DPM_ERR_OVERFLOW, "neighbour list rebuild did not converge");
 }
 
-int check_err_after_loop(dpm_set_s* S) {
-  int err = 0;
-  HIPCHK(hipMemcpyAsync(&err, S->d_err, sizeof(int), hipMemcpyDeviceToHost, S->stream));
-  HIPCHK(hipStreamSynchronize(S->stream));
+// status of a loop whose kernels left the error bits `err`
+int loop_status(dpm_set_s* S, int err) {
   if (err & ERR_STALE_BINS)
     return fail(S, DPM_ERR_STATE, "a vertex moved more than cutoff / 3 in one step: the neighbour-list skin "
                                   "does not cover the time step");
   if (err) return fail(S, DPM_ERR_OVERFLOW, err & ERR_LIST_OVERFLOW ? "neighbour list overflow during loop"
                                                                     : "cell overflow during loop");
   return DPM_OK;
+}
+
+int check_err_after_loop(dpm_set_s* S) {
+  int err = 0;
+  HIPCHK(hipMemcpyAsync(&err, S->d_err, sizeof(int), hipMemcpyDeviceToHost, S->stream));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return loop_status(S, err);
 }
 
 // area of system s's domain: the box, or pi R^2 for the round box (getPhi, DPM2D.cu:1208-1246)
@@ -325,6 +342,103 @@ int require_list(dpm_set_s* S) {
   if (S->P.interaction == INT_ALL) return DPM_OK;
   if (!S->listBuilt) return fail(S, DPM_ERR_STATE, "neighbour list not built: call dpm_calc_neighbor_list first");
   return DPM_OK;
+}
+
+// ------------------------------------------------------- replay after a list overflow ---
+// A loop's in-kernel rebuild that finds a row longer than the list width W keeps W entries and
+// flags ERR_LIST_OVERFLOW; the loop's later forces would miss pairs.  Each loop therefore runs
+// from a copy of what it changes: on that error the set is put back to the copy, the list is
+// widened with its rows rebuilt from the lastPos positions (the rows the loop started with) and
+// the loop runs again, so no force ever uses a truncated row.  The reference grows its width in
+// calcNeighborList before any force reads the rows (DPM2D.cu:3479-3507).
+enum : int { SNAP_FIRE = 1, SNAP_RIGID = 2, SNAP_RIGID_FIRE = 4 };
+
+// the (device pointer, 4-byte words) pairs a loop of kind `what` changes
+std::vector<std::pair<unsigned*, long>> snap_fields(dpm_set_s* S, int what) {
+  const long nv = S->nv, np = S->np, ns = S->nsys;
+  auto w = [](void* p, long bytes) { return std::make_pair((unsigned*)p, bytes / 4); };
+  std::vector<std::pair<unsigned*, long>> f = {
+      w(S->V.x, 8 * nv),  w(S->V.y, 8 * nv),  w(S->V.vx, 8 * nv), w(S->V.vy, 8 * nv),
+      w(S->V.fx, 8 * nv), w(S->V.fy, 8 * nv), w(S->V.lx, 8 * nv), w(S->V.ly, 8 * nv),
+      w(S->V.l0, 8 * nv), w(S->V.ang, 8 * np), w(S->V.cx, 8 * np), w(S->V.cy, 8 * np),
+      w(S->d_ctr, 4 * sizeof(unsigned long long)), w(S->d_updateCount, 4 * ns),
+      w(S->d_flag[0], 4 * ns), w(S->d_flag[1], 4 * ns), w(S->d_any, 4 * 3)};
+  if (what & SNAP_FIRE) f.push_back(w(S->d_fire[S->fireCur], sizeof(FireState) * ns));
+  if (what & (SNAP_RIGID | SNAP_RIGID_FIRE)) {
+    const Rigid& R = S->R;
+    for (double* p : {R.pvx, R.pvy, R.pw, R.pfx, R.pfy, R.ptq, R.moi, R.plx, R.ply, R.pla}) f.push_back(w(p, 8 * np));
+    f.push_back(w(R.vtq, 8 * nv));
+  }
+  if (what & SNAP_RIGID_FIRE) f.push_back(w(S->d_rfire[S->rfireCur], sizeof(FireState) * ns));
+  return f;
+}
+
+int snap_copy(dpm_set_s* S, int what, bool save) {
+  const auto f = snap_fields(S, what);
+  long words = 0;
+  for (auto& q : f) words += q.second;
+  if (save && words > S->snapWords) {
+    HIPCHK(hipStreamSynchronize(S->stream));
+    hipFree(S->d_snap);
+    S->d_snap = nullptr;
+    HIPCHK(dalloc(&S->d_snap, words));
+    S->snapWords = words;
+  }
+  CopySegs c{};
+  long off = 0;
+  for (auto& q : f) {
+    if (c.n == kMaxCopySegs) return fail(S, DPM_ERR_STATE, "loop snapshot: too many fields");
+    c.src[c.n] = save ? q.first : S->d_snap + off;
+    c.dst[c.n] = save ? S->d_snap + off : q.first;
+    c.words[c.n] = q.second;
+    c.n++;
+    off += q.second;
+  }
+  HIPCHK(launch_copy_segs(c, S->stream));
+  return DPM_OK;
+}
+
+int snap_save(dpm_set_s* S, int what) {
+  S->snapCur = S->cur;
+  S->snapFireCur = S->fireCur;
+  S->snapRfireCur = S->rfireCur;
+  return snap_copy(S, what, true);
+}
+
+int snap_restore(dpm_set_s* S, int what) {
+  S->cur = S->snapCur;
+  S->fireCur = S->snapFireCur;
+  S->rfireCur = S->snapRfireCur;
+  set_pos_view(S);
+  return snap_copy(S, what, false);
+}
+
+// Run `loop` (which leaves its errors in d_err) from a copy of the state it changes, and again on
+// a wider list for as long as it reports ERR_LIST_OVERFLOW; returns the loop's status (one host
+// synchronisation per run).
+template <class F>
+int with_replay(dpm_set_s* S, int what, F loop) {
+  int rc = snap_save(S, what);
+  if (rc) return rc;
+  for (int attempt = 0; attempt < 6; attempt++) {
+    HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
+    rc = loop();
+    if (rc) return rc;
+    int err = 0;
+    HIPCHK(hipMemcpyAsync(&err, S->d_err, sizeof(int), hipMemcpyDeviceToHost, S->stream));
+    HIPCHK(hipStreamSynchronize(S->stream));
+    if (!(err & ERR_LIST_OVERFLOW)) return loop_status(S, err);
+    rc = snap_restore(S, what);
+    if (rc) return rc;
+    rc = alloc_list(S, 2 * S->V.W);
+    if (rc) return rc;
+    std::vector<int> all(S->nsys, 1);
+    rc = rebuild(S, all, 0, true);
+    if (rc) return rc;
+    rc = snap_restore(S, what);  // the rebuild cleared the rebuild flags the loop started with
+    if (rc) return rc;
+  }
+  return fail(S, DPM_ERR_OVERFLOW, "neighbour list still overflowing after widening it six times");
 }
 
 // One loop step's launches at position parity cur: bin (normally idle) + step.  The FIRE step
@@ -488,11 +602,8 @@ int run_graph(dpm_set_s* S, std::tuple<int, long, int> key, F seq) {
 bool graphs_enabled() { return getenv("DPM_NO_GRAPH") == nullptr; }
 
 // Run `steps` Verlet steps (a captured graph per (kind, steps, parity) when steps >= 4).
-int run_verlet(dpm_set_s* S, long steps, bool langevin) {
-  if (steps <= 0) return DPM_OK;
-  int rc = require_list(S);
-  if (rc) return rc;
-  HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
+int run_verlet_once(dpm_set_s* S, long steps, bool langevin) {
+  int rc;
   const int noise = langevin ? S->noiseType : -1;
   auto seq = [&]() -> int {
     switch (noise) {
@@ -509,6 +620,13 @@ int run_verlet(dpm_set_s* S, long steps, bool langevin) {
   if (rc) return rc;
   if ((steps % 2) == 1) swap_pos(S);
   return DPM_OK;
+}
+
+int run_verlet(dpm_set_s* S, long steps, bool langevin) {
+  if (steps <= 0) return DPM_OK;
+  int rc = require_list(S);
+  if (rc) return rc;
+  return with_replay(S, 0, [&]() { return run_verlet_once(S, steps, langevin); });
 }
 
 int fire_iteration(dpm_set_s* S) {
@@ -572,8 +690,9 @@ int clean_protocol(dpm_set_s* S) {
 }
 
 int adjoint_supported(dpm_set_s* S) {
-  if (S->P.interaction != INT_VV)
-    return fail(S, DPM_ERR_UNSUPPORTED, "adjoint: vertexVertex interaction only");
+  if (S->nwide > 0) return fail(S, DPM_ERR_UNSUPPORTED, "adjoint: rings of at most 64 vertices only");
+  if (S->P.interaction != INT_VV && S->P.interaction != INT_SMOOTH)
+    return fail(S, DPM_ERR_UNSUPPORTED, "adjoint: vertexVertex or vertexSmooth interaction only");
   if (S->P.monomer != MONO_HARMONIC) return fail(S, DPM_ERR_UNSUPPORTED, "adjoint: harmonic monomers only");
   if (S->P.particle != PART_PASSIVE) return fail(S, DPM_ERR_UNSUPPORTED, "adjoint: passive particles only");
   if (S->P.geometry != GEOM_NORMAL) return fail(S, DPM_ERR_UNSUPPORTED, "adjoint: periodic geometry only");
@@ -770,6 +889,7 @@ int displacement_sums(dpm_set_s* S, bool rings, double waveNum, std::vector<doub
 
 // ------------------------------------------------------------- rigid rings ---
 int rigid_supported(dpm_set_s* S) {
+  if (S->nwide > 0) return fail(S, DPM_ERR_UNSUPPORTED, "rigid rings: rings of at most 64 vertices only");
   if (S->P.interaction != INT_VV && S->P.interaction != INT_SMOOTH)
     return fail(S, DPM_ERR_UNSUPPORTED, "rigid rings: vertexVertex or vertexSmooth interaction only");
   if (S->P.geometry != GEOM_NORMAL) return fail(S, DPM_ERR_UNSUPPORTED, "rigid rings: periodic geometry only");
@@ -831,16 +951,13 @@ int run_rigid(dpm_set_s* S, long steps, bool langevin) {
   if (steps <= 0) return DPM_OK;
   int rc = rigid_supported(S);
   if (rc) return rc;
-  HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
   auto seq = [&]() -> int { return rigid_sequence(S, steps, langevin); };
-  if (steps < 4 || !graphs_enabled()) {
-    rc = seq();
-  } else {
-    rc = run_graph(S, std::make_tuple(langevin ? 20 : 21, steps, S->cur), seq);
-    if (!rc && (steps % 2) == 1) swap_pos(S);
-  }
-  if (rc) return rc;
-  return check_err_after_loop(S);
+  return with_replay(S, SNAP_RIGID, [&]() -> int {
+    if (steps < 4 || !graphs_enabled()) return seq();
+    int r = run_graph(S, std::make_tuple(langevin ? 20 : 21, steps, S->cur), seq);
+    if (!r && (steps % 2) == 1) swap_pos(S);
+    return r;
+  });
 }
 
 // rigid FIRE iterations: the tile partials of the current state, then per iteration the ring
@@ -888,7 +1005,8 @@ int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPer
     if (ringsPerSystem[s] <= 0) { delete S; return DPM_ERR_ARG; }
     for (long k = 0; k < ringsPerSystem[s]; k++) {
       const long n = numVertexInParticleList[np];
-      if (n < 3 || n > 64) { delete S; return DPM_ERR_UNSUPPORTED; }
+      if (n < 3 || n > (1L << 24)) { delete S; return DPM_ERR_UNSUPPORTED; }
+      if (n > 64) S->h_wide.push_back((int)np);
       S->h_rnv.push_back((int)n);
       S->h_rfirst.push_back((int)nv);
       S->h_rsys.push_back((int)s);
@@ -917,9 +1035,15 @@ int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPer
         v1 += S->h_rnv[r];
         r++;
       }
-      if (v1 == v0) {  // a ring larger than the cap gets a tile of its own
-        v1 += S->h_rnv[r];
+      if (v1 == v0) {  // a wide ring: tiles of 64 consecutive vertices, the last one shorter
+        const int end = v0 + S->h_rnv[r];
+        for (int a = v0; a < end; a += cap) {
+          S->h_tile_v0.push_back(a);
+          S->h_tile_v1.push_back(std::min(a + cap, end));
+          S->h_tile_sys.push_back((int)s);
+        }
         r++;
+        continue;
       }
       S->h_tile_v0.push_back(v0);
       S->h_tile_v1.push_back(v1);
@@ -962,6 +1086,8 @@ int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPer
   A(S->d_err, 1); A(S->d_maxcnt, 1); A(S->d_ctr, 4); A(S->d_notDone, 1);
   A(S->d_fire[0], numSystems); A(S->d_fire[1], numSystems);
   A(S->d_part, (long)V.ntiles * 4);
+  S->nwide = (long)S->h_wide.size();
+  if (S->nwide > 0) A(S->d_wide, S->nwide);
   A(S->d_red, numSystems * 8); A(S->d_scale, numSystems);
 #undef A
   set_pos_view(S);
@@ -977,7 +1103,8 @@ int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPer
             up(V.rsys, S->h_rsys, st) == hipSuccess && up(V.svoff, S->h_svoff, st) == hipSuccess &&
             up(V.sroff, S->h_sroff, st) == hipSuccess && up(V.tile_v0, S->h_tile_v0, st) == hipSuccess &&
             up(V.tile_v1, S->h_tile_v1, st) == hipSuccess && up(V.tile_sys, S->h_tile_sys, st) == hipSuccess &&
-            up(V.stile, S->h_stile, st) == hipSuccess;
+            up(V.stile, S->h_stile, st) == hipSuccess &&
+            (S->nwide == 0 || up(S->d_wide, S->h_wide, st) == hipSuccess);
   std::vector<double> ones(numSystems, 1.0);
   ok = ok && up(V.bx, ones, st) == hipSuccess && up(V.by, ones, st) == hipSuccess &&
        up(V.ibx, ones, st) == hipSuccess && up(V.iby, ones, st) == hipSuccess &&
@@ -1040,7 +1167,8 @@ int dpm_destroy(dpm_set_t h) {
                   S->d_contacts, S->d_ccount, S->d_x0, S->d_obs, S->mdCount[0], S->mdCount[1], S->mdCount[2],
                   S->mdSlots[0], S->mdSlots[1], S->mdSlots[2], S->mdFlag[0], S->mdFlag[1], S->mdFlag[2],
                   S->mdFlag[3], S->R.pvx, S->R.pvy, S->R.pw, S->R.pfx, S->R.pfy, S->R.ptq, S->R.moi,
-                  S->R.prad, S->R.plx, S->R.ply, S->R.pla, S->R.vtq, S->d_rfire[0], S->d_rfire[1], S->d_rg};
+                  S->R.prad, S->R.plx, S->R.ply, S->R.pla, S->R.vtq, S->d_rfire[0], S->d_rfire[1], S->d_rg,
+                  S->d_snap, S->d_ftape, S->d_ftapeState, S->d_fadj, S->d_wide};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (S->ownStream) hipStreamDestroy(S->ownStream);
@@ -1154,6 +1282,8 @@ int dpm_set_types(dpm_set_t h, int potential, int interaction, int monomer, int 
       (interaction != INT_VV && interaction != INT_SMOOTH && interaction != INT_ALL) || monomer < 0 ||
       monomer > 1 || particle < 0 || particle > 2)
     return fail(S, DPM_ERR_ARG, "bad type");
+  if (S->nwide > 0 && particle != PART_PASSIVE)
+    return fail(S, DPM_ERR_UNSUPPORTED, "active rings: rings of at most 64 vertices only");
   S->P.potential = potential;
   S->P.interaction = interaction;
   S->P.monomer = monomer;
@@ -1303,6 +1433,14 @@ int dpm_set_displacement_cutoff(dpm_set_t h, double cutoff, double size, double*
   if (cutDistance) *cutDistance = cd;
   bump(S);
   return configure_cells(S);
+}
+
+int dpm_set_cell_scale(dpm_set_t h, double scale) {
+  CHECK_HANDLE(h);
+  if (!(scale >= 1.0) || scale > 64.0) return fail(S, DPM_ERR_ARG, "dpm_set_cell_scale: scale in [1, 64]");
+  if (scale == S->cellScale) return DPM_OK;
+  S->cellScale = scale;
+  return S->cutSet ? configure_cells(S) : DPM_OK;
 }
 
 int dpm_calc_neighbor_list(dpm_set_t h, double cutDistance) {
@@ -1495,6 +1633,44 @@ int dpm_init_langevin(dpm_set_t h, double T, double gamma, int readState, int ze
   return DPM_OK;
 }
 
+int dpm_inject_kinetic_energy(dpm_set_t h, double T) {
+  CHECK_HANDLE(h);
+  HIPCHK(launch_inject(S, sqrt(T), S->injectCalls++));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
+int dpm_update_position(dpm_set_t h, double timeStep) {
+  CHECK_HANDLE(h);
+  HIPCHK(launch_update_pos(S, timeStep));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
+int dpm_update_velocity(dpm_set_t h, double timeStep) {
+  CHECK_HANDLE(h);
+  HIPCHK(launch_update_vel(S, timeStep));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
+int dpm_update_thermal_vel(dpm_set_t h) {
+  CHECK_HANDLE(h);
+  if (S->simKind != 1 || S->noiseType != NOISE_LANGEVIN)
+    return fail(S, DPM_ERR_STATE, "dpm_update_thermal_vel: dpm_init_langevin (Langevin noise) not called");
+  HIPCHK(launch_thermal_force(S));
+  HIPCHK(launch_ctr_add(S, 1, 0));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
+int dpm_conserve_momentum(dpm_set_t h) {
+  CHECK_HANDLE(h);
+  HIPCHK(launch_sub_mean(S));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
 int dpm_init_nve(dpm_set_t h, double T, int readState, int zeroMomentum) {
   CHECK_HANDLE(h);
   S->simKind = 2;
@@ -1513,17 +1689,13 @@ int dpm_init_nve(dpm_set_t h, double T, int readState, int zeroMomentum) {
 int dpm_langevin_loop(dpm_set_t h, long steps) {
   CHECK_HANDLE(h);
   if (S->simKind != 1) return fail(S, DPM_ERR_STATE, "dpm_init_langevin not called");
-  int rc = run_verlet(S, steps, true);
-  if (rc) return rc;
-  return check_err_after_loop(S);
+  return run_verlet(S, steps, true);
 }
 
 int dpm_nve_loop(dpm_set_t h, long steps) {
   CHECK_HANDLE(h);
   if (S->simKind != 2 && S->simKind != 3) return fail(S, DPM_ERR_STATE, "dpm_init_nve not called");
-  int rc = run_verlet(S, steps, false);
-  if (rc) return rc;
-  return check_err_after_loop(S);
+  return run_verlet(S, steps, false);
 }
 
 int dpm_init_nve_rescale(dpm_set_t h, double T) {
@@ -1546,7 +1718,7 @@ int dpm_nve_rescale_loop(dpm_set_t h, long steps) {
   for (long i = 0; i < steps; i++) {
     rc = temperature_scale(S, S->Tinject);  // NVERescale::injectKineticEnergy, Simulator.cu:222-238
     if (rc) return rc;
-    rc = verlet_sequence(S, 1, false);
+    rc = with_replay(S, 0, [&]() { return verlet_sequence(S, 1, false); });
     if (rc) return rc;
   }
   return check_err_after_loop(S);
@@ -1607,10 +1779,7 @@ int dpm_fire_loop(dpm_set_t h, long iterations) {
     S->fp.tol = -1;
     bump(S);
   }
-  HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
-  rc = run_fire(S, iterations);
-  if (rc) return rc;
-  return check_err_after_loop(S);
+  return with_replay(S, SNAP_FIRE, [&]() { return run_fire(S, iterations); });
 }
 
 // The compressFIRE.cpp:121-131 loop for every system at once.  A system whose max |F_c| has
@@ -1628,8 +1797,9 @@ int dpm_fire_minimize(dpm_set_t h, double tol, long maxIter, long* iterations) {
     bump(S);
   }
   HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
+  HIPCHK(launch_fire_resume(S));
   long done = 0;
-  if (S->fireKernel == 1 && S->P.particle == PART_PASSIVE && S->P.geometry == GEOM_NORMAL) {
+  if (S->fireKernel == 1 && S->P.particle == PART_PASSIVE && S->P.geometry == GEOM_NORMAL && S->nwide == 0) {
     // one workgroup per system, at most `pchunk` iterations per launch
     rc = clean_protocol(S);
     if (rc) return rc;
@@ -1641,7 +1811,11 @@ int dpm_fire_minimize(dpm_set_t h, double tol, long maxIter, long* iterations) {
     }
     const long pchunk = 2048;
     while (done <= maxIter) {
-      HIPCHK(launch_fire_sys(S, (int)pchunk, waves));
+      rc = with_replay(S, SNAP_FIRE, [&]() -> int {
+        HIPCHK(launch_fire_sys(S, (int)pchunk, waves));
+        return DPM_OK;
+      });
+      if (rc) return rc;
       done += pchunk;
       int notDone = 0;
       HIPCHK(launch_fill_int(S->d_notDone, 0, 1, S->stream));
@@ -1654,7 +1828,7 @@ int dpm_fire_minimize(dpm_set_t h, double tol, long maxIter, long* iterations) {
   }
   const long chunk = 256;
   while (done <= maxIter) {
-    rc = run_fire(S, chunk);
+    rc = with_replay(S, SNAP_FIRE, [&]() { return run_fire(S, chunk); });
     if (rc) return rc;
     done += chunk;
     int notDone = 0;
@@ -1678,6 +1852,8 @@ int dpm_set_fire_kernel(dpm_set_t h, int persistent, int wavesPerSystem) {
   CHECK_HANDLE(h);
   if (persistent < 0 || persistent > 1 || wavesPerSystem < 0 || wavesPerSystem > 8)
     return fail(S, DPM_ERR_ARG, "dpm_set_fire_kernel: persistent in {0, 1}, waves in 0..8");
+  if (persistent && S->nwide > 0)
+    return fail(S, DPM_ERR_UNSUPPORTED, "one-workgroup-per-system FIRE: rings of at most 64 vertices only");
   S->fireKernel = persistent;
   S->fireWaves = wavesPerSystem;
   return DPM_OK;
@@ -2158,6 +2334,131 @@ int dpm_nve_forward(dpm_set_t h, long steps, long every) {
   return DPM_OK;
 }
 
+int dpm_fire_forward(dpm_set_t h, long iterations) {
+  CHECK_HANDLE(h);
+  if (iterations < 1) return fail(S, DPM_ERR_ARG, "dpm_fire_forward: iterations must be >= 1");
+  if (!S->fireInit) return fail(S, DPM_ERR_STATE, "dpm_init_fire not called");
+  int rc = adjoint_supported(S);
+  if (rc) return rc;
+  const long nv = S->nv, ns = S->nsys;
+  if (iterations > S->fireTapeCap || !S->d_fadj) {
+    HIPCHK(hipStreamSynchronize(S->stream));
+    hipFree(S->d_ftape);
+    hipFree(S->d_ftapeState);
+    hipFree(S->d_fadj);
+    S->d_ftape = nullptr;
+    S->d_ftapeState = nullptr;
+    S->d_fadj = nullptr;
+    HIPCHK(dalloc(&S->d_ftape, (iterations + 1) * 6 * nv));
+    HIPCHK(dalloc(&S->d_ftapeState, iterations * ns));
+    HIPCHK(dalloc(&S->d_fadj, 10 * nv));
+    S->fireTapeCap = iterations;
+  }
+  if (S->fp.tol >= 0) {  // a fixed number of iterations: no system stops
+    S->fp.tol = -1;
+    bump(S);
+  }
+  // every iteration one at a time, its starting state recorded before it and its FIRE scalars
+  // after its parameter update; replayed on a wider list as the loops are
+  rc = with_replay(S, SNAP_FIRE, [&]() -> int {
+    for (long k = 0; k < iterations; k++) {
+      int r = save_state(S, S->d_ftape + k * 6 * nv);
+      if (r) return r;
+      r = fire_sequence(S, 1);
+      if (r) return r;
+      HIPCHK(hipMemcpyAsync(S->d_ftapeState + k * ns, S->d_fire[S->fireCur], sizeof(FireState) * ns,
+                            hipMemcpyDeviceToDevice, S->stream));
+    }
+    return save_state(S, S->d_ftape + iterations * 6 * nv);
+  });
+  if (rc) return rc;
+  S->fireTapeN = iterations;
+  S->fireTapeP = S->P;
+  S->fireTapeSerial = S->paramSerial;
+  return DPM_OK;
+}
+
+int dpm_fire_backward(dpm_set_t h, const double* ax, const double* av, double* gx0, double* gv0, double* gring,
+                      double* gl0) {
+  CHECK_HANDLE(h);
+  if (!ax || !av) return fail(S, DPM_ERR_ARG, "dpm_fire_backward: null adjoint");
+  if (S->fireTapeN < 1) return fail(S, DPM_ERR_STATE, "dpm_fire_backward: no FIRE trajectory recorded");
+  if (memcmp(&S->P, &S->fireTapeP, sizeof(Params)) != 0 || S->paramSerial != S->fireTapeSerial)
+    return fail(S, DPM_ERR_STATE, "dpm_fire_backward: parameters changed since dpm_fire_forward");
+  int rc = adjoint_supported(S);
+  if (rc) return rc;
+  rc = adj_alloc(S);
+  if (rc) return rc;
+  const long nv = S->nv, ns = S->nsys, N = S->fireTapeN;
+  double *xa = S->d_fadj, *bv = xa + 2 * nv, *pend = xa + 4 * nv, *fc = xa + 6 * nv, *hw = xa + 8 * nv;
+  HIPCHK(launch_copy_d(xa, ax, 2 * nv, S->stream));
+  HIPCHK(launch_copy_d(bv, av, 2 * nv, S->stream));
+  HIPCHK(launch_fill_d(pend, 0.0, 2 * nv, S->stream));
+  HIPCHK(launch_fill_d(adj_g(S), 0.0, 4 * nv, S->stream));
+  HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
+  rc = clean_protocol(S);
+  if (rc) return rc;
+  // positions x_N, x_{N-1}, ..., x_0, each with the step loop's row protocol: the displacement
+  // test of the next position is made by the kernel of this one
+  auto slot = [&](long k) { return S->d_ftape + k * 6 * nv; };
+  auto hvp = [&](long n, int p, const double* w) -> int {
+    const View Vn = view_at(S, slot(n), slot(n) + nv);
+    HIPCHK(launch_bin_view(S, Vn, p));
+    AdjArgs A{};
+    A.wx = w;
+    A.wy = w + nv;
+    A.hx = hw;
+    A.hy = hw + nv;
+    A.g = adj_g(S);
+    A.cd = 1.0;
+    A.mode = 2;
+    A.flag = S->d_flag[p];
+    A.count = S->G.count[p];
+    A.cellOther = S->G.count[1 - p];
+    A.ncells = S->G.ncells;
+    if (n > 0) {
+      A.px = slot(n - 1);
+      A.py = slot(n - 1) + nv;
+      A.flagOut = S->d_flag[1 - p];
+      A.anyOut = S->d_any + 1 - p;
+    }
+    HIPCHK(launch_adj_hvp(S, Vn, A));
+    return DPM_OK;
+  };
+  HIPCHK(launch_adj_flag(S, view_at(S, slot(N), slot(N) + nv), 0));
+  int p = 0;
+  for (long k = N - 1; k >= 0; k--) {
+    const FireState* st = S->d_ftapeState + k * ns;
+    HIPCHK(launch_fadj_prep(S, st, bv, pend, fc));
+    rc = hvp(k + 1, p, fc);
+    if (rc) return rc;
+    p ^= 1;
+    HIPCHK(launch_fadj_partials(S, st, slot(k), xa, bv, hw));
+    HIPCHK(launch_fadj_update(S, st, slot(k), xa, bv, pend, hw));
+  }
+  rc = hvp(0, p, pend);  // F(x_0) enters the first iteration only: x bar -= H(x_0) pend
+  if (rc) return rc;
+  HIPCHK(launch_axpy(xa, hw, -1.0, 2 * nv, S->stream));
+  {
+    int err = 0;
+    HIPCHK(hipMemcpyAsync(&err, S->d_err, sizeof(int), hipMemcpyDeviceToHost, S->stream));
+    HIPCHK(hipStreamSynchronize(S->stream));
+    if (err) return fail(S, DPM_ERR_OVERFLOW, "dpm_fire_backward: neighbour list overflow");
+  }
+  if (gx0) HIPCHK(launch_copy_d(gx0, xa, 2 * nv, S->stream));
+  if (gv0) HIPCHK(launch_copy_d(gv0, bv, 2 * nv, S->stream));
+  if (gring) HIPCHK(launch_ring_sum3(S, adj_g(S), gring));
+  if (gl0) HIPCHK(launch_copy_d(gl0, adj_g(S) + 3 * nv, nv, S->stream));
+  // the rows again for the set's own positions (the reverse pass left them at x_0)
+  rc = clean_protocol(S);
+  if (rc) return rc;
+  std::vector<int> all(S->nsys, 1);
+  rc = rebuild(S, all, 0);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(S->stream));
+  return DPM_OK;
+}
+
 int dpm_nve_backward(dpm_set_t h, const double* ax, const double* av, double* gx0, double* gv0, double* gring,
                      double* gl0) {
   CHECK_HANDLE(h);
@@ -2227,6 +2528,7 @@ int dpm_calc_rigid_force_energy(dpm_set_t h) {
 
 // initRigidLangevin / initRigidNVE, DPM2D.cu:3823-3868
 static int rigid_init(dpm_set_s* S, double T, double gamma, bool langevin, int readState) {
+  if (S->nwide > 0) return fail(S, DPM_ERR_UNSUPPORTED, "rigid rings: rings of at most 64 vertices only");
   S->rigidKind = langevin ? 1 : 2;
   S->Tinject = T;
   if (langevin) S->P.gamma = gamma;  // the rigid kick adds unit draws: no noise amplitude
@@ -2267,6 +2569,7 @@ int dpm_rigid_nve_loop(dpm_set_t h, long steps) {
 // initRigidFIRE, DPM2D.cu:3700-3706 (FIRE parameters as dpm_init_fire's)
 int dpm_init_rigid_fire(dpm_set_t h, const double* p, long minStep, long numStep) {
   CHECK_HANDLE(h);
+  if (S->nwide > 0) return fail(S, DPM_ERR_UNSUPPORTED, "rigid rings: rings of at most 64 vertices only");
   if (!p) return fail(S, DPM_ERR_ARG, "null FIRE params");
   S->fp.a_start = p[0]; S->fp.f_dec = p[1]; S->fp.f_inc = p[2]; S->fp.f_a = p[3];
   S->fp.dt_max = p[5];
@@ -2295,20 +2598,17 @@ int dpm_rigid_fire_loop(dpm_set_t h, long iterations) {
   int rc = rigid_supported(S);
   if (rc) return rc;
   if (iterations <= 0) return DPM_OK;
-  HIPCHK(launch_fill_int(S->d_err, 0, 1, S->stream));
-  if (iterations < 4 || !graphs_enabled()) {
-    rc = rigid_fire_sequence(S, iterations);
-  } else {
+  return with_replay(S, SNAP_RIGID_FIRE, [&]() -> int {
+    if (iterations < 4 || !graphs_enabled()) return rigid_fire_sequence(S, iterations);
     // the sequence flips the position and FIRE-state buffers once per iteration
-    rc = run_graph(S, std::make_tuple(22 + S->rfireCur, iterations, S->cur),
-                   [&]() { return rigid_fire_sequence(S, iterations); });
-    if (!rc && (iterations % 2) == 1) {
+    int r = run_graph(S, std::make_tuple(22 + S->rfireCur, iterations, S->cur),
+                      [&]() { return rigid_fire_sequence(S, iterations); });
+    if (!r && (iterations % 2) == 1) {
       swap_pos(S);
       S->rfireCur = 1 - S->rfireCur;
     }
-  }
-  if (rc) return rc;
-  return check_err_after_loop(S);
+    return r;
+  });
 }
 
 // FIRE scalars of the rigid minimiser per system: dt, a, numStep, last power, iterations

@@ -85,6 +85,16 @@ struct StepArgs {
   int manage;                     // inside loops: rebuild flagged rows, clear cells, count updates
   int rigidLangevin;              // MODE_RIGID: the RigidLangevin kick (else RigidNVE's)
   int xcd;                        // XCD-aware workgroup -> tile mapping
+  int rowsBuilt;                  // inside loops: flagged rows already rebuilt (k_rows_loop)
+};
+
+// Up to kMaxCopySegs device-to-device copies in one launch (k_copy_segs): 4-byte words.
+constexpr int kMaxCopySegs = 32;
+struct CopySegs {
+  const unsigned* src[kMaxCopySegs];
+  unsigned* dst[kMaxCopySegs];
+  long words[kMaxCopySegs];
+  int n;
 };
 
 // Arguments of the H w kernel (dpm_adjoint.hip).
@@ -125,10 +135,15 @@ struct dpm_set_s {
   std::string err;
   int split = 4;  // wavefronts per tile in the step kernel (1..8)
   int xcdMap = 1; // XCD-aware workgroup -> tile mapping in the step kernel
+  double cellScale = 1.0;  // cell side / (list radius + cutoff / 3), >= 1 (dpm_set_cell_scale)
 
   // host copies of topology
   std::vector<int> h_rnv, h_rfirst, h_rsys, h_vring, h_vsys, h_svoff, h_sroff;
   std::vector<int> h_tile_v0, h_tile_v1, h_tile_sys, h_stile;  // h_stile: per-system tile offsets
+  // rings of more than 64 vertices, each cut into tiles of 64-vertex slices
+  std::vector<int> h_wide;
+  long nwide = 0;
+  int* d_wide = nullptr;
   std::vector<double> h_box;    // 2 per system
   std::vector<double> h_maxrad; // per system
 
@@ -200,6 +215,19 @@ struct dpm_set_s {
   long adjNv = 0;
   int adjCur = 0, wCur = 0;
 
+  // FIRE tape (dpm_fire_forward / dpm_fire_backward): slots 0..fireTapeN of x, y, vx, vy, fx, fy
+  // (6 nv doubles each) and the per-system FIRE state of every iteration
+  double* d_ftape = nullptr;
+  dpm::FireState* d_ftapeState = nullptr;
+  double* d_fadj = nullptr;   // work: ax, bv, pend, Fc, H Fc (2 nv each)
+  long fireTapeN = 0, fireTapeCap = 0, fireTapeSerial = -1;
+  dpm::Params fireTapeP{};
+
+  // copy of what a loop changes, restored to replay the loop after a list overflow
+  unsigned* d_snap = nullptr;
+  long snapWords = 0;
+  int snapCur = 0, snapFireCur = 0, snapRfireCur = 0;
+
   // graph cache: key = (kind, steps, cur)
   std::map<std::tuple<int, long, int>, hipGraphExec_t> graphs;
   long version = 0;  // bumped when launch arguments change
@@ -221,6 +249,9 @@ hipError_t launch_fill_int(int* p, int value, long n, hipStream_t st);
 hipError_t launch_reduce(dpm_set_s* S);
 hipError_t launch_inject(dpm_set_s* S, double amplitude, unsigned long long call);
 hipError_t launch_sub_mean(dpm_set_s* S);
+hipError_t launch_update_pos(dpm_set_s* S, double dt);
+hipError_t launch_update_vel(dpm_set_s* S, double dt);
+hipError_t launch_thermal_force(dpm_set_s* S);
 hipError_t launch_scale_vel(dpm_set_s* S);
 hipError_t launch_scale_vertices(dpm_set_s* S);
 hipError_t launch_scale_box(dpm_set_s* S);
@@ -230,6 +261,7 @@ hipError_t launch_fire_mid(dpm_set_s* S);
 hipError_t launch_fire_partials(dpm_set_s* S);
 hipError_t launch_fire_step(dpm_set_s* S);
 hipError_t launch_fire_done(dpm_set_s* S, int* notDone);
+hipError_t launch_fire_resume(dpm_set_s* S);
 StepArgs loop_args(dpm_set_s* S);
 hipError_t launch_bin_view(dpm_set_s* S, const View& Vn, int parity);
 hipError_t launch_fire_sys(dpm_set_s* S, int chunk, int waves);
@@ -237,6 +269,11 @@ hipError_t launch_adj_flag(dpm_set_s* S, const View& Vn, int parity);
 hipError_t launch_adj_seg_check(dpm_set_s* S, const double* seg, int slots, int* bad);
 hipError_t launch_adj_hvp(dpm_set_s* S, const View& Vn, AdjArgs A);
 hipError_t launch_energy_param_partials(dpm_set_s* S, double* out);
+hipError_t launch_fadj_prep(dpm_set_s* S, const FireState* st, const double* bv, const double* pend, double* fc);
+hipError_t launch_fadj_partials(dpm_set_s* S, const FireState* st, const double* slot, const double* ax,
+                                const double* bv, const double* hw);
+hipError_t launch_fadj_update(dpm_set_s* S, const FireState* st, const double* slot, double* ax, double* bv,
+                              double* pend, const double* hw);
 hipError_t launch_ring_sum3(dpm_set_s* S, const double* in, double* out);
 hipError_t launch_fill_d(double* p, double value, long n, hipStream_t st);
 hipError_t launch_contacts(dpm_set_s* S, int* list, int* count, int width, int* overflow, double gap = -1.0);
@@ -245,6 +282,9 @@ hipError_t launch_displacement(dpm_set_s* S, const double* x, const double* y, c
                                long n, int rings, double waveNum, double* d2, double* sf);
 hipError_t launch_particle_stress(dpm_set_s* S, double* out);
 hipError_t launch_copy_d(double* dst, const double* src, long n, hipStream_t st);
+hipError_t launch_copy_segs(const CopySegs& segs, hipStream_t st);
+hipError_t launch_axpy(double* y, const double* x, double alpha, long n, hipStream_t st);
+hipError_t launch_build_rows_view(dpm_set_s* S, const View& Vn, int parity, int countUpdates);
 hipError_t launch_rigid_start(dpm_set_s* S, bool langevin, const StepArgs& A);
 hipError_t launch_rigid_inject(dpm_set_s* S, double amplitude, unsigned long long call);
 hipError_t launch_rigid_fire_partials(dpm_set_s* S);

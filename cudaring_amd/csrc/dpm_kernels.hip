@@ -42,44 +42,72 @@ struct ShapeOut {
   double cx, cy;  // ring centre (calcParticlePos, cudaKernel.cuh:365-388)
 };
 
+// turning angle at a vertex from its own, next and previous positions (calcAngle 423-427)
+__device__ __forceinline__ double vertex_theta(const Box& B, double x, double y, double xn, double yn, double xp,
+                                               double yp) {
+  const double dnx = pbx(xn - x, B), dny = pby(yn - y, B);
+  const double psx = -pbx(xp - x, B), psy = -pby(yp - y, B);
+  return atan2(dnx * psy - dny * psx, dnx * psx + dny * psy);
+}
+
 __device__ __forceinline__ ShapeOut ring_shape(const View& V, const Params& P, const Lane& L, const Box& B,
                                                double x, double y, bool write) {
   ShapeOut o;
-  const double xn = shfl_d(x, L.nl), yn = shfl_d(y, L.nl);
-  const double xp = shfl_d(x, L.pl), yp = shfl_d(y, L.pl);
-  const double xf = shfl_d(x, L.segS), yf = shfl_d(y, L.segS);
-  // segments of the reference's unwrapped walk
-  const double dnx = pbx(xn - x, B), dny = pby(yn - y, B);
-  const double psx = -pbx(xp - x, B), psy = -pby(yp - y, B);
-  const double theta = atan2(dnx * psy - dny * psx, dnx * psx + dny * psy);  // calcAngle 423-427
-  const double seglen = sqrt(dnx * dnx + dny * dny);
-  // Unwrapped position relative to the first vertex: the reference walks the ring one
-  // minimum-image segment at a time; here that walk is an exclusive segmented scan of the
-  // segment vectors, valid for rings wider than half the box.
-  const double Ix = seg_scan(L.act ? dnx : 0.0, L.lane, L.segS);
-  const double Iy = seg_scan(L.act ? dny : 0.0, L.lane, L.segS);
-  const double ux = L.act ? Ix - dnx : 0.0;
-  const double uy = L.act ? Iy - dny : 0.0;
-  const double shoe = L.act ? (ux * dny - dnx * uy) : 0.0;  // shoelace: cur x next = cur x delta
-  const double perim = seg_sum(L.act ? seglen : 0.0, L.lane, L.segS, L.segE);
-  const double shoeSum = seg_sum(shoe, L.lane, L.segS, L.segE);
-  const double sux = seg_sum(ux, L.lane, L.segS, L.segE);
-  const double suy = seg_sum(uy, L.lane, L.segS, L.segE);
-  const double nd = (double)L.n, ind = 1.0 / nd;
-  const double area = fabs(shoeSum) * 0.5;
-  const double comx = xf + sux * ind, comy = yf + suy * ind;
+  double xn, yn, xp, yp, theta, seglen, area, perim, comx, comy;
+  int vn = 0, vp = 0;
+  if (!L.wide) {
+    xn = shfl_d(x, L.nl); yn = shfl_d(y, L.nl);
+    xp = shfl_d(x, L.pl); yp = shfl_d(y, L.pl);
+    const double xf = shfl_d(x, L.segS), yf = shfl_d(y, L.segS);
+    // segments of the reference's unwrapped walk
+    const double dnx = pbx(xn - x, B), dny = pby(yn - y, B);
+    const double psx = -pbx(xp - x, B), psy = -pby(yp - y, B);
+    theta = atan2(dnx * psy - dny * psx, dnx * psx + dny * psy);  // calcAngle 423-427
+    seglen = sqrt(dnx * dnx + dny * dny);
+    // Unwrapped position relative to the first vertex: the reference walks the ring one
+    // minimum-image segment at a time; here that walk is an exclusive segmented scan of the
+    // segment vectors, valid for rings wider than half the box.
+    const double Ix = seg_scan(L.act ? dnx : 0.0, L.lane, L.segS);
+    const double Iy = seg_scan(L.act ? dny : 0.0, L.lane, L.segS);
+    const double ux = L.act ? Ix - dnx : 0.0;
+    const double uy = L.act ? Iy - dny : 0.0;
+    const double shoe = L.act ? (ux * dny - dnx * uy) : 0.0;  // shoelace: cur x next = cur x delta
+    perim = seg_sum(L.act ? seglen : 0.0, L.lane, L.segS, L.segE);
+    const double shoeSum = seg_sum(shoe, L.lane, L.segS, L.segE);
+    const double sux = seg_sum(ux, L.lane, L.segS, L.segE);
+    const double suy = seg_sum(uy, L.lane, L.segS, L.segE);
+    const double ind = 1.0 / (double)L.n;
+    area = fabs(shoeSum) * 0.5;
+    comx = xf + sux * ind;
+    comy = yf + suy * ind;
+  } else {
+    // a slice of a wide ring: ring neighbours from memory (this launch reads the current positions
+    // only), the ring's area, perimeter and centre from k_wide_rings, launched just before
+    vn = ring_next(L.v, L.f, L.n);
+    vp = ring_prev(L.v, L.f, L.n);
+    xn = V.x[vn]; yn = V.y[vn];
+    xp = V.x[vp]; yp = V.y[vp];
+    theta = vertex_theta(B, x, y, xn, yn, xp, yp);
+    const double dnx = pbx(xn - x, B), dny = pby(yn - y, B);
+    seglen = sqrt(dnx * dnx + dny * dny);
+    area = V.area[L.r];
+    perim = V.perim[L.r];
+    comx = V.cx[L.r];
+    comy = V.cy[L.r];
+  }
   o.cx = comx;
   o.cy = comy;
   if (write && L.act) {
     V.th[L.v] = theta;
     V.len[L.v] = seglen;
-    if (L.lane == L.segS) {
+    if (L.lane == L.segS && !L.wide) {
       V.area[L.r] = area;
       V.perim[L.r] = perim;
       V.cx[L.r] = comx;
       V.cy[L.r] = comy;
     }
   }
+  const double ind = 1.0 / (double)L.n;
   // positions relative to the ring centre (getRelativeVertexPos 273-279)
   const double rvx = pbx(x - comx, B), rvy = pby(y - comy, B);
   const double rnx = pbx(xn - comx, B), rny = pby(yn - comy, B);
@@ -101,7 +129,8 @@ __device__ __forceinline__ ShapeOut ring_shape(const View& V, const Params& P, c
   const double pLen = sqrt(qsx * qsx + qsy * qsy);
   const double nLen = sqrt(nsx * nsx + nsy * nsy);
   const double l0v = V.l0[L.v];
-  const double l0p = shfl_d(l0v, L.pl);
+  const double l0s = shfl_d(l0v, L.pl);
+  const double l0p = L.wide ? V.l0[vp] : l0s;
   if (P.monomer == MONO_HARMONIC) {
     const double pDelta = (pLen / l0p) - 1.;
     const double nDelta = (nLen / l0v) - 1.;
@@ -119,7 +148,12 @@ __device__ __forceinline__ ShapeOut ring_shape(const View& V, const Params& P, c
   }
   // bending term (calcBendingForceEnergy 457-471)
   const double D = theta - V.th0[L.v];
-  const double Dp = shfl_d(D, L.pl), Dn = shfl_d(D, L.nl);
+  double Dp = shfl_d(D, L.pl), Dn = shfl_d(D, L.nl);
+  if (L.wide) {  // the neighbours' angles from their own neighbours, as their lanes form them
+    const int vpp = ring_prev(vp, L.f, L.n), vnn = ring_next(vn, L.f, L.n);
+    Dp = vertex_theta(B, xp, yp, x, y, V.x[vpp], V.y[vpp]) - V.th0[vp];
+    Dn = vertex_theta(B, xn, yn, V.x[vnn], V.y[vnn], x, y) - V.th0[vn];
+  }
   const double preVar = (D - Dp) / (qsx * qsx + qsy * qsy);
   const double nextVar = (D - Dn) / (nsx * nsx + nsy * nsy);
   fx += eb * (preVar * qsy + nextVar * nsy);
@@ -194,10 +228,13 @@ __global__ void k_build_rows(View V, Params P, CellGrid G, const int* count, con
 // Interparticle gather of lane L's vertex over entries k0, k0 + kstep, ... of its row:
 // accumulates the force on v, the vertex-vertex energy e, the smooth energy es and the share
 // (qx, qy) that v's `other` evaluation owes prev(v) (handed to prev(v)'s lane by the caller).
+// On the last lane of a wide ring's slice, whose next vertex lies in another tile, (qnx, qny)
+// accumulates the share that next(v)'s evaluations owe v, from next(v)'s row (complete before
+// the launch: wide sets rebuild rows in their own launch).
 template <int INTER, int POT>
 __device__ __forceinline__ void gather_pairs(const View& V, const Params& P, const Lane& L, const Box& B, double x,
                                              double y, int k0, int kstep, double& fx, double& fy, double& e,
-                                             double& es, double& qx, double& qy) {
+                                             double& es, double& qx, double& qy, double& qnx, double& qny) {
   const double radv = V.rad[L.v];
   // Row entries are fetched one iteration ahead, the first one before the row length is known:
   // rows are W entries wide (unused entries hold -1), so k < W keeps every fetch in bounds, and
@@ -217,8 +254,12 @@ __device__ __forceinline__ void gather_pairs(const View& V, const Params& P, con
     // (v, segment of u) gives v's `this` share, and (u, segment (prev(v), v)) gives v's
     // `other` share and prev(v)'s `previous` share (handed to prev(v)'s lane by the caller).
     // Ring neighbours of v; inactive lanes past the tile end use v itself (in bounds).
-    const int vp = L.act ? L.v0 + L.pl : L.v;
-    const int vpp = L.act ? L.v0 + ((L.pl > L.segS) ? L.pl - 1 : L.segE) : L.v;
+    int vp = L.act ? L.v0 + L.pl : L.v;
+    int vpp = L.act ? L.v0 + ((L.pl > L.segS) ? L.pl - 1 : L.segE) : L.v;
+    if (L.wide && L.act) {
+      vp = ring_prev(L.v, L.f, L.n);
+      vpp = ring_prev(vp, L.f, L.n);
+    }
     const double xp = V.x[vp], yp = V.y[vp];
     const double xpp = V.x[vpp], ypp = V.y[vpp];
     int u = 0;
@@ -243,6 +284,18 @@ __device__ __forceinline__ void gather_pairs(const View& V, const Params& P, con
       es += smooth_pair_op<POT>(P, B, xu, yu, x, y, xp, yp, xpp, ypp, radSum, fx, fy, qx, qy);
       u = un;
       pp = ppn;
+    }
+    if (L.wide && L.act && L.lane == L.segE) {
+      // (u, segment (v, next(v))) for u in list(next(v)): next(v)'s `other` share is its own
+      // lane's; the `previous` share is v's (the energy is counted by next(v)'s lane)
+      const int j = ring_next(L.v, L.f, L.n);
+      const double xj = V.x[j], yj = V.y[j], radj = V.rad[j];
+      const int cj = V.ncnt[j];
+      for (int k = k0; k < cj; k += kstep) {
+        const int w = V.nbr[(long)k * V.nv + j];
+        double ox = 0, oy = 0;
+        (void)smooth_pair_op<POT>(P, B, V.x[w], V.y[w], xj, yj, x, y, xp, yp, radj + V.rad[w], ox, oy, qnx, qny);
+      }
     }
   } else {
     // kernelCalcAllToAllVertexInteraction, cudaKernel.cuh:725-757 (test mode)
@@ -424,7 +477,7 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
     // vertices instead (the reference's all-pairs search)
     const bool allPairs = A.staleCheck && (flag & FLAG_FAR) != 0;
     if (rebuild && L.tile == V.stile[L.s] && threadIdx.x == 0) A.updateCount[L.s] += 1;
-    if (rebuild && w == 0 && L.act) {
+    if (rebuild && w == 0 && L.act && !A.rowsBuilt) {
       if (A.staleCheck && !allPairs) {  // the previous position is this lane's own, in the
                                         // buffer the drift below overwrites
         const double ddx = pbx(x - A.xout[L.v], B), ddy = pby(y - A.yout[L.v], B);
@@ -448,18 +501,24 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
   }
 
   // gather partials: force on v, vertex-vertex energy, smooth energy, share for prev(v)
-  double fx = 0, fy = 0, e = 0, es = 0, qx = 0, qy = 0;
+  double fx = 0, fy = 0, e = 0, es = 0, qx = 0, qy = 0, qnx = 0, qny = 0;
   const int g0 = nw > 1 ? 1 : 0;
   const int ng = nw - g0;
-  if (w >= g0) gather_pairs<INTER, POT>(V, P, L, B, x, y, w - g0, ng, fx, fy, e, es, qx, qy);
+  // partials per gather wave: six, eight when the tile is a slice of a wide ring
+  const int nq = (INTER == INT_SMOOTH && L.wide) ? 8 : 6;
+  if (w >= g0) gather_pairs<INTER, POT>(V, P, L, B, x, y, w - g0, ng, fx, fy, e, es, qx, qy, qnx, qny);
   if (nw > 1 && w > 0) {
-    double* r = red + (w - 1) * 6 * WAVE + L.lane;
+    double* r = red + (w - 1) * nq * WAVE + L.lane;
     r[0 * WAVE] = fx;
     r[1 * WAVE] = fy;
     r[2 * WAVE] = e;
     r[3 * WAVE] = es;
     r[4 * WAVE] = qx;
     r[5 * WAVE] = qy;
+    if (nq == 8) {
+      r[6 * WAVE] = qnx;
+      r[7 * WAVE] = qny;
+    }
   }
 
   // ---- wave 0: work that does not need the gather, overlapped with it
@@ -510,24 +569,30 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
     __syncthreads();
     if (w > 0) return;
     for (int q = 1; q < nw; q++) {
-      const double* r = red + (q - 1) * 6 * WAVE + L.lane;
+      const double* r = red + (q - 1) * nq * WAVE + L.lane;
       fx += r[0 * WAVE];
       fy += r[1 * WAVE];
       e += r[2 * WAVE];
       es += r[3 * WAVE];
       qx += r[4 * WAVE];
       qy += r[5 * WAVE];
+      if (nq == 8) {
+        qnx += r[6 * WAVE];
+        qny += r[7 * WAVE];
+      }
     }
   }
 
   // ---- wave 0: assemble F = shape + interparticle (+ shares from the next lane)
   if constexpr (INTER == INT_SMOOTH) {
-    fx += shfl_d(qx, L.nl);
-    fy += shfl_d(qy, L.nl);
+    const double sx = shfl_d(qx, L.nl), sy = shfl_d(qy, L.nl);
+    const bool fromMemory = L.wide && L.lane == L.segE;  // next(v) in another tile
+    fx += fromMemory ? qnx : sx;
+    fy += fromMemory ? qny : sy;
     const double pe = seg_sum(L.act ? es : 0.0, L.lane, L.segS, L.segE);
     if (L.act) {
       V.es[L.v] = es;
-      if (L.lane == L.segS) V.pe[L.r] = pe;
+      if (L.lane == L.segS && !L.wide) V.pe[L.r] = pe;  // wide rings: k_wide_pe sums es
     }
   }
   if constexpr ((MODE & MODE_RIGID) != 0) {
@@ -620,6 +685,118 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
   }
   V.vx[L.v] = vx;
   V.vy[L.v] = vy;
+}
+
+// ------------------------------------------------------------ wide rings ---
+// Ring sums of the rings of more than 64 vertices, whose slices k_step and k_shape read: area,
+// perimeter and centre by ring_shape's unwrapped walk (exclusive prefix sums of the minimum-image
+// segments from the ring's first vertex), here a block-wide scan over chunks of 256 vertices.
+// One 256-thread block per wide ring.
+__global__ __launch_bounds__(256) void k_wide_rings(View V, const int* wide) {
+  __shared__ double wsum[2][4];
+  __shared__ double red[4][4];
+  const int r = wide[blockIdx.x];
+  const int f = V.rfirst[r], n = V.rnv[r];
+  const Box B = box_of(V, V.rsys[r]);
+  const int t = threadIdx.x, lane = t & (WAVE - 1), wv = t / WAVE;
+  double carryX = 0, carryY = 0;  // unwrapped offset of the chunk's first vertex
+  double aP = 0, aS = 0, aX = 0, aY = 0;
+  for (int b = 0; b < n; b += 256) {
+    const int i = b + t;
+    const bool act = i < n;
+    double dnx = 0, dny = 0;
+    if (act) {
+      const int v = f + i, vn = (i + 1 == n) ? f : v + 1;
+      dnx = pbx(V.x[vn] - V.x[v], B);
+      dny = pby(V.y[vn] - V.y[v], B);
+    }
+    double ix = dnx, iy = dny;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+      const double tx = __shfl_up(ix, d, WAVE), ty = __shfl_up(iy, d, WAVE);
+      if (lane >= d) {
+        ix += tx;
+        iy += ty;
+      }
+    }
+    if (lane == WAVE - 1) {
+      wsum[0][wv] = ix;
+      wsum[1][wv] = iy;
+    }
+    __syncthreads();
+    double ox = carryX, oy = carryY;
+    for (int q = 0; q < wv; q++) {
+      ox += wsum[0][q];
+      oy += wsum[1][q];
+    }
+    const double ux = ox + (ix - dnx), uy = oy + (iy - dny);
+    for (int q = 0; q < 4; q++) {
+      carryX += wsum[0][q];
+      carryY += wsum[1][q];
+    }
+    __syncthreads();
+    if (act) {
+      aP += sqrt(dnx * dnx + dny * dny);
+      aS += ux * dny - dnx * uy;
+      aX += ux;
+      aY += uy;
+    }
+  }
+  aP = wave_sum(aP);
+  aS = wave_sum(aS);
+  aX = wave_sum(aX);
+  aY = wave_sum(aY);
+  if (lane == 0) {
+    red[0][wv] = aP;
+    red[1][wv] = aS;
+    red[2][wv] = aX;
+    red[3][wv] = aY;
+  }
+  __syncthreads();
+  if (t == 0) {
+    const double perim = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+    const double shoe = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    const double sux = red[2][0] + red[2][1] + red[2][2] + red[2][3];
+    const double suy = red[3][0] + red[3][1] + red[3][2] + red[3][3];
+    const double ind = 1.0 / (double)n;
+    V.area[r] = fabs(shoe) * 0.5;
+    V.perim[r] = perim;
+    V.cx[r] = V.x[f] + sux * ind;
+    V.cy[r] = V.y[f] + suy * ind;
+  }
+}
+
+// smooth interaction energy of each wide ring: the sum of its vertices' shares, which the
+// slices of k_step store per vertex
+__global__ __launch_bounds__(256) void k_wide_pe(View V, const int* wide) {
+  __shared__ double red[4];
+  const int r = wide[blockIdx.x];
+  const int f = V.rfirst[r], n = V.rnv[r];
+  double acc = 0;
+  for (int i = threadIdx.x; i < n; i += 256) acc += V.es[f + i];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & (WAVE - 1)) == 0) red[threadIdx.x / WAVE] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) V.pe[r] = red[0] + red[1] + red[2] + red[3];
+}
+
+// The step loop's row rebuild of k_step as a launch of its own, for sets with wide rings under
+// the vertex-segment interaction: the last lane of a slice reads the row of a vertex in another
+// tile, so every flagged row is complete before the step kernel starts (which then skips its
+// rebuild, A.rowsBuilt).  One thread per vertex.
+__global__ void k_rows_loop(View V, Params P, StepArgs A, CellGrid G) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V.nv) return;
+  const int s = V.vsys[v];
+  const int flag = A.flagCur[s];
+  if ((flag & 1) == 0) return;
+  const Box B = box_of(V, s);
+  const bool allPairs = A.staleCheck && (flag & FLAG_FAR) != 0;
+  if (A.staleCheck && !allPairs) {
+    const double ddx = pbx(V.x[v] - A.xout[v], B), ddy = pby(V.y[v] - A.yout[v], B);
+    if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) atomicOr(A.err, ERR_STALE_BINS);
+  }
+  build_row(V, P, G, A.cellCur, A.slotsCur, v, s, B, A.err, A.maxcnt, allPairs);
 }
 
 // geometry only (calcParticleShape + calcParticlePositions)
@@ -843,6 +1020,33 @@ __global__ void k_inject(View V, Params P, double amplitude, unsigned long long 
   gauss_pair(P.seed, (uint32_t)s, TAG_INJECT, call, (uint32_t)(r - V.sroff[s]), z0, z1);
   V.vx[v] = amplitude * z0;
   V.vy[v] = amplitude * z1;
+}
+
+// Stages of the reference's Langevin integrator for callers that drive them one by one
+// (Simulator.cu:85-95 kernelUpdateVertexPos / kernelUpdateVertexVel, 54-77 updateThermalVel):
+// x += dt v; v += dt F; F += noise xi_ring - gamma v with the ring's draw of call `call`
+__global__ void k_update_pos(View V, double dt) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V.nv) return;
+  V.x[v] += dt * V.vx[v];
+  V.y[v] += dt * V.vy[v];
+}
+
+__global__ void k_update_vel(View V, double dt) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V.nv) return;
+  V.vx[v] += dt * V.fx[v];
+  V.vy[v] += dt * V.fy[v];
+}
+
+__global__ void k_thermal_force(View V, Params P, const unsigned long long* ctr) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V.nv) return;
+  const int r = V.vring[v], s = V.vsys[v];
+  double z0, z1;
+  gauss_pair(P.seed, (uint32_t)s, TAG_THERMAL, ctr[0], (uint32_t)(r - V.sroff[s]), z0, z1);
+  V.fx[v] += P.noise * z0 - P.gamma * V.vx[v];
+  V.fy[v] += P.noise * z1 - P.gamma * V.vy[v];
 }
 
 // subtract the per-system mean velocity
@@ -1145,6 +1349,13 @@ __global__ void k_fire_done(const FireState* fs, int nsys, int* notDone) {
   if (s < nsys && !fs[s].converged) atomicOr(notDone, 1);
 }
 
+// a new minimisation call tests every system's stopping rule again (the driver's loop condition
+// is evaluated per call, compressFIRE.cpp:121-131)
+__global__ void k_fire_resume(FireState* fs, int nsys) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nsys) fs[s].converged = 0;
+}
+
 // ------------------------------------------------------- persistent FIRE ---
 // One workgroup per system runs whole FIRE iterations (FIRE::minimizerVertexLoop,
 // FIRE.cu:397-411) back to back, separated by workgroup barriers only: systems share nothing,
@@ -1315,7 +1526,8 @@ __global__ __launch_bounds__(WAVE* FS_WAVES) void k_fire_sys(View V, Params P, F
       lane_ctx(V, t, lane, L);
       const double x = V.x[L.v], y = V.y[L.v];
       double fx = 0, fy = 0, e = 0, es = 0, qx = 0, qy = 0;
-      gather_pairs<INTER, POT>(V, P, L, B, x, y, 0, 1, fx, fy, e, es, qx, qy);
+      double qnx = 0, qny = 0;  // sets with wide rings do not run this kernel
+      gather_pairs<INTER, POT>(V, P, L, B, x, y, 0, 1, fx, fy, e, es, qx, qy, qnx, qny);
       const ShapeOut so = ring_shape(V, P, L, B, x, y, true);
       if constexpr (INTER == INT_SMOOTH) {
         fx += shfl_d(qx, L.nl);
@@ -1348,14 +1560,16 @@ __global__ __launch_bounds__(WAVE* FS_WAVES) void k_fire_sys(View V, Params P, F
 static inline int grid_v(long n) { return (int)((n + 255) / 256); }
 static inline int grid_w(int ntiles) { return (ntiles + 3) / 4; }
 
-// LDS of the step kernel: six partials per lane of each gather wave
-static inline size_t step_lds(int split) { return (size_t)(split - 1) * 6 * WAVE * sizeof(double); }
+// LDS of the step kernel: six partials per lane of each gather wave (eight with wide rings)
+static inline size_t step_lds(const dpm_set_s* S) {
+  return (size_t)(S->split - 1) * (S->nwide > 0 ? 8 : 6) * WAVE * sizeof(double);
+}
 
 template <int INTER, int POT>
 static hipError_t step_mode(dpm_set_s* S, int mode, const StepArgs& A) {
   const int g = S->V.ntiles;
   const int b = WAVE * S->split;
-  const size_t sh = step_lds(S->split);
+  const size_t sh = step_lds(S);
   hipStream_t st = S->stream;
   switch (mode) {
     case MODE_FORCE: k_step<INTER, POT, MODE_FORCE><<<g, b, sh, st>>>(S->V, S->P, A, S->G); break;
@@ -1408,7 +1622,31 @@ static hipError_t step_pot(dpm_set_s* S, int mode, const StepArgs& A) {
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_step(dpm_set_s* S, int mode, const StepArgs& A) {
+static hipError_t launch_step_kernel(dpm_set_s* S, int mode, const StepArgs& A);
+
+// Sets with wide rings (more than 64 vertices) run, around each step kernel, the ring sums
+// of their slices before it (and, under the vertex-segment interaction inside loops, the row
+// rebuild), and the wide rings' interaction energies after it.
+hipError_t launch_step(dpm_set_s* S, int mode, const StepArgs& A0) {
+  if (S->nwide == 0) return launch_step_kernel(S, mode, A0);
+  if ((mode & MODE_RIGID) || S->P.particle == PART_ACTIVE || S->P.particle == PART_ACTIVE_SHAPE)
+    return hipErrorInvalidValue;  // refused by the entry points before this point
+  StepArgs A = A0;
+  const bool smooth = S->P.interaction == INT_SMOOTH;
+  if (smooth && A.manage) {
+    if (!A.flagCur || !A.cellCur || !A.err || !A.maxcnt || (A.staleCheck && (!A.xout || !A.yout)))
+      return hipErrorInvalidValue;
+    k_rows_loop<<<grid_v(S->nv), 256, 0, S->stream>>>(S->V, S->P, A, S->G);
+    A.rowsBuilt = 1;
+  }
+  k_wide_rings<<<(int)S->nwide, 256, 0, S->stream>>>(S->V, S->d_wide);
+  hipError_t e = launch_step_kernel(S, mode, A);
+  if (e != hipSuccess) return e;
+  if (smooth) k_wide_pe<<<(int)S->nwide, 256, 0, S->stream>>>(S->V, S->d_wide);
+  return hipGetLastError();
+}
+
+static hipError_t launch_step_kernel(dpm_set_s* S, int mode, const StepArgs& A) {
   // refuse argument sets the chosen epilogue would dereference as null
   if (!A.ctr) return hipErrorInvalidValue;
   if (S->P.particle == PART_ACTIVE_SHAPE && !S->V.l0r) return hipErrorInvalidValue;
@@ -1423,7 +1661,7 @@ hipError_t launch_step(dpm_set_s* S, int mode, const StepArgs& A) {
     case INT_ALL: {
       if (mode != MODE_FORCE) return hipErrorInvalidValue;
       const int g = S->V.ntiles, b = WAVE * S->split;
-      const size_t sh = step_lds(S->split);
+      const size_t sh = step_lds(S);
       hipStream_t st = S->stream;
       switch (S->P.potential) {
         case POT_HARMONIC: k_step<INT_ALL, POT_HARMONIC, MODE_FORCE><<<g, b, sh, st>>>(S->V, S->P, A, S->G); break;
@@ -1438,6 +1676,7 @@ hipError_t launch_step(dpm_set_s* S, int mode, const StepArgs& A) {
 }
 
 hipError_t launch_shape_only(dpm_set_s* S) {
+  if (S->nwide > 0) k_wide_rings<<<(int)S->nwide, 256, 0, S->stream>>>(S->V, S->d_wide);
   k_shape<<<grid_w(S->V.ntiles), 256, 0, S->stream>>>(S->V, S->P);
   return hipGetLastError();
 }
@@ -1490,6 +1729,15 @@ hipError_t launch_build_rows(dpm_set_s* S, int countUpdates) {
   return hipGetLastError();
 }
 
+// k_build_rows on positions other than the set's current buffer (the list positions), with the
+// flags and cells of `parity`
+hipError_t launch_build_rows_view(dpm_set_s* S, const View& Vn, int parity, int countUpdates) {
+  k_build_rows<<<grid_w(S->V.ntiles), 256, 0, S->stream>>>(Vn, S->P, S->G, S->G.count[parity], S->d_flag[parity],
+                                                             S->d_updateCount, countUpdates, S->d_err,
+                                                             S->d_maxcnt);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill_int(int* p, int value, long n, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   k_fill_int<<<grid_v(n), 256, 0, st>>>(p, value, n);
@@ -1503,6 +1751,21 @@ hipError_t launch_reduce(dpm_set_s* S) {
 
 hipError_t launch_inject(dpm_set_s* S, double amplitude, unsigned long long call) {
   k_inject<<<grid_v(S->nv), 256, 0, S->stream>>>(S->V, S->P, amplitude, call);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_pos(dpm_set_s* S, double dt) {
+  k_update_pos<<<grid_v(S->nv), 256, 0, S->stream>>>(S->V, dt);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_vel(dpm_set_s* S, double dt) {
+  k_update_vel<<<grid_v(S->nv), 256, 0, S->stream>>>(S->V, dt);
+  return hipGetLastError();
+}
+
+hipError_t launch_thermal_force(dpm_set_s* S) {
+  k_thermal_force<<<grid_v(S->nv), 256, 0, S->stream>>>(S->V, S->P, S->d_ctr);
   return hipGetLastError();
 }
 
@@ -1605,6 +1868,11 @@ hipError_t launch_fire_sys(dpm_set_s* S, int chunk, int waves) {
     case INT_SMOOTH: return fire_sys_pot<INT_SMOOTH>(S, chunk, threads);
     default: return fire_sys_pot<INT_ALL>(S, chunk, threads);
   }
+}
+
+hipError_t launch_fire_resume(dpm_set_s* S) {
+  k_fire_resume<<<grid_v(S->nsys), 256, 0, S->stream>>>(S->d_fire[S->fireCur], (int)S->nsys);
+  return hipGetLastError();
 }
 
 hipError_t launch_fire_done(dpm_set_s* S, int* notDone) {

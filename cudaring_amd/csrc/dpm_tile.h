@@ -42,9 +42,12 @@ __device__ __forceinline__ double wave_max(double v) {
 // Lane context of a ring-aligned tile: the lane's vertex v (the tile's first vertex on lanes
 // past the tile end, act = false), its ring r and system s, the lanes [segS, segE] holding the
 // ring, and the lanes of the next and previous vertex of the ring.
+// A ring of more than 64 vertices (wide) is cut into tiles of 64 consecutive vertices, one
+// slice each: then [segS, segE] is the slice, the slice's first lane has pl = itself and its
+// last lane nl = itself (their ring neighbours lie in other tiles and are read from memory).
 struct Lane {
-  int lane, tile, v0, v, r, s, segS, segE, nl, pl, n;
-  bool act;
+  int lane, tile, v0, v, r, s, segS, segE, nl, pl, n, f;  // f: the ring's first vertex
+  bool act, wide;
 };
 
 __device__ __forceinline__ void lane_ctx(const View& V, int tile, int lane, Lane& L) {
@@ -58,13 +61,26 @@ __device__ __forceinline__ void lane_ctx(const View& V, int tile, int lane, Lane
   const int4 vi = V.vinfo[L.v];
   L.r = vi.x;
   const int first = vi.y;
+  L.f = first;
   L.n = vi.z;
+  L.wide = L.n > WAVE;
+  if (L.wide) {
+    L.segS = L.act ? 0 : lane;
+    L.segE = L.act ? v1 - L.v0 - 1 : lane;
+    L.nl = (lane < L.segE) ? lane + 1 : lane;
+    L.pl = (lane > L.segS) ? lane - 1 : lane;
+    return;
+  }
   L.segS = L.act ? first - L.v0 : lane;
   const int segN = L.act ? L.n : 1;
   L.segE = L.segS + segN - 1;
   L.nl = (lane < L.segE) ? lane + 1 : L.segS;
   L.pl = (lane > L.segS) ? lane - 1 : L.segE;
 }
+
+// next and previous vertex of v in its ring (first vertex f, n vertices)
+__device__ __forceinline__ int ring_next(int v, int f, int n) { return (v + 1 == f + n) ? f : v + 1; }
+__device__ __forceinline__ int ring_prev(int v, int f, int n) { return (v == f) ? f + n - 1 : v - 1; }
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b mod 8), each
 // with its own L2.  This maps the workgroups of XCD k onto the k-th contiguous eighth of the tiles,

@@ -67,6 +67,7 @@ SIGNATURES = {
     "dpm_set_seed": (_i, [_vp, C.c_uint64]),
     "dpm_set_displacement_cutoff": (_i, [_vp, _d, _d, C.POINTER(_d)]),
     "dpm_calc_neighbor_list": (_i, [_vp, _d]),
+    "dpm_set_cell_scale": (_i, [_vp, _d]),
     "dpm_check_neighbors": (_i, [_vp]),
     "dpm_get_neighbor_list": (_i, [_vp, _vp, _vp, _l, C.POINTER(_l)]),
     "dpm_reset_last_positions": (_i, [_vp]),
@@ -76,6 +77,11 @@ SIGNATURES = {
     "dpm_calc_force_energy": (_i, [_vp]),
     "dpm_get_interaction_forces": (_i, [_vp, _vp, _l]),
     "dpm_init_langevin": (_i, [_vp, _d, _d, _i, _i]),
+    "dpm_inject_kinetic_energy": (_i, [_vp, _d]),
+    "dpm_update_position": (_i, [_vp, _d]),
+    "dpm_update_velocity": (_i, [_vp, _d]),
+    "dpm_update_thermal_vel": (_i, [_vp]),
+    "dpm_conserve_momentum": (_i, [_vp]),
     "dpm_langevin_loop": (_i, [_vp, _l]),
     "dpm_init_nve": (_i, [_vp, _d, _i, _i]),
     "dpm_nve_loop": (_i, [_vp, _l]),
@@ -113,6 +119,8 @@ SIGNATURES = {
     "dpm_energy_param_grad_noalloc": (_i, [_vp, _vp, _vp]),
     "dpm_nve_forward": (_i, [_vp, _l, _l]),
     "dpm_nve_backward": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "dpm_fire_forward": (_i, [_vp, _l]),
+    "dpm_fire_backward": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "dpm_time_step_kernel": (_i, [_vp, _l, C.POINTER(_d), C.POINTER(_d)]),
     "dpm_set_launch_config": (_i, [_vp, _i]),
 }
@@ -276,6 +284,10 @@ class DPMSet:
         self._chk(self.L.dpm_set_displacement_cutoff(self.h, cutoff, size, C.byref(cd)), "set_cutoff")
         return cd.value
 
+    def set_cell_scale(self, scale):
+        """Cell side as a multiple (>= 1) of the smallest one the neighbour search allows."""
+        self._chk(self.L.dpm_set_cell_scale(self.h, float(scale)), "set_cell_scale")
+
     def calc_neighbor_list(self, cut):
         self._chk(self.L.dpm_calc_neighbor_list(self.h, cut), "calc_neighbor_list")
 
@@ -317,6 +329,22 @@ class DPMSet:
 
     def langevin(self, n):
         self._chk(self.L.dpm_langevin_loop(self.h, n), "langevin_loop")
+
+    # the stages of Langevin::integrate, one call each (Simulator.cu:23-95)
+    def inject_kinetic_energy(self, T):
+        self._chk(self.L.dpm_inject_kinetic_energy(self.h, float(T)), "inject_kinetic_energy")
+
+    def update_position(self, dt):
+        self._chk(self.L.dpm_update_position(self.h, float(dt)), "update_position")
+
+    def update_velocity(self, dt):
+        self._chk(self.L.dpm_update_velocity(self.h, float(dt)), "update_velocity")
+
+    def update_thermal_vel(self):
+        self._chk(self.L.dpm_update_thermal_vel(self.h), "update_thermal_vel")
+
+    def conserve_momentum(self):
+        self._chk(self.L.dpm_conserve_momentum(self.h), "conserve_momentum")
 
     def init_nve(self, T, read_state=False, zero_momentum=False):
         self._chk(self.L.dpm_init_nve(self.h, T, int(read_state), int(zero_momentum)), "init_nve")
@@ -534,6 +562,15 @@ class DPMSet:
         """NVE steps with checkpoints every `every` steps for nve_backward."""
         self._chk(self.L.dpm_nve_forward(self.h, steps, every), "nve_forward")
         self.tape_serial += 1
+
+    def fire_forward(self, iterations):
+        """FIRE iterations (no stopping rule) recorded for fire_backward."""
+        self._chk(self.L.dpm_fire_forward(self.h, int(iterations)), "fire_forward")
+
+    def fire_backward(self, ax_ptr, av_ptr, gx_ptr, gv_ptr, gring_ptr, gl0_ptr):
+        """Reverse pass of the last fire_forward with its branches held fixed (2V SoA)."""
+        p = [C.c_void_p(x) if x else None for x in (ax_ptr, av_ptr, gx_ptr, gv_ptr, gring_ptr, gl0_ptr)]
+        self._chk(self.L.dpm_fire_backward(self.h, *p), "fire_backward")
 
     def nve_backward(self, ax_ptr, av_ptr, gx_ptr, gv_ptr, gring_ptr, gl0_ptr):
         """Reverse pass of the last nve_forward (2V SoA adjoints in, gradients out)."""

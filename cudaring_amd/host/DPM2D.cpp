@@ -30,8 +30,7 @@ inline double pbc(double d, double L) { return d - L * std::round(d / L); }
 DPM2D::DPM2D(long nParticles, long dim, long nVertexPerParticle) {
   if (dim != 2) throw std::invalid_argument("DPM2D: only nDim = 2 is supported");
   if (nParticles < 1) throw std::invalid_argument("DPM2D: nParticles must be >= 1");
-  if (nVertexPerParticle < 3 || nVertexPerParticle > 64)
-    throw std::invalid_argument("DPM2D: vertices per particle must be in [3, 64]");
+  if (nVertexPerParticle < 3) throw std::invalid_argument("DPM2D: vertices per particle must be >= 3");
   srand48(time(0));  // as DPM2D.cu:43; setSeed() makes a run reproducible
   seed = (uint64_t)time(0);
   nDim = dim;
@@ -222,7 +221,7 @@ void DPM2D::setNumVertexInParticleList(std::vector<long>& list) {
   if ((long)list.size() != numParticles)
     throw std::invalid_argument("DPM2D::setNumVertexInParticleList: size must equal numParticles");
   for (long n : list)
-    if (n < 3 || n > 64) throw std::invalid_argument("DPM2D: ring sizes must be in [3, 64]");
+    if (n < 3) throw std::invalid_argument("DPM2D: ring sizes must be >= 3");
   long nv = std::accumulate(list.begin(), list.end(), 0L);
   bool same = (list == numVertexInParticleList);
   if (!same) release();
@@ -339,6 +338,18 @@ void DPM2D::setFENEconstants(double stiff_, double ext_) {
   extSq = ext_ * ext_;
   if (h_) check(dpm_set_fene_constants(h_, stiff, ext_), "setFENEconstants");
 }
+double DPM2D::setTimeScale(double dt_) {
+  const double l00 = getRestLengths()[0];
+  const double ta = rho0 / std::sqrt(ea);
+  const double tl = (rho0 * l00) / std::sqrt(ea * el);
+  const double tb = (rho0 * l00) / std::sqrt(ea * eb);
+  double tmin = 1e08;
+  if (ta < tmin) tmin = ta;
+  if (tl < tmin) tmin = tl;
+  if (tb < tmin) tmin = tb;
+  return setTimeStep(tmin * dt_);
+}
+
 double DPM2D::setTimeStep(double dt_) {
   dt = dt_;
   if (h_) check(dpm_set_time_step(h_, dt), "setTimeStep");
@@ -541,6 +552,17 @@ void DPM2D::sizeRings(std::vector<long>& n, double minN) {
   setVertexRadii(rad);
 }
 
+// DPM2D.cu:1590-1598
+void DPM2D::setSinusoidalRestAngles(double thetaA, double thetaK) {
+  std::vector<double> th(numVertices);
+  for (long p = 0; p < numParticles; p++) {
+    const double thetaR = 2. * PI / (double)numVertexInParticleList[p];
+    for (long i = 0; i < numVertexInParticleList[p]; i++)
+      th[firstVertexInParticleId[p] + i] = thetaA * thetaR * std::cos(thetaR * thetaK * (double)i);
+  }
+  setRestAngles(th);
+}
+
 void DPM2D::setPolySizeDistribution(double calA0_, double polyDispersity) {
   // DPM2D.cu:1545-1588: one Box-Muller draw per ring from drand48, floor, clamp below
   calA0 = calA0_;
@@ -552,7 +574,6 @@ void DPM2D::setPolySizeDistribution(double calA0_, double polyDispersity) {
     double g = std::sqrt(-2. * std::log(r1)) * std::cos(2. * PI * r2);
     double k = std::floor(polyDispersity * numVertexPerParticle * g + numVertexPerParticle);
     if (k < minN) k = minN;
-    if (k > 64) throw std::invalid_argument("DPM2D::setPolySizeDistribution: a ring drew more than 64 vertices");
     n[p] = (long)k;
   }
   sizeRings(n, minN);

@@ -119,6 +119,9 @@ class DPM2D {
   void setLJcutoff(double LJcutoff_);
   void setFENEconstants(double stiff_, double ext_);
   double setTimeStep(double dt_);
+  // setTimeScale, DPM2D.cu:1941-1954: dt_ times the smallest of the area, segment and bending
+  // time scales (rho0 / sqrt(ea), rho0 l0 / sqrt(ea el), rho0 l0 / sqrt(ea eb), l0 of vertex 0)
+  double setTimeScale(double dt_);
   void setSeed(uint64_t seed_);
 
   // ---- observables (DPM2D.h:408-440, 296-300)
@@ -174,6 +177,8 @@ class DPM2D {
   // ---- initialisation and compression (DPM2D.h:482-495)
   void setMonoSizeDistribution();
   void setPolySizeDistribution(double calA0_, double polyDispersity);
+  // rest angle of vertex i of an n-vertex ring: thetaA (2 pi / n) cos((2 pi / n) thetaK i)
+  void setSinusoidalRestAngles(double thetaA, double thetaK);
   // 50/50 (or `fracLarge`) mixture of nSmall- and nLarge-vertex rings; the reference leaves
   // setBiSizeDistribution as a stub (DPM2D.h:484).  Sizing as setPolySizeDistribution.
   void setBiSizeDistribution(double calA0_, long nSmall, long nLarge, double fracLarge);

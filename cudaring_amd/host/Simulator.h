@@ -3,8 +3,10 @@
 //
 // One integrate() is one fused device step (kick, drift, neighbour check, forces, thermostat,
 // kick) of libdpmring.so.  The reference's separate stages (updatePosition, updateVelocity,
-// updateThermalVel, conserveMomentum) are fused into that step and throw std::logic_error if
-// called alone; injectKineticEnergy is the velocity draw of the matching init call.
+// updateThermalVel, conserveMomentum, injectKineticEnergy) are also callable one by one, each a
+// device call of its own: in Langevin::integrate's order (with DPM2D::checkNeighbors and
+// calcForceEnergy between) they reproduce the fused step.  integrate() itself does not call the
+// virtual stages, so a subclass that overrides one changes only stage-wise use.
 #ifndef DPM_HOST_SIMULATOR_H
 #define DPM_HOST_SIMULATOR_H
 
@@ -55,7 +57,6 @@ class NVE : public Langevin {
  public:
   NVE() = default;
   NVE(DPM2D* dpmPtr, SimConfig config) : Langevin(dpmPtr, config) {}
-  void injectKineticEnergy() override;
   void integrate() override;
 };
 

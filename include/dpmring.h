@@ -79,7 +79,11 @@ enum { DPM_GEOM_NORMAL = 0, DPM_GEOM_FIXED_BOX = 1, DPM_GEOM_FIXED_SIDES = 2, DP
 /* ---- lifecycle ------------------------------------------------------------ */
 /* Replaces DPM2D(long nParticles, long dim, long nVertexPerParticle) (DPM2D.h:47) followed by
  * setNumVertexInParticleList (DPM2D.h:267) and initParticleIdList (DPM2D.h:219), for B systems:
- * ringsPerSystem[B], numVertexInParticleList[sum(ringsPerSystem)] (3 <= n <= 64). */
+ * ringsPerSystem[B], numVertexInParticleList[sum(ringsPerSystem)] (n >= 3).  A ring of more than
+ * 64 vertices is cut into slices of 64 (one tile each) with its ring sums taken by an extra launch
+ * per step; sets holding such rings support passive particles only, and no rigid-ring, adjoint
+ * or one-workgroup-per-system FIRE path (those calls return DPM_ERR_UNSUPPORTED; FIRE runs the
+ * multi-launch kernels). */
 int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPerSystem,
                const long* numVertexInParticleList);
 int dpm_destroy(dpm_set_t h);
@@ -140,6 +144,11 @@ int dpm_set_displacement_cutoff(dpm_set_t h, double cutoff, double size, double*
 /* calcNeighborList(cutDistance), DPM2D.h:640 / DPM2D.cu:3479-3507 (built with a cell list;
  * the list holds the same sets as the reference's all-pairs build) */
 int dpm_calc_neighbor_list(dpm_set_t h, double cutDistance);
+/* Cell side of the neighbour search as a multiple (>= 1) of its smallest value, the list radius
+ * plus the cutoff / 3 a vertex may move between binning and rebuild; a tuning knob (the
+ * reference builds its lists without cells, DPM2D.cu:3479-3507, so no counterpart).  The rows
+ * are the same sets for every scale. */
+int dpm_set_cell_scale(dpm_set_t h, double scale);
 /* checkNeighbors(), DPM2D.h:459 / DPM2D.cu:1408-1423: device-side displacement test and rebuild */
 int dpm_check_neighbors(dpm_set_t h);
 /* getNeighbors + getMaxNeighborList (DPM2D.h:411-412): rows of `cap` ascending global ids,
@@ -196,6 +205,18 @@ int dpm_set_noise_type(dpm_set_t h, int noiseType);
  * zeroMomentum removes the mean velocity after injection (an addition; the reference does not). */
 int dpm_init_langevin(dpm_set_t h, double T, double gamma, int readState, int zeroMomentum);
 int dpm_langevin_loop(dpm_set_t h, long steps);
+/* The stages of Langevin::integrate (Simulator.cu:23-95) for callers that drive them one by
+ * one (SimInterface, Simulator.h:47-52): injectKineticEnergy (per-ring Gaussian velocities of
+ * amplitude sqrt(T), nothing else), updatePosition (x += timeStep v), updateVelocity
+ * (v += timeStep F), updateThermalVel (F += noise xi - gamma v with one draw per ring, the draw the
+ * fused step of the same call index uses; needs dpm_init_langevin with Langevin noise) and
+ * conserveMomentum (the mean vertex velocity of each system removed).  In that order with
+ * checkNeighbors and calcForceEnergy, they reproduce one dpm_langevin_loop step. */
+int dpm_inject_kinetic_energy(dpm_set_t h, double T);
+int dpm_update_position(dpm_set_t h, double timeStep);
+int dpm_update_velocity(dpm_set_t h, double timeStep);
+int dpm_update_thermal_vel(dpm_set_t h);
+int dpm_conserve_momentum(dpm_set_t h);
 /* initNVE(T, readState) / NVELoop(), DPM2D.h:684-686 */
 int dpm_init_nve(dpm_set_t h, double T, int readState, int zeroMomentum);
 int dpm_nve_loop(dpm_set_t h, long steps);
@@ -224,8 +245,10 @@ int dpm_set_fire_kernel(dpm_set_t h, int persistent, int wavesPerSystem);
 /* FIRE state per system: out[5*s + {0..4}] = fire_dt, a, numStep, last P = v.F, iterations */
 int dpm_get_fire_state(dpm_set_t h, double* out);
 /* The compressFIRE.cpp:121-131 loop for every system at once: iterate each system until
- * getMaxUnbalancedForce() <= tol or maxIter; converged systems stop moving.  Runs without
- * host synchronisation; iterations[s] receives the per-system count. */
+ * getMaxUnbalancedForce() <= tol or maxIter; converged systems stop moving.  Each call tests the
+ * stopping rule of every system afresh and continues the FIRE state (dt, alpha, N+) where the
+ * last call left it; maxIter bounds the iterations since dpm_init_fire, whose per-system count
+ * iterations[s] receives.  Runs without host synchronisation between iterations. */
 int dpm_fire_minimize(dpm_set_t h, double tol, long maxIter, long* iterations);
 
 /* ---- observables (per system) --------------------------------------------- */
@@ -287,9 +310,11 @@ int dpm_set_launch_config(dpm_set_t h, int wavesPerTile);
 int dpm_time_step_kernel(dpm_set_t h, long reps, double* avgMicroseconds, double* meanNeighbors);
 
 /* ---- adjoint (no reference counterpart: differentiation is an addition) ---- */
-/* Supported for the vertexVertex interaction with harmonic monomers and passive particles, all
- * four potentials.  Vectors are device arrays owned by the caller and used on the handle's
- * stream; a "2V SoA" vector holds the V x components followed by the V y components. */
+/* Supported for the vertexVertex and vertexSmooth interactions with harmonic monomers and
+ * passive particles in a periodic box, all four potentials (vertexSmooth: exact on the branch
+ * the positions select, the interaction being piecewise smooth).  Vectors are device arrays owned
+ * by the caller and used on the handle's stream; a "2V SoA" vector holds the V x components
+ * followed by the V y components. */
 /* Hessian-vector product of the potential energy at the current positions: out = H u
  * (ux, uy, outx, outy: device arrays of V). */
 int dpm_force_hvp(dpm_set_t h, const double* ux, const double* uy, double* outx, double* outy);
@@ -315,6 +340,19 @@ int dpm_nve_forward(dpm_set_t h, long steps, long every);
  * recomputed; the set is left in the state the forward pass ended in. */
 int dpm_nve_backward(dpm_set_t h, const double* ax, const double* av, double* gx0, double* gv0, double* gring,
                      double* gl0);
+
+/* vertexFIRELoop() x iterations (DPM2D.h:669, FIRE.cu:338-411) from the current state after
+ * dpm_init_fire, every system advancing (no stopping rule), recording for dpm_fire_backward the
+ * state before every iteration (positions, velocities, forces) and each system's FIRE scalars
+ * after its parameter update.  The tape holds (iterations + 1) x 6 V doubles. */
+int dpm_fire_forward(dpm_set_t h, long iterations);
+/* Reverse pass of the last dpm_fire_forward with its branch sequence held fixed (each system's
+ * dt, alpha and restart of every iteration, as recorded): given ax = dL/dx_N and av = dL/dv_N
+ * (2V SoA), returns gx0 = dL/dx_0, gv0 = dL/dv_0 (2V SoA), gring = dL/d(ka, kl, kb) per ring (3R)
+ * and gl0 = dL/dl0 (V); outputs may be NULL.  One Hessian-vector product per recorded position;
+ * the set keeps the state the forward pass ended in. */
+int dpm_fire_backward(dpm_set_t h, const double* ax, const double* av, double* gx0, double* gv0, double* gring,
+                      double* gl0);
 
 #ifdef __cplusplus
 }

@@ -45,3 +45,20 @@ def relaxed_state(nrings, phi, seed, final=("wca", "vertexSmooth")):
                                      final=final, final_iters=1500)
     s.close()
     return st
+
+
+def thermalized_state(st, potential, interaction, T=1e-3, steps=300, seed=11):
+    """The positions of `st` after `steps` Langevin steps at temperature T (gamma 1) on the GPU.
+    A packing relaxed under a purely repulsive potential leaves many pairs parked at the cutoff,
+    where the force's derivative jumps, so finite differences there straddle a kink; thermal
+    motion spreads the pair distances.  GPU tests only."""
+    g = gpu_from_state(st, potential=potential, interaction=interaction, dt=1e-3, seed=seed)
+    cut = g.set_displacement_cutoff(0.5, 2 * float(np.mean(st["rad"])))
+    g.calc_neighbor_list(cut)
+    g.calc_force_energy()
+    g.init_langevin(T, 1.0)
+    g.langevin(steps)
+    out = dict(st)
+    out["pos"] = g.get("pos")
+    g.close()
+    return out

@@ -260,3 +260,183 @@ def test_two_trajectories_in_one_loss():
     scale = max(float(ga.abs().max()), float(gb.abs().max()))
     assert float((xa_.grad - ga).abs().max()) <= 1e-12 * scale
     assert float((xb_.grad - gb).abs().max()) <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("pot", ["harmonic", "wca", "lj", "adhesive"])
+def test_smooth_hvp_matches_oracle_fd(oracle_mod, pot):
+    """The vertex-segment interaction's Hessian-vector product (every branch: segment interiors,
+    convex corners, concave notches) against central differences of the oracle's forces, on a
+    packing relaxed with the same potential (no deep overlaps, where the stiff potentials'
+    curvature makes differences meaningless); the WCA packing is then thermalized (relaxed, it
+    parks pairs at the cutoff, where the force's derivative jumps within the difference step)."""
+    print(f"case smooth hvp {pot}", flush=True)
+    from tests.helpers import relaxed_state, thermalized_state
+    st = relaxed_state(12, 0.8, 31, final=(pot, "vertexSmooth"))
+    if pot == "wca":
+        st = thermalized_state(st, pot, "vertexSmooth")
+    ka, kl, kb = ring_costs(len(st["nvp"]), 2)
+    g = gpu_from_state(st, potential=pot, interaction="vertexSmooth", costs=COSTS)
+    g.set_ring_costs(ka, kl, kb)
+    cut = g.set_displacement_cutoff(0.5, 2 * float(np.mean(st["rad"])))
+    g.calc_neighbor_list(cut)
+    g.calc_force_energy()
+    o = oracle_from_state(oracle_mod, st, potential=pot, interaction="vertexSmooth", costs=COSTS)
+    o.set_ring_costs(ka, kl, kb)
+    o.set_displacement_cutoff(0.5, 2 * float(np.mean(st["rad"])))
+    o.calc_neighbor_list(cut)
+    rng = np.random.default_rng(17)
+    for _ in range(2):
+        u = rng.standard_normal(2 * g.nv)
+        hu = gpu_hvp(g, u)
+        fd = -(oracle_force(o, st["pos"] + H * u) - oracle_force(o, st["pos"] - H * u)) / (2 * H)
+        scale = np.abs(fd).max()
+        assert np.max(np.abs(hu - fd)) / scale < 1e-6, (pot, np.max(np.abs(hu - fd)) / scale)
+    # the interaction part alone is checked too (shape costs zero): its Hessian is symmetric
+    g.set_ring_costs(0 * ka, 0 * kl, 0 * kb)
+    w = rng.standard_normal(2 * g.nv)
+    hw, hu = gpu_hvp(g, w), gpu_hvp(g, u)
+    assert abs(w @ hu - u @ hw) <= 1e-9 * max(abs(w @ hu), 1e-6)
+
+
+def test_smooth_nve_backward_matches_oracle_fd(oracle_mod):
+    """Gradients through 60 NVE steps of WCA vertex-segment rings (checkpoints every 20) against
+    the oracle's central differences."""
+    print("case smooth nve backward", flush=True)
+    from tests.helpers import relaxed_state, thermalized_state
+    st = thermalized_state(relaxed_state(9, 0.78, 12), "wca", "vertexSmooth")
+    nr, dt, steps = len(st["nvp"]), 2e-3, 60
+    ka, kl, kb = ring_costs(nr, 6)
+    size = 2 * float(np.mean(st["rad"]))
+    g = gpu_from_state(st, potential="wca", interaction="vertexSmooth", costs=COSTS, dt=dt)
+    g.set_ring_costs(ka, kl, kb)
+    cut = g.set_displacement_cutoff(0.5, size)
+    g.calc_neighbor_list(cut)
+    g.calc_force_energy()
+    g.init_nve(1e-3, zero_momentum=True)
+    x0, v0 = g.get("pos"), g.get("vel")
+    g.nve_forward(steps, 20)
+    rng = np.random.default_rng(5)
+    nv = g.nv
+    ax, av = rng.standard_normal(2 * nv), rng.standard_normal(2 * nv)
+
+    def soa(a):
+        return dev(np.concatenate([a[0::2], a[1::2]]))
+
+    gx, gv = (torch.empty(2 * nv, dtype=torch.float64, device="cuda") for _ in range(2))
+    axt, avt = soa(ax), soa(av)
+    torch.cuda.synchronize()
+    g.nve_backward(axt.data_ptr(), avt.data_ptr(), gx.data_ptr(), gv.data_ptr(), 0, 0)
+    gxa = np.empty(2 * nv)
+    gxa[0::2], gxa[1::2] = gx.cpu().numpy()[:nv], gx.cpu().numpy()[nv:]
+    o = oracle_from_state(oracle_mod, st, potential="wca", interaction="vertexSmooth", costs=COSTS, dt=dt)
+    o.set_ring_costs(ka, kl, kb)
+    o.set_displacement_cutoff(0.5, size)
+
+    def L(x):
+        o.set("pos", x)
+        o.set("vel", v0)
+        o.init_nve(0.0, read_state=True)
+        o.calc_neighbor_list(cut)
+        o.calc_force_energy()
+        o.nve(steps)
+        return float(ax @ o.get("pos") + av @ o.get("vel"))
+
+    d = rng.standard_normal(2 * nv)
+    fd = (L(x0 + 1e-7 * d) - L(x0 - 1e-7 * d)) / 2e-7
+    assert abs(gxa @ d - fd) <= 1e-5 * abs(fd), (gxa @ d, fd)
+
+
+FIRE_P7 = (0.1, 0.5, 1.1, 0.99, 5e-3, 5e-2, 0.1)  # a_start, f_dec, f_inc, f_a, dt, dt_max, a
+
+
+@pytest.mark.parametrize("interaction", ["vertexVertex", "vertexSmooth"])
+def test_fire_backward_matches_oracle_fd(oracle_mod, interaction):
+    """Gradients through 40 FIRE iterations (vertexFIRELoop, FIRE.cu:338-411) against the oracle's
+    central differences.  The start velocities point against the forces, so the first iteration
+    restarts; the parameters grow dt and shrink alpha after 5 uphill-free steps, and later
+    restarts are counted on the oracle.  Harmonic contacts on a soft packing: their force
+    derivative jumps only by ec / radSum^2 where a contact opens."""
+    print(f"case fire backward {interaction}", flush=True)
+    pot = "harmonic"
+    st = soft_state(seed=14)
+    nr, N = len(st["nvp"]), 40
+    ka, kl, kb = ring_costs(nr, 4)
+    size = 2 * float(np.mean(st["rad"]))
+    g = gpu_from_state(st, potential=pot, interaction=interaction, costs=COSTS)
+    g.set_ring_costs(ka, kl, kb)
+    cut = g.set_displacement_cutoff(0.5, size)
+    g.calc_neighbor_list(cut)
+    g.calc_force_energy()
+    rng = np.random.default_rng(8)
+    f0 = g.get("force")
+    v0 = -0.2 * f0 + 1e-3 * rng.standard_normal(f0.size)
+    assert v0 @ f0 < 0
+    g.set("vel", v0)
+    g.init_fire(FIRE_P7, 5, 0)
+    g.fire_forward(N)
+    xg, vg = g.get("pos"), g.get("vel")
+    o = oracle_from_state(oracle_mod, st, potential=pot, interaction=interaction, costs=COSTS)
+    o.set_ring_costs(ka, kl, kb)
+    o.set_displacement_cutoff(0.5, size)
+
+    def run(x0=st["pos"], v=v0, ka_=ka, kl_=kl, kb_=kb, l0=st["l0"], count=False):
+        o.set_ring_costs(ka_, kl_, kb_)
+        o.set("l0", l0)
+        o.set("pos", x0)
+        o.set("vel", v)
+        o.calc_neighbor_list(cut)
+        o.calc_force_energy()
+        o.init_fire(FIRE_P7, 5, 0)
+        restarts = 0
+        for _ in range(N if count else 1):
+            o.fire(1 if count else N)
+            restarts += count and o.fire_state()["numStep"] == 0
+        return o.get("pos"), o.get("vel"), restarts
+
+    xo, vo, restarts = run(count=True)
+    print("restarts", restarts, flush=True)
+    assert restarts >= 2, restarts  # the first iteration and at least one more
+    np.testing.assert_allclose(xg, xo, rtol=0, atol=1e-11)
+    np.testing.assert_allclose(vg, vo, rtol=0, atol=1e-10)
+    nv = g.nv
+    cx, cv = rng.standard_normal(2 * nv), rng.standard_normal(2 * nv)
+
+    def soa(a):
+        return dev(np.concatenate([a[0::2], a[1::2]]))
+
+    ax, av = soa(cx), soa(cv)
+    gx, gv = torch.empty_like(ax), torch.empty_like(ax)
+    gring = torch.empty(3 * nr, dtype=torch.float64, device="cuda")
+    gl0 = torch.empty(nv, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    g.fire_backward(ax.data_ptr(), av.data_ptr(), gx.data_ptr(), gv.data_ptr(), gring.data_ptr(), gl0.data_ptr())
+    gx, gv = gx.cpu().numpy(), gv.cpu().numpy()
+    gx = np.stack([gx[:nv], gx[nv:]], 1).ravel()
+    gv = np.stack([gv[:nv], gv[nv:]], 1).ravel()
+    gring, gl0 = gring.cpu().numpy(), gl0.cpu().numpy()
+    np.testing.assert_array_equal(g.get("pos"), xg)  # the set stays at the end of the forward pass
+
+    def L(**kw):
+        x, v, _ = run(**kw)
+        return cx @ x + cv @ v
+
+    h = 1e-7
+    d = rng.standard_normal(2 * nv)
+    fd = (L(x0=st["pos"] + h * d) - L(x0=st["pos"] - h * d)) / (2 * h)
+    assert abs(gx @ d - fd) <= 1e-5 * abs(fd), (gx @ d, fd)
+    fd = (L(v=v0 + h * d) - L(v=v0 - h * d)) / (2 * h)
+    assert abs(gv @ d - fd) <= 1e-5 * abs(fd), (gv @ d, fd)
+    # the parameter derivatives are small against L (|L| ~ 1e2): a step of 1e-7 would leave
+    # differences at the rounding level of L, so the parameters take 1e-5
+    hp = 1e-5
+    p = 2
+    for q, base in enumerate((ka, kl, kb)):
+        up, dn = base.copy(), base.copy()
+        up[p] += hp
+        dn[p] -= hp
+        key = ("ka_", "kl_", "kb_")[q]
+        fd = (L(**{key: up}) - L(**{key: dn})) / (2 * hp)
+        assert abs(gring[q * nr + p] - fd) <= 1e-4 * max(abs(fd), 1e-3), (q, gring[q * nr + p], fd)
+    dl = rng.standard_normal(nv)
+    fd = (L(l0=st["l0"] + hp * dl) - L(l0=st["l0"] - hp * dl)) / (2 * hp)
+    assert abs(gl0 @ dl - fd) <= 1e-4 * abs(fd), (gl0 @ dl, fd)

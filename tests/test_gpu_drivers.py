@@ -143,20 +143,24 @@ def test_run_rigid_matches_oracle(packing_dir, tmp_path, oracle_mod, mode):
     o.set_rng(seed, 0)
     o.calc_neighbor_list(o.set_displacement_cutoff(0.5, 2 * np.mean(st["rad"])))
     o.calc_rigid_force_energy()
+    e = load(os.path.join(out, "energy.dat"), 6)
+    done = int(e[-1, 0])  # FIRE stops early once the largest ring force and torque are below 1e-12
     if mode == "nvt":
         o.init_rigid_langevin(T, damping / sigma)
         o.calc_rigid_force_energy()
         o.rigid_loop(steps)
+        assert e.shape[0] == 100 and done == steps
     else:
         o.init_rigid_fire([0.2, 0.5, 1.1, 0.99, dt, 10 * dt, 0.2], 20, 0)
         o.calc_rigid_force_energy()
-        o.rigid_fire(steps)
+        o.rigid_fire(done)
+        assert e.shape[0] == done and 1 <= done <= steps
+        if done < steps:
+            assert o.rigid_max_unbalanced_force() < 2e-12
     pos = load(os.path.join(out, "positions.dat"), 2).ravel()
     np.testing.assert_allclose(pos, o.get("pos"), rtol=0, atol=1e-10)
     pvel = load(os.path.join(out, "particleVel.dat"), 2).ravel()
     np.testing.assert_allclose(pvel, o.get("pvel"), rtol=1e-9, atol=1e-12)
-    e = load(os.path.join(out, "energy.dat"), 6)
-    assert e.shape[0] == 100 and int(e[-1, 0]) == steps
     if mode == "nvt":
         assert "T: " in log
 
@@ -266,3 +270,24 @@ def test_run_nve_and_active_reference_command_lines(packing_dir, tmp_path):
     params = dict(ln.split("\t") for ln in open(act + "driveParams.dat").read().splitlines())
     assert {"damping", "taup", "f0"} <= set(params)
     assert os.path.exists(act + "t0/contacts.dat") and os.path.exists(act + "t0/positions.dat")
+
+
+def test_make_dpm_minimised_packing(tmp_path, oracle_mod):
+    """makeDPM (makeDPM.cpp): sinusoidal rest angles with thetaK = 0 are the regular polygon's
+    2 pi / n, and vertex FIRE stops once the total force magnitude sqrt(sum F^2 / (2 V))
+    (DPM2D.cu:986-995) is below 1e-10; the oracle recomputes it from the saved files."""
+    out = str(tmp_path / "mk")
+    log = run([os.path.join(BIN, "makeDPM"), out, "numParticles=8", "numVertexPerParticle=16", "seed=3",
+               "maxIter=400000"])
+    last = [ln for ln in log.splitlines() if ln.startswith("FIRE: iteration")][-1]
+    assert int(last.split()[2]) < 400000, last
+    st = state_from_dir(out)
+    n = np.repeat(st["nvp"], st["nvp"])
+    np.testing.assert_allclose(st["theta0"], 2 * np.pi / n, rtol=1e-13)
+    o = oracle_mod.Oracle(st["nvp"])
+    o.load_state(st)
+    o.set_types("harmonic", "vertexVertex")
+    o.set_energy_costs(1e3, 1, 1e-2, 1)
+    o.calc_neighbor_list(2.0)
+    o.calc_force_energy()
+    assert o.total_force_magnitude() < 1e-8  # the files keep 14 digits
