@@ -19,7 +19,8 @@ namespace dpm {
 // torque drive ring velocities, with the rigid Langevin kick when MODE_LANGEVIN is also set and
 // the per-system FIRE step when MODE_FIRE is)
 enum : int { MODE_FORCE = 0, MODE_KICK = 1, MODE_LANGEVIN = 2, MODE_START = 4, MODE_FIRE = 8, MODE_BAOAB = 16,
-             MODE_BROWN = 32, MODE_RIGID = 64 };
+             MODE_BROWN = 32, MODE_RIGID = 64,
+             MODE_WIDE = 128 };  // the launch serves a set holding rings of more than 64 vertices
 
 // noise types, simControlStruct::noiseEnum order (include/DPM2D.h:27)
 enum : int { NOISE_LANGEVIN = 0, NOISE_BAOAB = 1, NOISE_BROWNIAN = 2, NOISE_DRIVEN_BROWNIAN = 3 };
@@ -86,6 +87,7 @@ struct StepArgs {
   int rigidLangevin;              // MODE_RIGID: the RigidLangevin kick (else RigidNVE's)
   int xcd;                        // XCD-aware workgroup -> tile mapping
   int rowsBuilt;                  // inside loops: flagged rows already rebuilt (k_rows_loop)
+  int rowsLdsW;                   // set by the launcher: list width the launch's LDS holds rows of
 };
 
 // Up to kMaxCopySegs device-to-device copies in one launch (k_copy_segs): 4-byte words.

@@ -455,7 +455,9 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
   const int w = threadIdx.x / WAVE;
   const int nw = blockDim.x / WAVE;
   Lane L;
-  lane_ctx(V, A.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x, threadIdx.x & (WAVE - 1), L);
+  constexpr bool WIDE = (MODE & MODE_WIDE) != 0;
+  constexpr int MB = MODE & ~MODE_WIDE;  // the integrator mode
+  lane_ctx<WIDE>(V, A.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x, threadIdx.x & (WAVE - 1), L);
   const Box B = box_of(V, L.s);
   // own position first: the row rebuild below does not change it, so its load overlaps the
   // rebuild-flag load instead of waiting behind it
@@ -477,12 +479,17 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
     // vertices instead (the reference's all-pairs search)
     const bool allPairs = A.staleCheck && (flag & FLAG_FAR) != 0;
     if (rebuild && L.tile == V.stile[L.s] && threadIdx.x == 0) A.updateCount[L.s] += 1;
-    if (rebuild && w == 0 && L.act && !A.rowsBuilt) {
-      if (A.staleCheck && !allPairs) {  // the previous position is this lane's own, in the
-                                        // buffer the drift below overwrites
-        const double ddx = pbx(x - A.xout[L.v], B), ddy = pby(y - A.yout[L.v], B);
-        if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) atomicOr(A.err, ERR_STALE_BINS);
-      }
+    if (rebuild && w == 0 && L.act && !A.rowsBuilt && A.staleCheck && !allPairs) {
+      // the previous position is this lane's own, in the buffer the drift below overwrites
+      const double ddx = pbx(x - A.xout[L.v], B), ddy = pby(y - A.yout[L.v], B);
+      if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) atomicOr(A.err, ERR_STALE_BINS);
+    }
+    // the cell search shared by the tile's waves (rows in LDS) where the launch sized LDS for it
+    const bool tileBuild = rebuild && !A.rowsBuilt && !allPairs && A.rowsLdsW == V.W && !G.brute[L.s];
+    if (tileBuild) {
+      build_rows_tile(V, P, G, A.cellCur, A.slotsCur, L.lane, L.v, L.r, L.s, L.act, w, nw, B, x, y, (int*)red,
+                      A.err, A.maxcnt);
+    } else if (rebuild && w == 0 && L.act && !A.rowsBuilt) {
       build_row(V, P, G, A.cellCur, A.slotsCur, L.v, L.s, B, A.err, A.maxcnt, allPairs);
     }
     if (rebuild) __syncthreads();
@@ -490,7 +497,7 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
   // wave 0's epilogue operands, fetched before the force work (lastPos after the rebuild above)
   double vx = 0, vy = 0, lx = 0, ly = 0;
   if (w == 0) {
-    if constexpr (MODE != MODE_FORCE && (MODE & MODE_RIGID) == 0) {
+    if constexpr (MB != MODE_FORCE && (MODE & MODE_RIGID) == 0) {
       vx = V.vx[L.v];
       vy = V.vy[L.v];
     }
@@ -505,7 +512,7 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
   const int g0 = nw > 1 ? 1 : 0;
   const int ng = nw - g0;
   // partials per gather wave: six, eight when the tile is a slice of a wide ring
-  const int nq = (INTER == INT_SMOOTH && L.wide) ? 8 : 6;
+  constexpr int nq = (INTER == INT_SMOOTH && WIDE) ? 8 : 6;
   if (w >= g0) gather_pairs<INTER, POT>(V, P, L, B, x, y, w - g0, ng, fx, fy, e, es, qx, qy, qnx, qny);
   if (nw > 1 && w > 0) {
     double* r = red + (w - 1) * nq * WAVE + L.lane;
@@ -596,7 +603,7 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
     }
   }
   if constexpr ((MODE & MODE_RIGID) != 0) {
-    rigid_epilogue<MODE, INTER == INT_VV>(V, P, A, L, B, x, y, fx, fy, e, so.cx, so.cy, lx, ly, zt0, zt1);
+    rigid_epilogue<MB, INTER == INT_VV>(V, P, A, L, B, x, y, fx, fy, e, so.cx, so.cy, lx, ly, zt0, zt1);
     return;
   }
   fx = so.fx + fx;
@@ -635,7 +642,7 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
   V.fx[L.v] = fx;
   V.fy[L.v] = fy;
   V.ev[L.v] = e;
-  if constexpr (MODE == MODE_FORCE) return;
+  if constexpr (MB == MODE_FORCE) return;
   if constexpr ((MODE & MODE_BROWN) != 0) {
     // overdamped velocity (Brownian: (F + noise xi) / gamma; driven: F / gamma), then the drift
     if constexpr ((MODE & MODE_LANGEVIN) != 0) {
@@ -800,9 +807,12 @@ __global__ void k_rows_loop(View V, Params P, StepArgs A, CellGrid G) {
 }
 
 // geometry only (calcParticleShape + calcParticlePositions)
+template <bool WIDE>
 __global__ void k_shape(View V, Params P) {
   Lane L;
-  if (!wave_tile_ctx(V, L)) return;
+  const int tile = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  if (tile >= V.ntiles) return;
+  lane_ctx<WIDE>(V, tile, threadIdx.x & (WAVE - 1), L);
   ring_shape(V, P, L, box_of(V, L.s), V.x[L.v], V.y[L.v], true);
 }
 
@@ -1562,40 +1572,19 @@ static inline int grid_w(int ntiles) { return (ntiles + 3) / 4; }
 
 // LDS of the step kernel: six partials per lane of each gather wave (eight with wide rings)
 static inline size_t step_lds(const dpm_set_s* S) {
-  return (size_t)(S->split - 1) * (S->nwide > 0 ? 8 : 6) * WAVE * sizeof(double);
+  const size_t red = (size_t)(S->split - 1) * (S->nwide > 0 ? 8 : 6) * WAVE * sizeof(double);
+  const size_t rows = (S->split >= 2 && S->V.W <= 64) ? (size_t)(64 + 64 * S->V.W) * sizeof(int) : 0;
+  return red > rows ? red : rows;
 }
 
+// rigid-ring modes (sets of rings of at most 64 vertices only)
 template <int INTER, int POT>
-static hipError_t step_mode(dpm_set_s* S, int mode, const StepArgs& A) {
+static hipError_t step_rigid(dpm_set_s* S, int mode, const StepArgs& A) {
   const int g = S->V.ntiles;
   const int b = WAVE * S->split;
   const size_t sh = step_lds(S);
   hipStream_t st = S->stream;
   switch (mode) {
-    case MODE_FORCE: k_step<INTER, POT, MODE_FORCE><<<g, b, sh, st>>>(S->V, S->P, A, S->G); break;
-    case MODE_KICK: k_step<INTER, POT, MODE_KICK><<<g, b, sh, st>>>(S->V, S->P, A, S->G); break;
-    case MODE_KICK | MODE_LANGEVIN:
-      k_step<INTER, POT, MODE_KICK | MODE_LANGEVIN><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
-      break;
-    case MODE_KICK | MODE_START:
-      k_step<INTER, POT, MODE_KICK | MODE_START><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
-      break;
-    case MODE_KICK | MODE_LANGEVIN | MODE_START:
-      k_step<INTER, POT, MODE_KICK | MODE_LANGEVIN | MODE_START><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
-      break;
-    case MODE_FIRE: k_step<INTER, POT, MODE_FIRE><<<g, b, sh, st>>>(S->V, S->P, A, S->G); break;
-    case MODE_KICK | MODE_BAOAB:
-      k_step<INTER, POT, MODE_KICK | MODE_BAOAB><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
-      break;
-    case MODE_KICK | MODE_BAOAB | MODE_START:
-      k_step<INTER, POT, MODE_KICK | MODE_BAOAB | MODE_START><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
-      break;
-    case MODE_BROWN | MODE_START:
-      k_step<INTER, POT, MODE_BROWN | MODE_START><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
-      break;
-    case MODE_BROWN | MODE_LANGEVIN | MODE_START:
-      k_step<INTER, POT, MODE_BROWN | MODE_LANGEVIN | MODE_START><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
-      break;
     case MODE_RIGID: k_step<INTER, POT, MODE_RIGID><<<g, b, sh, st>>>(S->V, S->P, A, S->G); break;
     case MODE_RIGID | MODE_KICK:
       k_step<INTER, POT, MODE_RIGID | MODE_KICK><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
@@ -1611,15 +1600,70 @@ static hipError_t step_mode(dpm_set_s* S, int mode, const StepArgs& A) {
   return hipGetLastError();
 }
 
-template <int INTER>
+template <int INTER, int POT, int WB>
+static hipError_t step_mode(dpm_set_s* S, int mode, const StepArgs& A) {
+  if constexpr (WB != 0) {
+    if (mode & MODE_RIGID) return hipErrorInvalidValue;  // refused for wide sets before this point
+  } else {
+    if (mode & MODE_RIGID) return step_rigid<INTER, POT>(S, mode, A);
+  }
+  const int g = S->V.ntiles;
+  const int b = WAVE * S->split;
+  const size_t sh = step_lds(S);
+  hipStream_t st = S->stream;
+  switch (mode) {
+    case MODE_FORCE: k_step<INTER, POT, MODE_FORCE | WB><<<g, b, sh, st>>>(S->V, S->P, A, S->G); break;
+    case MODE_KICK: k_step<INTER, POT, MODE_KICK | WB><<<g, b, sh, st>>>(S->V, S->P, A, S->G); break;
+    case MODE_KICK | MODE_LANGEVIN:
+      k_step<INTER, POT, MODE_KICK | MODE_LANGEVIN | WB><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
+      break;
+    case MODE_KICK | MODE_START:
+      k_step<INTER, POT, MODE_KICK | MODE_START | WB><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
+      break;
+    case MODE_KICK | MODE_LANGEVIN | MODE_START:
+      k_step<INTER, POT, MODE_KICK | MODE_LANGEVIN | MODE_START | WB><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
+      break;
+    case MODE_FIRE: k_step<INTER, POT, MODE_FIRE | WB><<<g, b, sh, st>>>(S->V, S->P, A, S->G); break;
+    case MODE_KICK | MODE_BAOAB:
+      k_step<INTER, POT, MODE_KICK | MODE_BAOAB | WB><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
+      break;
+    case MODE_KICK | MODE_BAOAB | MODE_START:
+      k_step<INTER, POT, MODE_KICK | MODE_BAOAB | MODE_START | WB><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
+      break;
+    case MODE_BROWN | MODE_START:
+      k_step<INTER, POT, MODE_BROWN | MODE_START | WB><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
+      break;
+    case MODE_BROWN | MODE_LANGEVIN | MODE_START:
+      k_step<INTER, POT, MODE_BROWN | MODE_LANGEVIN | MODE_START | WB><<<g, b, sh, st>>>(S->V, S->P, A, S->G);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int INTER, int WB>
 static hipError_t step_pot(dpm_set_s* S, int mode, const StepArgs& A) {
   switch (S->P.potential) {
-    case POT_HARMONIC: return step_mode<INTER, POT_HARMONIC>(S, mode, A);
-    case POT_LJ: return step_mode<INTER, POT_LJ>(S, mode, A);
-    case POT_ADHESIVE: return step_mode<INTER, POT_ADHESIVE>(S, mode, A);
-    case POT_WCA: return step_mode<INTER, POT_WCA>(S, mode, A);
+    case POT_HARMONIC: return step_mode<INTER, POT_HARMONIC, WB>(S, mode, A);
+    case POT_LJ: return step_mode<INTER, POT_LJ, WB>(S, mode, A);
+    case POT_ADHESIVE: return step_mode<INTER, POT_ADHESIVE, WB>(S, mode, A);
+    case POT_WCA: return step_mode<INTER, POT_WCA, WB>(S, mode, A);
   }
   return hipErrorInvalidValue;
+}
+
+template <int WB>
+static hipError_t step_all(dpm_set_s* S, const StepArgs& A) {
+  const int g = S->V.ntiles, b = WAVE * S->split;
+  const size_t sh = step_lds(S);
+  hipStream_t st = S->stream;
+  switch (S->P.potential) {
+    case POT_HARMONIC: k_step<INT_ALL, POT_HARMONIC, MODE_FORCE | WB><<<g, b, sh, st>>>(S->V, S->P, A, S->G); break;
+    case POT_LJ: k_step<INT_ALL, POT_LJ, MODE_FORCE | WB><<<g, b, sh, st>>>(S->V, S->P, A, S->G); break;
+    case POT_ADHESIVE: k_step<INT_ALL, POT_ADHESIVE, MODE_FORCE | WB><<<g, b, sh, st>>>(S->V, S->P, A, S->G); break;
+    case POT_WCA: k_step<INT_ALL, POT_WCA, MODE_FORCE | WB><<<g, b, sh, st>>>(S->V, S->P, A, S->G); break;
+  }
+  return hipGetLastError();
 }
 
 static hipError_t launch_step_kernel(dpm_set_s* S, int mode, const StepArgs& A);
@@ -1646,38 +1690,38 @@ hipError_t launch_step(dpm_set_s* S, int mode, const StepArgs& A0) {
   return hipGetLastError();
 }
 
-static hipError_t launch_step_kernel(dpm_set_s* S, int mode, const StepArgs& A) {
+static hipError_t launch_step_kernel(dpm_set_s* S, int mode, const StepArgs& A0) {
   // refuse argument sets the chosen epilogue would dereference as null
-  if (!A.ctr) return hipErrorInvalidValue;
+  if (!A0.ctr) return hipErrorInvalidValue;
+  // in-loop rebuilds by the whole workgroup need LDS for the tile's rows: 64 + 64 W ints, in the
+  // region the gather partials use afterwards (launches of more than one wave, W <= 64)
+  StepArgs A = A0;
+  A.rowsLdsW = (A.manage && S->split >= 2 && S->V.W <= 64) ? S->V.W : 0;
   if (S->P.particle == PART_ACTIVE_SHAPE && !S->V.l0r) return hipErrorInvalidValue;
   if ((mode & MODE_FIRE) && (!A.fire || !A.part)) return hipErrorInvalidValue;
   if ((mode & MODE_RIGID) && S->P.interaction == INT_ALL) return hipErrorInvalidValue;
   if ((mode & MODE_START) && (!A.xout || !A.yout || !A.flagOut || !A.anyOut)) return hipErrorInvalidValue;
   if (A.manage && (!A.flagCur || !A.cellCur || !A.cellOther || !A.err || !A.maxcnt || !A.updateCount))
     return hipErrorInvalidValue;
+  const bool wide = S->nwide > 0;
   switch (S->P.interaction) {
-    case INT_VV: return step_pot<INT_VV>(S, mode, A);
-    case INT_SMOOTH: return step_pot<INT_SMOOTH>(S, mode, A);
-    case INT_ALL: {
+    case INT_VV: return wide ? step_pot<INT_VV, MODE_WIDE>(S, mode, A) : step_pot<INT_VV, 0>(S, mode, A);
+    case INT_SMOOTH:
+      return wide ? step_pot<INT_SMOOTH, MODE_WIDE>(S, mode, A) : step_pot<INT_SMOOTH, 0>(S, mode, A);
+    case INT_ALL:
       if (mode != MODE_FORCE) return hipErrorInvalidValue;
-      const int g = S->V.ntiles, b = WAVE * S->split;
-      const size_t sh = step_lds(S);
-      hipStream_t st = S->stream;
-      switch (S->P.potential) {
-        case POT_HARMONIC: k_step<INT_ALL, POT_HARMONIC, MODE_FORCE><<<g, b, sh, st>>>(S->V, S->P, A, S->G); break;
-        case POT_LJ: k_step<INT_ALL, POT_LJ, MODE_FORCE><<<g, b, sh, st>>>(S->V, S->P, A, S->G); break;
-        case POT_ADHESIVE: k_step<INT_ALL, POT_ADHESIVE, MODE_FORCE><<<g, b, sh, st>>>(S->V, S->P, A, S->G); break;
-        case POT_WCA: k_step<INT_ALL, POT_WCA, MODE_FORCE><<<g, b, sh, st>>>(S->V, S->P, A, S->G); break;
-      }
-      return hipGetLastError();
-    }
+      return wide ? step_all<MODE_WIDE>(S, A) : step_all<0>(S, A);
   }
   return hipErrorInvalidValue;
 }
 
 hipError_t launch_shape_only(dpm_set_s* S) {
-  if (S->nwide > 0) k_wide_rings<<<(int)S->nwide, 256, 0, S->stream>>>(S->V, S->d_wide);
-  k_shape<<<grid_w(S->V.ntiles), 256, 0, S->stream>>>(S->V, S->P);
+  if (S->nwide > 0) {
+    k_wide_rings<<<(int)S->nwide, 256, 0, S->stream>>>(S->V, S->d_wide);
+    k_shape<true><<<grid_w(S->V.ntiles), 256, 0, S->stream>>>(S->V, S->P);
+  } else {
+    k_shape<false><<<grid_w(S->V.ntiles), 256, 0, S->stream>>>(S->V, S->P);
+  }
   return hipGetLastError();
 }
 

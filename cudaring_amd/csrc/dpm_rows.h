@@ -97,6 +97,72 @@ __device__ void build_row(const View& V, const Params& P, const CellGrid& G, con
   V.ly[v] = y;
 }
 
+// The rows of a tile's vertices (lane = vertex, L.act lanes), built by all nw waves of its
+// workgroup: thread (w, lane) scans the cells w, w + nw, ... of the 3 x 3 block around its lane's
+// vertex and appends the hits to the lane's list in LDS (an LDS counter per lane, so the order
+// of arrival varies); after a barrier each lane of wave 0 sorts its list ascending, and all waves
+// write the sorted entries with their prev pairs into the ELL rows.  The same rows as build_row
+// from the same cells (its predicate, ascending order), with the dependent steps of an insertion
+// in LDS instead of memory.  Every thread of the workgroup calls it (it holds barriers); lds holds
+// 64 + 64 W ints; the caller synchronises before reading the rows or reusing lds.
+__device__ void build_rows_tile(const View& V, const Params& P, const CellGrid& G, const int* count,
+                                const int* slots, int lane, int v, int r, int s, bool act, int w, int nw,
+                                const Box& B, double x, double y, int* lds, int* err, int* maxcnt) {
+  int* lcnt = lds;
+  int* lrow = lds + 64;
+  const int W = V.W;
+  if (w == 0) lcnt[lane] = 0;
+  __syncthreads();
+  if (act) {
+    const double rv = V.rad[v];
+    const int ncx = G.ncx[s], ncy = G.ncy[s];
+    int cx, cy;
+    cell_of(x, y, B, ncx, ncy, cx, cy);
+    for (int k = w; k < 9; k += nw) {
+      const int yy = (cy + k / 3 - 1 + ncy) % ncy;
+      const int xx = (cx + k % 3 - 1 + ncx) % ncx;
+      const int cell = G.coff[s] + yy * ncx + xx;
+      const int m = min(count[cell], G.cap);
+      for (int q = 0; q < m; q++) {
+        const int j = slots[(long)cell * G.cap + q];
+        if (V.vring[j] == r) continue;
+        const double dx = pbx(x - V.x[j], B), dy = pby(y - V.y[j], B);
+        const double d = sqrt(dx * dx + dy * dy);
+        if (-(1 - d / (rv + V.rad[j])) < P.cutDistance) {
+          const int idx = atomicAdd(&lcnt[lane], 1);
+          if (idx < W) lrow[idx * 64 + lane] = j;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int c = act ? lcnt[lane] : 0;
+  const int n = c < W ? c : W;
+  if (w == 0 && act) {
+    for (int a = 1; a < n; a++) {
+      const int key = lrow[a * 64 + lane];
+      int b = a - 1;
+      while (b >= 0 && lrow[b * 64 + lane] > key) {
+        lrow[(b + 1) * 64 + lane] = lrow[b * 64 + lane];
+        b--;
+      }
+      lrow[(b + 1) * 64 + lane] = key;
+    }
+    if (c > W) atomicOr(err, ERR_LIST_OVERFLOW);
+    atomicMax(maxcnt, c);
+    V.ncnt[v] = n;
+    V.lx[v] = x;
+    V.ly[v] = y;
+  }
+  __syncthreads();
+  if (act)
+    for (int k = w; k < n; k += nw) {
+      const int j = lrow[k * 64 + lane];
+      V.nbr[(long)k * V.nv + v] = j;
+      V.nbrp[(long)k * V.nv + v] = prev_pair(V, j);
+    }
+}
+
 // Bin vertex v at (x, y) of system s into a cell list (counts + slots of capacity G.cap);
 // systems searched by brute force have no cells.
 __device__ __forceinline__ void bin_vertex(const CellGrid& G, const Box& B, int* count, int* slots, int v, int s,

@@ -50,6 +50,9 @@ struct Lane {
   bool act, wide;
 };
 
+// WIDE: the launch may hold slices of wide rings (sets with rings of more than 64 vertices); the
+// other kernels either never run on such sets or use only lane, tile, v, r, s and act.
+template <bool WIDE = false>
 __device__ __forceinline__ void lane_ctx(const View& V, int tile, int lane, Lane& L) {
   L.lane = lane;
   L.tile = tile;
@@ -63,7 +66,7 @@ __device__ __forceinline__ void lane_ctx(const View& V, int tile, int lane, Lane
   const int first = vi.y;
   L.f = first;
   L.n = vi.z;
-  L.wide = L.n > WAVE;
+  L.wide = WIDE && L.n > WAVE;
   if (L.wide) {
     L.segS = L.act ? 0 : lane;
     L.segE = L.act ? v1 - L.v0 - 1 : lane;
