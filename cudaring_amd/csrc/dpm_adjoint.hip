@@ -17,6 +17,8 @@
 // trajectory costs one H w product:
 //     ax -= c dt H(x_n) w,  g_theta += c dt (dF/dtheta)^T w,  w <- w + dt ax,
 // with c = 1/2 at the two ends of the trajectory and 1 inside.
+#include <algorithm>
+
 #include "dpm_internal.h"
 #include "dpm_rows.h"
 #include "dpm_tile.h"
@@ -209,7 +211,11 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_adj_hvp(View V, Params P, Ce
       A.cellOther[c] = 0;
   if (A.flag) {
     const bool rebuild = A.flag[L.s] != 0;
-    if (rebuild && w == 0 && L.act) build_row(V, P, G, A.count, G.slots, L.v, L.s, B, A.err, A.maxcnt);
+    if (rebuild && A.rowsLdsW == V.W && !G.brute[L.s])  // the tile's waves share the cell search
+      build_rows_tile(V, P, G, A.count, G.slots, L.lane, L.v, L.r, L.s, L.act, w, nw, B, x, y, (int*)red, A.err,
+                      A.maxcnt);
+    else if (rebuild && w == 0 && L.act)
+      build_row(V, P, G, A.count, G.slots, L.v, L.s, B, A.err, A.maxcnt);
     if (rebuild) __syncthreads();
   }
 
@@ -536,8 +542,13 @@ hipError_t launch_adj_hvp(dpm_set_s* S, const View& Vn, AdjArgs A) {
   A.xcd = S->xcdMap;
   const unsigned g = (unsigned)S->V.ntiles, b = WAVE * S->split;
   hipStream_t st = S->stream;
+  // rows rebuilt by the whole workgroup need 64 + 64 W ints of LDS (launches of more than one
+  // wave, W <= 64), in the region the gather partials use afterwards
+  const bool tileRows = A.flag && S->split >= 2 && S->V.W <= 64;
+  A.rowsLdsW = tileRows ? S->V.W : 0;
+  const size_t rows = tileRows ? (size_t)(64 + 64 * S->V.W) * sizeof(int) : 0;
   if (S->P.interaction == INT_SMOOTH) {
-    const size_t sh = (size_t)(S->split - 1) * 4 * WAVE * sizeof(double);
+    const size_t sh = std::max((size_t)(S->split - 1) * 4 * WAVE * sizeof(double), rows);
     switch (S->P.potential) {
       case POT_HARMONIC: k_adj_hvp<INT_SMOOTH, POT_HARMONIC><<<g, b, sh, st>>>(Vn, S->P, S->G, A); break;
       case POT_LJ: k_adj_hvp<INT_SMOOTH, POT_LJ><<<g, b, sh, st>>>(Vn, S->P, S->G, A); break;
@@ -546,7 +557,7 @@ hipError_t launch_adj_hvp(dpm_set_s* S, const View& Vn, AdjArgs A) {
     }
     return hipGetLastError();
   }
-  const size_t sh = (size_t)(S->split - 1) * 2 * WAVE * sizeof(double);
+  const size_t sh = std::max((size_t)(S->split - 1) * 2 * WAVE * sizeof(double), rows);
   switch (S->P.potential) {
     case POT_HARMONIC: k_adj_hvp<INT_VV, POT_HARMONIC><<<g, b, sh, st>>>(Vn, S->P, S->G, A); break;
     case POT_LJ: k_adj_hvp<INT_VV, POT_LJ><<<g, b, sh, st>>>(Vn, S->P, S->G, A); break;

@@ -125,6 +125,7 @@ struct AdjArgs {
   int mode;
   int update;
   int xcd;              // XCD-aware workgroup -> tile mapping
+  int rowsLdsW;         // set by the launcher: list width the launch's LDS holds rows of
 };
 
 }  // namespace dpm
