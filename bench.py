@@ -156,6 +156,63 @@ def cpu_baseline(st, dt, steps, runs=5):
                       f"rebuilt on skin crossing), median of {runs} after 1 warm-up, at {n} threads and at 1"}
 
 
+CFG1_SEED, CFG1_STEPS = 20261202 + 1, 10000
+
+
+def nve_small_bench(args):
+    """The plain NVE configuration: 32 bidisperse rings (16/22 vertices, 608 vertices), harmonic
+    vertex-vertex, phi 0.80, uniform random centres, per-ring Gaussian velocities at T 1e-3 with
+    the mean removed, dt = 0.01 sigma / sqrt(ec), skin 0.3; 1e4 velocity-Verlet steps on the oracle
+    (one thread and the process's CPU share; median of 3 after 1 warm-up) and on the GPU."""
+    from cudaring_amd import DPMSet, packing
+    from oracle import oracle
+    st = packing.random_system(32, 0.80, CFG1_SEED)
+    nv = int(np.sum(st["nvp"]))
+    sigma = float(2 * np.sqrt(np.mean(st["a0"]) / np.pi))
+    dt, size = 0.01 * sigma, 2 * float(np.mean(st["rad"]))
+
+    def setup(obj):
+        obj.load_state(st)
+        obj.set_types("harmonic", "vertexVertex")
+        obj.set_energy_costs(*COSTS)
+        obj.set_dt(dt)
+        cut = obj.set_displacement_cutoff(0.3, size)
+        obj.calc_neighbor_list(cut)
+        obj.calc_force_energy()
+        obj.init_nve(1e-3, zero_momentum=True)
+
+    def cpu_run(threads):
+        o = oracle.Oracle(st["nvp"])
+        o.set_rng(CFG1_SEED, 0)
+        o.set_threads(threads)
+        setup(o)
+        t0 = time.perf_counter()
+        o.nve(CFG1_STEPS)
+        return time.perf_counter() - t0
+
+    def cpu_rate(threads):
+        cpu_run(threads)
+        return nv * CFG1_STEPS / float(np.median([cpu_run(threads) for _ in range(3)]))
+
+    g = DPMSet(st["nvp"])
+    g.set_seed(CFG1_SEED)
+    setup(g)
+    g.nve(CFG1_STEPS // 10)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.nve(CFG1_STEPS)
+    g.synchronize()
+    gpu = nv * CFG1_STEPS / (time.perf_counter() - t0)
+    g.close()
+    n = cpu_threads()
+    out = {"workload": "NVE, 32 rings 16/22 (608 vertices), harmonic vertexVertex, phi 0.80, T 1e-3, "
+                       "dt 0.01 sigma, 1e4 steps", "gpu_vertex_steps_per_s": round(gpu, 1)}
+    if not args.no_cpu:
+        out.update({"cpu_vertex_steps_per_s_1_thread": round(cpu_rate(1), 1),
+                    f"cpu_vertex_steps_per_s_{n}_threads": round(cpu_rate(n), 1)})
+    return out
+
+
 FIRE_SYSTEMS, FIRE_SEED = 512, 20261202 + 3
 FIRE_PARAMS = (0.1, 0.5, 1.1, 0.99, 0.01, 0.1, 0.1)  # a_start, f_dec, f_inc, f_a, dt0, dt_max, a
 FIRE_TOL, FIRE_DPHI, FIRE_MAXIT = 1e-12, 1e-3, 100000
@@ -268,6 +325,7 @@ def batched_fire(args):
             p.close()
     n_sample = len(pick) * args.fire_steps
     engine_rate, host_rate = n_sample / t_path["engine"], n_sample / t_path["host"]
+    cpu = None if args.no_cpu else cpu_fire(systems, pick, start, voff, roff, size, skin)
     batch_rate = done / t_batch
     vit = float(np.sum(iters * nv_sys))
     s.close()
@@ -287,7 +345,50 @@ def batched_fire(args):
                       f"{it_path['host']} iterations in {t_path['host']:.2f} s",
             "start_relax_iterations_max": int(it0.max()), "start_relax_s": round(relax_s, 2),
             "fire_kernel": {"persistent": bool(fire_kernel[0]), "waves": int(fire_kernel[1])},
-            "fire_tune_s": fire_tune, "skin": skin, "cell_scale": cell_scale, "neighbor_tune_s": nbr_tune}
+            "fire_tune_s": fire_tune, "skin": skin, "cell_scale": cell_scale, "neighbor_tune_s": nbr_tune,
+            "cpu": cpu}
+
+
+def cpu_fire(systems, pick, start, voff, roff, size, skin):
+    """The oracle on the sampled packings' first phi step (same start, same skin), one packing per
+    thread across the process's CPU share: packings/s over the wall time, and at one thread from
+    the sum of the per-packing times (each packing runs single-threaded)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import oracle
+
+    def one(i):
+        sl, rl = slice(voff[i], voff[i + 1]), slice(roff[i], roff[i + 1])
+        sub = dict(systems[i])
+        sub["pos"] = start["pos"][2 * voff[i]:2 * voff[i + 1]]
+        sub["vel"] = start["vel"][2 * voff[i]:2 * voff[i + 1]]
+        sub["rad"], sub["l0"], sub["a0"] = start["rad"][sl], start["l0"][sl], start["a0"][rl]
+        o = oracle.Oracle(sub["nvp"])
+        o.load_state(sub)
+        o.set_threads(1)
+        o.set_types("harmonic", "vertexVertex")
+        o.set_energy_costs(*FIRE_COSTS)
+        cut = o.set_displacement_cutoff(skin, size)
+        t0 = time.perf_counter()
+        o.calc_neighbor_list(cut)
+        o.calc_force_energy()  # the shapes phi() reads
+        phi = o.phi()
+        o.scale_vertices(np.sqrt((phi + FIRE_DPHI) / phi))
+        o.calc_neighbor_list(cut)
+        o.calc_force_energy()
+        o.init_fire(FIRE_PARAMS, 20, 0)
+        it = o.fire_minimize(FIRE_TOL, FIRE_MAXIT)
+        return time.perf_counter() - t0, int(it), bool(o.max_unbalanced_force() <= FIRE_TOL)
+
+    n = cpu_threads()
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=n) as ex:
+        res = list(ex.map(one, list(pick)))
+    wall = time.perf_counter() - t0
+    serial = sum(r[0] for r in res)
+    return {"packings_per_s": round(len(pick) / wall, 3), "threads": n,
+            "packings_per_s_1_thread": round(len(pick) / serial, 3),
+            "iterations": int(sum(r[1] for r in res)), "converged": int(sum(r[2] for r in res)),
+            "sample": f"the first phi step of the {len(pick)} sampled packings, one per thread"}
 
 
 ADJ_RINGS, ADJ_PHI, ADJ_SEED, ADJ_T = 256, 0.85, 20261202 + 4, 1e-3
@@ -430,8 +531,45 @@ def cfg5_bench(args, torch, batched=False):
            "rebuilds": [int(x) for x in s.update_count()[:4]], "T_end": float(np.mean(s.temperature())),
            "prep_s": round(prep, 1), "waves_per_tile": waves,
            "finite": bool(np.all(np.isfinite(s.get("pos"))))}
+    if not args.no_cpu and not batched:
+        out["cpu"] = cpu_cfg5(s, st, sigma, cut)
     s.close()
     return out
+
+
+def cpu_cfg5(s, st, sigma, cut, steps=10):
+    """The oracle on the active-adhesive state the GPU run ended in: `steps` Langevin steps at the
+    process's CPU share and at one thread (median of 3 after 1 warm-up)."""
+    from oracle import oracle
+    state = {"nvp": st["nvp"], "theta0": st["theta0"], "box": s.get("box")}
+    for k in ("pos", "vel", "rad", "l0", "a0", "pangle"):
+        state[k] = s.get(k)
+
+    def run(threads):
+        o = oracle.Oracle(state["nvp"])
+        o.load_state(state)
+        o.set_threads(threads)
+        o.set_types("adhesive", "vertexSmooth", "harmonic", "active")
+        o.set_attraction(0.1, 0.2)
+        o.set_energy_costs(*CFG5_COSTS)
+        o.set_self_propulsion(0.1 * GAMMA / sigma, 100.0 * sigma)
+        o.set_dt(DT_UNITS * sigma)
+        o.set_rng(CFG5_SEED, 0)
+        o.set_displacement_cutoff(SKIN, 2 * float(np.mean(state["rad"])))
+        o.calc_neighbor_list(cut)
+        o.calc_force_energy()
+        o.init_langevin(1e-4, GAMMA / sigma, read_state=True)
+        t0 = time.perf_counter()
+        o.langevin(steps)
+        return time.perf_counter() - t0
+
+    def rate(threads):
+        run(threads)
+        return len(state["rad"]) * steps / float(np.median([run(threads) for _ in range(3)]))
+
+    n = cpu_threads()
+    return {"vertex_steps_per_s": round(rate(n), 1), "threads": n, "vertex_steps_per_s_1_thread": round(rate(1), 1),
+            "sample": f"{steps} Langevin steps of the final state, median of 3 after 1 warm-up"}
 
 
 def integrators_bench(s, dt, steps):
@@ -633,6 +771,7 @@ def main():
     fire = None if args.no_fire else batched_fire(args)
     adjoint = None if args.no_adjoint else adjoint_bench(args, torch)
     active = None if args.no_cfg5 else cfg5_bench(args, torch)
+    nve_small = nve_small_bench(args)
     out = {
         "metric": METRIC, "value": value, "unit": "vertex-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True,
@@ -653,6 +792,7 @@ def main():
         "batched_fire": fire,
         "adjoint": adjoint,
         "active_adhesive": active,
+        "nve_small": nve_small,
         "rigid": rigid,
     }
     print(json.dumps(out), flush=True)
