@@ -149,22 +149,30 @@ int upload_box(dpm_set_s* S) {
   return DPM_OK;
 }
 
-// the step loop's three cell lists (counts + slots at the current capacity) and four flag rows
+// the step loop's three cell lists (counts + slots at the current capacity, one allocation
+// each), four flag rows and three generation rows
 int alloc_md_cells(dpm_set_s* S) {
   HIPCHK(hipStreamSynchronize(S->stream));
-  const long C = S->G.ncells;
-  for (int i = 0; i < 3; i++) {
-    hipFree(S->mdCount[i]);
-    hipFree(S->mdSlots[i]);
-    S->mdCount[i] = S->mdSlots[i] = nullptr;
-    HIPCHK(dalloc(&S->mdCount[i], C > 0 ? C : 1));
-    HIPCHK(dalloc(&S->mdSlots[i], (C > 0 ? C : 1) * S->G.cap));
-    HIPCHK(launch_fill_int(S->mdCount[i], 0, C, S->stream));
+  const long C = S->G.ncells > 0 ? S->G.ncells : 1;
+  hipFree(S->mdCount[0]);
+  hipFree(S->mdSlots[0]);
+  for (int i = 0; i < 3; i++) S->mdCount[i] = S->mdSlots[i] = nullptr;
+  HIPCHK(dalloc(&S->mdCount[0], 3 * C));
+  HIPCHK(dalloc(&S->mdSlots[0], 3 * C * S->G.cap));
+  for (int i = 1; i < 3; i++) {
+    S->mdCount[i] = S->mdCount[0] + i * C;
+    S->mdSlots[i] = S->mdSlots[0] + i * C * S->G.cap;
   }
+  HIPCHK(launch_fill_int(S->mdCount[0], 0, 3 * C, S->stream));
   for (int i = 0; i < 4; i++)
     if (!S->mdFlag[i]) {
       HIPCHK(dalloc(&S->mdFlag[i], S->nsys));
       HIPCHK(launch_fill_int(S->mdFlag[i], 0, S->nsys, S->stream));
+    }
+  for (int i = 0; i < 3; i++)
+    if (!S->mdGen[i]) {
+      HIPCHK(dalloc(&S->mdGen[i], S->nsys));
+      HIPCHK(launch_fill_int(S->mdGen[i], 0, S->nsys, S->stream));
     }
   HIPCHK(hipStreamSynchronize(S->stream));
   return DPM_OK;
@@ -177,9 +185,10 @@ int configure_cells(dpm_set_s* S) {
   long C = 0;
   double maxMean = 1;
   for (long s = 0; s < B; s++) {
-    // list radius, plus the cutoff / 3 a vertex may move between the step loop's binning and
-    // its rebuild
-    const double rc = ((1 + S->P.cutDistance) * 2 * S->h_maxrad[s] + S->P.cutoff / 3) * S->cellScale;
+    // list radius, plus the 2 cutoff / 3 a vertex may lie from its list position (where the step
+    // loop binned it) when its system rebuilds: at most cutoff / 3 before the drift that
+    // triggers the rebuild (the displacement test) and cutoff / 3 in that drift (FLAG_FAR)
+    const double rc = ((1 + S->P.cutDistance) * 2 * S->h_maxrad[s] + 2 * S->P.cutoff / 3) * S->cellScale;
     const double Lx = S->h_box[2 * s], Ly = S->h_box[2 * s + 1];
     int nx = rc > 0 ? (int)floor(Lx / rc) : 0;
     int ny = rc > 0 ? (int)floor(Ly / rc) : 0;
@@ -307,8 +316,8 @@ int rebuild(dpm_set_s* S, const std::vector<int>& flags, int countUpdates, bool 
 // status of a loop whose kernels left the error bits `err`
 int loop_status(dpm_set_s* S, int err) {
   if (err & ERR_STALE_BINS)
-    return fail(S, DPM_ERR_STATE, "a vertex moved more than cutoff / 3 in one step: the neighbour-list skin "
-                                  "does not cover the time step");
+    return fail(S, DPM_ERR_STATE, "a rebuilding vertex lay more than 2 cutoff / 3 from the position its cell "
+                                  "was binned at: the neighbour-list skin does not cover the time step");
   if (err) return fail(S, DPM_ERR_OVERFLOW, err & ERR_LIST_OVERFLOW ? "neighbour list overflow during loop"
                                                                     : "cell overflow during loop");
   return DPM_OK;
@@ -457,20 +466,22 @@ int loop_step(dpm_set_s* S, int mode, int bumpT, double* recx = nullptr, double*
 }
 
 // Arguments of the step-loop kernel of rotation index r: it reads the rebuild flags F[r] of its
-// positions, rebuilds flagged rows from cells C[r - 1] (the previous positions, one step stale:
-// the cells are widened by the cutoff / 3 the displacement test allows), bins its own positions
-// into C[r], writes the flags F[r + 1] of its drift destination and clears C[r + 1] and F[r + 2]
-// for the kernels after it.  Its draws use the call indices ctr[k] + offset.  The loop needs no
-// binning launch.
+// positions and the generations G[r] of its systems' list-position cells, writes the flags
+// F[r + 1] of its drift destination and the generations G[r + 1], and clears F[r + 2] for the
+// kernel after it.  A flagged system rebuilds its rows from the cells of its list positions
+// (widened by the 2 cutoff / 3 a vertex may have moved from them), bins its new list positions
+// into the next cell list and clears the one after; no other step touches the cells, so the
+// loop needs no binning, in a launch of its own or in the step kernel.  Its draws use the call
+// indices ctr[k] + offset.
 StepArgs md_args(dpm_set_s* S, long r, long long offT, long long offA) {
   StepArgs A = loop_args(S);
   const int a = (int)(r % 3), b = (int)((r + 1) % 3), c = (int)((r + 2) % 3);
   A.flagCur = S->mdFlag[a];
-  A.cellCur = S->mdCount[c];  // r - 1
-  A.slotsCur = S->mdSlots[c];
-  A.cellOut = S->mdCount[a];
-  A.slotsOut = S->mdSlots[a];
-  A.cellOther = S->mdCount[b];
+  A.cellCur = S->mdCount[0];
+  A.slotsCur = S->mdSlots[0];
+  A.cellOther = nullptr;
+  A.genCur = S->mdGen[a];
+  A.genOut = S->mdGen[b];
   A.flagOut = S->mdFlag[b];
   A.flagClear = S->mdFlag[c];
   A.staleCheck = 1;
@@ -480,18 +491,22 @@ StepArgs md_args(dpm_set_s* S, long r, long long offT, long long offA) {
   return A;
 }
 
-int md_clear_slot(dpm_set_s* S, int r) {
-  HIPCHK(launch_fill_int(S->mdCount[r], 0, S->G.ncells, S->stream));
-  HIPCHK(launch_fill_int(S->mdFlag[r], 0, S->nsys, S->stream));
+// A step loop's cell protocol at its start: cell lists 0 and 1 cleared (the start kernel bins
+// the list positions into list 0, the first rebuild bins into list 1) and the flags of the first
+// step's drift destination F[1] cleared.
+int md_start(dpm_set_s* S) {
+  HIPCHK(launch_fill_int(S->mdCount[0], 0, 2L * (S->G.ncells > 0 ? S->G.ncells : 1), S->stream));
+  HIPCHK(launch_fill_int(S->mdFlag[1], 0, S->nsys, S->stream));
   return DPM_OK;
 }
 
 long long active_draws(const dpm_set_s* S, long steps) { return S->P.particle != PART_PASSIVE ? steps : 0; }
 
 // Launch sequence of a velocity-Verlet loop (NVE / Langevin):
-//   start: v += dt/2 F, x' = x + dt v, bin + displacement test of x'  (buffer b -> 1-b)
+//   start: list positions binned; v += dt/2 F, x' = x + dt v, displacement test of x'
+//          (buffer b -> 1-b)
 //   per step: step [rebuild rows if flagged; F(x'); thermostat; v += dt/2 F;
-//             unless last: v += dt/2 F, x'' = x' + dt v, bin + displacement test of x'']
+//             unless last: v += dt/2 F, x'' = x' + dt v, displacement test of x'']
 //   end: RNG call counters advanced by the loop's draws.
 // The fused form performs the reference's operations in the reference's order.
 // With `rec` (slots of 2 nv doubles, x then y), the position after step i is also written to
@@ -499,17 +514,20 @@ long long active_draws(const dpm_set_s* S, long steps) { return S->P.particle !=
 int verlet_sequence(dpm_set_s* S, long steps, bool langevin, double* rec = nullptr) {
   const double h = 0.5 * S->P.dt;
   const long nv = S->nv;
-  HIPCHK(launch_fill_int(S->mdCount[0], 0, S->G.ncells, S->stream));
-  HIPCHK(launch_fill_int(S->mdFlag[1], 0, S->nsys, S->stream));
+  int rc = md_start(S);
+  if (rc) return rc;
+  if (rec) {
+    // a recomputed segment starts from a restored checkpoint without a valid list: its list
+    // positions become the start positions (whose cells the start kernel bins), and every
+    // system rebuilds its rows in the first step (below)
+    HIPCHK(launch_copy_d(S->V.lx, S->V.x, nv, S->stream));
+    HIPCHK(launch_copy_d(S->V.ly, S->V.y, nv, S->stream));
+  }
   StepArgs A0 = md_args(S, 0, 0, 0);
   A0.recx = rec ? rec + 2 * nv : nullptr;
   A0.recy = rec ? rec + 3 * nv : nullptr;
   HIPCHK(launch_start_md(S, h, S->P.dt, A0));
-  if (rec) {
-    // a recomputed segment starts from a restored checkpoint without a valid list: every
-    // system rebuilds its rows in the first step
-    HIPCHK(launch_fill_int(S->mdFlag[1], 1, S->nsys, S->stream));
-  }
+  if (rec) HIPCHK(launch_fill_int(S->mdFlag[1], 1, S->nsys, S->stream));
   swap_pos(S);
   for (long i = 1; i <= steps; i++) {
     const bool more = i < steps;
@@ -531,8 +549,8 @@ int verlet_sequence(dpm_set_s* S, long steps, bool langevin, double* rec = nullp
 // loop draws at counter ctr[0] + k.
 int baoab_sequence(dpm_set_s* S, long steps) {
   const double h = 0.5 * S->P.dt;
-  HIPCHK(launch_fill_int(S->mdCount[0], 0, S->G.ncells, S->stream));
-  HIPCHK(launch_fill_int(S->mdFlag[1], 0, S->nsys, S->stream));
+  int rc = md_start(S);
+  if (rc) return rc;
   HIPCHK(launch_start_baoab(S, h, md_args(S, 0, 0, 0)));
   swap_pos(S);
   for (long i = 1; i <= steps; i++) {
@@ -544,20 +562,19 @@ int baoab_sequence(dpm_set_s* S, long steps) {
   return DPM_OK;
 }
 
-// Overdamped loop (Brownian::integrate 147-152, DrivenBrownian::integrate 182-187): the cells
-// and list validity of the current positions first, then per step [rebuild if flagged; F(x);
-// v = (F + noise xi) / gamma or F / gamma; x' = x + dt v; bin + displacement test of x'].
+// Overdamped loop (Brownian::integrate 147-152, DrivenBrownian::integrate 182-187): the cells of
+// the list positions and the list validity of the current positions first, then per step
+// [rebuild if flagged; F(x); v = (F + noise xi) / gamma or F / gamma; x' = x + dt v; displacement
+// test of x'].
 int brown_sequence(dpm_set_s* S, long steps, bool noisy) {
-  // step i has rotation index i - 1; step 1 rebuilds from exact cells of x_0 (C[2])
-  int rc = md_clear_slot(S, 2);
-  if (!rc) rc = md_clear_slot(S, 0);
+  // step i has rotation index i - 1: the prologue writes the flags F[0] and generations G[0]
+  int rc = md_start(S);
   if (rc) return rc;
-  HIPCHK(launch_fill_int(S->mdFlag[1], 0, S->nsys, S->stream));
-  HIPCHK(launch_bin_flag(S, S->mdCount[2], S->mdSlots[2], S->mdFlag[0]));
+  HIPCHK(launch_fill_int(S->mdFlag[0], 0, S->nsys, S->stream));
+  HIPCHK(launch_bin_flag(S, S->mdCount[0], S->mdSlots[0], S->mdFlag[0], S->mdGen[0]));
   for (long i = 1; i <= steps; i++) {
     const int mode = MODE_BROWN | MODE_START | (noisy ? MODE_LANGEVIN : 0);
     StepArgs A = md_args(S, i - 1, i - 1, i - 1);
-    A.staleCheck = i > 1;
     HIPCHK(launch_step(S, mode, A));
     swap_pos(S);
   }
@@ -917,8 +934,8 @@ int rigid_clear(dpm_set_s* S) {
 // time step, so a ring can move far in one step.
 int rigid_sequence(dpm_set_s* S, long steps, bool langevin) {
   if (!langevin) {
-    HIPCHK(launch_fill_int(S->mdCount[0], 0, S->G.ncells, S->stream));
-    HIPCHK(launch_fill_int(S->mdFlag[1], 0, S->nsys, S->stream));
+    int rc = md_start(S);
+    if (rc) return rc;
     HIPCHK(launch_rigid_start(S, false, md_args(S, 0, 0, 0)));
     swap_pos(S);
     for (long i = 1; i <= steps; i++) {
@@ -1164,9 +1181,8 @@ int dpm_destroy(dpm_set_t h) {
                   S->d_flag[1], S->d_any, S->d_updateCount, S->d_err, S->d_maxcnt, S->d_ctr, S->d_notDone,
                   S->d_fire[0], S->d_fire[1], S->d_part, S->d_red, S->d_scale, V.kr,
                   S->d_ck, S->d_fin, S->d_seg, S->d_adj, V.vinfo, V.l0r, V.mbx, V.mby, V.wfx, V.wfy,
-                  S->d_contacts, S->d_ccount, S->d_x0, S->d_obs, S->mdCount[0], S->mdCount[1], S->mdCount[2],
-                  S->mdSlots[0], S->mdSlots[1], S->mdSlots[2], S->mdFlag[0], S->mdFlag[1], S->mdFlag[2],
-                  S->mdFlag[3], S->R.pvx, S->R.pvy, S->R.pw, S->R.pfx, S->R.pfy, S->R.ptq, S->R.moi,
+                  S->d_contacts, S->d_ccount, S->d_x0, S->d_obs, S->mdCount[0], S->mdSlots[0], S->mdFlag[0],
+                  S->mdFlag[1], S->mdFlag[2], S->mdFlag[3], S->mdGen[0], S->mdGen[1], S->mdGen[2], S->R.pvx, S->R.pvy, S->R.pw, S->R.pfx, S->R.pfy, S->R.ptq, S->R.moi,
                   S->R.prad, S->R.plx, S->R.ply, S->R.pla, S->R.vtq, S->d_rfire[0], S->d_rfire[1], S->d_rg,
                   S->d_snap, S->d_ftape, S->d_ftapeState, S->d_fadj, S->d_wide};
   for (void* p : ptrs)
@@ -2189,11 +2205,10 @@ int dpm_time_step_kernel(dpm_set_t h, long reps, double* avgUs, double* meanNeig
   HIPCHK(dalloc(&svy, S->nv));
   HIPCHK(hipMemcpyAsync(svx, S->V.vx, sizeof(double) * S->nv, hipMemcpyDeviceToDevice, S->stream));
   HIPCHK(hipMemcpyAsync(svy, S->V.vy, sizeof(double) * S->nv, hipMemcpyDeviceToDevice, S->stream));
-  // the Langevin step kernel as launched inside loops (binning its drift destination through
-  // the rotating cell lists), with rebuild flags that stay clear: no rebuild, and the drift
-  // writes into the other position buffer only
+  // the Langevin step kernel as launched inside loops, with rebuild flags that stay clear: no
+  // rebuild, and the drift writes into the other position buffer only
   HIPCHK(launch_fill_int(S->mdFlag[3], 0, S->nsys, S->stream));
-  for (int r = 0; r < 3; r++) HIPCHK(launch_fill_int(S->mdCount[r], 0, S->G.ncells, S->stream));
+  for (int r = 0; r < 3; r++) HIPCHK(launch_fill_int(S->mdGen[r], 0, S->nsys, S->stream));
   auto args = [&](long i) {
     StepArgs A = md_args(S, i, 0, 0);
     A.flagCur = S->mdFlag[3];

@@ -29,7 +29,8 @@ enum : int { NOISE_LANGEVIN = 0, NOISE_BAOAB = 1, NOISE_BROWNIAN = 2, NOISE_DRIV
 enum : int { ERR_LIST_OVERFLOW = 1, ERR_CELL_OVERFLOW = 2, ERR_STALE_BINS = 4 };
 
 // rebuild-flag bits (per system): 1 = rebuild, FLAG_FAR = a vertex moved more than cutoff / 3 in
-// the drift that produced these positions
+// the drift that produced these positions, or (first positions of a step loop) lies more than
+// 2 cutoff / 3 from its list position: the cells of the list positions may miss its neighbours
 enum : int { FLAG_FAR = 2 };
 
 // per-system FIRE scalars (double buffered by iteration parity)
@@ -69,13 +70,13 @@ struct StepArgs {
   int* flagCur;                   // rebuild flags of this step's positions
   int* flagOut;                   // rebuild flags of the drift destination
   int* anyOut;                    // any-flag of the drift destination
-  int* cellCur;                   // cells the rows are rebuilt from (this step's positions, or
-  int* slotsCur;                  //   in the step loop the previous step's) and their slots
-  int* cellOther;                 // cell counts cleared here (the next binning's target)
+  int* cellCur;                   // cells the rows are rebuilt from (this step's positions) and
+  int* slotsCur;                  //   their slots; in the step loop the base of its three lists
+  int* cellOther;                 // cell counts cleared here (the next binning's target), or null
   int* flagClear;                 // rebuild flags cleared here, or null
-  int* cellOut;                   // step loop: this step's positions are binned into these, or null
-  int* slotsOut;
-  int staleCheck;                 // rows from the previous step's cells: check its displacement
+  int* genCur;                    // step loop: per-system index of the list-position cells (the
+  int* genOut;                    //   cells of the positions of the last rebuild), and its next value
+  int staleCheck;                 // rows from cells of earlier positions: check the displacement
   const unsigned long long* ctr;  // [0] thermal calls, [1] active calls
   long long ctrOffT, ctrOffA;     // call index of this step's draws = ctr[k] + offset
   FireState* fire;                // MODE_FIRE: per-system state of this iteration
@@ -138,7 +139,7 @@ struct dpm_set_s {
   std::string err;
   int split = 4;  // wavefronts per tile in the step kernel (1..8)
   int xcdMap = 1; // XCD-aware workgroup -> tile mapping in the step kernel
-  double cellScale = 1.0;  // cell side / (list radius + cutoff / 3), >= 1 (dpm_set_cell_scale)
+  double cellScale = 1.0;  // cell side / (list radius + 2 cutoff / 3), >= 1 (dpm_set_cell_scale)
 
   // host copies of topology
   std::vector<int> h_rnv, h_rfirst, h_rsys, h_vring, h_vsys, h_svoff, h_sroff;
@@ -160,12 +161,15 @@ struct dpm_set_s {
   // neighbour machinery
   dpm::CellGrid G{};
   int* d_flag[2] = {nullptr, nullptr};  // per-system rebuild flags, by position parity
-  // Step-loop cells and flags, rotated over three buffers so that a step kernel rebuilds from
-  // one, bins its drift destination into the next and clears the one after (no separate
-  // binning launch); mdFlag[3] is a sink for launches whose flags are not wanted.
+  // Step-loop cells: three cell lists in one allocation (mdCount[k] = mdCount[0] + k ncells,
+  // mdSlots[k] likewise), holding per system the cells of its list positions (the positions
+  // of its last rebuild) in list gen % 3; a rebuild reads list gen % 3, bins its positions into
+  // list gen + 1 and clears list gen + 2.  Flags and generations rotate over three buffers by
+  // step; mdFlag[3] is a sink for launches whose flags are not wanted.
   int* mdCount[3] = {nullptr, nullptr, nullptr};
   int* mdSlots[3] = {nullptr, nullptr, nullptr};
   int* mdFlag[4] = {nullptr, nullptr, nullptr, nullptr};
+  int* mdGen[3] = {nullptr, nullptr, nullptr};
   int* d_any = nullptr;                 // [2] any-flag by parity
   int* d_updateCount = nullptr;
   int* d_err = nullptr;
@@ -244,7 +248,7 @@ hipError_t launch_shape_only(dpm_set_s* S);
 hipError_t launch_step(dpm_set_s* S, int mode, const StepArgs& A);
 hipError_t launch_start_baoab(dpm_set_s* S, double h, const StepArgs& A);
 hipError_t launch_start_md(dpm_set_s* S, double h, double dt, const StepArgs& A);
-hipError_t launch_bin_flag(dpm_set_s* S, int* count, int* slots, int* flag);
+hipError_t launch_bin_flag(dpm_set_s* S, int* count, int* slots, int* flag, int* genOut);
 hipError_t launch_ctr_add(dpm_set_s* S, long long addT, long long addA);
 hipError_t launch_bin(dpm_set_s* S, int bumpThermal, int bumpActive);
 hipError_t launch_build_rows(dpm_set_s* S, int countUpdates);

@@ -14,11 +14,12 @@
 //   * wave 0 applies the integrator epilogue (thermostat, kicks, drift,
 //     displacement test).
 // Neighbour rows are rebuilt inside the step kernel when the system's rebuild
-// flag is set: the cell lists were filled by the preceding k_bin launch, and a
-// tile's rows (including next(v) of each of its vertices) are all built by the
-// same workgroup before its gather starts.  Rebuild flags and cell counts are
-// double buffered by position-buffer parity, so a step costs two launches
-// (k_bin, normally idle, and k_step) with no host synchronisation.
+// flag is set, and a tile's rows are all built by its own workgroup before its
+// gather starts.  In the step loops the cells searched are those of the list
+// positions (the positions of the system's last rebuild, binned by that rebuild:
+// md_cells), so a step is one launch with no binning and no host
+// synchronisation; the FIRE graph path bins the current positions in a k_bin
+// launch before each step kernel.
 #include "dpm_internal.h"
 #include "dpm_rows.h"
 #include "dpm_tile.h"
@@ -313,11 +314,37 @@ __device__ __forceinline__ void gather_pairs(const View& V, const Params& P, con
 // Waves of a tile: with nw > 1, wave 0 does the ring geometry, shape force and random draws
 // while waves 1..nw-1 share the neighbour gather; with nw == 1 wave 0 does both in turn.
 // A drift that moved a vertex more than cutoff / 3 marks its system FLAG_FAR: the next rebuild
-// cannot use cells of the positions before the drift
+// cannot rely on the cells of its list positions (md_cells)
 __device__ __forceinline__ void far_test(const Params& P, const Box& B, double dx, double dy, int* flag) {
   dx = pbx(dx, B);
   dy = pby(dy, B);
   if (3 * sqrt(dx * dx + dy * dy) > P.cutoff) atomicOr(flag, FLAG_FAR);
+}
+
+// The step loop's cell lists of a system whose list positions (the positions of its last
+// rebuild) have generation g: its rows are rebuilt from list g % 3, the rebuild bins its own
+// positions (the next list positions) into list (g + 1) % 3, cleared by the rebuild before, and
+// clears list (g + 2) % 3, which that rebuild read.  The cells are wide enough for a vertex 2
+// cutoff / 3 from its list position: at most cutoff / 3 before the drift that triggers the
+// rebuild (the displacement test) plus cutoff / 3 in that drift (else FLAG_FAR, all pairs).
+struct MdCells {
+  const int *cnt, *slots;  // read
+  int *binCnt, *binSlots;  // binned into
+  int* clear;              // cleared
+};
+__device__ __forceinline__ MdCells md_cells(const StepArgs& A, const CellGrid& G, int g) {
+  const long C = G.ncells, CS = (long)G.ncells * G.cap;
+  const int k0 = g % 3, k1 = (g + 1) % 3, k2 = (g + 2) % 3;
+  return MdCells{A.cellCur + k0 * C, A.slotsCur + k0 * CS, A.cellCur + k1 * C, A.slotsCur + k1 * CS,
+                 A.cellCur + k2 * C};
+}
+
+// a rebuilding vertex at (x, y) farther than 2 cutoff / 3 from its list position: its cells may
+// miss a neighbour (cannot happen while the displacement and far tests hold)
+__device__ __forceinline__ void md_stale_test(const View& V, const Params& P, const Box& B, int v, double x, double y,
+                                              int* err) {
+  const double ddx = pbx(x - V.lx[v], B), ddy = pby(y - V.ly[v], B);
+  if (3 * sqrt(ddx * ddx + ddy * ddy) > 2 * P.cutoff) atomicOr(err, ERR_STALE_BINS);
 }
 
 // ---------------------------------------------------------- rigid rings ---
@@ -457,40 +484,60 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
   Lane L;
   constexpr bool WIDE = (MODE & MODE_WIDE) != 0;
   constexpr int MB = MODE & ~MODE_WIDE;  // the integrator mode
+  // A step of a velocity-Verlet, Langevin or BAOAB loop other than its last consumes the force it
+  // computes in this launch (the closing and next opening half kicks); its force, energies, ring
+  // geometry and wall forces are stored by the loop's last step only.
+  constexpr bool STORE = (MODE & MODE_START) == 0 || (MODE & (MODE_BROWN | MODE_RIGID)) != 0;
   lane_ctx<WIDE>(V, A.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x, threadIdx.x & (WAVE - 1), L);
   const Box B = box_of(V, L.s);
   // own position first: the row rebuild below does not change it, so its load overlaps the
   // rebuild-flag load instead of waiting behind it
   const double x = V.x[L.v], y = V.y[L.v];
 
-  // loop bookkeeping: clear the other parity's cells, count this step's rebuild, rebuild rows
+  // loop bookkeeping: clear flags (and cells) for later steps, count this step's rebuild, rebuild rows
   if (A.manage) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    for (int c = gid; c < G.ncells; c += gridDim.x * blockDim.x) A.cellOther[c] = 0;
+    if (A.cellOther)
+      for (int c = gid; c < G.ncells; c += gridDim.x * blockDim.x) A.cellOther[c] = 0;
     if (A.flagClear)
       for (int t = gid; t < V.nsys; t += gridDim.x * blockDim.x) A.flagClear[t] = 0;
-    // the step loop bins these positions for the next step's rebuild, on the last wave
-    if (A.cellOut && w == nw - 1 && L.act) bin_vertex(G, B, A.cellOut, A.slotsOut, L.v, L.s, x, y, A.err);
     const int flag = A.flagCur[L.s];
     const bool rebuild = (flag & 1) != 0;
-    // step loop: rows come from the previous positions' cells, complete while no vertex moved
-    // more than cutoff / 3 in the last step (the margin the cells are widened by); a system
-    // whose previous drift moved a vertex farther carries FLAG_FAR and searches all its
-    // vertices instead (the reference's all-pairs search)
+    // a system flagged FLAG_FAR searches all its vertices instead of cells that may miss a
+    // neighbour (the reference's all-pairs search)
     const bool allPairs = A.staleCheck && (flag & FLAG_FAR) != 0;
-    if (rebuild && L.tile == V.stile[L.s] && threadIdx.x == 0) A.updateCount[L.s] += 1;
-    if (rebuild && w == 0 && L.act && !A.rowsBuilt && A.staleCheck && !allPairs) {
-      // the previous position is this lane's own, in the buffer the drift below overwrites
-      const double ddx = pbx(x - A.xout[L.v], B), ddy = pby(y - A.yout[L.v], B);
-      if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) atomicOr(A.err, ERR_STALE_BINS);
+    const int* cells = A.cellCur;
+    const int* slots = A.slotsCur;
+    if (A.genCur) {
+      // step loop: rows from the cells of the list positions (md_cells)
+      const int g = A.genCur[L.s];
+      if (L.tile == V.stile[L.s] && threadIdx.x == 0) A.genOut[L.s] = rebuild ? g + 1 : g;
+      if (rebuild) {
+        const MdCells M = md_cells(A, G, g);
+        cells = M.cnt;
+        slots = M.slots;
+        if (!A.rowsBuilt) {
+          if (w == 0 && L.act && !allPairs) md_stale_test(V, P, B, L.v, x, y, A.err);
+          // these positions are the system's next list positions (its rows are built from them)
+          if (w == nw - 1 && L.act) bin_vertex(G, B, M.binCnt, M.binSlots, L.v, L.s, x, y, A.err);
+          if (!G.brute[L.s]) {
+            const int t0 = V.stile[L.s], nt = V.stile[L.s + 1] - t0;
+            const int n = G.ncx[L.s] * G.ncy[L.s];
+            int* clr = M.clear + G.coff[L.s];
+            for (int c = (L.tile - t0) * (int)blockDim.x + (int)threadIdx.x; c < n; c += nt * (int)blockDim.x)
+              clr[c] = 0;
+          }
+        }
+      }
     }
+    if (rebuild && L.tile == V.stile[L.s] && threadIdx.x == 0) A.updateCount[L.s] += 1;
     // the cell search shared by the tile's waves (rows in LDS) where the launch sized LDS for it
     const bool tileBuild = rebuild && !A.rowsBuilt && !allPairs && A.rowsLdsW == V.W && !G.brute[L.s];
     if (tileBuild) {
-      build_rows_tile(V, P, G, A.cellCur, A.slotsCur, L.lane, L.v, L.r, L.s, L.act, w, nw, B, x, y, (int*)red,
-                      A.err, A.maxcnt);
+      build_rows_tile(V, P, G, cells, slots, L.lane, L.v, L.r, L.s, L.act, w, nw, B, x, y, (int*)red, A.err,
+                      A.maxcnt);
     } else if (rebuild && w == 0 && L.act && !A.rowsBuilt) {
-      build_row(V, P, G, A.cellCur, A.slotsCur, L.v, L.s, B, A.err, A.maxcnt, allPairs);
+      build_row(V, P, G, cells, slots, L.v, L.s, B, A.err, A.maxcnt, allPairs);
     }
     if (rebuild) __syncthreads();
   }
@@ -533,7 +580,7 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
   double zt0 = 0, zt1 = 0, ang = 0;
   if (w == 0) {
     if constexpr ((MODE & MODE_RIGID) != 0) so = ring_centre(V, L, B, x, y);
-    else so = ring_shape(V, P, L, B, x, y, true);
+    else so = ring_shape(V, P, L, B, x, y, STORE);
     // one thermal draw per integrate call and ring: Langevin::updateThermalVel (Simulator.cu:54-77)
     // and Brownian::updateThermalVel (156-179) after this call's force, counter = call index;
     // BAOAB::updateThermalVel (122-144) in the next call's first half, whose index is one more
@@ -548,7 +595,7 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
     if (P.geometry != GEOM_NORMAL) {  // walls, after the interactions (DPM2D.cu:2090-2103)
       double wfx, wfy;
       so.e += wall_forces(P, B, x, y, V.rad[L.v], so.fx, so.fy, wfx, wfy);
-      if (L.act) {
+      if (STORE && L.act) {
         V.wfx[L.v] = wfx;
         V.wfy[L.v] = wfy;
       }
@@ -596,10 +643,12 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
     const bool fromMemory = L.wide && L.lane == L.segE;  // next(v) in another tile
     fx += fromMemory ? qnx : sx;
     fy += fromMemory ? qny : sy;
-    const double pe = seg_sum(L.act ? es : 0.0, L.lane, L.segS, L.segE);
-    if (L.act) {
-      V.es[L.v] = es;
-      if (L.lane == L.segS && !L.wide) V.pe[L.r] = pe;  // wide rings: k_wide_pe sums es
+    if constexpr (STORE) {
+      const double pe = seg_sum(L.act ? es : 0.0, L.lane, L.segS, L.segE);
+      if (L.act) {
+        V.es[L.v] = es;
+        if (L.lane == L.segS && !L.wide) V.pe[L.r] = pe;  // wide rings: k_wide_pe sums es
+      }
     }
   }
   if constexpr ((MODE & MODE_RIGID) != 0) {
@@ -639,9 +688,11 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
     return;
   }
   if (!L.act) return;
-  V.fx[L.v] = fx;
-  V.fy[L.v] = fy;
-  V.ev[L.v] = e;
+  if constexpr (STORE) {
+    V.fx[L.v] = fx;
+    V.fy[L.v] = fy;
+    V.ev[L.v] = e;
+  }
   if constexpr (MB == MODE_FORCE) return;
   if constexpr ((MODE & MODE_BROWN) != 0) {
     // overdamped velocity (Brownian: (F + noise xi) / gamma; driven: F / gamma), then the drift
@@ -799,11 +850,23 @@ __global__ void k_rows_loop(View V, Params P, StepArgs A, CellGrid G) {
   if ((flag & 1) == 0) return;
   const Box B = box_of(V, s);
   const bool allPairs = A.staleCheck && (flag & FLAG_FAR) != 0;
-  if (A.staleCheck && !allPairs) {
-    const double ddx = pbx(V.x[v] - A.xout[v], B), ddy = pby(V.y[v] - A.yout[v], B);
-    if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) atomicOr(A.err, ERR_STALE_BINS);
+  const int* cnt = A.cellCur;  // FIRE: cells of these positions
+  const int* slots = A.slotsCur;
+  if (A.genCur) {
+    // step loop: the cells of the list positions (md_cells); the step kernel after this launch
+    // advances the system's generation
+    const MdCells M = md_cells(A, G, A.genCur[s]);
+    const double x = V.x[v], y = V.y[v];
+    if (!allPairs) md_stale_test(V, P, B, v, x, y, A.err);
+    bin_vertex(G, B, M.binCnt, M.binSlots, v, s, x, y, A.err);
+    if (!G.brute[s]) {
+      const int n = G.ncx[s] * G.ncy[s];
+      for (int c = v - V.svoff[s]; c < n; c += V.svoff[s + 1] - V.svoff[s]) M.clear[G.coff[s] + c] = 0;
+    }
+    cnt = M.cnt;
+    slots = M.slots;
   }
-  build_row(V, P, G, A.cellCur, A.slotsCur, v, s, B, A.err, A.maxcnt, allPairs);
+  build_row(V, P, G, cnt, slots, v, s, B, A.err, A.maxcnt, allPairs);
 }
 
 // geometry only (calcParticleShape + calcParticlePositions)
@@ -816,16 +879,33 @@ __global__ void k_shape(View V, Params P) {
   ring_shape(V, P, L, box_of(V, L.s), V.x[L.v], V.y[L.v], true);
 }
 
+// Prologue of a step loop's cell protocol for vertex v of system s at positions (x, y): its list
+// position binned into list 0 (generation 0), and the rebuild flag of (x, y) (the displacement
+// test, with FLAG_FAR where (x, y) lies more than 2 cutoff / 3 from the list position, beyond what
+// the cells allow for)
+__device__ __forceinline__ void md_prologue(const View& V, const Params& P, const CellGrid& G, const StepArgs& A,
+                                            const Box& B, int v, int s, double x, double y, int* flag, int* any) {
+  const double lx = V.lx[v], ly = V.ly[v];
+  bin_vertex(G, B, A.cellCur, A.slotsCur, v, s, lx, ly, A.err);
+  const double ddx = pbx(x - lx, B), ddy = pby(y - ly, B);
+  const double d3 = 3 * sqrt(ddx * ddx + ddy * ddy);
+  if (d3 > P.cutoff) {
+    atomicOr(flag, d3 > 2 * P.cutoff ? 1 | FLAG_FAR : 1);
+    if (any) atomicOr(any, 1);
+  }
+}
+
 // The step loop's first half step (velocity Verlet, or BAOAB's B A O A with the draw of call
-// ctr[0] + ctrOffT) from the stored force, with the loop's cell protocol: the start positions
-// are binned into A.cellOut / A.slotsOut (the first step rebuilds from them), the destination
-// is flagged into A.flagOut, and A.cellOther / A.flagClear are cleared (BAOAB::integrate,
-// Simulator.cu:112-120, up to its force evaluation).
+// ctr[0] + ctrOffT) from the stored force, with the loop's cell protocol (md_prologue on the
+// destination, whose flags go to A.flagOut; generation 0 into A.genOut; A.flagClear cleared)
+// (BAOAB::integrate, Simulator.cu:112-120, up to its force evaluation).
 template <bool BAOAB>
 __global__ void k_start_loop(View V, Params P, CellGrid G, double h, double dt, StepArgs A) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  for (int c = v; c < G.ncells; c += gridDim.x * blockDim.x) A.cellOther[c] = 0;
-  for (int t = v; t < V.nsys; t += gridDim.x * blockDim.x) A.flagClear[t] = 0;
+  for (int t = v; t < V.nsys; t += gridDim.x * blockDim.x) {
+    A.flagClear[t] = 0;
+    A.genOut[t] = 0;
+  }
   if (v >= V.nv) return;
   const int s = V.vsys[v];
   const Box B = box_of(V, s);
@@ -853,26 +933,25 @@ __global__ void k_start_loop(View V, Params P, CellGrid G, double h, double dt, 
     A.recx[v] = xn;
     A.recy[v] = yn;
   }
-  bin_vertex(G, B, A.cellOut, A.slotsOut, v, s, V.x[v], V.y[v], A.err);  // the start positions' cells
-  const double ddx = pbx(xn - V.lx[v], B), ddy = pby(yn - V.ly[v], B);
-  if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) {
-    atomicOr(&A.flagOut[s], 1);
-    atomicOr(A.anyOut, 1);
-  }
+  md_prologue(V, P, G, A, B, v, s, xn, yn, A.flagOut + s, A.anyOut);
   far_test(P, B, xn - V.x[v], yn - V.y[v], A.flagOut + s);
 }
 
 // The rigid loop's first half step from the stored ring force and torque, one thread per ring
 // (RigidNVE / RigidLangevin::integrate, Simulator.cu:241-252, 316-326, up to the vertex move):
 // last centre and angle stored, half kick with the draw of call ctr[0] + ctrOffT, centre and
-// angle advanced, and each vertex moved into A.xout and tested into A.flagOut (binned at its
-// start position into A.cellOut when given); A.cellOther (and A.flagClear when given) cleared.
+// angle advanced, and each vertex moved into A.xout and tested into A.flagOut; in the step loop
+// (A.genCur) with its cell protocol (md_prologue, generation 0 into A.genOut), else A.cellOther
+// cleared; A.flagClear cleared when given.
 __global__ void k_rigid_start(View V, Params P, CellGrid G, StepArgs A) {
   const Rigid R = *V.rg;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  for (int c = t; c < G.ncells; c += gridDim.x * blockDim.x) A.cellOther[c] = 0;
+  if (A.cellOther)
+    for (int c = t; c < G.ncells; c += gridDim.x * blockDim.x) A.cellOther[c] = 0;
   if (A.flagClear)
     for (int q = t; q < V.nsys; q += gridDim.x * blockDim.x) A.flagClear[q] = 0;
+  if (A.genCur)
+    for (int q = t; q < V.nsys; q += gridDim.x * blockDim.x) A.genOut[q] = 0;
   if (t >= V.np) return;
   const int r = t, s = V.rsys[r];
   const Box B = box_of(V, s);
@@ -888,13 +967,16 @@ __global__ void k_rigid_start(View V, Params P, CellGrid G, StepArgs A) {
   bool flag = false;
   for (int v = V.rfirst[r]; v < V.rfirst[r] + V.rnv[r]; v++) {
     const double x = V.x[v], y = V.y[v];
-    if (A.cellOut) bin_vertex(G, B, A.cellOut, A.slotsOut, v, s, x, y, A.err);
     double xn, yn;
     rigid_move(B, x, y, cx, cy, ncx, ncy, c, sn, xn, yn);
     A.xout[v] = xn;
     A.yout[v] = yn;
-    const double ddx = pbx(xn - V.lx[v], B), ddy = pby(yn - V.ly[v], B);
-    flag = flag || 3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff;
+    if (A.genCur) {
+      md_prologue(V, P, G, A, B, v, s, xn, yn, A.flagOut + s, nullptr);
+    } else {
+      const double ddx = pbx(xn - V.lx[v], B), ddy = pby(yn - V.ly[v], B);
+      flag = flag || 3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff;
+    }
     far_test(P, B, xn - x, yn - y, A.flagOut + s);
   }
   if (flag) {
@@ -950,17 +1032,14 @@ __global__ __launch_bounds__(256) void k_rigid_inject(View V, Params P, double a
   }
 }
 
-// the current positions into a cell list with their displacement test (the overdamped loops'
-// prologue: their first step rebuilds from these when flagged)
-__global__ void k_bin_flag(View V, Params P, CellGrid G, int* count, int* slots, int* flag, int* err) {
+// The overdamped loops' prologue: md_prologue on the current positions (their first step
+// rebuilds when flagged), flags into `flag`, generation 0 into `genOut`.
+__global__ void k_bin_flag(View V, Params P, CellGrid G, StepArgs A, int* flag) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < V.nsys) A.genOut[v] = 0;
   if (v >= V.nv) return;
   const int s = V.vsys[v];
-  const Box B = box_of(V, s);
-  const double x = V.x[v], y = V.y[v];
-  bin_vertex(G, B, count, slots, v, s, x, y, err);
-  const double ddx = pbx(x - V.lx[v], B), ddy = pby(y - V.ly[v], B);
-  if (3 * sqrt(ddx * ddx + ddy * ddy) > P.cutoff) atomicOr(&flag[s], 1);
+  md_prologue(V, P, G, A, box_of(V, s), v, s, V.x[v], V.y[v], flag + s, nullptr);
 }
 
 // RNG call counters advanced by the draws of a finished loop
@@ -1678,8 +1757,7 @@ hipError_t launch_step(dpm_set_s* S, int mode, const StepArgs& A0) {
   StepArgs A = A0;
   const bool smooth = S->P.interaction == INT_SMOOTH;
   if (smooth && A.manage) {
-    if (!A.flagCur || !A.cellCur || !A.err || !A.maxcnt || (A.staleCheck && (!A.xout || !A.yout)))
-      return hipErrorInvalidValue;
+    if (!A.flagCur || !A.cellCur || !A.slotsCur || !A.err || !A.maxcnt) return hipErrorInvalidValue;
     k_rows_loop<<<grid_v(S->nv), 256, 0, S->stream>>>(S->V, S->P, A, S->G);
     A.rowsBuilt = 1;
   }
@@ -1701,7 +1779,8 @@ static hipError_t launch_step_kernel(dpm_set_s* S, int mode, const StepArgs& A0)
   if ((mode & MODE_FIRE) && (!A.fire || !A.part)) return hipErrorInvalidValue;
   if ((mode & MODE_RIGID) && S->P.interaction == INT_ALL) return hipErrorInvalidValue;
   if ((mode & MODE_START) && (!A.xout || !A.yout || !A.flagOut || !A.anyOut)) return hipErrorInvalidValue;
-  if (A.manage && (!A.flagCur || !A.cellCur || !A.cellOther || !A.err || !A.maxcnt || !A.updateCount))
+  if (A.manage && (!A.flagCur || !A.cellCur || !A.slotsCur || (A.genCur && !A.genOut) || !A.err || !A.maxcnt ||
+                   !A.updateCount))
     return hipErrorInvalidValue;
   const bool wide = S->nwide > 0;
   switch (S->P.interaction) {
@@ -1726,19 +1805,27 @@ hipError_t launch_shape_only(dpm_set_s* S) {
 }
 
 hipError_t launch_start_baoab(dpm_set_s* S, double h, const StepArgs& A) {
+  if (!A.genOut || !A.flagClear || !A.flagOut || !A.anyOut || !A.cellCur || !A.slotsCur || !A.ctr)
+    return hipErrorInvalidValue;
   k_start_loop<true><<<grid_v(S->nv > S->G.ncells ? S->nv : S->G.ncells), 256, 0, S->stream>>>(S->V, S->P, S->G, h,
                                                                                            S->P.dt, A);
   return hipGetLastError();
 }
 
 hipError_t launch_start_md(dpm_set_s* S, double h, double dt, const StepArgs& A) {
+  if (!A.genOut || !A.flagClear || !A.flagOut || !A.anyOut || !A.cellCur || !A.slotsCur) return hipErrorInvalidValue;
   k_start_loop<false><<<grid_v(S->nv > S->G.ncells ? S->nv : S->G.ncells), 256, 0, S->stream>>>(S->V, S->P, S->G, h,
                                                                                             dt, A);
   return hipGetLastError();
 }
 
-hipError_t launch_bin_flag(dpm_set_s* S, int* count, int* slots, int* flag) {
-  k_bin_flag<<<grid_v(S->nv), 256, 0, S->stream>>>(S->V, S->P, S->G, count, slots, flag, S->d_err);
+hipError_t launch_bin_flag(dpm_set_s* S, int* count, int* slots, int* flag, int* genOut) {
+  StepArgs A{};
+  A.cellCur = count;
+  A.slotsCur = slots;
+  A.genOut = genOut;
+  A.err = S->d_err;
+  k_bin_flag<<<grid_v(S->nv > S->nsys ? S->nv : S->nsys), 256, 0, S->stream>>>(S->V, S->P, S->G, A, flag);
   return hipGetLastError();
 }
 
@@ -1836,7 +1923,8 @@ hipError_t launch_scale_box(dpm_set_s* S) {
 }
 
 hipError_t launch_rigid_start(dpm_set_s* S, bool langevin, const StepArgs& A) {
-  if (!A.ctr || (A.cellOut && !A.slotsOut) || !A.cellOther || !A.flagOut || !A.anyOut || !A.xout || !A.yout)
+  if (!A.ctr || (A.genCur && (!A.genOut || !A.cellCur || !A.slotsCur)) || (!A.genCur && !A.cellOther) ||
+      !A.flagOut || !A.anyOut || !A.xout || !A.yout)
     return hipErrorInvalidValue;
   StepArgs B = A;
   B.rigidLangevin = langevin ? 1 : 0;

@@ -145,7 +145,7 @@ int dpm_set_displacement_cutoff(dpm_set_t h, double cutoff, double size, double*
  * the list holds the same sets as the reference's all-pairs build) */
 int dpm_calc_neighbor_list(dpm_set_t h, double cutDistance);
 /* Cell side of the neighbour search as a multiple (>= 1) of its smallest value, the list radius
- * plus the cutoff / 3 a vertex may move between binning and rebuild; a tuning knob (the
+ * plus the 2 cutoff / 3 a vertex may move between binning and rebuild; a tuning knob (the
  * reference builds its lists without cells, DPM2D.cu:3479-3507, so no counterpart).  The rows
  * are the same sets for every scale. */
 int dpm_set_cell_scale(dpm_set_t h, double scale);
