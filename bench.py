@@ -34,11 +34,11 @@ DT_UNITS = 1e-3                # dt = 1e-3 * sigma / sqrt(ec), sigma = mean part
 CHUNK = 250                    # steps per graph replay
 # HBM bytes per launch of the step kernel on this workload, from the FETCH_SIZE and WRITE_SIZE
 # counter runs of `bench.py --traffic` (each scaled by its reading for a 1 GiB device copy:
-# 2.000 and 1.000): 2 x 3381.8 KiB + 2249.1 KiB, at mean neighbour count 4.432.  The summary
+# 2.000 and 1.000): 2 x 3381.19 KiB + 608.66 KiB, at mean neighbour count 4.421.  The summary
 # with its commands is profiles/k_step_traffic_nvt1024_wca_smooth.md; re-measure after changing
 # the step kernel.
-TRAFFIC_BYTES_PER_LAUNCH = 9229005
-TRAFFIC_MEAN_NEIGHBORS = 4.432
+TRAFFIC_BYTES_PER_LAUNCH = 7547936
+TRAFFIC_MEAN_NEIGHBORS = 4.421
 
 
 def alg_bytes_per_step(nv, np_, kbar):
@@ -647,18 +647,17 @@ def rigid_bench(s, dt, steps):
 COPY_ELEMS = 1 << 27  # float64 elements of the calibration copy: 1 GiB read + 1 GiB written
 
 
-def traffic_run(s, torch, args):
-    """Launch sequence for `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE`: three copies of
-    COPY_ELEMS doubles (the calibration kernel: 8 * COPY_ELEMS bytes read and written each), the
-    benchmark's warm-up and timed steps, then 50 back-to-back launches of the step kernel on the
-    state the roofline figure is measured on."""
+def traffic_run(s, torch):
+    """Launches for `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE`, after the benchmark's own
+    warm-up and timed steps: three copies of COPY_ELEMS doubles (the calibration kernel:
+    8 * COPY_ELEMS bytes read and written each), then 50 back-to-back launches of the step kernel
+    on the state the roofline figure is measured on."""
     src = torch.rand(COPY_ELEMS, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
     for _ in range(3):
         dst.copy_(src)
     torch.cuda.synchronize()
     del src, dst
-    run_steps(s, args.warmup)
     us, kbar = s.time_step_kernel(50)
     s.synchronize()
     print(json.dumps({"traffic_run": True, "copy_bytes_each_way": 8 * COPY_ELEMS, "step_launches": 50,
@@ -698,7 +697,8 @@ def main():
     ap.add_argument("--rigid", action="store_true",
                     help="the benchmark packing as rigid rings: rigid NVE, Langevin and FIRE throughput")
     ap.add_argument("--traffic", action="store_true",
-                    help="counter-collection run: a copy of known size, then step-kernel launches only")
+                    help="counter-collection run: the line's warm-up and timed steps, then a copy of known "
+                         "size and step-kernel launches only")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the engine measures one GPU's share (batched systems "
@@ -725,9 +725,6 @@ def main():
     run_steps(s, args.equil - args.equil // 2)
     s.synchronize()
     equil_rebuilds = int(s.update_count()[0])
-    if args.traffic:
-        traffic_run(s, torch, args)
-        return
     if args.integrators:
         print(json.dumps({"integrators": integrators_bench(s, dt, 5000)}), flush=True)
         return
@@ -751,6 +748,9 @@ def main():
     temp = float(s.temperature()[0])
     if not np.isfinite(temp):
         raise RuntimeError("non-finite temperature after the timed run")
+    if args.traffic:
+        traffic_run(s, torch)
+        return
 
     value = nv * args.steps / el
     us, kbar = s.time_step_kernel(200)

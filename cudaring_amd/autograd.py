@@ -1,7 +1,8 @@
 """torch.autograd over the adjoint entry points of libdpmring.so.
 
-`DPMAdjoint` wraps one DPMSet (vertexVertex interaction, harmonic monomers, passive rings) and
-exposes two differentiable functions of torch float64 tensors:
+`DPMAdjoint` wraps one DPMSet (vertexVertex or vertexSmooth interaction, harmonic monomers,
+passive or self-propelled rings) and exposes two differentiable functions of torch float64
+tensors:
 
   trajectory(x0, v0, ka, kl, kb, l0, steps, every) -> (xN, vN)
       `steps` NVE steps (NVELoop, DPM2D.h:686) from (x0, v0).  The backward pass is the library's
@@ -122,6 +123,11 @@ class _Trajectory(torch.autograd.Function):
     def backward(ctx, gx, gv):
         s = ctx.adj.s
         if s.tape_serial != ctx.tape:  # the set's tape is another trajectory's: record this one again
+            if getattr(s, "particle", "passive") != "passive":
+                # self-propelled rings: the angles and random draws the recorded trajectory started
+                # from have moved on, so it cannot be recorded again
+                raise RuntimeError("DPMAdjoint: with self-propelled rings only the last trajectory of a set can "
+                                   "be differentiated (its tape was replaced by a later forward)")
             ctx.tape = _Trajectory._record(ctx.adj, *ctx.inputs, ctx.steps, ctx.every)
         nv, np_ = s.nv, s.np
         dev = torch.device("cuda")

@@ -706,12 +706,18 @@ int clean_protocol(dpm_set_s* S) {
   return DPM_OK;
 }
 
-int adjoint_supported(dpm_set_s* S) {
+// Self-propelled rings (`active`: the Hessian-vector product, the energy's parameter gradient and
+// the NVE reverse pass) add a force that does not depend on positions or parameters; their angles
+// advance by per-step draws, which the NVE tape's checkpoints carry (save_state).  FIRE's reverse
+// pass takes passive rings only.
+int adjoint_supported(dpm_set_s* S, bool active = false) {
   if (S->nwide > 0) return fail(S, DPM_ERR_UNSUPPORTED, "adjoint: rings of at most 64 vertices only");
   if (S->P.interaction != INT_VV && S->P.interaction != INT_SMOOTH)
     return fail(S, DPM_ERR_UNSUPPORTED, "adjoint: vertexVertex or vertexSmooth interaction only");
   if (S->P.monomer != MONO_HARMONIC) return fail(S, DPM_ERR_UNSUPPORTED, "adjoint: harmonic monomers only");
-  if (S->P.particle != PART_PASSIVE) return fail(S, DPM_ERR_UNSUPPORTED, "adjoint: passive particles only");
+  if (S->P.particle != PART_PASSIVE && !(active && S->P.particle == PART_ACTIVE))
+    return fail(S, DPM_ERR_UNSUPPORTED, active ? "adjoint: passive or self-propelled (active) particles only"
+                                               : "adjoint: passive particles only");
   if (S->P.geometry != GEOM_NORMAL) return fail(S, DPM_ERR_UNSUPPORTED, "adjoint: periodic geometry only");
   return require_list(S);
 }
@@ -749,17 +755,28 @@ int hvp_launches(dpm_set_s* S, const View& Vn, int p, AdjArgs A) {
   return DPM_OK;
 }
 
-// state copies: x, y (current buffer), vx, vy, fx, fy <-> 6 nv doubles at `dst` / `src`
-int save_state(dpm_set_s* S, double* dst) {
+// state copies: x, y (current buffer), vx, vy, fx, fy <-> 6 nv doubles at `dst` / `src`; with
+// `rings`, then the ring angles (np doubles) and the RNG call counters (4 words of 8 bytes, copied
+// bit for bit), which a recomputed segment of self-propelled rings replays its draws from
+long state_words(const dpm_set_s* S, bool rings) { return 6 * S->nv + (rings ? S->np + 4 : 0); }
+int save_state(dpm_set_s* S, double* dst, bool rings = false) {
   const long nv = S->nv;
   const double* src[6] = {S->V.x, S->V.y, S->V.vx, S->V.vy, S->V.fx, S->V.fy};
   for (int q = 0; q < 6; q++) HIPCHK(launch_copy_d(dst + q * nv, src[q], nv, S->stream));
+  if (rings) {
+    HIPCHK(launch_copy_d(dst + 6 * nv, S->V.ang, S->np, S->stream));
+    HIPCHK(launch_copy_d(dst + 6 * nv + S->np, (const double*)S->d_ctr, 4, S->stream));
+  }
   return DPM_OK;
 }
-int load_state(dpm_set_s* S, const double* src) {
+int load_state(dpm_set_s* S, const double* src, bool rings = false) {
   const long nv = S->nv;
   double* dst[6] = {S->V.x, S->V.y, S->V.vx, S->V.vy, S->V.fx, S->V.fy};
   for (int q = 0; q < 6; q++) HIPCHK(launch_copy_d(dst[q], src + q * nv, nv, S->stream));
+  if (rings) {
+    HIPCHK(launch_copy_d(S->V.ang, src + 6 * nv, S->np, S->stream));
+    HIPCHK(launch_copy_d((double*)S->d_ctr, src + 6 * nv + S->np, 4, S->stream));
+  }
   return DPM_OK;
 }
 
@@ -2248,7 +2265,7 @@ int dpm_time_step_kernel(dpm_set_t h, long reps, double* avgUs, double* meanNeig
 int dpm_force_hvp(dpm_set_t h, const double* ux, const double* uy, double* outx, double* outy) {
   CHECK_HANDLE(h);
   if (!ux || !uy || !outx || !outy) return fail(S, DPM_ERR_ARG, "dpm_force_hvp: null vector");
-  int rc = adjoint_supported(S);
+  int rc = adjoint_supported(S, true);
   if (rc) return rc;
   rc = adj_alloc(S);
   if (rc) return rc;
@@ -2279,7 +2296,7 @@ int dpm_force_hvp(dpm_set_t h, const double* ux, const double* uy, double* outx,
 
 int dpm_energy_param_grad(dpm_set_t h, double* gring, double* gl0) {
   CHECK_HANDLE(h);
-  int rc = adjoint_supported(S);
+  int rc = adjoint_supported(S, true);
   if (rc) return rc;
   rc = adj_alloc(S);
   if (rc) return rc;
@@ -2298,7 +2315,7 @@ int dpm_energy_param_grad_noalloc(dpm_set_t h, double* gring, double* gl0) {
     return fail(S, DPM_ERR_ARG, "dpm_energy_param_grad_noalloc: the set has " + std::to_string(S->nv) +
                                     " vertices, the scratch buffer holds " +
                                     std::to_string((long)DPM_NOALLOC_MAX_VERTICES));
-  int rc = adjoint_supported(S);
+  int rc = adjoint_supported(S, true);
   if (rc) return rc;
   double* tmp = nullptr;
   HIPCHK(scratch_address(S->device, &tmp));
@@ -2313,30 +2330,33 @@ int dpm_energy_param_grad_noalloc(dpm_set_t h, double* gring, double* gl0) {
 int dpm_nve_forward(dpm_set_t h, long steps, long every) {
   CHECK_HANDLE(h);
   if (steps < 1 || every < 1) return fail(S, DPM_ERR_ARG, "dpm_nve_forward: steps and every must be >= 1");
-  int rc = adjoint_supported(S);
+  int rc = adjoint_supported(S, true);
   if (rc) return rc;
   const long nv = S->nv;
   const long nck = (steps + every - 1) / every;
-  if (nck > S->tapeNck || every > S->tapeEvery || !S->d_fin) {
+  const bool rings = S->P.particle == PART_ACTIVE;
+  const long words = state_words(S, rings);
+  if (nck > S->tapeNck || every > S->tapeEvery || !S->d_fin || words != S->ckWords) {
     HIPCHK(hipStreamSynchronize(S->stream));
     hipFree(S->d_ck);
     hipFree(S->d_seg);
     hipFree(S->d_fin);
     S->d_ck = S->d_seg = S->d_fin = nullptr;
-    HIPCHK(dalloc(&S->d_ck, nck * 6 * nv));
+    HIPCHK(dalloc(&S->d_ck, nck * words));
     HIPCHK(dalloc(&S->d_seg, (every + 1) * 2 * nv));
-    HIPCHK(dalloc(&S->d_fin, 6 * nv));
+    HIPCHK(dalloc(&S->d_fin, words));
+    S->ckWords = words;
     drop_graphs(S);
   }
   S->tapeSteps = 0;
   for (long k = 0; k < nck; k++) {
     const long len = std::min(every, steps - k * every);
-    rc = save_state(S, S->d_ck + k * 6 * nv);
+    rc = save_state(S, S->d_ck + k * words, rings);
     if (rc) return rc;
     rc = run_verlet(S, len, false);
     if (rc) return rc;
   }
-  rc = save_state(S, S->d_fin);
+  rc = save_state(S, S->d_fin, rings);
   if (rc) return rc;
   rc = check_err_after_loop(S);
   if (rc) return rc;
@@ -2481,11 +2501,13 @@ int dpm_nve_backward(dpm_set_t h, const double* ax, const double* av, double* gx
   if (S->tapeSteps < 1) return fail(S, DPM_ERR_STATE, "dpm_nve_backward: no trajectory recorded");
   if (memcmp(&S->P, &S->tapeP, sizeof(Params)) != 0 || S->paramSerial != S->tapeSerial)
     return fail(S, DPM_ERR_STATE, "dpm_nve_backward: parameters changed since dpm_nve_forward");
-  int rc = adjoint_supported(S);
+  int rc = adjoint_supported(S, true);
   if (rc) return rc;
   rc = adj_alloc(S);
   if (rc) return rc;
   const long nv = S->nv, N = S->tapeSteps, every = S->tapeEvery;
+  const bool rings = S->P.particle == PART_ACTIVE;
+  const long words = S->ckWords;
   S->wCur = 0;
   HIPCHK(launch_copy_d(adj_w(S, 0), av, 2 * nv, S->stream));
   HIPCHK(launch_copy_d(adj_ax(S), ax, 2 * nv, S->stream));
@@ -2496,7 +2518,7 @@ int dpm_nve_backward(dpm_set_t h, const double* ax, const double* av, double* gx
     const long a = k * every, b = std::min(a + every, N);
     // restore the checkpoint, recompute the segment (its first step rebuilds every row), then
     // the reverse pass over it; no host synchronisation until the end
-    rc = load_state(S, S->d_ck + k * 6 * nv);
+    rc = load_state(S, S->d_ck + k * words, rings);
     if (rc) return rc;
     rc = clean_protocol(S);
     if (rc) return rc;
@@ -2518,7 +2540,7 @@ int dpm_nve_backward(dpm_set_t h, const double* ax, const double* av, double* gx
   if (gring) HIPCHK(launch_ring_sum3(S, adj_g(S), gring));
   if (gl0) HIPCHK(launch_copy_d(gl0, adj_g(S) + 3 * nv, nv, S->stream));
   // leave the set in the state the forward pass ended in
-  rc = load_state(S, S->d_fin);
+  rc = load_state(S, S->d_fin, rings);
   if (rc) return rc;
   rc = rebuild(S, all, 0);
   if (rc) return rc;

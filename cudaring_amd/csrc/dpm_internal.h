@@ -215,8 +215,10 @@ struct dpm_set_s {
   dpm::Params tapeP{};        // parameters the trajectory was recorded with
   long tapeSerial = -1;
   long paramSerial = 0;       // bumped by set_ring_costs and by set_field of rad, l0, theta0, a0, box
-  double* d_ck = nullptr;     // checkpoints: tapeNck x 6 x nv (x, y, vx, vy, fx, fy)
-  double* d_fin = nullptr;    // final state 6 x nv
+  double* d_ck = nullptr;     // checkpoints: tapeNck x ckWords (x, y, vx, vy, fx, fy; self-propelled
+                              //   rings: then ring angles and RNG call counters)
+  long ckWords = 0;           // doubles per checkpoint of d_ck and d_fin
+  double* d_fin = nullptr;    // final state, ckWords doubles
   double* d_seg = nullptr;    // one segment's positions: (tapeEvery + 1) x 2 x nv
   double* d_adj = nullptr;    // work: w[2] (2nv each), ax (2nv), partials (4nv), ring area and dA (2np)
   long adjNv = 0;

@@ -230,6 +230,7 @@ class DPMSet:
                   particle="passive"):
         self._chk(self.L.dpm_set_types(self.h, POT[potential], INTER[interaction], MONO[monomer],
                                        PART[particle]), "set_types")
+        self.particle = particle
 
     def set_energy_costs(self, ea, el, eb, ec):
         self._chk(self.L.dpm_set_energy_costs(self.h, ea, el, eb, ec), "set_energy_costs")

@@ -310,9 +310,11 @@ int dpm_set_launch_config(dpm_set_t h, int wavesPerTile);
 int dpm_time_step_kernel(dpm_set_t h, long reps, double* avgMicroseconds, double* meanNeighbors);
 
 /* ---- adjoint (no reference counterpart: differentiation is an addition) ---- */
-/* Supported for the vertexVertex and vertexSmooth interactions with harmonic monomers and
- * passive particles in a periodic box, all four potentials (vertexSmooth: exact on the branch
- * the positions select, the interaction being piecewise smooth).  Vectors are device arrays owned
+/* Supported for the vertexVertex and vertexSmooth interactions with harmonic monomers in a
+ * periodic box, all four potentials (vertexSmooth: exact on the branch the positions select, the
+ * interaction being piecewise smooth); passive particles, and for every entry point but the FIRE
+ * pair also self-propelled ones (active: their driving force depends on neither positions nor
+ * parameters).  Vectors are device arrays owned
  * by the caller and used on the handle's stream; a "2V SoA" vector holds the V x components
  * followed by the V y components. */
 /* Hessian-vector product of the potential energy at the current positions: out = H u
@@ -332,7 +334,8 @@ int dpm_energy_param_grad(dpm_set_t h, double* gring, double* gl0);
 #define DPM_NOALLOC_MAX_VERTICES 262144
 int dpm_energy_param_grad_noalloc(dpm_set_t h, double* gring, double* gl0);
 /* NVELoop() x steps (DPM2D.h:686) from the current state, keeping a checkpoint of positions,
- * velocities and forces every `every` steps for dpm_nve_backward. */
+ * velocities and forces (self-propelled rings: also ring angles and RNG call counters, so that the
+ * recomputed segments draw the same angle increments) every `every` steps for dpm_nve_backward. */
 int dpm_nve_forward(dpm_set_t h, long steps, long every);
 /* Reverse pass of the last dpm_nve_forward: given ax = dL/dx_N and av = dL/dv_N (2V SoA), returns
  * gx0 = dL/dx_0, gv0 = dL/dv_0 (2V SoA), gring = dL/d(ka, kl, kb) per ring (3R) and

@@ -399,6 +399,99 @@ def test_cfg5_active_adhesive_step_for_step(oracle_mod):
     g.close()
 
 
+def test_cfg5_adjoint_through_active_adhesive_nve(oracle_mod):
+    """The reverse pass on the active + adhesive vertex-segment model (16 rings x 32 vertices):
+    40 NVE steps with self-propulsion, whose ring angles advance by one draw per step, recorded
+    with checkpoints every 15 steps (each recomputed segment restarts from its checkpoint's angles
+    and RNG counters); gradients of L = c . x_N + d . v_N with respect to x0, v0 and per-ring ka,
+    kl, kb against the oracle's central differences.  The self-propulsion force depends on
+    neither positions nor parameters, so it enters the gradients through the trajectory only."""
+    print("case cfg5 adjoint, 16 rings x 32 vertices, 40 active NVE steps", flush=True)
+    torch = pytest.importorskip("torch")
+    st = active_state(16, 20261202 + 7)
+    nr, nv = len(st["nvp"]), int(np.sum(st["nvp"]))
+    size = 2 * float(np.mean(st["rad"]))
+    sigma = float(2 * np.sqrt(np.mean(st["a0"]) / np.pi))
+    dt, steps, every = 1e-3 * sigma, 40, 15
+    rng = np.random.default_rng(8)
+    angles = rng.uniform(-np.pi, np.pi, nr)
+    v0 = 1e-3 * rng.standard_normal(2 * nv)
+    ka, kl, kb = (c * (0.8 + 0.4 * rng.random(nr)) for c in ACT_COSTS[:3])
+
+    def setup(obj, x0, v, ka_, kl_, kb_, oracle):
+        obj.set_types("adhesive", "vertexSmooth", "harmonic", "active")
+        obj.set_attraction(0.1, 0.2)
+        obj.set_energy_costs(*ACT_COSTS)
+        obj.set_ring_costs(ka_, kl_, kb_)
+        obj.set_self_propulsion(0.1 / sigma, 100.0 * sigma)
+        obj.set_dt(dt)
+        if oracle:
+            obj.set_rng(9, 0)
+        else:
+            obj.set_seed(9)
+        obj.set("pangle", angles)
+        obj.set("pos", x0)
+        obj.set("vel", v)
+        obj.init_nve(0.0, read_state=True)
+        cut = obj.set_displacement_cutoff(0.5, size)
+        obj.calc_neighbor_list(cut)
+        obj.calc_force_energy()
+
+    g = gpu_from_state(st)
+    setup(g, st["pos"], v0, ka, kl, kb, False)
+    g.nve_forward(steps, every)
+    xg, vg, ag = g.get("pos"), g.get("vel"), g.get("pangle")
+    o = oracle_from_state(oracle_mod, st)
+
+    def final(x0=st["pos"], v=v0, ka_=ka, kl_=kl, kb_=kb):
+        setup(o, x0, v, ka_, kl_, kb_, True)
+        o.nve(steps)
+        return o.get("pos"), o.get("vel")
+
+    xo, vo = final()
+    assert rel_err(xg, xo, scale=1.0) < 1e-11
+    assert rel_err(ag, o.get("pangle"), scale=np.pi) < 1e-13
+    cx, cv = rng.standard_normal(2 * nv), rng.standard_normal(2 * nv)
+
+    def soa(a):
+        return torch.as_tensor(np.concatenate([a[0::2], a[1::2]]), device="cuda")
+
+    def aos(t):
+        a = t.cpu().numpy()
+        return np.stack([a[:nv], a[nv:]], 1).ravel()
+
+    axt, avt = soa(cx), soa(cv)
+    gx, gv = (torch.empty(2 * nv, dtype=torch.float64, device="cuda") for _ in range(2))
+    gring = torch.empty(3 * nr, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    g.nve_backward(axt.data_ptr(), avt.data_ptr(), gx.data_ptr(), gv.data_ptr(), gring.data_ptr(), 0)
+    gx, gv, gring = aos(gx), aos(gv), gring.cpu().numpy()
+    # the set is left where the forward pass ended, angles included
+    np.testing.assert_array_equal(g.get("pos"), xg)
+    np.testing.assert_array_equal(g.get("pangle"), ag)
+
+    def L(**kw):
+        x, v = final(**kw)
+        return cx @ x + cv @ v
+
+    d = rng.standard_normal(2 * nv)
+    # contacts opening or closing across a difference step sit on the adhesive force's kinks:
+    # positions are perturbed by 1e-8 (as for the harmonic configuration above)
+    fd = (L(x0=st["pos"] + 1e-8 * d) - L(x0=st["pos"] - 1e-8 * d)) / 2e-8
+    assert abs(gx @ d - fd) <= 2e-5 * abs(fd), (gx @ d, fd)
+    fd = (L(v=v0 + 1e-8 * d) - L(v=v0 - 1e-8 * d)) / 2e-8
+    assert abs(gv @ d - fd) <= 2e-5 * abs(fd), (gv @ d, fd)
+    p = 5
+    for q, (name, base) in enumerate((("ka_", ka), ("kl_", kl), ("kb_", kb))):
+        h = 1e-6 * abs(base[p])
+        up, dn = base.copy(), base.copy()
+        up[p] += h
+        dn[p] -= h
+        fd = (L(**{name: up}) - L(**{name: dn})) / (2 * h)
+        assert abs(gring[q * nr + p] - fd) <= 1e-4 * abs(fd) + 1e-9, (name, gring[q * nr + p], fd)
+    g.close()
+
+
 def test_cfg5_full_size_force_balance():
     print("case cfg5 full size, 16384 rings x 32 vertices", flush=True)
     st = active_state(16384, 20261202 + 5, dphi=0.1, iters=300)
