@@ -33,12 +33,12 @@ SKIN = 0.5                     # setDisplacementCutoff(0.5, 2 <r_v>), runNVT.cpp
 DT_UNITS = 1e-3                # dt = 1e-3 * sigma / sqrt(ec), sigma = mean particle size
 CHUNK = 250                    # steps per graph replay
 # HBM bytes per launch of the step kernel on this workload, from the FETCH_SIZE and WRITE_SIZE
-# counter runs of `bench.py --traffic` (each scaled by its reading for a 1 GiB device copy:
-# 2.000 and 1.000): 2 x 3381.19 KiB + 608.66 KiB, at mean neighbour count 4.421.  The summary
-# with its commands is profiles/k_step_traffic_nvt1024_wca_smooth.md; re-measure after changing
-# the step kernel.
-TRAFFIC_BYTES_PER_LAUNCH = 7547936
-TRAFFIC_MEAN_NEIGHBORS = 4.421
+# counter runs of `bench.py --traffic` (the line's own warm-up and timed steps, so the same state
+# and mean neighbour count 4.443; each counter scaled by its reading for a 1 GiB device copy:
+# 2.000 and 1.000): 2 x 3834.32 KiB + 608.66 KiB.  The summary with its commands is
+# profiles/k_step_traffic_nvt1024_wca_smooth.md; re-measure after changing the step kernel.
+TRAFFIC_BYTES_PER_LAUNCH = 8475959
+TRAFFIC_MEAN_NEIGHBORS = 4.443
 
 
 def alg_bytes_per_step(nv, np_, kbar):
