@@ -565,8 +565,10 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
     double* r = red + (w - 1) * nq * WAVE + L.lane;
     r[0 * WAVE] = fx;
     r[1 * WAVE] = fy;
-    r[2 * WAVE] = e;
-    r[3 * WAVE] = es;
+    if constexpr (STORE) {  // energies are outputs only: a step that stores none leaves them out
+      r[2 * WAVE] = e;
+      r[3 * WAVE] = es;
+    }
     r[4 * WAVE] = qx;
     r[5 * WAVE] = qy;
     if (nq == 8) {
@@ -626,8 +628,10 @@ __global__ __launch_bounds__(WAVE* MAXSPLIT) void k_step(View V, Params P, StepA
       const double* r = red + (q - 1) * nq * WAVE + L.lane;
       fx += r[0 * WAVE];
       fy += r[1 * WAVE];
-      e += r[2 * WAVE];
-      es += r[3 * WAVE];
+      if constexpr (STORE) {
+        e += r[2 * WAVE];
+        es += r[3 * WAVE];
+      }
       qx += r[4 * WAVE];
       qy += r[5 * WAVE];
       if (nq == 8) {
