@@ -302,6 +302,13 @@ def test_nvt_workload_full_size(oracle_mod):
         o.langevin(chunk); g.langevin(chunk)
         assert rel_err(g.get("pos"), o.get("pos"), scale=1.0) < 1e-11, chunk
         assert rel_err(g.get("vel"), o.get("vel")) < 1e-9, chunk
+        # only a loop's last step stores the force, the energies and the ring geometry: those of
+        # the final positions
+        assert rel_err(g.get("force"), o.get("force")) < 1e-9, chunk
+        assert rel_err(g.get("energy"), o.get("energy")) < 1e-9, chunk
+        assert rel_err(g.get("penergy"), o.get("penergy")) < 1e-9, chunk
+        for name in ("area", "perimeter", "theta", "length", "ppos"):
+            assert rel_err(g.get(name), o.get(name), scale=1.0) < 1e-10, (name, chunk)
     assert g.update_count()[0] == o.update_count() >= 1
     g.close()
 
