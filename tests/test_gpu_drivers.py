@@ -291,3 +291,30 @@ def test_make_dpm_minimised_packing(tmp_path, oracle_mod):
     o.calc_neighbor_list(2.0)
     o.calc_force_energy()
     assert o.total_force_magnitude() < 1e-8  # the files keep 14 digits
+
+
+def test_compress_fire_reference_command_line(tmp_path, oracle_mod):
+    """compressFIRE with the reference's command line (the output directory alone; here with
+    smaller optional sizes, a denser soft-disk start so that the thermalisations run, caps, and an
+    area cost of 100: with the reference's 1e3 the ring-area mode is unstable at FIRE's largest
+    step, in the oracle as on the device): soft disks and their rings saved, then the jamming
+    search's steps, each a FIRE-minimised packing saved under its packing fraction, deltaPhi = 2e-3
+    above the last (the Langevin thermalisation between them changes the ring areas a little)."""
+    out = str(tmp_path / "cf") + "/"
+    r = run([os.path.join(BIN, "compressFIRE"), out, "seed=3", "numParticles=16", "numVertexPerParticle=16",
+             "phi0=0.3", "maxSearchStep=3", "maxStep=200", "costs=100,1,0,1"])
+    assert "Thermalization" in r and r.count("Found underjammed configuration") == 3
+    for d in ("sp", "dp"):
+        assert os.path.isfile(os.path.join(out, d, "boxSize.dat"))
+    phis = sorted(float(d) for d in os.listdir(out) if d not in ("sp", "dp"))
+    assert len(phis) == 3
+    assert np.all(np.diff(phis) > 1e-3) and np.all(np.diff(phis) < 3e-3), phis
+    for phi in phis:
+        st = state_from_dir(os.path.join(out, f"{phi:.6f}"))
+        o = oracle_mod.Oracle(st["nvp"])
+        o.load_state(st)
+        o.set_types("harmonic", "vertexVertex")
+        o.set_energy_costs(100, 1, 0, 1)
+        o.calc_neighbor_list(1.0)
+        o.calc_force_energy()
+        assert o.max_unbalanced_force() < 1e-7, phi  # FIRE stopped at 1e-10 / sigma; 14-digit files
