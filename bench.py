@@ -163,8 +163,9 @@ def nve_small_bench(args):
     """The plain NVE configuration: 32 bidisperse rings (16/22 vertices, 608 vertices), harmonic
     vertex-vertex, phi 0.80, uniform random centres, per-ring Gaussian velocities at T 1e-3 with
     the mean removed, dt = 0.01 sigma / sqrt(ec), skin 0.3; 1e4 velocity-Verlet steps on the oracle
-    (one thread and the process's CPU share; median of 3 after 1 warm-up) and on the GPU."""
-    from cudaring_amd import DPMSet, packing
+    (one thread and the process's CPU share; median of 3 after 1 warm-up) and on the GPU, with
+    the waves per tile tuned on the loop (untimed chunks of it) first."""
+    from cudaring_amd import DPMSet, autotune, packing
     from oracle import oracle
     st = packing.random_system(32, 0.80, CFG1_SEED)
     nv = int(np.sum(st["nvp"]))
@@ -198,7 +199,10 @@ def nve_small_bench(args):
     g.set_seed(CFG1_SEED)
     setup(g)
     chunk = CFG1_STEPS // 10
-    g.nve(chunk)  # captures the graph of a chunk, which the timed calls replay
+    # the list rebuilds about every third step here, so the launch shape is tuned on the loop itself
+    # (chunks of it; the graph of a chunk is captured for the chosen shape, which the timed calls replay)
+    waves, _ = autotune.tune_waves_per_tile_loop(g, lambda: g.nve(chunk))
+    g.nve(chunk)  # setting the chosen shape dropped the graphs: capture the chunk's graph again
     g.synchronize()
     t0 = time.perf_counter()
     for _ in range(CFG1_STEPS // chunk):
@@ -208,7 +212,8 @@ def nve_small_bench(args):
     g.close()
     n = cpu_threads()
     out = {"workload": "NVE, 32 rings 16/22 (608 vertices), harmonic vertexVertex, phi 0.80, T 1e-3, "
-                       "dt 0.01 sigma, 1e4 steps", "gpu_vertex_steps_per_s": round(gpu, 1)}
+                       "dt 0.01 sigma, 1e4 steps", "gpu_vertex_steps_per_s": round(gpu, 1),
+           "waves_per_tile": waves}
     if not args.no_cpu:
         out.update({"cpu_vertex_steps_per_s_1_thread": round(cpu_rate(1), 1),
                     f"cpu_vertex_steps_per_s_{n}_threads": round(cpu_rate(n), 1)})

@@ -5,7 +5,8 @@ tile's neighbour gather (1..8) trades latency hiding against idle lanes and LDS
 traffic, and the best value depends on the neighbour count and on how many
 tiles the set has.  `tune_waves_per_tile` times the fused Langevin step kernel
 for each candidate on the caller's own state (velocities are restored by the
-library after timing) and keeps the fastest.  `tune_skin` picks the neighbour
+library after timing) and keeps the fastest; `tune_waves_per_tile_loop` times a
+stretch of the caller's own loop instead, rebuilds included.  `tune_skin` picks the neighbour
 skin (setDisplacementCutoff's `cutoff`) that minimises the measured time of a
 short stretch of the real loop, rebuilds included; `tune_fire_neighbors` picks the skin and the
 cell size of the batched FIRE minimisation the same way.
@@ -21,6 +22,29 @@ def tune_waves_per_tile(s, candidates=(1, 2, 4, 8), reps=100):
     for w in candidates:
         s.set_launch_config(w)
         times[w] = min(s.time_step_kernel(reps)[0] for _ in range(2))
+    best = min(times, key=times.get)
+    s.set_launch_config(best)
+    return best, times
+
+
+def tune_waves_per_tile_loop(s, run, candidates=(2, 4, 8), reps=2):
+    """Pick the waves per tile minimising the wall time of `run()` (a stretch of the caller's own
+    loop, rebuilds included: a set whose list rebuilds every few steps spends much of its time in
+    the row build, which more waves share).  The state is advanced while tuning, so call it on a
+    warm-up stretch of the run."""
+    times = {}
+    for w in candidates:
+        s.set_launch_config(w)
+        run()  # capture the graph for this configuration
+        s.synchronize()
+        best = None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            run()
+            s.synchronize()
+            el = time.perf_counter() - t0
+            best = el if best is None else min(best, el)
+        times[w] = best
     best = min(times, key=times.get)
     s.set_launch_config(best)
     return best, times

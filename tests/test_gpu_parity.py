@@ -192,6 +192,24 @@ def test_split_invariance(oracle_mod, inter):
         assert rel_err(g.get("force"), ref) < TOL_F, split
 
 
+def test_loop_tuner_keeps_the_trajectory(oracle_mod):
+    """The loop-timed waves-per-tile tuner runs stretches of the caller's own NVE loop under each
+    candidate shape (a shape change re-captures the loop's graph): the trajectory it leaves is the
+    oracle's for the same number of steps, and the chosen shape is the one set."""
+    from cudaring_amd import autotune
+    print("case loop_tuner", flush=True)
+    st = packing.lattice_system(16, 0.8, 17)
+    o, g = prepare(oracle_mod, st, "harmonic", "vertexVertex", dt=0.005)
+    o.calc_force_energy(); g.calc_force_energy()
+    o.init_nve(1e-2, zero_momentum=True); g.init_nve(1e-2, zero_momentum=True)
+    best, times = autotune.tune_waves_per_tile_loop(g, lambda: g.nve(20), candidates=(2, 4, 8), reps=2)
+    assert best in (2, 4, 8) and sorted(times) == [2, 4, 8]
+    o.nve(3 * 3 * 20)
+    assert rel_err(g.get("pos"), o.get("pos"), scale=1.0) < 1e-11
+    assert rel_err(g.get("vel"), o.get("vel")) < 1e-9
+    assert list(g.update_count()) == [o.update_count()]
+
+
 @pytest.mark.parametrize("persistent", [True, False])
 def test_batched_fire_minimize(oracle_mod, persistent):
     print(f"case batched_fire_minimize persistent={persistent}", flush=True)
