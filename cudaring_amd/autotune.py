@@ -71,8 +71,8 @@ def tune_skin(s, size, candidates=(0.2, 0.3, 0.5, 0.8), steps=400, kind="langevi
     return best, times
 
 
-def tune_fire(s, params7, cut, candidates=((1, 2), (1, 4), (1, 8), (0, 0)), iters=200):
-    """Pick the fire_minimize kernel (persistent workgroup per system with 2/4/8 waves, or graphs
+def tune_fire(s, params7, cut, candidates=((1, 2), (1, 4), (1, 8), (1, 16), (0, 0)), iters=200):
+    """Pick the fire_minimize kernel (persistent workgroup per system with 2/4/8/16 waves, or graphs
     of the split step kernels) by timing `iters` iterations of each from the caller's state,
     which is restored after every trial and at the end."""
     keep = {k: s.get(k) for k in ("pos", "vel")}

@@ -1840,7 +1840,7 @@ int dpm_fire_minimize(dpm_set_t h, double tol, long maxIter, long* iterations) {
     if (waves <= 0) {
       int maxTiles = 1;
       for (long s = 0; s < S->nsys; s++) maxTiles = std::max(maxTiles, S->h_stile[s + 1] - S->h_stile[s]);
-      waves = std::min(8, maxTiles);
+      waves = std::min(16, maxTiles);
     }
     const long pchunk = 2048;
     while (done <= maxIter) {
@@ -1883,8 +1883,8 @@ int dpm_fire_minimize(dpm_set_t h, double tol, long maxIter, long* iterations) {
 
 int dpm_set_fire_kernel(dpm_set_t h, int persistent, int wavesPerSystem) {
   CHECK_HANDLE(h);
-  if (persistent < 0 || persistent > 1 || wavesPerSystem < 0 || wavesPerSystem > 8)
-    return fail(S, DPM_ERR_ARG, "dpm_set_fire_kernel: persistent in {0, 1}, waves in 0..8");
+  if (persistent < 0 || persistent > 1 || wavesPerSystem < 0 || wavesPerSystem > 16)
+    return fail(S, DPM_ERR_ARG, "dpm_set_fire_kernel: persistent in {0, 1}, waves in 0..16");
   if (persistent && S->nwide > 0)
     return fail(S, DPM_ERR_UNSUPPORTED, "one-workgroup-per-system FIRE: rings of at most 64 vertices only");
   S->fireKernel = persistent;

@@ -1456,7 +1456,11 @@ __global__ void k_fire_resume(FireState* fs, int nsys) {
 // Each wave walks the system's ring-aligned tiles; the force phase is the step kernel's
 // (ring_shape + gather_pairs).  At most `chunk` iterations per launch; the loop exit is
 // decided by thread 0 and read by every thread after a barrier, so all waves leave together.
+// Two builds: up to 8 waves per system (about 190 VGPRs, no spills) and up to 16 (the 1024-lane
+// workgroup caps a wave at 128 VGPRs, so 40-75 of them spill to scratch; halving the tiles each
+// wave walks still makes an iteration about 12% shorter on systems of 76 tiles).
 #define FS_WAVES 8
+#define FS_WAVES_WIDE 16
 
 // workgroup sum of a and maximum of b (fixed order: lanes by butterfly, waves by index)
 __device__ __forceinline__ void block_sum_max(double a, double b, double (*red)[2], double& sa, double& mb) {
@@ -1493,11 +1497,11 @@ __device__ __forceinline__ void fire_power(const View& V, int t0, int t1, double
   block_sum_max(p, m, red, P, Fm);
 }
 
-template <int INTER, int POT>
-__global__ __launch_bounds__(WAVE* FS_WAVES) void k_fire_sys(View V, Params P, FireParams fp, CellGrid G,
-                                                             FireState* fst, int chunk, int* updateCount, int* err,
-                                                             int* maxcnt) {
-  __shared__ double red[FS_WAVES][2];
+template <int INTER, int POT, int MAXW>
+__global__ __launch_bounds__(WAVE* MAXW) void k_fire_sys(View V, Params P, FireParams fp, CellGrid G,
+                                                         FireState* fst, int chunk, int* updateCount, int* err,
+                                                         int* maxcnt) {
+  __shared__ double red[MAXW][2];
   __shared__ FireState sfs;
   __shared__ int shStop, shReset;
   const int s = blockIdx.x;
@@ -1982,8 +1986,12 @@ hipError_t launch_fire_partials(dpm_set_s* S) {
 
 template <int INTER, int POT>
 static hipError_t fire_sys_t(dpm_set_s* S, int chunk, int threads) {
-  k_fire_sys<INTER, POT><<<(unsigned)S->nsys, threads, 0, S->stream>>>(
-      S->V, S->P, S->fp, S->G, S->d_fire[S->fireCur], chunk, S->d_updateCount, S->d_err, S->d_maxcnt);
+  if (threads > WAVE * FS_WAVES)
+    k_fire_sys<INTER, POT, FS_WAVES_WIDE><<<(unsigned)S->nsys, threads, 0, S->stream>>>(
+        S->V, S->P, S->fp, S->G, S->d_fire[S->fireCur], chunk, S->d_updateCount, S->d_err, S->d_maxcnt);
+  else
+    k_fire_sys<INTER, POT, FS_WAVES><<<(unsigned)S->nsys, threads, 0, S->stream>>>(
+        S->V, S->P, S->fp, S->G, S->d_fire[S->fireCur], chunk, S->d_updateCount, S->d_err, S->d_maxcnt);
   return hipGetLastError();
 }
 
@@ -1998,7 +2006,7 @@ static hipError_t fire_sys_pot(dpm_set_s* S, int chunk, int threads) {
 }
 
 hipError_t launch_fire_sys(dpm_set_s* S, int chunk, int waves) {
-  const int threads = WAVE * (waves < 1 ? 1 : (waves > FS_WAVES ? FS_WAVES : waves));
+  const int threads = WAVE * (waves < 1 ? 1 : (waves > FS_WAVES_WIDE ? FS_WAVES_WIDE : waves));
   switch (S->P.interaction) {
     case INT_VV: return fire_sys_pot<INT_VV>(S, chunk, threads);
     case INT_SMOOTH: return fire_sys_pot<INT_SMOOTH>(S, chunk, threads);

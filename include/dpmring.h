@@ -238,9 +238,9 @@ int dpm_init_fire(dpm_set_t h, const double* params7, long minStep, long numStep
 /* vertexFIRELoop() x iterations, DPM2D.h:669 / FIRE.cu:397-411 (all systems advance) */
 int dpm_fire_loop(dpm_set_t h, long iterations);
 /* Kernel of dpm_fire_minimize: persistent = 1 (default) runs one workgroup per system that
- * iterates with workgroup barriers only (wavesPerSystem 1..8, 0 = from the largest system);
- * persistent = 0 replays graphs of the split FIRE step kernels with per-system freezing.
- * No reference counterpart (launch configuration, chosen by the autotuner). */
+ * iterates with workgroup barriers only (wavesPerSystem 1..16, 0 = one per tile of the largest
+ * system up to 16); persistent = 0 replays graphs of the split FIRE step kernels with per-system
+ * freezing.  No reference counterpart (launch configuration, chosen by the autotuner). */
 int dpm_set_fire_kernel(dpm_set_t h, int persistent, int wavesPerSystem);
 /* FIRE state per system: out[5*s + {0..4}] = fire_dt, a, numStep, last P = v.F, iterations */
 int dpm_get_fire_state(dpm_set_t h, double* out);

@@ -265,14 +265,15 @@ def test_cfg1_nve_full_length(oracle_mod):
     assert np.all(np.isfinite(g.get("pos")))
 
 
+@pytest.mark.parametrize("waves", [0, 16])
 @pytest.mark.parametrize("pot,inter", [("harmonic", "vertexVertex"), ("wca", "vertexSmooth")])
-def test_persistent_fire_matches_step_kernels(pot, inter):
+def test_persistent_fire_matches_step_kernels(pot, inter, waves):
     """fire_minimize on one workgroup per system (tol 0, 30 iterations) against 30 iterations of
     the split FIRE step kernels, on a batched set with rebuilds along the way.  The two paths add
     the pair forces in different orders; at dt_max the stiff WCA trajectory amplifies that
     roundoff about 1e3-fold per 10 iterations (1e-16 at iteration 20, 7e-13 in the velocities at
     30, 2e-9 at 40), so the positions are compared at 30 iterations."""
-    print(f"case persistent_fire {pot} {inter}", flush=True)
+    print(f"case persistent_fire {pot} {inter} waves={waves}", flush=True)
     systems = [relaxed_state(n, 0.8, 500 + n, final=(pot, inter)) for n in (9, 16)]
     cat = packing.concat_systems(systems)
     sets = []
@@ -286,7 +287,7 @@ def test_persistent_fire_matches_step_kernels(pot, inter):
         g.init_fire([0.2, 0.5, 1.1, 0.99, dt, 10 * dt, 0.2], 20, 0)
         sets.append(g)
     a, b = sets
-    a.set_fire_kernel(True)
+    a.set_fire_kernel(True, waves)  # 0: one wave per tile of the larger system; 16: the 1024-lane build
     it = a.fire_minimize(0.0, 30)
     b.fire(30)
     assert list(it) == [30, 30]
