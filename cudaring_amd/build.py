@@ -26,7 +26,7 @@ DRIVERS = os.path.join(HERE, "drivers")
 BIN = os.path.join(HERE, "bin")
 HOST_SOURCES = ["DPM2D.cpp", "FIRE.cpp", "Simulator.cpp"]
 DRIVER_SOURCES = ["compressFIRE.cpp", "makeDPM.cpp", "compressNVE.cpp", "runDPM.cpp", "runRigid.cpp", "testInteraction.cpp",
-                  "runNVE.cpp", "runNVT.cpp", "runActive.cpp"]
+                  "runNVE.cpp", "runNVT.cpp", "runActive.cpp", "runActiveShape.cpp"]
 CXX = os.environ.get("CXX", "g++")
 CXXFLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-variable"]
 

@@ -1,6 +1,6 @@
-// run_common.h -- the directory plan and the step loop shared by the runNVE, runNVT and runActive
-// drivers, whose command lines and output layout are those of runNVE.cpp, runNVT.cpp and
-// runActive.cpp in farceri/cudaRing:
+// run_common.h -- the directory plan and the step loop shared by the runNVE, runNVT, runActive and
+// runActiveShape drivers, whose command lines and output layout are those of runNVE.cpp,
+// runNVT.cpp, runActive.cpp and runActiveShape.cpp in farceri/cudaRing:
 //   * the mode argument: "change" reads the packing and its state and writes a new directory
 //     beside it, "over" runs again in the sample directory, "run" / "runlog" write the trajectory
 //     to its dynamics/ or dynamics-log/ subdirectory, anything else starts a new sample directory;

@@ -272,6 +272,40 @@ def test_run_nve_and_active_reference_command_lines(packing_dir, tmp_path):
     assert os.path.exists(act + "t0/contacts.dat") and os.path.exists(act + "t0/positions.dat")
 
 
+def test_run_active_shape_reference_command_line(packing_dir, tmp_path, oracle_mod):
+    """runActiveShape with runActiveShape.cpp's argument list: the sample directory
+    shape/l0Tau<l0Tau>-l0Diff<l0Diff>/ inside the input directory, the saved velocities read, and
+    the trajectory and fluctuating rest lengths of the oracle's activeShape Langevin run with the
+    same seed and the driver's units (l0Tau in sigma / sqrt(ec), l0Diff in sigma^2 per that)."""
+    import shutil
+    pack = str(tmp_path / "pack") + "/"
+    shutil.copytree(packing_dir, pack)
+    steps, dt_units, T, damping, l0Tau, l0Diff, seed = 200, 2e-4, 1e-3, 1.0, 0.5, 1e-4, 9
+    run([os.path.join(BIN, "runActiveShape"), pack, str(dt_units), str(T), str(damping), "0.5", "1e-4", str(steps),
+         "0", "16", "wca", "langevin", "default", str(seed)])
+    out = pack + "shape/l0Tau0.5-l0Diff1e-4/"
+    e = load(out + "energy.dat", 8)
+    assert e.shape[0] == steps and int(e[-1, 0]) == steps
+    st = state_from_dir(packing_dir)
+    st["vel"] = load(os.path.join(packing_dir, "velocities.dat"), 2).ravel()
+    sigma = 2 * np.sqrt(np.mean(st["a0"]) / np.pi)
+    o = oracle_mod.Oracle(st["nvp"])
+    o.load_state(st)
+    o.set_types("wca", "vertexSmooth", "harmonic", "activeShape")
+    o.set_energy_costs(1e5, 20, 10, 1)
+    o.set_dt(dt_units * sigma)
+    o.set_rng(seed, 0)
+    o.set_active_tension(l0Tau * sigma, l0Diff * sigma)  # time unit sigma, diffusion unit sigma^2 / sigma
+    o.init_langevin(T, damping / sigma, read_state=True)
+    cut = o.set_displacement_cutoff(0.5, 2 * np.mean(st["rad"]))
+    o.calc_neighbor_list(cut)
+    o.calc_force_energy()
+    o.langevin(steps)
+    np.testing.assert_allclose(load(out + "positions.dat", 2).ravel(), o.get("pos"), rtol=0, atol=1e-10)
+    np.testing.assert_allclose(load(out + "restLengths.dat"), o.get("l0"), rtol=1e-10)
+    assert np.any(load(out + "restLengths.dat") != st["l0"])
+
+
 def test_make_dpm_minimised_packing(tmp_path, oracle_mod):
     """makeDPM (makeDPM.cpp): sinusoidal rest angles with thetaK = 0 are the regular polygon's
     2 pi / n, and vertex FIRE stops once the total force magnitude sqrt(sum F^2 / (2 V))
