@@ -1117,7 +1117,7 @@ int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPer
   A(V.stile, numSystems + 1);
   A(S->G.ncx, numSystems); A(S->G.ncy, numSystems); A(S->G.coff, numSystems); A(S->G.brute, numSystems);
   A(S->d_flag[0], numSystems); A(S->d_flag[1], numSystems); A(S->d_any, 3); A(S->d_updateCount, numSystems);
-  A(S->d_err, 1); A(S->d_maxcnt, 1); A(S->d_ctr, 4); A(S->d_notDone, 1);
+  A(S->d_err, 1); A(S->d_maxcnt, 1); A(S->d_ctr, 4); A(S->d_notDone, 1); A(S->d_sysOrder, numSystems);
   A(S->d_fire[0], numSystems); A(S->d_fire[1], numSystems);
   A(S->d_part, (long)V.ntiles * 4);
   S->nwide = (long)S->h_wide.size();
@@ -1139,6 +1139,14 @@ int dpm_create(dpm_set_t* out, int device, long numSystems, const long* ringsPer
             up(V.tile_v1, S->h_tile_v1, st) == hipSuccess && up(V.tile_sys, S->h_tile_sys, st) == hipSuccess &&
             up(V.stile, S->h_stile, st) == hipSuccess &&
             (S->nwide == 0 || up(S->d_wide, S->h_wide, st) == hipSuccess);
+  // the persistent FIRE kernel's workgroup b runs system order[b]: the systems with the most tiles
+  // first, so that the longest iterations start in the first wave of workgroups
+  std::vector<int> order(numSystems);
+  for (long s = 0; s < numSystems; s++) order[s] = (int)s;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return S->h_stile[a + 1] - S->h_stile[a] > S->h_stile[b + 1] - S->h_stile[b];
+  });
+  ok = ok && up(S->d_sysOrder, order, st) == hipSuccess;
   std::vector<double> ones(numSystems, 1.0);
   ok = ok && up(V.bx, ones, st) == hipSuccess && up(V.by, ones, st) == hipSuccess &&
        up(V.ibx, ones, st) == hipSuccess && up(V.iby, ones, st) == hipSuccess &&
@@ -1196,7 +1204,7 @@ int dpm_destroy(dpm_set_t h) {
                   V.ncnt, V.nbr, V.nbrp, V.tile_v0, V.tile_v1, V.tile_sys, V.stile, S->G.ncx, S->G.ncy,
                   S->G.coff, S->G.brute, S->G.count[0], S->G.count[1], S->G.slots, S->G.cellSys, S->d_flag[0],
                   S->d_flag[1], S->d_any, S->d_updateCount, S->d_err, S->d_maxcnt, S->d_ctr, S->d_notDone,
-                  S->d_fire[0], S->d_fire[1], S->d_part, S->d_red, S->d_scale, V.kr,
+                  S->d_sysOrder, S->d_fire[0], S->d_fire[1], S->d_part, S->d_red, S->d_scale, V.kr,
                   S->d_ck, S->d_fin, S->d_seg, S->d_adj, V.vinfo, V.l0r, V.mbx, V.mby, V.wfx, V.wfy,
                   S->d_contacts, S->d_ccount, S->d_x0, S->d_obs, S->mdCount[0], S->mdSlots[0], S->mdFlag[0],
                   S->mdFlag[1], S->mdFlag[2], S->mdFlag[3], S->mdGen[0], S->mdGen[1], S->mdGen[2], S->R.pvx, S->R.pvy, S->R.pw, S->R.pfx, S->R.pfy, S->R.ptq, S->R.moi,

@@ -203,6 +203,7 @@ struct dpm_set_s {
   int fireKernel = 1;   // dpm_fire_minimize: 1 = one persistent workgroup per system, 0 = graph of step kernels
   int fireWaves = 0;    // waves per system workgroup (0 = from the largest system's tile count)
   int* d_notDone = nullptr;
+  int* d_sysOrder = nullptr;  // persistent FIRE: workgroup b runs system d_sysOrder[b] (most tiles first)
 
   // reductions
   double* d_red = nullptr;   // per-system outputs [nsys * 8]

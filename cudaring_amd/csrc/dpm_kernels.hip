@@ -1500,11 +1500,11 @@ __device__ __forceinline__ void fire_power(const View& V, int t0, int t1, double
 template <int INTER, int POT, int MAXW>
 __global__ __launch_bounds__(WAVE* MAXW) void k_fire_sys(View V, Params P, FireParams fp, CellGrid G,
                                                          FireState* fst, int chunk, int* updateCount, int* err,
-                                                         int* maxcnt) {
+                                                         int* maxcnt, const int* order) {
   __shared__ double red[MAXW][2];
   __shared__ FireState sfs;
   __shared__ int shStop, shReset;
-  const int s = blockIdx.x;
+  const int s = order[blockIdx.x];
   const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
   const int t0 = V.stile[s], t1 = V.stile[s + 1];
   const Box B = box_of(V, s);
@@ -1988,10 +1988,12 @@ template <int INTER, int POT>
 static hipError_t fire_sys_t(dpm_set_s* S, int chunk, int threads) {
   if (threads > WAVE * FS_WAVES)
     k_fire_sys<INTER, POT, FS_WAVES_WIDE><<<(unsigned)S->nsys, threads, 0, S->stream>>>(
-        S->V, S->P, S->fp, S->G, S->d_fire[S->fireCur], chunk, S->d_updateCount, S->d_err, S->d_maxcnt);
+        S->V, S->P, S->fp, S->G, S->d_fire[S->fireCur], chunk, S->d_updateCount, S->d_err, S->d_maxcnt,
+        S->d_sysOrder);
   else
     k_fire_sys<INTER, POT, FS_WAVES><<<(unsigned)S->nsys, threads, 0, S->stream>>>(
-        S->V, S->P, S->fp, S->G, S->d_fire[S->fireCur], chunk, S->d_updateCount, S->d_err, S->d_maxcnt);
+        S->V, S->P, S->fp, S->G, S->d_fire[S->fireCur], chunk, S->d_updateCount, S->d_err, S->d_maxcnt,
+        S->d_sysOrder);
   return hipGetLastError();
 }
 
