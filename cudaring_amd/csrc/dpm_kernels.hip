@@ -1462,6 +1462,9 @@ __global__ void k_fire_resume(FireState* fs, int nsys) {
 #define FS_WAVES 8
 #define FS_WAVES_WIDE 16
 
+// Workgroup reductions of the persistent kernel, with one barrier each: a caller alternates two
+// exchange buffers `red`, so that between two exchanges through the same buffer every thread
+// passes another barrier (no exchange overwrites one that a wave is still reading).
 // workgroup sum of a and maximum of b (fixed order: lanes by butterfly, waves by index)
 __device__ __forceinline__ void block_sum_max(double a, double b, double (*red)[2], double& sa, double& mb) {
   const int w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
@@ -1479,7 +1482,25 @@ __device__ __forceinline__ void block_sum_max(double a, double b, double (*red)[
   }
   sa = s;
   mb = m;
+}
+
+// workgroup sums of a and of b in one exchange (the order of block_sum_max's sum for each)
+__device__ __forceinline__ void block_sum_sum(double a, double b, double (*red)[2], double& sa, double& sb) {
+  const int w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    red[w][0] = a;
+    red[w][1] = b;
+  }
   __syncthreads();
+  double s = 0, t = 0;
+  for (int q = 0; q < nw; q++) {
+    s += red[q][0];
+    t += red[q][1];
+  }
+  sa = s;
+  sb = t;
 }
 
 // P = v.F and max F_c^2 over the system's vertices
@@ -1501,7 +1522,9 @@ template <int INTER, int POT, int MAXW>
 __global__ __launch_bounds__(WAVE* MAXW) void k_fire_sys(View V, Params P, FireParams fp, CellGrid G,
                                                          FireState* fst, int chunk, int* updateCount, int* err,
                                                          int* maxcnt, const int* order) {
-  __shared__ double red[MAXW][2];
+  // exchange buffers: [0] for |v|^2 and |F|^2 after the half kick, [1] for P and max F_c^2 after
+  // the force (each iteration passes the barrier at its start and the displacement test's between)
+  __shared__ double red[2][MAXW][2];
   __shared__ FireState sfs;
   __shared__ int shStop, shReset;
   const int s = order[blockIdx.x];
@@ -1510,7 +1533,7 @@ __global__ __launch_bounds__(WAVE* MAXW) void k_fire_sys(View V, Params P, FireP
   const Box B = box_of(V, s);
   if (threadIdx.x == 0) sfs = fst[s];
   double Pv, Fm;
-  fire_power(V, t0, t1, red, Pv, Fm);  // also publishes sfs (barrier inside)
+  fire_power(V, t0, t1, red[1], Pv, Fm);  // also publishes sfs (barrier inside)
   for (int it = 0; it < chunk; it++) {
     // ---- checkFIREParams (FIRE.cu:338-371) and the stopping rule, thread 0
     if (threadIdx.x == 0) {
@@ -1566,9 +1589,8 @@ __global__ __launch_bounds__(WAVE* MAXW) void k_fire_sys(View V, Params P, FireP
         ff += fx * fx + fy * fy;
       }
     }
-    double VV, FF, dummy;
-    block_sum_max(vv, -1.0, red, VV, dummy);
-    block_sum_max(ff, -1.0, red, FF, dummy);
+    double VV, FF;
+    block_sum_sum(vv, ff, red[0], VV, FF);
     // ---- bendVertexVelocityTowardsForce (FIRE.cu:374-394), drift, displacement test
     int moved = 0;
     for (int t = t0 + w; t < t1; t += nw) {
@@ -1648,7 +1670,7 @@ __global__ __launch_bounds__(WAVE* MAXW) void k_fire_sys(View V, Params P, FireP
         m = fmax(m, fmax(fx * fx, fy * fy));
       }
     }
-    block_sum_max(p, m, red, Pv, Fm);
+    block_sum_max(p, m, red[1], Pv, Fm);
   }
   if (threadIdx.x == 0) fst[s] = sfs;
 }
