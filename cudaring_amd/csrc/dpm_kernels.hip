@@ -1523,51 +1523,38 @@ __global__ __launch_bounds__(WAVE* MAXW) void k_fire_sys(View V, Params P, FireP
                                                          FireState* fst, int chunk, int* updateCount, int* err,
                                                          int* maxcnt, const int* order) {
   // exchange buffers: [0] for |v|^2 and |F|^2 after the half kick, [1] for P and max F_c^2 after
-  // the force (each iteration passes the barrier at its start and the displacement test's between)
+  // the force (between two exchanges through one buffer every thread passes the other's barrier
+  // and the displacement test's)
   __shared__ double red[2][MAXW][2];
-  __shared__ FireState sfs;
-  __shared__ int shStop, shReset;
   const int s = order[blockIdx.x];
   const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
   const int t0 = V.stile[s], t1 = V.stile[s + 1];
   const Box B = box_of(V, s);
-  if (threadIdx.x == 0) sfs = fst[s];
+  // every thread carries the system's FIRE state and updates it from the same reduced values in
+  // the same order, so all hold the same state and take the same exit without a barrier
+  FireState fs = fst[s];
   double Pv, Fm;
-  fire_power(V, t0, t1, red[1], Pv, Fm);  // also publishes sfs (barrier inside)
+  fire_power(V, t0, t1, red[1], Pv, Fm);
   for (int it = 0; it < chunk; it++) {
-    // ---- checkFIREParams (FIRE.cu:338-371) and the stopping rule, thread 0
-    if (threadIdx.x == 0) {
-      FireState fs = sfs;
-      int stop = fs.converged;
-      if (!stop && (sqrt(Fm) <= fp.tol || fs.iters >= fp.maxIter)) {
-        fs.converged = 1;
-        stop = 1;
+    // ---- checkFIREParams (FIRE.cu:338-371) and the stopping rule
+    if (!fs.converged && (sqrt(Fm) <= fp.tol || fs.iters >= fp.maxIter)) fs.converged = 1;
+    if (fs.converged) break;
+    int reset = 0;
+    fs.P = Pv;
+    if (Pv > 0) {
+      fs.numStep++;
+      if (fs.numStep > fp.minStep) {
+        fs.dt = fmin(fs.dt * fp.f_inc, fp.dt_max);
+        fs.a *= fp.f_a;
       }
-      int reset = 0;
-      if (!stop) {
-        fs.P = Pv;
-        if (Pv > 0) {
-          fs.numStep++;
-          if (fs.numStep > fp.minStep) {
-            fs.dt = fmin(fs.dt * fp.f_inc, fp.dt_max);
-            fs.a *= fp.f_a;
-          }
-        } else {
-          fs.numStep = 0;
-          fs.dt = fmax(fs.dt * fp.f_dec, fp.dt_max / 10);
-          fs.a = fp.a_start;
-          reset = 1;
-        }
-        fs.iters++;
-      }
-      sfs = fs;
-      shStop = stop;
-      shReset = reset;
+    } else {
+      fs.numStep = 0;
+      fs.dt = fmax(fs.dt * fp.f_dec, fp.dt_max / 10);
+      fs.a = fp.a_start;
+      reset = 1;
     }
-    __syncthreads();
-    if (shStop) break;
-    const double dt = sfs.dt, h = 0.5 * dt, fa = sfs.a;
-    const int reset = shReset;
+    fs.iters++;
+    const double dt = fs.dt, h = 0.5 * dt, fa = fs.a;
     // ---- back-step on reset (with the decreased dt), half kick, |v|^2 and |F|^2
     double vv = 0, ff = 0;
     for (int t = t0 + w; t < t1; t += nw) {
@@ -1672,7 +1659,7 @@ __global__ __launch_bounds__(WAVE* MAXW) void k_fire_sys(View V, Params P, FireP
     }
     block_sum_max(p, m, red[1], Pv, Fm);
   }
-  if (threadIdx.x == 0) fst[s] = sfs;
+  if (threadIdx.x == 0) fst[s] = fs;
 }
 
 // --------------------------------------------------------- launchers ---
