@@ -422,7 +422,8 @@ class DPMSet:
         return self._per_sys(self.L.dpm_get_rigid_kinetic_energy, "rigid_kinetic_energy")
 
     def set_fire_kernel(self, persistent=True, waves=0):
-        """fire_minimize on one persistent workgroup per system, or on graphs of step kernels."""
+        """fire_minimize on one persistent workgroup per system (waves 1..16, 0 = one per tile of
+        the largest system up to 16), or on graphs of step kernels."""
         self._chk(self.L.dpm_set_fire_kernel(self.h, int(bool(persistent)), int(waves)), "set_fire_kernel")
 
     def fire_minimize(self, tol, max_iter):
