@@ -1454,8 +1454,8 @@ __global__ void k_fire_resume(FireState* fs, int nsys) {
 // FIRE.cu:397-411) back to back, separated by workgroup barriers only: systems share nothing,
 // so no grid-wide synchronisation is needed and a converged system's workgroup simply exits.
 // Each wave walks the system's ring-aligned tiles; the force phase is the step kernel's
-// (ring_shape + gather_pairs).  At most `chunk` iterations per launch; the loop exit is
-// decided by thread 0 and read by every thread after a barrier, so all waves leave together.
+// (ring_shape + gather_pairs).  At most `chunk` iterations per launch; every thread decides the
+// loop exit from the same FIRE state, so all waves leave together.
 // Two builds: up to 8 waves per system (about 190 VGPRs, no spills) and up to 16 (the 1024-lane
 // workgroup caps a wave at 128 VGPRs, so 40-75 of them spill to scratch; halving the tiles each
 // wave walks still makes an iteration about 12% shorter on systems of 76 tiles).
