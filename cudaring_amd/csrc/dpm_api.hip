@@ -944,40 +944,23 @@ int rigid_clear(dpm_set_s* S) {
 //   per step: step [rebuild flagged rows; interaction F(x'); ring force, torque, moment of
 //             inertia; half kick; unless last: the next step's half kick and move]
 //   end: thermal call counter advanced by the loop's draws.
-// NVE runs the velocity-Verlet loop's protocol (each step kernel bins its own positions and
-// rebuilds from the previous step's cells).  Langevin bins the flagged systems' current
-// positions in a launch of its own and rebuilds from those exact cells, with no bound on a
-// step's displacement: its kick adds unit Gaussian draws to the ring velocities whatever the
-// time step, so a ring can move far in one step.
+// Both run the velocity-Verlet loop's protocol (a rebuilding step builds its rows from the cells
+// of the list positions and bins the new ones).  Langevin's kick adds unit Gaussian draws to the
+// ring velocities whatever the time step, so a ring can move far in one step: a drift longer than
+// cutoff / 3 flags its system FLAG_FAR, whose next rebuild searches all the system's vertices.
 int rigid_sequence(dpm_set_s* S, long steps, bool langevin) {
-  if (!langevin) {
-    int rc = md_start(S);
-    if (rc) return rc;
-    HIPCHK(launch_rigid_start(S, false, md_args(S, 0, 0, 0)));
-    swap_pos(S);
-    for (long i = 1; i <= steps; i++) {
-      const bool more = i < steps;
-      HIPCHK(launch_step(S, MODE_RIGID | MODE_KICK | (more ? MODE_START : 0), md_args(S, i, i - 1, 0)));
-      if (more) swap_pos(S);
-    }
-    return DPM_OK;
-  }
-  HIPCHK(launch_fill_int(S->d_flag[1 - S->cur], 0, S->nsys, S->stream));
-  HIPCHK(launch_fill_int(S->d_any + 1 - S->cur, 0, 1, S->stream));
-  StepArgs A0 = loop_args(S);
-  A0.ctrOffT = 0;
-  HIPCHK(launch_rigid_start(S, true, A0));
+  int rc = md_start(S);
+  if (rc) return rc;
+  HIPCHK(launch_rigid_start(S, langevin, md_args(S, 0, 0, 0)));
   swap_pos(S);
   for (long i = 1; i <= steps; i++) {
     const bool more = i < steps;
-    HIPCHK(launch_bin(S, 0, 0));
-    StepArgs A = loop_args(S);
-    A.ctrOffT = i - 1;
-    A.rigidLangevin = 1;
+    StepArgs A = md_args(S, i, i - 1, 0);
+    A.rigidLangevin = langevin ? 1 : 0;
     HIPCHK(launch_step(S, MODE_RIGID | MODE_KICK | (more ? MODE_START : 0), A));
     if (more) swap_pos(S);
   }
-  HIPCHK(launch_ctr_add(S, steps, 0));
+  if (langevin) HIPCHK(launch_ctr_add(S, steps, 0));
   return DPM_OK;
 }
 

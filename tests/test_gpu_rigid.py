@@ -83,6 +83,7 @@ def test_rigid_langevin_trajectory(oracle_mod):
         assert rel_err(g.get("pos"), o.get("pos"), scale=1.0) < 1e-11, chunk
         assert rel_err(g.get("pvel"), o.get("pvel")) < 1e-10, chunk
         assert rel_err(g.get("pangvel"), o.get("pangvel")) < 1e-9, chunk
+    assert list(g.update_count()) == [o.update_count()]
     st_rad = st["R"]
     for s in (o, g):
         s.set("prad", st_rad)
