@@ -78,3 +78,35 @@ def test_loop_start_far_from_list_positions(oracle_mod, inter):
         assert rel_err(g.get("force"), o.get("force")) < 1e-9, chunk
     assert g.update_count()[0] > u0  # the first step rebuilt the rows (all pairs)
     g.close()
+
+
+def test_fast_rigid_rings_langevin(oracle_mod):
+    """Rigid Langevin (whose kick adds unit draws whatever the time step) with ring velocities that
+    cover 0.6 cutoff per step: each drift flags the system as a far move, so the next rebuild
+    searches all pairs; the trajectory matches the oracle, which rebuilds from scratch.  The rings
+    start apart (rigid rings cannot give way: a start in contact at these speeds overlaps deeply
+    within two steps and the forces blow up) and first touch in the third step, through pairs the
+    far-move rebuilds found."""
+    print("case far_moves rigid langevin", flush=True)
+    st = ring_block(gap=4.0, seed=7)
+    dt = 2e-3
+    o = oracle_from_state(oracle_mod, st, potential="wca", interaction="vertexVertex", costs=(0, 0, 0, 1), dt=dt)
+    g = gpu_from_state(st, potential="wca", interaction="vertexVertex", costs=(0, 0, 0, 1), dt=dt)
+    size = 2 * st["rad"].mean()
+    for obj in (o, g):
+        obj.calc_neighbor_list(obj.set_displacement_cutoff(0.5, size))
+        obj.calc_rigid_force_energy()
+        obj.init_rigid_langevin(1e-2, 1.0)
+    rng = np.random.default_rng(2)
+    speed = 0.6 * 0.5 * size / dt
+    ang = rng.uniform(0, 2 * np.pi, len(st["nvp"]))
+    pv = (np.c_[np.cos(ang), np.sin(ang)] * speed).ravel()
+    o.set("pvel", pv)
+    g.set("pvel", pv)
+    for chunk in (1, 2):
+        o.rigid_loop(chunk)
+        g.rigid_langevin(chunk)
+        assert rel_err(g.get("pos"), o.get("pos"), scale=1.0) < 1e-11, chunk
+        assert rel_err(g.get("pvel"), o.get("pvel")) < 1e-10, chunk
+    assert g.update_count()[0] >= 2  # the list was rebuilt (all pairs) in these steps
+    g.close()
