@@ -982,15 +982,21 @@ int run_rigid(dpm_set_s* S, long steps, bool langevin) {
 // buffers), the bin launch and the step kernel (rebuild of flagged rows, forces, ring sums, the
 // second half kick, the next iteration's partials)
 int rigid_fire_sequence(dpm_set_s* S, long iterations) {
-  HIPCHK(launch_fill_int(S->d_flag[1 - S->cur], 0, S->nsys, S->stream));
-  HIPCHK(launch_fill_int(S->d_any + 1 - S->cur, 0, 1, S->stream));
+  // the step loop's cell protocol: the list positions binned into cell list 0 and the current
+  // positions' list validity in F[0]; iteration i's move ORs its moved positions' validity into
+  // F[i] (a drift longer than cutoff / 3 flags FLAG_FAR), and its step (rotation index i) reads
+  // F[i], rebuilds flagged systems from the list-position cells (all pairs on FLAG_FAR) and clears
+  // F[i + 2] for the move after next
+  int rc = md_start(S);
+  if (rc) return rc;
+  HIPCHK(launch_fill_int(S->mdFlag[0], 0, S->nsys, S->stream));
+  HIPCHK(launch_bin_flag(S, S->mdCount[0], S->mdSlots[0], S->mdFlag[0], S->mdGen[0]));
   HIPCHK(launch_rigid_fire_partials(S));
   for (long it = 0; it < iterations; it++) {
-    HIPCHK(launch_rigid_fire_move(S));
+    HIPCHK(launch_rigid_fire_move(S, S->mdFlag[it % 3], S->d_any + 2));
     S->rfireCur = 1 - S->rfireCur;
     swap_pos(S);
-    HIPCHK(launch_bin(S, 0, 0));
-    StepArgs A = loop_args(S);
+    StepArgs A = md_args(S, it, 0, 0);
     A.fire = S->d_rfire[S->rfireCur];
     A.part = S->d_part;
     HIPCHK(launch_step(S, MODE_RIGID | MODE_FIRE, A));

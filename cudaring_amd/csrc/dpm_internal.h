@@ -298,6 +298,6 @@ hipError_t launch_build_rows_view(dpm_set_s* S, const View& Vn, int parity, int 
 hipError_t launch_rigid_start(dpm_set_s* S, bool langevin, const StepArgs& A);
 hipError_t launch_rigid_inject(dpm_set_s* S, double amplitude, unsigned long long call);
 hipError_t launch_rigid_fire_partials(dpm_set_s* S);
-hipError_t launch_rigid_fire_move(dpm_set_s* S);
+hipError_t launch_rigid_fire_move(dpm_set_s* S, int* flagOut, int* anyOut);
 
 }  // namespace dpm

@@ -1313,6 +1313,7 @@ __global__ void k_rigid_fire_move(View V, Params P, FireParams fp, const FireSta
     atomicOr(&flagOut[L.s], 1);
     atomicOr(anyOut, 1);
   }
+  far_test(P, B, xn - V.x[L.v], yn - V.y[L.v], flagOut + L.s);
   if (L.lane == L.segS) {
     R.plx[L.r] = lcx;
     R.ply[L.r] = lcy;
@@ -1961,11 +1962,12 @@ hipError_t launch_rigid_fire_partials(dpm_set_s* S) {
   return hipGetLastError();
 }
 
-hipError_t launch_rigid_fire_move(dpm_set_s* S) {
+hipError_t launch_rigid_fire_move(dpm_set_s* S, int* flagOut, int* anyOut) {
+  if (!flagOut || !anyOut) return hipErrorInvalidValue;
   const int nb = 1 - S->cur;
   k_rigid_fire_move<<<grid_w(S->V.ntiles), 256, 0, S->stream>>>(
       S->V, S->P, S->fp, S->d_rfire[S->rfireCur], S->d_rfire[1 - S->rfireCur], S->d_part, S->pos[nb][0],
-      S->pos[nb][1], S->d_flag[nb], S->d_any + nb);
+      S->pos[nb][1], flagOut, anyOut);
   return hipGetLastError();
 }
 

@@ -429,6 +429,10 @@ class Oracle:
     def reset_update_count(self):
         self.L.orc_reset_update_count(self.h)
 
+    def reset_last_positions(self):
+        """resetLastPositions: the list positions become the current positions."""
+        self.L.orc_reset_last_positions(self.h)
+
 
 def philox(ctr, key):
     c = np.ascontiguousarray(ctr, dtype=np.uint32)
